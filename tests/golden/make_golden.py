@@ -1,0 +1,78 @@
+"""Extract the golden numbers of the reference's shipped outputs into
+tests/golden/reference_outputs.json.
+
+The reference ships captured stdout of its own runs (data/*_out).  This
+script reads those text files (data, not code) and records the energies,
+iteration counts and basis-set counts that pin the oracle.  It only runs in
+a checkout that has the reference tree:
+
+    python tests/golden/make_golden.py /root/reference/data
+"""
+import json
+import os
+import re
+import sys
+
+CASES = {
+    "H2O_ccpVDZ_spherical": "H2O_CCSD_ccpVDZ_out",
+    "BH_STO3G": "BH_EOM_out",
+    "FH_STO3G_zeroT": "FH_finiteT_MBPT_zeroT_HF_ref_out",
+    "FH_STO3G_T1e5": "FH_finiteT_MBPT_finiteT_HF_ref_out",
+}
+
+PATTERNS = {
+    "e_nuc": r"^NUCLEAR REPULSION ENERGY\s+=\s+(\S+) HARTREE",
+    "e_kin": r"^KINETIC ENERGY\s+=\s+(\S+) HARTREE",
+    "e_nuc_att": r"^NUCLEAR ATTRACTION ENERGY\s+=\s+(\S+) HARTREE",
+    "e_coul": r"^COULOMB ENERGY\s+=\s+(\S+) HARTREE",
+    "e_hfx": r"^HF-EXCHANGE ENERGY\s+=\s+(\S+) HARTREE",
+    "e_total": r"^TOTAL SCF ENERGY\s+=\s+(\S+) HARTREE",
+    "npgs": r"^NUMBER OF PRIMITIVE GAUSSIANS  \(INCLUDING GHOST FUNCTIONS\) =\s+(\d+)",
+    "ncgs": r"^NUMBER OF CONTRACTED GAUSSIANS \(INCLUDING GHOST FUNCTIONS\) =\s+(\d+)",
+    "npshell": r"^NUMBER OF SHELLS\s+\(INCLUDING GHOST FUNCTIONS\) =\s+(\d+)",
+    "e2_alg1": r"^E\(2\)\s+=\s+(\S+) HARTREE",
+    "e2_cc": r"^E\(2\) =\s+(\S+) \(",
+    "e2_zero_t": r"^ZERO-TEMPERATURE MP2 ENERGY =\s+(\S+) HARTREE",
+}
+
+
+def parse(path):
+    out = {"source": os.path.basename(path)}
+    lines = open(path).read().splitlines()
+    for key, pat in PATTERNS.items():
+        rx = re.compile(pat)
+        for n, line in enumerate(lines, 1):
+            m = rx.match(line)
+            if m:
+                v = m.group(1)
+                out[key] = int(v) if v.isdigit() else float(v)
+                out[key + "_line"] = n
+                if key != "e_nuc":
+                    break
+    # SCF iterations: the last "ITR" table row before "SCF CONVERGED"
+    it = None
+    trace = []
+    for line in lines:
+        if line.startswith("SCF CONVERGED"):
+            break
+        m = re.match(r"^\s*(\d+)\s+(\d+)\s+\S+E[-+]\d+\s+(\S+)\s+(\S+)", line)
+        if m:
+            it = int(m.group(1))
+            trace.append([it, int(m.group(2)), float(m.group(3)), float(m.group(4))])
+    out["iterations"] = it
+    for n, line in enumerate(lines):
+        if line.strip() == "X   MAXIMUM OVERLAP":
+            out["max_overlap"] = float(lines[n + 1].split()[1])  # F18.13
+            out["max_overlap_line"] = n + 2
+            break
+    out["trace"] = trace  # ITR, DIIS, DENSITY CHANGE, TOTAL ENERGY
+    return out
+
+
+if __name__ == "__main__":
+    src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/data"
+    res = {k: parse(os.path.join(src, v)) for k, v in CASES.items()}
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "reference_outputs.json"), "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print(json.dumps(res, indent=1, sort_keys=True))

@@ -1,0 +1,117 @@
+"""Pins the oracle's Obara-Saika engine (src/pfourindexdirect.f90:543-690)
+and its contraction (src/pfourindexstore.f90:704-708, src/pfourindexdirect.
+f90:765-781) against an independent McMurchie-Davidson implementation, the
+(ss|ss) closed form, and the invariants every ERI tensor obeys."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import molecules as M
+from tests.conftest import basis_path
+from tests.md_reference import contracted_eri
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    b = oracle.Basis(basis_path("cc-pVDZ"), M.H2O_GOLDEN)
+    return b, b.eri_tensor()
+
+
+def _real(b):
+    return [c for c in range(b.ncgs) if np.any(b.cc[c])]
+
+
+def test_against_mcmurchie_davidson(h2o):
+    b, feri = h2o
+    real = _real(b)
+    # pick functions by angular momentum so every class up to (dd|dd) appears
+    lval = {c: int(max(b.pax[p] + b.pay[p] + b.paz[p] for p in np.nonzero(b.cc[c])[0])) for c in real}
+    few = [c for c in real if np.count_nonzero(b.cc[c]) <= 4]
+    rng = np.random.default_rng(7)
+    picks = []
+    for want in [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 1, 1), (2, 0, 2, 0), (2, 2, 2, 2), (2, 1, 2, 1), (1, 1, 2, 2)]:
+        cand = [[c for c in few if lval[c] == w] for w in want]
+        for _ in range(2):
+            picks.append(tuple(int(rng.choice(cs)) for cs in cand))
+    for _ in range(10):
+        picks.append(tuple(int(rng.choice(few)) for _ in range(4)))
+    for i, j, k, l in picks:
+        ref = contracted_eri(b, i, j, k, l)
+        assert abs(feri[i, j, k, l] - ref) < 1e-12, (i, j, k, l, feri[i, j, k, l], ref)
+
+
+def test_eightfold_symmetry_and_ghosts(h2o):
+    b, g = h2o
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.max(np.abs(g - g.transpose(perm))) < 1e-13
+    ghosts = [c for c in range(b.ncgs) if not np.any(b.cc[c])]
+    assert np.all(g[ghosts] == 0.0)
+
+
+def test_supermatrix_positive_semidefinite(h2o):
+    b, g = h2o
+    n = b.ncgs
+    w = np.linalg.eigvalsh(g.reshape(n * n, n * n, order="F"))
+    assert w.min() > -1e-12 * w.max()
+
+
+def test_translation_invariance(h2o):
+    b, g = h2o
+    moved = [(a[0], a[1] + 3.7, a[2] - 1.1, a[3] + 0.4) for a in M.H2O_GOLDEN]
+    g2 = oracle.Basis(basis_path("cc-pVDZ"), moved).eri_tensor()
+    assert np.max(np.abs(g - g2)) < 1e-12
+
+
+def test_ssss_closed_form():
+    """(ss|ss) = 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd F_0(T) for H2."""
+    b = oracle.Basis(basis_path("STO-3G"), [(1, 0, 0, 0), (1, 0.3, -0.2, 1.4)])
+    g = b.eri_tensor()
+
+    def prims(c):
+        return [(p, b.cc[c, p]) for p in np.nonzero(b.cc[c])[0]]
+
+    def ss(p1, p2, p3, p4):
+        P = [b.zt[p] for p in (p1, p2, p3, p4)]
+        R = [np.array([b.pgsx[p], b.pgsy[p], b.pgsz[p]]) for p in (p1, p2, p3, p4)]
+        p, q = P[0] + P[1], P[2] + P[3]
+        A = (P[0] * R[0] + P[1] * R[1]) / p
+        B = (P[2] * R[2] + P[3] * R[3]) / q
+        kab = math.exp(-P[0] * P[1] / p * np.sum((R[0] - R[1]) ** 2))
+        kcd = math.exp(-P[2] * P[3] / q * np.sum((R[2] - R[3]) ** 2))
+        T = p * q / (p + q) * np.sum((A - B) ** 2)
+        f0 = 1.0 if T < 1e-14 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        return 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * kab * kcd * f0
+
+    for idx in [(0, 0, 0, 0), (0, 1, 0, 1), (0, 0, 1, 1), (0, 1, 1, 1)]:
+        ref = sum(c1 * c2 * c3 * c4 * ss(p1, p2, p3, p4)
+                  for p1, c1 in prims(idx[0]) for p2, c2 in prims(idx[1])
+                  for p3, c3 in prims(idx[2]) for p4, c4 in prims(idx[3]))
+        assert abs(g[idx] - ref) < 1e-13
+
+
+def test_direct_jk_equals_tensor_contraction(h2o):
+    """FOURINDEX_ERI (primitive-basis J/K then CC contraction) equals the
+    brute-force contraction of the stored tensor at INTTOL 0."""
+    b, g = h2o
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((b.ncgs, b.ncgs))
+    p = p + p.T
+    j, k, info = b.fourindex_eri(p, 0.0)
+    jr = np.einsum("ijkl,kl->ij", g, p)
+    kr = np.einsum("ijkl,kj->il", g, p)
+    assert np.max(np.abs(j - jr)) < 1e-11
+    assert np.max(np.abs(k - kr)) < 1e-11
+    assert info["composite_quartets"] == b.npshell ** 4
+
+
+def test_screening_drops_only_small_quartets(h2o):
+    b, g = h2o
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal((b.ncgs, b.ncgs))
+    p = 0.1 * (p + p.T)
+    j0, k0, i0 = b.fourindex_eri(p, 0.0)
+    j1, k1, i1 = b.fourindex_eri(p, 1e-10)
+    assert i1["composite_quartets"] < i0["composite_quartets"]
+    assert np.max(np.abs(j1 - j0)) < 1e-8 and np.max(np.abs(k1 - k0)) < 1e-8

@@ -1,0 +1,83 @@
+"""Pins the oracle's SCF (src/pselfconsistentfield.f90, src/pfock.f90,
+src/pdensitymatrix.f90) and MP2 (src/pmp2energy.f90) against the energies,
+iteration counts and per-iteration traces of the reference's shipped
+outputs (tests/golden/reference_outputs.json, from data/*_out)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import molecules as M
+from tests.conftest import basis_path
+
+EH = 1e-8
+
+
+def _check_components(r, g, tol=1e-10):
+    for k in ["e_nuc", "e_kin", "e_nuc_att", "e_coul", "e_hfx", "e_total"]:
+        assert abs(r[k] - g[k]) < tol, (k, r[k], g[k])
+
+
+def test_h2o_ccpvdz_spherical_stored(golden):
+    g = golden["H2O_ccpVDZ_spherical"]
+    b = oracle.Basis(basis_path("cc-pVDZ"), M.H2O_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11, lindep=1e-12, diis=2, spherical=True, direct=False)
+    _check_components(r, g)
+    assert r["iterations"] == g["iterations"] == 30
+    tr = np.array(g["trace"])
+    assert np.max(np.abs(r["trace"][:, 0] - tr[:, 2])) < 2e-12
+    assert np.max(np.abs(r["trace"][:, 1] - tr[:, 3])) < 2e-10
+    assert abs(r["eps"][4] - (-0.4913)) < 5e-5 and abs(r["eps"][5] - 0.1844) < 5e-5
+    feri = b.eri_tensor()
+    s, t = oracle.mp2(feri, r["co"], r["eps"], 0, r["iocc"], r["n_indep"])
+    assert abs(s + t - g["e2_cc"]) < 1e-12
+
+
+def test_bh_sto3g_mp2_frozen_core(golden):
+    g = golden["BH_STO3G"]
+    b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11, direct=False)
+    _check_components(r, g)
+    assert r["iterations"] == g["iterations"]
+    s, t = oracle.mp2(b.eri_tensor(), r["co"], r["eps"], 1, r["iocc"], r["n_indep"])
+    assert abs(s + t - g["e2_alg1"]) < 1e-13
+
+
+def test_fh_sto3g_zero_t(golden):
+    g = golden["FH_STO3G_zeroT"]
+    b = oracle.Basis(basis_path("STO-3G"), M.FH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-12, direct=False)
+    _check_components(r, g)
+    assert r["iterations"] == g["iterations"]
+    s, t = oracle.mp2(b.eri_tensor(), r["co"], r["eps"], 0, r["iocc"], r["n_indep"])
+    assert abs(s + t - g["e2_zero_t"]) < 1e-13
+
+
+def test_fh_sto3g_finite_temperature(golden):
+    """TEMPHF 1e5 K pins the Fermi-Dirac bisection (src/pfock.f90:339-423)."""
+    g = golden["FH_STO3G_T1e5"]
+    b = oracle.Basis(basis_path("STO-3G"), M.FH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-12, direct=False, temphf=1e5)
+    # the bisection stops at |N - N0| < 1e-10 electrons, so components carry
+    # ~1e-10 of Fermi-level noise; the total is stationary and much tighter
+    _check_components(r, g, tol=1e-9)
+    assert abs(r["e_total"] - g["e_total"]) < 1e-10
+    assert r["iterations"] == g["iterations"]
+
+
+def test_direct_path_h2o(golden):
+    """The direct (FOURINDEX_ERI every iteration) path reaches the same
+    energy and the same iteration count as the stored path."""
+    g = golden["H2O_ccpVDZ_spherical"]
+    b = oracle.Basis(basis_path("cc-pVDZ"), M.H2O_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11, lindep=1e-12, diis=2, spherical=True, direct=True)
+    assert abs(r["e_total"] - g["e_total"]) < 1e-10
+    assert r["iterations"] == 30
+
+
+def test_planted_mistake_is_caught(golden):
+    """A plausible mistake in the restatement (exchange scaled by the wrong
+    factor) moves the energy far beyond the parity bar."""
+    g = golden["BH_STO3G"]
+    b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11, direct=False, hfexchange=0.98)
+    assert abs(r["e_total"] - g["e_total"]) > 1e-3
