@@ -1,0 +1,163 @@
+/*
+ * polyeri.h -- C ABI of libpolyeri, the MI355X (gfx950) direct two-electron
+ * path for polymer: shell-quartet ERIs contracted on the fly into Coulomb /
+ * exchange matrices, the SCF loop around them, and the MP2 AO->MO transform.
+ *
+ * Plain pointers and sizes only.  Matrices use the reference's Fortran
+ * column-major layout over the contracted functions INCLUDING ghost
+ * functions (NCGS of src/pread.f90 READ3): element (i,j) is a[i + ncgs*j].
+ * Every entry point returns 0 on success or a negative POLYERI_E_* code;
+ * polyeri_last_error() then holds a message (thread-local).
+ */
+#ifndef POLYERI_H
+#define POLYERI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POLYERI_OK 0
+#define POLYERI_E_ARG (-1)       /* invalid argument / layout              */
+#define POLYERI_E_BASIS (-2)     /* basis file or shell structure          */
+#define POLYERI_E_HIP (-3)       /* HIP runtime / device error             */
+#define POLYERI_E_NOMEM (-4)     /* device allocation failed               */
+#define POLYERI_E_NOTCONV (-5)   /* SCF CONVERGENCE NOT MET                */
+#define POLYERI_E_NONFINITE (-6) /* non-finite result                      */
+#define POLYERI_E_LIB (-7)       /* rocBLAS / rocSOLVER error              */
+
+typedef struct polyeri_ctx polyeri_ctx;
+
+/* Precision modes of the J/K build. */
+#define POLYERI_FP64 0
+#define POLYERI_MIXED 1   /* FP32 for quartets whose Schwarz bound < mixed_cut */
+
+typedef struct polyeri_opts {
+    double schwarz_tol;  /* drop quartets with Q_ab Q_cd < tol (INTTOL)       */
+    double prim_tol;     /* drop primitive pairs with |c_a c_b| e^{-mu R^2} < */
+    int precision;       /* POLYERI_FP64 or POLYERI_MIXED                      */
+    double mixed_cut;    /* Schwarz bound below which quartets run in FP32     */
+    int device;          /* HIP device ordinal                                 */
+} polyeri_opts;
+
+/* Reads a Gaussian-format basis-set file for the given atoms and returns the
+ * reference's arrays (READ3, src/pread.f90:584-1482).  Call once with all
+ * array pointers NULL to get the sizes, then with arrays of those sizes.
+ * xyz: 3*natom bohr.  cc: ncgs*npgs column-major.  p_shell: npshell*125
+ * (slot lx*25+ly*5+lz, -1 = empty). */
+int polyeri_basis_read(const char *path, int natom, const int *iatom, const double *xyz,
+                       int *npgs, int *ncgs, int *npshell, int *pax, int *pay, int *paz,
+                       int *pgs_atom, double *pgsx, double *pgsy, double *pgsz, double *zt,
+                       double *cc, double *cgsx, double *cgsy, double *cgsz, int *p_shell,
+                       int *p_shell_ang);
+
+/* Builds a context from the reference's module arrays (the inputs
+ * FOURINDEX_ERI reads from BASISSET / STRUCTURE, src/pfourindexdirect.f90:
+ * 1-99, src/pmodules.f90:64-81): primitive angular exponents, centres,
+ * exponents, the contraction matrix CC and the atoms.  Derives contracted
+ * shells, primitive pairs and Schwarz bounds on the device.  opts may be
+ * NULL (FP64, tol 1e-12). */
+int polyeri_ctx_create(int npgs, int ncgs, const int *pax, const int *pay, const int *paz,
+                       const double *pgsx, const double *pgsy, const double *pgsz,
+                       const double *zt, const double *cc, int natom, const int *iatom,
+                       const double *xyz, const polyeri_opts *opts, polyeri_ctx **out);
+void polyeri_ctx_destroy(polyeri_ctx *ctx);
+
+/* Drop-in for CALL FOURINDEX_ERI (src/pselfconsistentfield.f90:42): from
+ * the contracted density P_C (ncgs x ncgs, host) build the Coulomb matrix
+ * C_C = J and the exchange matrix X_C = K (src/pfourindexdirect.f90:773,
+ * 777), host arrays, ghost rows/columns zero. */
+int polyeri_jk_build(polyeri_ctx *ctx, const double *p_c, double *c_c, double *x_c);
+
+/* Same on device pointers (ncgs x ncgs each), enqueued on the context's
+ * stream; returns after the kernels are enqueued.  polyeri_sync waits. */
+int polyeri_jk_build_device(polyeri_ctx *ctx, const double *d_p_c, double *d_c_c, double *d_x_c);
+int polyeri_sync(polyeri_ctx *ctx);
+
+/* Device memory for callers without their own allocator (e.g. a Fortran
+ * driver): allocation on the context's device, and blocking copies
+ * (to_device = 1: host -> device, 0: device -> host). */
+int polyeri_device_alloc(polyeri_ctx *ctx, size_t bytes, void **ptr);
+int polyeri_device_free(polyeri_ctx *ctx, void *ptr);
+int polyeri_memcpy(polyeri_ctx *ctx, void *dst, const void *src, size_t bytes, int to_device);
+
+/* Drop-in for FOURINDEX_ERI followed by the Fock assembly of FOCK_BUILD
+ * (src/pfock.f90:44, with the Hermitian average of :224-233):
+ * F_C = T_C + N_C + C_C - hfexchange/2 X_C on device pointers (ncgs^2
+ * each), enqueued on the context's stream. */
+int polyeri_fock_build_device(polyeri_ctx *ctx, const double *d_p_c, double hfexchange,
+                              double *d_f_c, double *d_c_c, double *d_x_c);
+
+/* One-electron matrices S_C, T_C, N_C (src/ponee.f90:1-414, :418-728),
+ * host, ncgs x ncgs; the nuclear repulsion energy
+ * (src/pselfconsistentfield.f90:652-704). */
+int polyeri_one_electron(polyeri_ctx *ctx, double *s_c, double *t_c, double *n_c,
+                         double *e_nuc);
+
+/* Contracted ERI tensor FERI(i,j,k,l) = feri[i + n(j + n(k + n l))], n =
+ * ncgs, the values src/pfourindexstore.f90:704-708 accumulates; for tests
+ * and small systems (ncgs^4 doubles on host and device). */
+int polyeri_eri_tensor(polyeri_ctx *ctx, double *feri);
+
+typedef struct polyeri_scf_opts {
+    double scfconv;     /* DOPTN(14) RMS density change                     */
+    double scfrelax;    /* DOPTN(15)                                        */
+    double hfexchange;  /* DOPTN(19)                                        */
+    double lindep;      /* DOPTN(85)                                        */
+    double temphf;      /* DOPTN(108), K; 0 -> 1e-4 K                       */
+    int scfcycles;      /* IOPTN(13)                                        */
+    int diis;           /* IOPTN(16)                                        */
+    int charge;         /* IOPTN(5)                                         */
+    int spherical;      /* LOPTN(69)                                        */
+    int guess;          /* 0: GUESS_DENSITYMATRIX; 1: caller's P_C          */
+} polyeri_scf_opts;
+
+typedef struct polyeri_scf_result {
+    double e_total, e_nuc, e_kin, e_nuc_att, e_coul, e_hfx, fermi;
+    int iterations, converged, n_indep, iocc;
+    double jk_seconds;   /* total J/K build time (device events)            */
+    double fock_seconds; /* total J/K + Fock assembly time                  */
+    double scf_seconds;  /* wall time of the whole loop                     */
+} polyeri_scf_result;
+
+/* SELF_CONSISTENT_FIELD (src/pselfconsistentfield.f90:1-162) with FOCK_BUILD
+ * (src/pfock.f90, one k-point), DIIS / relaxation and the energy of
+ * ENERGY_CALC, all on the device.  p_c (ncgs^2) is the guess when
+ * opts->guess == 1 and the converged density on return.  eps (ncgs), co
+ * (ncgs^2, Cartesian MO coefficients) and trace (2*scfcycles: density
+ * change, energy per iteration) may be NULL. */
+int polyeri_scf(polyeri_ctx *ctx, const polyeri_scf_opts *opts, double *p_c, double *eps,
+                double *co, double *trace, polyeri_scf_result *res);
+
+/* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
+ * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the
+ * device (no .ao files), four quarter transforms, singlet and triplet E(2).
+ * co: ncgs x ncgs Cartesian MO coefficients (columns = orbitals); orbitals
+ * icore..iocc-1 active occupied, iocc..nall-1-ivirtcore virtual. */
+int polyeri_mp2_energy(polyeri_ctx *ctx, const double *co, const double *eps, int icore,
+                       int iocc, int nall, int ivirtcore, double *e2s, double *e2t);
+
+typedef struct polyeri_stats {
+    int nshell, nbf, npair, nprim_pair;
+    long long quartets;        /* surviving contracted shell quartets        */
+    double prim_quartets;      /* surviving primitive quartets (8-fold)      */
+    double model_flops;        /* algorithmic FLOPs of one J/K build         */
+    double last_jk_ms;         /* device time of the last J/K build          */
+    double last_class_ms[32];  /* per class-pair kernel time of the last build */
+    double class_prim_quartets[32];
+    double class_flops[32];
+    int class_l[32];           /* la*1000+lb*100+lc*10+ld                    */
+    int nclass;
+} polyeri_stats;
+int polyeri_get_stats(polyeri_ctx *ctx, polyeri_stats *st);
+
+/* Per-class kernel timing on/off (adds events between launches). */
+int polyeri_set_profiling(polyeri_ctx *ctx, int on);
+
+const char *polyeri_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -45,6 +45,8 @@ def lib():
         L.or_fourindex_eri.restype = ll
         L.or_fourindex_eri_range.argtypes = [p, p, p, d, i, ll, ll, p, p, p, p]
         L.or_fourindex_eri_range.restype = ll
+        L.or_fourindex_eri_rows.argtypes = [p, p, p, d, i, p, ll, p, p, p, p, p, p]
+        L.or_fourindex_eri_rows.restype = ll
         L.or_eri_tensor.argtypes = [p, p, p]
         L.or_decontract.argtypes = [p, p, p]
         L.or_form_c2s.argtypes = [p, i, p]
@@ -159,6 +161,22 @@ class Basis:
                                           int(recompute_kl), int(ij_begin), int(ij_end),
                                           _ptr(c_p), _ptr(x_p), ctypes.byref(nreal), ctypes.byref(fl))
         return nq, nreal.value, fl.value
+
+    def fourindex_eri_rows(self, p_c, inttol, rows, recompute_kl=True):
+        """Times the reference loop nest on selected bra rows (I*npshell+J):
+        returns (quartets, real quartets, setup seconds, row seconds)."""
+        s_p = np.zeros((self.npgs, self.npgs), order="F")
+        p_p = self.decontract(p_c)
+        c_p = np.zeros((self.npgs, self.npgs), order="F")
+        x_p = np.zeros((self.npgs, self.npgs), order="F")
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        nreal = ctypes.c_longlong()
+        fl = ctypes.c_double()
+        ts, tr = ctypes.c_double(), ctypes.c_double()
+        nq = lib().or_fourindex_eri_rows(self.h, _ptr(s_p), _ptr(p_p), float(inttol), int(recompute_kl),
+                                         _ptr(r), len(r), _ptr(c_p), _ptr(x_p), ctypes.byref(nreal),
+                                         ctypes.byref(fl), ctypes.byref(ts), ctypes.byref(tr))
+        return nq, nreal.value, ts.value, tr.value
 
     def eri_tensor(self):
         m = self.ncgs

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* src/pmodules.f90:7 -- the reference's (truncated) value of pi. */
 #define PI_REF 3.14159265358979
@@ -499,6 +500,12 @@ static void cpair_make(const or_basis *b, int I, int J, cpair *cp) {
     cp->real = 1;
 }
 
+static double wall_seconds(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
 /* Composite quartet OS recursion: ket first ([00|kl]), then bra, as in
  * src/pfourindexdirect.f90:554-690.  e holds ERI(m, kl, ij) at
  * e[(ij*nkl + kl)*ms + m] with ms = L+1. */
@@ -606,10 +613,11 @@ static int shell_real_mask(const or_basis *b, int s, const double *maxc) {
 /* ------------------------------------------------------------------ */
 /* Direct J/K: src/pfourindexdirect.f90:1-814 (molecule: Q1=Q2=Q3=0)  */
 /* ------------------------------------------------------------------ */
-long long or_fourindex_eri_range(const or_basis *b, const double *s_p, const double *p_p,
-                                 double inttol, int recompute_kl, long long ij_begin,
-                                 long long ij_end, double *c_p, double *x_p,
-                                 long long *real_quartets, double *flop_model) {
+long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const double *p_p,
+                                double inttol, int recompute_kl, const long long *rows,
+                                long long nrows, double *c_p, double *x_p,
+                                long long *real_quartets, double *flop_model,
+                                double *setup_seconds, double *row_seconds) {
     (void)s_p;
     boys_init();
     pairtab_init();
@@ -621,15 +629,20 @@ long long or_fourindex_eri_range(const or_basis *b, const double *s_p, const dou
     cpair *kl = (cpair *)malloc(sizeof(cpair) * nsq);
     cpair ij;
     double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
+    double t0 = wall_seconds();
     for (int k = 0; k < ns; ++k)
         for (int l = 0; l < ns; ++l) {
             cpair_make(b, k, l, &kl[(size_t)k * ns + l]);
             cpair_bound(&kl[(size_t)k * ns + l], maxc, e);
         }
+    double t1 = wall_seconds();
+    if (setup_seconds) *setup_seconds = t1 - t0;
     long long nq = 0, nreal = 0;
     double flops = 0;
     const double c1f = 2.0 / sqrt(PI_REF);
-    for (long long sij = ij_begin; sij < ij_end && sij < (long long)nsq; ++sij) {
+    for (long long r = 0; r < nrows; ++r) {
+        long long sij = rows[r];
+        if (sij < 0 || sij >= (long long)nsq) continue;
         int I = (int)(sij / ns), J = (int)(sij % ns);
         cpair_make(b, I, J, &ij);
         cpair_bound(&ij, maxc, e);
@@ -673,10 +686,24 @@ long long or_fourindex_eri_range(const or_basis *b, const double *s_p, const dou
             }
         }
     }
+    if (row_seconds) *row_seconds = wall_seconds() - t1;
     free(e); free(kl); free(maxc); free(treal);
     if (real_quartets) *real_quartets = nreal;
     if (flop_model) *flop_model = flops;
     return nq;
+}
+
+long long or_fourindex_eri_range(const or_basis *b, const double *s_p, const double *p_p,
+                                 double inttol, int recompute_kl, long long ij_begin,
+                                 long long ij_end, double *c_p, double *x_p,
+                                 long long *real_quartets, double *flop_model) {
+    long long n = ij_end > ij_begin ? ij_end - ij_begin : 0;
+    long long *rows = (long long *)malloc(sizeof(long long) * (n ? n : 1));
+    for (long long i = 0; i < n; ++i) rows[i] = ij_begin + i;
+    long long r = or_fourindex_eri_rows(b, s_p, p_p, inttol, recompute_kl, rows, n, c_p, x_p,
+                                        real_quartets, flop_model, NULL, NULL);
+    free(rows);
+    return r;
 }
 
 static void contract_cc(const or_basis *b, const double *m_p, double *m_c) {

@@ -76,6 +76,15 @@ long long or_fourindex_eri_range(const or_basis *b, const double *s_p,
                                  long long ij_end, double *c_p, double *x_p,
                                  long long *real_quartets, double *flop_model);
 
+/* Same engine over an explicit list of bra rows (I*npshell+J); the KL pair
+ * table and its bounds are built once (setup_seconds) and every row's work,
+ * the per-row KL rebuild included, is timed together (row_seconds). */
+long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const double *p_p,
+                                double inttol, int recompute_kl, const long long *rows,
+                                long long nrows, double *c_p, double *x_p,
+                                long long *real_quartets, double *flop_model,
+                                double *setup_seconds, double *row_seconds);
+
 /* Contracted ERI tensor FERI(i,j,k,l) = sum CC.CC.CC.CC (pq|rs), the
  * quantity src/pfourindexstore.f90:704-708 accumulates (no screening). */
 void or_eri_tensor(const or_basis *b, const double *s_p, double *feri);

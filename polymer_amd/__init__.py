@@ -1,0 +1,192 @@
+"""polymer_amd -- MI355X-native direct two-electron path for polymer.
+
+Host-side mirror of the reference's interface for this path, over the C ABI
+of libpolyeri (include/polyeri.h):
+
+    read_basis          READ3            src/pread.f90:584-1482
+    Context.fourindex_eri   FOURINDEX_ERI    src/pfourindexdirect.f90:1-814
+    Context.self_consistent_field  SELF_CONSISTENT_FIELD + FOCK_BUILD
+                                     src/pselfconsistentfield.f90:1-162,
+                                     src/pfock.f90:1-506
+    Context.mp2_energy  MP2_ENERGY_ALG1  src/pmp2energy.f90:1-571
+
+Matrices are (ncgs, ncgs) numpy arrays in the reference's contracted layout
+(ghost functions included, Fortran order).  Every compute call runs on the
+GPU; errors raise PolyeriError with the reference's message where it has one
+(e.g. 'SCF CONVERGENCE NOT MET').
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._lib import (LIB_PATH, Opts, PolyeriError, ScfOpts, ScfResult, Stats, check, lib)
+
+BASIS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basis")
+ANG_BOHR = 0.529177249  # src/pmodules.f90:9
+FP64, MIXED = 0, 1
+
+__all__ = ["read_basis", "Context", "PolyeriError", "BASIS_DIR", "ANG_BOHR", "FP64", "MIXED"]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def basis_path(name):
+    return name if os.path.sep in name else os.path.join(BASIS_DIR, name)
+
+
+class RefBasis:
+    """The reference's basis arrays for a molecule (module BASISSET)."""
+
+    def __init__(self, atoms, arrays):
+        self.atoms = atoms
+        self.__dict__.update(arrays)
+
+    @property
+    def real_functions(self):
+        return np.nonzero(np.any(self.cc != 0.0, axis=1))[0]
+
+
+def read_basis(basis, atoms):
+    """READ3 (src/pread.f90:584-1482): atoms = [(Z, x, y, z)] in bohr."""
+    L = lib()
+    z = np.ascontiguousarray([a[0] for a in atoms], dtype=np.int32)
+    xyz = np.ascontiguousarray([[a[1], a[2], a[3]] for a in atoms], dtype=np.float64)
+    n = [ctypes.c_int() for _ in range(3)]
+    path = basis_path(basis).encode()
+    check(L.polyeri_basis_read(path, len(atoms), _p(z), _p(xyz), *[ctypes.byref(x) for x in n],
+                               *([None] * 14)))
+    npgs, ncgs, nsh = [x.value for x in n]
+    ar = dict(npgs=npgs, ncgs=ncgs, npshell=nsh,
+              pax=np.zeros(npgs, np.int32), pay=np.zeros(npgs, np.int32), paz=np.zeros(npgs, np.int32),
+              pgs_atom=np.zeros(npgs, np.int32), pgsx=np.zeros(npgs), pgsy=np.zeros(npgs),
+              pgsz=np.zeros(npgs), zt=np.zeros(npgs), cc=np.zeros((ncgs, npgs), order="F"),
+              cgsx=np.zeros(ncgs), cgsy=np.zeros(ncgs), cgsz=np.zeros(ncgs),
+              p_shell=np.zeros((nsh, 5, 5, 5), np.int32), p_shell_ang=np.zeros(nsh, np.int32))
+    check(L.polyeri_basis_read(path, len(atoms), _p(z), _p(xyz), *[ctypes.byref(x) for x in n],
+                               _p(ar["pax"]), _p(ar["pay"]), _p(ar["paz"]), _p(ar["pgs_atom"]),
+                               _p(ar["pgsx"]), _p(ar["pgsy"]), _p(ar["pgsz"]), _p(ar["zt"]),
+                               _p(ar["cc"]), _p(ar["cgsx"]), _p(ar["cgsy"]), _p(ar["cgsz"]),
+                               _p(ar["p_shell"]), _p(ar["p_shell_ang"])))
+    return RefBasis([tuple(a) for a in atoms], ar)
+
+
+class Context:
+    """A libpolyeri context for one geometry and basis (device resident)."""
+
+    def __init__(self, basis, schwarz_tol=1e-12, prim_tol=1e-20, precision=FP64, mixed_cut=1e-5,
+                 device=0):
+        if not isinstance(basis, RefBasis):
+            raise TypeError("Context(basis): basis must come from read_basis()")
+        self.basis = basis
+        b = basis
+        z = np.ascontiguousarray([a[0] for a in b.atoms], dtype=np.int32)
+        xyz = np.ascontiguousarray([[a[1], a[2], a[3]] for a in b.atoms], dtype=np.float64)
+        o = Opts(schwarz_tol, prim_tol, precision, mixed_cut, device)
+        h = ctypes.c_void_p()
+        self._keep = (z, xyz)
+        check(lib().polyeri_ctx_create(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz), _p(b.pgsx),
+                                       _p(b.pgsy), _p(b.pgsz), _p(b.zt), _p(b.cc), len(b.atoms),
+                                       _p(z), _p(xyz), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self.ncgs = b.ncgs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().polyeri_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _mat(self):
+        return np.zeros((self.ncgs, self.ncgs), order="F")
+
+    def fourindex_eri(self, p_c):
+        """J (C_C) and K (X_C) from the contracted density P_C."""
+        p = np.asfortranarray(p_c, dtype=np.float64)
+        j, k = self._mat(), self._mat()
+        check(lib().polyeri_jk_build(self.h, _p(p), _p(j), _p(k)))
+        return j, k
+
+    def one_electron(self):
+        s, t, v = self._mat(), self._mat(), self._mat()
+        e = ctypes.c_double()
+        check(lib().polyeri_one_electron(self.h, _p(s), _p(t), _p(v), ctypes.byref(e)))
+        return s, t, v, e.value
+
+    def eri_tensor(self):
+        n = self.ncgs
+        f = np.zeros((n, n, n, n), order="F")
+        check(lib().polyeri_eri_tensor(self.h, _p(f)))
+        return f
+
+    def self_consistent_field(self, scfconv=1e-8, scfrelax=1.0, hfexchange=1.0, lindep=1e-6,
+                              temphf=0.0, scfcycles=50, diis=5, charge=0, spherical=False,
+                              guess=None):
+        """Defaults are the reference's keyword defaults (src/pread.f90:65-172)."""
+        o = ScfOpts(scfconv, scfrelax, hfexchange, lindep, temphf, scfcycles, diis, charge,
+                    int(spherical), 0 if guess is None else 1)
+        p = self._mat() if guess is None else np.asfortranarray(guess, dtype=np.float64).copy()
+        eps = np.zeros(self.ncgs)
+        co = self._mat()
+        trace = np.zeros(2 * scfcycles)
+        r = ScfResult()
+        rc = lib().polyeri_scf(self.h, ctypes.byref(o), _p(p), _p(eps), _p(co), _p(trace),
+                               ctypes.byref(r))
+        res = {f: getattr(r, f) for f, _ in ScfResult._fields_}
+        res.update(P=p, eps=eps, co=co, trace=trace[:2 * max(r.iterations, 0)].reshape(-1, 2))
+        if rc != 0:
+            check(rc)
+        return res
+
+    def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0):
+        co = np.asfortranarray(co, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        s, t = ctypes.c_double(), ctypes.c_double()
+        check(lib().polyeri_mp2_energy(self.h, _p(co), _p(eps), icore, iocc, nall, ivirtcore,
+                                       ctypes.byref(s), ctypes.byref(t)))
+        return s.value, t.value
+
+    # device-resident buffers (used by bench.py: inputs live in HBM)
+    def device_alloc(self, nbytes):
+        ptr = ctypes.c_void_p()
+        check(lib().polyeri_device_alloc(self.h, nbytes, ctypes.byref(ptr)))
+        return ptr
+
+    def device_free(self, ptr):
+        check(lib().polyeri_device_free(self.h, ptr))
+
+    def to_device(self, dptr, arr):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        check(lib().polyeri_memcpy(self.h, dptr, _p(a), a.nbytes, 1))
+
+    def from_device(self, dptr, shape):
+        a = np.zeros(shape, order="F")
+        check(lib().polyeri_memcpy(self.h, _p(a), dptr, a.nbytes, 0))
+        return a
+
+    def fock_build_device(self, d_p, d_f, d_j, d_k, hfexchange=1.0):
+        check(lib().polyeri_fock_build_device(self.h, d_p, hfexchange, d_f, d_j, d_k))
+
+    def sync(self):
+        check(lib().polyeri_sync(self.h))
+
+    def set_profiling(self, on=True):
+        check(lib().polyeri_set_profiling(self.h, int(on)))
+
+    def stats(self):
+        st = Stats()
+        check(lib().polyeri_get_stats(self.h, ctypes.byref(st)))
+        d = {f: getattr(st, f) for f, _ in Stats._fields_
+             if f not in ("last_class_ms", "class_prim_quartets", "class_flops", "class_l")}
+        n = st.nclass
+        d["classes"] = [dict(cls=f"{st.class_l[i]:04d}", ms=st.last_class_ms[i],
+                             prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i])
+                        for i in range(n)]
+        return d

@@ -1,0 +1,103 @@
+"""ctypes binding of libpolyeri (include/polyeri.h).
+
+The library is the product: every compute call below runs HIP kernels on the
+MI355X.  There is no CPU fallback -- if the shared library is missing or a
+device call fails, a PolyeriError is raised.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpolyeri.so")
+
+# every entry point include/polyeri.h declares
+EXPORTS = [
+    "polyeri_basis_read", "polyeri_ctx_create", "polyeri_ctx_destroy", "polyeri_jk_build",
+    "polyeri_jk_build_device", "polyeri_sync", "polyeri_one_electron", "polyeri_eri_tensor",
+    "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
+    "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
+    "polyeri_fock_build_device",
+]
+
+
+class PolyeriError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"polyeri error {code}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("schwarz_tol", ctypes.c_double), ("prim_tol", ctypes.c_double),
+                ("precision", ctypes.c_int), ("mixed_cut", ctypes.c_double),
+                ("device", ctypes.c_int)]
+
+
+class ScfOpts(ctypes.Structure):
+    _fields_ = [("scfconv", ctypes.c_double), ("scfrelax", ctypes.c_double),
+                ("hfexchange", ctypes.c_double), ("lindep", ctypes.c_double),
+                ("temphf", ctypes.c_double), ("scfcycles", ctypes.c_int),
+                ("diis", ctypes.c_int), ("charge", ctypes.c_int),
+                ("spherical", ctypes.c_int), ("guess", ctypes.c_int)]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("e_total", ctypes.c_double), ("e_nuc", ctypes.c_double),
+                ("e_kin", ctypes.c_double), ("e_nuc_att", ctypes.c_double),
+                ("e_coul", ctypes.c_double), ("e_hfx", ctypes.c_double),
+                ("fermi", ctypes.c_double), ("iterations", ctypes.c_int),
+                ("converged", ctypes.c_int), ("n_indep", ctypes.c_int),
+                ("iocc", ctypes.c_int), ("jk_seconds", ctypes.c_double),
+                ("fock_seconds", ctypes.c_double), ("scf_seconds", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int), ("nbf", ctypes.c_int), ("npair", ctypes.c_int),
+                ("nprim_pair", ctypes.c_int), ("quartets", ctypes.c_longlong),
+                ("prim_quartets", ctypes.c_double), ("model_flops", ctypes.c_double),
+                ("last_jk_ms", ctypes.c_double), ("last_class_ms", ctypes.c_double * 32),
+                ("class_prim_quartets", ctypes.c_double * 32),
+                ("class_flops", ctypes.c_double * 32), ("class_l", ctypes.c_int * 32),
+                ("nclass", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise PolyeriError(-1, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    p, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.polyeri_basis_read.argtypes = [ctypes.c_char_p, i, p, p] + [p] * 17
+    L.polyeri_ctx_create.argtypes = [i, i, p, p, p, p, p, p, p, p, i, p, p, p, ctypes.POINTER(p)]
+    L.polyeri_ctx_destroy.argtypes = [p]
+    L.polyeri_ctx_destroy.restype = None
+    L.polyeri_jk_build.argtypes = [p, p, p, p]
+    L.polyeri_jk_build_device.argtypes = [p, p, p, p]
+    L.polyeri_sync.argtypes = [p]
+    L.polyeri_one_electron.argtypes = [p, p, p, p, p]
+    L.polyeri_eri_tensor.argtypes = [p, p]
+    L.polyeri_scf.argtypes = [p, p, p, p, p, p, p]
+    L.polyeri_mp2_energy.argtypes = [p, p, p, i, i, i, i, p, p]
+    L.polyeri_get_stats.argtypes = [p, p]
+    L.polyeri_set_profiling.argtypes = [p, i]
+    L.polyeri_last_error.restype = ctypes.c_char_p
+    L.polyeri_device_alloc.argtypes = [p, ctypes.c_size_t, ctypes.POINTER(p)]
+    L.polyeri_device_free.argtypes = [p, p]
+    L.polyeri_memcpy.argtypes = [p, p, p, ctypes.c_size_t, i]
+    L.polyeri_fock_build_device.argtypes = [p, p, d, p, p, p]
+    for name in EXPORTS:
+        if name not in ("polyeri_ctx_destroy", "polyeri_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().polyeri_last_error()
+        raise PolyeriError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,167 @@
+// eri_engine.h -- per-shell-quartet electron-repulsion integrals for gfx950.
+//
+// One contracted shell quartet (ab|cd) is evaluated by looping over its
+// primitive pairs: Boys function (downward recursion from a Taylor table),
+// Obara-Saika VRR to [e0|f0] accumulated over primitives, then HRR to the
+// final Cartesian components (code from gen_eri.py).  The integrals are the
+// ones src/pfourindexdirect.f90:528-690 evaluates, over contracted shells
+// instead of primitive composite shells.
+#pragma once
+
+#include <math.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define POLY_HD __host__ __device__ __forceinline__
+#else
+#define POLY_HD inline
+#endif
+
+// src/pmodules.f90:7 -- polymer's value of pi (used by its normalisation and
+// ERI prefactors; kept so results agree with the reference to the last bits).
+#define POLY_PI 3.14159265358979
+
+// Boys table: F_m(T) on T = i/20, i = 0..BOYS_NT-1, m = 0..BOYS_NM-1,
+// row-major [i][m] so the seven values of one Taylor step are contiguous.
+#define BOYS_NT 801
+#define BOYS_NM 16
+#define BOYS_TMAX 40.0
+
+// Primitive pair of a shell pair (a,b): exponent sum, prefactor with both
+// contraction coefficients, centre P and P - A.  64 bytes.
+struct PrimPair {
+    double zeta;
+    double K;   // c_a c_b sqrt(2) pi^{5/4} exp(-ab/zeta |AB|^2) / zeta
+    double Px, Py, Pz;
+    double PAx, PAy, PAz;
+};
+
+POLY_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// F_0..F_L (T) scaled by pref.  FP64: 7-term Taylor step for F_L around the
+// nearest grid point, then the downward recursion
+// F_{m-1} = (2T F_m + e^-T) / (2m-1); large T: asymptotic F_0 and the
+// upward recursion without the (negligible) e^-T term.
+template <int L>
+POLY_HD void boys_fp64(double T, double pref, const double* __restrict__ tab,
+                       double* __restrict__ F) {
+    if (T < BOYS_TMAX) {
+        const int ts = (int)(T * 20.0 + 0.5);
+        const double h = 0.05 * ts - T;
+        const double* g = tab + ts * BOYS_NM + L;
+        double f = g[6];
+        f = g[5] + h * (1.0 / 6.0) * f;
+        f = g[4] + h * (1.0 / 5.0) * f;
+        f = g[3] + h * (1.0 / 4.0) * f;
+        f = g[2] + h * (1.0 / 3.0) * f;
+        f = g[1] + h * (1.0 / 2.0) * f;
+        f = g[0] + h * f;
+        F[L] = f;
+        if (L > 0) {
+            const double e = exp(-T);
+            const double t2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double it = 1.0 / T;
+        F[0] = 0.5 * sqrt(POLY_PI * it);
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = F[m] * (m + 0.5) * it;
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] *= pref;
+}
+
+#include "eri_generated.h"
+
+// Dispatch to the generated per-class recursion.
+template <int LA, int LB, int LC, int LD, typename real>
+struct ClassCode;
+
+#define POLY_CLASS(a, b, c, d)                                                          \
+    template <typename real>                                                            \
+    struct ClassCode<a, b, c, d, real> {                                                \
+        static POLY_HD void vrr(real PAx, real PAy, real PAz, real WPx, real WPy,       \
+                                real WPz, real QCx, real QCy, real QCz, real WQx,       \
+                                real WQy, real WQz, real oo2z, real oo2e, real oo2ze,   \
+                                real roz, real roe, const real* F, real* acc) {         \
+            vrr_##a##b##c##d<real>(PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, \
+                                   WQz, oo2z, oo2e, oo2ze, roz, roe, F, acc);           \
+        }                                                                               \
+        template <typename Cb>                                                          \
+        static POLY_HD void hrr(real ABx, real ABy, real ABz, real CDx, real CDy,       \
+                                real CDz, const real* acc, Cb& cb) {                    \
+            hrr_##a##b##c##d<real, Cb>(ABx, ABy, ABz, CDx, CDy, CDz, acc, cb);           \
+        }                                                                               \
+    };
+POLY_CLASS(0, 0, 0, 0)
+POLY_CLASS(1, 0, 0, 0)
+POLY_CLASS(1, 0, 1, 0)
+POLY_CLASS(1, 1, 0, 0)
+POLY_CLASS(1, 1, 1, 0)
+POLY_CLASS(1, 1, 1, 1)
+POLY_CLASS(2, 0, 0, 0)
+POLY_CLASS(2, 0, 1, 0)
+POLY_CLASS(2, 0, 1, 1)
+POLY_CLASS(2, 0, 2, 0)
+POLY_CLASS(2, 1, 0, 0)
+POLY_CLASS(2, 1, 1, 0)
+POLY_CLASS(2, 1, 1, 1)
+POLY_CLASS(2, 1, 2, 0)
+POLY_CLASS(2, 1, 2, 1)
+POLY_CLASS(2, 2, 0, 0)
+POLY_CLASS(2, 2, 1, 0)
+POLY_CLASS(2, 2, 1, 1)
+POLY_CLASS(2, 2, 2, 0)
+POLY_CLASS(2, 2, 2, 1)
+POLY_CLASS(2, 2, 2, 2)
+#undef POLY_CLASS
+
+// Contracted (ab|cd) for one shell quartet in FP64.  bra/ket point at the
+// primitive pairs of the two shell pairs; AB = A - B, CD = C - D.  Every
+// final integral is passed to cb(ia, ib, ic, id, value) with the component
+// indices of gen_eri.py (compile-time constants after inlining).
+template <int LA, int LB, int LC, int LD, typename Cb>
+POLY_HD void quartet_fp64(const PrimPair* __restrict__ bra, int nbra,
+                          const PrimPair* __restrict__ ket, int nket, double ABx,
+                          double ABy, double ABz, double CDx, double CDy, double CDz,
+                          const double* __restrict__ boys_tab, Cb& cb) {
+    constexpr int L = LA + LB + LC + LD;
+    // components of every l in [LA, LA+LB] (and [LC, LC+LD])
+    constexpr int NE = ((LA + LB + 1) * (LA + LB + 2) * (LA + LB + 3) - LA * (LA + 1) * (LA + 2)) / 6;
+    constexpr int NF = ((LC + LD + 1) * (LC + LD + 2) * (LC + LD + 3) - LC * (LC + 1) * (LC + 2)) / 6;
+    double acc[NE * NF];
+#pragma unroll
+    for (int i = 0; i < NE * NF; ++i) acc[i] = 0.0;
+    for (int p = 0; p < nbra; ++p) {
+        const PrimPair bp = bra[p];
+        for (int q = 0; q < nket; ++q) {
+            const PrimPair kq = ket[q];
+            const double ze = bp.zeta + kq.zeta;
+            const double ize = 1.0 / ze;
+            const double rho = bp.zeta * kq.zeta * ize;
+            const double PQx = bp.Px - kq.Px, PQy = bp.Py - kq.Py, PQz = bp.Pz - kq.Pz;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = bp.K * kq.K * sqrt(ize);
+            double F[L + 1];
+            boys_fp64<L>(T, pref, boys_tab, F);
+            const double roz = kq.zeta * ize;   // rho / zeta
+            const double roe = bp.zeta * ize;   // rho / eta
+            ClassCode<LA, LB, LC, LD, double>::vrr(
+                bp.PAx, bp.PAy, bp.PAz, -roz * PQx, -roz * PQy, -roz * PQz, kq.PAx, kq.PAy,
+                kq.PAz, roe * PQx, roe * PQy, roe * PQz, 0.5 / bp.zeta, 0.5 / kq.zeta, 0.5 * ize,
+                roz, roe, F, acc);
+        }
+    }
+    ClassCode<LA, LB, LC, LD, double>::hrr(ABx, ABy, ABz, CDx, CDy, CDz, acc, cb);
+}
+
+// Store variant: out[((ia*NB + ib)*NC + ic)*ND + id].
+template <int NB, int NC, int ND>
+struct StoreCb {
+    double* out;
+    POLY_HD void operator()(int a, int b, int c, int d, double v) {
+        out[((a * NB + b) * NC + c) * ND + d] = v;
+    }
+};

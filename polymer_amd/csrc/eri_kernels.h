@@ -1,0 +1,192 @@
+// eri_kernels.h -- the shell-quartet kernel template of libpolyeri.
+//
+// One thread per contracted shell quartet (ab|cd) of one class: primitive
+// loop, Boys + VRR + HRR (eri_engine.h), then the quartet's contribution
+// to the Coulomb and exchange matrices with FP64 atomics (8-fold
+// permutational symmetry: each unique quartet is computed once).  Each
+// class is instantiated in its own translation unit (kgen_*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "eri_engine.h"
+
+enum { MODE_JK = 0, MODE_SCHWARZ = 1, MODE_TENSOR = 2 };
+
+struct PairMeta {
+    int prim_off, nprim, fa, fb;
+};
+
+struct EriArgs {
+    const PrimPair* prims;
+    const PairMeta* meta;
+    const double* ab;          // 4 per pair: ABx ABy ABz 0
+    const int* bra_ids;
+    const int* ket_ids;
+    const long long* row_off;  // nrows + 1
+    int nrows;
+    long long nwork;
+    const double* boys;
+    const double* D;
+    double* J;
+    double* K;
+    int nbf;
+    double* out;               // tensor (MODE_TENSOR) or Schwarz (MODE_SCHWARZ)
+};
+
+// Digestion with the quartet's six density / Fock blocks held in registers
+// (light classes).  With s the permutational degeneracy weight:
+// Jacc[ab] += s v D[cd], Jacc[cd] += s v D[ab], Kacc[ac] += s v D[bd],
+// Kacc[ad] += s v D[bc], Kacc[bc] += s v D[ad], Kacc[bd] += s v D[ac];
+// afterwards J = 2(Jacc + Jacc^T) and K = Kacc + Kacc^T.
+template <int NA, int NB, int NC, int ND>
+struct RegDigest {
+    double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dad[NA * ND], Dbc[NB * NC], Dbd[NB * ND];
+    double Jab[NA * NB], Jcd[NC * ND], Kac[NA * NC], Kad[NA * ND], Kbc[NB * NC], Kbd[NB * ND];
+    __device__ __forceinline__ void load(const double* __restrict__ D, int n, int fa, int fb, int fc, int fd) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) { Dab[a * NB + b] = D[(size_t)(fa + a) * n + fb + b]; Jab[a * NB + b] = 0.0; }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) { Dac[a * NC + c] = D[(size_t)(fa + a) * n + fc + c]; Kac[a * NC + c] = 0.0; }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) { Dad[a * ND + d] = D[(size_t)(fa + a) * n + fd + d]; Kad[a * ND + d] = 0.0; }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) { Dbc[b * NC + c] = D[(size_t)(fb + b) * n + fc + c]; Kbc[b * NC + c] = 0.0; }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) { Dbd[b * ND + d] = D[(size_t)(fb + b) * n + fd + d]; Kbd[b * ND + d] = 0.0; }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) { Dcd[c * ND + d] = D[(size_t)(fc + c) * n + fd + d]; Jcd[c * ND + d] = 0.0; }
+    }
+    __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
+        Jab[a * NB + b] += x * Dcd[c * ND + d];
+        Jcd[c * ND + d] += x * Dab[a * NB + b];
+        Kac[a * NC + c] += x * Dbd[b * ND + d];
+        Kad[a * ND + d] += x * Dbc[b * NC + c];
+        Kbc[b * NC + c] += x * Dad[a * ND + d];
+        Kbd[b * ND + d] += x * Dac[a * NC + c];
+    }
+    __device__ __forceinline__ void flush(double* J, double* K, int n, int fa, int fb, int fc, int fd, double s) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) atomicAdd(&J[(size_t)(fa + a) * n + fb + b], s * Jab[a * NB + b]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) atomicAdd(&K[(size_t)(fa + a) * n + fc + c], s * Kac[a * NC + c]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomicAdd(&K[(size_t)(fa + a) * n + fd + d], s * Kad[a * ND + d]);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) atomicAdd(&K[(size_t)(fb + b) * n + fc + c], s * Kbc[b * NC + c]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomicAdd(&K[(size_t)(fb + b) * n + fd + d], s * Kbd[b * ND + d]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) atomicAdd(&J[(size_t)(fc + c) * n + fd + d], s * Jcd[c * ND + d]);
+    }
+};
+
+// Digestion for heavy classes: each integral goes straight to the six
+// matrix elements it touches (density read through the cache), so no block
+// of integrals or accumulators is held per thread.
+struct StreamDigest {
+    const double* __restrict__ D;
+    double* J;
+    double* K;
+    int n, fa, fb, fc, fd;
+    double s;
+    __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
+        const size_t A = fa + a, B = fb + b, C = fc + c, E = fd + d;
+        const double sx = s * x;
+        atomicAdd(&J[A * n + B], sx * D[C * n + E]);
+        atomicAdd(&J[C * n + E], sx * D[A * n + B]);
+        atomicAdd(&K[A * n + C], sx * D[B * n + E]);
+        atomicAdd(&K[A * n + E], sx * D[B * n + C]);
+        atomicAdd(&K[B * n + C], sx * D[A * n + E]);
+        atomicAdd(&K[B * n + E], sx * D[A * n + C]);
+    }
+};
+
+struct SchwarzCb {
+    double m = 0.0;
+    __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
+        if (a == c && b == d) m = fmax(m, fabs(x));
+    }
+};
+
+struct TensorCb {
+    double* out;
+    size_t n;
+    int fa, fb, fc, fd;
+    __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
+        const size_t n1 = n, n2 = n1 * n, n3 = n2 * n;
+        const size_t i = fa + a, j = fb + b, k = fc + c, l = fd + d;
+        out[i * n3 + j * n2 + k * n1 + l] = x;
+        out[j * n3 + i * n2 + k * n1 + l] = x;
+        out[i * n3 + j * n2 + l * n1 + k] = x;
+        out[j * n3 + i * n2 + l * n1 + k] = x;
+        out[k * n3 + l * n2 + i * n1 + j] = x;
+        out[l * n3 + k * n2 + i * n1 + j] = x;
+        out[k * n3 + l * n2 + j * n1 + i] = x;
+        out[l * n3 + k * n2 + j * n1 + i] = x;
+    }
+};
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ void __launch_bounds__(256) k_eri(EriArgs g) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nwork) return;
+    int bp, kp;
+    if (MODE == MODE_SCHWARZ) {
+        bp = kp = g.bra_ids[t];
+    } else {
+        int lo = 0, hi = g.nrows - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (g.row_off[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        bp = g.bra_ids[lo];
+        kp = g.ket_ids[t - g.row_off[lo]];
+    }
+    const PairMeta mb = g.meta[bp], mk = g.meta[kp];
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
+    const double CDx = g.ab[4 * kp], CDy = g.ab[4 * kp + 1], CDz = g.ab[4 * kp + 2];
+    const PrimPair* bra = g.prims + mb.prim_off;
+    const PrimPair* ket = g.prims + mk.prim_off;
+    const int fa = mb.fa, fb = mb.fb, fc = mk.fa, fd = mk.fb;
+    const int n = g.nbf;
+    if (MODE == MODE_SCHWARZ) {
+        SchwarzCb cb;
+        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+        g.out[bp] = sqrt(cb.m);
+    } else if (MODE == MODE_TENSOR) {
+        TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
+        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+    } else {
+        const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
+        if constexpr (NA * NB * NC * ND > 108) {
+            StreamDigest cb{g.D, g.J, g.K, n, fa, fb, fc, fd, s};
+            quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+        } else {
+            RegDigest<NA, NB, NC, ND> cb;
+            cb.load(g.D, n, fa, fb, fc, fd);
+            quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+            cb.flush(g.J, g.K, n, fa, fb, fc, fd, s);
+        }
+    }
+}
+
+typedef void (*EriKernel)(EriArgs);
+// Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).
+EriKernel polyeri_class_kernel(int la, int lb, int lc, int ld, int mode);

@@ -1,0 +1,1033 @@
+// polyeri.hip -- libpolyeri: MI355X (gfx950) direct two-electron path.
+//
+// Device side: one kernel instantiation per shell-quartet class
+// (la lb|lc ld), one thread per contracted shell quartet, primitive loop,
+// Boys + VRR + HRR from eri_engine.h, then the quartet's contribution to the
+// Coulomb and exchange matrices with FP64 atomics (8-fold permutational
+// symmetry, so each unique quartet is computed once).
+// Host side: context set-up, class-sorted Schwarz-screened work lists, the
+// J/K build that replaces FOURINDEX_ERI, the SCF loop and MP2.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/polyeri.h"
+#include "polyeri_internal.h"
+
+using namespace polyeri;
+
+static thread_local std::string g_err;
+
+const char* polyeri_last_error(void) { return g_err.c_str(); }
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? POLYERI_E_NOMEM : POLYERI_E_HIP,     \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+#define RBCHK(x)                                                                         \
+    do {                                                                                 \
+        int s_ = (int)(x);                                                               \
+        if (s_ != 0) return fail(POLYERI_E_LIB, std::string(#x) + " failed, status " + std::to_string(s_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+#include "eri_kernels.h"
+
+__global__ void k_fill(double* p, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// D_int[mu][nu] = s_mu s_nu P_C(ref mu, ref nu)
+__global__ void k_pack_density(const double* __restrict__ pc, int nc, const int* __restrict__ ref,
+                               const double* __restrict__ sc, int nbf, double* __restrict__ D) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nbf * nbf) return;
+    int mu = (int)(t / nbf), nu = (int)(t % nbf);
+    D[t] = sc[mu] * sc[nu] * pc[ref[mu] + (size_t)nc * ref[nu]];
+}
+
+// C_C(ref mu, ref nu) = s s 2(J + J^T), X_C = s s (K + K^T)
+__global__ void k_unpack_jk(const double* __restrict__ J, const double* __restrict__ K, int nbf,
+                            const int* __restrict__ ref, const double* __restrict__ sc, int nc,
+                            double* __restrict__ jc, double* __restrict__ kc) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nbf * nbf) return;
+    int mu = (int)(t / nbf), nu = (int)(t % nbf);
+    double f = sc[mu] * sc[nu];
+    size_t r = ref[mu] + (size_t)nc * ref[nu];
+    jc[r] = f * 2.0 * (J[(size_t)mu * nbf + nu] + J[(size_t)nu * nbf + mu]);
+    kc[r] = f * (K[(size_t)mu * nbf + nu] + K[(size_t)nu * nbf + mu]);
+}
+
+// F = T + V + J - hfx/2 K, then the Hermitian average (src/pfock.f90:44, :224-233)
+__global__ void k_fock(const double* __restrict__ T, const double* __restrict__ V,
+                       const double* __restrict__ J, const double* __restrict__ K, double hfx, int n,
+                       double* __restrict__ F) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    int i = (int)(t % n), j = (int)(t / n);
+    size_t ij = t, ji = (size_t)i * n + j;
+    double a = T[ij] + V[ij] + J[ij] - 0.5 * hfx * K[ij];
+    double b = T[ji] + V[ji] + J[ji] - 0.5 * hfx * K[ji];
+    F[ij] = 0.5 * (a + b);
+}
+
+// X(:, k) = U(:, keep[k]) / sqrt(e[keep[k]])
+__global__ void k_orth(const double* __restrict__ U, const double* __restrict__ e,
+                       const int* __restrict__ keep, int n, int nk, double* __restrict__ X) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nk) return;
+    int r = (int)(t % n), k = (int)(t / n);
+    X[t] = U[r + (size_t)n * keep[k]] / sqrt(e[keep[k]]);
+}
+
+// W(:, l) = occ[l] * C(:, l)
+__global__ void k_scale_cols(const double* __restrict__ C, const double* __restrict__ occ, int n,
+                             int m, double* __restrict__ W) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * m) return;
+    W[t] = C[t] * occ[t / n];
+}
+
+// Cint[mu, i] = s_mu CO[ref mu, i]
+__global__ void k_mo_int(const double* __restrict__ co, int nc, const int* __restrict__ ref,
+                         const double* __restrict__ sc, int nbf, int nmo, double* __restrict__ C) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nbf * nmo) return;
+    int mu = (int)(t % nbf), i = (int)(t / nbf);
+    C[t] = sc[mu] * co[ref[mu] + (size_t)nc * i];
+}
+
+// E(2) from T(b, j, a, i) = (ia|jb): src/pmp2energy.f90:395-444 at one k-point
+__global__ void k_mp2_energy(const double* __restrict__ T, const double* __restrict__ eps, int no,
+                             int nv, int icore, int iocc, double* __restrict__ acc) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t tot = (size_t)no * nv * no * nv;
+    double es = 0.0, et = 0.0;
+    if (t < tot) {
+        int b = (int)(t % nv);
+        size_t r = t / nv;
+        int j = (int)(r % no); r /= no;
+        int a = (int)(r % nv);
+        int i = (int)(r / nv);
+        double x = T[t];
+        double xp = T[b + (size_t)nv * (i + (size_t)no * (a + (size_t)nv * j))];  // (ja|ib)
+        double den = eps[icore + i] + eps[icore + j] - eps[iocc + a] - eps[iocc + b];
+        es = (0.5 * x * x + 0.5 * x * xp) / den;
+        et = (1.5 * x * x - 1.5 * x * xp) / den;
+    }
+    __shared__ double ss[256], st[256];
+    ss[threadIdx.x] = es;
+    st[threadIdx.x] = et;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            st[threadIdx.x] += st[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], ss[0]);
+        atomicAdd(&acc[1], st[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch table
+// ---------------------------------------------------------------------------
+template <int MODE>
+static EriKernel kernel_for(int la, int lb, int lc, int ld) {
+    return polyeri_class_kernel(la, lb, lc, ld, MODE);
+}
+
+static const int kPairL[NPAIRCLASS][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+
+static int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Algorithmic FLOPs of one primitive quartet of class (la lb|lc ld): Boys +
+// pair assembly, the OS VRR targets (8 flops each), and J/K digestion under
+// 8-fold symmetry.  The same model prices the CPU baseline.
+static double flop_model(int la, int lb, int lc, int ld) {
+    int L = la + lb + lc + ld;
+    double f = 25 + 3 * L, v = 0;
+    for (int e = 0; e <= la + lb; ++e)
+        for (int g = 0; g <= lc + ld; ++g)
+            if (e + g >= 1) v += (double)ncart_h(e) * ncart_h(g) * (L - e - g + 1);
+    return f + 8 * v + 12.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct ClassPairWork {
+    int X, Y;
+    int la, lb, lc, ld;
+    int* d_ket;            // ket ids (class Y, sorted by Q desc)
+    int* d_bra;            // bra ids (class X, sorted by Q desc)
+    long long* d_row;
+    int nrows;
+    long long nwork;
+    double prim_quartets;
+    double flops;
+};
+
+struct polyeri_ctx {
+    RefBasis rb;
+    IntBasis ib;
+    std::vector<PairInfo> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<double> Q;
+    polyeri_opts opts;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    PrimPair* d_prims = nullptr;
+    PairMeta* d_meta = nullptr;
+    double* d_ab = nullptr;
+    double* d_boys = nullptr;
+    double* d_Q = nullptr;
+    int* d_class_ids[NPAIRCLASS] = {};
+    std::vector<int> class_ids[NPAIRCLASS];
+    std::vector<ClassPairWork> work;
+    int* d_ref = nullptr;
+    double* d_scale = nullptr;
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
+    double *d_pc = nullptr, *d_jc = nullptr, *d_kc = nullptr;
+    // one-electron (reference layout)
+    std::vector<double> S_C, T_C, N_C;
+    double e_nuc = 0.0;
+    // stats
+    long long quartets = 0;
+    double prim_quartets = 0, model_flops = 0;
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_jk_ms = 0;
+    std::vector<double> last_class_ms;
+    rocblas_handle blas = nullptr;
+    double *d_T = nullptr, *d_V = nullptr;   // T_C, N_C on the device (Fock assembly)
+};
+
+static int ensure_onee_device(polyeri_ctx* c) {
+    if (c->d_T) return 0;
+    size_t n2 = c->T_C.size();
+    HIPCHK(hipMalloc((void**)&c->d_T, n2 * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->d_V, n2 * sizeof(double)));
+    HIPCHK(hipMemcpy(c->d_T, c->T_C.data(), n2 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_V, c->N_C.data(), n2 * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(POLYERI_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
+                        const int* ket, const long long* row, int nrows, long long nwork, double* out) {
+    if (nwork <= 0) return 0;
+    EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
+                : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
+                : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
+    if (!k) return fail(POLYERI_E_ARG, "no kernel for class");
+    EriArgs a;
+    a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
+    a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
+    a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
+    hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One stream (one hardware queue) per device shared by every context: the
+// device's scratch reservation for the spilling high-l kernels is then made
+// once, not once per context.
+static hipStream_t shared_stream(int device, int* rc) {
+    static std::mutex mu;
+    static std::vector<hipStream_t> streams;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)streams.size() <= device) streams.resize(device + 1, nullptr);
+    if (!streams[device]) {
+        hipError_t e = hipStreamCreateWithFlags(&streams[device], hipStreamNonBlocking);
+        if (e != hipSuccess) { *rc = fail(POLYERI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); return nullptr; }
+    }
+    *rc = 0;
+    return streams[device];
+}
+
+static void free_ctx(polyeri_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
+    for (int i = 0; i < NPAIRCLASS; ++i) hipFree(c->d_class_ids[i]);
+    for (auto& w : c->work) hipFree(w.d_row);
+    hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
+    hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
+    for (auto e : c->ev) hipEventDestroy(e);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+void polyeri_ctx_destroy(polyeri_ctx* ctx) { free_ctx(ctx); }
+
+static int build_ctx(polyeri_ctx* c) {
+    std::string err;
+    if (extract_shells(c->rb, c->ib, err)) return fail(POLYERI_E_BASIS, err);
+    build_pairs(c->ib, c->opts.prim_tol, c->pairs, c->prims);
+    HIPCHK(hipSetDevice(c->device));
+    int src = 0;
+    c->stream = shared_stream(c->device, &src);
+    if (src) return src;
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    int np = (int)c->pairs.size(), nbf = c->ib.nbf, nc = c->rb.ncgs;
+    std::vector<PairMeta> meta(np);
+    std::vector<double> ab(4 * (size_t)np);
+    for (int i = 0; i < np; ++i) {
+        const PairInfo& p = c->pairs[i];
+        meta[i] = {p.prim_off, p.nprim, c->ib.shells[p.a].f0, c->ib.shells[p.b].f0};
+        ab[4 * i] = p.ABx; ab[4 * i + 1] = p.ABy; ab[4 * i + 2] = p.ABz; ab[4 * i + 3] = 0;
+    }
+    std::vector<double> boys;
+    boys_table(boys);
+    int rc;
+    if ((rc = dalloc(&c->d_prims, c->prims.size()))) return rc;
+    if ((rc = dalloc(&c->d_meta, np))) return rc;
+    if ((rc = dalloc(&c->d_ab, 4 * (size_t)np))) return rc;
+    if ((rc = dalloc(&c->d_boys, boys.size()))) return rc;
+    if ((rc = dalloc(&c->d_Q, np))) return rc;
+    if ((rc = dalloc(&c->d_ref, nbf))) return rc;
+    if ((rc = dalloc(&c->d_scale, nbf))) return rc;
+    if ((rc = dalloc(&c->d_D, (size_t)nbf * nbf))) return rc;
+    if ((rc = dalloc(&c->d_J, (size_t)nbf * nbf))) return rc;
+    if ((rc = dalloc(&c->d_K, (size_t)nbf * nbf))) return rc;
+    if ((rc = dalloc(&c->d_pc, (size_t)nc * nc))) return rc;
+    if ((rc = dalloc(&c->d_jc, (size_t)nc * nc))) return rc;
+    if ((rc = dalloc(&c->d_kc, (size_t)nc * nc))) return rc;
+    HIPCHK(hipMemcpy(c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_meta, meta.data(), np * sizeof(PairMeta), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_ab, ab.data(), ab.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_boys, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_ref, c->ib.ref_of_int.data(), nbf * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_scale, c->ib.scale.data(), nbf * sizeof(double), hipMemcpyHostToDevice));
+    // Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), one class at a time
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        c->class_ids[X].clear();
+        for (int i = 0; i < np; ++i) if (c->pairs[i].cls == X) c->class_ids[X].push_back(i);
+        if ((rc = dalloc(&c->d_class_ids[X], c->class_ids[X].size()))) return rc;
+        if (c->class_ids[X].empty()) continue;
+        HIPCHK(hipMemcpy(c->d_class_ids[X], c->class_ids[X].data(), c->class_ids[X].size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+        int la = kPairL[X][0], lb = kPairL[X][1];
+        if ((rc = launch_class(c, MODE_SCHWARZ, la, lb, la, lb, c->d_class_ids[X], nullptr, nullptr, 0,
+                               (long long)c->class_ids[X].size(), c->d_Q)))
+            return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->Q.resize(np);
+    HIPCHK(hipMemcpy(c->Q.data(), c->d_Q, np * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i)
+        if (!std::isfinite(c->Q[i])) return fail(POLYERI_E_NONFINITE, "non-finite Schwarz bound");
+    // class lists sorted by Q descending (Schwarz screening becomes a prefix)
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        auto& ids = c->class_ids[X];
+        std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return c->Q[a] > c->Q[b]; });
+        if (!ids.empty())
+            HIPCHK(hipMemcpy(c->d_class_ids[X], ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    // work lists per class pair X >= Y: bra i in X, ket j in Y with
+    // Q_i Q_j >= tol (and j <= i within one class)
+    const double tol = c->opts.schwarz_tol;
+    c->quartets = 0; c->prim_quartets = 0; c->model_flops = 0;
+    for (int X = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const auto& bx = c->class_ids[X];
+            const auto& ky = c->class_ids[Y];
+            ClassPairWork w;
+            w.X = X; w.Y = Y;
+            w.la = kPairL[X][0]; w.lb = kPairL[X][1]; w.lc = kPairL[Y][0]; w.ld = kPairL[Y][1];
+            w.d_bra = c->d_class_ids[X]; w.d_ket = c->d_class_ids[Y];
+            w.nrows = (int)bx.size();
+            std::vector<long long> row(bx.size() + 1, 0);
+            std::vector<double> kprim(ky.size() + 1, 0.0);   // prefix of ket primitive pairs
+            for (size_t j = 0; j < ky.size(); ++j) kprim[j + 1] = kprim[j] + c->pairs[ky[j]].nprim;
+            double pq = 0;
+            for (size_t i = 0; i < bx.size(); ++i) {
+                double qi = c->Q[bx[i]];
+                // number of ket pairs with Q_j >= tol / qi
+                size_t lo = 0, hi = ky.size();
+                while (lo < hi) {
+                    size_t mid = (lo + hi) / 2;
+                    if (qi * c->Q[ky[mid]] >= tol) lo = mid + 1; else hi = mid;
+                }
+                size_t cnt = lo;
+                if (X == Y) cnt = std::min(cnt, i + 1);
+                row[i + 1] = row[i] + (long long)cnt;
+                pq += c->pairs[bx[i]].nprim * kprim[cnt];
+            }
+            w.nwork = row[bx.size()];
+            w.prim_quartets = pq;
+            w.flops = pq * flop_model(w.la, w.lb, w.lc, w.ld);
+            w.d_row = nullptr;
+            if ((rc = dalloc(&w.d_row, row.size()))) return rc;
+            HIPCHK(hipMemcpy(w.d_row, row.data(), row.size() * sizeof(long long), hipMemcpyHostToDevice));
+            c->quartets += w.nwork;
+            c->prim_quartets += pq;
+            c->model_flops += pq * flop_model(w.la, w.lb, w.lc, w.ld);
+            c->work.push_back(w);
+        }
+    c->ev.resize(c->work.size() + 1);
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    c->last_class_ms.assign(c->work.size(), 0.0);
+    // one-electron matrices in the reference layout
+    std::vector<double> S, T, V;
+    one_electron(c->ib, boys, S, T, V);
+    c->S_C.assign((size_t)nc * nc, 0.0); c->T_C.assign((size_t)nc * nc, 0.0); c->N_C.assign((size_t)nc * nc, 0.0);
+    for (int a = 0; a < nbf; ++a)
+        for (int b = 0; b < nbf; ++b) {
+            size_t r = c->ib.ref_of_int[a] + (size_t)nc * c->ib.ref_of_int[b];
+            double f = c->ib.scale[a] * c->ib.scale[b];
+            c->S_C[r] = f * S[(size_t)a * nbf + b];
+            c->T_C[r] = f * T[(size_t)a * nbf + b];
+            c->N_C[r] = f * V[(size_t)a * nbf + b];
+        }
+    c->e_nuc = nuclear_repulsion(c->ib);
+    return 0;
+}
+
+static void default_opts(polyeri_opts* o) {
+    o->schwarz_tol = 1e-12;
+    o->prim_tol = 1e-20;
+    o->precision = POLYERI_FP64;
+    o->mixed_cut = 1e-5;
+    o->device = 0;
+}
+
+int polyeri_ctx_create(int npgs, int ncgs, const int* pax, const int* pay, const int* paz,
+                       const double* pgsx, const double* pgsy, const double* pgsz, const double* zt,
+                       const double* cc, int natom, const int* iatom, const double* xyz,
+                       const polyeri_opts* opts, polyeri_ctx** out) {
+    if (!out || npgs <= 0 || ncgs <= 0 || !pax || !pay || !paz || !pgsx || !pgsy || !pgsz || !zt || !cc ||
+        natom <= 0 || !iatom || !xyz)
+        return fail(POLYERI_E_ARG, "polyeri_ctx_create: null or empty argument");
+    *out = nullptr;
+    polyeri_ctx* c = new polyeri_ctx();
+    if (opts) c->opts = *opts; else default_opts(&c->opts);
+    c->device = c->opts.device;
+    RefBasis& rb = c->rb;
+    rb.natom = natom;
+    rb.iatom.assign(iatom, iatom + natom);
+    for (int i = 0; i < natom; ++i) {
+        rb.ax.push_back(xyz[3 * i]); rb.ay.push_back(xyz[3 * i + 1]); rb.az.push_back(xyz[3 * i + 2]);
+    }
+    rb.npgs = npgs; rb.ncgs = ncgs;
+    rb.pax.assign(pax, pax + npgs); rb.pay.assign(pay, pay + npgs); rb.paz.assign(paz, paz + npgs);
+    rb.pgsx.assign(pgsx, pgsx + npgs); rb.pgsy.assign(pgsy, pgsy + npgs); rb.pgsz.assign(pgsz, pgsz + npgs);
+    rb.zt.assign(zt, zt + npgs);
+    rb.cc.assign(cc, cc + (size_t)npgs * ncgs);
+    rb.pgs_atom.assign(npgs, 0);
+    int rc = build_ctx(c);
+    if (rc) { free_ctx(c); return rc; }
+    *out = c;
+    return 0;
+}
+
+int polyeri_basis_read(const char* path, int natom, const int* iatom, const double* xyz, int* npgs,
+                       int* ncgs, int* npshell, int* pax, int* pay, int* paz, int* pgs_atom,
+                       double* pgsx, double* pgsy, double* pgsz, double* zt, double* cc, double* cgsx,
+                       double* cgsy, double* cgsz, int* p_shell, int* p_shell_ang) {
+    if (!path || natom <= 0 || !iatom || !xyz || !npgs || !ncgs || !npshell)
+        return fail(POLYERI_E_ARG, "polyeri_basis_read: bad argument");
+    RefBasis rb;
+    std::string err;
+    if (read_basis(path, natom, iatom, xyz, rb, err)) return fail(POLYERI_E_BASIS, err);
+    *npgs = rb.npgs; *ncgs = rb.ncgs; *npshell = rb.npshell;
+    auto cpi = [](int* d, const std::vector<int>& s) { if (d) std::copy(s.begin(), s.end(), d); };
+    auto cpd = [](double* d, const std::vector<double>& s) { if (d) std::copy(s.begin(), s.end(), d); };
+    cpi(pax, rb.pax); cpi(pay, rb.pay); cpi(paz, rb.paz); cpi(pgs_atom, rb.pgs_atom);
+    cpd(pgsx, rb.pgsx); cpd(pgsy, rb.pgsy); cpd(pgsz, rb.pgsz); cpd(zt, rb.zt); cpd(cc, rb.cc);
+    cpd(cgsx, rb.cgsx); cpd(cgsy, rb.cgsy); cpd(cgsz, rb.cgsz);
+    cpi(p_shell, rb.p_shell); cpi(p_shell_ang, rb.p_shell_ang);
+    return 0;
+}
+
+int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    ctx->profiling = on != 0;
+    return 0;
+}
+
+// J/K from the context's device density (reference layout, ncgs^2).
+static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc) {
+    int nbf = c->ib.nbf, nc = c->rb.ncgs;
+    size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_pack_density, dim3(nblk(n2)), dim3(256), 0, c->stream, d_pc, nc, c->d_ref,
+                       c->d_scale, nbf, c->d_D);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_K, n2, 0.0);
+    HIPCHK(hipGetLastError());
+    for (size_t w = 0; w < c->work.size(); ++w) {
+        const ClassPairWork& cw = c->work[w];
+        if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
+        int rc = launch_class(c, MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
+                              cw.nwork, nullptr);
+        if (rc) return rc;
+    }
+    if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_jc, c2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_kc, c2, 0.0);
+    hipLaunchKernelGGL(k_unpack_jk, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, c->d_K, nbf, c->d_ref,
+                       c->d_scale, nc, d_jc, d_kc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return 0;
+}
+
+static int jk_collect_timing(polyeri_ctx* c) {
+    float ms = 0;
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_jk_ms = ms;
+    if (c->profiling)
+        for (size_t w = 0; w < c->work.size(); ++w) {
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, c->ev[w], c->ev[w + 1]));
+            c->last_class_ms[w] = m;
+        }
+    return 0;
+}
+
+int polyeri_jk_build_device(polyeri_ctx* ctx, const double* d_p_c, double* d_c_c, double* d_x_c) {
+    if (!ctx || !d_p_c || !d_c_c || !d_x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build_device: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    return jk_device(ctx, d_p_c, d_c_c, d_x_c);
+}
+
+int polyeri_sync(polyeri_ctx* ctx) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return jk_collect_timing(ctx);
+}
+
+int polyeri_jk_build(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
+    if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t c2 = (size_t)ctx->rb.ncgs * ctx->rb.ncgs;
+    HIPCHK(hipMemcpyAsync(ctx->d_pc, p_c, c2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = jk_device(ctx, ctx->d_pc, ctx->d_jc, ctx->d_kc);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c_c, ctx->d_jc, c2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(x_c, ctx->d_kc, c2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = jk_collect_timing(ctx))) return rc;
+    for (size_t i = 0; i < c2; ++i)
+        if (!std::isfinite(c_c[i]) || !std::isfinite(x_c[i])) return fail(POLYERI_E_NONFINITE, "non-finite J/K");
+    return 0;
+}
+
+int polyeri_device_alloc(polyeri_ctx* ctx, size_t bytes, void** ptr) {
+    if (!ctx || !ptr) return fail(POLYERI_E_ARG, "polyeri_device_alloc: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int polyeri_device_free(polyeri_ctx* ctx, void* ptr) {
+    if (!ctx) return fail(POLYERI_E_ARG, "polyeri_device_free: null context");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(ptr));
+    return 0;
+}
+
+int polyeri_memcpy(polyeri_ctx* ctx, void* dst, const void* src, size_t bytes, int to_device) {
+    if (!ctx || !dst || !src) return fail(POLYERI_E_ARG, "polyeri_memcpy: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int polyeri_fock_build_device(polyeri_ctx* ctx, const double* d_p_c, double hfexchange, double* d_f_c,
+                              double* d_c_c, double* d_x_c) {
+    if (!ctx || !d_p_c || !d_f_c || !d_c_c || !d_x_c) return fail(POLYERI_E_ARG, "polyeri_fock_build_device: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_onee_device(ctx);
+    if (rc) return rc;
+    if ((rc = jk_device(ctx, d_p_c, d_c_c, d_x_c))) return rc;
+    int nc = ctx->rb.ncgs;
+    hipLaunchKernelGGL(k_fock, dim3(nblk((size_t)nc * nc)), dim3(256), 0, ctx->stream, ctx->d_T, ctx->d_V, d_c_c, d_x_c,
+                       hfexchange, nc, d_f_c);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int polyeri_one_electron(polyeri_ctx* ctx, double* s_c, double* t_c, double* n_c, double* e_nuc) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    if (s_c) std::copy(ctx->S_C.begin(), ctx->S_C.end(), s_c);
+    if (t_c) std::copy(ctx->T_C.begin(), ctx->T_C.end(), t_c);
+    if (n_c) std::copy(ctx->N_C.begin(), ctx->N_C.end(), n_c);
+    if (e_nuc) *e_nuc = ctx->e_nuc;
+    return 0;
+}
+
+// Full internal ERI tensor on the device (nbf^4), screened like the J/K build.
+static int tensor_device(polyeri_ctx* c, double* d_t) {
+    size_t n = c->ib.nbf;
+    size_t n4 = n * n * n * n;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n4)), dim3(256), 0, c->stream, d_t, n4, 0.0);
+    HIPCHK(hipGetLastError());
+    for (const ClassPairWork& cw : c->work) {
+        int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
+                              cw.nrows, cw.nwork, d_t);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int polyeri_eri_tensor(polyeri_ctx* ctx, double* feri) {
+    if (!ctx || !feri) return fail(POLYERI_E_ARG, "polyeri_eri_tensor: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t n = ctx->ib.nbf, nc = ctx->rb.ncgs;
+    double* d_t = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_t, n * n * n * n))) return rc;
+    rc = tensor_device(ctx, d_t);
+    std::vector<double> t;
+    if (!rc) {
+        t.resize(n * n * n * n);
+        hipError_t e = hipMemcpyAsync(t.data(), d_t, t.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(POLYERI_E_HIP, hipGetErrorString(e));
+    }
+    hipFree(d_t);
+    if (rc) return rc;
+    std::fill(feri, feri + nc * nc * nc * nc, 0.0);
+    const auto& ref = ctx->ib.ref_of_int;
+    const auto& sc = ctx->ib.scale;
+    for (size_t a = 0; a < n; ++a)
+        for (size_t b = 0; b < n; ++b)
+            for (size_t cc = 0; cc < n; ++cc)
+                for (size_t d = 0; d < n; ++d)
+                    feri[ref[a] + nc * (ref[b] + nc * (ref[cc] + nc * ref[d]))] =
+                        sc[a] * sc[b] * sc[cc] * sc[d] * t[((a * n + b) * n + cc) * n + d];
+    return 0;
+}
+
+int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
+    if (!ctx || !st) return fail(POLYERI_E_ARG, "null argument");
+    memset(st, 0, sizeof(*st));
+    st->nshell = (int)ctx->ib.shells.size();
+    st->nbf = ctx->ib.nbf;
+    st->npair = (int)ctx->pairs.size();
+    st->nprim_pair = (int)ctx->prims.size();
+    st->quartets = ctx->quartets;
+    st->prim_quartets = ctx->prim_quartets;
+    st->model_flops = ctx->model_flops;
+    st->last_jk_ms = ctx->last_jk_ms;
+    st->nclass = (int)std::min<size_t>(32, ctx->work.size());
+    for (int i = 0; i < st->nclass; ++i) {
+        const ClassPairWork& w = ctx->work[i];
+        st->last_class_ms[i] = ctx->last_class_ms[i];
+        st->class_prim_quartets[i] = w.prim_quartets;
+        st->class_flops[i] = w.flops;
+        st->class_l[i] = w.la * 1000 + w.lb * 100 + w.lc * 10 + w.ld;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SCF on the device
+// ---------------------------------------------------------------------------
+// src/pmodules.f90:12
+static const double kBoltzmann = 1.0 / 3.1577464e5;
+
+// Solve the DIIS system with partial pivoting and one refinement step (the
+// role of LUDCMP / LUBKSB / MPROVE, src/pdensitymatrix.f90:385-512).
+static void lu_solve(int n, const std::vector<double>& A0, const std::vector<double>& b0, std::vector<double>& x) {
+    std::vector<double> A = A0, sc(n, 1.0), r(n);
+    std::vector<int> piv(n);
+    for (int i = 0; i < n; ++i) {
+        double m = 0;
+        for (int j = 0; j < n; ++j) m = std::max(m, std::fabs(A[i + n * j]));
+        sc[i] = m > 0 ? 1.0 / m : 1.0;
+    }
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = -1;
+        for (int i = k; i < n; ++i) {
+            double v = std::fabs(A[i + n * k]) * sc[i];
+            if (v > best) { best = v; p = i; }
+        }
+        piv[k] = p;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[k + n * j], A[p + n * j]);
+            std::swap(sc[k], sc[p]);
+        }
+        if (A[k + n * k] == 0.0) A[k + n * k] = 1e-20;
+        for (int i = k + 1; i < n; ++i) {
+            double f = A[i + n * k] / A[k + n * k];
+            A[i + n * k] = f;
+            for (int j = k + 1; j < n; ++j) A[i + n * j] -= f * A[k + n * j];
+        }
+    }
+    x.assign(n, 0.0);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 0) r = b0;
+        else
+            for (int i = 0; i < n; ++i) {
+                double s = -b0[i];
+                for (int j = 0; j < n; ++j) s += A0[i + n * j] * x[j];
+                r[i] = s;
+            }
+        for (int k = 0; k < n; ++k) if (piv[k] != k) std::swap(r[k], r[piv[k]]);
+        for (int i = 0; i < n; ++i) for (int j = 0; j < i; ++j) r[i] -= A[i + n * j] * r[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) r[i] -= A[i + n * j] * r[j];
+            r[i] /= A[i + n * i];
+        }
+        if (pass == 0) x = r;
+        else for (int i = 0; i < n; ++i) x[i] -= r[i];
+    }
+}
+
+// FORM_C2S (src/pbasis.f90:31-130) for d shells.
+static int form_c2s(const RefBasis& rb, int spherical, std::vector<double>& c2s) {
+    int nc = rb.ncgs, np = rb.npgs;
+    c2s.assign((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; ++i) c2s[i + (size_t)nc * i] = 1.0;
+    if (!spherical) return 0;
+    std::vector<int> p2c(np, 0);
+    for (int p = 0; p < np; ++p) {
+        int k = 0;
+        for (int c = 0; c < nc; ++c)
+            if (rb.cc[c + (size_t)nc * p] != 0.0) { ++k; p2c[p] = c; }
+        if (k > 1) return fail(POLYERI_E_BASIS, "GENERAL CONTRACTION NOT YET IMPLEMENTED");
+    }
+    auto C = [&](int i, int j) -> double& { return c2s[i + (size_t)nc * j]; };
+    for (int p = 0; p < np; ++p) {
+        int x = rb.pax[p], y = rb.pay[p], z = rb.paz[p];
+        if (x + y + z == 3) return fail(POLYERI_E_BASIS, "spherical f functions not supported");
+        if (x == 2 && y == 0 && z == 0) C(p2c[p], p2c[p]) = 0.0;
+        else if (x == 0 && y == 2 && z == 0) {
+            C(p2c[p], p2c[p]) = -std::sqrt(3.0) / 2.0;
+            C(p2c[p], p2c[p - 1]) = std::sqrt(3.0) / 2.0;
+        } else if (x == 0 && y == 0 && z == 2) {
+            C(p2c[p], p2c[p - 2]) = -0.5;
+            C(p2c[p], p2c[p - 1]) = -0.5;
+        }
+    }
+    return 0;
+}
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    ~DevBuf() { for (void* p : ptrs) hipFree(p); }
+    template <typename T>
+    int get(T** p, size_t n) {
+        int rc = dalloc(p, n);
+        if (!rc) ptrs.push_back(*p);
+        return rc;
+    }
+};
+
+static int ensure_blas(polyeri_ctx* c) {
+    if (!c->blas) {
+        RBCHK(rocblas_create_handle(&c->blas));
+        RBCHK(rocblas_set_stream(c->blas, c->stream));
+    }
+    return 0;
+}
+
+int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double* eps_out, double* co_out,
+                double* trace, polyeri_scf_result* res) {
+    if (!ctx || !o || !p_c || !res) return fail(POLYERI_E_ARG, "polyeri_scf: null argument");
+    if (o->scfcycles < 1) return fail(POLYERI_E_ARG, "polyeri_scf: scfcycles < 1");
+    auto t_start = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_blas(ctx))) return rc;
+    rocblas_handle h = ctx->blas;
+    const RefBasis& rb = ctx->rb;
+    int nc = rb.ncgs;
+    size_t n2 = (size_t)nc * nc;
+    hipStream_t s = ctx->stream;
+    DevBuf buf;
+    double *dT, *dV, *dS, *dC2S, *dF, *dW1, *dW2, *dX, *dE, *dWk, *dCO, *dP, *dPout, *dOcc, *dFp;
+    int* dInfo;
+    if ((rc = buf.get(&dT, n2)) || (rc = buf.get(&dV, n2)) || (rc = buf.get(&dS, n2)) || (rc = buf.get(&dC2S, n2)) ||
+        (rc = buf.get(&dF, n2)) || (rc = buf.get(&dW1, n2)) || (rc = buf.get(&dW2, n2)) || (rc = buf.get(&dX, n2)) ||
+        (rc = buf.get(&dE, nc)) || (rc = buf.get(&dWk, nc)) || (rc = buf.get(&dCO, n2)) || (rc = buf.get(&dP, n2)) ||
+        (rc = buf.get(&dPout, n2)) || (rc = buf.get(&dOcc, nc)) || (rc = buf.get(&dFp, n2)) || (rc = buf.get(&dInfo, 1)))
+        return rc;
+    const int HMAX = 9;   // DIIS uses at most 8 previous vectors
+    double *dHP, *dHD;
+    if ((rc = buf.get(&dHP, n2 * HMAX)) || (rc = buf.get(&dHD, n2 * HMAX))) return rc;
+    std::vector<double> c2s;
+    if ((rc = form_c2s(rb, o->spherical, c2s))) return rc;
+    HIPCHK(hipMemcpy(dT, ctx->T_C.data(), n2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dV, ctx->N_C.data(), n2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dS, ctx->S_C.data(), n2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dC2S, c2s.data(), n2 * 8, hipMemcpyHostToDevice));
+    const double one = 1.0, zero = 0.0;
+    // canonical orthogonaliser of C2S S C2S^T (src/pfock.f90:270-281)
+    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dS, nc, &zero, dW1, nc));
+    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dW2, nc));
+    RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nc, dW2, nc, dE, dWk, dInfo));
+    std::vector<double> e(nc);
+    HIPCHK(hipMemcpyAsync(e.data(), dE, nc * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> keep;
+    for (int j = 0; j < nc; ++j) if (e[j] > o->lindep) keep.push_back(j);
+    int nind = (int)keep.size();
+    if (nind == 0) return fail(POLYERI_E_ARG, "no linearly independent functions");
+    int* dKeep;
+    if ((rc = buf.get(&dKeep, nind))) return rc;
+    HIPCHK(hipMemcpy(dKeep, keep.data(), nind * sizeof(int), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_orth, dim3(nblk((size_t)nc * nind)), dim3(256), 0, s, dW2, dE, dKeep, nc, nind, dX);
+    // electrons (src/porbitals.f90:15-19)
+    int nel = 0;
+    for (int z : rb.iatom) nel += z;
+    nel -= o->charge;
+    if (nel % 2) return fail(POLYERI_E_ARG, "ODD NUMBER OF ELECTRONS");
+    int iocc = nel / 2;
+    // guess
+    std::vector<double> P(n2, 0.0);
+    if (o->guess == 1) {
+        std::copy(p_c, p_c + n2, P.begin());
+    } else {
+        // GUESS_DENSITYMATRIX (src/pdensitymatrix.f90:83-127): Z / (number of
+        // non-ghost functions on the atom) on the diagonal
+        std::vector<int> fatom(nc, -1);
+        for (int c = 0; c < nc; ++c) {
+            int mu = ctx->ib.int_of_ref[c];
+            if (mu < 0) continue;
+            for (const Shell& sh : ctx->ib.shells)
+                if (mu >= sh.f0 && mu < sh.f0 + ncart(sh.L))
+                    for (int a = 0; a < rb.natom; ++a)
+                        if (sh.x == rb.ax[a] && sh.y == rb.ay[a] && sh.z == rb.az[a]) { fatom[c] = a; break; }
+        }
+        for (int a = 0; a < rb.natom; ++a) {
+            int L = 0;
+            for (int c = 0; c < nc; ++c) L += fatom[c] == a;
+            if (!L) return fail(POLYERI_E_ARG, "atom without basis functions");
+            for (int c = 0; c < nc; ++c) if (fatom[c] == a) P[c + (size_t)nc * c] = (double)rb.iatom[a] / L;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dP, P.data(), n2 * 8, hipMemcpyHostToDevice, s));
+    double temphf = o->temphf > 0 ? o->temphf : 1.0e-4;
+    std::vector<double> pulay((size_t)(o->scfcycles + 2) * (o->scfcycles + 2), 0.0);
+    int ld = o->scfcycles + 2;
+    std::vector<double> eps(nc, 1e99), occ(nc, 0.0);
+    double etot = 0, ek = 0, ev = 0, ej = 0, ex = 0, fermi = 0;
+    int converged = 0, iscf;
+    double jk_total = 0, fock_total = 0;
+    hipEvent_t ef0, ef1;
+    HIPCHK(hipEventCreate(&ef0));
+    HIPCHK(hipEventCreate(&ef1));
+    for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
+        HIPCHK(hipEventRecord(ef0, s));
+        if ((rc = jk_device(ctx, dP, ctx->d_jc, ctx->d_kc))) break;
+        hipLaunchKernelGGL(k_fock, dim3(nblk(n2)), dim3(256), 0, s, dT, dV, ctx->d_jc, ctx->d_kc, o->hfexchange, nc, dF);
+        HIPCHK(hipEventRecord(ef1, s));
+        // C2S F C2S^T, X^T (.) X
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dF, nc, &zero, dW1, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dW2, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nc, &one, dW2, nc, dX, nc, &zero, dW1, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nind, nind, nc, &one, dX, nc, dW1, nc, &zero, dFp, nind));
+        RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nind, dFp, nind, dE, dWk, dInfo));
+        // C = X C', CO = C2S^T C
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &one, dX, nc, dFp, nind, &zero, dW1, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, nind, nc, &one, dC2S, nc, dW1, nc, &zero, dCO, nc));
+        HIPCHK(hipMemcpyAsync(eps.data(), dE, nind * 8, hipMemcpyDeviceToHost, s));
+        int info = 0;
+        HIPCHK(hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (info != 0) { rc = fail(POLYERI_E_LIB, "EIGENVECTORS NOT CONVERGED"); break; }
+        {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, ef0, ef1));
+            HIPCHK(hipEventElapsedTime(&b, ctx->ev0, ctx->ev1));
+            fock_total += a * 1e-3;
+            jk_total += b * 1e-3;
+            ctx->last_jk_ms = b;
+        }
+        for (int j = nind; j < nc; ++j) eps[j] = 1.0e99;
+        // Fermi level by bisection (src/pfock.f90:339-423)
+        double mine = 1e99, maxe = -1e99;
+        for (int j = 0; j < nc; ++j) { maxe = std::max(maxe, eps[j]); mine = std::min(mine, eps[j]); }
+        double f1 = mine, f2 = maxe, f12 = 0.5 * (mine + maxe);
+        for (int it = 0; it < 100000; ++it) {
+            double n1 = 0, nn2 = 0, n12 = 0;
+            for (int j = 0; j < nc; ++j) {
+                n1 += 2.0 / (std::exp((eps[j] - f1) / kBoltzmann / temphf) + 1.0);
+                nn2 += 2.0 / (std::exp((eps[j] - f2) / kBoltzmann / temphf) + 1.0);
+                n12 += 2.0 / (std::exp((eps[j] - f12) / kBoltzmann / temphf) + 1.0);
+            }
+            n1 -= nel; nn2 -= nel; n12 -= nel;
+            if (std::fabs(n12) < 1e-10) { fermi = f12; break; }
+            else if (n1 > 0 && nn2 > 0) { f2 = f1; f1 -= 0.1; }
+            else if (n1 < 0 && nn2 < 0) { f1 = f2; f2 += 0.1; }
+            else if (n1 < 0 && n12 >= 0) f2 = f12;
+            else if (n12 <= 0 && nn2 > 0) f1 = f12;
+            else break;
+            f12 = 0.5 * (f1 + f2);
+        }
+        for (int j = 0; j < nc; ++j) occ[j] = j < nind ? 2.0 / (std::exp((eps[j] - fermi) / kBoltzmann / temphf) + 1.0) : 0.0;
+        HIPCHK(hipMemcpyAsync(dOcc, occ.data(), nind * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_scale_cols, dim3(nblk((size_t)nc * nind)), dim3(256), 0, s, dCO, dOcc, nc, nind, dW1);
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nind, &one, dCO, nc, dW1, nc, &zero, dPout, nc));
+        // STORE_DENSITYMATRIX: Delta P, RMS change, Pulay row (src/pdensitymatrix.f90:131-208)
+        int slot = (iscf - 1) % HMAX;
+        double* hp = dHP + n2 * slot;
+        double* hd = dHD + n2 * slot;
+        HIPCHK(hipMemcpyAsync(hp, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(hd, dPout, n2 * 8, hipMemcpyDeviceToDevice, s));
+        const double mone = -1.0;
+        RBCHK(rocblas_daxpy(h, (int)n2, &mone, dP, 1, hd, 1));
+        RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+        double dchg = 0.0;
+        RBCHK(rocblas_ddot(h, (int)n2, hd, 1, hd, 1, &dchg));
+        pulay[iscf + (size_t)ld * iscf] = dchg;
+        for (int i = std::max(1, iscf - HMAX + 1); i < iscf; ++i) {
+            double v = 0.0;
+            RBCHK(rocblas_ddot(h, (int)n2, dHD + n2 * ((i - 1) % HMAX), 1, hd, 1, &v));
+            pulay[i + (size_t)ld * iscf] = v;
+            pulay[iscf + (size_t)ld * i] = v;
+        }
+        dchg = std::sqrt(dchg / (double)n2);
+        // DIIS every other iteration from IOPTN(16) on, else relaxation
+        if (o->diis != 0 && iscf >= o->diis && ((iscf - o->diis) % 2) == 0) {
+            int nt = std::min(8, iscf), n = nt + 1;
+            std::vector<double> B((size_t)n * n, 0.0), rhs(n, 0.0), cv;
+            for (int i = 0; i < nt; ++i) {
+                for (int j = 0; j < nt; ++j) B[j + (size_t)n * i] = pulay[(iscf - j) + (size_t)ld * (iscf - i)];
+                B[i + (size_t)n * nt] = -1.0;
+                B[nt + (size_t)n * i] = -1.0;
+            }
+            rhs[nt] = -1.0;
+            lu_solve(n, B, rhs, cv);
+            hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, s, dP, n2, 0.0);
+            for (int i = iscf - nt + 1; i <= iscf; ++i) {
+                double w = cv[iscf - i];
+                RBCHK(rocblas_daxpy(h, (int)n2, &w, dHP + n2 * ((i - 1) % HMAX), 1, dP, 1));
+            }
+        } else {
+            double a = o->scfrelax, b = 1.0 - o->scfrelax;
+            RBCHK(rocblas_dscal(h, (int)n2, &b, dP, 1));
+            RBCHK(rocblas_daxpy(h, (int)n2, &a, dPout, 1, dP, 1));
+        }
+        // ENERGY_CALC with the updated density (src/pselfconsistentfield.f90:487-574)
+        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, &ek));
+        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, &ev));
+        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_jc, 1, &ej));
+        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_kc, 1, &ex));
+        ej *= 0.5;
+        ex *= -0.25 * o->hfexchange;
+        etot = ctx->e_nuc + ek + ev + ej + ex;
+        if (!std::isfinite(etot)) { rc = fail(POLYERI_E_NONFINITE, "non-finite SCF energy"); break; }
+        if (trace) { trace[2 * (iscf - 1)] = dchg; trace[2 * (iscf - 1) + 1] = etot; }
+        if (dchg < o->scfconv) { converged = 1; break; }
+    }
+    hipEventDestroy(ef0);
+    hipEventDestroy(ef1);
+    if (rc) return rc;
+    if (iscf > o->scfcycles) iscf = o->scfcycles;
+    HIPCHK(hipMemcpyAsync(p_c, dP, n2 * 8, hipMemcpyDeviceToHost, s));
+    if (co_out) HIPCHK(hipMemcpyAsync(co_out, dCO, (size_t)nc * nind * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (co_out) std::fill(co_out + (size_t)nc * nind, co_out + n2, 0.0);
+    if (eps_out) std::copy(eps.begin(), eps.end(), eps_out);
+    res->e_total = etot; res->e_nuc = ctx->e_nuc; res->e_kin = ek; res->e_nuc_att = ev;
+    res->e_coul = ej; res->e_hfx = ex; res->fermi = fermi;
+    res->iterations = iscf; res->converged = converged; res->n_indep = nind; res->iocc = iocc;
+    res->jk_seconds = jk_total;
+    res->fock_seconds = fock_total;
+    res->scf_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return converged ? 0 : fail(POLYERI_E_NOTCONV, "SCF CONVERGENCE NOT MET");
+}
+
+// ---------------------------------------------------------------------------
+// MP2 (one k-point): AO integrals on the device, four quarter transforms
+// ---------------------------------------------------------------------------
+int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
+                       int ivirtcore, double* e2s, double* e2t) {
+    if (!ctx || !co || !eps || !e2s || !e2t) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: null argument");
+    int no = iocc - icore, nv = nall - ivirtcore - iocc;
+    if (no <= 0 || nv <= 0 || icore < 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: empty orbital space");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_blas(ctx))) return rc;
+    rocblas_handle h = ctx->blas;
+    hipStream_t s = ctx->stream;
+    int n = ctx->ib.nbf, nc = ctx->rb.ncgs;
+    size_t n4 = (size_t)n * n * n * n;
+    DevBuf buf;
+    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc;
+    size_t x1 = (size_t)nv * n * n * n, x2 = (size_t)nv * no * n * n, x3 = (size_t)nv * no * nv * n;
+    size_t x4 = (size_t)nv * no * nv * no;
+    if ((rc = buf.get(&dTen, n4)) || (rc = buf.get(&dCO, (size_t)nc * nc)) || (rc = buf.get(&dC, (size_t)n * nc)) ||
+        (rc = buf.get(&dX1, x1)) || (rc = buf.get(&dX2, x2)) || (rc = buf.get(&dX3, x3)) || (rc = buf.get(&dT4, x4)) ||
+        (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)))
+        return rc;
+    if ((rc = tensor_device(ctx, dTen))) return rc;
+    HIPCHK(hipMemcpyAsync(dCO, co, (size_t)nc * nc * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dEps, eps, (size_t)nc * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mo_int, dim3(nblk((size_t)n * nc)), dim3(256), 0, s, dCO, nc, ctx->d_ref, ctx->d_scale, n, nc, dC);
+    const double one = 1.0, zero = 0.0;
+    const double* Cv = dC + (size_t)n * iocc;    // virtual columns
+    const double* Co = dC + (size_t)n * icore;   // active occupied columns
+    // tensor A(sigma, lambda, nu, mu) column-major; X1(b, lam nu mu) = Cv^T A
+    RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, n * n * n, n, &one, Cv, n, dTen, n,
+                        &zero, dX1, nv));
+    // X2(b, j, nu mu) = X1(b, lam, nu mu) Co(lam, j)  [batched over nu mu]
+    RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, no, n, &one, dX1, nv,
+                                        (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv, (rocblas_stride)nv * no, n * n));
+    // X3(bj, a, mu) = X2(bj, nu, mu) Cv(nu, a)  [batched over mu]
+    RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * no, nv, n, &one, dX2,
+                                        nv * no, (rocblas_stride)nv * no * n, Cv, n, 0, &zero, dX3, nv * no,
+                                        (rocblas_stride)nv * no * nv, n));
+    // T(bja, i) = X3(bja, mu) Co(mu, i)
+    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * no * nv, no, n, &one, dX3, nv * no * nv,
+                        Co, n, &zero, dT4, nv * no * nv));
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
+    hipLaunchKernelGGL(k_mp2_energy, dim3(nblk(x4)), dim3(256), 0, s, dT4, dEps, no, nv, icore, iocc, dAcc);
+    HIPCHK(hipGetLastError());
+    double acc[2];
+    HIPCHK(hipMemcpyAsync(acc, dAcc, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *e2s = acc[0];
+    *e2t = acc[1];
+    if (!std::isfinite(acc[0]) || !std::isfinite(acc[1])) return fail(POLYERI_E_NONFINITE, "non-finite E(2)");
+    return 0;
+}

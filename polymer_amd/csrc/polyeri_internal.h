@@ -1,0 +1,74 @@
+// polyeri_internal.h -- host-side data structures of libpolyeri.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "eri_engine.h"
+
+namespace polyeri {
+
+// The reference's basis arrays (module BASISSET, src/pmodules.f90:64-81),
+// 0-based, CC column-major ncgs x npgs.
+struct RefBasis {
+    int natom = 0;
+    std::vector<int> iatom;
+    std::vector<double> ax, ay, az;
+    int npgs = 0, ncgs = 0, npshell = 0;
+    std::vector<int> pax, pay, paz, pgs_atom;
+    std::vector<double> pgsx, pgsy, pgsz, zt, cc;
+    std::vector<double> cgsx, cgsy, cgsz;
+    std::vector<int> p_shell;       // npshell x 125, -1 = empty
+    std::vector<int> p_shell_ang;
+};
+
+// A contracted Cartesian shell of this library.  Component mu of the shell
+// is chi = sum_p coef[p] (x-A)^lx (y-A)^ly (z-A)^lz exp(-alpha[p] r^2) and
+// the reference's contracted function phi_c = scale * chi.
+struct Shell {
+    int L = 0;
+    int atom = 0;
+    double x = 0, y = 0, z = 0;
+    std::vector<double> alpha, coef;
+    int f0 = 0;                     // first internal basis-function index
+};
+
+struct PairInfo {
+    int a, b;                       // shells, L_a >= L_b
+    int cls;                        // pair class (La,Lb) index
+    int prim_off, nprim;            // into the PrimPair array
+    double ABx, ABy, ABz;
+};
+
+// Basis as this library lays it out: shells sorted by L (d, p, s), every
+// contracted function mapped to the reference's index and scale.
+struct IntBasis {
+    std::vector<Shell> shells;
+    int nbf = 0;
+    std::vector<int> ref_of_int;    // nbf -> ncgs index
+    std::vector<double> scale;      // nbf: phi_ref = scale * chi
+    std::vector<int> int_of_ref;    // ncgs -> nbf or -1 (ghost)
+    std::vector<int> natom_z;       // atomic numbers
+    std::vector<double> atom_xyz;   // 3 per atom
+};
+
+inline int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+constexpr int NPAIRCLASS = 6;       // (ss) (ps) (pp) (ds) (dp) (dd)
+constexpr int MAXL = 2;
+
+// Host functions (host.cpp).
+int read_basis(const std::string& path, int natom, const int* iatom, const double* xyz,
+               RefBasis& out, std::string& err);
+int extract_shells(const RefBasis& rb, IntBasis& ib, std::string& err);
+void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pairs,
+                 std::vector<PrimPair>& prims);
+void boys_table(std::vector<double>& tab);       // BOYS_NT x BOYS_NM
+// S, T, V in the internal ordering (nbf x nbf, row-major, symmetric).
+void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vector<double>& S,
+                  std::vector<double>& T, std::vector<double>& V);
+double nuclear_repulsion(const IntBasis& ib);
+// Cartesian exponents of component k of a shell of angular momentum l.
+void cart_comp(int l, int k, int& lx, int& ly, int& lz);
+
+}  // namespace polyeri

@@ -1,0 +1,155 @@
+"""GPU parity of libpolyeri against the oracle and the reference's goldens.
+
+Bars: contracted ERIs within 1e-12 absolute; J/K within 1e-10; SCF and MP2
+energies within 1e-8 Eh of the shipped outputs and of the oracle, with the
+same SCF iteration count.  At the full (H2O)16 size: properties that do not
+depend on size (symmetry, linearity, positivity, screened vs unscreened)."""
+import numpy as np
+import pytest
+
+import oracle
+import polymer_amd as pa
+from polymer_amd import geometry as G
+from tests import molecules as M
+from tests.conftest import basis_path
+
+pytestmark = pytest.mark.gpu
+
+ERI_TOL = 1e-12
+EH = 1e-8
+
+
+def _ctx(basis, atoms, **kw):
+    return pa.Context(pa.read_basis(basis, atoms), **kw)
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    b = pa.read_basis("cc-pVDZ", M.H2O_GOLDEN)
+    return b, pa.Context(b, schwarz_tol=0.0), oracle.Basis(basis_path("cc-pVDZ"), M.H2O_GOLDEN)
+
+
+def test_eri_tensor_h2o_every_class(h2o):
+    b, ctx, ob = h2o
+    print("case: ERI tensor H2O/cc-pVDZ", flush=True)
+    g = ctx.eri_tensor()
+    ref = ob.eri_tensor()
+    assert np.max(np.abs(g - ref)) < ERI_TOL
+
+
+def test_eri_tensor_631gss():
+    atoms = G.alkane(2, seed=3)
+    print("case: ERI tensor C2H6/6-31G**", flush=True)
+    ctx = _ctx("6-31Gss", atoms, schwarz_tol=0.0)
+    ref = oracle.Basis(basis_path("6-31Gss"), atoms).eri_tensor()
+    assert np.max(np.abs(ctx.eri_tensor() - ref)) < ERI_TOL
+
+
+def test_one_electron_h2o(h2o):
+    b, ctx, ob = h2o
+    s, t, v, enuc = ctx.one_electron()
+    _, rs, rt, rv = ob.one_electron()
+    assert np.max(np.abs(s - rs)) < 1e-13
+    assert np.max(np.abs(t - rt)) < 1e-12
+    assert np.max(np.abs(v - rv)) < 1e-12
+    assert abs(enuc - ob.nuclear_repulsion()) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_jk_random_density(h2o, seed):
+    b, ctx, ob = h2o
+    print(f"case: J/K H2O random density seed {seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal((b.ncgs, b.ncgs))
+    p = 0.05 * (p + p.T)
+    j, k = ctx.fourindex_eri(p)
+    jr, kr, _ = ob.fourindex_eri(p, 0.0)
+    assert np.max(np.abs(j - jr)) < 1e-10
+    assert np.max(np.abs(k - kr)) < 1e-10
+    ghosts = [c for c in range(b.ncgs) if not np.any(b.cc[c])]
+    assert np.all(j[ghosts] == 0) and np.all(k[:, ghosts] == 0)
+
+
+def test_jk_water_dimer_unscreened():
+    atoms = G.water_cluster(2, 11)
+    print("case: J/K (H2O)2/cc-pVDZ", flush=True)
+    b = pa.read_basis("cc-pVDZ", atoms)
+    ctx = pa.Context(b, schwarz_tol=0.0)
+    rng = np.random.default_rng(4)
+    p = rng.standard_normal((b.ncgs, b.ncgs))
+    p = 0.02 * (p + p.T)
+    j, k = ctx.fourindex_eri(p)
+    jr, kr, _ = oracle.Basis(basis_path("cc-pVDZ"), atoms).fourindex_eri(p, 0.0)
+    assert np.max(np.abs(j - jr)) < 1e-10 and np.max(np.abs(k - kr)) < 1e-10
+
+
+def test_scf_h2o_spherical_golden(golden):
+    g = golden["H2O_ccpVDZ_spherical"]
+    print("case: SCF H2O/cc-pVDZ spherical golden", flush=True)
+    ctx = _ctx("cc-pVDZ", M.H2O_GOLDEN, schwarz_tol=0.0)
+    r = ctx.self_consistent_field(scfconv=1e-11, lindep=1e-12, diis=2, spherical=True)
+    for k in ["e_nuc", "e_kin", "e_nuc_att", "e_coul", "e_hfx", "e_total"]:
+        assert abs(r[k] - g[k]) < EH, k
+    assert r["iterations"] == g["iterations"] == 30
+    tr = np.array(g["trace"])
+    assert np.max(np.abs(r["trace"][:, 0] - tr[:, 2])) < 1e-11
+    e2s, e2t = ctx.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"])
+    assert abs(e2s + e2t - g["e2_cc"]) < EH
+
+
+def test_scf_mp2_bh_frozen_core(golden):
+    g = golden["BH_STO3G"]
+    ctx = _ctx("STO-3G", M.BH_GOLDEN, schwarz_tol=0.0)
+    r = ctx.self_consistent_field(scfconv=1e-11)
+    assert abs(r["e_total"] - g["e_total"]) < EH and r["iterations"] == g["iterations"]
+    e2s, e2t = ctx.mp2_energy(r["co"], r["eps"], 1, r["iocc"], r["n_indep"])
+    assert abs(e2s + e2t - g["e2_alg1"]) < EH
+
+
+def test_scf_mp2_fh(golden):
+    g = golden["FH_STO3G_zeroT"]
+    ctx = _ctx("STO-3G", M.FH_GOLDEN, schwarz_tol=0.0)
+    r = ctx.self_consistent_field(scfconv=1e-12)
+    assert abs(r["e_total"] - g["e_total"]) < EH and r["iterations"] == g["iterations"]
+    e2s, e2t = ctx.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"])
+    assert abs(e2s + e2t - g["e2_zero_t"]) < EH
+    gt = golden["FH_STO3G_T1e5"]
+    r = ctx.self_consistent_field(scfconv=1e-12, temphf=1e5)
+    assert abs(r["e_total"] - gt["e_total"]) < EH and r["iterations"] == gt["iterations"]
+
+
+def test_scf_config1_water_vs_oracle():
+    """Config 1: H2O cc-pVDZ Cartesian, INTTOL 1e-12, SCFCONV 1e-10."""
+    atoms = G.water()
+    print("case: SCF config-1 water vs oracle direct path", flush=True)
+    r = _ctx("cc-pVDZ", atoms, schwarz_tol=1e-12).self_consistent_field(scfconv=1e-10)
+    ro = oracle.Basis(basis_path("cc-pVDZ"), atoms).scf(inttol=1e-12, scfconv=1e-10, direct=True)
+    assert abs(r["e_total"] - ro["e_total"]) < EH
+    assert r["iterations"] == ro["iterations"]
+
+
+def test_water16_size_independent_properties():
+    atoms = G.water_cluster(16, 0)
+    print("case: (H2O)16/cc-pVDZ J/K properties", flush=True)
+    b = pa.read_basis("cc-pVDZ", atoms)
+    ctx = pa.Context(b, schwarz_tol=1e-12)
+    full = pa.Context(b, schwarz_tol=0.0)
+    rng = np.random.default_rng(9)
+    real = b.real_functions
+    x = rng.standard_normal((b.ncgs, 40))
+    x[[i for i in range(b.ncgs) if i not in set(real)]] = 0.0
+    p1 = 0.1 * x @ x.T                       # positive semidefinite
+    y = rng.standard_normal((b.ncgs, b.ncgs))
+    p2 = 0.01 * (y + y.T)
+    j1, k1 = ctx.fourindex_eri(p1)
+    j2, k2 = ctx.fourindex_eri(p2)
+    j12, k12 = ctx.fourindex_eri(p1 + p2)
+    scale = np.max(np.abs(j1))
+    assert np.max(np.abs(j1 - j1.T)) < 1e-12 * scale and np.max(np.abs(k1 - k1.T)) < 1e-12 * scale
+    assert np.max(np.abs(j12 - j1 - j2)) < 1e-11 * scale
+    assert np.max(np.abs(k12 - k1 - k2)) < 1e-11 * scale
+    assert np.sum(p1 * j1) > 0 and np.sum(p1 * k1) > 0   # Coulomb/exchange positivity
+    jf, kf = full.fourindex_eri(p1)
+    assert np.max(np.abs(jf - j1)) < 1e-8 and np.max(np.abs(kf - k1)) < 1e-8
+    st = ctx.stats()
+    assert st["nbf"] == 400 and st["quartets"] > 0
