@@ -153,3 +153,19 @@ def test_water16_size_independent_properties():
     assert np.max(np.abs(jf - j1)) < 1e-8 and np.max(np.abs(kf - k1)) < 1e-8
     st = ctx.stats()
     assert st["nbf"] == 400 and st["quartets"] > 0
+
+
+def test_fortran_driver_golden(golden):
+    """The Fortran driver (include/polyeri_mod.f90 ISO_C_BINDING module,
+    amdflang) reproduces the H2O/cc-pVDZ spherical golden on the GPU."""
+    import os
+    import re
+    import subprocess
+    from tests.conftest import ROOT
+    exe = os.path.join(ROOT, "polymer_amd", "fortran", "pscf_driver")
+    out = subprocess.run([exe, basis_path("cc-pVDZ")], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-800:], out.stderr[-400:], flush=True)
+    assert out.returncode == 0
+    e = float(re.search(r"TOTAL SCF ENERGY\s+=\s+(\S+)", out.stdout).group(1))
+    assert abs(e - golden["H2O_ccpVDZ_spherical"]["e_total"]) < EH
+    assert re.search(r"ITERATIONS =\s+30", out.stdout)

@@ -56,3 +56,18 @@ def test_water_cluster_constraints():
 def test_missing_element_raises():
     with pytest.raises(pa.PolyeriError):
         pa.read_basis("cc-pVDZ", [(17, 0.0, 0.0, 0.0)])
+
+
+def test_fortran_interface_module_compiles(tmp_path):
+    """include/polyeri_mod.f90 (the ISO_C_BINDING module a maintainer adds to
+    the reference) compiles with amdflang and links against libpolyeri."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not installed")
+    mod = os.path.join(ROOT, "include", "polyeri_mod.f90")
+    drv = os.path.join(ROOT, "polymer_amd", "fortran", "pscf_driver.f90")
+    subprocess.check_call([fc, "-O0", "-J", str(tmp_path), "-c", mod, "-o", str(tmp_path / "m.o")])
+    subprocess.check_call([fc, "-O0", "-I", str(tmp_path), "-J", str(tmp_path), drv, str(tmp_path / "m.o"),
+                           "-L" + os.path.join(ROOT, "polymer_amd"), "-lpolyeri", "-o", str(tmp_path / "d")])
