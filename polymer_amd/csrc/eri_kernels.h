@@ -25,6 +25,9 @@ struct EriArgs {
     const long long* row_off;  // nrows + 1
     int nrows;
     long long nwork;
+    const int* rb_off;         // nrows x (nbucket+1): ket-bucket offsets inside each row
+    const int* bucket_start;   // nbucket: first ket of each bucket in ket_ids
+    int nbucket;
     const double* boys;
     const double* D;
     double* J;
@@ -142,6 +145,24 @@ struct TensorCb {
     }
 };
 
+// Work item t of a class pair -> (bra pair, ket pair).  Rows are bra pairs in
+// Schwarz order; inside a row the surviving kets are grouped in buckets of
+// equal primitive-pair count (each bucket a Schwarz prefix), so neighbouring
+// threads run primitive loops of the same length.
+__device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& bp, int& kp) {
+    int lo = 0, hi = g.nrows - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g.row_off[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    bp = g.bra_ids[lo];
+    const int local = (int)(t - g.row_off[lo]);
+    const int* rb = g.rb_off + (size_t)lo * (g.nbucket + 1);
+    int b = 0;
+    while (b + 1 < g.nbucket && rb[b + 1] <= local) ++b;
+    kp = g.ket_ids[g.bucket_start[b] + (local - rb[b])];
+}
+
 template <int LA, int LB, int LC, int LD, int MODE>
 __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,13 +171,7 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
     if (MODE == MODE_SCHWARZ) {
         bp = kp = g.bra_ids[t];
     } else {
-        int lo = 0, hi = g.nrows - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (g.row_off[mid] <= t) lo = mid; else hi = mid - 1;
-        }
-        bp = g.bra_ids[lo];
-        kp = g.ket_ids[t - g.row_off[lo]];
+        decode_work(g, t, bp, kp);
     }
     const PairMeta mb = g.meta[bp], mk = g.meta[kp];
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -175,18 +190,31 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
         quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
     } else {
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
-        if constexpr (NA * NB * NC * ND > 108) {
-            StreamDigest cb{g.D, g.J, g.K, n, fa, fb, fc, fd, s};
-            quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
-        } else {
-            RegDigest<NA, NB, NC, ND> cb;
-            cb.load(g.D, n, fa, fb, fc, fd);
-            quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
-            cb.flush(g.J, g.K, n, fa, fb, fc, fd, s);
-        }
+        // light classes only (heavier ones run in kcoop.hip): the (ab|cd)
+        // block in registers, then the six J/K blocks
+        double v[NA * NB * NC * ND];
+        StoreCb<NB, NC, ND> st{v};
+        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
+        RegDigest<NA, NB, NC, ND> cb;
+        cb.load(g.D, n, fa, fb, fc, fd);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) cb(a, b, c, d, v[((a * NB + b) * NC + c) * ND + d]);
+        cb.flush(g.J, g.K, n, fa, fb, fc, fd, s);
     }
 }
 
 typedef void (*EriKernel)(EriArgs);
+// Cooperative (one wavefront per quartet, LDS-resident recursion) kernels.
+int polyeri_coop_nslot(int la, int lb, int lc, int ld);
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,
+                        hipStream_t stream);
+// Classes with more than this many integrals per quartet run cooperatively.
+#define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).
 EriKernel polyeri_class_kernel(int la, int lb, int lc, int ld, int mode);

@@ -24,6 +24,9 @@ import os
 import sys
 
 LMAX = 2
+# classes with more integrals per quartet than this run in the cooperative
+# LDS kernel (POLYERI_COOP_MIN_INTEGRALS in eri_kernels.h)
+COOP_MIN_INTEGRALS = 108
 
 
 def comps(l):
@@ -244,11 +247,15 @@ def write_kernel_units(out_dir):
         path = os.path.join(out_dir, f"kgen_{tag}.hip")
         body = ["// Generated by gen_eri.py -- do not edit.",
                 '#include "../eri_kernels.h"', "",
-                f"EriKernel polyeri_kernel_{tag}(int mode) {{",
-                "    switch (mode) {"]
-        for m in range(3):
-            body.append(f"    case {m}: return k_eri<{la}, {lb}, {lc}, {ld}, {m}>;")
-        body += ["    }", "    return nullptr;", "}", ""]
+                f"EriKernel polyeri_kernel_{tag}(int mode) {{"]
+        if ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > COOP_MIN_INTEGRALS:
+            # runs only through the cooperative kernel (kcoop.hip)
+            body += ["    (void)mode;", "    return nullptr;", "}", ""]
+        else:
+            body.append("    switch (mode) {")
+            for m in range(3):
+                body.append(f"    case {m}: return k_eri<{la}, {lb}, {lc}, {ld}, {m}>;")
+            body += ["    }", "    return nullptr;", "}", ""]
         text = "\n".join(body)
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as f:
@@ -282,3 +289,201 @@ if __name__ == "__main__":
         os.replace(tmp, out)
     names = write_kernel_units(os.path.join(here, "kgen"))
     print(f"{len(cls)} classes -> {out}, {len(names)} kernel units")
+
+
+# ---------------------------------------------------------------------------
+# Level-scheduled programs for the cooperative (one wavefront per quartet)
+# kernel: node values live in LDS; every op of one level is independent.
+# ---------------------------------------------------------------------------
+def class_program(la, lb, lc, ld):
+    """Returns dict with op lists.  Slot 0 holds 0.0; slots 1..nacc the
+    contracted accumulators; the VRR region (F_0..F_L then VRR nodes) and,
+    after the primitive loop, the HRR region both start at nacc+1."""
+    L = la + lb + lc + ld
+    ec = [c for l in range(la, la + lb + 1) for c in comps(l)]
+    fc = [c for l in range(lc, lc + ld + 1) for c in comps(l)]
+    nacc = len(ec) * len(fc)
+    base = nacc + 1
+    slot, level, ops = {}, {}, []   # VRR
+    for m in range(L + 1):
+        slot[((0, 0, 0), (0, 0, 0), m)] = base + m
+        level[((0, 0, 0), (0, 0, 0), m)] = 0
+    nxt = [base + L + 1]
+
+    def node(e, f, m):
+        key = (e, f, m)
+        if key in slot:
+            return key
+        if sum(f) == 0:
+            i = pick_dir(e)
+            e1 = sub(e, i)
+            s1, s2 = node(e1, f, m), node(e1, f, m + 1)
+            s3 = s4 = None
+            n3 = 0
+            if e[i] >= 2:
+                s3, s4 = node(sub(e, i, 2), f, m), node(sub(e, i, 2), f, m + 1)
+                n3 = e[i] - 1
+            srcs, typ, n5, s5 = [s1, s2, s3, s4], 0, 0, None
+        else:
+            i = pick_dir(f)
+            f1 = sub(f, i)
+            s1, s2 = node(e, f1, m), node(e, f1, m + 1)
+            s3 = s4 = s5 = None
+            n3 = n5 = 0
+            if f[i] >= 2:
+                s3, s4 = node(e, sub(f, i, 2), m), node(e, sub(f, i, 2), m + 1)
+                n3 = f[i] - 1
+            if e[i] >= 1:
+                s5 = node(sub(e, i), f1, m + 1)
+                n5 = e[i]
+            srcs, typ = [s1, s2, s3, s4, s5], 1
+        lv = 1 + max(level[s] for s in srcs if s is not None)
+        slot[key] = nxt[0]
+        nxt[0] += 1
+        level[key] = lv
+        get = lambda k: slot[k] if k is not None else 0
+        ops.append((lv, slot[key], get(s1), get(s2), get(s3), get(s4), get(s5 if typ else None), typ, i, n3, n5))
+        return key
+
+    accmap = []
+    for ie, e in enumerate(ec):
+        for jf, f in enumerate(fc):
+            k = node(e, f, 0)
+            accmap.append((1 + ie * len(fc) + jf, slot[k]))
+    nvrr_slots = nxt[0]
+    # HRR
+    hslot, hlevel, hops = {}, {}, []
+    hn = [base]
+    fidx = {f: k for k, f in enumerate(fc)}
+    eidx = {e: k for k, e in enumerate(ec)}
+
+    def ket(e, c, d):
+        key = ("k", e, c, d)
+        if key in hslot:
+            return key
+        if sum(d) == 0:
+            hslot[key] = 1 + eidx[e] * len(fc) + fidx[c]
+            hlevel[key] = 0
+            return key
+        i = pick_dir(d)
+        a = ket(e, add(c, i), sub(d, i))
+        b = ket(e, c, sub(d, i))
+        hslot[key] = hn[0]; hn[0] += 1
+        hlevel[key] = 1 + max(hlevel[a], hlevel[b])
+        hops.append((hlevel[key], hslot[key], hslot[a], hslot[b], 1, i))
+        return key
+
+    def bra(a_, b_, c, d):
+        key = ("b", a_, b_, c, d)
+        if key in hslot:
+            return key
+        if sum(b_) == 0:
+            return ket(a_, c, d)
+        i = pick_dir(b_)
+        x = bra(add(a_, i), sub(b_, i), c, d)
+        y = bra(a_, sub(b_, i), c, d)
+        hslot[key] = hn[0]; hn[0] += 1
+        hlevel[key] = 1 + max(hlevel[x], hlevel[y])
+        hops.append((hlevel[key], hslot[key], hslot[x], hslot[y], 0, i))
+        return key
+
+    finals = []
+    for ia, a_ in enumerate(comps(la)):
+        for ib, b_ in enumerate(comps(lb)):
+            for ic, c in enumerate(comps(lc)):
+                for id_, d in enumerate(comps(ld)):
+                    k = bra(a_, b_, c, d)
+                    finals.append((hslot[k], ia, ib, ic, id_))
+    nslot = max(nvrr_slots, hn[0])
+    return dict(L=L, nacc=nacc, base=base, ops=sorted(ops), hops=sorted(hops), accmap=accmap,
+                finals=finals, nslot=nslot)
+
+
+def write_programs(path):
+    _, cls = classes()
+    vops, hops, aux, descs = [], [], [], []
+    for c in cls:
+        p = class_program(*c)
+        d = {}
+        # VRR ops grouped by level
+        lv_off = []
+        cur = None
+        for op in p["ops"]:
+            if op[0] != cur:
+                cur = op[0]
+                lv_off.append(len(vops))
+            _, dst, s1, s2, s3, s4, s5, typ, i, n3, n5 = op
+            vops.append((dst | (s1 << 16), s2 | (s3 << 16), s4 | (s5 << 16), typ | (i << 1) | (n3 << 3) | (n5 << 6)))
+        lv_off.append(len(vops))
+        h_off = []
+        cur = None
+        for op in p["hops"]:
+            if op[0] != cur:
+                cur = op[0]
+                h_off.append(len(hops))
+            _, dst, s1, s2, typ, i = op
+            hops.append((dst | (s1 << 16), s2, 0, typ | (i << 1)))
+        h_off.append(len(hops))
+        acc_off = len(aux)
+        for a, s in p["accmap"]:
+            aux.append(a | (s << 16))
+        fin_off = len(aux)
+        for s, a, b, cc, dd in p["finals"]:
+            aux.append(s | (a << 16) | (b << 20) | (cc << 24) | (dd << 28))
+        descs.append((c, lv_off, h_off, acc_off, len(p["accmap"]), fin_off, len(p["finals"]), p["nslot"], p["L"], p["nacc"], p["base"]))
+    maxlv = max(max(len(d[1]), len(d[2])) for d in descs)
+    out = ["// Generated by gen_eri.py -- do not edit.", "#pragma once", "#include <stdint.h>", "",
+           "// The tables have external linkage (no const at namespace scope): the HIP",
+           "// runtime resolves every registered device variable through the code",
+           "// object's global symbols.",
+           f"#define PROG_MAXLV {maxlv}",
+           "struct ProgDesc {",
+           "    int vrr_lv[PROG_MAXLV];   // level starts into g_prog_vrr, last = end",
+           "    int nvrr_lv;",
+           "    int hrr_lv[PROG_MAXLV];",
+           "    int nhrr_lv;",
+           "    int acc_off, nacc_map, fin_off, nfin, nslot, L, nacc, base;",
+           "};", ""]
+
+    def arr(name, rows):
+        s = [f"__device__ uint4 {name}[] = {{"]
+        for r in rows:
+            s.append("    {%du, %du, %du, %du}," % r)
+        s.append("};")
+        return s
+    out += arr("g_prog_vrr", vops) + [""] + arr("g_prog_hrr", hops) + [""]
+    out.append("__device__ uint32_t g_prog_aux[] = {")
+    for k in range(0, len(aux), 12):
+        out.append("    " + ", ".join(f"{x}u" for x in aux[k:k + 12]) + ",")
+    out.append("};")
+    out.append("")
+    out.append(f"__device__ ProgDesc g_prog_desc[{len(descs)}] = {{")
+    for (c, lv, hv, ao, na, fo, nf, ns, L, nacc, base) in descs:
+        lvs = ", ".join(str(x) for x in lv + [0] * (maxlv - len(lv)))
+        hvs = ", ".join(str(x) for x in hv + [0] * (maxlv - len(hv)))
+        out.append(f"    // ({c[0]}{c[1]}|{c[2]}{c[3]})")
+        out.append(f"    {{{{{lvs}}}, {len(lv) - 1}, {{{hvs}}}, {len(hv) - 1}, {ao}, {na}, {fo}, {nf}, {ns}, {L}, {nacc}, {base}}},")
+    out.append("};")
+    out.append("")
+    out.append("// LDS slots per class for the host (no device-symbol lookup needed)")
+    out.append(f"static const int g_prog_nslot[{len(descs)}] = {{" + ", ".join(str(d[7]) for d in descs) + "};")
+    out.append("")
+    out.append("// class index of (la lb|lc ld) in g_prog_desc")
+    out.append("static inline int prog_index(int la, int lb, int lc, int ld) {")
+    out.append("    switch (la * 1000 + lb * 100 + lc * 10 + ld) {")
+    for k, (c, *_r) in enumerate(descs):
+        out.append(f"    case {c[0] * 1000 + c[1] * 100 + c[2] * 10 + c[3]}: return {k};")
+    out.append("    }")
+    out.append("    return -1;")
+    out.append("}")
+    text = "\n".join(out) + "\n"
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return max(d[7] for d in descs)
+
+
+if __name__ == "__main__" and os.environ.get("GEN_ERI_PROGRAMS", "1") == "1":
+    here = os.path.dirname(os.path.abspath(__file__))
+    mx = write_programs(os.path.join(here, "eri_programs.h"))
+    print(f"programs -> eri_programs.h (max {mx} LDS slots)")

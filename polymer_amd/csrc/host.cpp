@@ -335,6 +335,73 @@ void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pair
         }
 }
 
+void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
+                      const std::vector<int> class_ids[], std::vector<int> ket_ids[],
+                      std::vector<int> bstart[], std::vector<WorkList>& out) {
+    out.clear();
+    std::vector<int> rank(pairs.size(), 0);
+    for (int X = 0; X < NPAIRCLASS; ++X)
+        for (size_t r = 0; r < class_ids[X].size(); ++r) rank[class_ids[X][r]] = (int)r;
+    for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+        auto& kl = ket_ids[Y];
+        kl = class_ids[Y];
+        std::stable_sort(kl.begin(), kl.end(), [&](int a, int b) {
+            if (pairs[a].nprim != pairs[b].nprim) return pairs[a].nprim > pairs[b].nprim;
+            return rank[a] < rank[b];
+        });
+        auto& bs = bstart[Y];
+        bs.clear();
+        for (size_t j = 0; j < kl.size(); ++j)
+            if (j == 0 || pairs[kl[j]].nprim != pairs[kl[j - 1]].nprim) bs.push_back((int)j);
+        bs.push_back((int)kl.size());
+    }
+    for (int X = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const auto& bx = class_ids[X];
+            const auto& ky = ket_ids[Y];
+            const auto& bs = bstart[Y];
+            const int nb = (int)bs.size() - 1;
+            WorkList w;
+            w.X = X; w.Y = Y;
+            w.nbucket = std::max(nb, 1);
+            w.nrows = (int)bx.size();
+            w.row.assign(bx.size() + 1, 0);
+            w.rb.assign(bx.size() * (size_t)(w.nbucket + 1), 0);
+            for (size_t i = 0; i < bx.size(); ++i) {
+                const double qi = Q[bx[i]];
+                const int ri = rank[bx[i]];
+                int* r = w.rb.data() + i * (size_t)(w.nbucket + 1);
+                long long tot = 0;
+                double kp = 0;
+                for (int b = 0; b < nb; ++b) {
+                    const int lo = bs[b], hi = bs[b + 1];
+                    int a0 = lo, a1 = hi;        // Schwarz prefix of the bucket
+                    while (a0 < a1) {
+                        const int mid = (a0 + a1) / 2;
+                        if (qi * Q[ky[mid]] >= tol) a0 = mid + 1; else a1 = mid;
+                    }
+                    int cnt = a0 - lo;
+                    if (X == Y) {                // rank_j <= rank_i
+                        int b0 = lo, b1 = hi;
+                        while (b0 < b1) {
+                            const int mid = (b0 + b1) / 2;
+                            if (rank[ky[mid]] <= ri) b0 = mid + 1; else b1 = mid;
+                        }
+                        cnt = std::min(cnt, b0 - lo);
+                    }
+                    r[b] = (int)tot;
+                    tot += cnt;
+                    if (cnt) kp += (double)cnt * pairs[ky[lo]].nprim;
+                }
+                for (int b = std::max(nb, 0); b <= w.nbucket; ++b) r[b] = (int)tot;
+                w.row[i + 1] = w.row[i] + tot;
+                w.prim_quartets += pairs[bx[i]].nprim * kp;
+            }
+            w.nwork = w.row[bx.size()];
+            out.push_back(std::move(w));
+        }
+}
+
 // F_m(T) on the grid by the convergent series
 // F_m(T) = e^-T sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)), all terms > 0.
 void boys_table(std::vector<double>& tab) {

@@ -1,0 +1,141 @@
+// kcoop.hip -- cooperative shell-quartet kernel for the d-heavy classes.
+//
+// One wavefront (64 lanes) evaluates one contracted shell quartet: the
+// class's VRR / HRR recursion is executed level by level from generated op
+// tables (eri_programs.h); every node value lives in LDS, so no per-lane
+// array has to fit in registers or spill to scratch.  Same integrals and
+// the same digestion as k_eri.
+#include "eri_kernels.h"
+#include "eri_programs.h"
+
+__device__ __forceinline__ double boys_taylor(int m, double T, const double* __restrict__ tab) {
+    if (T < BOYS_TMAX) {
+        const int ts = (int)(T * 20.0 + 0.5);
+        const double h = 0.05 * ts - T;
+        const double* gg = tab + ts * BOYS_NM + m;
+        double f = gg[6];
+        f = gg[5] + h * (1.0 / 6.0) * f;
+        f = gg[4] + h * (1.0 / 5.0) * f;
+        f = gg[3] + h * (1.0 / 4.0) * f;
+        f = gg[2] + h * (1.0 / 3.0) * f;
+        f = gg[1] + h * (1.0 / 2.0) * f;
+        return gg[0] + h * f;
+    }
+    const double it = 1.0 / T;
+    double f = 0.5 * sqrt(POLY_PI * it);
+    for (int k = 0; k < m; ++k) f *= (k + 0.5) * it;
+    return f;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd) {
+    extern __shared__ double sl[];
+    const ProgDesc& d = g_prog_desc[prog];
+    const int lane = threadIdx.x;
+    for (long long t = blockIdx.x; t < g.nwork; t += gridDim.x) {
+        int bp, kp;
+        if (MODE == MODE_SCHWARZ) {
+            bp = kp = g.bra_ids[t];
+        } else {
+            decode_work(g, t, bp, kp);
+        }
+        const PairMeta mb = g.meta[bp], mk = g.meta[kp];
+        for (int k = lane; k <= d.nacc; k += 64) sl[k] = 0.0;
+        for (int p = 0; p < mb.nprim; ++p) {
+            const PrimPair bq = g.prims[mb.prim_off + p];
+            for (int q = 0; q < mk.nprim; ++q) {
+                const PrimPair kq = g.prims[mk.prim_off + q];
+                const double ze = bq.zeta + kq.zeta, ize = 1.0 / ze;
+                const double rho = bq.zeta * kq.zeta * ize;
+                const double PQx = bq.Px - kq.Px, PQy = bq.Py - kq.Py, PQz = bq.Pz - kq.Pz;
+                const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+                const double pref = bq.K * kq.K * sqrt(ize);
+                const double roz = kq.zeta * ize, roe = bq.zeta * ize;
+                const double oo2z = 0.5 / bq.zeta, oo2e = 0.5 / kq.zeta, oo2ze = 0.5 * ize;
+                if (lane <= d.L) sl[d.base + lane] = pref * boys_taylor(lane, T, g.boys);
+                __syncthreads();
+                for (int lv = 0; lv < d.nvrr_lv; ++lv) {
+                    for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
+                        const uint4 op = g_prog_vrr[o];
+                        const int typ = op.w & 1, i = (op.w >> 1) & 3;
+                        const double n3 = (double)((op.w >> 3) & 7), n5 = (double)((op.w >> 6) & 7);
+                        const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
+                        const double c1 = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
+                                              : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
+                        const double c2 = typ ? roe * pq : -roz * pq;
+                        const double x3 = typ ? oo2e : oo2z, r3 = typ ? roe : roz;
+                        sl[op.x & 0xffff] = c1 * sl[op.x >> 16] + c2 * sl[op.y & 0xffff] +
+                                            n3 * x3 * (sl[op.y >> 16] - r3 * sl[op.z & 0xffff]) +
+                                            n5 * oo2ze * sl[op.z >> 16];
+                    }
+                    __syncthreads();
+                }
+                for (int k = lane; k < d.nacc_map; k += 64) {
+                    const uint32_t a = g_prog_aux[d.acc_off + k];
+                    sl[a & 0xffff] += sl[a >> 16];
+                }
+                __syncthreads();
+            }
+        }
+        const double ab[3] = {g.ab[4 * bp], g.ab[4 * bp + 1], g.ab[4 * bp + 2]};
+        const double cd[3] = {g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2]};
+        for (int lv = 0; lv < d.nhrr_lv; ++lv) {
+            for (int o = d.hrr_lv[lv] + lane; o < d.hrr_lv[lv + 1]; o += 64) {
+                const uint4 op = g_prog_hrr[o];
+                const int typ = op.w & 1, i = (op.w >> 1) & 3;
+                const double gv = typ ? (i == 0 ? cd[0] : (i == 1 ? cd[1] : cd[2]))
+                                      : (i == 0 ? ab[0] : (i == 1 ? ab[1] : ab[2]));
+                sl[op.x & 0xffff] = sl[op.x >> 16] + gv * sl[op.y];
+            }
+            __syncthreads();
+        }
+        const int fa = mb.fa, fb = mb.fb, fc = mk.fa, fd = mk.fb;
+        const int n = g.nbf;
+        if (MODE == MODE_SCHWARZ) {
+            double m = 0.0;
+            for (int k = lane; k < d.nfin; k += 64) {
+                const uint32_t f = g_prog_aux[d.fin_off + k];
+                const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
+                if (a == c && b == e) m = fmax(m, fabs(sl[f & 0xffff]));
+            }
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+            if (lane == 0) g.out[bp] = sqrt(m);
+        } else if (MODE == MODE_TENSOR) {
+            TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
+            for (int k = lane; k < d.nfin; k += 64) {
+                const uint32_t f = g_prog_aux[d.fin_off + k];
+                cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);
+            }
+        } else {
+            const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
+            StreamDigest cb{g.D, g.J, g.K, n, fa, fb, fc, fd, s};
+            for (int k = lane; k < d.nfin; k += 64) {
+                const uint32_t f = g_prog_aux[d.fin_off + k];
+                cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);
+            }
+        }
+        __syncthreads();
+    }
+    (void)na; (void)nb; (void)nc; (void)nd;
+}
+
+int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
+    const int p = prog_index(la, lb, lc, ld);
+    return p < 0 ? -1 : g_prog_nslot[p];
+}
+
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,
+                        hipStream_t stream) {
+    const int p = prog_index(la, lb, lc, ld);
+    if (p < 0 || a.nwork <= 0) return p < 0 ? -1 : 0;
+    const long long grid = a.nwork < 4096 ? a.nwork : 4096;
+    const size_t lds = (size_t)nslot * sizeof(double);
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    if (mode == MODE_JK)
+        hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+    else if (mode == MODE_SCHWARZ)
+        hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+    else
+        hipLaunchKernelGGL(k_eri_coop<MODE_TENSOR>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}

@@ -184,9 +184,12 @@ static double flop_model(int la, int lb, int lc, int ld) {
 struct ClassPairWork {
     int X, Y;
     int la, lb, lc, ld;
-    int* d_ket;            // ket ids (class Y, sorted by Q desc)
+    int* d_ket;            // ket ids of class Y grouped by primitive count
     int* d_bra;            // bra ids (class X, sorted by Q desc)
     long long* d_row;
+    int* d_rb;             // nrows x (nbucket+1) bucket offsets inside each row
+    int* d_bstart;         // bucket starts of class Y
+    int nbucket;
     int nrows;
     long long nwork;
     double prim_quartets;
@@ -209,6 +212,11 @@ struct polyeri_ctx {
     double* d_Q = nullptr;
     int* d_class_ids[NPAIRCLASS] = {};
     std::vector<int> class_ids[NPAIRCLASS];
+    // ket order of each class: buckets of equal primitive-pair count, each
+    // bucket in Schwarz order
+    int* d_ket_ids[NPAIRCLASS] = {};
+    int* d_bstart[NPAIRCLASS] = {};
+    std::vector<int> ket_ids[NPAIRCLASS], bstart[NPAIRCLASS];
     std::vector<ClassPairWork> work;
     int* d_ref = nullptr;
     double* d_scale = nullptr;
@@ -250,8 +258,21 @@ static int dalloc(T** p, size_t n) {
 static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
-                        const int* ket, const long long* row, int nrows, long long nwork, double* out) {
+                        const int* ket, const long long* row, int nrows, long long nwork, double* out,
+                        const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0) {
     if (nwork <= 0) return 0;
+    if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) {
+        EriArgs a;
+        a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
+        a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
+        a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
+        a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
+        int ns = polyeri_coop_nslot(la, lb, lc, ld);
+        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
+        int r = polyeri_coop_launch(a, mode, la, lb, lc, ld, ns, c->stream);
+        if (r) return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
+        return 0;
+    }
     EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
                 : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
                 : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
@@ -259,6 +280,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     EriArgs a;
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
+    a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
     a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
@@ -285,8 +307,8 @@ static void free_ctx(polyeri_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
-    for (int i = 0; i < NPAIRCLASS; ++i) hipFree(c->d_class_ids[i]);
-    for (auto& w : c->work) hipFree(w.d_row);
+    for (int i = 0; i < NPAIRCLASS; ++i) { hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); }
+    for (auto& w : c->work) { hipFree(w.d_row); hipFree(w.d_rb); }
     hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
     hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
     for (auto e : c->ev) hipEventDestroy(e);
@@ -364,47 +386,38 @@ static int build_ctx(polyeri_ctx* c) {
         if (!ids.empty())
             HIPCHK(hipMemcpy(c->d_class_ids[X], ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     }
-    // work lists per class pair X >= Y: bra i in X, ket j in Y with
-    // Q_i Q_j >= tol (and j <= i within one class)
-    const double tol = c->opts.schwarz_tol;
+    // class-pair work lists (host.cpp build_work_lists)
+    std::vector<WorkList> wl;
+    build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, c->class_ids, c->ket_ids, c->bstart, wl);
+    for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+        if ((rc = dalloc(&c->d_ket_ids[Y], c->ket_ids[Y].size()))) return rc;
+        if ((rc = dalloc(&c->d_bstart[Y], c->bstart[Y].size()))) return rc;
+        if (!c->ket_ids[Y].empty())
+            HIPCHK(hipMemcpy(c->d_ket_ids[Y], c->ket_ids[Y].data(), c->ket_ids[Y].size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_bstart[Y], c->bstart[Y].data(), c->bstart[Y].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     c->quartets = 0; c->prim_quartets = 0; c->model_flops = 0;
-    for (int X = 0; X < NPAIRCLASS; ++X)
-        for (int Y = 0; Y <= X; ++Y) {
-            const auto& bx = c->class_ids[X];
-            const auto& ky = c->class_ids[Y];
-            ClassPairWork w;
-            w.X = X; w.Y = Y;
-            w.la = kPairL[X][0]; w.lb = kPairL[X][1]; w.lc = kPairL[Y][0]; w.ld = kPairL[Y][1];
-            w.d_bra = c->d_class_ids[X]; w.d_ket = c->d_class_ids[Y];
-            w.nrows = (int)bx.size();
-            std::vector<long long> row(bx.size() + 1, 0);
-            std::vector<double> kprim(ky.size() + 1, 0.0);   // prefix of ket primitive pairs
-            for (size_t j = 0; j < ky.size(); ++j) kprim[j + 1] = kprim[j] + c->pairs[ky[j]].nprim;
-            double pq = 0;
-            for (size_t i = 0; i < bx.size(); ++i) {
-                double qi = c->Q[bx[i]];
-                // number of ket pairs with Q_j >= tol / qi
-                size_t lo = 0, hi = ky.size();
-                while (lo < hi) {
-                    size_t mid = (lo + hi) / 2;
-                    if (qi * c->Q[ky[mid]] >= tol) lo = mid + 1; else hi = mid;
-                }
-                size_t cnt = lo;
-                if (X == Y) cnt = std::min(cnt, i + 1);
-                row[i + 1] = row[i] + (long long)cnt;
-                pq += c->pairs[bx[i]].nprim * kprim[cnt];
-            }
-            w.nwork = row[bx.size()];
-            w.prim_quartets = pq;
-            w.flops = pq * flop_model(w.la, w.lb, w.lc, w.ld);
-            w.d_row = nullptr;
-            if ((rc = dalloc(&w.d_row, row.size()))) return rc;
-            HIPCHK(hipMemcpy(w.d_row, row.data(), row.size() * sizeof(long long), hipMemcpyHostToDevice));
-            c->quartets += w.nwork;
-            c->prim_quartets += pq;
-            c->model_flops += pq * flop_model(w.la, w.lb, w.lc, w.ld);
-            c->work.push_back(w);
-        }
+    for (const WorkList& l : wl) {
+        ClassPairWork w;
+        w.X = l.X; w.Y = l.Y;
+        w.la = kPairL[l.X][0]; w.lb = kPairL[l.X][1]; w.lc = kPairL[l.Y][0]; w.ld = kPairL[l.Y][1];
+        w.d_bra = c->d_class_ids[l.X]; w.d_ket = c->d_ket_ids[l.Y]; w.d_bstart = c->d_bstart[l.Y];
+        w.nbucket = l.nbucket;
+        w.nrows = l.nrows;
+        w.nwork = l.nwork;
+        w.prim_quartets = l.prim_quartets;
+        w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
+        w.d_row = nullptr;
+        w.d_rb = nullptr;
+        if ((rc = dalloc(&w.d_row, l.row.size()))) return rc;
+        if ((rc = dalloc(&w.d_rb, l.rb.size()))) return rc;
+        HIPCHK(hipMemcpy(w.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
+        if (!l.rb.empty()) HIPCHK(hipMemcpy(w.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
+        c->quartets += w.nwork;
+        c->prim_quartets += w.prim_quartets;
+        c->model_flops += w.flops;
+        c->work.push_back(w);
+    }
     c->ev.resize(c->work.size() + 1);
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     c->last_class_ms.assign(c->work.size(), 0.0);
@@ -500,7 +513,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         int rc = launch_class(c, MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
-                              cw.nwork, nullptr);
+                              cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -609,7 +622,7 @@ static int tensor_device(polyeri_ctx* c, double* d_t) {
     HIPCHK(hipGetLastError());
     for (const ClassPairWork& cw : c->work) {
         int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
-                              cw.nrows, cw.nwork, d_t);
+                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket);
         if (rc) return rc;
     }
     return 0;

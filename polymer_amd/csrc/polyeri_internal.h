@@ -57,6 +57,19 @@ inline int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
 constexpr int NPAIRCLASS = 6;       // (ss) (ps) (pp) (ds) (dp) (dd)
 constexpr int MAXL = 2;
 
+// Screened work list of one class pair (X >= Y): bra rows are the pairs of
+// class X in Schwarz order; the kets of class Y are grouped in buckets of
+// equal primitive-pair count (Schwarz order inside each bucket); row i holds,
+// bucket by bucket, the kets j with Q_i Q_j >= tol (and rank_j <= rank_i
+// when X == Y), each a prefix of its bucket.
+struct WorkList {
+    int X = 0, Y = 0, nrows = 0, nbucket = 1;
+    long long nwork = 0;
+    std::vector<long long> row;     // nrows + 1, exclusive prefix
+    std::vector<int> rb;            // nrows x (nbucket + 1), offsets inside a row
+    double prim_quartets = 0;
+};
+
 // Host functions (host.cpp).
 int read_basis(const std::string& path, int natom, const int* iatom, const double* xyz,
                RefBasis& out, std::string& err);
@@ -64,6 +77,11 @@ int extract_shells(const RefBasis& rb, IntBasis& ib, std::string& err);
 void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pairs,
                  std::vector<PrimPair>& prims);
 void boys_table(std::vector<double>& tab);       // BOYS_NT x BOYS_NM
+// class_ids[X] must hold the pairs of class X in Schwarz order (bra rows);
+// fills ket_ids / bstart (bucketed ket order) and the lists.
+void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
+                      const std::vector<int> class_ids[], std::vector<int> ket_ids[],
+                      std::vector<int> bstart[], std::vector<WorkList>& out);
 // S, T, V in the internal ordering (nbf x nbf, row-major, symmetric).
 void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vector<double>& S,
                   std::vector<double>& T, std::vector<double>& V);
