@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--nwater", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-mib", type=int, default=0,
+                    help="run the library's copy kernel over this many MiB first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -98,6 +100,13 @@ def main():
     nb = n * n * 8
     d_p, d_f, d_j, d_k = (ctx.device_alloc(nb) for _ in range(4))
     ctx.to_device(d_p, p)
+    if args.calibrate_mib:
+        nbytes = args.calibrate_mib << 20
+        s_, d_ = ctx.device_alloc(nbytes), ctx.device_alloc(nbytes)
+        print(f"bench: calibration copy of {nbytes} bytes", file=sys.stderr, flush=True)
+        ctx.device_copy(d_, s_, nbytes)
+        ctx.device_free(s_)
+        ctx.device_free(d_)
     for _ in range(args.warmup):
         ctx.fock_build_device(d_p, d_f, d_j, d_k)
         ctx.sync()

@@ -81,6 +81,9 @@ int polyeri_sync(polyeri_ctx *ctx);
 int polyeri_device_alloc(polyeri_ctx *ctx, size_t bytes, void **ptr);
 int polyeri_device_free(polyeri_ctx *ctx, void *ptr);
 int polyeri_memcpy(polyeri_ctx *ctx, void *dst, const void *src, size_t bytes, int to_device);
+/* Device-to-device copy with the library's own streaming kernel (bytes a
+ * multiple of 16); blocking.  A known-size reference for hardware counters. */
+int polyeri_device_copy(polyeri_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Drop-in for FOURINDEX_ERI followed by the Fock assembly of FOCK_BUILD
  * (src/pfock.f90:44, with the Hermitian average of :224-233):

@@ -174,6 +174,9 @@ class Context:
     def fock_build_device(self, d_p, d_f, d_j, d_k, hfexchange=1.0):
         check(lib().polyeri_fock_build_device(self.h, d_p, hfexchange, d_f, d_j, d_k))
 
+    def device_copy(self, dst, src, nbytes):
+        check(lib().polyeri_device_copy(self.h, dst, src, nbytes))
+
     def sync(self):
         check(lib().polyeri_sync(self.h))
 

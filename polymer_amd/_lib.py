@@ -16,7 +16,7 @@ EXPORTS = [
     "polyeri_jk_build_device", "polyeri_sync", "polyeri_one_electron", "polyeri_eri_tensor",
     "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
-    "polyeri_fock_build_device",
+    "polyeri_fock_build_device", "polyeri_device_copy",
 ]
 
 
@@ -89,6 +89,7 @@ def lib():
     L.polyeri_device_free.argtypes = [p, p]
     L.polyeri_memcpy.argtypes = [p, p, p, ctypes.c_size_t, i]
     L.polyeri_fock_build_device.argtypes = [p, p, d, p, p, p]
+    L.polyeri_device_copy.argtypes = [p, p, p, ctypes.c_size_t]
     for name in EXPORTS:
         if name not in ("polyeri_ctx_destroy", "polyeri_last_error"):
             getattr(L, name).restype = ctypes.c_int

@@ -34,7 +34,25 @@ struct EriArgs {
     double* K;
     int nbf;
     double* out;               // tensor (MODE_TENSOR) or Schwarz (MODE_SCHWARZ)
+    int jk_copies;             // 1: one J/K (device-scope atomics); 8: one per XCD
 };
+
+// J/K accumulation.  With one copy per XCD (jk_copies == 8) a block adds into
+// the copy of the XCD it runs on (hardware XCC_ID register, masked to 0..7)
+// with workgroup-scope atomics, which complete in that XCD's L2; the copies
+// are summed afterwards.  With one copy the atomics are device scope.
+__device__ __forceinline__ int xcd_slot(int copies) {
+    if (copies <= 1) return 0;
+    const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);   // HW_REG_XCC_ID[3:0]
+    return xcc & (copies - 1);
+}
+
+__device__ __forceinline__ void jk_add(double* p, double v, int copies) {
+    if (copies > 1)
+        __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        atomicAdd(p, v);
+}
 
 // Digestion with the quartet's six density / Fock blocks held in registers
 // (light classes).  With s the permutational degeneracy weight:
@@ -75,27 +93,28 @@ struct RegDigest {
         Kbc[b * NC + c] += x * Dad[a * ND + d];
         Kbd[b * ND + d] += x * Dac[a * NC + c];
     }
-    __device__ __forceinline__ void flush(double* J, double* K, int n, int fa, int fb, int fc, int fd, double s) {
+    __device__ __forceinline__ void flush(double* J, double* K, int n, int fa, int fb, int fc, int fd, double s,
+                                          int copies) {
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) atomicAdd(&J[(size_t)(fa + a) * n + fb + b], s * Jab[a * NB + b]);
+            for (int b = 0; b < NB; ++b) jk_add(&J[(size_t)(fa + a) * n + fb + b], s * Jab[a * NB + b], copies);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) atomicAdd(&K[(size_t)(fa + a) * n + fc + c], s * Kac[a * NC + c]);
+            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fa + a) * n + fc + c], s * Kac[a * NC + c], copies);
 #pragma unroll
-            for (int d = 0; d < ND; ++d) atomicAdd(&K[(size_t)(fa + a) * n + fd + d], s * Kad[a * ND + d]);
+            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fa + a) * n + fd + d], s * Kad[a * ND + d], copies);
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) atomicAdd(&K[(size_t)(fb + b) * n + fc + c], s * Kbc[b * NC + c]);
+            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fb + b) * n + fc + c], s * Kbc[b * NC + c], copies);
 #pragma unroll
-            for (int d = 0; d < ND; ++d) atomicAdd(&K[(size_t)(fb + b) * n + fd + d], s * Kbd[b * ND + d]);
+            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fb + b) * n + fd + d], s * Kbd[b * ND + d], copies);
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) atomicAdd(&J[(size_t)(fc + c) * n + fd + d], s * Jcd[c * ND + d]);
+            for (int d = 0; d < ND; ++d) jk_add(&J[(size_t)(fc + c) * n + fd + d], s * Jcd[c * ND + d], copies);
     }
 };
 
@@ -108,15 +127,16 @@ struct StreamDigest {
     double* K;
     int n, fa, fb, fc, fd;
     double s;
+    int copies;
     __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
         const size_t A = fa + a, B = fb + b, C = fc + c, E = fd + d;
         const double sx = s * x;
-        atomicAdd(&J[A * n + B], sx * D[C * n + E]);
-        atomicAdd(&J[C * n + E], sx * D[A * n + B]);
-        atomicAdd(&K[A * n + C], sx * D[B * n + E]);
-        atomicAdd(&K[A * n + E], sx * D[B * n + C]);
-        atomicAdd(&K[B * n + C], sx * D[A * n + E]);
-        atomicAdd(&K[B * n + E], sx * D[A * n + C]);
+        jk_add(&J[A * n + B], sx * D[C * n + E], copies);
+        jk_add(&J[C * n + E], sx * D[A * n + B], copies);
+        jk_add(&K[A * n + C], sx * D[B * n + E], copies);
+        jk_add(&K[A * n + E], sx * D[B * n + C], copies);
+        jk_add(&K[B * n + C], sx * D[A * n + E], copies);
+        jk_add(&K[B * n + E], sx * D[A * n + C], copies);
     }
 };
 
@@ -197,6 +217,7 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
         quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
         RegDigest<NA, NB, NC, ND> cb;
         cb.load(g.D, n, fa, fb, fc, fd);
+        const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -205,7 +226,7 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
                     for (int d = 0; d < ND; ++d) cb(a, b, c, d, v[((a * NB + b) * NC + c) * ND + d]);
-        cb.flush(g.J, g.K, n, fa, fb, fc, fd, s);
+        cb.flush(g.J + off, g.K + off, n, fa, fb, fc, fd, s, g.jk_copies);
     }
 }
 

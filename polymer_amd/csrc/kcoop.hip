@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             }
         } else {
             const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
-            StreamDigest cb{g.D, g.J, g.K, n, fa, fb, fc, fd, s};
+            const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
+            StreamDigest cb{g.D, g.J + off, g.K + off, n, fa, fb, fc, fd, s, g.jk_copies};
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);

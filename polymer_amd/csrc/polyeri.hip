@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -67,8 +68,9 @@ __global__ void k_pack_density(const double* __restrict__ pc, int nc, const int*
     D[t] = sc[mu] * sc[nu] * pc[ref[mu] + (size_t)nc * ref[nu]];
 }
 
-// C_C(ref mu, ref nu) = s s 2(J + J^T), X_C = s s (K + K^T)
-__global__ void k_unpack_jk(const double* __restrict__ J, const double* __restrict__ K, int nbf,
+// C_C(ref mu, ref nu) = s s 2(J + J^T), X_C = s s (K + K^T), summed over the
+// per-XCD copies
+__global__ void k_unpack_jk(const double* __restrict__ J, const double* __restrict__ K, int nbf, int copies,
                             const int* __restrict__ ref, const double* __restrict__ sc, int nc,
                             double* __restrict__ jc, double* __restrict__ kc) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,8 +78,14 @@ __global__ void k_unpack_jk(const double* __restrict__ J, const double* __restri
     int mu = (int)(t / nbf), nu = (int)(t % nbf);
     double f = sc[mu] * sc[nu];
     size_t r = ref[mu] + (size_t)nc * ref[nu];
-    jc[r] = f * 2.0 * (J[(size_t)mu * nbf + nu] + J[(size_t)nu * nbf + mu]);
-    kc[r] = f * (K[(size_t)mu * nbf + nu] + K[(size_t)nu * nbf + mu]);
+    const size_t n2 = (size_t)nbf * nbf;
+    double js = 0.0, ks = 0.0;
+    for (int x = 0; x < copies; ++x) {
+        js += J[x * n2 + (size_t)mu * nbf + nu] + J[x * n2 + (size_t)nu * nbf + mu];
+        ks += K[x * n2 + (size_t)mu * nbf + nu] + K[x * n2 + (size_t)nu * nbf + mu];
+    }
+    jc[r] = f * 2.0 * js;
+    kc[r] = f * ks;
 }
 
 // F = T + V + J - hfx/2 K, then the Hermitian average (src/pfock.f90:44, :224-233)
@@ -117,6 +125,13 @@ __global__ void k_mo_int(const double* __restrict__ co, int nc, const int* __res
     if (t >= (size_t)nbf * nmo) return;
     int mu = (int)(t % nbf), i = (int)(t / nbf);
     C[t] = sc[mu] * co[ref[mu] + (size_t)nc * i];
+}
+
+// Plain streaming copy (16 B per lane per step): the reference point that
+// scales the FETCH_SIZE / WRITE_SIZE counters to bytes.
+__global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // E(2) from T(b, j, a, i) = (ia|jb): src/pmp2energy.f90:395-444 at one k-point
@@ -235,6 +250,7 @@ struct polyeri_ctx {
     std::vector<double> last_class_ms;
     rocblas_handle blas = nullptr;
     double *d_T = nullptr, *d_V = nullptr;   // T_C, N_C on the device (Fock assembly)
+    int jk_copies = 8;                       // J/K accumulation copies (one per XCD)
 };
 
 static int ensure_onee_device(polyeri_ctx* c) {
@@ -267,6 +283,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
         a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
         a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
         a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
+        a.jk_copies = c->jk_copies;
         int ns = polyeri_coop_nslot(la, lb, lc, ld);
         if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
         int r = polyeri_coop_launch(a, mode, la, lb, lc, ld, ns, c->stream);
@@ -282,6 +299,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
     a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
+    a.jk_copies = c->jk_copies;
     hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return 0;
@@ -350,8 +368,13 @@ static int build_ctx(polyeri_ctx* c) {
     if ((rc = dalloc(&c->d_ref, nbf))) return rc;
     if ((rc = dalloc(&c->d_scale, nbf))) return rc;
     if ((rc = dalloc(&c->d_D, (size_t)nbf * nbf))) return rc;
-    if ((rc = dalloc(&c->d_J, (size_t)nbf * nbf))) return rc;
-    if ((rc = dalloc(&c->d_K, (size_t)nbf * nbf))) return rc;
+    {
+        const char* e = getenv("POLYERI_JK_COPIES");
+        c->jk_copies = e ? atoi(e) : 8;
+        if (c->jk_copies != 1 && c->jk_copies != 8) c->jk_copies = 8;
+    }
+    if ((rc = dalloc(&c->d_J, (size_t)nbf * nbf * c->jk_copies))) return rc;
+    if ((rc = dalloc(&c->d_K, (size_t)nbf * nbf * c->jk_copies))) return rc;
     if ((rc = dalloc(&c->d_pc, (size_t)nc * nc))) return rc;
     if ((rc = dalloc(&c->d_jc, (size_t)nc * nc))) return rc;
     if ((rc = dalloc(&c->d_kc, (size_t)nc * nc))) return rc;
@@ -506,8 +529,8 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_pack_density, dim3(nblk(n2)), dim3(256), 0, c->stream, d_pc, nc, c->d_ref,
                        c->d_scale, nbf, c->d_D);
-    hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, n2, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_K, n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n2 * c->jk_copies)), dim3(256), 0, c->stream, c->d_J, n2 * c->jk_copies, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n2 * c->jk_copies)), dim3(256), 0, c->stream, c->d_K, n2 * c->jk_copies, 0.0);
     HIPCHK(hipGetLastError());
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
@@ -519,8 +542,8 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_jc, c2, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_kc, c2, 0.0);
-    hipLaunchKernelGGL(k_unpack_jk, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, c->d_K, nbf, c->d_ref,
-                       c->d_scale, nc, d_jc, d_kc);
+    hipLaunchKernelGGL(k_unpack_jk, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, c->d_K, nbf, c->jk_copies,
+                       c->d_ref, c->d_scale, nc, d_jc, d_kc);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return 0;
@@ -587,6 +610,16 @@ int polyeri_memcpy(polyeri_ctx* ctx, void* dst, const void* src, size_t bytes, i
     if (!ctx || !dst || !src) return fail(POLYERI_E_ARG, "polyeri_memcpy: null argument");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int polyeri_device_copy(polyeri_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || !dst || !src || bytes % 16) return fail(POLYERI_E_ARG, "polyeri_device_copy: bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, ctx->stream, (const double2*)src, (double2*)dst, n);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
