@@ -107,22 +107,82 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);
             }
         } else {
+            // Gather digestion: the quartet's integrals are copied into a dense
+            // (a,b,c,d)-ordered array and the six density blocks into LDS; then
+            // each lane forms whole output elements of the six J/K blocks, so
+            // one atomic per output element replaces six per integral.
             const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
-            const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
-            StreamDigest cb{g.D, g.J + off, g.K + off, n, fa, fb, fc, fd, s, g.jk_copies};
+            double* dv = sl + d.nslot;
+            double* Dab = dv + d.nfin;
+            double* Dcd = Dab + na * nb;
+            double* Dac = Dcd + nc * nd;
+            double* Dad = Dac + na * nc;
+            double* Dbc = Dad + na * nd;
+            double* Dbd = Dbc + nb * nc;
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
-                cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);
+                const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
+                dv[((a * nb + b) * nc + c) * nd + e] = sl[f & 0xffff];
+            }
+            for (int k = lane; k < na * nb; k += 64) Dab[k] = g.D[(size_t)(fa + k / nb) * n + fb + k % nb];
+            for (int k = lane; k < nc * nd; k += 64) Dcd[k] = g.D[(size_t)(fc + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < na * nc; k += 64) Dac[k] = g.D[(size_t)(fa + k / nc) * n + fc + k % nc];
+            for (int k = lane; k < na * nd; k += 64) Dad[k] = g.D[(size_t)(fa + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < nb * nc; k += 64) Dbc[k] = g.D[(size_t)(fb + k / nc) * n + fc + k % nc];
+            for (int k = lane; k < nb * nd; k += 64) Dbd[k] = g.D[(size_t)(fb + k / nd) * n + fd + k % nd];
+            __syncthreads();
+            const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
+            double* J = g.J + off;
+            double* K = g.K + off;
+            const int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
+                      o6 = o5 + nb * nd;
+            for (int o = lane; o < o6; o += 64) {
+                double acc = 0.0;
+                double* dst;
+                if (o < o1) {                          // J[a,b] += v(ab|cd) D[c,d]
+                    const int a = o / nb, b = o % nb;
+                    const double* v = dv + (a * nb + b) * nc * nd;
+                    for (int k = 0; k < nc * nd; ++k) acc += v[k] * Dcd[k];
+                    dst = &J[(size_t)(fa + a) * n + fb + b];
+                } else if (o < o2) {                   // J[c,d] += v(ab|cd) D[a,b]
+                    const int k0 = o - o1;
+                    for (int k = 0; k < na * nb; ++k) acc += dv[k * nc * nd + k0] * Dab[k];
+                    dst = &J[(size_t)(fc + k0 / nd) * n + fd + k0 % nd];
+                } else if (o < o3) {                   // K[a,c] += v(ab|cd) D[b,d]
+                    const int a = (o - o2) / nc, c = (o - o2) % nc;
+                    for (int b = 0; b < nb; ++b)
+                        for (int e = 0; e < nd; ++e) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dbd[b * nd + e];
+                    dst = &K[(size_t)(fa + a) * n + fc + c];
+                } else if (o < o4) {                   // K[a,d] += v(ab|cd) D[b,c]
+                    const int a = (o - o3) / nd, e = (o - o3) % nd;
+                    for (int b = 0; b < nb; ++b)
+                        for (int c = 0; c < nc; ++c) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dbc[b * nc + c];
+                    dst = &K[(size_t)(fa + a) * n + fd + e];
+                } else if (o < o5) {                   // K[b,c] += v(ab|cd) D[a,d]
+                    const int b = (o - o4) / nc, c = (o - o4) % nc;
+                    for (int a = 0; a < na; ++a)
+                        for (int e = 0; e < nd; ++e) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dad[a * nd + e];
+                    dst = &K[(size_t)(fb + b) * n + fc + c];
+                } else {                               // K[b,d] += v(ab|cd) D[a,c]
+                    const int b = (o - o5) / nd, e = (o - o5) % nd;
+                    for (int a = 0; a < na; ++a)
+                        for (int c = 0; c < nc; ++c) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dac[a * nc + c];
+                    dst = &K[(size_t)(fb + b) * n + fd + e];
+                }
+                jk_add(dst, s * acc, g.jk_copies);
             }
         }
         __syncthreads();
     }
-    (void)na; (void)nb; (void)nc; (void)nd;
 }
 
+// LDS doubles one wavefront of the class needs: the program's node slots,
+// plus, for J/K, the dense integral block and the six density blocks.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
     const int p = prog_index(la, lb, lc, ld);
-    return p < 0 ? -1 : g_prog_nslot[p];
+    if (p < 0) return -1;
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
 }
 
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,
