@@ -7,11 +7,17 @@ a 4.5 A ball), Schwarz threshold 1e-12.  One step = one Fock build: the
 direct J/K build over every surviving shell quartet plus the Fock assembly
 F = T + V + J - K/2, with the density already resident in HBM.
 
-Reported value: ERI primitive quartets per second (unique 8-fold quartets
-of real shells that survive the screen), with the Fock build seconds per
-iteration beside it.  The CPU baseline is the oracle (a C restatement of the
+Reported value: contracted shell quartets per second of the FP64 build --
+the symmetry-unique (ab|cd) quartets of real contracted shells whose Schwarz
+product Q_ab Q_cd survives the 1e-12 screen, divided by the Fock-build time
+-- with the Fock build seconds per iteration beside it, and the same for the
+mixed FP32/FP64 build (quartets below Q_ab Q_cd = 1e-5 in FP32).  Both SCFs
+are then converged (1e-8 RMS density change) and their energy difference is
+reported.  The CPU baseline is the oracle (a C restatement of the
 reference's FOURINDEX_ERI loop nest, single thread) timed on a bounded
-sample of its bra shell pairs on this host and extrapolated to the full job.
+sample of its bra shell pairs on this host and extrapolated to the full job,
+reported in the same unit (the GPU's quartet count per build divided by the
+CPU's time per build).
 """
 import argparse
 import json
@@ -26,7 +32,7 @@ METRIC = "direct-SCF Fock build s/iter + ERI quartets/s, FP64 vs mixed at 1 µEh
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
 
-def cpu_baseline(atoms, basis, unique_prim_quartets, budget_s):
+def cpu_baseline(atoms, basis, quartets_per_build, budget_s):
     """Oracle FOURINDEX_ERI (INTTOL 1e-12, KL table rebuilt per (I,J) as the
     reference does) on evenly spaced single bra rows until budget_s of CPU
     time is spent; the mean row time times the number of rows is the full
@@ -50,7 +56,7 @@ def cpu_baseline(atoms, basis, unique_prim_quartets, budget_s):
     _, _, setup, spent = ob.fourindex_eri_rows(p, 1e-12, rows, recompute_kl=True)
     rows = nrows
     t_full = setup + spent / rows * nsq
-    return {"value": unique_prim_quartets / t_full, "unit": "quartets/s", "cores": 1,
+    return {"value": quartets_per_build / t_full, "unit": "quartets/s", "cores": 1,
             "kind": "port",
             "sample": f"{rows} evenly spaced of {nsq} bra shell-pair rows (I,J) of the oracle's "
                       f"FOURINDEX_ERI loop nest (INTTOL 1e-12, KL table rebuilt per row), "
@@ -68,12 +74,13 @@ def main():
     ap.add_argument("--nwater", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the two converged SCF runs")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="run the library's copy kernel over this many MiB first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the J/K build "
+              "is not sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -107,37 +114,73 @@ def main():
         ctx.device_copy(d_, s_, nbytes)
         ctx.device_free(s_)
         ctx.device_free(d_)
-    for _ in range(args.warmup):
-        ctx.fock_build_device(d_p, d_f, d_j, d_k)
-        ctx.sync()
-    print("bench: timing", file=sys.stderr, flush=True)
-    ctx.set_profiling(True)
-    st0 = ctx.stats()
-    cls_ms = np.zeros(st0["nclass"])
-    jk_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.fock_build_device(d_p, d_f, d_j, d_k)
-        ctx.sync()
-        st = ctx.stats()
-        cls_ms += np.array([c["ms"] for c in st["classes"]])
-        jk_ms += st["last_jk_ms"]
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) / args.steps * 1e3
-    st = ctx.stats()
-    cls_ms /= args.steps
-    jk_ms /= args.steps
+
+    def timed(c, dp, df, dj, dk):
+        """Wall time per build over args.steps builds (no events between the
+        launches), then the same number of builds with per-list events for
+        the class table."""
+        for _ in range(args.warmup):
+            c.fock_build_device(dp, df, dj, dk)
+            c.sync()
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            c.fock_build_device(dp, df, dj, dk)
+        c.sync()
+        t1 = time.perf_counter()
+        c.set_profiling(True)
+        st0 = c.stats()
+        cls_ms = np.zeros(st0["nclass"])
+        jk_ms = 0.0
+        for _ in range(args.steps):
+            c.fock_build_device(dp, df, dj, dk)
+            c.sync()
+            st = c.stats()
+            cls_ms += np.array([x["ms"] for x in st["classes"]])
+            jk_ms += st["last_jk_ms"]
+        c.set_profiling(False)
+        return (t1 - t0) / args.steps * 1e3, cls_ms / args.steps, jk_ms / args.steps, c.stats()
+
+    print("bench: timing fp64", file=sys.stderr, flush=True)
+    ms_per_step, cls_ms, jk_ms, st = timed(ctx, d_p, d_f, d_j, d_k)
     s_iter = ms_per_step * 1e-3
-    value = st["prim_quartets"] / s_iter
+    value = st["quartets"] / s_iter
     dom = int(np.argmax(cls_ms))
     dc = st["classes"][dom]
     achieved = dc["flops"] / (cls_ms[dom] * 1e-3) / 1e12
-    classes = {("(" + "spd"[int(c["cls"][0])] + "spd"[int(c["cls"][1])] + "|" + "spd"[int(c["cls"][2])]
-                + "spd"[int(c["cls"][3])] + ")"): {"ms": round(float(m), 4),
-                                                    "prim_quartets": c["prim_quartets"],
-                                                    "tflops": round(c["flops"] / max(m, 1e-9) / 1e9, 3)}
-               for c, m in zip(st["classes"], cls_ms)}
+
+    def cname(c):
+        l = c["cls"]
+        return "(" + "spd"[int(l[0])] + "spd"[int(l[1])] + "|" + "spd"[int(l[2])] + "spd"[int(l[3])] + ")" + \
+            ("" if c["prec"] == "fp64" else " fp32")
+
+    def class_table(stt, ms):
+        return {cname(c): {"ms": round(float(m), 4), "quartets": c["quartets"],
+                           "prim_quartets": c["prim_quartets"],
+                           "tflops": round(c["flops"] / max(m, 1e-9) / 1e9, 3)}
+                for c, m in zip(stt["classes"], ms)}
+
+    classes = class_table(st, cls_ms)
     fsum = float(np.sum(np.abs(ctx.from_device(d_f, (n, n)))))
+
+    print("bench: timing mixed", file=sys.stderr, flush=True)
+    cmx = pa.Context(b, schwarz_tol=1e-12, precision=pa.MIXED, mixed_cut=1e-5)
+    mx_ms, mx_cls, mx_jk, mst = timed(cmx, d_p, d_f, d_j, d_k)
+    n32 = sum(c["quartets"] for c in mst["classes"] if c["prec"] == "fp32")
+    mixed = {"fock_build_s_per_iter": mx_ms * 1e-3, "quartets_per_s": mst["quartets"] / (mx_ms * 1e-3),
+             "mixed_cut": 1e-5, "fp32_quartet_fraction": n32 / max(mst["quartets"], 1),
+             "jk_build_ms": mx_jk, "speedup_vs_fp64": ms_per_step / mx_ms}
+    scf = None
+    if not args.no_scf:
+        print("bench: converging both SCFs", file=sys.stderr, flush=True)
+        scf = {}
+        for name, c in (("fp64", ctx), ("mixed", cmx)):
+            t0 = time.perf_counter()
+            r = c.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80)
+            scf[name] = {"e_total": r["e_total"], "iterations": r["iterations"], "converged": bool(r["converged"]),
+                         "wall_s": time.perf_counter() - t0}
+        scf["dE_mixed_minus_fp64"] = scf["mixed"]["e_total"] - scf["fp64"]["e_total"]
+        scf["wall_speedup_mixed"] = scf["fp64"]["wall_s"] / scf["mixed"]["wall_s"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -156,8 +199,10 @@ def main():
                    "precision": "fp64"},
         "fock_build_s_per_iter": s_iter,
         "jk_build_ms": jk_ms,
-        "eri_prim_quartets": st["prim_quartets"],
+        "quartet_unit": "symmetry-unique contracted shell quartet (ab|cd) of real shells with "
+                        "Q_ab Q_cd >= 1e-12",
         "contracted_quartets": st["quartets"],
+        "eri_prim_quartets": st["prim_quartets"],
         "model_tflops_achieved": st["model_flops"] / s_iter / 1e12,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "compute": "fp64 VALU (same 78.6 TFLOP/s peak as FP64 matrix)",
@@ -165,11 +210,14 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None, "kernel_ms": float(cls_ms[dom]), "kernel_model_flops": dc["flops"]},
         "classes": classes,
+        "mixed": mixed,
+        "mixed_classes": class_table(mst, mx_cls),
+        "scf": scf,
         "checksum_abs_F": fsum,
     }
     if not args.no_cpu:
         print("bench: cpu baseline sample", file=sys.stderr, flush=True)
-        cb = cpu_baseline(atoms, basis, st["prim_quartets"], args.cpu_budget)
+        cb = cpu_baseline(atoms, basis, st["quartets"], args.cpu_budget)
         out["cpu_baseline"] = {k: v for k, v in cb.items() if k != "fock_build_s"}
         out["cpu_fock_build_s_per_iter"] = cb["fock_build_s"]
     for d in (d_p, d_f, d_j, d_k):

@@ -141,16 +141,21 @@ int polyeri_scf(polyeri_ctx *ctx, const polyeri_scf_opts *opts, double *p_c, dou
 int polyeri_mp2_energy(polyeri_ctx *ctx, const double *co, const double *eps, int icore,
                        int iocc, int nall, int ivirtcore, double *e2s, double *e2t);
 
+/* One entry per work list: class pair (X >= Y) and, in mixed precision,
+ * FP64 and FP32 lists of each. */
+#define POLYERI_MAX_LISTS 64
 typedef struct polyeri_stats {
     int nshell, nbf, npair, nprim_pair;
-    long long quartets;        /* surviving contracted shell quartets        */
-    double prim_quartets;      /* surviving primitive quartets (8-fold)      */
+    long long quartets;        /* surviving contracted shell quartets (8-fold unique) */
+    double prim_quartets;      /* their primitive quartets                   */
     double model_flops;        /* algorithmic FLOPs of one J/K build         */
     double last_jk_ms;         /* device time of the last J/K build          */
-    double last_class_ms[32];  /* per class-pair kernel time of the last build */
-    double class_prim_quartets[32];
-    double class_flops[32];
-    int class_l[32];           /* la*1000+lb*100+lc*10+ld                    */
+    double last_class_ms[POLYERI_MAX_LISTS];  /* per list kernel time of the last build */
+    double class_prim_quartets[POLYERI_MAX_LISTS];
+    double class_flops[POLYERI_MAX_LISTS];
+    long long class_quartets[POLYERI_MAX_LISTS];
+    int class_l[POLYERI_MAX_LISTS];     /* la*1000+lb*100+lc*10+ld           */
+    int class_prec[POLYERI_MAX_LISTS];  /* 0: FP64, 1: FP32                  */
     int nclass;
 } polyeri_stats;
 int polyeri_get_stats(polyeri_ctx *ctx, polyeri_stats *st);

@@ -187,9 +187,11 @@ class Context:
         st = Stats()
         check(lib().polyeri_get_stats(self.h, ctypes.byref(st)))
         d = {f: getattr(st, f) for f, _ in Stats._fields_
-             if f not in ("last_class_ms", "class_prim_quartets", "class_flops", "class_l")}
+             if f not in ("last_class_ms", "class_prim_quartets", "class_flops", "class_l",
+                          "class_quartets", "class_prec")}
         n = st.nclass
         d["classes"] = [dict(cls=f"{st.class_l[i]:04d}", ms=st.last_class_ms[i],
-                             prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i])
+                             prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i],
+                             quartets=st.class_quartets[i], prec="fp32" if st.class_prec[i] else "fp64")
                         for i in range(n)]
         return d

@@ -50,13 +50,18 @@ class ScfResult(ctypes.Structure):
                 ("fock_seconds", ctypes.c_double), ("scf_seconds", ctypes.c_double)]
 
 
+MAX_LISTS = 64   # POLYERI_MAX_LISTS
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("nshell", ctypes.c_int), ("nbf", ctypes.c_int), ("npair", ctypes.c_int),
                 ("nprim_pair", ctypes.c_int), ("quartets", ctypes.c_longlong),
                 ("prim_quartets", ctypes.c_double), ("model_flops", ctypes.c_double),
-                ("last_jk_ms", ctypes.c_double), ("last_class_ms", ctypes.c_double * 32),
-                ("class_prim_quartets", ctypes.c_double * 32),
-                ("class_flops", ctypes.c_double * 32), ("class_l", ctypes.c_int * 32),
+                ("last_jk_ms", ctypes.c_double), ("last_class_ms", ctypes.c_double * MAX_LISTS),
+                ("class_prim_quartets", ctypes.c_double * MAX_LISTS),
+                ("class_flops", ctypes.c_double * MAX_LISTS),
+                ("class_quartets", ctypes.c_longlong * MAX_LISTS),
+                ("class_l", ctypes.c_int * MAX_LISTS), ("class_prec", ctypes.c_int * MAX_LISTS),
                 ("nclass", ctypes.c_int)]
 
 

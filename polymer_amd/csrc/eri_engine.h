@@ -118,22 +118,24 @@ POLY_CLASS(2, 2, 2, 1)
 POLY_CLASS(2, 2, 2, 2)
 #undef POLY_CLASS
 
-// Contracted (ab|cd) for one shell quartet in FP64.  bra/ket point at the
+// Contracted (ab|cd) for one shell quartet.  bra/ket point at the
 // primitive pairs of the two shell pairs; AB = A - B, CD = C - D.  Every
 // final integral is passed to cb(ia, ib, ic, id, value) with the component
-// indices of gen_eri.py (compile-time constants after inlining).
-template <int LA, int LB, int LC, int LD, typename Cb>
-POLY_HD void quartet_fp64(const PrimPair* __restrict__ bra, int nbra,
-                          const PrimPair* __restrict__ ket, int nket, double ABx,
-                          double ABy, double ABz, double CDx, double CDy, double CDz,
-                          const double* __restrict__ boys_tab, Cb& cb) {
+// indices of gen_eri.py (compile-time constants after inlining).  The pair
+// quantities, T and the Boys function are FP64 in both precisions; real is
+// the type of the VRR / HRR recursion and of the integrals (double, or float
+// for the quartets the mixed-precision mode evaluates in FP32).
+template <typename real, int LA, int LB, int LC, int LD, typename Cb>
+POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const PrimPair* __restrict__ ket,
+                     int nket, double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,
+                     const double* __restrict__ boys_tab, Cb& cb) {
     constexpr int L = LA + LB + LC + LD;
     // components of every l in [LA, LA+LB] (and [LC, LC+LD])
     constexpr int NE = ((LA + LB + 1) * (LA + LB + 2) * (LA + LB + 3) - LA * (LA + 1) * (LA + 2)) / 6;
     constexpr int NF = ((LC + LD + 1) * (LC + LD + 2) * (LC + LD + 3) - LC * (LC + 1) * (LC + 2)) / 6;
-    double acc[NE * NF];
+    real acc[NE * NF];
 #pragma unroll
-    for (int i = 0; i < NE * NF; ++i) acc[i] = 0.0;
+    for (int i = 0; i < NE * NF; ++i) acc[i] = real(0);
     for (int p = 0; p < nbra; ++p) {
         const PrimPair bp = bra[p];
         for (int q = 0; q < nket; ++q) {
@@ -146,22 +148,34 @@ POLY_HD void quartet_fp64(const PrimPair* __restrict__ bra, int nbra,
             const double pref = bp.K * kq.K * sqrt(ize);
             double F[L + 1];
             boys_fp64<L>(T, pref, boys_tab, F);
+            real Fr[L + 1];
+#pragma unroll
+            for (int m = 0; m <= L; ++m) Fr[m] = (real)F[m];
             const double roz = kq.zeta * ize;   // rho / zeta
             const double roe = bp.zeta * ize;   // rho / eta
-            ClassCode<LA, LB, LC, LD, double>::vrr(
-                bp.PAx, bp.PAy, bp.PAz, -roz * PQx, -roz * PQy, -roz * PQz, kq.PAx, kq.PAy,
-                kq.PAz, roe * PQx, roe * PQy, roe * PQz, 0.5 / bp.zeta, 0.5 / kq.zeta, 0.5 * ize,
-                roz, roe, F, acc);
+            ClassCode<LA, LB, LC, LD, real>::vrr(
+                (real)bp.PAx, (real)bp.PAy, (real)bp.PAz, (real)(-roz * PQx), (real)(-roz * PQy),
+                (real)(-roz * PQz), (real)kq.PAx, (real)kq.PAy, (real)kq.PAz, (real)(roe * PQx),
+                (real)(roe * PQy), (real)(roe * PQz), (real)(0.5 / bp.zeta), (real)(0.5 / kq.zeta),
+                (real)(0.5 * ize), (real)roz, (real)roe, Fr, acc);
         }
     }
-    ClassCode<LA, LB, LC, LD, double>::hrr(ABx, ABy, ABz, CDx, CDy, CDz, acc, cb);
+    ClassCode<LA, LB, LC, LD, real>::hrr((real)ABx, (real)ABy, (real)ABz, (real)CDx, (real)CDy, (real)CDz,
+                                         acc, cb);
+}
+
+template <int LA, int LB, int LC, int LD, typename Cb>
+POLY_HD void quartet_fp64(const PrimPair* __restrict__ bra, int nbra, const PrimPair* __restrict__ ket,
+                          int nket, double ABx, double ABy, double ABz, double CDx, double CDy,
+                          double CDz, const double* __restrict__ boys_tab, Cb& cb) {
+    quartet<double, LA, LB, LC, LD>(bra, nbra, ket, nket, ABx, ABy, ABz, CDx, CDy, CDz, boys_tab, cb);
 }
 
 // Store variant: out[((ia*NB + ib)*NC + ic)*ND + id].
-template <int NB, int NC, int ND>
+template <int NB, int NC, int ND, typename real = double>
 struct StoreCb {
-    double* out;
-    POLY_HD void operator()(int a, int b, int c, int d, double v) {
+    real* out;
+    POLY_HD void operator()(int a, int b, int c, int d, real v) {
         out[((a * NB + b) * NC + c) * ND + d] = v;
     }
 };

@@ -8,9 +8,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "eri_engine.h"
 
-enum { MODE_JK = 0, MODE_SCHWARZ = 1, MODE_TENSOR = 2 };
+// MODE_JK32: J/K of the quartets below the mixed-precision cut, recursion in FP32.
+enum { MODE_JK = 0, MODE_SCHWARZ = 1, MODE_TENSOR = 2, MODE_JK32 = 3 };
 
 struct PairMeta {
     int prim_off, nprim, fa, fb;
@@ -27,6 +30,8 @@ struct EriArgs {
     long long nwork;
     const int* rb_off;         // nrows x (nbucket+1): ket-bucket offsets inside each row
     const int* bucket_start;   // nbucket: first ket of each bucket in ket_ids
+    const int* rb_first;       // nrows x nbucket: first ket of the row's range inside each
+                               // bucket (null: every range starts at the bucket start)
     int nbucket;
     const double* boys;
     const double* D;
@@ -59,33 +64,33 @@ __device__ __forceinline__ void jk_add(double* p, double v, int copies) {
 // Jacc[ab] += s v D[cd], Jacc[cd] += s v D[ab], Kacc[ac] += s v D[bd],
 // Kacc[ad] += s v D[bc], Kacc[bc] += s v D[ad], Kacc[bd] += s v D[ac];
 // afterwards J = 2(Jacc + Jacc^T) and K = Kacc + Kacc^T.
-template <int NA, int NB, int NC, int ND>
+template <int NA, int NB, int NC, int ND, typename real = double>
 struct RegDigest {
-    double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dad[NA * ND], Dbc[NB * NC], Dbd[NB * ND];
-    double Jab[NA * NB], Jcd[NC * ND], Kac[NA * NC], Kad[NA * ND], Kbc[NB * NC], Kbd[NB * ND];
+    real Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dad[NA * ND], Dbc[NB * NC], Dbd[NB * ND];
+    real Jab[NA * NB], Jcd[NC * ND], Kac[NA * NC], Kad[NA * ND], Kbc[NB * NC], Kbd[NB * ND];
     __device__ __forceinline__ void load(const double* __restrict__ D, int n, int fa, int fb, int fc, int fd) {
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) { Dab[a * NB + b] = D[(size_t)(fa + a) * n + fb + b]; Jab[a * NB + b] = 0.0; }
+            for (int b = 0; b < NB; ++b) { Dab[a * NB + b] = (real)D[(size_t)(fa + a) * n + fb + b]; Jab[a * NB + b] = 0; }
 #pragma unroll
-            for (int c = 0; c < NC; ++c) { Dac[a * NC + c] = D[(size_t)(fa + a) * n + fc + c]; Kac[a * NC + c] = 0.0; }
+            for (int c = 0; c < NC; ++c) { Dac[a * NC + c] = (real)D[(size_t)(fa + a) * n + fc + c]; Kac[a * NC + c] = 0; }
 #pragma unroll
-            for (int d = 0; d < ND; ++d) { Dad[a * ND + d] = D[(size_t)(fa + a) * n + fd + d]; Kad[a * ND + d] = 0.0; }
+            for (int d = 0; d < ND; ++d) { Dad[a * ND + d] = (real)D[(size_t)(fa + a) * n + fd + d]; Kad[a * ND + d] = 0; }
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) { Dbc[b * NC + c] = D[(size_t)(fb + b) * n + fc + c]; Kbc[b * NC + c] = 0.0; }
+            for (int c = 0; c < NC; ++c) { Dbc[b * NC + c] = (real)D[(size_t)(fb + b) * n + fc + c]; Kbc[b * NC + c] = 0; }
 #pragma unroll
-            for (int d = 0; d < ND; ++d) { Dbd[b * ND + d] = D[(size_t)(fb + b) * n + fd + d]; Kbd[b * ND + d] = 0.0; }
+            for (int d = 0; d < ND; ++d) { Dbd[b * ND + d] = (real)D[(size_t)(fb + b) * n + fd + d]; Kbd[b * ND + d] = 0; }
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) { Dcd[c * ND + d] = D[(size_t)(fc + c) * n + fd + d]; Jcd[c * ND + d] = 0.0; }
+            for (int d = 0; d < ND; ++d) { Dcd[c * ND + d] = (real)D[(size_t)(fc + c) * n + fd + d]; Jcd[c * ND + d] = 0; }
     }
-    __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
+    __device__ __forceinline__ void operator()(int a, int b, int c, int d, real x) {
         Jab[a * NB + b] += x * Dcd[c * ND + d];
         Jcd[c * ND + d] += x * Dab[a * NB + b];
         Kac[a * NC + c] += x * Dbd[b * ND + d];
@@ -98,23 +103,23 @@ struct RegDigest {
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
 #pragma unroll
-            for (int b = 0; b < NB; ++b) jk_add(&J[(size_t)(fa + a) * n + fb + b], s * Jab[a * NB + b], copies);
+            for (int b = 0; b < NB; ++b) jk_add(&J[(size_t)(fa + a) * n + fb + b], s * (double)Jab[a * NB + b], copies);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fa + a) * n + fc + c], s * Kac[a * NC + c], copies);
+            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fa + a) * n + fc + c], s * (double)Kac[a * NC + c], copies);
 #pragma unroll
-            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fa + a) * n + fd + d], s * Kad[a * ND + d], copies);
+            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fa + a) * n + fd + d], s * (double)Kad[a * ND + d], copies);
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fb + b) * n + fc + c], s * Kbc[b * NC + c], copies);
+            for (int c = 0; c < NC; ++c) jk_add(&K[(size_t)(fb + b) * n + fc + c], s * (double)Kbc[b * NC + c], copies);
 #pragma unroll
-            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fb + b) * n + fd + d], s * Kbd[b * ND + d], copies);
+            for (int d = 0; d < ND; ++d) jk_add(&K[(size_t)(fb + b) * n + fd + d], s * (double)Kbd[b * ND + d], copies);
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) jk_add(&J[(size_t)(fc + c) * n + fd + d], s * Jcd[c * ND + d], copies);
+            for (int d = 0; d < ND; ++d) jk_add(&J[(size_t)(fc + c) * n + fd + d], s * (double)Jcd[c * ND + d], copies);
     }
 };
 
@@ -180,7 +185,8 @@ __device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& 
     const int* rb = g.rb_off + (size_t)lo * (g.nbucket + 1);
     int b = 0;
     while (b + 1 < g.nbucket && rb[b + 1] <= local) ++b;
-    kp = g.ket_ids[g.bucket_start[b] + (local - rb[b])];
+    const int first = g.rb_first ? g.rb_first[(size_t)lo * g.nbucket + b] : 0;
+    kp = g.ket_ids[g.bucket_start[b] + first + (local - rb[b])];
 }
 
 template <int LA, int LB, int LC, int LD, int MODE>
@@ -212,10 +218,11 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
         // light classes only (heavier ones run in kcoop.hip): the (ab|cd)
         // block in registers, then the six J/K blocks
-        double v[NA * NB * NC * ND];
-        StoreCb<NB, NC, ND> st{v};
-        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
-        RegDigest<NA, NB, NC, ND> cb;
+        using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
+        real v[NA * NB * NC * ND];
+        StoreCb<NB, NC, ND, real> st{v};
+        quartet<real, LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
+        RegDigest<NA, NB, NC, ND, real> cb;
         cb.load(g.D, n, fa, fb, fc, fd);
         const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
 #pragma unroll

@@ -336,7 +336,7 @@ void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pair
 }
 
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
-                      const std::vector<int> class_ids[], std::vector<int> ket_ids[],
+                      double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
                       std::vector<int> bstart[], std::vector<WorkList>& out) {
     out.clear();
     std::vector<int> rank(pairs.size(), 0);
@@ -355,51 +355,63 @@ void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<doub
             if (j == 0 || pairs[kl[j]].nprim != pairs[kl[j - 1]].nprim) bs.push_back((int)j);
         bs.push_back((int)kl.size());
     }
+    const bool split = cut > tol;
     for (int X = 0; X < NPAIRCLASS; ++X)
-        for (int Y = 0; Y <= X; ++Y) {
-            const auto& bx = class_ids[X];
-            const auto& ky = ket_ids[Y];
-            const auto& bs = bstart[Y];
-            const int nb = (int)bs.size() - 1;
-            WorkList w;
-            w.X = X; w.Y = Y;
-            w.nbucket = std::max(nb, 1);
-            w.nrows = (int)bx.size();
-            w.row.assign(bx.size() + 1, 0);
-            w.rb.assign(bx.size() * (size_t)(w.nbucket + 1), 0);
-            for (size_t i = 0; i < bx.size(); ++i) {
-                const double qi = Q[bx[i]];
-                const int ri = rank[bx[i]];
-                int* r = w.rb.data() + i * (size_t)(w.nbucket + 1);
-                long long tot = 0;
-                double kp = 0;
-                for (int b = 0; b < nb; ++b) {
-                    const int lo = bs[b], hi = bs[b + 1];
-                    int a0 = lo, a1 = hi;        // Schwarz prefix of the bucket
+        for (int Y = 0; Y <= X; ++Y)
+            for (int prec = 0; prec <= (split ? 1 : 0); ++prec) {
+                const auto& bx = class_ids[X];
+                const auto& ky = ket_ids[Y];
+                const auto& bs = bstart[Y];
+                const int nb = (int)bs.size() - 1;
+                WorkList w;
+                w.X = X; w.Y = Y; w.prec = prec;
+                w.nbucket = std::max(nb, 1);
+                w.nrows = (int)bx.size();
+                w.row.assign(bx.size() + 1, 0);
+                w.rb.assign(bx.size() * (size_t)(w.nbucket + 1), 0);
+                if (prec) w.rf.assign(bx.size() * (size_t)w.nbucket, 0);
+                // prefix of bucket [lo, hi) with qi * Q >= thr
+                auto prefix = [&](double qi, int lo, int hi, double thr) {
+                    int a0 = lo, a1 = hi;
                     while (a0 < a1) {
                         const int mid = (a0 + a1) / 2;
-                        if (qi * Q[ky[mid]] >= tol) a0 = mid + 1; else a1 = mid;
+                        if (qi * Q[ky[mid]] >= thr) a0 = mid + 1; else a1 = mid;
                     }
-                    int cnt = a0 - lo;
-                    if (X == Y) {                // rank_j <= rank_i
-                        int b0 = lo, b1 = hi;
-                        while (b0 < b1) {
-                            const int mid = (b0 + b1) / 2;
-                            if (rank[ky[mid]] <= ri) b0 = mid + 1; else b1 = mid;
+                    return a0 - lo;
+                };
+                for (size_t i = 0; i < bx.size(); ++i) {
+                    const double qi = Q[bx[i]];
+                    const int ri = rank[bx[i]];
+                    int* r = w.rb.data() + i * (size_t)(w.nbucket + 1);
+                    long long tot = 0;
+                    double kp = 0;
+                    for (int b = 0; b < nb; ++b) {
+                        const int lo = bs[b], hi = bs[b + 1];
+                        int lim = hi - lo;
+                        if (X == Y) {            // rank_j <= rank_i (a prefix: buckets are in rank order)
+                            int b0 = lo, b1 = hi;
+                            while (b0 < b1) {
+                                const int mid = (b0 + b1) / 2;
+                                if (rank[ky[mid]] <= ri) b0 = mid + 1; else b1 = mid;
+                            }
+                            lim = b0 - lo;
                         }
-                        cnt = std::min(cnt, b0 - lo);
+                        const int n_all = std::min(lim, prefix(qi, lo, hi, tol));
+                        const int n_hi = split ? std::min(n_all, prefix(qi, lo, hi, cut)) : n_all;
+                        const int first = prec ? n_hi : 0;
+                        const int cnt = prec ? n_all - n_hi : n_hi;
+                        if (prec) w.rf[i * (size_t)w.nbucket + b] = first;
+                        r[b] = (int)tot;
+                        tot += cnt;
+                        if (cnt) kp += (double)cnt * pairs[ky[lo]].nprim;
                     }
-                    r[b] = (int)tot;
-                    tot += cnt;
-                    if (cnt) kp += (double)cnt * pairs[ky[lo]].nprim;
+                    for (int b = std::max(nb, 0); b <= w.nbucket; ++b) r[b] = (int)tot;
+                    w.row[i + 1] = w.row[i] + tot;
+                    w.prim_quartets += pairs[bx[i]].nprim * kp;
                 }
-                for (int b = std::max(nb, 0); b <= w.nbucket; ++b) r[b] = (int)tot;
-                w.row[i + 1] = w.row[i] + tot;
-                w.prim_quartets += pairs[bx[i]].nprim * kp;
+                w.nwork = w.row[bx.size()];
+                out.push_back(std::move(w));
             }
-            w.nwork = w.row[bx.size()];
-            out.push_back(std::move(w));
-        }
 }
 
 // F_m(T) on the grid by the convergent series

@@ -29,7 +29,10 @@ __device__ __forceinline__ double boys_taylor(int m, double T, const double* __r
 
 template <int MODE>
 __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd) {
-    extern __shared__ double sl[];
+    // node values in the recursion's precision (FP32 for MODE_JK32)
+    using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
+    extern __shared__ double sl_raw[];
+    real* sl = reinterpret_cast<real*>(sl_raw);
     const ProgDesc& d = g_prog_desc[prog];
     const int lane = threadIdx.x;
     for (long long t = blockIdx.x; t < g.nwork; t += gridDim.x) {
@@ -40,7 +43,7 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             decode_work(g, t, bp, kp);
         }
         const PairMeta mb = g.meta[bp], mk = g.meta[kp];
-        for (int k = lane; k <= d.nacc; k += 64) sl[k] = 0.0;
+        for (int k = lane; k <= d.nacc; k += 64) sl[k] = real(0);
         for (int p = 0; p < mb.nprim; ++p) {
             const PrimPair bq = g.prims[mb.prim_off + p];
             for (int q = 0; q < mk.nprim; ++q) {
@@ -52,21 +55,21 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 const double pref = bq.K * kq.K * sqrt(ize);
                 const double roz = kq.zeta * ize, roe = bq.zeta * ize;
                 const double oo2z = 0.5 / bq.zeta, oo2e = 0.5 / kq.zeta, oo2ze = 0.5 * ize;
-                if (lane <= d.L) sl[d.base + lane] = pref * boys_taylor(lane, T, g.boys);
+                if (lane <= d.L) sl[d.base + lane] = (real)(pref * boys_taylor(lane, T, g.boys));
                 __syncthreads();
                 for (int lv = 0; lv < d.nvrr_lv; ++lv) {
                     for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
                         const uint4 op = g_prog_vrr[o];
                         const int typ = op.w & 1, i = (op.w >> 1) & 3;
-                        const double n3 = (double)((op.w >> 3) & 7), n5 = (double)((op.w >> 6) & 7);
+                        const real n3 = (real)((op.w >> 3) & 7), n5 = (real)((op.w >> 6) & 7);
                         const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
-                        const double c1 = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
-                                              : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
-                        const double c2 = typ ? roe * pq : -roz * pq;
-                        const double x3 = typ ? oo2e : oo2z, r3 = typ ? roe : roz;
+                        const real c1 = (real)(typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
+                                                   : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz)));
+                        const real c2 = (real)(typ ? roe * pq : -roz * pq);
+                        const real x3 = (real)(typ ? oo2e : oo2z), r3 = (real)(typ ? roe : roz);
                         sl[op.x & 0xffff] = c1 * sl[op.x >> 16] + c2 * sl[op.y & 0xffff] +
                                             n3 * x3 * (sl[op.y >> 16] - r3 * sl[op.z & 0xffff]) +
-                                            n5 * oo2ze * sl[op.z >> 16];
+                                            n5 * (real)oo2ze * sl[op.z >> 16];
                     }
                     __syncthreads();
                 }
@@ -77,14 +80,14 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 __syncthreads();
             }
         }
-        const double ab[3] = {g.ab[4 * bp], g.ab[4 * bp + 1], g.ab[4 * bp + 2]};
-        const double cd[3] = {g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2]};
+        const real ab[3] = {(real)g.ab[4 * bp], (real)g.ab[4 * bp + 1], (real)g.ab[4 * bp + 2]};
+        const real cd[3] = {(real)g.ab[4 * kp], (real)g.ab[4 * kp + 1], (real)g.ab[4 * kp + 2]};
         for (int lv = 0; lv < d.nhrr_lv; ++lv) {
             for (int o = d.hrr_lv[lv] + lane; o < d.hrr_lv[lv + 1]; o += 64) {
                 const uint4 op = g_prog_hrr[o];
                 const int typ = op.w & 1, i = (op.w >> 1) & 3;
-                const double gv = typ ? (i == 0 ? cd[0] : (i == 1 ? cd[1] : cd[2]))
-                                      : (i == 0 ? ab[0] : (i == 1 ? ab[1] : ab[2]));
+                const real gv = typ ? (i == 0 ? cd[0] : (i == 1 ? cd[1] : cd[2]))
+                                    : (i == 0 ? ab[0] : (i == 1 ? ab[1] : ab[2]));
                 sl[op.x & 0xffff] = sl[op.x >> 16] + gv * sl[op.y];
             }
             __syncthreads();
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
-                if (a == c && b == e) m = fmax(m, fabs(sl[f & 0xffff]));
+                if (a == c && b == e) m = fmax(m, fabs((double)sl[f & 0xffff]));
             }
             for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
             if (lane == 0) g.out[bp] = sqrt(m);
@@ -104,7 +107,7 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
-                cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, sl[f & 0xffff]);
+                cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, (double)sl[f & 0xffff]);
             }
         } else {
             // Gather digestion: the quartet's integrals are copied into a dense
@@ -112,24 +115,24 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             // each lane forms whole output elements of the six J/K blocks, so
             // one atomic per output element replaces six per integral.
             const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
-            double* dv = sl + d.nslot;
-            double* Dab = dv + d.nfin;
-            double* Dcd = Dab + na * nb;
-            double* Dac = Dcd + nc * nd;
-            double* Dad = Dac + na * nc;
-            double* Dbc = Dad + na * nd;
-            double* Dbd = Dbc + nb * nc;
+            real* dv = sl + d.nslot;
+            real* Dab = dv + d.nfin;
+            real* Dcd = Dab + na * nb;
+            real* Dac = Dcd + nc * nd;
+            real* Dad = Dac + na * nc;
+            real* Dbc = Dad + na * nd;
+            real* Dbd = Dbc + nb * nc;
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
                 dv[((a * nb + b) * nc + c) * nd + e] = sl[f & 0xffff];
             }
-            for (int k = lane; k < na * nb; k += 64) Dab[k] = g.D[(size_t)(fa + k / nb) * n + fb + k % nb];
-            for (int k = lane; k < nc * nd; k += 64) Dcd[k] = g.D[(size_t)(fc + k / nd) * n + fd + k % nd];
-            for (int k = lane; k < na * nc; k += 64) Dac[k] = g.D[(size_t)(fa + k / nc) * n + fc + k % nc];
-            for (int k = lane; k < na * nd; k += 64) Dad[k] = g.D[(size_t)(fa + k / nd) * n + fd + k % nd];
-            for (int k = lane; k < nb * nc; k += 64) Dbc[k] = g.D[(size_t)(fb + k / nc) * n + fc + k % nc];
-            for (int k = lane; k < nb * nd; k += 64) Dbd[k] = g.D[(size_t)(fb + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < na * nb; k += 64) Dab[k] = (real)g.D[(size_t)(fa + k / nb) * n + fb + k % nb];
+            for (int k = lane; k < nc * nd; k += 64) Dcd[k] = (real)g.D[(size_t)(fc + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < na * nc; k += 64) Dac[k] = (real)g.D[(size_t)(fa + k / nc) * n + fc + k % nc];
+            for (int k = lane; k < na * nd; k += 64) Dad[k] = (real)g.D[(size_t)(fa + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < nb * nc; k += 64) Dbc[k] = (real)g.D[(size_t)(fb + k / nc) * n + fc + k % nc];
+            for (int k = lane; k < nb * nd; k += 64) Dbd[k] = (real)g.D[(size_t)(fb + k / nd) * n + fd + k % nd];
             __syncthreads();
             const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
             double* J = g.J + off;
@@ -137,11 +140,11 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             const int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
                       o6 = o5 + nb * nd;
             for (int o = lane; o < o6; o += 64) {
-                double acc = 0.0;
+                real acc = 0;
                 double* dst;
                 if (o < o1) {                          // J[a,b] += v(ab|cd) D[c,d]
                     const int a = o / nb, b = o % nb;
-                    const double* v = dv + (a * nb + b) * nc * nd;
+                    const real* v = dv + (a * nb + b) * nc * nd;
                     for (int k = 0; k < nc * nd; ++k) acc += v[k] * Dcd[k];
                     dst = &J[(size_t)(fa + a) * n + fb + b];
                 } else if (o < o2) {                   // J[c,d] += v(ab|cd) D[a,b]
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                         for (int c = 0; c < nc; ++c) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dac[a * nc + c];
                     dst = &K[(size_t)(fb + b) * n + fd + e];
                 }
-                jk_add(dst, s * acc, g.jk_copies);
+                jk_add(dst, s * (double)acc, g.jk_copies);
             }
         }
         __syncthreads();
@@ -194,6 +197,8 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     if (mode == MODE_JK)
         hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+    else if (mode == MODE_JK32)
+        hipLaunchKernelGGL(k_eri_coop<MODE_JK32>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
     else if (mode == MODE_SCHWARZ)
         hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
     else

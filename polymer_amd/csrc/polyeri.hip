@@ -204,6 +204,8 @@ struct ClassPairWork {
     long long* d_row;
     int* d_rb;             // nrows x (nbucket+1) bucket offsets inside each row
     int* d_bstart;         // bucket starts of class Y
+    int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
+    int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int nbucket;
     int nrows;
     long long nwork;
@@ -275,13 +277,14 @@ static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b)
 
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
-                        const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0) {
+                        const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
+                        const int* rf = nullptr) {
     if (nwork <= 0) return 0;
     if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) {
         EriArgs a;
         a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
         a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
-        a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
+        a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
         a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
         a.jk_copies = c->jk_copies;
         int ns = polyeri_coop_nslot(la, lb, lc, ld);
@@ -291,13 +294,14 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
         return 0;
     }
     EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
+                : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
                 : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
                 : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
     if (!k) return fail(POLYERI_E_ARG, "no kernel for class");
     EriArgs a;
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
-    a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket;
+    a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
     a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     a.jk_copies = c->jk_copies;
     hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
@@ -326,7 +330,7 @@ static void free_ctx(polyeri_ctx* c) {
     hipSetDevice(c->device);
     hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
     for (int i = 0; i < NPAIRCLASS; ++i) { hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); }
-    for (auto& w : c->work) { hipFree(w.d_row); hipFree(w.d_rb); }
+    for (auto& w : c->work) { hipFree(w.d_row); hipFree(w.d_rb); if (w.d_rf) hipFree(w.d_rf); }
     hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
     hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
     for (auto e : c->ev) hipEventDestroy(e);
@@ -411,7 +415,8 @@ static int build_ctx(polyeri_ctx* c) {
     }
     // class-pair work lists (host.cpp build_work_lists)
     std::vector<WorkList> wl;
-    build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, c->class_ids, c->ket_ids, c->bstart, wl);
+    const double cut = c->opts.precision == POLYERI_MIXED ? c->opts.mixed_cut : 0.0;
+    build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, cut, c->class_ids, c->ket_ids, c->bstart, wl);
     for (int Y = 0; Y < NPAIRCLASS; ++Y) {
         if ((rc = dalloc(&c->d_ket_ids[Y], c->ket_ids[Y].size()))) return rc;
         if ((rc = dalloc(&c->d_bstart[Y], c->bstart[Y].size()))) return rc;
@@ -428,6 +433,8 @@ static int build_ctx(polyeri_ctx* c) {
         w.nbucket = l.nbucket;
         w.nrows = l.nrows;
         w.nwork = l.nwork;
+        w.prec = l.prec;
+        w.d_rf = nullptr;
         w.prim_quartets = l.prim_quartets;
         w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
         w.d_row = nullptr;
@@ -436,6 +443,11 @@ static int build_ctx(polyeri_ctx* c) {
         if ((rc = dalloc(&w.d_rb, l.rb.size()))) return rc;
         HIPCHK(hipMemcpy(w.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
         if (!l.rb.empty()) HIPCHK(hipMemcpy(w.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (l.prec) {
+            if ((rc = dalloc(&w.d_rf, l.rf.size()))) return rc;
+            if (!l.rf.empty())
+                HIPCHK(hipMemcpy(w.d_rf, l.rf.data(), l.rf.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
         c->quartets += w.nwork;
         c->prim_quartets += w.prim_quartets;
         c->model_flops += w.flops;
@@ -535,8 +547,8 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
-        int rc = launch_class(c, MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
-                              cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket);
+        int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
+                              cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -655,7 +667,7 @@ static int tensor_device(polyeri_ctx* c, double* d_t) {
     HIPCHK(hipGetLastError());
     for (const ClassPairWork& cw : c->work) {
         int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
-                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket);
+                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf);
         if (rc) return rc;
     }
     return 0;
@@ -701,10 +713,12 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
     st->prim_quartets = ctx->prim_quartets;
     st->model_flops = ctx->model_flops;
     st->last_jk_ms = ctx->last_jk_ms;
-    st->nclass = (int)std::min<size_t>(32, ctx->work.size());
+    st->nclass = (int)std::min<size_t>(POLYERI_MAX_LISTS, ctx->work.size());
     for (int i = 0; i < st->nclass; ++i) {
         const ClassPairWork& w = ctx->work[i];
         st->last_class_ms[i] = ctx->last_class_ms[i];
+        st->class_quartets[i] = w.nwork;
+        st->class_prec[i] = w.prec;
         st->class_prim_quartets[i] = w.prim_quartets;
         st->class_flops[i] = w.flops;
         st->class_l[i] = w.la * 1000 + w.lb * 100 + w.lc * 10 + w.ld;

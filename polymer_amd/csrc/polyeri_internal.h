@@ -61,12 +61,17 @@ constexpr int MAXL = 2;
 // class X in Schwarz order; the kets of class Y are grouped in buckets of
 // equal primitive-pair count (Schwarz order inside each bucket); row i holds,
 // bucket by bucket, the kets j with Q_i Q_j >= tol (and rank_j <= rank_i
-// when X == Y), each a prefix of its bucket.
+// when X == Y), each a prefix of its bucket.  With a mixed-precision cut
+// the class pair has two lists: prec 0 holds the kets with Q_i Q_j >= cut
+// (a prefix of each bucket), prec 1 those with tol <= Q_i Q_j < cut (a
+// range of each bucket that starts at rf[row][bucket]).
 struct WorkList {
     int X = 0, Y = 0, nrows = 0, nbucket = 1;
+    int prec = 0;                   // 0: FP64, 1: FP32
     long long nwork = 0;
     std::vector<long long> row;     // nrows + 1, exclusive prefix
     std::vector<int> rb;            // nrows x (nbucket + 1), offsets inside a row
+    std::vector<int> rf;            // prec 1: nrows x nbucket, first ket inside each bucket
     double prim_quartets = 0;
 };
 
@@ -78,9 +83,10 @@ void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pair
                  std::vector<PrimPair>& prims);
 void boys_table(std::vector<double>& tab);       // BOYS_NT x BOYS_NM
 // class_ids[X] must hold the pairs of class X in Schwarz order (bra rows);
-// fills ket_ids / bstart (bucketed ket order) and the lists.
+// fills ket_ids / bstart (bucketed ket order) and the lists.  cut > tol
+// splits every class pair into an FP64 list and an FP32 list (see WorkList).
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
-                      const std::vector<int> class_ids[], std::vector<int> ket_ids[],
+                      double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
                       std::vector<int> bstart[], std::vector<WorkList>& out);
 // S, T, V in the internal ordering (nbf x nbf, row-major, symmetric).
 void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vector<double>& S,
