@@ -174,13 +174,17 @@ def main():
     if not args.no_scf:
         print("bench: converging both SCFs", file=sys.stderr, flush=True)
         scf = {}
-        for name, c in (("fp64", ctx), ("mixed", cmx)):
+        runs = (("fp64", ctx, {}), ("mixed", cmx, {}),
+                ("mixed_incremental", cmx, {"incremental": 8, "dens_tol": 1e-11}))
+        for name, c, kw in runs:
             t0 = time.perf_counter()
-            r = c.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80)
+            r = c.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80, **kw)
             scf[name] = {"e_total": r["e_total"], "iterations": r["iterations"], "converged": bool(r["converged"]),
-                         "wall_s": time.perf_counter() - t0}
-        scf["dE_mixed_minus_fp64"] = scf["mixed"]["e_total"] - scf["fp64"]["e_total"]
-        scf["wall_speedup_mixed"] = scf["fp64"]["wall_s"] / scf["mixed"]["wall_s"]
+                         "wall_s": time.perf_counter() - t0, "jk_s": r["jk_seconds"], "quartets": r["quartets"],
+                         **kw}
+        for name in ("mixed", "mixed_incremental"):
+            scf[name]["dE_vs_fp64"] = scf[name]["e_total"] - scf["fp64"]["e_total"]
+            scf[name]["wall_speedup_vs_fp64"] = scf["fp64"]["wall_s"] / scf[name]["wall_s"]
     out = {
         "metric": METRIC,
         "value": value,

@@ -73,6 +73,14 @@ int polyeri_jk_build(polyeri_ctx *ctx, const double *p_c, double *c_c, double *x
 /* Same on device pointers (ncgs x ncgs each), enqueued on the context's
  * stream; returns after the kernels are enqueued.  polyeri_sync waits. */
 int polyeri_jk_build_device(polyeri_ctx *ctx, const double *d_p_c, double *d_c_c, double *d_x_c);
+
+/* polyeri_jk_build with density screening (no counterpart in the reference,
+ * whose screen is IJIJ*KLKL < INTTOL only, src/pfourindexdirect.f90:529):
+ * quartets whose Schwarz bound times the largest |P| over the six density
+ * blocks they read is below dens_tol are skipped.  Used with density
+ * differences in incremental Fock builds; dens_tol = 0 is polyeri_jk_build. */
+int polyeri_jk_build_screened(polyeri_ctx *ctx, const double *p_c, double dens_tol, double *c_c,
+                              double *x_c);
 int polyeri_sync(polyeri_ctx *ctx);
 
 /* Device memory for callers without their own allocator (e.g. a Fortran
@@ -114,6 +122,14 @@ typedef struct polyeri_scf_opts {
     int charge;         /* IOPTN(5)                                         */
     int spherical;      /* LOPTN(69)                                        */
     int guess;          /* 0: GUESS_DENSITYMATRIX; 1: caller's P_C          */
+    /* Incremental Fock builds (no counterpart in the reference): with
+     * incremental = n > 0, J/K of iteration i are J/K of the previous
+     * iteration plus J/K of the density difference, with a full build every
+     * n iterations; 0 = full build every iteration (the reference).  dens_tol
+     * > 0 drops quartets whose Schwarz bound times the largest |density
+     * (difference)| they touch is below dens_tol; 0 = no density screening. */
+    int incremental;
+    double dens_tol;
 } polyeri_scf_opts;
 
 typedef struct polyeri_scf_result {
@@ -122,6 +138,7 @@ typedef struct polyeri_scf_result {
     double jk_seconds;   /* total J/K build time (device events)            */
     double fock_seconds; /* total J/K + Fock assembly time                  */
     double scf_seconds;  /* wall time of the whole loop                     */
+    long long quartets;  /* shell quartets evaluated over all J/K builds    */
 } polyeri_scf_result;
 
 /* SELF_CONSISTENT_FIELD (src/pselfconsistentfield.f90:1-162) with FOCK_BUILD
@@ -150,6 +167,7 @@ typedef struct polyeri_stats {
     double prim_quartets;      /* their primitive quartets                   */
     double model_flops;        /* algorithmic FLOPs of one J/K build         */
     double last_jk_ms;         /* device time of the last J/K build          */
+    long long last_quartets;   /* quartets the last build evaluated (after density screening) */
     double last_class_ms[POLYERI_MAX_LISTS];  /* per list kernel time of the last build */
     double class_prim_quartets[POLYERI_MAX_LISTS];
     double class_flops[POLYERI_MAX_LISTS];

@@ -24,12 +24,15 @@ MODULE POLYERI
    TYPE, BIND(C) :: POLYERI_SCF_OPTS
       REAL(C_DOUBLE) :: SCFCONV, SCFRELAX, HFEXCHANGE, LINDEP, TEMPHF
       INTEGER(C_INT) :: SCFCYCLES, DIIS, CHARGE, SPHERICAL, GUESS
+      INTEGER(C_INT) :: INCREMENTAL
+      REAL(C_DOUBLE) :: DENS_TOL
    END TYPE
 
    TYPE, BIND(C) :: POLYERI_SCF_RESULT
       REAL(C_DOUBLE) :: E_TOTAL, E_NUC, E_KIN, E_NUC_ATT, E_COUL, E_HFX, FERMI
       INTEGER(C_INT) :: ITERATIONS, CONVERGED, N_INDEP, IOCC
       REAL(C_DOUBLE) :: JK_SECONDS, FOCK_SECONDS, SCF_SECONDS
+      INTEGER(C_LONG_LONG) :: QUARTETS
    END TYPE
 
    INTERFACE
@@ -64,6 +67,14 @@ MODULE POLYERI
          IMPORT :: C_INT, C_DOUBLE, C_PTR
          TYPE(C_PTR), VALUE :: CTX
          REAL(C_DOUBLE) :: P_C(*), C_C(*), X_C(*)
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_JK_BUILD_SCREENED(CTX, P_C, DENS_TOL, C_C, X_C) &
+            BIND(C, NAME="polyeri_jk_build_screened")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         REAL(C_DOUBLE) :: P_C(*), C_C(*), X_C(*)
+         REAL(C_DOUBLE), VALUE :: DENS_TOL
       END FUNCTION
 
       INTEGER(C_INT) FUNCTION POLYERI_ONE_ELECTRON(CTX, S_C, T_C, N_C, E_NUC) &

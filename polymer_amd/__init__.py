@@ -107,11 +107,16 @@ class Context:
     def _mat(self):
         return np.zeros((self.ncgs, self.ncgs), order="F")
 
-    def fourindex_eri(self, p_c):
-        """J (C_C) and K (X_C) from the contracted density P_C."""
+    def fourindex_eri(self, p_c, dens_tol=0.0):
+        """J (C_C) and K (X_C) from the contracted density P_C; dens_tol > 0
+        skips quartets whose Schwarz bound times the largest |P| they read
+        is below dens_tol."""
         p = np.asfortranarray(p_c, dtype=np.float64)
         j, k = self._mat(), self._mat()
-        check(lib().polyeri_jk_build(self.h, _p(p), _p(j), _p(k)))
+        if dens_tol > 0.0:
+            check(lib().polyeri_jk_build_screened(self.h, _p(p), float(dens_tol), _p(j), _p(k)))
+        else:
+            check(lib().polyeri_jk_build(self.h, _p(p), _p(j), _p(k)))
         return j, k
 
     def one_electron(self):
@@ -128,10 +133,13 @@ class Context:
 
     def self_consistent_field(self, scfconv=1e-8, scfrelax=1.0, hfexchange=1.0, lindep=1e-6,
                               temphf=0.0, scfcycles=50, diis=5, charge=0, spherical=False,
-                              guess=None):
-        """Defaults are the reference's keyword defaults (src/pread.f90:65-172)."""
+                              guess=None, incremental=0, dens_tol=0.0):
+        """Defaults are the reference's keyword defaults (src/pread.f90:65-172).
+        incremental = n > 0 builds J/K from the density difference with a
+        full build every n iterations; dens_tol > 0 screens quartets by their
+        Schwarz bound times the density (difference) they touch."""
         o = ScfOpts(scfconv, scfrelax, hfexchange, lindep, temphf, scfcycles, diis, charge,
-                    int(spherical), 0 if guess is None else 1)
+                    int(spherical), 0 if guess is None else 1, int(incremental), float(dens_tol))
         p = self._mat() if guess is None else np.asfortranarray(guess, dtype=np.float64).copy()
         eps = np.zeros(self.ncgs)
         co = self._mat()

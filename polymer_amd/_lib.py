@@ -16,7 +16,7 @@ EXPORTS = [
     "polyeri_jk_build_device", "polyeri_sync", "polyeri_one_electron", "polyeri_eri_tensor",
     "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
-    "polyeri_fock_build_device", "polyeri_device_copy",
+    "polyeri_fock_build_device", "polyeri_device_copy", "polyeri_jk_build_screened",
 ]
 
 
@@ -37,7 +37,8 @@ class ScfOpts(ctypes.Structure):
                 ("hfexchange", ctypes.c_double), ("lindep", ctypes.c_double),
                 ("temphf", ctypes.c_double), ("scfcycles", ctypes.c_int),
                 ("diis", ctypes.c_int), ("charge", ctypes.c_int),
-                ("spherical", ctypes.c_int), ("guess", ctypes.c_int)]
+                ("spherical", ctypes.c_int), ("guess", ctypes.c_int),
+                ("incremental", ctypes.c_int), ("dens_tol", ctypes.c_double)]
 
 
 class ScfResult(ctypes.Structure):
@@ -47,7 +48,8 @@ class ScfResult(ctypes.Structure):
                 ("fermi", ctypes.c_double), ("iterations", ctypes.c_int),
                 ("converged", ctypes.c_int), ("n_indep", ctypes.c_int),
                 ("iocc", ctypes.c_int), ("jk_seconds", ctypes.c_double),
-                ("fock_seconds", ctypes.c_double), ("scf_seconds", ctypes.c_double)]
+                ("fock_seconds", ctypes.c_double), ("scf_seconds", ctypes.c_double),
+                ("quartets", ctypes.c_longlong)]
 
 
 MAX_LISTS = 64   # POLYERI_MAX_LISTS
@@ -57,7 +59,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("nshell", ctypes.c_int), ("nbf", ctypes.c_int), ("npair", ctypes.c_int),
                 ("nprim_pair", ctypes.c_int), ("quartets", ctypes.c_longlong),
                 ("prim_quartets", ctypes.c_double), ("model_flops", ctypes.c_double),
-                ("last_jk_ms", ctypes.c_double), ("last_class_ms", ctypes.c_double * MAX_LISTS),
+                ("last_jk_ms", ctypes.c_double), ("last_quartets", ctypes.c_longlong),
+                ("last_class_ms", ctypes.c_double * MAX_LISTS),
                 ("class_prim_quartets", ctypes.c_double * MAX_LISTS),
                 ("class_flops", ctypes.c_double * MAX_LISTS),
                 ("class_quartets", ctypes.c_longlong * MAX_LISTS),
@@ -81,6 +84,7 @@ def lib():
     L.polyeri_ctx_destroy.argtypes = [p]
     L.polyeri_ctx_destroy.restype = None
     L.polyeri_jk_build.argtypes = [p, p, p, p]
+    L.polyeri_jk_build_screened.argtypes = [p, p, ctypes.c_double, p, p]
     L.polyeri_jk_build_device.argtypes = [p, p, p, p]
     L.polyeri_sync.argtypes = [p]
     L.polyeri_one_electron.argtypes = [p, p, p, p, p]

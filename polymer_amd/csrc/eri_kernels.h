@@ -40,7 +40,15 @@ struct EriArgs {
     int nbf;
     double* out;               // tensor (MODE_TENSOR) or Schwarz (MODE_SCHWARZ)
     int jk_copies;             // 1: one J/K (device-scope atomics); 8: one per XCD
+    const int2* items;         // explicit (bra, ket) work items (density-screened list), or null
+    const unsigned* count;     // number of items (device), with items
 };
+
+// Number of work items of a launch: the device count of a screened list, or
+// the static list size.
+__device__ __forceinline__ long long work_count(const EriArgs& g) {
+    return g.items ? (long long)*g.count : g.nwork;
+}
 
 // J/K accumulation.  With one copy per XCD (jk_copies == 8) a block adds into
 // the copy of the XCD it runs on (hardware XCC_ID register, masked to 0..7)
@@ -190,12 +198,13 @@ __device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& 
 }
 
 template <int LA, int LB, int LC, int LD, int MODE>
-__global__ void __launch_bounds__(256) k_eri(EriArgs g) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.nwork) return;
+__device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
     int bp, kp;
     if (MODE == MODE_SCHWARZ) {
         bp = kp = g.bra_ids[t];
+    } else if (g.items) {
+        const int2 it = g.items[t];
+        bp = it.x; kp = it.y;
     } else {
         decode_work(g, t, bp, kp);
     }
@@ -237,7 +246,24 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
     }
 }
 
+// One thread per work item; grid-stride, so a density-screened list (item
+// count known only on the device) runs with a fixed grid.
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ void __launch_bounds__(256) k_eri(EriArgs g) {
+    const long long nw = work_count(g);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nw;
+         t += (long long)gridDim.x * blockDim.x)
+        eri_item<LA, LB, LC, LD, MODE>(g, t);
+}
+
 typedef void (*EriKernel)(EriArgs);
+// Density screening (kscreen.hip): shell-block maxima of |D|, and the
+// compaction of one work list to the items whose bound
+// Q_ab Q_cd max(|D| over the six blocks the quartet reads) reaches tol.
+int polyeri_shell_dmax(const double* D, int nbf, const int* shell_f0, const int* shell_nf, int nsh,
+                       double* dsh, hipStream_t stream);
+int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shells, const double* dsh,
+                        int nsh, double tol, int2* items, unsigned* count, hipStream_t stream);
 // Cooperative (one wavefront per quartet, LDS-resident recursion) kernels.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld);
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,

@@ -35,10 +35,14 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
     real* sl = reinterpret_cast<real*>(sl_raw);
     const ProgDesc& d = g_prog_desc[prog];
     const int lane = threadIdx.x;
-    for (long long t = blockIdx.x; t < g.nwork; t += gridDim.x) {
+    const long long nw = work_count(g);
+    for (long long t = blockIdx.x; t < nw; t += gridDim.x) {
         int bp, kp;
         if (MODE == MODE_SCHWARZ) {
             bp = kp = g.bra_ids[t];
+        } else if (g.items) {
+            const int2 it = g.items[t];
+            bp = it.x; kp = it.y;
         } else {
             decode_work(g, t, bp, kp);
         }

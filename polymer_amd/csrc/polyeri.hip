@@ -59,6 +59,13 @@ __global__ void k_fill(double* p, size_t n, double v) {
     if (i < n) p[i] = v;
 }
 
+// out = x + a y
+__global__ void k_axpby(const double* __restrict__ x, const double* __restrict__ y, double a, size_t n,
+                        double* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[i] + a * y[i];
+}
+
 // D_int[mu][nu] = s_mu s_nu P_C(ref mu, ref nu)
 __global__ void k_pack_density(const double* __restrict__ pc, int nc, const int* __restrict__ ref,
                                const double* __restrict__ sc, int nbf, double* __restrict__ D) {
@@ -205,6 +212,7 @@ struct ClassPairWork {
     int* d_rb;             // nrows x (nbucket+1) bucket offsets inside each row
     int* d_bstart;         // bucket starts of class Y
     int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
+    int2* d_items;         // density-screened items (capacity nwork), allocated on first use
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int nbucket;
     int nrows;
@@ -253,6 +261,15 @@ struct polyeri_ctx {
     rocblas_handle blas = nullptr;
     double *d_T = nullptr, *d_V = nullptr;   // T_C, N_C on the device (Fock assembly)
     int jk_copies = 8;                       // J/K accumulation copies (one per XCD)
+    // density screening: shells of every pair, shell extents, shell-block
+    // maxima of |D|, compacted item lists (allocated on first use) and counts
+    int2* d_psh = nullptr;
+    int *d_sh_f0 = nullptr, *d_sh_nf = nullptr;
+    double* d_dsh = nullptr;
+    unsigned* d_counts = nullptr;
+    std::vector<unsigned> h_counts;
+    bool screened_last = false;
+    long long last_quartets = 0;
 };
 
 static int ensure_onee_device(polyeri_ctx* c) {
@@ -278,13 +295,14 @@ static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b)
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
                         const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
-                        const int* rf = nullptr) {
+                        const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr) {
     if (nwork <= 0) return 0;
     if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) {
         EriArgs a;
         a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
         a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
         a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
+        a.items = items; a.count = count;
         a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
         a.jk_copies = c->jk_copies;
         int ns = polyeri_coop_nslot(la, lb, lc, ld);
@@ -302,9 +320,13 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
+    a.items = items; a.count = count;
     a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     a.jk_copies = c->jk_copies;
-    hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
+    // a screened list holds at most nwork items; its grid strides over them
+    unsigned grid = nblk((size_t)nwork);
+    if (items && grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -328,16 +350,21 @@ static hipStream_t shared_stream(int device, int* rc) {
 static void free_ctx(polyeri_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
     hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
     for (int i = 0; i < NPAIRCLASS; ++i) { hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); }
-    for (auto& w : c->work) { hipFree(w.d_row); hipFree(w.d_rb); if (w.d_rf) hipFree(w.d_rf); }
+    for (auto& w : c->work) {
+        hipFree(w.d_row); hipFree(w.d_rb);
+        if (w.d_rf) hipFree(w.d_rf);
+        if (w.d_items) hipFree(w.d_items);
+    }
+    hipFree(c->d_psh); hipFree(c->d_sh_f0); hipFree(c->d_sh_nf); hipFree(c->d_dsh); hipFree(c->d_counts);
     hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
     hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
     for (auto e : c->ev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->blas) rocblas_destroy_handle(c->blas);
-    if (c->stream) hipStreamSynchronize(c->stream);
     delete c;
 }
 
@@ -435,6 +462,7 @@ static int build_ctx(polyeri_ctx* c) {
         w.nwork = l.nwork;
         w.prec = l.prec;
         w.d_rf = nullptr;
+        w.d_items = nullptr;
         w.prim_quartets = l.prim_quartets;
         w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
         w.d_row = nullptr;
@@ -452,6 +480,21 @@ static int build_ctx(polyeri_ctx* c) {
         c->prim_quartets += w.prim_quartets;
         c->model_flops += w.flops;
         c->work.push_back(w);
+    }
+    // density-screening tables
+    {
+        const int nsh = (int)c->ib.shells.size();
+        std::vector<int2> psh(np);
+        for (int i = 0; i < np; ++i) psh[i] = make_int2(c->pairs[i].a, c->pairs[i].b);
+        std::vector<int> f0(nsh), nf(nsh);
+        for (int i = 0; i < nsh; ++i) { f0[i] = c->ib.shells[i].f0; nf[i] = ncart(c->ib.shells[i].L); }
+        if ((rc = dalloc(&c->d_psh, np)) || (rc = dalloc(&c->d_sh_f0, nsh)) || (rc = dalloc(&c->d_sh_nf, nsh)) ||
+            (rc = dalloc(&c->d_dsh, (size_t)nsh * nsh)) || (rc = dalloc(&c->d_counts, c->work.size())))
+            return rc;
+        HIPCHK(hipMemcpy(c->d_psh, psh.data(), np * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_sh_f0, f0.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_sh_nf, nf.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+        c->h_counts.assign(c->work.size(), 0);
     }
     c->ev.resize(c->work.size() + 1);
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -535,20 +578,53 @@ int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
 }
 
 // J/K from the context's device density (reference layout, ncgs^2).
-static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc) {
+__global__ void k_fill_u32(unsigned* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// J/K of the contracted density d_pc into d_jc / d_kc (reference layout).
+// dens_tol > 0 drops the quartets whose density-weighted Schwarz bound is
+// below dens_tol (kscreen.hip); the surviving items of every list are
+// compacted on the device and the class kernels stride over them.
+static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol = 0.0) {
     int nbf = c->ib.nbf, nc = c->rb.ncgs;
     size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
+    const bool screen = dens_tol > 0.0;
+    if (screen)
+        for (ClassPairWork& w : c->work)
+            if (!w.d_items && w.nwork > 0) {
+                int rc = dalloc(&w.d_items, (size_t)w.nwork);
+                if (rc) return rc;
+            }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_pack_density, dim3(nblk(n2)), dim3(256), 0, c->stream, d_pc, nc, c->d_ref,
                        c->d_scale, nbf, c->d_D);
     hipLaunchKernelGGL(k_fill, dim3(nblk(n2 * c->jk_copies)), dim3(256), 0, c->stream, c->d_J, n2 * c->jk_copies, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nblk(n2 * c->jk_copies)), dim3(256), 0, c->stream, c->d_K, n2 * c->jk_copies, 0.0);
     HIPCHK(hipGetLastError());
+    if (screen) {
+        const int nsh = (int)c->ib.shells.size();
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(c->work.size())), dim3(256), 0, c->stream, c->d_counts, c->work.size());
+        if (polyeri_shell_dmax(c->d_D, nbf, c->d_sh_f0, c->d_sh_nf, nsh, c->d_dsh, c->stream))
+            return fail(POLYERI_E_HIP, "shell density maxima launch failed");
+        for (size_t w = 0; w < c->work.size(); ++w) {
+            const ClassPairWork& cw = c->work[w];
+            if (cw.nwork <= 0) continue;
+            EriArgs a{};
+            a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = cw.nwork;
+            a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket; a.rb_first = cw.d_rf;
+            if (polyeri_screen_list(a, c->d_Q, c->d_psh, c->d_dsh, nsh, dens_tol, cw.d_items, c->d_counts + w,
+                                    c->stream))
+                return fail(POLYERI_E_HIP, "screening launch failed");
+        }
+    }
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
-                              cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf);
+                              cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
+                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -557,6 +633,10 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     hipLaunchKernelGGL(k_unpack_jk, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, c->d_K, nbf, c->jk_copies,
                        c->d_ref, c->d_scale, nc, d_jc, d_kc);
     HIPCHK(hipGetLastError());
+    if (screen && !c->work.empty())
+        HIPCHK(hipMemcpyAsync(c->h_counts.data(), c->d_counts, c->work.size() * sizeof(unsigned),
+                              hipMemcpyDeviceToHost, c->stream));
+    c->screened_last = screen;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return 0;
 }
@@ -566,6 +646,9 @@ static int jk_collect_timing(polyeri_ctx* c) {
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_jk_ms = ms;
+    c->last_quartets = 0;
+    for (size_t w = 0; w < c->work.size(); ++w)
+        c->last_quartets += c->screened_last ? (long long)c->h_counts[w] : c->work[w].nwork;
     if (c->profiling)
         for (size_t w = 0; w < c->work.size(); ++w) {
             float m = 0;
@@ -587,12 +670,11 @@ int polyeri_sync(polyeri_ctx* ctx) {
     return jk_collect_timing(ctx);
 }
 
-int polyeri_jk_build(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
-    if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build: null argument");
+static int jk_build_host(polyeri_ctx* ctx, const double* p_c, double dens_tol, double* c_c, double* x_c) {
     HIPCHK(hipSetDevice(ctx->device));
     size_t c2 = (size_t)ctx->rb.ncgs * ctx->rb.ncgs;
     HIPCHK(hipMemcpyAsync(ctx->d_pc, p_c, c2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    int rc = jk_device(ctx, ctx->d_pc, ctx->d_jc, ctx->d_kc);
+    int rc = jk_device(ctx, ctx->d_pc, ctx->d_jc, ctx->d_kc, dens_tol);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(c_c, ctx->d_jc, c2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(x_c, ctx->d_kc, c2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -601,6 +683,17 @@ int polyeri_jk_build(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x
     for (size_t i = 0; i < c2; ++i)
         if (!std::isfinite(c_c[i]) || !std::isfinite(x_c[i])) return fail(POLYERI_E_NONFINITE, "non-finite J/K");
     return 0;
+}
+
+int polyeri_jk_build(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
+    if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build: null argument");
+    return jk_build_host(ctx, p_c, 0.0, c_c, x_c);
+}
+
+int polyeri_jk_build_screened(polyeri_ctx* ctx, const double* p_c, double dens_tol, double* c_c, double* x_c) {
+    if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build_screened: null argument");
+    if (!(dens_tol >= 0.0)) return fail(POLYERI_E_ARG, "polyeri_jk_build_screened: dens_tol < 0");
+    return jk_build_host(ctx, p_c, dens_tol, c_c, x_c);
 }
 
 int polyeri_device_alloc(polyeri_ctx* ctx, size_t bytes, void** ptr) {
@@ -713,6 +806,7 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
     st->prim_quartets = ctx->prim_quartets;
     st->model_flops = ctx->model_flops;
     st->last_jk_ms = ctx->last_jk_ms;
+    st->last_quartets = ctx->last_quartets;
     st->nclass = (int)std::min<size_t>(POLYERI_MAX_LISTS, ctx->work.size());
     for (int i = 0; i < st->nclass; ++i) {
         const ClassPairWork& w = ctx->work[i];
@@ -850,6 +944,14 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         (rc = buf.get(&dE, nc)) || (rc = buf.get(&dWk, nc)) || (rc = buf.get(&dCO, n2)) || (rc = buf.get(&dP, n2)) ||
         (rc = buf.get(&dPout, n2)) || (rc = buf.get(&dOcc, nc)) || (rc = buf.get(&dFp, n2)) || (rc = buf.get(&dInfo, 1)))
         return rc;
+    // incremental builds: density / J / K of the last build, the difference
+    double *dPprev = nullptr, *dJprev = nullptr, *dKprev = nullptr, *dDel = nullptr, *dJd = nullptr, *dKd = nullptr;
+    if (o->incremental > 0)
+        if ((rc = buf.get(&dPprev, n2)) || (rc = buf.get(&dJprev, n2)) || (rc = buf.get(&dKprev, n2)) ||
+            (rc = buf.get(&dDel, n2)) || (rc = buf.get(&dJd, n2)) || (rc = buf.get(&dKd, n2)))
+            return rc;
+    const double dens_tol = o->dens_tol > 0 ? o->dens_tol : 0.0;
+    int since_full = 0;
     const int HMAX = 9;   // DIIS uses at most 8 previous vectors
     double *dHP, *dHD;
     if ((rc = buf.get(&dHP, n2 * HMAX)) || (rc = buf.get(&dHD, n2 * HMAX))) return rc;
@@ -912,12 +1014,29 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     double etot = 0, ek = 0, ev = 0, ej = 0, ex = 0, fermi = 0;
     int converged = 0, iscf;
     double jk_total = 0, fock_total = 0;
+    long long build_quartets = 0;
     hipEvent_t ef0, ef1;
     HIPCHK(hipEventCreate(&ef0));
     HIPCHK(hipEventCreate(&ef1));
     for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
         HIPCHK(hipEventRecord(ef0, s));
-        if ((rc = jk_device(ctx, dP, ctx->d_jc, ctx->d_kc))) break;
+        if (o->incremental > 0 && iscf > 1 && since_full < o->incremental) {
+            // J/K(P) = J/K(P_prev) + J/K(P - P_prev)
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(n2)), dim3(256), 0, s, dP, dPprev, -1.0, n2, dDel);
+            if ((rc = jk_device(ctx, dDel, dJd, dKd, dens_tol))) break;
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(n2)), dim3(256), 0, s, dJprev, dJd, 1.0, n2, ctx->d_jc);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(n2)), dim3(256), 0, s, dKprev, dKd, 1.0, n2, ctx->d_kc);
+            ++since_full;
+        } else {
+            if ((rc = jk_device(ctx, dP, ctx->d_jc, ctx->d_kc, dens_tol))) break;
+            since_full = 1;
+        }
+        if (o->incremental > 0) {
+            HIPCHK(hipMemcpyAsync(dPprev, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(dJprev, ctx->d_jc, n2 * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(dKprev, ctx->d_kc, n2 * 8, hipMemcpyDeviceToDevice, s));
+        }
+
         hipLaunchKernelGGL(k_fock, dim3(nblk(n2)), dim3(256), 0, s, dT, dV, ctx->d_jc, ctx->d_kc, o->hfexchange, nc, dF);
         HIPCHK(hipEventRecord(ef1, s));
         // C2S F C2S^T, X^T (.) X
@@ -940,7 +1059,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             HIPCHK(hipEventElapsedTime(&b, ctx->ev0, ctx->ev1));
             fock_total += a * 1e-3;
             jk_total += b * 1e-3;
-            ctx->last_jk_ms = b;
+            if ((rc = jk_collect_timing(ctx))) break;
+            build_quartets += ctx->last_quartets;
         }
         for (int j = nind; j < nc; ++j) eps[j] = 1.0e99;
         // Fermi level by bisection (src/pfock.f90:339-423)
@@ -1032,6 +1152,7 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     res->e_coul = ej; res->e_hfx = ex; res->fermi = fermi;
     res->iterations = iscf; res->converged = converged; res->n_indep = nind; res->iocc = iocc;
     res->jk_seconds = jk_total;
+    res->quartets = build_quartets;
     res->fock_seconds = fock_total;
     res->scf_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     return converged ? 0 : fail(POLYERI_E_NOTCONV, "SCF CONVERGENCE NOT MET");
