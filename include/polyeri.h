@@ -174,12 +174,27 @@ typedef struct polyeri_stats {
     long long class_quartets[POLYERI_MAX_LISTS];
     int class_l[POLYERI_MAX_LISTS];     /* la*1000+lb*100+lc*10+ld           */
     int class_prec[POLYERI_MAX_LISTS];  /* 0: FP64, 1: FP32                  */
+    int class_kind[POLYERI_MAX_LISTS];  /* 1: thread per quartet, 2: wavefront per quartet */
+    int class_block[POLYERI_MAX_LISTS]; /* threads per block                 */
+    int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass           */
     int nclass;
 } polyeri_stats;
 int polyeri_get_stats(polyeri_ctx *ctx, polyeri_stats *st);
 
-/* Per-class kernel timing on/off (adds events between launches). */
+/* Per-class kernel timing on/off (adds events between launches; J/K builds
+ * then run eagerly instead of from a captured HIP graph). */
 int polyeri_set_profiling(polyeri_ctx *ctx, int on);
+
+/* Graph replay of J/K builds on (default) or off (every build launched
+ * eagerly).  No counterpart in the reference. */
+int polyeri_set_graph(polyeri_ctx *ctx, int on);
+
+/* Autotuner (no counterpart in the reference): times every work list with
+ * each candidate launch shape (thread-per-quartet kernel with 256/128/64
+ * threads per block, one pass or a persistent grid; wavefront-per-quartet
+ * kernel with three grid sizes) reps times and keeps the fastest.  Results
+ * are unchanged; later builds use the chosen shapes (polyeri_stats). */
+int polyeri_autotune(polyeri_ctx *ctx, int reps);
 
 const char *polyeri_last_error(void);
 

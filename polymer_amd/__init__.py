@@ -191,15 +191,27 @@ class Context:
     def set_profiling(self, on=True):
         check(lib().polyeri_set_profiling(self.h, int(on)))
 
+    def set_graph(self, on):
+        """J/K builds replayed from captured HIP graphs (default) or launched
+        eagerly."""
+        check(lib().polyeri_set_graph(self.h, int(bool(on))))
+
+    def autotune(self, reps=2):
+        """Time every work list with each candidate launch shape and keep the
+        fastest (results do not change)."""
+        check(lib().polyeri_autotune(self.h, int(reps)))
+
     def stats(self):
         st = Stats()
         check(lib().polyeri_get_stats(self.h, ctypes.byref(st)))
         d = {f: getattr(st, f) for f, _ in Stats._fields_
              if f not in ("last_class_ms", "class_prim_quartets", "class_flops", "class_l",
-                          "class_quartets", "class_prec")}
+                          "class_quartets", "class_prec", "class_kind", "class_block", "class_grid")}
         n = st.nclass
         d["classes"] = [dict(cls=f"{st.class_l[i]:04d}", ms=st.last_class_ms[i],
                              prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i],
-                             quartets=st.class_quartets[i], prec="fp32" if st.class_prec[i] else "fp64")
+                             quartets=st.class_quartets[i], prec="fp32" if st.class_prec[i] else "fp64",
+                             kernel="wave" if st.class_kind[i] == 2 else "thread", block=st.class_block[i],
+                             grid=st.class_grid[i])
                         for i in range(n)]
         return d

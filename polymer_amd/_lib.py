@@ -17,6 +17,7 @@ EXPORTS = [
     "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
     "polyeri_fock_build_device", "polyeri_device_copy", "polyeri_jk_build_screened",
+    "polyeri_autotune", "polyeri_set_graph",
 ]
 
 
@@ -65,7 +66,8 @@ class Stats(ctypes.Structure):
                 ("class_flops", ctypes.c_double * MAX_LISTS),
                 ("class_quartets", ctypes.c_longlong * MAX_LISTS),
                 ("class_l", ctypes.c_int * MAX_LISTS), ("class_prec", ctypes.c_int * MAX_LISTS),
-                ("nclass", ctypes.c_int)]
+                ("class_kind", ctypes.c_int * MAX_LISTS), ("class_block", ctypes.c_int * MAX_LISTS),
+                ("class_grid", ctypes.c_int * MAX_LISTS), ("nclass", ctypes.c_int)]
 
 
 _LIB = None
@@ -85,6 +87,8 @@ def lib():
     L.polyeri_ctx_destroy.restype = None
     L.polyeri_jk_build.argtypes = [p, p, p, p]
     L.polyeri_jk_build_screened.argtypes = [p, p, ctypes.c_double, p, p]
+    L.polyeri_autotune.argtypes = [p, ctypes.c_int]
+    L.polyeri_set_graph.argtypes = [p, ctypes.c_int]
     L.polyeri_jk_build_device.argtypes = [p, p, p, p]
     L.polyeri_sync.argtypes = [p]
     L.polyeri_one_electron.argtypes = [p, p, p, p, p]

@@ -266,7 +266,7 @@ int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shel
                         int nsh, double tol, int2* items, unsigned* count, hipStream_t stream);
 // Cooperative (one wavefront per quartet, LDS-resident recursion) kernels.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld);
-int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
                         hipStream_t stream);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108

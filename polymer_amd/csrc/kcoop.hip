@@ -192,11 +192,11 @@ int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
     return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
 }
 
-int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot,
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
                         hipStream_t stream) {
     const int p = prog_index(la, lb, lc, ld);
-    if (p < 0 || a.nwork <= 0) return p < 0 ? -1 : 0;
-    const long long grid = a.nwork < 4096 ? a.nwork : 4096;
+    if (p < 0 || a.nwork <= 0 || max_grid <= 0) return p < 0 || max_grid <= 0 ? -1 : 0;
+    const long long grid = a.nwork < max_grid ? a.nwork : max_grid;
     const size_t lds = (size_t)nslot * sizeof(double);
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     if (mode == MODE_JK)

@@ -203,6 +203,16 @@ static double flop_model(int la, int lb, int lc, int ld) {
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
+// How one work list is launched.  kind: LAUNCH_AUTO picks the register
+// kernel for classes of at most POLYERI_COOP_MIN_INTEGRALS integrals and the
+// cooperative kernel above; block: threads per block of the register kernel;
+// grid: block cap (0 = one block per 256 items for the register kernel,
+// 4096 for the cooperative one).  The autotuner fills these per list.
+enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2 };
+struct LaunchCfg {
+    int kind = LAUNCH_AUTO, block = 256, grid = 0;
+};
+
 struct ClassPairWork {
     int X, Y;
     int la, lb, lc, ld;
@@ -213,6 +223,7 @@ struct ClassPairWork {
     int* d_bstart;         // bucket starts of class Y
     int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
+    LaunchCfg cfg;         // launch shape (autotuned or default)
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int nbucket;
     int nrows;
@@ -270,7 +281,23 @@ struct polyeri_ctx {
     std::vector<unsigned> h_counts;
     bool screened_last = false;
     long long last_quartets = 0;
+    // J/K builds replayed from captured HIP graphs, one per (density, J, K,
+    // screening) combination; dropped when the launch shapes change
+    struct GraphEntry {
+        const double* pc;
+        double *jc, *kc;
+        double tol;
+        hipGraphExec_t exec;
+    };
+    bool use_graph = true;
+    std::vector<GraphEntry> graphs;
+    bool tuned = false;
 };
+
+static void drop_graphs(polyeri_ctx* c) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
 
 static int ensure_onee_device(polyeri_ctx* c) {
     if (c->d_T) return 0;
@@ -295,38 +322,40 @@ static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b)
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
                         const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
-                        const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr) {
+                        const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr,
+                        LaunchCfg cfg = LaunchCfg()) {
     if (nwork <= 0) return 0;
-    if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) {
-        EriArgs a;
-        a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
-        a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
-        a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
-        a.items = items; a.count = count;
-        a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
-        a.jk_copies = c->jk_copies;
-        int ns = polyeri_coop_nslot(la, lb, lc, ld);
-        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
-        int r = polyeri_coop_launch(a, mode, la, lb, lc, ld, ns, c->stream);
-        if (r) return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
-        return 0;
-    }
-    EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
-                : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
-                : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
-                : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
-    if (!k) return fail(POLYERI_E_ARG, "no kernel for class");
-    EriArgs a;
+    EriArgs a{};
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
     a.items = items; a.count = count;
     a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     a.jk_copies = c->jk_copies;
-    // a screened list holds at most nwork items; its grid strides over them
-    unsigned grid = nblk((size_t)nwork);
-    if (items && grid > 8192) grid = 8192;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, a);
+    EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
+                : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
+                : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
+                : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
+    int kind = cfg.kind;
+    if (kind == LAUNCH_AUTO)
+        kind = ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS || !k
+                   ? LAUNCH_COOP : LAUNCH_REG;
+    if (kind == LAUNCH_REG && !k) return fail(POLYERI_E_ARG, "no register kernel for class");
+    if (kind == LAUNCH_COOP) {
+        int ns = polyeri_coop_nslot(la, lb, lc, ld);
+        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
+        const int grid = cfg.grid > 0 ? cfg.grid : 4096;
+        if (polyeri_coop_launch(a, mode, la, lb, lc, ld, ns, grid, c->stream))
+            return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
+        return 0;
+    }
+    const int block = cfg.block == 64 || cfg.block == 128 ? cfg.block : 256;
+    // one pass over the list, or a fixed (persistent) grid that strides; a
+    // screened list holds at most nwork items and always strides
+    unsigned grid = nblk((size_t)nwork, block);
+    unsigned cap = cfg.grid > 0 ? (unsigned)cfg.grid : (items ? 8192u * (256 / block) : 0u);
+    if (cap && grid > cap) grid = cap;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -365,6 +394,7 @@ static void free_ctx(polyeri_ctx* c) {
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->blas) rocblas_destroy_handle(c->blas);
+    drop_graphs(c);
     delete c;
 }
 
@@ -402,6 +432,8 @@ static int build_ctx(polyeri_ctx* c) {
     {
         const char* e = getenv("POLYERI_JK_COPIES");
         c->jk_copies = e ? atoi(e) : 8;
+        const char* gr = getenv("POLYERI_GRAPH");
+        c->use_graph = !(gr && gr[0] == '0');
         if (c->jk_copies != 1 && c->jk_copies != 8) c->jk_copies = 8;
     }
     if ((rc = dalloc(&c->d_J, (size_t)nbf * nbf * c->jk_copies))) return rc;
@@ -571,6 +603,69 @@ int polyeri_basis_read(const char* path, int natom, const int* iatom, const doub
     return 0;
 }
 
+// Per-list launch shapes chosen by timing: the register kernel with 256,
+// 128 or 64 threads per block (one pass, or a persistent grid of 2048 blocks
+// striding over the list) and, for classes of 36 or more integrals, the
+// cooperative kernel with grids of 2048, 4096 or 8192 wavefronts.  Each
+// candidate runs once untimed and reps times between events on the
+// context's density buffer (zeroed: the cost does not depend on values).
+int polyeri_autotune(polyeri_ctx* ctx, int reps) {
+    if (!ctx) return fail(POLYERI_E_ARG, "polyeri_autotune: null context");
+    if (reps < 1) reps = 1;
+    HIPCHK(hipSetDevice(ctx->device));
+    polyeri_ctx* c = ctx;
+    size_t n2 = (size_t)c->ib.nbf * c->ib.nbf;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_D, n2, 0.0);
+    HIPCHK(hipGetLastError());
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    for (ClassPairWork& cw : c->work) {
+        if (cw.nwork <= 0) continue;
+        const int mode = cw.prec ? MODE_JK32 : MODE_JK;
+        const int nint = ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld);
+        std::vector<LaunchCfg> cand;
+        const bool reg = polyeri_class_kernel(cw.la, cw.lb, cw.lc, cw.ld, mode) != nullptr;
+        if (reg) {
+            for (int b : {256, 128, 64}) { LaunchCfg x; x.kind = LAUNCH_REG; x.block = b; cand.push_back(x); }
+            LaunchCfg x; x.kind = LAUNCH_REG; x.block = 256; x.grid = 2048; cand.push_back(x);
+        }
+        if (!reg || nint >= 36)
+            for (int g : {2048, 4096, 8192}) { LaunchCfg x; x.kind = LAUNCH_COOP; x.grid = g; cand.push_back(x); }
+        double best = 1e300;
+        LaunchCfg pick = cw.cfg;
+        for (const LaunchCfg& x : cand) {
+            for (int r = 0; r <= reps && !rc; ++r) {
+                if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
+                rc = launch_class(c, mode, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
+                                  cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr, x);
+            }
+            if (rc) break;
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; pick = x; }
+        }
+        if (rc) break;
+        cw.cfg = pick;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    drop_graphs(c);
+    c->tuned = true;
+    return 0;
+}
+
+int polyeri_set_graph(polyeri_ctx* ctx, int on) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    ctx->use_graph = on != 0;
+    if (!ctx->use_graph) drop_graphs(ctx);
+    return 0;
+}
+
 int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
     if (!ctx) return fail(POLYERI_E_ARG, "null context");
     ctx->profiling = on != 0;
@@ -583,21 +678,17 @@ __global__ void k_fill_u32(unsigned* p, size_t n) {
     if (i < n) p[i] = 0u;
 }
 
-// J/K of the contracted density d_pc into d_jc / d_kc (reference layout).
-// dens_tol > 0 drops the quartets whose density-weighted Schwarz bound is
-// below dens_tol (kscreen.hip); the surviving items of every list are
-// compacted on the device and the class kernels stride over them.
-static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol = 0.0) {
+// The kernel sequence of one J/K build of the contracted density d_pc into
+// d_jc / d_kc (reference layout): density packing, zeroing (fill kernels,
+// so the sequence can be captured in a graph), the optional density
+// screening, the class kernels, and the unpacking.  dens_tol > 0 drops the
+// quartets whose density-weighted Schwarz bound is below dens_tol
+// (kscreen.hip); the surviving items of every list are compacted on the
+// device and the class kernels stride over them.
+static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol) {
     int nbf = c->ib.nbf, nc = c->rb.ncgs;
     size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
     const bool screen = dens_tol > 0.0;
-    if (screen)
-        for (ClassPairWork& w : c->work)
-            if (!w.d_items && w.nwork > 0) {
-                int rc = dalloc(&w.d_items, (size_t)w.nwork);
-                if (rc) return rc;
-            }
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_pack_density, dim3(nblk(n2)), dim3(256), 0, c->stream, d_pc, nc, c->d_ref,
                        c->d_scale, nbf, c->d_D);
     hipLaunchKernelGGL(k_fill, dim3(nblk(n2 * c->jk_copies)), dim3(256), 0, c->stream, c->d_J, n2 * c->jk_copies, 0.0);
@@ -624,7 +715,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
-                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr);
+                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -633,6 +724,46 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     hipLaunchKernelGGL(k_unpack_jk, dim3(nblk(n2)), dim3(256), 0, c->stream, c->d_J, c->d_K, nbf, c->jk_copies,
                        c->d_ref, c->d_scale, nc, d_jc, d_kc);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One J/K build.  Without per-class profiling the kernel sequence is
+// captured once per (density, J, K, screening) combination into a HIP graph
+// and replayed; the timing events stay outside the graph.
+static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol = 0.0) {
+    const bool screen = dens_tol > 0.0;
+    if (screen)
+        for (ClassPairWork& w : c->work)
+            if (!w.d_items && w.nwork > 0) {
+                int rc = dalloc(&w.d_items, (size_t)w.nwork);
+                if (rc) return rc;
+            }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (c->use_graph && !c->profiling) {
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : c->graphs)
+            if (g.pc == d_pc && g.jc == d_jc && g.kc == d_kc && g.tol == dens_tol) exec = g.exec;
+        if (!exec) {
+            if (c->graphs.size() >= 8) {
+                (void)hipGraphExecDestroy(c->graphs.front().exec);
+                c->graphs.erase(c->graphs.begin());
+            }
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = jk_enqueue(c, d_pc, d_jc, d_kc, dens_tol);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (e != hipSuccess) return fail(POLYERI_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(POLYERI_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            c->graphs.push_back({d_pc, d_jc, d_kc, dens_tol, exec});
+        }
+        HIPCHK(hipGraphLaunch(exec, c->stream));
+    } else {
+        int rc = jk_enqueue(c, d_pc, d_jc, d_kc, dens_tol);
+        if (rc) return rc;
+    }
     if (screen && !c->work.empty())
         HIPCHK(hipMemcpyAsync(c->h_counts.data(), c->d_counts, c->work.size() * sizeof(unsigned),
                               hipMemcpyDeviceToHost, c->stream));
@@ -813,6 +944,12 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
         st->last_class_ms[i] = ctx->last_class_ms[i];
         st->class_quartets[i] = w.nwork;
         st->class_prec[i] = w.prec;
+        const int kind = w.cfg.kind != LAUNCH_AUTO ? w.cfg.kind
+                       : (ncart_h(w.la) * ncart_h(w.lb) * ncart_h(w.lc) * ncart_h(w.ld) > POLYERI_COOP_MIN_INTEGRALS
+                              ? LAUNCH_COOP : LAUNCH_REG);
+        st->class_kind[i] = kind;
+        st->class_block[i] = kind == LAUNCH_COOP ? 64 : w.cfg.block;
+        st->class_grid[i] = w.cfg.grid;
         st->class_prim_quartets[i] = w.prim_quartets;
         st->class_flops[i] = w.flops;
         st->class_l[i] = w.la * 1000 + w.lb * 100 + w.lc * 10 + w.ld;
@@ -1020,7 +1157,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     HIPCHK(hipEventCreate(&ef1));
     for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
         HIPCHK(hipEventRecord(ef0, s));
-        if (o->incremental > 0 && iscf > 1 && since_full < o->incremental) {
+        const bool inc_now = o->incremental > 0 && iscf > 1 && since_full < o->incremental;
+        if (inc_now) {
             // J/K(P) = J/K(P_prev) + J/K(P - P_prev)
             hipLaunchKernelGGL(k_axpby, dim3(nblk(n2)), dim3(256), 0, s, dP, dPprev, -1.0, n2, dDel);
             if ((rc = jk_device(ctx, dDel, dJd, dKd, dens_tol))) break;
@@ -1126,6 +1264,16 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             double a = o->scfrelax, b = 1.0 - o->scfrelax;
             RBCHK(rocblas_dscal(h, (int)n2, &b, dP, 1));
             RBCHK(rocblas_daxpy(h, (int)n2, &a, dPout, 1, dP, 1));
+        }
+        // Converged on an incremental build: the energy below takes J/K of this
+        // iteration's input density from one full, unscreened build, so it
+        // carries no accumulated screening error.
+        if (dchg < o->scfconv && inc_now) {
+            if ((rc = jk_device(ctx, dPprev, ctx->d_jc, ctx->d_kc, 0.0))) break;
+            HIPCHK(hipStreamSynchronize(s));
+            if ((rc = jk_collect_timing(ctx))) break;
+            jk_total += ctx->last_jk_ms * 1e-3;
+            build_quartets += ctx->last_quartets;
         }
         // ENERGY_CALC with the updated density (src/pselfconsistentfield.f90:487-574)
         RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, &ek));
