@@ -198,6 +198,30 @@ int polyeri_autotune(polyeri_ctx *ctx, int reps);
 
 const char *polyeri_last_error(void);
 
+/* ---- Learned SCF guess: multilayer perceptron on the device ------------
+ * No counterpart in the reference (its guess is GUESS_DENSITYMATRIX,
+ * src/pdensitymatrix.f90:83-127).  Layer l: Z = A W^T + b, A = silu(Z),
+ * the last layer linear; GEMMs in BF16 on the MFMA units with FP32
+ * accumulation, FP32 master weights, Adam.  Device pointers are row-major
+ * FP32 (batch x width). */
+typedef struct polyeri_mlp polyeri_mlp;
+int polyeri_mlp_create(int nlayers, const int *dims, unsigned long long seed, polyeri_mlp **mlp);
+void polyeri_mlp_destroy(polyeri_mlp *mlp);
+long polyeri_mlp_nparam(const polyeri_mlp *mlp);
+/* set = 1: host -> device (and reset Adam state); 0: device -> host */
+int polyeri_mlp_params(polyeri_mlp *mlp, float *host, int set);
+/* gradients of the last train step (update = 0 leaves the weights alone) */
+int polyeri_mlp_grads(polyeri_mlp *mlp, float *host);
+int polyeri_mlp_forward(polyeri_mlp *mlp, const float *d_x, int batch, float *d_y);
+/* loss = sum w (y - t)^2 / sum w over the batch; update = 1 takes an Adam step */
+int polyeri_mlp_train_step(polyeri_mlp *mlp, const float *d_x, const float *d_t, const float *d_w, int batch,
+                           float lr, int update, float *loss);
+/* C = alpha A B + beta C on the MFMA units (operands rounded to BF16, FP32
+ * accumulation); A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn], C
+ * row-major with leading dimension ldc. */
+int polyeri_gemm_bf16(int m, int n, int k, const float *d_a, long sam, long sak, const float *d_b, long sbk,
+                      long sbn, float alpha, float beta, float *d_c, long ldc);
+
 #ifdef __cplusplus
 }
 #endif

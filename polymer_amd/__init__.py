@@ -179,6 +179,17 @@ class Context:
         check(lib().polyeri_memcpy(self.h, _p(a), dptr, a.nbytes, 0))
         return a
 
+    def to_device_raw(self, dptr, arr):
+        """Copy the bytes of a C-contiguous array to device memory."""
+        a = np.ascontiguousarray(arr)
+        check(lib().polyeri_memcpy(self.h, dptr, _p(a), a.nbytes, 1))
+
+    def from_device_raw(self, dptr, out):
+        """Fill a C-contiguous array from device memory."""
+        assert out.flags["C_CONTIGUOUS"]
+        check(lib().polyeri_memcpy(self.h, _p(out), dptr, out.nbytes, 0))
+        return out
+
     def fock_build_device(self, d_p, d_f, d_j, d_k, hfexchange=1.0):
         check(lib().polyeri_fock_build_device(self.h, d_p, hfexchange, d_f, d_j, d_k))
 

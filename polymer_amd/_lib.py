@@ -18,6 +18,8 @@ EXPORTS = [
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
     "polyeri_fock_build_device", "polyeri_device_copy", "polyeri_jk_build_screened",
     "polyeri_autotune", "polyeri_set_graph",
+    "polyeri_mlp_create", "polyeri_mlp_destroy", "polyeri_mlp_nparam", "polyeri_mlp_params",
+    "polyeri_mlp_grads", "polyeri_mlp_forward", "polyeri_mlp_train_step", "polyeri_gemm_bf16",
 ]
 
 
@@ -103,9 +105,20 @@ def lib():
     L.polyeri_memcpy.argtypes = [p, p, p, ctypes.c_size_t, i]
     L.polyeri_fock_build_device.argtypes = [p, p, d, p, p, p]
     L.polyeri_device_copy.argtypes = [p, p, p, ctypes.c_size_t]
+    L.polyeri_mlp_create.argtypes = [i, p, ctypes.c_ulonglong, ctypes.POINTER(p)]
+    L.polyeri_mlp_destroy.argtypes = [p]
+    L.polyeri_mlp_destroy.restype = None
+    L.polyeri_mlp_nparam.argtypes = [p]
+    L.polyeri_mlp_params.argtypes = [p, p, i]
+    L.polyeri_mlp_grads.argtypes = [p, p]
+    L.polyeri_mlp_forward.argtypes = [p, p, i, p]
+    L.polyeri_mlp_train_step.argtypes = [p, p, p, p, i, ctypes.c_float, i, ctypes.POINTER(ctypes.c_float)]
+    lg = ctypes.c_long
+    L.polyeri_gemm_bf16.argtypes = [i, i, i, p, lg, lg, p, lg, lg, ctypes.c_float, ctypes.c_float, p, lg]
     for name in EXPORTS:
-        if name not in ("polyeri_ctx_destroy", "polyeri_last_error"):
+        if name not in ("polyeri_ctx_destroy", "polyeri_last_error", "polyeri_mlp_destroy"):
             getattr(L, name).restype = ctypes.c_int
+    L.polyeri_mlp_nparam.restype = ctypes.c_long
     _LIB = L
     return L
 

@@ -38,6 +38,45 @@ struct PrimPair {
 
 POLY_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
+// sqrt(POLY_PI) rounded to double
+#define POLY_SQRT_PI 1.7724538509055152
+
+// 1/sqrt(x) for x > 0.  On the device: the hardware estimate refined by two
+// Newton-Raphson steps r <- r + r (1 - x r^2) / 2 (each step squares the
+// relative error, so the estimate's ~2^-22 ends below the double ulp),
+// one instruction sequence instead of a division and a square root.
+POLY_HD double rsqrt_fp64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rsq(x);
+    double e = __builtin_fma(-x * r, r, 1.0);
+    r = __builtin_fma(0.5 * r, e, r);
+    e = __builtin_fma(-x * r, r, 1.0);
+    return __builtin_fma(0.5 * r, e, r);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
+// Primitive pairs of the ket: an array of PrimPair, or the bucket-interleaved
+// field arrays (primitive q of the ket at p[q * nk], field f at f * fs), in
+// which neighbouring threads' kets are neighbouring doubles.
+struct AosKet {
+    const PrimPair* p;
+    POLY_HD PrimPair load(int q) const { return p[q]; }
+};
+struct SoaKet {
+    const double* p;
+    int nk;
+    long long fs;
+    POLY_HD PrimPair load(int q) const {
+        const double* x = p + (long long)q * nk;
+        PrimPair r;
+        r.zeta = x[0]; r.K = x[fs]; r.Px = x[2 * fs]; r.Py = x[3 * fs]; r.Pz = x[4 * fs];
+        r.PAx = x[5 * fs]; r.PAy = x[6 * fs]; r.PAz = x[7 * fs];
+        return r;
+    }
+};
+
 // F_0..F_L (T) scaled by pref.  FP64: 7-term Taylor step for F_L around the
 // nearest grid point, then the downward recursion
 // F_{m-1} = (2T F_m + e^-T) / (2m-1); large T: asymptotic F_0 and the
@@ -64,8 +103,8 @@ POLY_HD void boys_fp64(double T, double pref, const double* __restrict__ tab,
             for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
         }
     } else {
-        const double it = 1.0 / T;
-        F[0] = 0.5 * sqrt(POLY_PI * it);
+        const double r = rsqrt_fp64(T), it = r * r;
+        F[0] = 0.5 * POLY_SQRT_PI * r;
 #pragma unroll
         for (int m = 0; m < L; ++m) F[m + 1] = F[m] * (m + 0.5) * it;
     }
@@ -125,9 +164,9 @@ POLY_CLASS(2, 2, 2, 2)
 // quantities, T and the Boys function are FP64 in both precisions; real is
 // the type of the VRR / HRR recursion and of the integrals (double, or float
 // for the quartets the mixed-precision mode evaluates in FP32).
-template <typename real, int LA, int LB, int LC, int LD, typename Cb>
-POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const PrimPair* __restrict__ ket,
-                     int nket, double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,
+template <typename real, int LA, int LB, int LC, int LD, typename Ket, typename Cb>
+POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const Ket& ket, int nket, double ABx,
+                     double ABy, double ABz, double CDx, double CDy, double CDz,
                      const double* __restrict__ boys_tab, Cb& cb) {
     constexpr int L = LA + LB + LC + LD;
     // components of every l in [LA, LA+LB] (and [LC, LC+LD])
@@ -139,13 +178,14 @@ POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const PrimPair*
     for (int p = 0; p < nbra; ++p) {
         const PrimPair bp = bra[p];
         for (int q = 0; q < nket; ++q) {
-            const PrimPair kq = ket[q];
+            const PrimPair kq = ket.load(q);
             const double ze = bp.zeta + kq.zeta;
-            const double ize = 1.0 / ze;
+            const double rs = rsqrt_fp64(ze);
+            const double ize = rs * rs;
             const double rho = bp.zeta * kq.zeta * ize;
             const double PQx = bp.Px - kq.Px, PQy = bp.Py - kq.Py, PQz = bp.Pz - kq.Pz;
             const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = bp.K * kq.K * sqrt(ize);
+            const double pref = bp.K * kq.K * rs;
             double F[L + 1];
             boys_fp64<L>(T, pref, boys_tab, F);
             real Fr[L + 1];
@@ -168,7 +208,7 @@ template <int LA, int LB, int LC, int LD, typename Cb>
 POLY_HD void quartet_fp64(const PrimPair* __restrict__ bra, int nbra, const PrimPair* __restrict__ ket,
                           int nket, double ABx, double ABy, double ABz, double CDx, double CDy,
                           double CDz, const double* __restrict__ boys_tab, Cb& cb) {
-    quartet<double, LA, LB, LC, LD>(bra, nbra, ket, nket, ABx, ABy, ABz, CDx, CDy, CDz, boys_tab, cb);
+    quartet<double, LA, LB, LC, LD>(bra, nbra, AosKet{ket}, nket, ABx, ABy, ABz, CDx, CDy, CDz, boys_tab, cb);
 }
 
 // Store variant: out[((ia*NB + ib)*NC + ic)*ND + id].

@@ -40,8 +40,11 @@ struct EriArgs {
     int nbf;
     double* out;               // tensor (MODE_TENSOR) or Schwarz (MODE_SCHWARZ)
     int jk_copies;             // 1: one J/K (device-scope atomics); 8: one per XCD
-    const int2* items;         // explicit (bra, ket) work items (density-screened list), or null
+    const int2* items;         // explicit (bra pair, ket position) items of a density-screened list, or null
     const unsigned* count;     // number of items (device), with items
+    const double* ksoa;        // ket primitive pairs, bucket-interleaved fields (8 x kfs doubles)
+    long long kfs;             // field stride of ksoa
+    const int* bucket_soa;     // nbucket: offset of each ket bucket in ksoa
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -178,11 +181,12 @@ struct TensorCb {
     }
 };
 
-// Work item t of a class pair -> (bra pair, ket pair).  Rows are bra pairs in
-// Schwarz order; inside a row the surviving kets are grouped in buckets of
-// equal primitive-pair count (each bucket a Schwarz prefix), so neighbouring
-// threads run primitive loops of the same length.
-__device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& bp, int& kp) {
+// Work item t of a class pair -> (bra pair, ket position, ket bucket).
+// Rows are bra pairs in Schwarz order; inside a row the surviving kets are
+// grouped in buckets of equal primitive-pair count (each bucket a Schwarz
+// prefix, or a range from rb_first), so neighbouring threads run primitive
+// loops of the same length.  The ket pair is ket_ids[pos].
+__device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& bp, int& pos, int& b) {
     int lo = 0, hi = g.nrows - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -191,22 +195,33 @@ __device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& 
     bp = g.bra_ids[lo];
     const int local = (int)(t - g.row_off[lo]);
     const int* rb = g.rb_off + (size_t)lo * (g.nbucket + 1);
-    int b = 0;
+    b = 0;
     while (b + 1 < g.nbucket && rb[b + 1] <= local) ++b;
     const int first = g.rb_first ? g.rb_first[(size_t)lo * g.nbucket + b] : 0;
-    kp = g.ket_ids[g.bucket_start[b] + first + (local - rb[b])];
+    pos = g.bucket_start[b] + first + (local - rb[b]);
+}
+
+// Bucket of ket position pos.
+__device__ __forceinline__ int bucket_of(const EriArgs& g, int pos) {
+    int b = 0;
+    while (b + 1 < g.nbucket && g.bucket_start[b + 1] <= pos) ++b;
+    return b;
 }
 
 template <int LA, int LB, int LC, int LD, int MODE>
 __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
-    int bp, kp;
+    int bp, kp, pos = 0, b = 0;
     if (MODE == MODE_SCHWARZ) {
         bp = kp = g.bra_ids[t];
-    } else if (g.items) {
-        const int2 it = g.items[t];
-        bp = it.x; kp = it.y;
     } else {
-        decode_work(g, t, bp, kp);
+        if (g.items) {
+            const int2 it = g.items[t];
+            bp = it.x; pos = it.y;
+            b = bucket_of(g, pos);
+        } else {
+            decode_work(g, t, bp, pos, b);
+        }
+        kp = g.ket_ids[pos];
     }
     const PairMeta mb = g.meta[bp], mk = g.meta[kp];
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -220,9 +235,13 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
         SchwarzCb cb;
         quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
         g.out[bp] = sqrt(cb.m);
-    } else if (MODE == MODE_TENSOR) {
+        return;
+    }
+    const SoaKet kv{g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]), g.bucket_start[b + 1] - g.bucket_start[b],
+                    g.kfs};
+    if (MODE == MODE_TENSOR) {
         TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
-        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+        quartet<double, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
     } else {
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
         // light classes only (heavier ones run in kcoop.hip): the (ab|cd)
@@ -230,7 +249,7 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
         using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
         real v[NA * NB * NC * ND];
         StoreCb<NB, NC, ND, real> st{v};
-        quartet<real, LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
+        quartet<real, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
         RegDigest<NA, NB, NC, ND, real> cb;
         cb.load(g.D, n, fa, fb, fc, fd);
         const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;

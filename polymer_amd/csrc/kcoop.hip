@@ -42,9 +42,11 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             bp = kp = g.bra_ids[t];
         } else if (g.items) {
             const int2 it = g.items[t];
-            bp = it.x; kp = it.y;
+            bp = it.x; kp = g.ket_ids[it.y];
         } else {
-            decode_work(g, t, bp, kp);
+            int pos, b;
+            decode_work(g, t, bp, pos, b);
+            kp = g.ket_ids[pos];
         }
         const PairMeta mb = g.meta[bp], mk = g.meta[kp];
         for (int k = lane; k <= d.nacc; k += 64) sl[k] = real(0);

@@ -32,10 +32,12 @@ __global__ void __launch_bounds__(256) k_screen(EriArgs g, const double* __restr
     for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < g.nwork;
          base += stride) {
         const long long t = base + lane;
-        int bp = 0, kp = 0;
+        int bp = 0, pos = 0;
         bool keep = false;
         if (t < g.nwork) {
-            decode_work(g, t, bp, kp);
+            int bk;
+            decode_work(g, t, bp, pos, bk);
+            const int kp = g.ket_ids[pos];
             const int2 ab = psh[bp], cd = psh[kp];
             const size_t a = ab.x, b = ab.y, c = cd.x, d = cd.y;
             double m = fmax(dsh[c * nsh + d], dsh[a * nsh + b]);
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(256) k_screen(EriArgs g, const double* __restr
         unsigned slot = 0;
         if (lane == 0 && mask) slot = atomicAdd(count, (unsigned)__popcll(mask));
         slot = __shfl(slot, 0, 64);
-        if (keep) items[slot + below] = make_int2(bp, kp);
+        if (keep) items[slot + below] = make_int2(bp, pos);
     }
 }
 

@@ -35,6 +35,8 @@ static int fail(int code, const std::string& msg) {
     return code;
 }
 
+int polyeri_record_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
 #define HIPCHK(x)                                                                        \
     do {                                                                                 \
         hipError_t e_ = (x);                                                             \
@@ -221,6 +223,7 @@ struct ClassPairWork {
     long long* d_row;
     int* d_rb;             // nrows x (nbucket+1) bucket offsets inside each row
     int* d_bstart;         // bucket starts of class Y
+    int* d_bsoa;           // bucket offsets of class Y in the interleaved ket data
     int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
     LaunchCfg cfg;         // launch shape (autotuned or default)
@@ -253,6 +256,10 @@ struct polyeri_ctx {
     int* d_ket_ids[NPAIRCLASS] = {};
     int* d_bstart[NPAIRCLASS] = {};
     std::vector<int> ket_ids[NPAIRCLASS], bstart[NPAIRCLASS];
+    // ket primitive pairs bucket-interleaved: field f of primitive q of the
+    // i-th ket of bucket b of class Y at d_ksoa[f * nprims + bsoa[Y][b] + q * nk_b + i]
+    double* d_ksoa = nullptr;
+    int* d_bsoa[NPAIRCLASS] = {};
     std::vector<ClassPairWork> work;
     int* d_ref = nullptr;
     double* d_scale = nullptr;
@@ -323,9 +330,10 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
                         const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
                         const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr,
-                        LaunchCfg cfg = LaunchCfg()) {
+                        LaunchCfg cfg = LaunchCfg(), const int* bsoa = nullptr) {
     if (nwork <= 0) return 0;
     EriArgs a{};
+    a.ksoa = c->d_ksoa; a.kfs = (long long)c->prims.size(); a.bucket_soa = bsoa;
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
@@ -381,7 +389,10 @@ static void free_ctx(polyeri_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
-    for (int i = 0; i < NPAIRCLASS; ++i) { hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); }
+    for (int i = 0; i < NPAIRCLASS; ++i) {
+        hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); hipFree(c->d_bsoa[i]);
+    }
+    hipFree(c->d_ksoa);
     for (auto& w : c->work) {
         hipFree(w.d_row); hipFree(w.d_rb);
         if (w.d_rf) hipFree(w.d_rf);
@@ -483,12 +494,43 @@ static int build_ctx(polyeri_ctx* c) {
             HIPCHK(hipMemcpy(c->d_ket_ids[Y], c->ket_ids[Y].data(), c->ket_ids[Y].size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_bstart[Y], c->bstart[Y].data(), c->bstart[Y].size() * sizeof(int), hipMemcpyHostToDevice));
     }
+    {
+        const size_t P = c->prims.size();
+        std::vector<double> soa(8 * P, 0.0);
+        size_t off = 0;
+        for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+            const auto& kl = c->ket_ids[Y];
+            const auto& bs = c->bstart[Y];
+            std::vector<int> bo(std::max<size_t>(bs.size(), 1), 0);
+            for (size_t b = 0; b + 1 < bs.size(); ++b) {
+                const int nk = bs[b + 1] - bs[b];
+                const int np_b = c->pairs[kl[bs[b]]].nprim;
+                bo[b] = (int)off;
+                for (int i = 0; i < nk; ++i) {
+                    const PairInfo& pi = c->pairs[kl[bs[b] + i]];
+                    for (int q = 0; q < np_b; ++q) {
+                        const PrimPair& pp = c->prims[pi.prim_off + q];
+                        const size_t at = off + (size_t)q * nk + i;
+                        const double f[8] = {pp.zeta, pp.K, pp.Px, pp.Py, pp.Pz, pp.PAx, pp.PAy, pp.PAz};
+                        for (int k = 0; k < 8; ++k) soa[k * P + at] = f[k];
+                    }
+                }
+                off += (size_t)np_b * nk;
+            }
+            if ((rc = dalloc(&c->d_bsoa[Y], bo.size()))) return rc;
+            HIPCHK(hipMemcpy(c->d_bsoa[Y], bo.data(), bo.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (off != P) return fail(POLYERI_E_BASIS, "interleaved ket layout does not cover every primitive pair");
+        if ((rc = dalloc(&c->d_ksoa, soa.size()))) return rc;
+        HIPCHK(hipMemcpy(c->d_ksoa, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     c->quartets = 0; c->prim_quartets = 0; c->model_flops = 0;
     for (const WorkList& l : wl) {
         ClassPairWork w;
         w.X = l.X; w.Y = l.Y;
         w.la = kPairL[l.X][0]; w.lb = kPairL[l.X][1]; w.lc = kPairL[l.Y][0]; w.ld = kPairL[l.Y][1];
         w.d_bra = c->d_class_ids[l.X]; w.d_ket = c->d_ket_ids[l.Y]; w.d_bstart = c->d_bstart[l.Y];
+        w.d_bsoa = c->d_bsoa[l.Y];
         w.nbucket = l.nbucket;
         w.nrows = l.nrows;
         w.nwork = l.nwork;
@@ -639,7 +681,8 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
             for (int r = 0; r <= reps && !rc; ++r) {
                 if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
                 rc = launch_class(c, mode, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
-                                  cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr, x);
+                                  cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr, x,
+                                  cw.d_bsoa);
             }
             if (rc) break;
             HIPCHK(hipEventRecord(e1, c->stream));
@@ -715,7 +758,7 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
-                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg);
+                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg, cw.d_bsoa);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -891,7 +934,8 @@ static int tensor_device(polyeri_ctx* c, double* d_t) {
     HIPCHK(hipGetLastError());
     for (const ClassPairWork& cw : c->work) {
         int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
-                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf);
+                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr,
+                              LaunchCfg(), cw.d_bsoa);
         if (rc) return rc;
     }
     return 0;
