@@ -158,6 +158,22 @@ int polyeri_scf(polyeri_ctx *ctx, const polyeri_scf_opts *opts, double *p_c, dou
 int polyeri_mp2_energy(polyeri_ctx *ctx, const double *co, const double *eps, int icore,
                        int iocc, int nall, int ivirtcore, double *e2s, double *e2t);
 
+/* polyeri_mp2_energy with the quarter transforms' products chosen: rocBLAS
+ * dgemm, this library's FP64 MFMA kernel (v_mfma_f64_16x16x4f64), or its
+ * BF16x3-split MFMA kernel (three BF16 pieces per operand, six products,
+ * FP32 accumulation).  No counterpart in the reference (scalar loops,
+ * src/pmp2energy.f90:187-430). */
+#define POLYERI_GEMM_ROCBLAS 0
+#define POLYERI_GEMM_MFMA_F64 1
+#define POLYERI_GEMM_MFMA_BF16X3 2
+int polyeri_mp2_energy_ex(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc,
+                          int nall, int ivirtcore, int transform, double *e2s, double *e2t);
+/* One product C = A B on those kernels (transform = POLYERI_GEMM_MFMA_F64 or
+ * _BF16X3); A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn],
+ * C(m,n) = c[m*scm + n*scn]; device pointers. */
+int polyeri_gemm_f64(int transform, int m, int n, int k, const double *d_a, long sam, long sak,
+                     const double *d_b, long sbk, long sbn, double *d_c, long scm, long scn);
+
 /* One entry per work list: class pair (X >= Y) and, in mixed precision,
  * FP64 and FP32 lists of each. */
 #define POLYERI_MAX_LISTS 64

@@ -48,6 +48,8 @@ def lib():
         L.or_fourindex_eri_rows.argtypes = [p, p, p, d, i, p, ll, p, p, p, p, p, p]
         L.or_fourindex_eri_rows.restype = ll
         L.or_eri_tensor.argtypes = [p, p, p]
+        L.or_fourindex_eri_cells_c.argtypes = [p, d, i, i, i, p, d, p, p]
+        L.or_fourindex_eri_cells_c.restype = ll
         L.or_decontract.argtypes = [p, p, p]
         L.or_form_c2s.argtypes = [p, i, p]
         L.or_scf.argtypes = [p, p, p, p, p, p, p]
@@ -149,6 +151,21 @@ class Basis:
         nq = lib().or_fourindex_eri(self.h, _ptr(s_p), _ptr(p_p), float(inttol), int(recompute_kl),
                                     _ptr(c_c), _ptr(x_c), ctypes.byref(nreal), ctypes.byref(fl))
         return c_c, x_c, {"composite_quartets": nq, "real_quartets": nreal.value, "flops": fl.value}
+
+    def fourindex_eri_cells(self, p_cells, period, rc1, cel1, cel2, inttol):
+        """Periodic J_C, K_C per cell (src/pfourindexdirect.f90:222-750,
+        HELIX = 0, axis x).  p_cells: (2 cel1 + 1, ncgs, ncgs), cell Q at
+        index Q + cel1, P(Q)[k, l] = P(k in cell 0, l in cell Q)."""
+        nc, ncell = self.ncgs, 2 * cel1 + 1
+        pc = np.zeros((ncell, nc, nc))
+        for c in range(ncell):
+            pc[c] = np.asarray(p_cells[c], dtype=np.float64).T    # column-major blocks
+        jc = np.zeros((ncell, nc, nc))
+        kc = np.zeros((ncell, nc, nc))
+        nq = lib().or_fourindex_eri_cells_c(self.h, float(period), int(rc1), int(cel1), int(cel2),
+                                            _ptr(pc), float(inttol), _ptr(jc), _ptr(kc))
+        return (np.ascontiguousarray(jc.transpose(0, 2, 1)), np.ascontiguousarray(kc.transpose(0, 2, 1)),
+                {"composite_quartets": nq})
 
     def fourindex_eri_range(self, p_c, inttol, ij_begin, ij_end, recompute_kl=True):
         s_p = np.zeros((self.npgs, self.npgs), order="F")

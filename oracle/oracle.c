@@ -693,6 +693,113 @@ long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const doub
     return nq;
 }
 
+static void contract_cc(const or_basis *b, const double *m_p, double *m_c);
+
+/* ------------------------------------------------------------------ */
+/* Periodic direct J/K: src/pfourindexdirect.f90:222-750 with HELIX = 0 */
+/* and the periodic axis x (the reference's only 1-D axis).            */
+/* ------------------------------------------------------------------ */
+/* cpair_make with the two anchors translated along x by sI and sJ
+ * (src/pfourindexdirect.f90:262-270, 369-373, 470-479: shells of cell Q
+ * sit at PGSX + Q*PERIODX). */
+static void cpair_make_shift(const or_basis *b, int I, double sI, int J, double sJ, cpair *cp) {
+    cpair_make(b, I, J, cp);
+    if (sI == 0.0 && sJ == 0.0) return;
+    int ii = b->p_shell[I * 125], jj = b->p_shell[J * 125];
+    cp->ax += sI;
+    cp->bx += sJ;
+    double z = cp->zeta;
+    cp->px = (cp->ax * b->zt[ii] + cp->bx * b->zt[jj]) / z;
+    double r = (cp->ax - cp->bx) * (cp->ax - cp->bx) + (cp->ay - cp->by) * (cp->ay - cp->by) +
+               (cp->az - cp->bz) * (cp->az - cp->bz);
+    cp->ov = sqrt(PI_REF * PI_REF * PI_REF) / sqrt(z * z * z) * exp(-b->zt[ii] * b->zt[jj] / z * r);
+    const pairtab *t = cp->t;
+    double pa[3] = {cp->px - cp->ax, cp->py - cp->ay, cp->pz - cp->az};
+    double pb[3] = {cp->px - cp->bx, cp->py - cp->by, cp->pz - cp->bz};
+    for (int c = 0; c < t->n; ++c)
+        cp->dca[c] = (t->dir[c] < 0) ? 0.0 : (t->on_b[c] ? pb[t->dir[c]] : pa[t->dir[c]]);
+}
+
+long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int cel1, int cel2,
+                                 const double *p_p, double inttol, double *c_p, double *x_p) {
+    boys_init();
+    pairtab_init();
+    int ns = b->npshell, np = b->npgs;
+    size_t np2 = (size_t)np * np, nsq = (size_t)ns * ns;
+    double *maxc = maxc_make(b);
+    cpair *kl0 = (cpair *)malloc(sizeof(cpair) * nsq);
+    cpair ij, q;
+    double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
+    const double c1f = 2.0 / sqrt(PI_REF);
+    long long nq = 0;
+    for (int q1 = -rc1; q1 <= rc1; ++q1)
+        for (int q2 = -rc1; q2 <= rc1; ++q2) {
+            /* KLKL with K in cell 0 and L in cell Q2 (:258-364) */
+            for (int k = 0; k < ns; ++k)
+                for (int l = 0; l < ns; ++l) {
+                    cpair_make_shift(b, k, 0.0, l, q2 * period, &kl0[(size_t)k * ns + l]);
+                    cpair_bound(&kl0[(size_t)k * ns + l], maxc, e);
+                }
+            const double *pq2 = p_p + (size_t)(q2 + cel1) * np2;
+            for (int I = 0; I < ns; ++I)
+                for (int J = 0; J < ns; ++J) {
+                    cpair_make_shift(b, I, 0.0, J, q1 * period, &ij);
+                    cpair_bound(&ij, maxc, e);
+                    double c1 = ij.ov * c1f;
+                    double *cq1 = c_p + (size_t)(q1 + cel1) * np2;
+                    for (int q3 = -cel2; q3 <= cel2; ++q3) {
+                        /* LCEL1 (:247-259): exchange only into stored cells */
+                        int lx = (q3 + q2 >= -cel1 && q3 + q2 <= cel1 && q1 - q3 >= -cel1 && q1 - q3 <= cel1);
+                        double *xq = lx ? x_p + (size_t)(q3 + q2 + cel1) * np2 : NULL;
+                        const double *pq13 = lx ? p_p + (size_t)(q1 - q3 + cel1) * np2 : NULL;
+                        for (size_t skl = 0; skl < nsq; ++skl) {
+                            if (ij.bound * kl0[skl].bound < inttol) continue;
+                            int K = (int)(skl / ns), L = (int)(skl % ns);
+                            cpair_make_shift(b, K, q3 * period, L, (q3 + q2) * period, &q);
+                            ++nq;
+                            double ze = ij.zeta + q.zeta;
+                            double rho = ij.zeta * q.zeta / ze;
+                            double c2 = c1 * sqrt(rho) * q.ov;
+                            double dx = q.px - ij.px, dy = q.py - ij.py, dz = q.pz - ij.pz;
+                            double t = rho * (dx * dx + dy * dy + dz * dz);
+                            os_quartet(&ij, &q, c2, t, 0, e);
+                            int nij = ij.t->n, nk = q.t->n, ms = ij.lang + q.lang + 1;
+                            for (int a = 0; a < nij; ++a) {
+                                int pi = ij.pi[a], pj = ij.pj[a];
+                                for (int c = 0; c < nk; ++c) {
+                                    double v = e[((size_t)a * nk + c) * ms];
+                                    int pk = q.pi[c], pl = q.pj[c];
+                                    cq1[pi + (size_t)np * pj] += pq2[pk + (size_t)np * pl] * v;
+                                    if (lx) xq[pi + (size_t)np * pl] += pq13[pk + (size_t)np * pj] * v;
+                                }
+                            }
+                        }
+                    }
+                }
+        }
+    free(e); free(kl0); free(maxc);
+    return nq;
+}
+
+/* Contracted periodic J/K (C_C, X_C per cell, src/pfourindexdirect.f90:
+ * 765-781): p_c, c_c, x_c hold 2 cel1 + 1 column-major ncgs^2 blocks. */
+long long or_fourindex_eri_cells_c(const or_basis *b, double period, int rc1, int cel1, int cel2,
+                                   const double *p_c, double inttol, double *c_c, double *x_c) {
+    int np = b->npgs, nc = b->ncgs, ncell = 2 * cel1 + 1;
+    size_t np2 = (size_t)np * np, nc2 = (size_t)nc * nc;
+    double *pp = (double *)calloc(np2 * ncell, sizeof(double));
+    double *cp = (double *)calloc(np2 * ncell, sizeof(double));
+    double *xp = (double *)calloc(np2 * ncell, sizeof(double));
+    for (int c = 0; c < ncell; ++c) or_decontract(b, p_c + nc2 * c, pp + np2 * c);
+    long long n = or_fourindex_eri_cells(b, period, rc1, cel1, cel2, pp, inttol, cp, xp);
+    for (int c = 0; c < ncell; ++c) {
+        contract_cc(b, cp + np2 * c, c_c + nc2 * c);
+        contract_cc(b, xp + np2 * c, x_c + nc2 * c);
+    }
+    free(pp); free(cp); free(xp);
+    return n;
+}
+
 long long or_fourindex_eri_range(const or_basis *b, const double *s_p, const double *p_p,
                                  double inttol, int recompute_kl, long long ij_begin,
                                  long long ij_end, double *c_p, double *x_p,

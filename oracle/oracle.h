@@ -87,6 +87,17 @@ long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const doub
 
 /* Contracted ERI tensor FERI(i,j,k,l) = sum CC.CC.CC.CC (pq|rs), the
  * quantity src/pfourindexstore.f90:704-708 accumulates (no screening). */
+/* Periodic direct J/K (src/pfourindexdirect.f90:222-750, HELIX = 0, axis x):
+ * J(Q1)[i,j] = sum_{Q2,Q3} (i_0 j_Q1 | k_Q3 l_Q3+Q2) P(Q2)[k,l] and
+ * K(Q3+Q2)[i,l] += (i_0 j_Q1 | k_Q3 l_Q3+Q2) P(Q1-Q3)[k,j] when both cells
+ * lie in [-cel1, cel1]; Q1, Q2 in [-rc1, rc1], Q3 in [-cel2, cel2], no
+ * permutational symmetry, IJIJ*KLKL < inttol skipped.  Primitive-basis
+ * arrays hold 2 cel1 + 1 npgs^2 blocks (cell Q at Q + cel1); the _c form
+ * takes and returns contracted ncgs^2 blocks.  Returns quartets computed. */
+long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int cel1, int cel2,
+                                 const double *p_p, double inttol, double *c_p, double *x_p);
+long long or_fourindex_eri_cells_c(const or_basis *b, double period, int rc1, int cel1, int cel2,
+                                   const double *p_c, double inttol, double *c_c, double *x_c);
 void or_eri_tensor(const or_basis *b, const double *s_p, double *feri);
 
 /* src/pdensitymatrix.f90:343-381 (DECONTRACT): P_P = CC^T P_C CC.         */

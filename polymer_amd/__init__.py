@@ -153,12 +153,19 @@ class Context:
             check(rc)
         return res
 
-    def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0):
+    def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
+        """E(2) singlet and triplet parts.  transform: None (rocBLAS dgemm),
+        "f64" (own FP64 MFMA kernel) or "bf16x3" (own BF16-split MFMA kernel)."""
         co = np.asfortranarray(co, dtype=np.float64)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         s, t = ctypes.c_double(), ctypes.c_double()
-        check(lib().polyeri_mp2_energy(self.h, _p(co), _p(eps), icore, iocc, nall, ivirtcore,
-                                       ctypes.byref(s), ctypes.byref(t)))
+        if transform is None:
+            check(lib().polyeri_mp2_energy(self.h, _p(co), _p(eps), icore, iocc, nall, ivirtcore,
+                                           ctypes.byref(s), ctypes.byref(t)))
+        else:
+            mode = {"rocblas": 0, "f64": 1, "bf16x3": 2}[transform]
+            check(lib().polyeri_mp2_energy_ex(self.h, _p(co), _p(eps), icore, iocc, nall, ivirtcore, mode,
+                                              ctypes.byref(s), ctypes.byref(t)))
         return s.value, t.value
 
     # device-resident buffers (used by bench.py: inputs live in HBM)

@@ -20,6 +20,7 @@ EXPORTS = [
     "polyeri_autotune", "polyeri_set_graph",
     "polyeri_mlp_create", "polyeri_mlp_destroy", "polyeri_mlp_nparam", "polyeri_mlp_params",
     "polyeri_mlp_grads", "polyeri_mlp_forward", "polyeri_mlp_train_step", "polyeri_gemm_bf16",
+    "polyeri_mp2_energy_ex", "polyeri_gemm_f64",
 ]
 
 
@@ -97,6 +98,7 @@ def lib():
     L.polyeri_eri_tensor.argtypes = [p, p]
     L.polyeri_scf.argtypes = [p, p, p, p, p, p, p]
     L.polyeri_mp2_energy.argtypes = [p, p, p, i, i, i, i, p, p]
+    L.polyeri_mp2_energy_ex.argtypes = [p, p, p, i, i, i, i, i, p, p]
     L.polyeri_get_stats.argtypes = [p, p]
     L.polyeri_set_profiling.argtypes = [p, i]
     L.polyeri_last_error.restype = ctypes.c_char_p
@@ -115,6 +117,7 @@ def lib():
     L.polyeri_mlp_train_step.argtypes = [p, p, p, p, i, ctypes.c_float, i, ctypes.POINTER(ctypes.c_float)]
     lg = ctypes.c_long
     L.polyeri_gemm_bf16.argtypes = [i, i, i, p, lg, lg, p, lg, lg, ctypes.c_float, ctypes.c_float, p, lg]
+    L.polyeri_gemm_f64.argtypes = [i, i, i, i, p, lg, lg, p, lg, lg, p, lg, lg]
     for name in EXPORTS:
         if name not in ("polyeri_ctx_destroy", "polyeri_last_error", "polyeri_mlp_destroy"):
             getattr(L, name).restype = ctypes.c_int

@@ -276,6 +276,12 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
 }
 
 typedef void (*EriKernel)(EriArgs);
+// Strided batched FP64 products on the MFMA units (kgemm.hip):
+// C_b(m,n) = sum_k A_b(m,k) B_b(k,n), X_b(r,c) = X[b*sX + r*sXr + c*sXc].
+enum { GEMM_F64 = 1, GEMM_BF16X3 = 2 };
+int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam, long sak, long sA, const double* B,
+                        long sbk, long sbn, long sB, double* C, long scm, long scn, long sC, int batch,
+                        hipStream_t stream);
 // Density screening (kscreen.hip): shell-block maxima of |D|, and the
 // compaction of one work list to the items whose bound
 // Q_ab Q_cd max(|D| over the six blocks the quartet reads) reaches tol.

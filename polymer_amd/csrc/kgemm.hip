@@ -1,0 +1,149 @@
+// kgemm.hip -- strided, batched FP64 matrix products on the MFMA units for
+// the MP2 four-index transform.
+//
+//   C_b(m,n) = sum_k A_b(m,k) B_b(k,n),  X_b(r,c) = X[b*sX + r*sXr + c*sXc]
+//
+// GEMM_F64: v_mfma_f64_16x16x4f64, FP64 products and accumulation.
+// GEMM_BF16X3: each FP64 operand split into three BF16 pieces
+// (a = a0 + a1 + a2 + O(2^-24 |a|)); the six products a_i b_j with
+// i + j <= 2 on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation, the
+// result converted to FP64 (about FP32 accuracy at BF16 MFMA rate).
+//
+// One wavefront per 16 x 16 tile of C (four per block); lane l holds row
+// l % 16 of the A slice and column l % 16 of the B slice (k = l / 16 for the
+// FP64 instruction, k = 4 (l / 16) .. +3 for BF16) and receives column l % 16
+// of the result: rows q, q+4, q+8, q+12 (FP64) or 4q .. 4q+3 (BF16),
+// q = l / 16.  Rows, columns and k out of range are zero, so any size is
+// accepted.
+#include <hip/hip_runtime.h>
+
+#include "eri_kernels.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double doublex4;
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+struct GemmArgs {
+    int M, N, K;
+    const double* A;
+    long sam, sak, sA;
+    const double* B;
+    long sbk, sbn, sB;
+    double* C;
+    long scm, scn, sC;
+};
+
+__global__ void __launch_bounds__(256) k_gemm_f64(GemmArgs g) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tiles_n = (g.N + 15) >> 4;
+    const long tile = (long)blockIdx.x * 4 + wave;
+    const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
+    if ((long)tm * 16 >= g.M) return;
+    const long b = blockIdx.y;
+    const double* A = g.A + b * g.sA;
+    const double* B = g.B + b * g.sB;
+    const int r = lane & 15, q = lane >> 4;
+    const int m = tm * 16 + r, n = tn * 16 + r;
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < g.K; k0 += 4) {
+        const int k = k0 + q;
+        const double a = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.0;
+        const double bb = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    }
+    // the FP64 instruction returns rows q, q+4, q+8, q+12 of column l % 16
+    double* C = g.C + b * g.sC;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = tm * 16 + q + 4 * j, col = tn * 16 + r;
+        if (row < g.M && col < g.N) C[row * g.scm + col * g.scn] = acc[j];
+    }
+}
+
+__device__ __forceinline__ short bf16_bits(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+// a = p0 + p1 + p2 + O(2^-24 |a|), each piece a BF16 number.
+__device__ __forceinline__ void split3(double a, short& p0, short& p1, short& p2) {
+    p0 = bf16_bits((float)a);
+    const double r1 = a - (double)bf16_value(p0);
+    p1 = bf16_bits((float)r1);
+    const double r2 = r1 - (double)bf16_value(p1);
+    p2 = bf16_bits((float)r2);
+}
+
+__global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmArgs g) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tiles_n = (g.N + 15) >> 4;
+    const long tile = (long)blockIdx.x * 4 + wave;
+    const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
+    if ((long)tm * 16 >= g.M) return;
+    const long b = blockIdx.y;
+    const double* A = g.A + b * g.sA;
+    const double* B = g.B + b * g.sB;
+    const int r = lane & 15, q = lane >> 4;
+    const int m = tm * 16 + r, n = tn * 16 + r;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        bf16x4 a0, a1, a2, b0, b1, b2;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + 4 * q + j;
+            const double av = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.0;
+            const double bv = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.0;
+            short x0, x1, x2;
+            split3(av, x0, x1, x2);
+            a0[j] = x0; a1[j] = x1; a2[j] = x2;
+            split3(bv, x0, x1, x2);
+            b0[j] = x0; b1[j] = x1; b2[j] = x2;
+        }
+        // small terms first
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b2, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc, 0, 0, 0);
+    }
+    double* C = g.C + b * g.sC;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = tm * 16 + 4 * q + j, col = tn * 16 + r;
+        if (row < g.M && col < g.N) C[row * g.scm + col * g.scn] = (double)acc[j];
+    }
+}
+
+}  // namespace
+
+int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam, long sak, long sA, const double* B,
+                        long sbk, long sbn, long sB, double* C, long scm, long scn, long sC, int batch,
+                        hipStream_t stream) {
+    if (M < 1 || N < 1 || K < 1 || batch < 1) return -1;
+    if (mode != GEMM_F64 && mode != GEMM_BF16X3) return -1;
+    const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+    for (int b0 = 0; b0 < batch; b0 += 65535) {       // grid.y limit
+        const int nb = batch - b0 < 65535 ? batch - b0 : 65535;
+        GemmArgs g{M, N, K, A + b0 * sA, sam, sak, sA, B + b0 * sB, sbk, sbn, sB, C + b0 * sC, scm, scn, sC};
+        const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
+        if (mode == GEMM_F64)
+            hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, stream, g);
+        else
+            hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, stream, g);
+        if (hipGetLastError() != hipSuccess) return -2;
+    }
+    return 0;
+}
+
+extern "C" int polyeri_gemm_f64(int mode, int M, int N, int K, const double* d_a, long sam, long sak, const double* d_b,
+                                long sbk, long sbn, double* d_c, long scm, long scn) {
+    if (M < 1 || N < 1 || K < 1 || !d_a || !d_b || !d_c) return -1;
+    const int r = polyeri_gemm_launch(mode, M, N, K, d_a, sam, sak, 0, d_b, sbk, sbn, 0, d_c, scm, scn, 0, 1, nullptr);
+    if (r) return -1;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -2;
+}

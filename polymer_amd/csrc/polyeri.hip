@@ -1353,11 +1353,13 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
 // ---------------------------------------------------------------------------
 // MP2 (one k-point): AO integrals on the device, four quarter transforms
 // ---------------------------------------------------------------------------
-int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
-                       int ivirtcore, double* e2s, double* e2t) {
+static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
+                      int ivirtcore, int gemm, double* e2s, double* e2t) {
     if (!ctx || !co || !eps || !e2s || !e2t) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: null argument");
     int no = iocc - icore, nv = nall - ivirtcore - iocc;
     if (no <= 0 || nv <= 0 || icore < 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: empty orbital space");
+    if (gemm != POLYERI_GEMM_ROCBLAS && gemm != POLYERI_GEMM_MFMA_F64 && gemm != POLYERI_GEMM_MFMA_BF16X3)
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy: unknown transform precision");
     HIPCHK(hipSetDevice(ctx->device));
     int rc;
     if ((rc = ensure_blas(ctx))) return rc;
@@ -1380,19 +1382,34 @@ int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, in
     const double one = 1.0, zero = 0.0;
     const double* Cv = dC + (size_t)n * iocc;    // virtual columns
     const double* Co = dC + (size_t)n * icore;   // active occupied columns
-    // tensor A(sigma, lambda, nu, mu) column-major; X1(b, lam nu mu) = Cv^T A
-    RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, n * n * n, n, &one, Cv, n, dTen, n,
-                        &zero, dX1, nv));
-    // X2(b, j, nu mu) = X1(b, lam, nu mu) Co(lam, j)  [batched over nu mu]
-    RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, no, n, &one, dX1, nv,
-                                        (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv, (rocblas_stride)nv * no, n * n));
-    // X3(bj, a, mu) = X2(bj, nu, mu) Cv(nu, a)  [batched over mu]
-    RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * no, nv, n, &one, dX2,
-                                        nv * no, (rocblas_stride)nv * no * n, Cv, n, 0, &zero, dX3, nv * no,
-                                        (rocblas_stride)nv * no * nv, n));
-    // T(bja, i) = X3(bja, mu) Co(mu, i)
-    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * no * nv, no, n, &one, dX3, nv * no * nv,
-                        Co, n, &zero, dT4, nv * no * nv));
+    if (gemm == POLYERI_GEMM_ROCBLAS) {
+        // tensor A(sigma, lambda, nu, mu) column-major; X1(b, lam nu mu) = Cv^T A
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, n * n * n, n, &one, Cv, n, dTen,
+                            n, &zero, dX1, nv));
+        // X2(b, j, nu mu) = X1(b, lam, nu mu) Co(lam, j)  [batched over nu mu]
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, no, n, &one, dX1, nv,
+                                            (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv, (rocblas_stride)nv * no,
+                                            n * n));
+        // X3(bj, a, mu) = X2(bj, nu, mu) Cv(nu, a)  [batched over mu]
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * no, nv, n, &one,
+                                            dX2, nv * no, (rocblas_stride)nv * no * n, Cv, n, 0, &zero, dX3, nv * no,
+                                            (rocblas_stride)nv * no * nv, n));
+        // T(bja, i) = X3(bja, mu) Co(mu, i)
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * no * nv, no, n, &one, dX3,
+                            nv * no * nv, Co, n, &zero, dT4, nv * no * nv));
+    } else {
+        // the same four quarter transforms on this library's MFMA kernels
+        const int mode = gemm == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
+        const long N = n, NV = nv, NO = no;
+        int r = polyeri_gemm_launch(mode, nv, n * n * n, n, Cv, N, 1, 0, dTen, 1, N, 0, dX1, 1, NV, 0, 1, s);
+        if (!r) r = polyeri_gemm_launch(mode, nv, no, n, dX1, 1, NV, NV * N, Co, 1, N, 0, dX2, 1, NV, NV * NO, n * n, s);
+        if (!r)
+            r = polyeri_gemm_launch(mode, nv * no, nv, n, dX2, 1, NV * NO, NV * NO * N, Cv, 1, N, 0, dX3, 1, NV * NO,
+                                    NV * NO * NV, n, s);
+        if (!r) r = polyeri_gemm_launch(mode, nv * no * nv, no, n, dX3, 1, NV * NO * NV, 0, Co, 1, N, 0, dT4, 1,
+                                        NV * NO * NV, 0, 1, s);
+        if (r) return fail(POLYERI_E_HIP, "MP2 transform launch failed");
+    }
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
     hipLaunchKernelGGL(k_mp2_energy, dim3(nblk(x4)), dim3(256), 0, s, dT4, dEps, no, nv, icore, iocc, dAcc);
     HIPCHK(hipGetLastError());
@@ -1403,4 +1420,14 @@ int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, in
     *e2t = acc[1];
     if (!std::isfinite(acc[0]) || !std::isfinite(acc[1])) return fail(POLYERI_E_NONFINITE, "non-finite E(2)");
     return 0;
+}
+
+int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
+                       int ivirtcore, double* e2s, double* e2t) {
+    return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, POLYERI_GEMM_ROCBLAS, e2s, e2t);
+}
+
+int polyeri_mp2_energy_ex(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
+                          int ivirtcore, int transform, double* e2s, double* e2t) {
+    return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, transform, e2s, e2t);
 }
