@@ -62,6 +62,24 @@ int polyeri_ctx_create(int npgs, int ncgs, const int *pax, const int *pay, const
                        const double *pgsx, const double *pgsy, const double *pgsz,
                        const double *zt, const double *cc, int natom, const int *iatom,
                        const double *xyz, const polyeri_opts *opts, polyeri_ctx **out);
+
+/* Periodic context (the reference's 1-D polymer path with HELIX = 0): the
+ * basis arrays of one unit cell, translation period along x (bohr), CUTOFF1
+ * (density / Fock cells -cel1..cel1) and CUTOFF2 (Coulomb lattice sum
+ * -cel2..cel2), as CEL1X / CEL2X of src/pfourindexdirect.f90:222-750.  The
+ * pair cells are reduced to the overlap range as REDUCED_CEL1X
+ * (src/ponee.f90:359-371). */
+int polyeri_ctx_create_periodic(int npgs, int ncgs, const int *pax, const int *pay, const int *paz,
+                                const double *pgsx, const double *pgsy, const double *pgsz, const double *zt,
+                                const double *cc, int natom, const int *iatom, const double *xyz, double period,
+                                int cel1, int cel2, const polyeri_opts *opts, polyeri_ctx **ctx);
+
+/* Periodic CALL FOURINDEX_ERI (src/pselfconsistentfield.f90:42 with
+ * cells): p_c, c_c, x_c hold 2 cel1 + 1 column-major ncgs x ncgs blocks,
+ * P_C(:,:,-CEL1X..CEL1X) in order; C_C(i,j,Q1) = sum (i_0 j_Q1|k_Q3 l_Q3+Q2)
+ * P_C(k,l,Q2), X_C(i,l,Q3+Q2) += (i_0 j_Q1|k_Q3 l_Q3+Q2) P_C(k,j,Q1-Q3)
+ * (src/pfourindexdirect.f90:691-704, 765-781). */
+int polyeri_jk_build_cells(polyeri_ctx *ctx, const double *p_c, double *c_c, double *x_c);
 void polyeri_ctx_destroy(polyeri_ctx *ctx);
 
 /* Drop-in for CALL FOURINDEX_ERI (src/pselfconsistentfield.f90:42): from

@@ -77,7 +77,7 @@ class Context:
     """A libpolyeri context for one geometry and basis (device resident)."""
 
     def __init__(self, basis, schwarz_tol=1e-12, prim_tol=1e-20, precision=FP64, mixed_cut=1e-5,
-                 device=0):
+                 device=0, period=None, cel1=0, cel2=0):
         if not isinstance(basis, RefBasis):
             raise TypeError("Context(basis): basis must come from read_basis()")
         self.basis = basis
@@ -87,11 +87,19 @@ class Context:
         o = Opts(schwarz_tol, prim_tol, precision, mixed_cut, device)
         h = ctypes.c_void_p()
         self._keep = (z, xyz)
-        check(lib().polyeri_ctx_create(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz), _p(b.pgsx),
-                                       _p(b.pgsy), _p(b.pgsz), _p(b.zt), _p(b.cc), len(b.atoms),
-                                       _p(z), _p(xyz), ctypes.byref(o), ctypes.byref(h)))
+        if period is None:
+            check(lib().polyeri_ctx_create(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz), _p(b.pgsx),
+                                           _p(b.pgsy), _p(b.pgsz), _p(b.zt), _p(b.cc), len(b.atoms),
+                                           _p(z), _p(xyz), ctypes.byref(o), ctypes.byref(h)))
+        else:
+            # the reference's periodic axis is x (src/pfourindexdirect.f90:262-270)
+            check(lib().polyeri_ctx_create_periodic(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz),
+                                                    _p(b.pgsx), _p(b.pgsy), _p(b.pgsz), _p(b.zt), _p(b.cc),
+                                                    len(b.atoms), _p(z), _p(xyz), float(period), int(cel1),
+                                                    int(cel2), ctypes.byref(o), ctypes.byref(h)))
         self.h = h
         self.ncgs = b.ncgs
+        self.period, self.cel1, self.cel2 = period, cel1, cel2
 
     def close(self):
         if getattr(self, "h", None):
@@ -118,6 +126,21 @@ class Context:
         else:
             check(lib().polyeri_jk_build(self.h, _p(p), _p(j), _p(k)))
         return j, k
+
+    def fourindex_eri_cells(self, p_cells):
+        """Periodic J and K cells (polyeri_jk_build_cells).  p_cells:
+        (2 cel1 + 1, ncgs, ncgs), cell Q at index Q + cel1, P(Q)[k, l] =
+        P(k in cell 0, l in cell Q); returns J, K in the same layout."""
+        if self.period is None:
+            raise PolyeriError(-1, "fourindex_eri_cells needs a periodic context")
+        ncell, n = 2 * self.cel1 + 1, self.ncgs
+        p = np.zeros((ncell, n, n))
+        for c in range(ncell):
+            p[c] = np.asarray(p_cells[c], dtype=np.float64).T        # column-major blocks
+        j = np.zeros((ncell, n, n))
+        k = np.zeros((ncell, n, n))
+        check(lib().polyeri_jk_build_cells(self.h, _p(p), _p(j), _p(k)))
+        return np.ascontiguousarray(j.transpose(0, 2, 1)), np.ascontiguousarray(k.transpose(0, 2, 1))
 
     def one_electron(self):
         s, t, v = self._mat(), self._mat(), self._mat()

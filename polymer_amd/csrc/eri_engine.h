@@ -64,6 +64,16 @@ struct AosKet {
     const PrimPair* p;
     POLY_HD PrimPair load(int q) const { return p[q]; }
 };
+// A ket translated by dx along x (periodic lattice sums): P moves, P - A not.
+struct ShiftKet {
+    const PrimPair* p;
+    double dx;
+    POLY_HD PrimPair load(int q) const {
+        PrimPair r = p[q];
+        r.Px += dx;
+        return r;
+    }
+};
 struct SoaKet {
     const double* p;
     int nk;

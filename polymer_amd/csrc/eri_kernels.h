@@ -13,7 +13,9 @@
 #include "eri_engine.h"
 
 // MODE_JK32: J/K of the quartets below the mixed-precision cut, recursion in FP32.
-enum { MODE_JK = 0, MODE_SCHWARZ = 1, MODE_TENSOR = 2, MODE_JK32 = 3 };
+// MODE_PBC: periodic lattice sums (pairs with cell offsets, no permutational
+// symmetry, work item = (bra, ket, Q3) with Q3 innermost).
+enum { MODE_JK = 0, MODE_SCHWARZ = 1, MODE_TENSOR = 2, MODE_JK32 = 3, MODE_PBC = 4 };
 
 struct PairMeta {
     int prim_off, nprim, fa, fb;
@@ -45,6 +47,12 @@ struct EriArgs {
     const double* ksoa;        // ket primitive pairs, bucket-interleaved fields (8 x kfs doubles)
     long long kfs;             // field stride of ksoa
     const int* bucket_soa;     // nbucket: offset of each ket bucket in ksoa
+    // periodic (MODE_PBC): nwork = items x n3, Q3 = t % n3 - cel2
+    int n3, cel1, cel2, both;  // both: digest every integral also with bra and ket roles exchanged
+    double period;
+    const int2* pcell;         // per pair {cell offset Q of the second shell, swap}
+    const double* Dc;          // (2 cel1 + 1) x nbf^2 density cells, P(Q)[k][l] = P(k_0, l_Q)
+    double *Jc, *Kc;           // jk_copies x (2 cel1 + 1) x nbf^2
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -208,8 +216,64 @@ __device__ __forceinline__ int bucket_of(const EriArgs& g, int pos) {
     return b;
 }
 
+// Periodic digestion of one integral (i_0 j_Q1 | k_Q3 l_Q3+Q2) (src/pfourindexdirect.f90:691-704):
+// J(Q1)[i][j] += v P(Q2)[k][l]; K(Q3+Q2)[i][l] += v P(Q1-Q3)[k][j] when both
+// cells are within [-cel1, cel1].
+__device__ __forceinline__ void pbc_add(const EriArgs& g, double* J, double* K, int q1, int q2, int q3, int fi,
+                                        int fj, int fk, int fl, double v) {
+    const size_t n = g.nbf, n2 = n * n;
+    const int c1 = g.cel1;
+    jk_add(&J[(size_t)(q1 + c1) * n2 + fi * n + fj], v * g.Dc[(size_t)(q2 + c1) * n2 + fk * n + fl], g.jk_copies);
+    const int qk = q3 + q2, qd = q1 - q3;
+    if (qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1)
+        jk_add(&K[(size_t)(qk + c1) * n2 + fi * n + fl], v * g.Dc[(size_t)(qd + c1) * n2 + fk * n + fj],
+               g.jk_copies);
+}
+
+// Integral (a b | c d) of a stored bra pair (first functions fa, fb; cells cb)
+// and a stored ket pair (fc, fd; ck) translated by s cells: as the reference
+// quartet with this bra and ket, and (both) as the one with the roles
+// exchanged, which is the same integral translated by -s cells.
+__device__ __forceinline__ void pbc_integral(const EriArgs& g, double* J, double* K, int2 cb, int2 ck, int s, int fa,
+                                             int fb, int fc, int fd, int a, int b, int c, int d, double v) {
+    const int fi = cb.y ? fb + b : fa + a, fj = cb.y ? fa + a : fb + b;
+    const int fk = ck.y ? fd + d : fc + c, fl = ck.y ? fc + c : fd + d;
+    pbc_add(g, J, K, cb.x, ck.x, s, fi, fj, fk, fl, v);
+    if (g.both) pbc_add(g, J, K, ck.x, cb.x, -s, fk, fl, fi, fj, v);
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
+    const long long u = t / g.n3;
+    const int s = (int)(t - u * g.n3) - g.cel2;
+    int bp, pos, b;
+    decode_work(g, u, bp, pos, b);
+    const int kp = g.ket_ids[pos];
+    const PairMeta mb = g.meta[bp], mk = g.meta[kp];
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    double v[NA * NB * NC * ND];
+    StoreCb<NB, NC, ND, double> st{v};
+    const ShiftKet kv{g.prims + mk.prim_off, s * g.period};
+    quartet<double, LA, LB, LC, LD>(g.prims + mb.prim_off, mb.nprim, kv, mk.nprim, g.ab[4 * bp], g.ab[4 * bp + 1],
+                                    g.ab[4 * bp + 2], g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boys, st);
+    const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * g.cel1 + 1) * g.nbf * g.nbf;
+    double* J = g.Jc + off;
+    double* K = g.Kc + off;
+    const int2 cb = g.pcell[bp], ck = g.pcell[kp];
+    for (int a = 0; a < NA; ++a)
+        for (int bb = 0; bb < NB; ++bb)
+            for (int c = 0; c < NC; ++c)
+                for (int d = 0; d < ND; ++d)
+                    pbc_integral(g, J, K, cb, ck, s, mb.fa, mb.fb, mk.fa, mk.fb, a, bb, c, d,
+                                 v[((a * NB + bb) * NC + c) * ND + d]);
+}
+
 template <int LA, int LB, int LC, int LD, int MODE>
 __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
+    if (MODE == MODE_PBC) {
+        pbc_item<LA, LB, LC, LD>(g, t);
+        return;
+    }
     int bp, kp, pos = 0, b = 0;
     if (MODE == MODE_SCHWARZ) {
         bp = kp = g.bra_ids[t];

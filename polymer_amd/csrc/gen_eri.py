@@ -237,7 +237,7 @@ def main(out_path):
 
 
 def write_kernel_units(out_dir):
-    """One translation unit per class (four modes each) plus the getter, so
+    """One translation unit per class (five modes each) plus the getter, so
     the class kernels compile in parallel."""
     _, cls = classes()
     os.makedirs(out_dir, exist_ok=True)
@@ -253,7 +253,7 @@ def write_kernel_units(out_dir):
             body += ["    (void)mode;", "    return nullptr;", "}", ""]
         else:
             body.append("    switch (mode) {")
-            for m in range(4):
+            for m in range(5):
                 body.append(f"    case {m}: return k_eri<{la}, {lb}, {lc}, {ld}, {m}>;")
             body += ["    }", "    return nullptr;", "}", ""]
         text = "\n".join(body)

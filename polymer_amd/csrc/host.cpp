@@ -335,9 +335,57 @@ void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pair
         }
 }
 
+// Periodic pairs: for every cell offset Q in [-rc1, rc1] and every ordered
+// shell pair (s1, s2), s1 in cell 0 and s2 in cell Q (translated by Q*period
+// along x).  Stored with the higher angular momentum first, so the pair is
+// (s1_0, s2_Q) or, with swap = 1, (s2_Q, s1_0).
+void build_pairs_cells(const IntBasis& ib, double primtol, double period, int rc1, std::vector<PairInfo>& pairs,
+                       std::vector<PrimPair>& prims, std::vector<int2_host>& cells) {
+    pairs.clear();
+    prims.clear();
+    cells.clear();
+    const double c0 = std::sqrt(2.0) * std::pow(POLY_PI, 1.25);
+    const int ns = (int)ib.shells.size();
+    for (int q = -rc1; q <= rc1; ++q)
+        for (int s1 = 0; s1 < ns; ++s1)
+            for (int s2 = 0; s2 < ns; ++s2) {
+                const bool swap = ib.shells[s1].L < ib.shells[s2].L;
+                const int a = swap ? s2 : s1, b = swap ? s1 : s2;
+                const Shell& A = ib.shells[a];
+                const Shell& B = ib.shells[b];
+                const double axs = A.x + (swap ? q * period : 0.0), bxs = B.x + (swap ? 0.0 : q * period);
+                PairInfo pi;
+                pi.a = a; pi.b = b;
+                pi.cls = pair_class(A.L, B.L);
+                pi.ABx = axs - bxs; pi.ABy = A.y - B.y; pi.ABz = A.z - B.z;
+                const double r2 = pi.ABx * pi.ABx + pi.ABy * pi.ABy + pi.ABz * pi.ABz;
+                pi.prim_off = (int)prims.size();
+                for (size_t i = 0; i < A.alpha.size(); ++i)
+                    for (size_t j = 0; j < B.alpha.size(); ++j) {
+                        const double al = A.alpha[i], be = B.alpha[j], z = al + be;
+                        const double e = std::exp(-al * be / z * r2);
+                        const double cc = A.coef[i] * B.coef[j];
+                        if (std::fabs(cc) * e < primtol) continue;
+                        PrimPair pp;
+                        pp.zeta = z;
+                        pp.K = cc * e * c0 / z;
+                        pp.Px = (al * axs + be * bxs) / z;
+                        pp.Py = (al * A.y + be * B.y) / z;
+                        pp.Pz = (al * A.z + be * B.z) / z;
+                        pp.PAx = pp.Px - axs; pp.PAy = pp.Py - A.y; pp.PAz = pp.Pz - A.z;
+                        prims.push_back(pp);
+                    }
+                pi.nprim = (int)prims.size() - pi.prim_off;
+                if (pi.nprim > 0) {
+                    pairs.push_back(pi);
+                    cells.push_back({q, swap ? 1 : 0});
+                }
+            }
+}
+
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
                       double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
-                      std::vector<int> bstart[], std::vector<WorkList>& out) {
+                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym) {
     out.clear();
     std::vector<int> rank(pairs.size(), 0);
     for (int X = 0; X < NPAIRCLASS; ++X)
@@ -388,7 +436,7 @@ void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<doub
                     for (int b = 0; b < nb; ++b) {
                         const int lo = bs[b], hi = bs[b + 1];
                         int lim = hi - lo;
-                        if (X == Y) {            // rank_j <= rank_i (a prefix: buckets are in rank order)
+                        if (X == Y && !nosym) {  // rank_j <= rank_i (a prefix: buckets are in rank order)
                             int b0 = lo, b1 = hi;
                             while (b0 < b1) {
                                 const int mid = (b0 + b1) / 2;

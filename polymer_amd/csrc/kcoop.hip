@@ -37,8 +37,14 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
     const int lane = threadIdx.x;
     const long long nw = work_count(g);
     for (long long t = blockIdx.x; t < nw; t += gridDim.x) {
-        int bp, kp;
-        if (MODE == MODE_SCHWARZ) {
+        int bp, kp, s3 = 0;
+        if (MODE == MODE_PBC) {
+            const long long u = t / g.n3;
+            s3 = (int)(t - u * g.n3) - g.cel2;
+            int pos, b;
+            decode_work(g, u, bp, pos, b);
+            kp = g.ket_ids[pos];
+        } else if (MODE == MODE_SCHWARZ) {
             bp = kp = g.bra_ids[t];
         } else if (g.items) {
             const int2 it = g.items[t];
@@ -53,7 +59,8 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
         for (int p = 0; p < mb.nprim; ++p) {
             const PrimPair bq = g.prims[mb.prim_off + p];
             for (int q = 0; q < mk.nprim; ++q) {
-                const PrimPair kq = g.prims[mk.prim_off + q];
+                PrimPair kq = g.prims[mk.prim_off + q];
+                if (MODE == MODE_PBC) kq.Px += s3 * g.period;
                 const double ze = bq.zeta + kq.zeta, ize = 1.0 / ze;
                 const double rho = bq.zeta * kq.zeta * ize;
                 const double PQx = bq.Px - kq.Px, PQy = bq.Py - kq.Py, PQz = bq.Pz - kq.Pz;
@@ -109,6 +116,14 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             }
             for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
             if (lane == 0) g.out[bp] = sqrt(m);
+        } else if (MODE == MODE_PBC) {
+            const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * g.cel1 + 1) * n * n;
+            const int2 cb = g.pcell[bp], ck = g.pcell[kp];
+            for (int k = lane; k < d.nfin; k += 64) {
+                const uint32_t f = g_prog_aux[d.fin_off + k];
+                pbc_integral(g, g.Jc + off, g.Kc + off, cb, ck, s3, fa, fb, fc, fd, (f >> 16) & 15, (f >> 20) & 15,
+                             (f >> 24) & 15, f >> 28, (double)sl[f & 0xffff]);
+            }
         } else if (MODE == MODE_TENSOR) {
             TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
             for (int k = lane; k < d.nfin; k += 64) {
@@ -205,6 +220,8 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
         hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
     else if (mode == MODE_JK32)
         hipLaunchKernelGGL(k_eri_coop<MODE_JK32>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+    else if (mode == MODE_PBC)
+        hipLaunchKernelGGL(k_eri_coop<MODE_PBC>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
     else if (mode == MODE_SCHWARZ)
         hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
     else

@@ -228,6 +228,7 @@ struct ClassPairWork {
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
     LaunchCfg cfg;         // launch shape (autotuned or default)
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
+    int both;              // periodic lists of two different classes: both bra/ket roles
     int nbucket;
     int nrows;
     long long nwork;
@@ -299,6 +300,16 @@ struct polyeri_ctx {
     bool use_graph = true;
     std::vector<GraphEntry> graphs;
     bool tuned = false;
+    // periodic lattice sums along x (HELIX = 0): cells of the density and
+    // of J/K in [-cel1, cel1], Coulomb sum over Q3 in [-cel2, cel2], pair
+    // cells in [-rc1, rc1] (rc1: last cell with contracted overlap > 1e-15)
+    bool pbc = false;
+    double period = 0.0;
+    int cel1 = 0, cel2 = 0, rc1 = 0;
+    std::vector<int2_host> pcell;
+    int2* d_pcell = nullptr;
+    double *d_Dc = nullptr, *d_Jc = nullptr, *d_Kc = nullptr;
+    double *d_pcc = nullptr, *d_jcc = nullptr, *d_kcc = nullptr;
 };
 
 static void drop_graphs(polyeri_ctx* c) {
@@ -334,6 +345,10 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     if (nwork <= 0) return 0;
     EriArgs a{};
     a.ksoa = c->d_ksoa; a.kfs = (long long)c->prims.size(); a.bucket_soa = bsoa;
+    if (c->pbc) {
+        a.n3 = 2 * c->cel2 + 1; a.cel1 = c->cel1; a.cel2 = c->cel2; a.period = c->period;
+        a.pcell = c->d_pcell; a.Dc = c->d_Dc; a.Jc = c->d_Jc; a.Kc = c->d_Kc;
+    }
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
@@ -368,6 +383,28 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     return 0;
 }
 
+static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long nwork) {
+    EriArgs a{};
+    a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
+    a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = nwork;
+    a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket;
+    a.boys = c->d_boys; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
+    a.n3 = 2 * c->cel2 + 1; a.cel1 = c->cel1; a.cel2 = c->cel2; a.period = c->period; a.both = cw.both;
+    a.pcell = c->d_pcell; a.Dc = c->d_Dc; a.Jc = c->d_Jc; a.Kc = c->d_Kc;
+    if (nwork <= 0) return 0;
+    EriKernel k = kernel_for<MODE_PBC>(cw.la, cw.lb, cw.lc, cw.ld);
+    if (!k || ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld) > POLYERI_COOP_MIN_INTEGRALS) {
+        const int ns = polyeri_coop_nslot(cw.la, cw.lb, cw.lc, cw.ld);
+        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
+        if (polyeri_coop_launch(a, MODE_PBC, cw.la, cw.lb, cw.lc, cw.ld, ns, 4096, c->stream))
+            return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
+        return 0;
+    }
+    hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // One stream (one hardware queue) per device shared by every context: the
 // device's scratch reservation for the spilling high-l kernels is then made
 // once, not once per context.
@@ -398,6 +435,8 @@ static void free_ctx(polyeri_ctx* c) {
         if (w.d_rf) hipFree(w.d_rf);
         if (w.d_items) hipFree(w.d_items);
     }
+    hipFree(c->d_pcell); hipFree(c->d_Dc); hipFree(c->d_Jc); hipFree(c->d_Kc);
+    hipFree(c->d_pcc); hipFree(c->d_jcc); hipFree(c->d_kcc);
     hipFree(c->d_psh); hipFree(c->d_sh_f0); hipFree(c->d_sh_nf); hipFree(c->d_dsh); hipFree(c->d_counts);
     hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
     hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
@@ -411,10 +450,41 @@ static void free_ctx(polyeri_ctx* c) {
 
 void polyeri_ctx_destroy(polyeri_ctx* ctx) { free_ctx(ctx); }
 
+// Last cell offset q in [0, cel1] whose contracted overlap with cell 0 has
+// an element above 1e-15 (REDUCED_CEL1X, src/ponee.f90:359-371).
+static int reduced_cel1(const IntBasis& ib, double period, int cel1, const std::vector<double>& boys) {
+    int rc1 = 0;
+    for (int q = 1; q <= cel1; ++q) {
+        IntBasis two = ib;
+        for (const Shell& sh : ib.shells) {
+            Shell t = sh;
+            t.x += q * period;
+            t.f0 += ib.nbf;
+            two.shells.push_back(t);
+        }
+        two.nbf = 2 * ib.nbf;
+        std::vector<double> S, T, V;
+        one_electron(two, boys, S, T, V);
+        double mx = 0.0;
+        for (int a = 0; a < ib.nbf; ++a)
+            for (int b = 0; b < ib.nbf; ++b)
+                mx = std::max(mx, std::fabs(ib.scale[a] * ib.scale[b] * S[(size_t)a * two.nbf + ib.nbf + b]));
+        if (mx > 1.0e-15) rc1 = q;
+    }
+    return rc1;
+}
+
 static int build_ctx(polyeri_ctx* c) {
     std::string err;
     if (extract_shells(c->rb, c->ib, err)) return fail(POLYERI_E_BASIS, err);
-    build_pairs(c->ib, c->opts.prim_tol, c->pairs, c->prims);
+    if (c->pbc) {
+        std::vector<double> boys0;
+        boys_table(boys0);
+        c->rc1 = reduced_cel1(c->ib, c->period, c->cel1, boys0);
+        build_pairs_cells(c->ib, c->opts.prim_tol, c->period, c->rc1, c->pairs, c->prims, c->pcell);
+    } else {
+        build_pairs(c->ib, c->opts.prim_tol, c->pairs, c->prims);
+    }
     HIPCHK(hipSetDevice(c->device));
     int src = 0;
     c->stream = shared_stream(c->device, &src);
@@ -486,7 +556,8 @@ static int build_ctx(polyeri_ctx* c) {
     // class-pair work lists (host.cpp build_work_lists)
     std::vector<WorkList> wl;
     const double cut = c->opts.precision == POLYERI_MIXED ? c->opts.mixed_cut : 0.0;
-    build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, cut, c->class_ids, c->ket_ids, c->bstart, wl);
+    build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, c->pbc ? 0.0 : cut, c->class_ids, c->ket_ids, c->bstart,
+                     wl, c->pbc);
     for (int Y = 0; Y < NPAIRCLASS; ++Y) {
         if ((rc = dalloc(&c->d_ket_ids[Y], c->ket_ids[Y].size()))) return rc;
         if ((rc = dalloc(&c->d_bstart[Y], c->bstart[Y].size()))) return rc;
@@ -535,6 +606,7 @@ static int build_ctx(polyeri_ctx* c) {
         w.nrows = l.nrows;
         w.nwork = l.nwork;
         w.prec = l.prec;
+        w.both = c->pbc && l.X != l.Y;
         w.d_rf = nullptr;
         w.d_items = nullptr;
         w.prim_quartets = l.prim_quartets;
@@ -569,6 +641,18 @@ static int build_ctx(polyeri_ctx* c) {
         HIPCHK(hipMemcpy(c->d_sh_f0, f0.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_sh_nf, nf.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
         c->h_counts.assign(c->work.size(), 0);
+    }
+    if (c->pbc) {
+        const size_t ncell = 2 * (size_t)c->cel1 + 1, n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
+        if ((rc = dalloc(&c->d_pcell, c->pcell.size())) || (rc = dalloc(&c->d_Dc, ncell * n2)) ||
+            (rc = dalloc(&c->d_Jc, ncell * n2 * c->jk_copies)) || (rc = dalloc(&c->d_Kc, ncell * n2 * c->jk_copies)) ||
+            (rc = dalloc(&c->d_pcc, ncell * c2)) || (rc = dalloc(&c->d_jcc, ncell * c2)) ||
+            (rc = dalloc(&c->d_kcc, ncell * c2)))
+            return rc;
+        HIPCHK(hipMemcpy(c->d_pcell, c->pcell.data(), c->pcell.size() * sizeof(int2_host), hipMemcpyHostToDevice));
+        // work counts: every (bra, ket) item runs for 2 cel2 + 1 cells Q3
+        c->quartets = 0;
+        for (ClassPairWork& w : c->work) c->quartets += w.nwork * (2LL * c->cel2 + 1);
     }
     c->ev.resize(c->work.size() + 1);
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -626,6 +710,41 @@ int polyeri_ctx_create(int npgs, int ncgs, const int* pax, const int* pay, const
     return 0;
 }
 
+int polyeri_ctx_create_periodic(int npgs, int ncgs, const int* pax, const int* pay, const int* paz,
+                                const double* pgsx, const double* pgsy, const double* pgsz, const double* zt,
+                                const double* cc, int natom, const int* iatom, const double* xyz, double period,
+                                int cel1, int cel2, const polyeri_opts* opts, polyeri_ctx** out) {
+    if (!out || npgs <= 0 || ncgs <= 0 || !pax || !pay || !paz || !pgsx || !pgsy || !pgsz || !zt || !cc ||
+        natom <= 0 || !iatom || !xyz)
+        return fail(POLYERI_E_ARG, "polyeri_ctx_create_periodic: null or empty argument");
+    if (!(period > 0.0) || cel1 < 0 || cel2 < 0 || cel1 > 64 || cel2 > 256)
+        return fail(POLYERI_E_ARG, "polyeri_ctx_create_periodic: bad period or cutoffs");
+    *out = nullptr;
+    polyeri_ctx* c = new polyeri_ctx();
+    if (opts) c->opts = *opts; else default_opts(&c->opts);
+    c->device = c->opts.device;
+    c->pbc = true;
+    c->period = period;
+    c->cel1 = cel1;
+    c->cel2 = cel2;
+    RefBasis& rb = c->rb;
+    rb.natom = natom;
+    rb.iatom.assign(iatom, iatom + natom);
+    for (int i = 0; i < natom; ++i) {
+        rb.ax.push_back(xyz[3 * i]); rb.ay.push_back(xyz[3 * i + 1]); rb.az.push_back(xyz[3 * i + 2]);
+    }
+    rb.npgs = npgs; rb.ncgs = ncgs;
+    rb.pax.assign(pax, pax + npgs); rb.pay.assign(pay, pay + npgs); rb.paz.assign(paz, paz + npgs);
+    rb.pgsx.assign(pgsx, pgsx + npgs); rb.pgsy.assign(pgsy, pgsy + npgs); rb.pgsz.assign(pgsz, pgsz + npgs);
+    rb.zt.assign(zt, zt + npgs);
+    rb.cc.assign(cc, cc + (size_t)npgs * ncgs);
+    rb.pgs_atom.assign(npgs, 0);
+    int rc = build_ctx(c);
+    if (rc) { free_ctx(c); return rc; }
+    *out = c;
+    return 0;
+}
+
 int polyeri_basis_read(const char* path, int natom, const int* iatom, const double* xyz, int* npgs,
                        int* ncgs, int* npshell, int* pax, int* pay, int* paz, int* pgs_atom,
                        double* pgsx, double* pgsy, double* pgsz, double* zt, double* cc, double* cgsx,
@@ -653,6 +772,7 @@ int polyeri_basis_read(const char* path, int natom, const int* iatom, const doub
 // context's density buffer (zeroed: the cost does not depend on values).
 int polyeri_autotune(polyeri_ctx* ctx, int reps) {
     if (!ctx) return fail(POLYERI_E_ARG, "polyeri_autotune: null context");
+    if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_autotune: periodic context");
     if (reps < 1) reps = 1;
     HIPCHK(hipSetDevice(ctx->device));
     polyeri_ctx* c = ctx;
@@ -716,6 +836,27 @@ int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
 }
 
 // J/K from the context's device density (reference layout, ncgs^2).
+// Periodic: C_C(ref i, ref j, Q) = s_i s_j J(Q)[i][j] summed over the per-XCD
+// copies (no symmetrisation: the lattice sums run over every quartet).
+__global__ void k_unpack_cells(const double* __restrict__ J, const double* __restrict__ K, int nbf, int ncell,
+                               int copies, const int* __restrict__ ref, const double* __restrict__ sc, int nc,
+                               double* __restrict__ jc, double* __restrict__ kc) {
+    const size_t n2 = (size_t)nbf * nbf;
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2 * ncell) return;
+    const int cell = (int)(t / n2);
+    const size_t e = t % n2;
+    const int mu = (int)(e / nbf), nu = (int)(e % nbf);
+    double js = 0.0, ks = 0.0;
+    for (int x = 0; x < copies; ++x) {
+        js += J[((size_t)x * ncell + cell) * n2 + e];
+        ks += K[((size_t)x * ncell + cell) * n2 + e];
+    }
+    const size_t r = (size_t)cell * nc * nc + ref[mu] + (size_t)nc * ref[nu];
+    jc[r] = sc[mu] * sc[nu] * js;
+    kc[r] = sc[mu] * sc[nu] * ks;
+}
+
 __global__ void k_fill_u32(unsigned* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -774,6 +915,7 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
 // captured once per (density, J, K, screening) combination into a HIP graph
 // and replayed; the timing events stay outside the graph.
 static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol = 0.0) {
+    if (c->pbc) return fail(POLYERI_E_ARG, "periodic context: use polyeri_jk_build_cells");
     const bool screen = dens_tol > 0.0;
     if (screen)
         for (ClassPairWork& w : c->work)
@@ -859,6 +1001,54 @@ static int jk_build_host(polyeri_ctx* ctx, const double* p_c, double dens_tol, d
     return 0;
 }
 
+// Periodic J/K of the contracted density cells (reference layout, cell Q at
+// block Q + cel1) into the J/K cells.
+static int jk_device_cells(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc) {
+    const int nbf = c->ib.nbf, nc = c->rb.ncgs, ncell = 2 * c->cel1 + 1;
+    const size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int q = 0; q < ncell; ++q)
+        hipLaunchKernelGGL(k_pack_density, dim3(nblk(n2)), dim3(256), 0, c->stream, d_pc + q * c2, nc, c->d_ref,
+                           c->d_scale, nbf, c->d_Dc + q * n2);
+    const size_t nj = n2 * ncell * c->jk_copies;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(nj)), dim3(256), 0, c->stream, c->d_Jc, nj, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(nj)), dim3(256), 0, c->stream, c->d_Kc, nj, 0.0);
+    HIPCHK(hipGetLastError());
+    const long long n3 = 2LL * c->cel2 + 1;
+    for (size_t w = 0; w < c->work.size(); ++w) {
+        const ClassPairWork& cw = c->work[w];
+        if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
+        int rc = launch_class_pbc(c, cw, cw.nwork * n3);
+        if (rc) return rc;
+    }
+    if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2 * ncell)), dim3(256), 0, c->stream, d_jc, c2 * ncell, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2 * ncell)), dim3(256), 0, c->stream, d_kc, c2 * ncell, 0.0);
+    hipLaunchKernelGGL(k_unpack_cells, dim3(nblk(n2 * ncell)), dim3(256), 0, c->stream, c->d_Jc, c->d_Kc, nbf, ncell,
+                       c->jk_copies, c->d_ref, c->d_scale, nc, d_jc, d_kc);
+    HIPCHK(hipGetLastError());
+    c->screened_last = false;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return 0;
+}
+
+int polyeri_jk_build_cells(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
+    if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build_cells: null argument");
+    if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_jk_build_cells: not a periodic context");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->rb.ncgs * ctx->rb.ncgs * (2 * ctx->cel1 + 1);
+    HIPCHK(hipMemcpyAsync(ctx->d_pcc, p_c, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = jk_device_cells(ctx, ctx->d_pcc, ctx->d_jcc, ctx->d_kcc);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c_c, ctx->d_jcc, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(x_c, ctx->d_kcc, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = jk_collect_timing(ctx))) return rc;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(c_c[i]) || !std::isfinite(x_c[i])) return fail(POLYERI_E_NONFINITE, "non-finite J/K");
+    return 0;
+}
+
 int polyeri_jk_build(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
     if (!ctx || !p_c || !c_c || !x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build: null argument");
     return jk_build_host(ctx, p_c, 0.0, c_c, x_c);
@@ -928,6 +1118,7 @@ int polyeri_one_electron(polyeri_ctx* ctx, double* s_c, double* t_c, double* n_c
 
 // Full internal ERI tensor on the device (nbf^4), screened like the J/K build.
 static int tensor_device(polyeri_ctx* c, double* d_t) {
+    if (c->pbc) return fail(POLYERI_E_ARG, "periodic context: no molecular ERI tensor");
     size_t n = c->ib.nbf;
     size_t n4 = n * n * n * n;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n4)), dim3(256), 0, c->stream, d_t, n4, 0.0);

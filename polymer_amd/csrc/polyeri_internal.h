@@ -85,9 +85,18 @@ void boys_table(std::vector<double>& tab);       // BOYS_NT x BOYS_NM
 // class_ids[X] must hold the pairs of class X in Schwarz order (bra rows);
 // fills ket_ids / bstart (bucketed ket order) and the lists.  cut > tol
 // splits every class pair into an FP64 list and an FP32 list (see WorkList).
+// nosym: every ket of class Y for every bra of class X (no rank_j <= rank_i
+// restriction), for the periodic lattice sums.
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
                       double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
-                      std::vector<int> bstart[], std::vector<WorkList>& out);
+                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym = false);
+struct int2_host {
+    int x, y;
+};
+// Periodic pairs (cell offset Q in [-rc1, rc1] of the second shell), cells[i]
+// = {Q, swap}: swap = 1 when the stored first shell is the one in cell Q.
+void build_pairs_cells(const IntBasis& ib, double primtol, double period, int rc1, std::vector<PairInfo>& pairs,
+                       std::vector<PrimPair>& prims, std::vector<int2_host>& cells);
 // S, T, V in the internal ordering (nbf x nbf, row-major, symmetric).
 void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vector<double>& S,
                   std::vector<double>& T, std::vector<double>& V);

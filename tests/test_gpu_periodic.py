@@ -1,0 +1,70 @@
+"""Periodic direct J/K (lattice sums along x, HELIX = 0) against the oracle.
+
+polyeri_jk_build_cells vs the oracle's restatement of the periodic loop nest
+of src/pfourindexdirect.f90:222-750 (itself pinned against a supercell ERI
+tensor by tests/test_oracle_periodic.py), unscreened (threshold 0) so both
+sides evaluate every quartet: 1e-10 absolute on J and K of every cell.
+Cases: an H2 chain in 6-31G** (s and p shells), a Ne chain in cc-pVDZ
+(d shells: the wavefront-per-quartet path) and the polyethylene unit cell
+C2H4 (period 2.55 A along x) in STO-3G.  Cells within CUTOFF1 overlap, so
+the reduced cell range equals CUTOFF1 on both sides."""
+import numpy as np
+import pytest
+
+import oracle
+import polymer_amd as pa
+from tests.conftest import basis_path
+
+pytestmark = pytest.mark.gpu
+
+ANG = 1.0 / 0.529177249
+
+
+def _h2_chain(period):
+    return [(1, 0.0, 0.0, 0.0), (1, 1.4, 0.1, -0.05)]
+
+
+def _ne_chain(period):
+    return [(10, 0.0, 0.2, 0.0)]
+
+
+def _pe_cell():
+    h = 0.4229
+    return [(6, 0.0, 0.0, h * ANG), (6, 1.275 * ANG, 0.0, -h * ANG),
+            (1, 0.0, 0.876 * ANG, 1.071 * ANG), (1, 0.0, -0.876 * ANG, 1.071 * ANG),
+            (1, 1.275 * ANG, 0.876 * ANG, -1.071 * ANG), (1, 1.275 * ANG, -0.876 * ANG, -1.071 * ANG)]
+
+
+def _densities(nc, real, cel1, seed):
+    rng = np.random.default_rng(seed)
+    P = {}
+    a = rng.standard_normal((nc, nc))
+    P[0] = 0.1 * (a + a.T)
+    for q in range(1, cel1 + 1):
+        P[q] = 0.05 * rng.standard_normal((nc, nc)) / q
+        P[-q] = P[q].T.copy()
+    m = np.zeros((nc, nc))
+    m[np.ix_(real, real)] = 1.0
+    return np.array([P[q] * m for q in range(-cel1, cel1 + 1)])
+
+
+@pytest.mark.parametrize("case", ["h2_631gss", "ne_ccpvdz", "pe_sto3g"])
+def test_periodic_jk_matches_oracle(case):
+    if case == "h2_631gss":
+        basis, atoms, period, cel1, cel2 = "6-31Gss", _h2_chain(3.2), 3.2, 1, 2
+    elif case == "ne_ccpvdz":
+        basis, atoms, period, cel1, cel2 = "cc-pVDZ", _ne_chain(5.0), 5.0, 1, 1
+    else:
+        basis, atoms, period, cel1, cel2 = "STO-3G", _pe_cell(), 2.55 * ANG, 1, 2
+    print(f"case: periodic J/K {case} cel1 {cel1} cel2 {cel2}", flush=True)
+    b = pa.read_basis(basis, atoms)
+    real = list(b.real_functions)
+    pc = _densities(b.ncgs, real, cel1, 5)
+    ctx = pa.Context(b, schwarz_tol=0.0, period=period, cel1=cel1, cel2=cel2)
+    J, K = ctx.fourindex_eri_cells(pc)
+    ob = oracle.Basis(basis_path(basis), atoms)
+    Jr, Kr, _ = ob.fourindex_eri_cells(pc, period, cel1, cel1, cel2, 0.0)
+    dj, dk = np.max(np.abs(J - Jr)), np.max(np.abs(K - Kr))
+    print(f"max |dJ| {dj:.2e} |dK| {dk:.2e}  (|J| {np.max(np.abs(Jr)):.2e})", flush=True)
+    assert np.max(np.abs(Jr)) > 0 and np.max(np.abs(Kr)) > 0
+    assert dj < 1e-10 and dk < 1e-10
