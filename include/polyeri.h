@@ -80,6 +80,7 @@ int polyeri_ctx_create_periodic(int npgs, int ncgs, const int *pax, const int *p
  * P_C(k,l,Q2), X_C(i,l,Q3+Q2) += (i_0 j_Q1|k_Q3 l_Q3+Q2) P_C(k,j,Q1-Q3)
  * (src/pfourindexdirect.f90:691-704, 765-781). */
 int polyeri_jk_build_cells(polyeri_ctx *ctx, const double *p_c, double *c_c, double *x_c);
+
 void polyeri_ctx_destroy(polyeri_ctx *ctx);
 
 /* Drop-in for CALL FOURINDEX_ERI (src/pselfconsistentfield.f90:42): from
@@ -167,6 +168,18 @@ typedef struct polyeri_scf_result {
  * change, energy per iteration) may be NULL. */
 int polyeri_scf(polyeri_ctx *ctx, const polyeri_scf_opts *opts, double *p_c, double *eps,
                 double *co, double *trace, polyeri_scf_result *res);
+
+/* SELF_CONSISTENT_FIELD for a periodic context (1-D, HELIX = 0) with the
+ * k-point FOCK_BUILD of src/pfock.f90:180-470: 2 kvc k-points
+ * k a = pi IX / kvc, IX = -kvc .. kvc-1 (KPOINTS kvc; kvc = 0: the Gamma
+ * point only), Bloch sums of F and S over the cells, canonical
+ * orthogonalisation and Hermitian diagonalisation per k (rocSOLVER
+ * zheevd), one Fermi level over all k, P(Q) from the occupied bands, DIIS
+ * over all cells, energy per unit cell.  p_c: 2 cel1 + 1 blocks as in
+ * polyeri_jk_build_cells (guess on input when opts->guess == 1, converged
+ * density on return); eps: ncgs x 2 kvc band energies (may be NULL). */
+int polyeri_scf_periodic(polyeri_ctx *ctx, const polyeri_scf_opts *opts, int kvc, double *p_c, double *eps,
+                         double *trace, polyeri_scf_result *res);
 
 /* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
  * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the

@@ -1,0 +1,218 @@
+"""Periodic SCF of the reference (1-D, HELIX = 0, axis x), restated in numpy.
+
+TEST INFRASTRUCTURE ONLY (the oracle; the product never imports it).
+
+* one_electron_cells: S_C, T_C, N_C of cells Q in [-CEL1, CEL1] as blocks
+  (cell 0, cell Q) of the molecular one-electron matrices of a supercell
+  molecule spanning cells -CEL2..CEL2, so the nuclear attraction runs over
+  the nuclei of exactly those cells (src/ponee.f90:486-560), and the
+  nuclear repulsion per unit cell (src/pselfconsistentfield.f90:652-702);
+* reduced_cel1: REDUCED_CEL1X, the last cell whose contracted overlap with
+  cell 0 exceeds 1e-15 (src/ponee.f90:359-371);
+* bands / density: the k-point part of FOCK_BUILD (src/pfock.f90:180-470):
+  Bloch sums over the cells with phase exp(i pi IX Q / KVC), IX = -KVC ..
+  KVC-1, Hermitian part, C2S, canonical orthogonalisation with LINDEP,
+  diagonalisation, CO = C2S^T X C', one Fermi level over all k with weight
+  1/(2 KVC), P_out(L,K,Q) = Re sum_k w sum_l occ conj(CO(K,l)) CO(L,l) e^{ikQ};
+* scf: SELF_CONSISTENT_FIELD with DIIS every other iteration from IOPTN(16)
+  on the cell-stacked densities (src/pselfconsistentfield.f90:1-162,
+  src/pdensitymatrix.f90:131-339) and ENERGY_CALC summed over the cells
+  (src/pselfconsistentfield.f90:487-574); J/K from the oracle's periodic
+  FOURINDEX_ERI (oracle.c or_fourindex_eri_cells).
+"""
+import numpy as np
+
+import oracle
+
+BOLTZMANN = 1.0 / 3.1577464e5      # src/pmodules.f90:12
+
+
+def _supercell(atoms, period, cel2):
+    out = []
+    for c in range(-cel2, cel2 + 1):
+        out += [(z, x + c * period, y, w) for z, x, y, w in atoms]
+    return out
+
+
+def _real(b):
+    return [f for f in range(b.ncgs) if np.any(b.cc[f])]
+
+
+def one_electron_cells(basis_path, atoms, period, cel1, cel2):
+    if cel1 > cel2:
+        raise ValueError("CUTOFF1 > CUTOFF2 is not supported here")
+    unit = oracle.Basis(basis_path, atoms)
+    sup_atoms = _supercell(atoms, period, cel2)
+    sup = oracle.Basis(basis_path, sup_atoms)
+    _, s_c, t_c, n_c = sup.one_electron()
+    nat = len(atoms)
+    nc = unit.ncgs
+    # unit-cell function f of atom a -> supercell index in cell c
+    def per_atom(b, ats):
+        out = [[] for _ in ats]
+        for f in _real(b):
+            cen = np.array([b.cgsx[f], b.cgsy[f], b.cgsz[f]])
+            a = int(np.argmin([np.linalg.norm(cen - np.array(t[1:])) for t in ats]))
+            out[a].append(f)
+        return out
+    fu, fs = per_atom(unit, atoms), per_atom(sup, sup_atoms)
+    idx = {}
+    for a, fl in enumerate(fu):
+        for k, f in enumerate(fl):
+            for c in range(-cel2, cel2 + 1):
+                idx[c, f] = fs[(c + cel2) * nat + a][k]
+    S = np.zeros((2 * cel1 + 1, nc, nc))
+    T = np.zeros_like(S)
+    N = np.zeros_like(S)
+    real = _real(unit)
+    for q in range(-cel1, cel1 + 1):
+        for i in real:
+            for j in real:
+                I, J = idx[0, i], idx[q, j]
+                S[q + cel1, i, j] = s_c[I, J]
+                T[q + cel1, i, j] = t_c[I, J]
+                N[q + cel1, i, j] = n_c[I, J]
+    enuc = 0.0
+    for i, (zi, xi, yi, wi) in enumerate(atoms):
+        for j, (zj, xj, yj, wj) in enumerate(atoms):
+            for k in range(-cel2, cel2 + 1):
+                if k == 0 and i == j:
+                    continue
+                r = np.sqrt((xj + k * period - xi) ** 2 + (yj - yi) ** 2 + (wj - wi) ** 2)
+                enuc += 0.5 * zi * zj / r
+    return unit, S, T, N, enuc
+
+
+def reduced_cel1(S, cel1):
+    rc = 0
+    for q in range(0, cel1 + 1):
+        if np.max(np.abs(S[q + cel1])) > 1e-15:
+            rc = q
+    return rc
+
+
+def kphases(kvc):
+    return [np.pi * ix / kvc for ix in range(-kvc, max(0, kvc - 1) + 1)] if kvc > 0 else [0.0]
+
+
+def bloch(M, cel1, phi):
+    """Hermitian part of sum_Q M(Q) e^{i phi Q} (M[Q + cel1] = M(Q))."""
+    W = sum(M[q + cel1] * np.exp(1j * phi * q) for q in range(-cel1, cel1 + 1))
+    return 0.5 * (W + W.conj().T)
+
+
+def bands(F, S, c2s, cel1, kvc, lindep):
+    """Band energies (nk, nc) and Cartesian CO (nk, nc, nc) per k."""
+    nc = F.shape[1]
+    eps, cos, nind = [], [], []
+    for phi in kphases(kvc):
+        ws = c2s @ bloch(S, cel1, phi) @ c2s.T
+        wf = c2s @ bloch(F, cel1, phi) @ c2s.T
+        e, u = np.linalg.eigh(ws)
+        keep = e > lindep
+        X = u[:, keep] / np.sqrt(e[keep])
+        ek, ck = np.linalg.eigh(X.conj().T @ wf @ X)
+        co = np.zeros((nc, nc), dtype=complex)
+        co[:, :keep.sum()] = c2s.T @ (X @ ck)
+        ee = np.full(nc, 1.0e99)
+        ee[:keep.sum()] = ek
+        eps.append(ee)
+        cos.append(co)
+        nind.append(int(keep.sum()))
+    return np.array(eps), np.array(cos), nind
+
+
+def fermi_level(eps, nel, temphf):
+    nk = eps.shape[0]
+
+    def count(mu):
+        return np.sum(2.0 / (np.exp(np.clip((eps - mu) / BOLTZMANN / temphf, -700, 700)) + 1.0)) / nk - nel
+    mine, maxe = np.min(eps), np.max(eps)
+    f1, f2, f12 = mine, maxe, 0.5 * (mine + maxe)
+    for _ in range(100000):
+        n1, n2, n12 = count(f1), count(f2), count(f12)
+        if abs(n12) < 1e-10:
+            return f12
+        if n1 > 0 and n2 > 0:
+            f2, f1 = f1, f1 - 0.1
+        elif n1 < 0 and n2 < 0:
+            f1, f2 = f2, f2 + 0.1
+        elif n1 < 0 and n12 >= 0:
+            f2 = f12
+        elif n12 <= 0 and n2 > 0:
+            f1 = f12
+        else:
+            break
+        f12 = 0.5 * (f1 + f2)
+    return f12
+
+
+def density(eps, co, cel1, kvc, fermi, temphf):
+    nk, nc = eps.shape
+    occ = 2.0 / (np.exp(np.clip((eps - fermi) / BOLTZMANN / temphf, -700, 700)) + 1.0)
+    P = np.zeros((2 * cel1 + 1, nc, nc))
+    w = 1.0 / nk
+    for k, phi in enumerate(kphases(kvc)):
+        ws = (co[k].conj() * occ[k]) @ co[k].T          # WS(K,J) = sum_l occ conj(CO(K,l)) CO(J,l)
+        for q in range(-cel1, cel1 + 1):
+            # P_C_OUT(L,K,Q) += WS(L,K) e^{i phi Q} w
+            P[q + cel1] += np.real(ws * np.exp(1j * phi * q)) * w
+    return P
+
+
+def _lu_solve(B, rhs):
+    x = np.linalg.solve(B, rhs)
+    return x - np.linalg.solve(B, B @ x - rhs)
+
+
+def scf(basis_path, atoms, period, cel1, cel2, kvc, inttol=1e-12, scfconv=1e-8, scfrelax=1.0,
+        hfexchange=1.0, lindep=1e-6, temphf=0.0, scfcycles=50, diis=5, charge=0, spherical=False):
+    unit, S, T, N, enuc = one_electron_cells(basis_path, atoms, period, cel1, cel2)
+    rc1 = reduced_cel1(S, cel1)
+    nc = unit.ncgs
+    c2s = unit.c2s(spherical)
+    nel = sum(a[0] for a in atoms) - charge
+    temphf = temphf if temphf > 0 else 1.0e-4
+    # GUESS_DENSITYMATRIX in cell 0
+    P = np.zeros((2 * cel1 + 1, nc, nc))
+    for a, (z, x, y, w) in enumerate(atoms):
+        fl = [f for f in _real(unit) if (unit.cgsx[f], unit.cgsy[f], unit.cgsz[f]) == (x, y, w)]
+        for f in fl:
+            P[cel1, f, f] = z / len(fl)
+    hp, hd, pulay = {}, {}, np.zeros((scfcycles + 2, scfcycles + 2))
+    trace = []
+    res = None
+    for it in range(1, scfcycles + 1):
+        J, K, _ = unit.fourindex_eri_cells(P, period, rc1, cel1, cel2, inttol)
+        F = T + N + J - 0.5 * hfexchange * K
+        eps, co, nind = bands(F, S, c2s, cel1, kvc, lindep)
+        mu = fermi_level(eps, nel, temphf)
+        Pout = density(eps, co, cel1, kvc, mu, temphf)
+        hp[it], hd[it] = P.copy(), Pout - P
+        for i in range(max(1, it - 8), it + 1):
+            pulay[i, it] = pulay[it, i] = float(np.sum(hd[i] * hd[it]))
+        dchg = np.sqrt(pulay[it, it] / P.size)
+        if diis != 0 and it >= diis and (it - diis) % 2 == 0:
+            nt = min(8, it)
+            n = nt + 1
+            B = np.zeros((n, n))
+            rhs = np.zeros(n)
+            for i in range(nt):
+                for j in range(nt):
+                    B[i, j] = pulay[it - j, it - i]
+                B[nt, i] = B[i, nt] = -1.0
+            rhs[nt] = -1.0
+            cv = _lu_solve(B, rhs)
+            P = sum(cv[it - i] * hp[i] for i in range(it - nt + 1, it + 1))
+        else:
+            P = scfrelax * Pout + (1.0 - scfrelax) * P
+        ek, ev = float(np.sum(P * T)), float(np.sum(P * N))
+        ej, ex = 0.5 * float(np.sum(P * J)), -0.25 * hfexchange * float(np.sum(P * K))
+        etot = enuc + ek + ev + ej + ex
+        trace.append((dchg, etot))
+        res = dict(e_total=etot, e_nuc=enuc, e_kin=ek, e_nuc_att=ev, e_coul=ej, e_hfx=ex, fermi=mu,
+                   iterations=it, converged=dchg < scfconv, P=P, eps=eps, rc1=rc1, nind=nind)
+        if dchg < scfconv:
+            break
+    res["trace"] = np.array(trace)
+    return res

@@ -176,6 +176,34 @@ class Context:
             check(rc)
         return res
 
+    def self_consistent_field_periodic(self, kpoints=0, scfconv=1e-8, scfrelax=1.0, hfexchange=1.0,
+                                       lindep=1e-6, temphf=0.0, scfcycles=50, diis=5, charge=0,
+                                       spherical=False, guess=None):
+        """Periodic SCF with 2 * kpoints k-points (KPOINTS, src/pfock.f90:186;
+        0 = Gamma only).  Returns the result fields plus P (cells, as
+        fourindex_eri_cells) and eps (nk, ncgs)."""
+        if self.period is None:
+            raise PolyeriError(-1, "self_consistent_field_periodic needs a periodic context")
+        o = ScfOpts(scfconv, scfrelax, hfexchange, lindep, temphf, scfcycles, diis, charge,
+                    int(spherical), 0 if guess is None else 1, 0, 0.0)
+        ncell, n = 2 * self.cel1 + 1, self.ncgs
+        nk = 2 * kpoints if kpoints > 0 else 1
+        p = np.zeros((ncell, n, n))
+        if guess is not None:
+            for c in range(ncell):
+                p[c] = np.asarray(guess[c], dtype=np.float64).T
+        eps = np.zeros((nk, n))
+        trace = np.zeros(2 * scfcycles)
+        r = ScfResult()
+        rc = lib().polyeri_scf_periodic(self.h, ctypes.byref(o), int(kpoints), _p(p), _p(eps), _p(trace),
+                                        ctypes.byref(r))
+        res = {f: getattr(r, f) for f, _ in ScfResult._fields_}
+        res.update(P=np.ascontiguousarray(p.transpose(0, 2, 1)), eps=eps,
+                   trace=trace[:2 * max(r.iterations, 0)].reshape(-1, 2))
+        if rc != 0:
+            check(rc)
+        return res
+
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
         """E(2) singlet and triplet parts.  transform: None (rocBLAS dgemm),
         "f64" (own FP64 MFMA kernel) or "bf16x3" (own BF16-split MFMA kernel)."""

@@ -1622,3 +1622,375 @@ int polyeri_mp2_energy_ex(polyeri_ctx* ctx, const double* co, const double* eps,
                           int ivirtcore, int transform, double* e2s, double* e2t) {
     return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, transform, e2s, e2t);
 }
+
+// ---------------------------------------------------------------------------
+// Periodic SCF (1-D, HELIX = 0): SELF_CONSISTENT_FIELD with the k-point
+// FOCK_BUILD of src/pfock.f90:180-470
+// ---------------------------------------------------------------------------
+// S_C, T_C, N_C of every cell Q in [-cel1, cel1] (reference layout, cell Q at
+// block Q + cel1): the pair (i in cell 0, j in cell Q) from a two-cell basis,
+// the nuclear attraction summed over the nuclei of cells -cel2..cel2
+// (src/ponee.f90:486-560); and NUCLEAR_REPULSION_CALC per unit cell
+// (src/pselfconsistentfield.f90:652-702).
+static void one_electron_cells(const polyeri_ctx* c, std::vector<double>& S, std::vector<double>& T,
+                               std::vector<double>& N, double& enuc) {
+    const IntBasis& ib = c->ib;
+    const int nbf = ib.nbf, nc = c->rb.ncgs, ncell = 2 * c->cel1 + 1;
+    const size_t c2 = (size_t)nc * nc;
+    S.assign(c2 * ncell, 0.0); T.assign(c2 * ncell, 0.0); N.assign(c2 * ncell, 0.0);
+    std::vector<double> boys;
+    boys_table(boys);
+    std::vector<int> z;
+    std::vector<double> xyz;
+    for (int k = -c->cel2; k <= c->cel2; ++k)
+        for (int a = 0; a < c->rb.natom; ++a) {
+            z.push_back(c->rb.iatom[a]);
+            xyz.push_back(c->rb.ax[a] + k * c->period);
+            xyz.push_back(c->rb.ay[a]);
+            xyz.push_back(c->rb.az[a]);
+        }
+    for (int q = -c->cel1; q <= c->cel1; ++q) {
+        IntBasis two = ib;
+        for (const Shell& sh : ib.shells) {
+            Shell t = sh;
+            t.x += q * c->period;
+            t.f0 += nbf;
+            two.shells.push_back(t);
+        }
+        two.nbf = 2 * nbf;
+        two.natom_z = z;
+        two.atom_xyz = xyz;
+        std::vector<double> s2, t2, v2;
+        one_electron(two, boys, s2, t2, v2);
+        const size_t off = (size_t)(q + c->cel1) * c2;
+        for (int a = 0; a < nbf; ++a)
+            for (int b = 0; b < nbf; ++b) {
+                const size_t r = off + ib.ref_of_int[a] + (size_t)nc * ib.ref_of_int[b];
+                const size_t e = (size_t)a * two.nbf + nbf + b;
+                const double f = ib.scale[a] * ib.scale[b];
+                S[r] = f * s2[e]; T[r] = f * t2[e]; N[r] = f * v2[e];
+            }
+    }
+    enuc = 0.0;
+    for (int i = 0; i < c->rb.natom; ++i)
+        for (int j = 0; j < c->rb.natom; ++j)
+            for (int k = -c->cel2; k <= c->cel2; ++k) {
+                if (k == 0 && i == j) continue;
+                const double dx = c->rb.ax[j] + k * c->period - c->rb.ax[i], dy = c->rb.ay[j] - c->rb.ay[i],
+                             dz = c->rb.az[j] - c->rb.az[i];
+                enuc += 0.5 * c->rb.iatom[i] * c->rb.iatom[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+            }
+}
+
+// Mk[k](K,J) = sum_Q M(K,J,Q) exp(i pi IX Q / kvc), IX = k - kvc; Hermitian part.
+__global__ void k_bloch(const double* __restrict__ M, int nc, int ncell, int cel1, int nk, int kvc,
+                        rocblas_double_complex* __restrict__ Mk) {
+    const size_t c2 = (size_t)nc * nc;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c2 * nk) return;
+    const int k = (int)(t / c2);
+    const size_t e = t % c2;
+    const int K = (int)(e % nc), J = (int)(e / nc);
+    const double ph = kvc > 0 ? POLY_PI * (double)(k - kvc) / kvc : 0.0;
+    double re = 0.0, im = 0.0;
+    for (int q = -cel1; q <= cel1; ++q) {
+        const double a = M[(size_t)(q + cel1) * c2 + K + (size_t)nc * J];
+        const double b = M[(size_t)(q + cel1) * c2 + J + (size_t)nc * K];
+        double sn, cs;
+        sincos(ph * q, &sn, &cs);
+        // (M(K,J) e^{i ph q} + conj(M(J,K) e^{i ph q})) / 2
+        re += 0.5 * (a + b) * cs;
+        im += 0.5 * (a - b) * sn;
+    }
+    Mk[t] = rocblas_double_complex(re, im);
+}
+
+// F = T + V + J - hfx K / 2 elementwise over every cell (the Bloch sums
+// take the Hermitian part, src/pfock.f90:216-226)
+__global__ void k_fock_flat(const double* __restrict__ T, const double* __restrict__ V, const double* __restrict__ J,
+                            const double* __restrict__ K, double hfx, size_t n, double* __restrict__ F) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) F[t] = T[t] + V[t] + J[t] - 0.5 * hfx * K[t];
+}
+
+// P(L,K,Q) = w Re sum_k V_k(L,K) exp(-i pi IX Q / kvc), V_k = conj(W_k)
+__global__ void k_unbloch(const rocblas_double_complex* __restrict__ Wk, int nc, int ncell, int cel1, int nk, int kvc,
+                          double w, double* __restrict__ P) {
+    const size_t c2 = (size_t)nc * nc;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c2 * ncell) return;
+    const int q = (int)(t / c2) - cel1;
+    const size_t e = t % c2;
+    double s = 0.0;
+    for (int k = 0; k < nk; ++k) {
+        const double ph = kvc > 0 ? POLY_PI * (double)(k - kvc) / kvc : 0.0;
+        double sn, cs;
+        sincos(ph * q, &sn, &cs);
+        const rocblas_double_complex v = Wk[(size_t)k * c2 + e];
+        s += v.real() * cs + v.imag() * sn;
+    }
+    P[t] = w * s;
+}
+
+// Orthogonaliser columns: X(:,j) = U(:,j)/sqrt(e_j) for e_j > lindep, else 0.
+__global__ void k_orth_c(const rocblas_double_complex* __restrict__ U, const double* __restrict__ e, int nc, int nk,
+                         double lindep, rocblas_double_complex* __restrict__ X) {
+    const size_t c2 = (size_t)nc * nc;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c2 * nk) return;
+    const int k = (int)(t / c2), j = (int)((t % c2) / nc);
+    const double ev = e[(size_t)k * nc + j];
+    X[t] = ev > lindep ? U[t] * (1.0 / sqrt(ev)) : rocblas_double_complex(0.0, 0.0);
+}
+
+// Columns dropped by the orthogonaliser get a large diagonal so their
+// eigenvalues sort above every physical one.
+__global__ void k_shift_dropped(rocblas_double_complex* __restrict__ F, const double* __restrict__ e, int nc, int nk,
+                                double lindep) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc * nk) return;
+    const int k = t / nc, j = t % nc;
+    if (!(e[(size_t)k * nc + j] > lindep))
+        F[(size_t)k * nc * nc + j + (size_t)nc * j] = rocblas_double_complex(1.0e6, 0.0);
+}
+
+// CO(:,l) scaled by occupation: Y(K,l) = occ(k,l) CO(K,l)
+__global__ void k_scale_occ(const rocblas_double_complex* __restrict__ C, const double* __restrict__ occ, int nc,
+                            int nk, rocblas_double_complex* __restrict__ Y) {
+    const size_t c2 = (size_t)nc * nc;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= c2 * nk) return;
+    const int k = (int)(t / c2), l = (int)((t % c2) / nc);
+    Y[t] = C[t] * occ[(size_t)k * nc + l];
+}
+
+__global__ void k_real_to_complex(const double* __restrict__ a, size_t n, rocblas_double_complex* __restrict__ b) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = rocblas_double_complex(a[t], 0.0);
+}
+
+int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, double* p_c, double* eps_out,
+                         double* trace, polyeri_scf_result* res) {
+    if (!ctx || !o || !p_c || !res) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: null argument");
+    if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: not a periodic context");
+    if (o->scfcycles < 1 || kvc < 0 || kvc > 512) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: bad options");
+    auto t_start = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_blas(ctx))) return rc;
+    rocblas_handle h = ctx->blas;
+    hipStream_t s = ctx->stream;
+    const int nc = ctx->rb.ncgs, ncell = 2 * ctx->cel1 + 1, nk = kvc > 0 ? 2 * kvc : 1;
+    const size_t c2 = (size_t)nc * nc, nv = c2 * ncell, ck = c2 * nk;
+    // one-electron cells and nuclear repulsion (host), C2S
+    std::vector<double> Sh, Th, Nh, c2s;
+    double enuc = 0.0;
+    one_electron_cells(ctx, Sh, Th, Nh, enuc);
+    if ((rc = form_c2s(ctx->rb, o->spherical, c2s))) return rc;
+    DevBuf buf;
+    double *dS, *dT, *dN, *dF, *dP, *dPout, *dE, *dOcc, *dSE;
+    rocblas_double_complex *dSk, *dFk, *dW1, *dW2, *dX, *dC2S, *dCk;
+    int* dInfo;
+    double* dWk;
+    if ((rc = buf.get(&dS, nv)) || (rc = buf.get(&dT, nv)) || (rc = buf.get(&dN, nv)) || (rc = buf.get(&dF, nv)) ||
+        (rc = buf.get(&dP, nv)) || (rc = buf.get(&dPout, nv)) || (rc = buf.get(&dE, (size_t)nc * nk)) ||
+        (rc = buf.get(&dSE, (size_t)nc * nk)) || (rc = buf.get(&dOcc, (size_t)nc * nk)) || (rc = buf.get(&dSk, ck)) ||
+        (rc = buf.get(&dFk, ck)) || (rc = buf.get(&dW1, ck)) || (rc = buf.get(&dW2, ck)) || (rc = buf.get(&dX, ck)) ||
+        (rc = buf.get(&dC2S, c2)) || (rc = buf.get(&dCk, ck)) || (rc = buf.get(&dInfo, nk)) ||
+        (rc = buf.get(&dWk, (size_t)nc * nk)))
+        return rc;
+    const int HMAX = 9;
+    double *dHP, *dHD;
+    if ((rc = buf.get(&dHP, nv * HMAX)) || (rc = buf.get(&dHD, nv * HMAX))) return rc;
+    HIPCHK(hipMemcpy(dS, Sh.data(), nv * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dT, Th.data(), nv * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dN, Nh.data(), nv * 8, hipMemcpyHostToDevice));
+    {
+        double* dtmp;
+        if ((rc = buf.get(&dtmp, c2))) return rc;
+        HIPCHK(hipMemcpy(dtmp, c2s.data(), c2 * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_real_to_complex, dim3(nblk(c2)), dim3(256), 0, s, dtmp, c2, dC2S);
+    }
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    const rocblas_stride sc2 = (rocblas_stride)c2;
+    // S(k), C2S S(k) C2S^T, eigenvectors -> orthogonaliser X(k) (src/pfock.f90:236-281 per k)
+    hipLaunchKernelGGL(k_bloch, dim3(nblk(ck)), dim3(256), 0, s, dS, nc, ncell, ctx->cel1, nk, kvc, dSk);
+    RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc,
+                                        0, dSk, nc, sc2, &zero, dW1, nc, sc2, nk));
+    RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1,
+                                        nc, sc2, dC2S, nc, 0, &zero, dW2, nc, sc2, nk));
+    RBCHK(rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, nc, dW2, nc, sc2, dSE, nc,
+                                           dWk, nc, dInfo, nk));
+    hipLaunchKernelGGL(k_orth_c, dim3(nblk(ck)), dim3(256), 0, s, dW2, dSE, nc, nk, o->lindep, dX);
+    std::vector<double> se((size_t)nc * nk);
+    HIPCHK(hipMemcpyAsync(se.data(), dSE, se.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> nind(nk, 0);
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nc; ++j) nind[k] += se[(size_t)k * nc + j] > o->lindep;
+    int nel = 0;
+    for (int z : ctx->rb.iatom) nel += z;
+    nel -= o->charge;
+    if (nel % 2) return fail(POLYERI_E_ARG, "ODD NUMBER OF ELECTRONS");
+    // guess
+    std::vector<double> P(nv, 0.0);
+    if (o->guess == 1) {
+        std::copy(p_c, p_c + nv, P.begin());
+    } else {
+        // GUESS_DENSITYMATRIX in cell 0 (src/pdensitymatrix.f90:83-127)
+        std::vector<int> fatom(nc, -1);
+        for (int c = 0; c < nc; ++c) {
+            const int mu = ctx->ib.int_of_ref[c];
+            if (mu < 0) continue;
+            for (const Shell& sh : ctx->ib.shells)
+                if (mu >= sh.f0 && mu < sh.f0 + ncart(sh.L))
+                    for (int a = 0; a < ctx->rb.natom; ++a)
+                        if (sh.x == ctx->rb.ax[a] && sh.y == ctx->rb.ay[a] && sh.z == ctx->rb.az[a]) fatom[c] = a;
+        }
+        for (int a = 0; a < ctx->rb.natom; ++a) {
+            int L = 0;
+            for (int c = 0; c < nc; ++c) L += fatom[c] == a;
+            if (!L) return fail(POLYERI_E_ARG, "atom without basis functions");
+            for (int c = 0; c < nc; ++c)
+                if (fatom[c] == a) P[(size_t)ctx->cel1 * c2 + c + (size_t)nc * c] = (double)ctx->rb.iatom[a] / L;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dP, P.data(), nv * 8, hipMemcpyHostToDevice, s));
+    const double temphf = o->temphf > 0 ? o->temphf : 1.0e-4;
+    std::vector<double> pulay((size_t)(o->scfcycles + 2) * (o->scfcycles + 2), 0.0);
+    const int ld = o->scfcycles + 2;
+    std::vector<double> eps((size_t)nc * nk), occ((size_t)nc * nk);
+    double etot = 0, ek = 0, ev = 0, ej = 0, ex = 0, fermi = 0, jk_total = 0;
+    long long quartets = 0;
+    int converged = 0, iscf;
+    for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
+        if ((rc = jk_device_cells(ctx, dP, ctx->d_jcc, ctx->d_kcc))) break;
+        hipLaunchKernelGGL(k_fock_flat, dim3(nblk(nv)), dim3(256), 0, s, dT, dN, ctx->d_jcc, ctx->d_kcc,
+                           o->hfexchange, nv, dF);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_bloch, dim3(nblk(ck)), dim3(256), 0, s, dF, nc, ncell, ctx->cel1, nk, kvc, dFk);
+        // C2S F(k) C2S^T, X^H (.) X
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S,
+                                            nc, 0, dFk, nc, sc2, &zero, dW1, nc, sc2, nk));
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one,
+                                            dW1, nc, sc2, dC2S, nc, 0, &zero, dW2, nc, sc2, nk));
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dW2,
+                                            nc, sc2, dX, nc, sc2, &zero, dW1, nc, sc2, nk));
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, nc, nc,
+                                            nc, &one, dX, nc, sc2, dW1, nc, sc2, &zero, dFk, nc, sc2, nk));
+        hipLaunchKernelGGL(k_shift_dropped, dim3(nblk((size_t)nc * nk)), dim3(256), 0, s, dFk, dSE, nc, nk, o->lindep);
+        RBCHK(rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, nc, dFk, nc, sc2, dE, nc,
+                                               dWk, nc, dInfo, nk));
+        // C(k) = X C'(k), CO(k) = C2S^T C(k)
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dX, nc,
+                                            sc2, dFk, nc, sc2, &zero, dW1, nc, sc2, nk));
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, nc, nc, nc, &one,
+                                            dC2S, nc, 0, dW1, nc, sc2, &zero, dCk, nc, sc2, nk));
+        std::vector<int> info(nk, 0);
+        HIPCHK(hipMemcpyAsync(eps.data(), dE, eps.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(info.data(), dInfo, nk * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if ((rc = jk_collect_timing(ctx))) break;
+        jk_total += ctx->last_jk_ms * 1e-3;
+        quartets += ctx->quartets;   // every pair item runs over the 2 cel2 + 1 cells Q3
+        bool bad = false;
+        for (int k = 0; k < nk; ++k) bad |= info[k] != 0;
+        if (bad) { rc = fail(POLYERI_E_LIB, "EIGENVECTORS NOT CONVERGED"); break; }
+        for (int k = 0; k < nk; ++k)
+            for (int j = nind[k]; j < nc; ++j) eps[(size_t)k * nc + j] = 1.0e99;
+        // Fermi level over every k (src/pfock.f90:339-423), weights 1/nk
+        double mine = 1e99, maxe = -1e99;
+        for (double e : eps) { maxe = std::max(maxe, e); mine = std::min(mine, e); }
+        double f1 = mine, f2 = maxe, f12 = 0.5 * (mine + maxe);
+        auto count = [&](double mu) {
+            double nn = 0;
+            for (double e : eps) nn += 2.0 / (std::exp((e - mu) / kBoltzmann / temphf) + 1.0);
+            return nn / nk - nel;
+        };
+        for (int it = 0; it < 100000; ++it) {
+            const double n1 = count(f1), nn2 = count(f2), n12 = count(f12);
+            if (std::fabs(n12) < 1e-10) { fermi = f12; break; }
+            else if (n1 > 0 && nn2 > 0) { f2 = f1; f1 -= 0.1; }
+            else if (n1 < 0 && nn2 < 0) { f1 = f2; f2 += 0.1; }
+            else if (n1 < 0 && n12 >= 0) f2 = f12;
+            else if (n12 <= 0 && nn2 > 0) f1 = f12;
+            else break;
+            f12 = 0.5 * (f1 + f2);
+        }
+        for (size_t i = 0; i < eps.size(); ++i) occ[i] = 2.0 / (std::exp((eps[i] - fermi) / kBoltzmann / temphf) + 1.0);
+        HIPCHK(hipMemcpyAsync(dOcc, occ.data(), occ.size() * 8, hipMemcpyHostToDevice, s));
+        // W_k(K,J) = sum_l occ conj(CO(K,l)) CO(J,l): conj(CO) occ CO^T; P_out(L,K,Q) = Re sum w W_k(L,K) e^{ikQ}
+        hipLaunchKernelGGL(k_scale_occ, dim3(nblk(ck)), dim3(256), 0, s, dCk, dOcc, nc, nk, dW1);
+        // W2 = Y CO^H = conj(W_k), so Re(W_k e^{ikQ}) = Re(W2 e^{-ikQ})
+        RBCHK(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_conjugate_transpose, nc, nc,
+                                            nc, &one, dW1, nc, sc2, dCk, nc, sc2, &zero, dW2, nc, sc2, nk));
+        hipLaunchKernelGGL(k_unbloch, dim3(nblk(nv)), dim3(256), 0, s, dW2, nc, ncell, ctx->cel1, nk, kvc, 1.0 / nk,
+                           dPout);
+        HIPCHK(hipGetLastError());
+        // STORE_DENSITYMATRIX / DIIS / RELAX over all cells (src/pdensitymatrix.f90:131-339)
+        const int slot = (iscf - 1) % HMAX;
+        double* hp = dHP + nv * slot;
+        double* hd = dHD + nv * slot;
+        HIPCHK(hipMemcpyAsync(hp, dP, nv * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(hd, dPout, nv * 8, hipMemcpyDeviceToDevice, s));
+        const double mone = -1.0;
+        RBCHK(rocblas_daxpy(h, (int)nv, &mone, dP, 1, hd, 1));
+        RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+        double dchg = 0.0;
+        RBCHK(rocblas_ddot(h, (int)nv, hd, 1, hd, 1, &dchg));
+        pulay[iscf + (size_t)ld * iscf] = dchg;
+        for (int i = std::max(1, iscf - HMAX + 1); i < iscf; ++i) {
+            double v = 0.0;
+            RBCHK(rocblas_ddot(h, (int)nv, dHD + nv * ((i - 1) % HMAX), 1, hd, 1, &v));
+            pulay[i + (size_t)ld * iscf] = v;
+            pulay[iscf + (size_t)ld * i] = v;
+        }
+        dchg = std::sqrt(dchg / (double)nv);
+        if (o->diis != 0 && iscf >= o->diis && ((iscf - o->diis) % 2) == 0) {
+            int nt = std::min(8, iscf), n = nt + 1;
+            std::vector<double> B((size_t)n * n, 0.0), rhs(n, 0.0), cv;
+            for (int i = 0; i < nt; ++i) {
+                for (int j = 0; j < nt; ++j) B[j + (size_t)n * i] = pulay[(iscf - j) + (size_t)ld * (iscf - i)];
+                B[i + (size_t)n * nt] = -1.0;
+                B[nt + (size_t)n * i] = -1.0;
+            }
+            rhs[nt] = -1.0;
+            lu_solve(n, B, rhs, cv);
+            hipLaunchKernelGGL(k_fill, dim3(nblk(nv)), dim3(256), 0, s, dP, nv, 0.0);
+            for (int i = iscf - nt + 1; i <= iscf; ++i) {
+                double w = cv[iscf - i];
+                RBCHK(rocblas_daxpy(h, (int)nv, &w, dHP + nv * ((i - 1) % HMAX), 1, dP, 1));
+            }
+        } else {
+            double a = o->scfrelax, b = 1.0 - o->scfrelax;
+            RBCHK(rocblas_dscal(h, (int)nv, &b, dP, 1));
+            RBCHK(rocblas_daxpy(h, (int)nv, &a, dPout, 1, dP, 1));
+        }
+        // ENERGY_CALC over the cells (src/pselfconsistentfield.f90:487-574)
+        RBCHK(rocblas_ddot(h, (int)nv, dP, 1, dT, 1, &ek));
+        RBCHK(rocblas_ddot(h, (int)nv, dP, 1, dN, 1, &ev));
+        RBCHK(rocblas_ddot(h, (int)nv, dP, 1, ctx->d_jcc, 1, &ej));
+        RBCHK(rocblas_ddot(h, (int)nv, dP, 1, ctx->d_kcc, 1, &ex));
+        ej *= 0.5;
+        ex *= -0.25 * o->hfexchange;
+        etot = enuc + ek + ev + ej + ex;
+        if (!std::isfinite(etot)) { rc = fail(POLYERI_E_NONFINITE, "non-finite SCF energy"); break; }
+        if (trace) { trace[2 * (iscf - 1)] = dchg; trace[2 * (iscf - 1) + 1] = etot; }
+        if (dchg < o->scfconv) { converged = 1; break; }
+    }
+    if (rc) return rc;
+    if (iscf > o->scfcycles) iscf = o->scfcycles;
+    HIPCHK(hipMemcpyAsync(p_c, dP, nv * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (eps_out) std::copy(eps.begin(), eps.end(), eps_out);
+    memset(res, 0, sizeof(*res));
+    res->e_total = etot; res->e_nuc = enuc; res->e_kin = ek; res->e_nuc_att = ev;
+    res->e_coul = ej; res->e_hfx = ex; res->fermi = fermi;
+    res->iterations = iscf; res->converged = converged; res->n_indep = nind[0];
+    res->iocc = nel / 2;
+    res->jk_seconds = jk_total;
+    res->quartets = quartets;
+    res->scf_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (!converged) return fail(POLYERI_E_NOTCONV, "SCF CONVERGENCE NOT MET");
+    return 0;
+}

@@ -68,3 +68,30 @@ def test_periodic_jk_matches_oracle(case):
     print(f"max |dJ| {dj:.2e} |dK| {dk:.2e}  (|J| {np.max(np.abs(Jr)):.2e})", flush=True)
     assert np.max(np.abs(Jr)) > 0 and np.max(np.abs(Kr)) > 0
     assert dj < 1e-10 and dk < 1e-10
+
+
+@pytest.mark.parametrize("case", ["h2_chain_631gss", "pe_sto3g"])
+def test_periodic_scf_matches_oracle(case):
+    """polyeri_scf_periodic (k-point FOCK_BUILD on the device) vs the
+    oracle's numpy restatement (oracle/periodic.py, pinned by
+    tests/test_oracle_periodic.py): energy per cell 1e-8 Eh, same
+    iteration count, band energies 1e-7."""
+    from oracle import periodic
+    if case == "h2_chain_631gss":
+        basis, atoms, period, cel1, cel2, kvc = "6-31Gss", _h2_chain(4.0), 4.0, 2, 3, 2
+    else:
+        basis, atoms, period, cel1, cel2, kvc = "STO-3G", _pe_cell(), 2.55 * ANG, 1, 1, 2
+    print(f"case: periodic SCF {case} cel1 {cel1} cel2 {cel2} kpoints {2 * kvc}", flush=True)
+    b = pa.read_basis(basis, atoms)
+    ctx = pa.Context(b, schwarz_tol=0.0, period=period, cel1=cel1, cel2=cel2)
+    r = ctx.self_consistent_field_periodic(kpoints=kvc, scfconv=1e-8, diis=2, scfcycles=60)
+    ref = periodic.scf(basis_path(basis), atoms, period, cel1, cel2, kvc, inttol=0.0, scfconv=1e-8, diis=2,
+                       scfcycles=60)
+    print(f"E gpu {r['e_total']:.12f} oracle {ref['e_total']:.12f} iterations {r['iterations']} / "
+          f"{ref['iterations']}", flush=True)
+    assert r["converged"] and ref["converged"]
+    assert abs(r["e_total"] - ref["e_total"]) < 1e-8
+    assert r["iterations"] == ref["iterations"]
+    ea, eb = np.sort(r["eps"].ravel()), np.sort(ref["eps"].ravel())
+    fin = eb < 1e90
+    assert np.max(np.abs(ea[fin] - eb[fin])) < 1e-7

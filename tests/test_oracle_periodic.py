@@ -128,3 +128,50 @@ def test_periodic_isolated_cell_limit():
     img = sum(2.0 / (q * period) for q in range(1, cel2 + 1)) * q_el * s
     rel = np.max(np.abs(J[cel1] - jm - img)) / np.max(np.abs(img))
     assert rel < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# periodic SCF (oracle/periodic.py)
+# ---------------------------------------------------------------------------
+def test_periodic_scf_isolated_limit_is_molecular():
+    """Period 100 bohr, CUTOFF1 = CUTOFF2 = 0, Gamma only: the periodic SCF
+    is the molecular SCF (same energy, same iteration count)."""
+    from oracle import periodic
+    atoms = [(1, 0.0, 0.0, 0.0), (1, 1.4, 0.0, 0.0), (1, 0.3, 2.8, 0.1), (1, 0.4, 4.2, 0.0)]
+    r = periodic.scf(basis_path("6-31Gss"), atoms, 100.0, 0, 0, 0, inttol=1e-12, scfconv=1e-9, diis=2,
+                     scfcycles=60)
+    ob = oracle.Basis(basis_path("6-31Gss"), atoms)
+    m = ob.scf(inttol=1e-12, scfconv=1e-9, diis=2, scfcycles=60, direct=True)
+    assert r["converged"] and m["converged"]
+    assert abs(r["e_total"] - m["e_total"]) < 1e-10
+    assert r["iterations"] == m["iterations"]
+
+
+def test_band_folding_identity():
+    """Bands of a chain at 2K k-points of its primitive cell equal the bands
+    of the doubled cell (built from the same cell matrices) at K k-points:
+    checks the Bloch phase convention, the Hermitian part and the k grid."""
+    from oracle import periodic
+    rng = np.random.default_rng(8)
+    n, cel1 = 3, 2
+    F = rng.standard_normal((2 * cel1 + 1, n, n)) * np.array([1.0, 0.5, 1.0, 0.5, 0.2])[:, None, None]
+    F[cel1] = 0.5 * (F[cel1] + F[cel1].T)
+    for q in range(1, cel1 + 1):
+        F[cel1 - q] = F[cel1 + q].T
+    S = np.zeros_like(F)
+    S[cel1] = np.eye(n)
+    # doubled cell: index (i, a), a in {0, 1} sub-cell; F2(Q)[(i,a),(j,b)] = F(2Q + b - a)[i, j]
+    c2 = (2 * cel1 + 1) // 2 + 1
+    F2 = np.zeros((2 * c2 + 1, 2 * n, 2 * n))
+    for Q in range(-c2, c2 + 1):
+        for a in range(2):
+            for b in range(2):
+                q = 2 * Q + b - a
+                if abs(q) <= cel1:
+                    F2[Q + c2][a * n:(a + 1) * n, b * n:(b + 1) * n] = F[q + cel1]
+    S2 = np.zeros_like(F2)
+    S2[c2] = np.eye(2 * n)
+    K = 2
+    e1, _, _ = periodic.bands(F, S, np.eye(n), cel1, 2 * K, 1e-8)
+    e2, _, _ = periodic.bands(F2, S2, np.eye(2 * n), c2, K, 1e-8)
+    assert np.allclose(np.sort(e1.ravel()), np.sort(e2.ravel()), atol=1e-12)
