@@ -130,6 +130,17 @@ int polyeri_one_electron(polyeri_ctx *ctx, double *s_c, double *t_c, double *n_c
  * and small systems (ncgs^4 doubles on host and device). */
 int polyeri_eri_tensor(polyeri_ctx *ctx, double *feri);
 
+/* STORE_FOURINDEX_ERI's disk format (src/pfourindexstore.f90:1-801): the
+ * contracted ERIs with |(ij|kl)| > inttol written to path as the reference's
+ * .ao.NNNNN records (Fortran unformatted, 1000 entries of packed cells,
+ * packed 1-based ijkl and the value per record); molecules, NCGS < 256.
+ * stored = the number of integrals written (the count the reference prints,
+ * e.g. data/H2O_CCSD_ccpVDZ_out:212). */
+int polyeri_store_fourindex_eri(polyeri_ctx *ctx, const char *path, double inttol, long long *stored);
+/* RESTORE_FOURINDEX_ERI's reader of the same format into FERI (ncgs^4,
+ * column-major), molecular (cell 0) records. */
+int polyeri_restore_fourindex_eri(const char *path, int ncgs, double *feri, long long *restored);
+
 typedef struct polyeri_scf_opts {
     double scfconv;     /* DOPTN(14) RMS density change                     */
     double scfrelax;    /* DOPTN(15)                                        */

@@ -49,6 +49,15 @@ class RefBasis:
         return np.nonzero(np.any(self.cc != 0.0, axis=1))[0]
 
 
+def restore_fourindex_eri(path, ncgs):
+    """Read a molecular .ao file (RESTORE_FOURINDEX_ERI) into an (ncgs,)*4
+    Fortran-order tensor; returns (tensor, number of integrals read)."""
+    t = np.zeros((ncgs,) * 4, order="F")
+    n = ctypes.c_longlong()
+    check(lib().polyeri_restore_fourindex_eri(str(path).encode(), int(ncgs), _p(t), ctypes.byref(n)))
+    return t, n.value
+
+
 def read_basis(basis, atoms):
     """READ3 (src/pread.f90:584-1482): atoms = [(Z, x, y, z)] in bohr."""
     L = lib()
@@ -175,6 +184,13 @@ class Context:
         if rc != 0:
             check(rc)
         return res
+
+    def store_fourindex_eri(self, path, inttol=0.0):
+        """Write the contracted ERIs above inttol in the reference's .ao record
+        format (STORE_FOURINDEX_ERI); returns the number stored."""
+        n = ctypes.c_longlong()
+        check(lib().polyeri_store_fourindex_eri(self.h, str(path).encode(), float(inttol), ctypes.byref(n)))
+        return n.value
 
     def self_consistent_field_periodic(self, kpoints=0, scfconv=1e-8, scfrelax=1.0, hfexchange=1.0,
                                        lindep=1e-6, temphf=0.0, scfcycles=50, diis=5, charge=0,

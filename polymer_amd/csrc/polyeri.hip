@@ -1994,3 +1994,112 @@ int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, d
     if (!converged) return fail(POLYERI_E_NOTCONV, "SCF CONVERGENCE NOT MET");
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// The reference's .ao ERI files (STORE_FOURINDEX_ERI / RESTORE_FOURINDEX_ERI)
+// ---------------------------------------------------------------------------
+// Fortran unformatted sequential records as gfortran / amdflang write them:
+// a 4-byte little-endian length before and after each payload.  A payload is
+// 1000 entries: ICASHE1X, ICASHE1Y, ICASHE1Z (int32: cells packed
+// ((Q1+CEL1)*256 + Q2+CEL1)*256 + Q3+CEL2), ICASHE2 (int32: ((I*256+J)*256+K)
+// *256+L, 1-based) and DCASHE (float64); unused entries of the last record
+// are -1 (src/pfourindexstore.f90:731-747, 781).
+namespace {
+const int kAoCache = 1000;
+
+struct AoWriter {
+    FILE* f = nullptr;
+    std::vector<int32_t> x, y, z, ijkl;
+    std::vector<double> v;
+    int n = 0;
+    int reset() {
+        std::fill(x.begin(), x.end(), -1); std::fill(y.begin(), y.end(), -1); std::fill(z.begin(), z.end(), -1);
+        std::fill(ijkl.begin(), ijkl.end(), -1); std::fill(v.begin(), v.end(), 0.0);
+        n = 0;
+        return 0;
+    }
+    bool flush() {
+        const int32_t len = kAoCache * (4 * 4 + 8);
+        bool ok = fwrite(&len, 4, 1, f) == 1;
+        ok = ok && fwrite(x.data(), 4, kAoCache, f) == (size_t)kAoCache;
+        ok = ok && fwrite(y.data(), 4, kAoCache, f) == (size_t)kAoCache;
+        ok = ok && fwrite(z.data(), 4, kAoCache, f) == (size_t)kAoCache;
+        ok = ok && fwrite(ijkl.data(), 4, kAoCache, f) == (size_t)kAoCache;
+        ok = ok && fwrite(v.data(), 8, kAoCache, f) == (size_t)kAoCache;
+        ok = ok && fwrite(&len, 4, 1, f) == 1;
+        reset();
+        return ok;
+    }
+};
+}  // namespace
+
+int polyeri_store_fourindex_eri(polyeri_ctx* ctx, const char* path, double inttol, long long* stored) {
+    if (!ctx || !path) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: null argument");
+    if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: molecular contexts only");
+    const size_t nc = ctx->rb.ncgs;
+    if (nc >= 256) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: NCGS must be below 256");
+    std::vector<double> feri(nc * nc * nc * nc);
+    int rc = polyeri_eri_tensor(ctx, feri.data());
+    if (rc) return rc;
+    AoWriter w;
+    w.f = fopen(path, "wb");
+    if (!w.f) return fail(POLYERI_E_ARG, std::string("cannot open ") + path);
+    w.x.resize(kAoCache); w.y.resize(kAoCache); w.z.resize(kAoCache); w.ijkl.resize(kAoCache); w.v.resize(kAoCache);
+    w.reset();
+    long long count = 0;
+    bool ok = true;
+    // I outermost, L innermost, as src/pfourindexstore.f90:728-750; molecule: all cells 0
+    for (size_t i = 0; i < nc && ok; ++i)
+        for (size_t j = 0; j < nc && ok; ++j)
+            for (size_t k = 0; k < nc && ok; ++k)
+                for (size_t l = 0; l < nc && ok; ++l) {
+                    const double v = feri[i + nc * (j + nc * (k + nc * l))];
+                    if (!(std::fabs(v) > inttol)) continue;
+                    w.x[w.n] = 0; w.y[w.n] = 0; w.z[w.n] = 0;
+                    w.ijkl[w.n] = (int32_t)((((i + 1) * 256 + (j + 1)) * 256 + (k + 1)) * 256 + (l + 1));
+                    w.v[w.n] = v;
+                    ++count;
+                    if (++w.n == kAoCache) ok = w.flush();
+                }
+    if (ok && w.n) ok = w.flush();
+    ok = (fclose(w.f) == 0) && ok;
+    if (!ok) return fail(POLYERI_E_ARG, std::string("write error on ") + path);
+    if (stored) *stored = count;
+    return 0;
+}
+
+int polyeri_restore_fourindex_eri(const char* path, int ncgs, double* feri, long long* restored) {
+    if (!path || !feri || ncgs <= 0 || ncgs >= 256) return fail(POLYERI_E_ARG, "polyeri_restore_fourindex_eri: bad argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(POLYERI_E_ARG, std::string("cannot open ") + path);
+    const size_t nc = ncgs;
+    std::fill(feri, feri + nc * nc * nc * nc, 0.0);
+    std::vector<int32_t> x(kAoCache), y(kAoCache), z(kAoCache), ijkl(kAoCache);
+    std::vector<double> v(kAoCache);
+    long long count = 0;
+    int rc = 0;
+    for (;;) {
+        int32_t len = 0, len2 = 0;
+        if (fread(&len, 4, 1, f) != 1) break;
+        if (len != kAoCache * 24) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
+        bool ok = fread(x.data(), 4, kAoCache, f) == (size_t)kAoCache && fread(y.data(), 4, kAoCache, f) == (size_t)kAoCache &&
+                  fread(z.data(), 4, kAoCache, f) == (size_t)kAoCache &&
+                  fread(ijkl.data(), 4, kAoCache, f) == (size_t)kAoCache && fread(v.data(), 8, kAoCache, f) == (size_t)kAoCache &&
+                  fread(&len2, 4, 1, f) == 1 && len2 == len;
+        if (!ok) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
+        for (int e = 0; e < kAoCache; ++e) {
+            if (x[e] == -1) break;
+            if (x[e] != 0 || y[e] != 0 || z[e] != 0) { rc = fail(POLYERI_E_ARG, "periodic .ao records are not read here"); break; }
+            const size_t i = ((ijkl[e] >> 24) & 255) - 1, j = ((ijkl[e] >> 16) & 255) - 1, k = ((ijkl[e] >> 8) & 255) - 1,
+                         l = (ijkl[e] & 255) - 1;
+            if (i >= nc || j >= nc || k >= nc || l >= nc) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
+            feri[i + nc * (j + nc * (k + nc * l))] = v[e];
+            ++count;
+        }
+        if (rc) break;
+    }
+    fclose(f);
+    if (rc) return rc;
+    if (restored) *restored = count;
+    return 0;
+}

@@ -169,3 +169,23 @@ def test_fortran_driver_golden(golden):
     e = float(re.search(r"TOTAL SCF ENERGY\s+=\s+(\S+)", out.stdout).group(1))
     assert abs(e - golden["H2O_ccpVDZ_spherical"]["e_total"]) < EH
     assert re.search(r"ITERATIONS =\s+30", out.stdout)
+
+
+def test_store_fourindex_eri_golden_count_and_mp2(golden, tmp_path):
+    """STORE_FOURINDEX_ERI's .ao file for H2O/cc-pVDZ at INTTOL 0: the number
+    of stored integrals is the shipped output's 188013
+    (data/H2O_CCSD_ccpVDZ_out:212), the file restores the tensor exactly,
+    and the MP2 energy from the restored integrals is the golden E(2)."""
+    g = golden["H2O_ccpVDZ_spherical"]
+    b = pa.read_basis("cc-pVDZ", M.H2O_GOLDEN)
+    ctx = pa.Context(b, schwarz_tol=0.0)
+    path = tmp_path / "h2o.ao.00000"
+    n = ctx.store_fourindex_eri(path, 0.0)
+    print(f"stored {n}", flush=True)
+    assert n == 188013
+    t, m = pa.restore_fourindex_eri(path, b.ncgs)
+    assert m == n
+    assert np.array_equal(t, ctx.eri_tensor())
+    r = ctx.self_consistent_field(scfconv=1e-11, lindep=1e-12, diis=2, spherical=True)
+    e2s, e2t = oracle.mp2(t, r["co"], r["eps"], 0, r["iocc"], r["n_indep"])
+    assert abs(e2s + e2t - g["e2_cc"]) < EH
