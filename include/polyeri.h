@@ -130,16 +130,34 @@ int polyeri_one_electron(polyeri_ctx *ctx, double *s_c, double *t_c, double *n_c
  * and small systems (ncgs^4 doubles on host and device). */
 int polyeri_eri_tensor(polyeri_ctx *ctx, double *feri);
 
+/* The lattice integrals of a periodic context (src/pfourindexstore.f90:
+ * 258-747): (I_0 J_Q1 | K_Q3 L_Q3+Q2) for Q1, Q2 in [-rc1, rc1] (rc1 =
+ * polyeri_reduced_cel1) and Q3 in [-cel2, cel2], as (2 rc1 + 1)^2 (2 cel2 + 1)
+ * column-major ncgs^4 FERI blocks, block ((Q1 + rc1)(2 rc1 + 1) + Q2 + rc1)
+ * (2 cel2 + 1) + Q3 + cel2; quartets below the Schwarz threshold are zero. */
+int polyeri_eri_tensor_cells(polyeri_ctx *ctx, double *t_c);
+/* REDUCED_CEL1X of a periodic context (src/ponee.f90:359-371): the last
+ * cell whose contracted overlap with cell 0 exceeds 1e-15; < 0 on error. */
+int polyeri_reduced_cel1(const polyeri_ctx *ctx);
+
 /* STORE_FOURINDEX_ERI's disk format (src/pfourindexstore.f90:1-801): the
  * contracted ERIs with |(ij|kl)| > inttol written to path as the reference's
  * .ao.NNNNN records (Fortran unformatted, 1000 entries of packed cells,
- * packed 1-based ijkl and the value per record); molecules, NCGS < 256.
- * stored = the number of integrals written (the count the reference prints,
- * e.g. data/H2O_CCSD_ccpVDZ_out:212). */
+ * packed 1-based ijkl and the value per record), NCGS < 256.  Periodic
+ * contexts write every cell triple (Q1, Q2 in [-rc1, rc1], Q3 in
+ * [-cel2, cel2]) in the reference's loop order.  stored = the number of
+ * integrals written (the count the reference prints, e.g.
+ * data/H2O_CCSD_ccpVDZ_out:212). */
 int polyeri_store_fourindex_eri(polyeri_ctx *ctx, const char *path, double inttol, long long *stored);
 /* RESTORE_FOURINDEX_ERI's reader of the same format into FERI (ncgs^4,
  * column-major), molecular (cell 0) records. */
 int polyeri_restore_fourindex_eri(const char *path, int ncgs, double *feri, long long *restored);
+/* The same reader for periodic records (src/pmp2energy.f90:207-252): cells
+ * unpacked with CUTOFF1 = cel1 and CUTOFF2 = cel2 into (2 cel1 + 1)^2
+ * (2 cel2 + 1) FERI blocks laid out as polyeri_eri_tensor_cells with rc1 =
+ * cel1; a cell outside the ranges is "INTEGRAL FILE HAS BEEN DEGRADED". */
+int polyeri_restore_fourindex_eri_cells(const char *path, int ncgs, int cel1, int cel2, double *t_c,
+                                        long long *restored);
 
 typedef struct polyeri_scf_opts {
     double scfconv;     /* DOPTN(14) RMS density change                     */
@@ -188,9 +206,27 @@ int polyeri_scf(polyeri_ctx *ctx, const polyeri_scf_opts *opts, double *p_c, dou
  * zheevd), one Fermi level over all k, P(Q) from the occupied bands, DIIS
  * over all cells, energy per unit cell.  p_c: 2 cel1 + 1 blocks as in
  * polyeri_jk_build_cells (guess on input when opts->guess == 1, converged
- * density on return); eps: ncgs x 2 kvc band energies (may be NULL). */
+ * density on return); eps: ncgs x 2 kvc band energies (may be NULL); co:
+ * the band coefficients CO(ncgs, ncgs, k) as complex (re, im) pairs,
+ * column-major per k (2 ncgs^2 2 kvc doubles, may be NULL); nall: IALL(k),
+ * the bands kept by LINDEP at each k (may be NULL). */
 int polyeri_scf_periodic(polyeri_ctx *ctx, const polyeri_scf_opts *opts, int kvc, double *p_c, double *eps,
-                         double *trace, polyeri_scf_result *res);
+                         double *trace, polyeri_scf_result *res, double *co, int *nall);
+
+/* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422) of a
+ * periodic run (src/pmp2energy.f90:1-548, HELIX = 0, MODULO 1): the stored
+ * lattice integrals (i_0 j_Q1 | k_Q3 l_Q3+Q2) of STORE_FOURINDEX_ERI
+ * (src/pfourindexstore.f90:258-747) computed on the device instead of read
+ * from .ao files, then the four quarter transforms for every virtual band
+ * of one k_b at once as complex products on this library's MFMA kernels
+ * (transform POLYERI_GEMM_MFMA_F64 or POLYERI_GEMM_MFMA_BF16X3), E(2)
+ * singlet and triplet per unit cell (normalised by (2 kvc)^3).  co, eps,
+ * nall as returned by polyeri_scf_periodic; bands icore..iocc-1 occupied,
+ * iocc..nall(k)-1-ivirtcore virtual; every k must hold iocc bands.
+ * seconds (may be NULL): wall time of the call. */
+int polyeri_mp2_energy_periodic(polyeri_ctx *ctx, int kvc, const double *co, const double *eps, const int *nall,
+                                int icore, int iocc, int ivirtcore, int transform, double *e2s, double *e2t,
+                                double *seconds);
 
 /* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
  * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the

@@ -48,6 +48,7 @@ def lib():
         L.or_fourindex_eri_rows.argtypes = [p, p, p, d, i, p, ll, p, p, p, p, p, p]
         L.or_fourindex_eri_rows.restype = ll
         L.or_eri_tensor.argtypes = [p, p, p]
+        L.or_eri_tensor_cells.argtypes = [p, d, i, i, p]
         L.or_fourindex_eri_cells_c.argtypes = [p, d, i, i, i, p, d, p, p]
         L.or_fourindex_eri_cells_c.restype = ll
         L.or_decontract.argtypes = [p, p, p]
@@ -201,6 +202,14 @@ class Basis:
         feri = np.zeros((m, m, m, m), order="F")
         lib().or_eri_tensor(self.h, _ptr(s_p), _ptr(feri))
         return feri
+
+    def eri_tensor_cells(self, period, rc1, cel2):
+        """T[Q1 + rc1, Q2 + rc1, Q3 + cel2, i, j, k, l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2),
+        the lattice integrals of STORE_FOURINDEX_ERI (unscreened)."""
+        m, n1, n3 = self.ncgs, 2 * rc1 + 1, 2 * cel2 + 1
+        T = np.zeros((n1, n1, n3, m, m, m, m))
+        lib().or_eri_tensor_cells(self.h, float(period), int(rc1), int(cel2), _ptr(T))
+        return T
 
     def c2s(self, spherical):
         m = self.ncgs

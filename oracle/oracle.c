@@ -906,6 +906,72 @@ void or_eri_tensor(const or_basis *b, const double *s_p, double *feri) {
     free(ncp); free(cid); free(cv); free(e); free(pr); free(maxc);
 }
 
+/* The lattice integrals STORE_FOURINDEX_ERI writes to the .ao files
+ * (src/pfourindexstore.f90:258-747, unscreened):
+ * T[Q1][Q2][Q3][i][j][k][l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2), Q1, Q2 in
+ * [-rc1, rc1], Q3 in [-cel2, cel2], contracted functions in reference
+ * order, the last index fastest. */
+void or_eri_tensor_cells(const or_basis *b, double period, int rc1, int cel2, double *T) {
+    boys_init();
+    pairtab_init();
+    int ns = b->npshell, nc = b->ncgs, np = b->npgs;
+    const int n1 = 2 * rc1 + 1, n3 = 2 * cel2 + 1;
+    const size_t nc4 = (size_t)nc * nc * nc * nc;
+    memset(T, 0, sizeof(double) * nc4 * n1 * n1 * n3);
+    size_t nsq = (size_t)ns * ns;
+    cpair *bra = (cpair *)malloc(sizeof(cpair) * nsq);
+    cpair q;
+    double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
+    int *ncp = (int *)calloc(np, sizeof(int));
+    int *cid = (int *)malloc(sizeof(int) * np * 4);
+    double *cv = (double *)malloc(sizeof(double) * np * 4);
+    for (int p = 0; p < np; ++p)
+        for (int c = 0; c < nc; ++c) {
+            double v = b->cc[c + (size_t)nc * p];
+            if (v != 0.0 && ncp[p] < 4) { cid[p * 4 + ncp[p]] = c; cv[p * 4 + ncp[p]] = v; ncp[p]++; }
+        }
+    const double c1f = 2.0 / sqrt(PI_REF);
+    for (int q1 = -rc1; q1 <= rc1; ++q1) {
+        for (size_t sij = 0; sij < nsq; ++sij)
+            cpair_make_shift(b, (int)(sij / ns), 0.0, (int)(sij % ns), q1 * period, &bra[sij]);
+        for (int q2 = -rc1; q2 <= rc1; ++q2)
+            for (int q3 = -cel2; q3 <= cel2; ++q3) {
+                double *tq = T + (((size_t)(q1 + rc1) * n1 + (q2 + rc1)) * n3 + (q3 + cel2)) * nc4;
+                for (size_t skl = 0; skl < nsq; ++skl) {
+                    cpair_make_shift(b, (int)(skl / ns), q3 * period, (int)(skl % ns), (q3 + q2) * period, &q);
+                    for (size_t sij = 0; sij < nsq; ++sij) {
+                        const cpair *ij = &bra[sij];
+                        double ze = ij->zeta + q.zeta, rho = ij->zeta * q.zeta / ze;
+                        double c2 = ij->ov * c1f * sqrt(rho) * q.ov;
+                        double dx = q.px - ij->px, dy = q.py - ij->py, dz = q.pz - ij->pz;
+                        os_quartet(ij, &q, c2, rho * (dx * dx + dy * dy + dz * dz), 0, e);
+                        int nij = ij->t->n, nk = q.t->n, ms = ij->lang + q.lang + 1;
+                        for (int a = 0; a < nij; ++a) {
+                            int pi = ij->pi[a], pj = ij->pj[a];
+                            if (!ncp[pi] || !ncp[pj]) continue;
+                            for (int c = 0; c < nk; ++c) {
+                                int pk = q.pi[c], pl = q.pj[c];
+                                if (!ncp[pk] || !ncp[pl]) continue;
+                                double v = e[((size_t)a * nk + c) * ms];
+                                for (int u = 0; u < ncp[pi]; ++u)
+                                    for (int w = 0; w < ncp[pj]; ++w)
+                                        for (int x = 0; x < ncp[pk]; ++x)
+                                            for (int y = 0; y < ncp[pl]; ++y) {
+                                                double f = cv[pi * 4 + u] * cv[pj * 4 + w] * cv[pk * 4 + x] *
+                                                           cv[pl * 4 + y];
+                                                size_t idx = (((size_t)cid[pi * 4 + u] * nc + cid[pj * 4 + w]) * nc +
+                                                              cid[pk * 4 + x]) * nc + cid[pl * 4 + y];
+                                                tq[idx] += f * v;
+                                            }
+                            }
+                        }
+                    }
+                }
+            }
+    }
+    free(ncp); free(cid); free(cv); free(e); free(bra);
+}
+
 /* ------------------------------------------------------------------ */
 /* One-electron integrals: src/ponee.f90:1-414, :418-728              */
 /* ------------------------------------------------------------------ */

@@ -99,6 +99,7 @@ long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int 
 long long or_fourindex_eri_cells_c(const or_basis *b, double period, int rc1, int cel1, int cel2,
                                    const double *p_c, double inttol, double *c_c, double *x_c);
 void or_eri_tensor(const or_basis *b, const double *s_p, double *feri);
+void or_eri_tensor_cells(const or_basis *b, double period, int rc1, int cel2, double *T);
 
 /* src/pdensitymatrix.f90:343-381 (DECONTRACT): P_P = CC^T P_C CC.         */
 void or_decontract(const or_basis *b, const double *p_c, double *p_p);

@@ -18,7 +18,13 @@ TEST INFRASTRUCTURE ONLY (the oracle; the product never imports it).
   on the cell-stacked densities (src/pselfconsistentfield.f90:1-162,
   src/pdensitymatrix.f90:131-339) and ENERGY_CALC summed over the cells
   (src/pselfconsistentfield.f90:487-574); J/K from the oracle's periodic
-  FOURINDEX_ERI (oracle.c or_fourindex_eri_cells).
+  FOURINDEX_ERI (oracle.c or_fourindex_eri_cells);
+* ao_tensor_cells / mp2_kpoints: the stored lattice integrals
+  (i_0 j_Q1 | k_Q3 l_Q3+Q2) of STORE_FOURINDEX_ERI (src/pfourindexstore.f90:
+  258-747, from a supercell ERI tensor) and the k-point MP2_ENERGY_ALG1
+  (src/pmp2energy.f90:187-548, HELIX = 0, MODULO 1), pinned by the
+  isolated-molecule limit and the band-folding identity
+  (tests/test_oracle_periodic.py).
 """
 import numpy as np
 
@@ -211,8 +217,135 @@ def scf(basis_path, atoms, period, cel1, cel2, kvc, inttol=1e-12, scfconv=1e-8, 
         etot = enuc + ek + ev + ej + ex
         trace.append((dchg, etot))
         res = dict(e_total=etot, e_nuc=enuc, e_kin=ek, e_nuc_att=ev, e_coul=ej, e_hfx=ex, fermi=mu,
-                   iterations=it, converged=dchg < scfconv, P=P, eps=eps, rc1=rc1, nind=nind)
+                   iterations=it, converged=dchg < scfconv, P=P, eps=eps, co=co, rc1=rc1, nind=nind)
         if dchg < scfconv:
             break
     res["trace"] = np.array(trace)
     return res
+
+
+# ---------------------------------------------------------------------------
+# k-point MP2: MP2_ENERGY_ALG1 with HELIX = 0 and MODULO 1
+# (src/pmp2energy.f90:187-548)
+# ---------------------------------------------------------------------------
+def ao_tensor_cells(basis_path, atoms, period, rc1, cel2):
+    """(i_0 j_Q1 | k_Q3 l_Q3+Q2) as T[Q1+rc1, Q2+rc1, Q3+cel2, i, j, k, l]
+    (contracted, reference layout) from the molecular ERI tensor of a
+    supercell spanning every cell the indices reach."""
+    lo = -(cel2 + rc1)
+    sup_atoms = _supercell(atoms, period, cel2 + rc1)
+    unit = oracle.Basis(basis_path, atoms)
+    sup = oracle.Basis(basis_path, sup_atoms)
+    g = sup.eri_tensor()
+    nat, nc = len(atoms), unit.ncgs
+
+    def per_atom(b, ats):
+        out = [[] for _ in ats]
+        for f in _real(b):
+            cen = np.array([b.cgsx[f], b.cgsy[f], b.cgsz[f]])
+            a = int(np.argmin([np.linalg.norm(cen - np.array(t[1:])) for t in ats]))
+            out[a].append(f)
+        return out
+    fu, fs = per_atom(unit, atoms), per_atom(sup, sup_atoms)
+    real = _real(unit)
+    idx = np.zeros((2 * (cel2 + rc1) + 1, nc), dtype=int)
+    for a, fl in enumerate(fu):
+        for k, f in enumerate(fl):
+            for c in range(lo, -lo + 1):
+                idx[c - lo, f] = fs[(c - lo) * nat + a][k]
+    n1, n3 = 2 * rc1 + 1, 2 * cel2 + 1
+    T = np.zeros((n1, n1, n3, nc, nc, nc, nc))
+    R = np.array(real)
+    for q1 in range(-rc1, rc1 + 1):
+        for q2 in range(-rc1, rc1 + 1):
+            for q3 in range(-cel2, cel2 + 1):
+                I = idx[0 - lo, R]
+                J = idx[q1 - lo, R]
+                K = idx[q3 - lo, R]
+                L = idx[q3 + q2 - lo, R]
+                blk = g[np.ix_(I, J, K, L)]
+                T[q1 + rc1, q2 + rc1, q3 + cel2][np.ix_(R, R, R, R)] = blk
+    return T
+
+
+def _kwrap(k, kvc):
+    """K1 - K2 + K3 folded back into -KVC .. KVC-1 (src/pmp2energy.f90:366-374)."""
+    if kvc == 0:
+        return 0
+    if k >= kvc:
+        k -= 2 * kvc
+    if k < -kvc:
+        k += 2 * kvc
+    return k
+
+
+def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0):
+    """E(2) singlet and triplet per unit cell (src/pmp2energy.f90:187-548).
+
+    T[Q1 + rc1, Q2 + rc1, Q3 + cel2, i, j, k, l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2)
+    (the records STORE_FOURINDEX_ERI writes, src/pfourindexstore.f90:727-747);
+    co[k] (ncgs x ncgs, columns = bands) and eps[k] for the k grid
+    IX = -kvc .. kvc-1 (kvc 0: Gamma only); nall[k] = IALL(k), the bands
+    kept by LINDEP.  Per (k_b, b) the four quarter transforms of the
+    reference, here for every virtual b of one k_b at once:
+      TNNNKK(I,J,K,Q1,Q3) = sum_{L,Q2} T CO(L,b,kb) e^{i kb (Q2+Q3)}      :253-258
+      TOK(j,kj)           = sum_{K,Q3} TNNNKK conj(CO(K,j,kj)) e^{-i kj Q3} :294-299
+      TVOKK(a,j,ka,kj)    = sum_{J,Q1} TOK CO(J,a,ka) e^{i ka Q1}           :323-329
+      TOVOKK(i,a,j,ka,kj) = sum_I TVOKK conj(CO(I,i,ki)), ki = ka-kj+kb     :350-386
+    then E2S/E2T over i, j (occupied), a, b (virtual) divided by (2 kvc)^3
+    (:400-433, :542-543)."""
+    nk = 2 * kvc if kvc > 0 else 1
+    ks = list(range(-kvc, kvc)) if kvc > 0 else [0]
+
+    def ph(n):
+        return np.exp(1j * np.pi * n / kvc) if kvc > 0 else 1.0 + 0j
+    q1s = np.arange(-rc1, rc1 + 1)
+    q3s = np.arange(-cel2, cel2 + 1)
+    occ = np.arange(icore, iocc)
+    for k in range(nk):
+        if nall[k] - ivirtcore < iocc:
+            raise ValueError("fewer bands than occupied orbitals at a k-point")
+    es, et = 0.0, 0.0
+    for kb_i, kb in enumerate(ks):
+        vb = np.arange(iocc, nall[kb_i] - ivirtcore)
+        if len(vb) == 0:
+            continue
+        # TNNNKK[q1, q3, i, j, k, b]
+        cb = co[kb_i][:, vb]
+        tn = np.zeros((len(q1s), len(q3s)) + T.shape[3:6] + (len(vb),), dtype=complex)
+        for a3, q3 in enumerate(q3s):
+            for a2, q2 in enumerate(q1s):
+                tn[:, a3] += np.einsum("qijkl,lb->qijkb", T[:, a2, a3], cb) * ph(kb * (q2 + q3))
+        # TOK[q1, i, j, kj, jo, b]
+        tok = np.zeros((len(q1s),) + T.shape[3:5] + (nk, len(occ), len(vb)), dtype=complex)
+        for kj_i, kj in enumerate(ks):
+            cj = co[kj_i][:, occ].conj()
+            for a3, q3 in enumerate(q3s):
+                tok[:, :, :, kj_i] += np.einsum("qijkb,ko->qijob", tn[:, a3], cj) * ph(-kj * q3)
+        # TVOKK[i, ka, a, kj, jo, b] (a padded to the largest virtual count)
+        nvmax = max(nall) - ivirtcore - iocc
+        tv = np.zeros((T.shape[3], nk, nvmax, nk, len(occ), len(vb)), dtype=complex)
+        for ka_i, ka in enumerate(ks):
+            va = np.arange(iocc, nall[ka_i] - ivirtcore)
+            for a1, q1 in enumerate(q1s):
+                tv[:, ka_i, :len(va)] += np.einsum("ijkob,ja->iakob", tok[a1], co[ka_i][:, va]) * ph(ka * q1)
+        # TOVOKK[io, ka, a, kj, jo, b]
+        tov = np.zeros((len(occ), nk, nvmax, nk, len(occ), len(vb)), dtype=complex)
+        for ka_i, ka in enumerate(ks):
+            for kj_i, kj in enumerate(ks):
+                ki_i = ks.index(_kwrap(ka - kj + kb, kvc))
+                tov[:, ka_i, :, kj_i] = np.einsum("iaob,im->maob", tv[:, ka_i, :, kj_i], co[ki_i][:, occ].conj())
+        for ka_i, ka in enumerate(ks):
+            nva = nall[ka_i] - ivirtcore - iocc
+            for kj_i, kj in enumerate(ks):
+                ki_i = ks.index(_kwrap(ka - kj + kb, kvc))
+                t1 = tov[:, ka_i, :nva, kj_i]                                  # (i, a, j, b)
+                t2 = np.transpose(tov[:, ka_i, :nva, ki_i], (2, 1, 0, 3))     # TOVOKK(j, a, i) at (ka, ki)
+                d = (eps[ki_i][occ][:, None, None, None] + eps[kj_i][occ][None, None, :, None]
+                     - eps[ka_i][iocc:iocc + nva][None, :, None, None] - eps[kb_i][vb][None, None, None, :])
+                aa = np.real(t1 * t1.conj())
+                ab = np.real(t1 * t2.conj())
+                es += float(np.sum((0.5 * aa + 0.5 * ab) / d))
+                et += float(np.sum((1.5 * aa - 1.5 * ab) / d))
+    norm = float(nk) ** 3
+    return es / norm, et / norm

@@ -58,6 +58,18 @@ def restore_fourindex_eri(path, ncgs):
     return t, n.value
 
 
+def restore_fourindex_eri_cells(path, ncgs, cel1, cel2):
+    """Read a periodic .ao file into T[Q1 + cel1, Q2 + cel1, Q3 + cel2, i, j,
+    k, l]; returns (tensor, number of integrals read)."""
+    n1, n3 = 2 * cel1 + 1, 2 * cel2 + 1
+    t = np.zeros((n1, n1, n3) + (ncgs,) * 4)
+    n = ctypes.c_longlong()
+    check(lib().polyeri_restore_fourindex_eri_cells(str(path).encode(), int(ncgs), int(cel1), int(cel2), _p(t),
+                                                    ctypes.byref(n)))
+    # blocks are column-major FERI(I,J,K,L)
+    return np.ascontiguousarray(t.transpose(0, 1, 2, 6, 5, 4, 3)), n.value
+
+
 def read_basis(basis, atoms):
     """READ3 (src/pread.f90:584-1482): atoms = [(Z, x, y, z)] in bohr."""
     L = lib()
@@ -185,6 +197,21 @@ class Context:
             check(rc)
         return res
 
+    @property
+    def reduced_cel1(self):
+        """REDUCED_CEL1X of a periodic context."""
+        r = lib().polyeri_reduced_cel1(self.h)
+        check(min(r, 0))
+        return r
+
+    def eri_tensor_cells(self):
+        """T[Q1 + rc1, Q2 + rc1, Q3 + cel2, i, j, k, l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2)."""
+        rc1 = self.reduced_cel1
+        n1, n3, n = 2 * rc1 + 1, 2 * self.cel2 + 1, self.ncgs
+        t = np.zeros((n1, n1, n3) + (n,) * 4)
+        check(lib().polyeri_eri_tensor_cells(self.h, _p(t)))
+        return np.ascontiguousarray(t.transpose(0, 1, 2, 6, 5, 4, 3))
+
     def store_fourindex_eri(self, path, inttol=0.0):
         """Write the contracted ERIs above inttol in the reference's .ao record
         format (STORE_FOURINDEX_ERI); returns the number stored."""
@@ -197,7 +224,8 @@ class Context:
                                        spherical=False, guess=None):
         """Periodic SCF with 2 * kpoints k-points (KPOINTS, src/pfock.f90:186;
         0 = Gamma only).  Returns the result fields plus P (cells, as
-        fourindex_eri_cells) and eps (nk, ncgs)."""
+        fourindex_eri_cells), eps (nk, ncgs), co (nk, ncgs, ncgs complex,
+        columns = bands) and nall (IALL per k)."""
         if self.period is None:
             raise PolyeriError(-1, "self_consistent_field_periodic needs a periodic context")
         o = ScfOpts(scfconv, scfrelax, hfexchange, lindep, temphf, scfcycles, diis, charge,
@@ -210,15 +238,45 @@ class Context:
                 p[c] = np.asarray(guess[c], dtype=np.float64).T
         eps = np.zeros((nk, n))
         trace = np.zeros(2 * scfcycles)
+        co = np.zeros((nk, n, n), dtype=np.complex128)      # [k][band][function]: column-major per k
+        nall = np.zeros(nk, dtype=np.int32)
         r = ScfResult()
         rc = lib().polyeri_scf_periodic(self.h, ctypes.byref(o), int(kpoints), _p(p), _p(eps), _p(trace),
-                                        ctypes.byref(r))
+                                        ctypes.byref(r), co.ctypes.data_as(ctypes.c_void_p),
+                                        nall.ctypes.data_as(ctypes.c_void_p))
         res = {f: getattr(r, f) for f, _ in ScfResult._fields_}
         res.update(P=np.ascontiguousarray(p.transpose(0, 2, 1)), eps=eps,
+                   co=np.ascontiguousarray(co.transpose(0, 2, 1)), nall=nall.astype(int),
                    trace=trace[:2 * max(r.iterations, 0)].reshape(-1, 2))
         if rc != 0:
             check(rc)
         return res
+
+    def mp2_energy_periodic(self, kpoints, co, eps, nall, icore, iocc, ivirtcore=0, transform="f64",
+                            timing=False):
+        """k-point E(2) singlet and triplet per unit cell (MP2_ENERGY_ALG1 of a
+        periodic run) with the lattice integrals and all four quarter
+        transforms on the device.  co: (nk, ncgs, ncgs) complex, columns =
+        bands; eps: (nk, ncgs); nall: IALL per k.  transform "f64" (FP64 MFMA)
+        or "bf16x3" (BF16-split MFMA, FP32 accumulation)."""
+        if self.period is None:
+            raise PolyeriError(-1, "mp2_energy_periodic needs a periodic context")
+        nk = 2 * kpoints if kpoints > 0 else 1
+        n = self.ncgs
+        co = np.asarray(co, dtype=np.complex128)
+        if co.shape != (nk, n, n):
+            raise PolyeriError(-1, f"co must have shape {(nk, n, n)}")
+        cf = np.ascontiguousarray(co.transpose(0, 2, 1))      # column-major per k
+        eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(nk, n)
+        na = np.ascontiguousarray(nall, dtype=np.int32).reshape(nk)
+        mode = {"f64": 1, "bf16x3": 2}[transform]
+        s, t, sec = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().polyeri_mp2_energy_periodic(self.h, int(kpoints), cf.ctypes.data_as(ctypes.c_void_p), _p(eps),
+                                                na.ctypes.data_as(ctypes.c_void_p), icore, iocc, ivirtcore, mode,
+                                                ctypes.byref(s), ctypes.byref(t), ctypes.byref(sec)))
+        if timing:
+            return s.value, t.value, sec.value
+        return s.value, t.value
 
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
         """E(2) singlet and triplet parts.  transform: None (rocBLAS dgemm),

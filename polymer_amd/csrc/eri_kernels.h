@@ -40,7 +40,7 @@ struct EriArgs {
     double* J;
     double* K;
     int nbf;
-    double* out;               // tensor (MODE_TENSOR) or Schwarz (MODE_SCHWARZ)
+    double* out;               // tensor (MODE_TENSOR, MODE_PBC) or Schwarz (MODE_SCHWARZ)
     int jk_copies;             // 1: one J/K (device-scope atomics); 8: one per XCD
     const int2* items;         // explicit (bra pair, ket position) items of a density-screened list, or null
     const unsigned* count;     // number of items (device), with items
@@ -230,6 +230,19 @@ __device__ __forceinline__ void pbc_add(const EriArgs& g, double* J, double* K, 
                g.jk_copies);
 }
 
+// Lattice tensor store (MODE_PBC with out set, src/pfourindexstore.f90:258-747):
+// T[i][Q1][j][Q3][k][Q2][l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2), Q1 and Q2 in
+// [-cel1, cel1] (the launch passes the reduced pair range), Q3 in
+// [-cel2, cel2].  Each integral has exactly one producer, so plain stores.
+__device__ __forceinline__ void pbc_store(const EriArgs& g, int q1, int q2, int q3, int fi, int fj, int fk, int fl,
+                                          double v) {
+    const size_t n = g.nbf, n1 = 2 * g.cel1 + 1;
+    size_t x = (size_t)fi * n1 + (q1 + g.cel1);
+    x = (x * n + fj) * g.n3 + (q3 + g.cel2);
+    x = (x * n + fk) * n1 + (q2 + g.cel1);
+    g.out[x * n + fl] = v;
+}
+
 // Integral (a b | c d) of a stored bra pair (first functions fa, fb; cells cb)
 // and a stored ket pair (fc, fd; ck) translated by s cells: as the reference
 // quartet with this bra and ket, and (both) as the one with the roles
@@ -238,6 +251,11 @@ __device__ __forceinline__ void pbc_integral(const EriArgs& g, double* J, double
                                              int fb, int fc, int fd, int a, int b, int c, int d, double v) {
     const int fi = cb.y ? fb + b : fa + a, fj = cb.y ? fa + a : fb + b;
     const int fk = ck.y ? fd + d : fc + c, fl = ck.y ? fc + c : fd + d;
+    if (g.out) {
+        pbc_store(g, cb.x, ck.x, s, fi, fj, fk, fl, v);
+        if (g.both) pbc_store(g, ck.x, cb.x, -s, fk, fl, fi, fj, v);
+        return;
+    }
     pbc_add(g, J, K, cb.x, ck.x, s, fi, fj, fk, fl, v);
     if (g.both) pbc_add(g, J, K, ck.x, cb.x, -s, fk, fl, fi, fj, v);
 }
@@ -346,6 +364,11 @@ enum { GEMM_F64 = 1, GEMM_BF16X3 = 2 };
 int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam, long sak, long sA, const double* B,
                         long sbk, long sbn, long sB, double* C, long scm, long scn, long sC, int batch,
                         hipStream_t stream);
+// Planar complex products C_b = A_b B_b (A real when Ai is null), same
+// strides for the real and imaginary planes (kgemm.hip).
+int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const double* Ai, long sam, long sak,
+                         long sA, const double* Br, const double* Bi, long sbk, long sbn, long sB, double* Cr,
+                         double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream);
 // Density screening (kscreen.hip): shell-block maxima of |D|, and the
 // compaction of one work list to the items whose bound
 // Q_ab Q_cd max(|D| over the six blocks the quartet reads) reaches tol.

@@ -383,7 +383,9 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     return 0;
 }
 
-static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long nwork) {
+// tensor: store the integrals T[i][Q1][j][Q3][k][Q2][l] instead of digesting
+// them into J/K (pair cells in the reduced range [-rc1, rc1]).
+static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long nwork, double* tensor = nullptr) {
     EriArgs a{};
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = nwork;
@@ -391,6 +393,7 @@ static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long n
     a.boys = c->d_boys; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
     a.n3 = 2 * c->cel2 + 1; a.cel1 = c->cel1; a.cel2 = c->cel2; a.period = c->period; a.both = cw.both;
     a.pcell = c->d_pcell; a.Dc = c->d_Dc; a.Jc = c->d_Jc; a.Kc = c->d_Kc;
+    if (tensor) { a.out = tensor; a.cel1 = c->rc1; }
     if (nwork <= 0) return 0;
     EriKernel k = kernel_for<MODE_PBC>(cw.la, cw.lb, cw.lc, cw.ld);
     if (!k || ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld) > POLYERI_COOP_MIN_INTEGRALS) {
@@ -1770,7 +1773,7 @@ __global__ void k_real_to_complex(const double* __restrict__ a, size_t n, rocbla
 }
 
 int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, double* p_c, double* eps_out,
-                         double* trace, polyeri_scf_result* res) {
+                         double* trace, polyeri_scf_result* res, double* co_out, int* nall_out) {
     if (!ctx || !o || !p_c || !res) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: null argument");
     if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: not a periodic context");
     if (o->scfcycles < 1 || kvc < 0 || kvc > 512) return fail(POLYERI_E_ARG, "polyeri_scf_periodic: bad options");
@@ -1983,6 +1986,11 @@ int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, d
     HIPCHK(hipMemcpyAsync(p_c, dP, nv * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (eps_out) std::copy(eps.begin(), eps.end(), eps_out);
+    if (co_out) {
+        HIPCHK(hipMemcpyAsync(co_out, dCk, ck * sizeof(rocblas_double_complex), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (nall_out) std::copy(nind.begin(), nind.end(), nall_out);
     memset(res, 0, sizeof(*res));
     res->e_total = etot; res->e_nuc = enuc; res->e_kin = ek; res->e_nuc_att = ev;
     res->e_coul = ej; res->e_hfx = ex; res->fermi = fermi;
@@ -1992,6 +2000,291 @@ int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, d
     res->quartets = quartets;
     res->scf_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     if (!converged) return fail(POLYERI_E_NOTCONV, "SCF CONVERGENCE NOT MET");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// k-point MP2 (MP2_ENERGY_ALG1 with HELIX = 0, MODULO 1; src/pmp2energy.f90:1-548)
+// ---------------------------------------------------------------------------
+// The stored lattice integrals of STORE_FOURINDEX_ERI (src/pfourindexstore.f90:
+// 258-747) computed on the device: T[i][Q1][j][Q3][k][Q2][l] = (i_0 j_Q1 |
+// k_Q3 l_Q3+Q2), Q1, Q2 in [-rc1, rc1], Q3 in [-cel2, cel2], internal
+// function order; quartets below the context's Schwarz threshold are zero.
+static int tensor_device_cells(polyeri_ctx* c, double* dT, size_t nt) {
+    hipLaunchKernelGGL(k_fill, dim3(nblk(nt)), dim3(256), 0, c->stream, dT, nt, 0.0);
+    HIPCHK(hipGetLastError());
+    const long long n3 = 2LL * c->cel2 + 1;
+    for (const ClassPairWork& cw : c->work) {
+        int rc = launch_class_pbc(c, cw, cw.nwork * n3, dT);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// CO(k) (reference order, complex, column-major per k) -> planar internal
+// C[k][f][p] = scale(f) CO(ref(f), p; k).
+__global__ void k_co_planar(const double2* __restrict__ co, int nc, const int* __restrict__ ref,
+                            const double* __restrict__ sc, int nbf, int nk, double* __restrict__ Cr,
+                            double* __restrict__ Ci) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nk * nbf * nc) return;
+    const int p = (int)(t % nc);
+    const size_t r = t / nc;
+    const int f = (int)(r % nbf), k = (int)(r / nbf);
+    const double2 v = co[((size_t)k * nc + p) * nc + ref[f]];
+    Cr[t] = sc[f] * v.x;
+    Ci[t] = sc[f] * v.y;
+}
+
+// Band operand of one quarter transform:
+//   X[t][qi][f][p] = conj?(C[k][f][p0 + p]) exp(i pi q (sk (k - kvc) + sf (kf - kvc)) / kvc)
+// with k = (k0 + t) mod nk, q = qi - qh; zero for p0 + p >= plim[k] (the
+// bands IALL(k) - IVIRTCORE kept at that k).  kvc = 0: the Gamma point, no phase.
+__global__ void k_mp2k_operand(const double* __restrict__ Cr, const double* __restrict__ Ci, int nk, int nbf,
+                               int nc, int kvc, int k0, int nt, int nq, int qh, int np, int p0, int sk, int kf,
+                               int sf, int conj, const int* __restrict__ plim, double* __restrict__ Xr,
+                               double* __restrict__ Xi) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nt * nq * nbf * np) return;
+    const int p = (int)(t % np);
+    size_t r = t / np;
+    const int f = (int)(r % nbf); r /= nbf;
+    const int qi = (int)(r % nq);
+    const int tt = (int)(r / nq);
+    const int k = (k0 + tt) % nk;
+    if (p0 + p >= plim[k]) { Xr[t] = 0.0; Xi[t] = 0.0; return; }
+    const size_t e = ((size_t)k * nbf + f) * nc + p0 + p;
+    const double cr = Cr[e], ci = conj ? -Ci[e] : Ci[e];
+    double sn = 0.0, cs = 1.0;
+    if (kvc > 0) {
+        const int q = qi - qh;
+        // integer phase numerator, reduced mod 2 kvc before the angle is formed
+        long long m = (long long)q * (sk * (k - kvc) + sf * (kf - kvc));
+        m %= 2LL * kvc;
+        sincospi((double)m / kvc, &sn, &cs);
+    }
+    Xr[t] = cr * cs - ci * sn;
+    Xi[t] = cr * sn + ci * cs;
+}
+
+// E(2) of one k_b (src/pmp2energy.f90:395-433) from
+// TOVOKK[kj][ka][b][jo][a][io] (planar), T2 = TOVOKK(j, a, i) at (ka, ki),
+// ki = ka - kj + kb (mod nk); eps[k][band]; a < IALL(ka) - IVIRTCORE - IOCC.
+__global__ void k_mp2k_energy(const double* __restrict__ Tr, const double* __restrict__ Ti,
+                              const double* __restrict__ eps, const int* __restrict__ plim, int nk, int kb, int nb,
+                              int no, int nv, int nc, int icore, int iocc, double* __restrict__ acc) {
+    const size_t tot = (size_t)nk * nk * nb * no * nv * no;
+    double es = 0.0, et = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        const int io = (int)(t % no);
+        size_t r = t / no;
+        const int a = (int)(r % nv); r /= nv;
+        const int jo = (int)(r % no); r /= no;
+        const int b = (int)(r % nb); r /= nb;
+        const int ka = (int)(r % nk);
+        const int kj = (int)(r / nk);
+        if (iocc + a >= plim[ka]) continue;
+        const int ki = (ka - kj + kb + nk) % nk;
+        const double x = Tr[t], y = Ti[t];
+        const size_t t2 = (((((size_t)ki * nk + ka) * nb + b) * no + io) * nv + a) * no + jo;
+        const double x2 = Tr[t2], y2 = Ti[t2];
+        const double aa = x * x + y * y, ab = x * x2 + y * y2;
+        const double den = eps[(size_t)ki * nc + icore + io] + eps[(size_t)kj * nc + icore + jo] -
+                           eps[(size_t)ka * nc + iocc + a] - eps[(size_t)kb * nc + iocc + b];
+        es += (0.5 * aa + 0.5 * ab) / den;
+        et += (1.5 * aa - 1.5 * ab) / den;
+    }
+    __shared__ double ss[256], st[256];
+    ss[threadIdx.x] = es;
+    st[threadIdx.x] = et;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            st[threadIdx.x] += st[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], ss[0]);
+        atomicAdd(&acc[1], st[0]);
+    }
+}
+
+int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, const double* eps, const int* nall,
+                                int icore, int iocc, int ivirtcore, int transform, double* e2s, double* e2t,
+                                double* seconds) {
+    if (!ctx || !co || !eps || !nall || !e2s || !e2t)
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: null argument");
+    if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: not a periodic context");
+    if (kvc < 0 || kvc > 512) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: bad KPOINTS");
+    if (transform != POLYERI_GEMM_MFMA_F64 && transform != POLYERI_GEMM_MFMA_BF16X3)
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: transform must be MFMA_F64 or MFMA_BF16X3");
+    const int nk = kvc > 0 ? 2 * kvc : 1, nc = ctx->rb.ncgs, n = ctx->ib.nbf;
+    const int no = iocc - icore;
+    if (icore < 0 || no <= 0 || ivirtcore < 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: empty occupied space");
+    int nvmax = 0;
+    std::vector<int> plim(nk);
+    for (int k = 0; k < nk; ++k) {
+        if (nall[k] < 0 || nall[k] > nc) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: bad IALL");
+        plim[k] = nall[k] - ivirtcore;
+        if (plim[k] < iocc)
+            return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: fewer bands than occupied orbitals at a k-point");
+        nvmax = std::max(nvmax, plim[k] - iocc);
+    }
+    if (nvmax <= 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: empty virtual space");
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int mode = transform == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
+    const long N = n, n1 = 2 * ctx->rc1 + 1, n3 = 2 * ctx->cel2 + 1, NK = nk, NO = no, NV = nvmax;
+    const long NB = nvmax;                              // virtual bands of one k_b, at most
+    const long M1 = N * n1 * N * n3 * N, M2 = N * n1 * N;
+    if (M1 > INT_MAX || (long)NK * NB * NO > INT_MAX)
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: lattice tensor too large for one pass");
+    const size_t nT = (size_t)M1 * n1 * N;
+    const size_t nTN = (size_t)M1 * NB, nTOK = (size_t)NK * NB * NO * M2, nTV = (size_t)NK * NK * NB * NO * NV * N,
+                 nTOV = (size_t)NK * NK * NB * NO * NV * NO;
+    const size_t nB1 = (size_t)n1 * N * NB, nB2 = (size_t)NK * n3 * N * NO, nB3 = (size_t)NK * n1 * N * NV,
+                 nB4 = (size_t)2 * NK * N * NO, nC = (size_t)NK * N * nc;
+    const size_t need = 8 * (nT + 2 * (nTN + nTOK + nTV + nTOV + nB1 + nB2 + nB3 + nB4 + nC) + (size_t)nc * nk) +
+                        sizeof(double2) * (size_t)nc * nc * nk;
+    size_t freeb = 0, totalb = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &totalb));
+    if (need > freeb - freeb / 16)
+        return fail(POLYERI_E_NOMEM, "polyeri_mp2_energy_periodic: " + std::to_string(need >> 20) +
+                                         " MiB of transform buffers exceed free device memory");
+    DevBuf buf;
+    double *dT, *dTNr, *dTNi, *dTOKr, *dTOKi, *dTVr, *dTVi, *dTOVr, *dTOVi, *dB1r, *dB1i, *dB2r, *dB2i, *dB3r, *dB3i,
+        *dB4r, *dB4i, *dCr, *dCi, *dEps, *dAcc;
+    double2* dCO;
+    int* dPlim;
+    int rc;
+    if ((rc = buf.get(&dT, nT)) || (rc = buf.get(&dTNr, nTN)) || (rc = buf.get(&dTNi, nTN)) ||
+        (rc = buf.get(&dTOKr, nTOK)) || (rc = buf.get(&dTOKi, nTOK)) || (rc = buf.get(&dTVr, nTV)) ||
+        (rc = buf.get(&dTVi, nTV)) || (rc = buf.get(&dTOVr, nTOV)) || (rc = buf.get(&dTOVi, nTOV)) ||
+        (rc = buf.get(&dB1r, nB1)) || (rc = buf.get(&dB1i, nB1)) || (rc = buf.get(&dB2r, nB2)) ||
+        (rc = buf.get(&dB2i, nB2)) || (rc = buf.get(&dB3r, nB3)) || (rc = buf.get(&dB3i, nB3)) ||
+        (rc = buf.get(&dB4r, nB4)) || (rc = buf.get(&dB4i, nB4)) || (rc = buf.get(&dCr, nC)) ||
+        (rc = buf.get(&dCi, nC)) || (rc = buf.get(&dEps, (size_t)nc * nk)) || (rc = buf.get(&dAcc, 2)) ||
+        (rc = buf.get(&dCO, (size_t)nc * nc * nk)) || (rc = buf.get(&dPlim, (size_t)nk)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(dCO, co, sizeof(double2) * (size_t)nc * nc * nk, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dEps, eps, 8 * (size_t)nc * nk, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dPlim, plim.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_co_planar, dim3(nblk(nC)), dim3(256), 0, s, dCO, nc, ctx->d_ref, ctx->d_scale, n, nk, dCr,
+                       dCi);
+    HIPCHK(hipGetLastError());
+    if ((rc = tensor_device_cells(ctx, dT, nT))) return rc;
+    // k-independent operands: B3[ka][(Q1,j)][a] = CO(j,a,ka) e^{i ka Q1} (:323-329),
+    // B4[t][i][io] = conj(CO(i,io,t mod nk)), t < 2 nk (:350-386)
+    hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB3)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, nk, (int)n1,
+                       ctx->rc1, nvmax, iocc, 1, 0, 0, 0, dPlim, dB3r, dB3i);
+    hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB4)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, 2 * nk, 1, 0,
+                       no, icore, 0, 0, 0, 1, dPlim, dB4r, dB4i);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
+    HIPCHK(hipGetLastError());
+    auto gemm = [&](int M, int Nn, int K, const double* Ar, const double* Ai, long sam, long sak, long sA,
+                    const double* Br, const double* Bi, long sbk, long sbn, long sB, double* Cr, double* Ci, long scm,
+                    long scn, long sC, int batch) {
+        return polyeri_zgemm_launch(mode, M, Nn, K, Ar, Ai, sam, sak, sA, Br, Bi, sbk, sbn, sB, Cr, Ci, scm, scn, sC,
+                                    batch, s);
+    };
+    for (int kb = 0; kb < nk; ++kb) {
+        const int nb = plim[kb] - iocc;
+        if (nb <= 0) continue;
+        const long NBk = nb;
+        // Q1 quarter: TN[(i,Q1,j,Q3,k)][b] = sum_{Q2,l} T CO(l,b,kb) e^{i kb Q2} (:253-258;
+        // the e^{i kb Q3} factor moves into the next operand)
+        hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB1)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, kb, 1,
+                           (int)n1, ctx->rc1, nb, iocc, 1, 0, 0, 0, dPlim, dB1r, dB1i);
+        // B2[kj][(Q3,k)][jo] = conj(CO(k,jo,kj)) e^{i (kb - kj) Q3} (:294-299)
+        hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB2)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, nk,
+                           (int)n3, ctx->cel2, no, icore, -1, kb, 1, 1, dPlim, dB2r, dB2i);
+        HIPCHK(hipGetLastError());
+        if (gemm((int)M1, nb, (int)(n1 * N), dT, nullptr, n1 * N, 1, 0, dB1r, dB1i, NBk, 1, 0, dTNr, dTNi, NBk, 1, 0,
+                 1))
+            return fail(POLYERI_E_HIP, "k-point MP2: first quarter launch failed");
+        // Q2 quarter, per kj: TOK[kj][b][jo][(i,Q1,j)] = sum_{Q3,k} TN B2[kj]
+        for (int kj = 0; kj < nk; ++kj)
+            if (gemm((int)M2, no, (int)(n3 * N), dTNr, dTNi, n3 * N * NBk, NBk, 1, dB2r + kj * n3 * N * NO,
+                     dB2i + kj * n3 * N * NO, NO, 1, 0, dTOKr + kj * NBk * NO * M2, dTOKi + kj * NBk * NO * M2, 1,
+                     M2, NO * M2, nb))
+                return fail(POLYERI_E_HIP, "k-point MP2: second quarter launch failed");
+        // Q3 quarter, per ka: TV[ka][(kj,b,jo)][a][i] = sum_{Q1,j} TOK B3[ka]
+        for (int ka = 0; ka < nk; ++ka)
+            if (gemm((int)N, nvmax, (int)(n1 * N), dTOKr, dTOKi, n1 * N, 1, M2, dB3r + ka * n1 * N * NV,
+                     dB3i + ka * n1 * N * NV, NV, 1, 0, dTVr + ka * NK * NBk * NO * NV * N,
+                     dTVi + ka * NK * NBk * NO * NV * N, 1, N, NV * N, nk * nb * no))
+                return fail(POLYERI_E_HIP, "k-point MP2: third quarter launch failed");
+        // Q4 quarter, per kj, batched over ka with ki = ka + (kb - kj mod nk):
+        // TOV[kj][ka][(b,jo,a)][io] = sum_i TV[ka][kj] conj(CO(i,io,ki))
+        for (int kj = 0; kj < nk; ++kj) {
+            const long d = ((kb - kj) % nk + nk) % nk;
+            if (gemm(nb * no * nvmax, no, (int)N, dTVr + kj * NBk * NO * NV * N, dTVi + kj * NBk * NO * NV * N, N, 1,
+                     NK * NBk * NO * NV * N, dB4r + d * N * NO, dB4i + d * N * NO, NO, 1, N * NO,
+                     dTOVr + kj * NK * NBk * NO * NV * NO, dTOVi + kj * NK * NBk * NO * NV * NO, NO, 1,
+                     NBk * NO * NV * NO, nk))
+                return fail(POLYERI_E_HIP, "k-point MP2: fourth quarter launch failed");
+        }
+        const size_t ne = (size_t)nk * nk * nb * no * nvmax * no;
+        const size_t grid = std::min<size_t>(nblk(ne), 4096);
+        hipLaunchKernelGGL(k_mp2k_energy, dim3((unsigned)grid), dim3(256), 0, s, dTOVr, dTOVi, dEps, dPlim, nk, kb, nb,
+                           no, nvmax, nc, icore, iocc, dAcc);
+        HIPCHK(hipGetLastError());
+    }
+    double acc[2];
+    HIPCHK(hipMemcpyAsync(acc, dAcc, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double norm = (double)nk * nk * nk;
+    *e2s = acc[0] / norm;
+    *e2t = acc[1] / norm;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!std::isfinite(acc[0]) || !std::isfinite(acc[1])) return fail(POLYERI_E_NONFINITE, "non-finite E(2)");
+    return 0;
+}
+
+int polyeri_reduced_cel1(const polyeri_ctx* ctx) {
+    if (!ctx) return fail(POLYERI_E_ARG, "polyeri_reduced_cel1: null context");
+    if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_reduced_cel1: not a periodic context");
+    return ctx->rc1;
+}
+
+int polyeri_eri_tensor_cells(polyeri_ctx* ctx, double* t_c) {
+    if (!ctx || !t_c) return fail(POLYERI_E_ARG, "polyeri_eri_tensor_cells: null argument");
+    if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_eri_tensor_cells: not a periodic context");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = ctx->ib.nbf, nc = ctx->rb.ncgs, n1 = 2 * ctx->rc1 + 1, n3 = 2 * ctx->cel2 + 1;
+    const size_t nt = n * n1 * n * n3 * n * n1 * n, nc4 = nc * nc * nc * nc;
+    double* d_t = nullptr;
+    int rc;
+    if ((rc = dalloc(&d_t, nt))) return rc;
+    rc = tensor_device_cells(ctx, d_t, nt);
+    std::vector<double> t;
+    if (!rc) {
+        t.resize(nt);
+        hipError_t e = hipMemcpyAsync(t.data(), d_t, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(POLYERI_E_HIP, hipGetErrorString(e));
+    }
+    hipFree(d_t);
+    if (rc) return rc;
+    std::fill(t_c, t_c + nc4 * n1 * n1 * n3, 0.0);
+    const auto& ref = ctx->ib.ref_of_int;
+    const auto& sc = ctx->ib.scale;
+    // device T[i][Q1][j][Q3][k][Q2][l] -> block (Q1, Q2, Q3) of column-major FERI(I,J,K,L)
+    size_t x = 0;
+    for (size_t a = 0; a < n; ++a)
+        for (size_t q1 = 0; q1 < n1; ++q1)
+            for (size_t b = 0; b < n; ++b)
+                for (size_t q3 = 0; q3 < n3; ++q3)
+                    for (size_t c = 0; c < n; ++c)
+                        for (size_t q2 = 0; q2 < n1; ++q2)
+                            for (size_t d = 0; d < n; ++d, ++x) {
+                                double* blk = t_c + ((q1 * n1 + q2) * n3 + q3) * nc4;
+                                blk[ref[a] + nc * (ref[b] + nc * (ref[c] + nc * ref[d]))] =
+                                    sc[a] * sc[b] * sc[c] * sc[d] * t[x];
+                            }
     return 0;
 }
 
@@ -2035,11 +2328,15 @@ struct AoWriter {
 
 int polyeri_store_fourindex_eri(polyeri_ctx* ctx, const char* path, double inttol, long long* stored) {
     if (!ctx || !path) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: null argument");
-    if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: molecular contexts only");
     const size_t nc = ctx->rb.ncgs;
     if (nc >= 256) return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: NCGS must be below 256");
-    std::vector<double> feri(nc * nc * nc * nc);
-    int rc = polyeri_eri_tensor(ctx, feri.data());
+    if (ctx->pbc && (2 * ctx->cel1 + 1 > 255 || 2 * ctx->cel2 + 1 > 255))
+        return fail(POLYERI_E_ARG, "polyeri_store_fourindex_eri: cell offsets must pack into 8 bits");
+    // molecule: one cell triple (0, 0, 0); periodic: Q1, Q2 in [-rc1, rc1], Q3 in [-cel2, cel2]
+    const int rc1 = ctx->pbc ? ctx->rc1 : 0, cel1 = ctx->pbc ? ctx->cel1 : 0, cel2 = ctx->pbc ? ctx->cel2 : 0;
+    const size_t n1 = 2 * rc1 + 1, n3 = 2 * cel2 + 1, nc4 = nc * nc * nc * nc;
+    std::vector<double> feri(nc4 * n1 * n1 * n3);
+    int rc = ctx->pbc ? polyeri_eri_tensor_cells(ctx, feri.data()) : polyeri_eri_tensor(ctx, feri.data());
     if (rc) return rc;
     AoWriter w;
     w.f = fopen(path, "wb");
@@ -2048,19 +2345,26 @@ int polyeri_store_fourindex_eri(polyeri_ctx* ctx, const char* path, double intto
     w.reset();
     long long count = 0;
     bool ok = true;
-    // I outermost, L innermost, as src/pfourindexstore.f90:728-750; molecule: all cells 0
-    for (size_t i = 0; i < nc && ok; ++i)
-        for (size_t j = 0; j < nc && ok; ++j)
-            for (size_t k = 0; k < nc && ok; ++k)
-                for (size_t l = 0; l < nc && ok; ++l) {
-                    const double v = feri[i + nc * (j + nc * (k + nc * l))];
-                    if (!(std::fabs(v) > inttol)) continue;
-                    w.x[w.n] = 0; w.y[w.n] = 0; w.z[w.n] = 0;
-                    w.ijkl[w.n] = (int32_t)((((i + 1) * 256 + (j + 1)) * 256 + (k + 1)) * 256 + (l + 1));
-                    w.v[w.n] = v;
-                    ++count;
-                    if (++w.n == kAoCache) ok = w.flush();
-                }
+    // Q1, Q2, Q3 outer, then I outermost and L innermost (src/pfourindexstore.f90:258-262, 378, 727-750);
+    // ICASHE1X = ((Q1+CEL1)*256 + Q2+CEL1)*256 + Q3+CEL2; the y and z cells are 0
+    for (int q1 = -rc1; q1 <= rc1 && ok; ++q1)
+        for (int q2 = -rc1; q2 <= rc1 && ok; ++q2)
+            for (int q3 = -cel2; q3 <= cel2 && ok; ++q3) {
+                const double* blk = feri.data() + (((size_t)(q1 + rc1) * n1 + (q2 + rc1)) * n3 + (q3 + cel2)) * nc4;
+                const int32_t cx = ((q1 + cel1) * 256 + (q2 + cel1)) * 256 + (q3 + cel2);
+                for (size_t i = 0; i < nc && ok; ++i)
+                    for (size_t j = 0; j < nc && ok; ++j)
+                        for (size_t k = 0; k < nc && ok; ++k)
+                            for (size_t l = 0; l < nc && ok; ++l) {
+                                const double v = blk[i + nc * (j + nc * (k + nc * l))];
+                                if (!(std::fabs(v) > inttol)) continue;
+                                w.x[w.n] = cx; w.y[w.n] = 0; w.z[w.n] = 0;
+                                w.ijkl[w.n] = (int32_t)((((i + 1) * 256 + (j + 1)) * 256 + (k + 1)) * 256 + (l + 1));
+                                w.v[w.n] = v;
+                                ++count;
+                                if (++w.n == kAoCache) ok = w.flush();
+                            }
+            }
     if (ok && w.n) ok = w.flush();
     ok = (fclose(w.f) == 0) && ok;
     if (!ok) return fail(POLYERI_E_ARG, std::string("write error on ") + path);
@@ -2068,12 +2372,29 @@ int polyeri_store_fourindex_eri(polyeri_ctx* ctx, const char* path, double intto
     return 0;
 }
 
+static int restore_ao(const char* path, int ncgs, int cel1, int cel2, double* feri, long long* restored);
+
 int polyeri_restore_fourindex_eri(const char* path, int ncgs, double* feri, long long* restored) {
+    return restore_ao(path, ncgs, -1, 0, feri, restored);
+}
+
+int polyeri_restore_fourindex_eri_cells(const char* path, int ncgs, int cel1, int cel2, double* t_c,
+                                        long long* restored) {
+    if (cel1 < 0 || cel2 < 0 || 2 * cel1 + 1 > 255 || 2 * cel2 + 1 > 255)
+        return fail(POLYERI_E_ARG, "polyeri_restore_fourindex_eri_cells: bad cell range");
+    return restore_ao(path, ncgs, cel1, cel2, t_c, restored);
+}
+
+// cel1 < 0: molecular file (every record in cell 0); else Q1, Q2 in
+// [-cel1, cel1], Q3 in [-cel2, cel2] into (2 cel1 + 1)^2 (2 cel2 + 1) blocks.
+static int restore_ao(const char* path, int ncgs, int cel1, int cel2, double* feri, long long* restored) {
     if (!path || !feri || ncgs <= 0 || ncgs >= 256) return fail(POLYERI_E_ARG, "polyeri_restore_fourindex_eri: bad argument");
+    const bool cells = cel1 >= 0;
+    const size_t n1 = cells ? 2 * cel1 + 1 : 1, n3 = cells ? 2 * cel2 + 1 : 1;
     FILE* f = fopen(path, "rb");
     if (!f) return fail(POLYERI_E_ARG, std::string("cannot open ") + path);
-    const size_t nc = ncgs;
-    std::fill(feri, feri + nc * nc * nc * nc, 0.0);
+    const size_t nc = ncgs, nc4 = nc * nc * nc * nc;
+    std::fill(feri, feri + nc4 * n1 * n1 * n3, 0.0);
     std::vector<int32_t> x(kAoCache), y(kAoCache), z(kAoCache), ijkl(kAoCache);
     std::vector<double> v(kAoCache);
     long long count = 0;
@@ -2089,11 +2410,20 @@ int polyeri_restore_fourindex_eri(const char* path, int ncgs, double* feri, long
         if (!ok) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
         for (int e = 0; e < kAoCache; ++e) {
             if (x[e] == -1) break;
-            if (x[e] != 0 || y[e] != 0 || z[e] != 0) { rc = fail(POLYERI_E_ARG, "periodic .ao records are not read here"); break; }
+            size_t blk = 0;
+            if (!cells) {
+                if (x[e] != 0 || y[e] != 0 || z[e] != 0) { rc = fail(POLYERI_E_ARG, "periodic .ao records in a molecular read"); break; }
+            } else {
+                // src/pmp2energy.f90:221-252: unpack and range-check the cells
+                const int q1 = ((x[e] >> 16) & 255) - cel1, q2 = ((x[e] >> 8) & 255) - cel1, q3 = (x[e] & 255) - cel2;
+                if (y[e] != 0 || z[e] != 0 || q1 < -cel1 || q1 > cel1 || q2 < -cel1 || q2 > cel1 || q3 < -cel2 ||
+                    q3 > cel2) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
+                blk = (((size_t)(q1 + cel1) * n1 + (q2 + cel1)) * n3 + (q3 + cel2)) * nc4;
+            }
             const size_t i = ((ijkl[e] >> 24) & 255) - 1, j = ((ijkl[e] >> 16) & 255) - 1, k = ((ijkl[e] >> 8) & 255) - 1,
                          l = (ijkl[e] & 255) - 1;
             if (i >= nc || j >= nc || k >= nc || l >= nc) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
-            feri[i + nc * (j + nc * (k + nc * l))] = v[e];
+            feri[blk + i + nc * (j + nc * (k + nc * l))] = v[e];
             ++count;
         }
         if (rc) break;

@@ -100,6 +100,19 @@ def test_periodic_jk_against_supercell_tensor(basis):
     assert np.max(np.abs(K - Kr)) < 1e-12 * max(1.0, np.max(np.abs(Kr)))
 
 
+def test_lattice_tensor_against_supercell():
+    """The oracle's direct lattice integrals (or_eri_tensor_cells, the values
+    STORE_FOURINDEX_ERI writes) equal the supercell molecule's ERIs of the
+    same functions: (i_0 j_Q1 | k_Q3 l_Q3+Q2) for every cell triple."""
+    from oracle import periodic
+    atoms = _chain(0, 0, 3.2)
+    ub = oracle.Basis(basis_path("6-31Gss"), atoms)
+    T = ub.eri_tensor_cells(3.2, 1, 1)
+    T2 = periodic.ao_tensor_cells(basis_path("6-31Gss"), atoms, 3.2, 1, 1)
+    assert np.max(np.abs(T2)) > 0.5
+    assert np.max(np.abs(T - T2)) < 1e-13
+
+
 def test_periodic_isolated_cell_limit():
     """Period 200 bohr: images do not overlap, so K(0) is the molecule's K,
     K(Q != 0) = 0, and J(0) exceeds the molecule's J only by the image
@@ -175,3 +188,107 @@ def test_band_folding_identity():
     e1, _, _ = periodic.bands(F, S, np.eye(n), cel1, 2 * K, 1e-8)
     e2, _, _ = periodic.bands(F2, S2, np.eye(2 * n), c2, K, 1e-8)
     assert np.allclose(np.sort(e1.ravel()), np.sort(e2.ravel()), atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# k-point MP2 (oracle/periodic.py mp2_kpoints)
+# ---------------------------------------------------------------------------
+def test_mp2_kpoints_isolated_limit_is_molecular_golden(golden):
+    """CUTOFF1 = CUTOFF2 = 0, Gamma only: the lattice tensor is the molecular
+    ERI tensor and the k-point MP2 is MP2_ENERGY_ALG1 of the molecule: the
+    shipped BH/STO-3G frozen-core E(2) (data/BH_EOM_out:286)."""
+    from oracle import periodic
+    from tests import molecules as M
+    g = golden["BH_STO3G"]
+    b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11)
+    T = periodic.ao_tensor_cells(basis_path("STO-3G"), M.BH_GOLDEN, 100.0, 0, 0)
+    assert np.max(np.abs(T[0, 0, 0] - b.eri_tensor())) < 1e-14
+    co = np.asarray(r["co"], dtype=complex)[None]
+    eps = np.asarray(r["eps"])[None]
+    s, t = periodic.mp2_kpoints(T, co, eps, 0, 0, 0, 1, r["iocc"], [r["n_indep"]])
+    assert abs(s + t - g["e2_alg1"]) < 1e-10
+    s0, t0 = oracle.mp2(b.eri_tensor(), r["co"], r["eps"], 1, r["iocc"], r["n_indep"])
+    assert abs(s - s0) < 1e-12 and abs(t - t0) < 1e-12
+
+
+def _fold_case(seed):
+    """Random translation-invariant integrals G(i_0 j_dj | k_dk l_dl),
+    |dj|, |dk|, |dl| <= 1, on a 2-function primitive cell; primitive-cell
+    orbitals at 4 k-points (one occupied, one virtual band, gapped)."""
+    rng = np.random.default_rng(seed)
+    n = 2
+    G = rng.standard_normal((n, n, n, n, 3, 3, 3))
+    kvc_p = 2
+    nk_p = 2 * kvc_p
+    c = rng.standard_normal((nk_p, n, n)) + 1j * rng.standard_normal((nk_p, n, n))
+    e = np.stack([np.array([-1.0 - rng.random(), 1.0 + rng.random()]) for _ in range(nk_p)])
+    return n, G, kvc_p, c, e
+
+
+def _tensor_prim(n, G, rc1, cel2):
+    T = np.zeros((2 * rc1 + 1, 2 * rc1 + 1, 2 * cel2 + 1, n, n, n, n))
+    for q1 in range(-rc1, rc1 + 1):
+        for q2 in range(-rc1, rc1 + 1):
+            for q3 in range(-cel2, cel2 + 1):
+                dj, dk, dl = q1, q3, q3 + q2
+                if max(abs(dj), abs(dk), abs(dl)) <= 1:
+                    T[q1 + rc1, q2 + rc1, q3 + cel2] = G[..., dj + 1, dk + 1, dl + 1]
+    return T
+
+
+def _tensor_doubled(n, G, rc1, cel2):
+    """Doubled cell: function (s, f) = primitive function f in primitive cell
+    2 Q + s of doubled cell Q."""
+    T = np.zeros((2 * rc1 + 1, 2 * rc1 + 1, 2 * cel2 + 1, 2 * n, 2 * n, 2 * n, 2 * n))
+    for q1 in range(-rc1, rc1 + 1):
+        for q2 in range(-rc1, rc1 + 1):
+            for q3 in range(-cel2, cel2 + 1):
+                for si in range(2):
+                    for sj in range(2):
+                        for sk in range(2):
+                            for sl in range(2):
+                                dj = 2 * q1 + sj - si
+                                dk = 2 * q3 + sk - si
+                                dl = 2 * (q3 + q2) + sl - si
+                                if max(abs(dj), abs(dk), abs(dl)) <= 1:
+                                    T[q1 + rc1, q2 + rc1, q3 + cel2, si * n:(si + 1) * n, sj * n:(sj + 1) * n,
+                                      sk * n:(sk + 1) * n, sl * n:(sl + 1) * n] = G[..., dj + 1, dk + 1, dl + 1]
+    return T
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mp2_kpoints_band_folding_identity(seed):
+    """The same crystal described by its primitive cell at 4 k-points and by
+    the doubled cell at 2 k-points (Bloch states folded: k -> K = 2k, the
+    coefficients c e^{i k s} / sqrt(2) on sub-cell s): E(2) per doubled cell
+    is twice E(2) per primitive cell, singlet and triplet separately.  Fails
+    for a wrong Bloch phase sign, momentum-conservation index, swap-term
+    index or k normalisation."""
+    from oracle import periodic
+    n, G, kvc_p, c, e = _fold_case(seed)
+    nk_p = 2 * kvc_p
+    sp, tp = periodic.mp2_kpoints(_tensor_prim(n, G, 2, 1), c, e, 2, 1, kvc_p, 0, 1, [n] * nk_p)
+    kvc_d = kvc_p // 2
+    nk_d = 2 * kvc_d
+    cd = np.zeros((nk_d, 2 * n, 2 * n), dtype=complex)
+    ed = np.zeros((nk_d, 2 * n))
+    for K in range(nk_d):
+        IXd = K - kvc_d
+        cols, vals = [], []
+        for kp in range(nk_p):
+            IX = kp - kvc_p
+            if (IX - IXd) % nk_d:
+                continue
+            k = np.pi * IX / kvc_p
+            for band in range(n):
+                v = np.concatenate([c[kp][:, band] * np.exp(1j * k * s) for s in range(2)]) / np.sqrt(2.0)
+                cols.append(v)
+                vals.append(e[kp][band])
+        order = np.argsort(vals)
+        cd[K] = np.array(cols).T[:, order]
+        ed[K] = np.array(vals)[order]
+    sd, td = periodic.mp2_kpoints(_tensor_doubled(n, G, 2, 1), cd, ed, 2, 1, kvc_d, 0, 2, [2 * n] * nk_d)
+    assert abs(sp) > 1e-3 and abs(tp) > 1e-3
+    assert abs(sd - 2 * sp) < 1e-11 * max(1.0, abs(sd))
+    assert abs(td - 2 * tp) < 1e-11 * max(1.0, abs(td))
