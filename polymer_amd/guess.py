@@ -414,3 +414,49 @@ def core_guess(ctx, basis, lindep=1e-6):
     P = np.zeros((basis.ncgs, basis.ncgs), order="F")
     P[np.ix_(real, real)] = 2.0 * C[:, :nocc] @ C[:, :nocc].T
     return P
+
+
+_ATOM_CACHE = {}
+
+
+def atomic_density(basis_name, z, temphf=1000.0):
+    """Spherically averaged density of a free atom of element z in its own
+    basis (reference layout of the one-atom basis): closed-shell SCF with
+    Fermi-smeared occupations (TEMPHF, equal occupation of degenerate
+    levels) for even z; for hydrogen the exact one-electron ground state."""
+    key = (basis_name, z, temphf)
+    if key in _ATOM_CACHE:
+        return _ATOM_CACHE[key]
+    b = read_basis(basis_name, [(z, 0.0, 0.0, 0.0)])
+    ctx = Context(b, schwarz_tol=0.0)
+    if z % 2 == 0:
+        r = ctx.self_consistent_field(scfconv=1e-9, diis=2, scfcycles=200, temphf=temphf)
+        if not r["converged"]:
+            raise RuntimeError(f"atomic SCF of Z={z} did not converge")
+        P = np.asarray(r["P"])
+    elif z == 1:
+        S, T, V, _ = ctx.one_electron()
+        real = b.real_functions
+        s = S[np.ix_(real, real)]
+        e, U = np.linalg.eigh(s)
+        X = U / np.sqrt(e)
+        _, C = np.linalg.eigh(X.T @ (T + V)[np.ix_(real, real)] @ X)
+        c = X @ C[:, 0]
+        P = np.zeros((b.ncgs, b.ncgs))
+        P[np.ix_(real, real)] = np.outer(c, c)
+    else:
+        raise ValueError("atomic densities: even Z or hydrogen only")
+    fl = [f for f, _ in atom_functions(b)[0]]
+    _ATOM_CACHE[key] = P[np.ix_(fl, fl)]
+    return _ATOM_CACHE[key]
+
+
+def sad_guess(basis, basis_name):
+    """Superposition of atomic densities: the block of each atom is its free
+    atom's spherically averaged density, every other block zero."""
+    P = np.zeros((basis.ncgs, basis.ncgs), order="F")
+    for a, fl in enumerate(atom_functions(basis)):
+        idx = [f for f, _ in fl]
+        P[np.ix_(idx, idx)] = atomic_density(basis_name, int(basis.atoms[a][0]))
+    return P
+

@@ -1,0 +1,131 @@
+"""Learned SCF guess on the GPU.
+
+* polyeri_gemm_bf16 (v_mfma_f32_16x16x16bf16_1k) equals the product of the
+  BF16-rounded operands (float64 reference) to FP32 accumulation accuracy,
+  for plain and transposed operand layouts and ragged sizes;
+* the device MLP's forward pass and gradients equal the CPU reference
+  (oracle/mlp_ref.py, same BF16 operand rounding) to 1e-3 relative;
+* Adam steps on the device reduce a regression loss;
+* the trained guess (polymer_amd/guess_water_631gss.npz, made by
+  polymer_amd/guess_train.py on this GPU) converges held-out (H2O)4-8
+  6-31G** SCFs in fewer iterations than the reference's diagonal guess, to
+  the same energy (1e-8 Eh)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import polymer_amd as pa
+from oracle import mlp_ref
+from polymer_amd import geometry as Gm
+from polymer_amd import guess as G
+from polymer_amd._lib import check, lib
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(pa.__file__), "guess_water_631gss.npz")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return pa.Context(pa.read_basis("STO-3G", Gm.water()))
+
+
+def _gemm(ctx, A, B, transA=False, transB=False):
+    """C = op(A) op(B) through polyeri_gemm_bf16 with stride views."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = (A.shape[1], A.shape[0]) if transA else A.shape
+    K2, N = (B.shape[1], B.shape[0]) if transB else B.shape
+    assert K == K2
+    sam, sak = (1, A.shape[1]) if transA else (A.shape[1], 1)
+    sbk, sbn = (1, B.shape[1]) if transB else (B.shape[1], 1)
+    dev = G.DeviceArrays(ctx)
+    try:
+        da, db = dev.put(A), dev.put(B)
+        dc = dev.put(np.zeros((M, N), np.float32))
+        check(lib().polyeri_gemm_bf16(M, N, K, da, sam, sak, db, sbk, sbn, ctypes.c_float(1.0),
+                                      ctypes.c_float(0.0), dc, N))
+        C = np.zeros((M, N), np.float32)
+        ctx.from_device_raw(dc, C)
+        return C
+    finally:
+        dev.free()
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_bf16_matches_rounded_product(ctx, ta, tb):
+    rng = np.random.default_rng(1)
+    M, K, N = 37, 45, 29
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    print(f"case: gemm_bf16 transA {ta} transB {tb}", flush=True)
+    C = _gemm(ctx, A, B, ta, tb)
+    opA = mlp_ref.bf16(A.T if ta else A)
+    opB = mlp_ref.bf16(B.T if tb else B)
+    ref = opA @ opB
+    assert np.max(np.abs(C - ref)) < 1e-5 * np.max(np.abs(ref))
+
+
+def test_mlp_forward_and_grads_match_reference(ctx):
+    dims = [G.NFEAT, 64, 64, 75]
+    net = G.MLP(dims, seed=3)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((100, dims[0])).astype(np.float32)
+    T = rng.standard_normal((100, dims[-1])).astype(np.float32)
+    W = rng.uniform(0.0, 1.0, (100, dims[-1])).astype(np.float32)
+    p = net.params()
+    print("case: MLP forward/grad vs reference", flush=True)
+    Y = G.mlp_forward(net, ctx, X)
+    Yr, _, _ = mlp_ref.forward(p, dims, X)
+    assert np.max(np.abs(Y - Yr)) < 1e-3 * np.max(np.abs(Yr))
+    loss = G.mlp_train_step(net, ctx, X, T, W, lr=0.0, update=False)
+    lr_, gr = mlp_ref.loss_and_grads(p, dims, X, T, W)
+    g = net.grads()
+    assert abs(loss - lr_) < 1e-4 * abs(lr_)
+    assert np.max(np.abs(g - gr)) < 1e-3 * np.max(np.abs(gr))
+    assert np.array_equal(net.params(), p)          # update=0 leaves the weights
+
+
+def test_mlp_adam_reduces_loss(ctx):
+    dims = [8, 32, 4]
+    net = G.MLP(dims, seed=5)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((256, 8)).astype(np.float32)
+    A = rng.standard_normal((8, 4))
+    T = np.tanh(X @ A).astype(np.float32)
+    W = np.ones_like(T)
+    first = G.mlp_train_step(net, ctx, X, T, W, lr=1e-2)
+    for _ in range(300):
+        last = G.mlp_train_step(net, ctx, X, T, W, lr=1e-2)
+    print(f"loss {first:.3e} -> {last:.3e}", flush=True)
+    assert last < 0.1 * first
+
+
+def test_learned_guess_fewer_iterations():
+    """Held-out (H2O)4-8 / 6-31G** geometries, every SCF to RMS density
+    change 1e-12: all guesses reach the diagonal guess's energy within
+    1e-8 Eh, and the learned guess needs fewer iterations (counted until the
+    energy stays within 1e-8 Eh) than the reference's diagonal guess, on
+    average; the superposition-of-atomic-densities count is printed."""
+    from polymer_amd.guess_train import energy_iterations, geometry
+    assert os.path.exists(WEIGHTS), "trained weights missing (python -m polymer_amd.guess_train)"
+    model = G.GuessModel.load(WEIGHTS)
+    its = {"diag": [], "sad": [], "learned": []}
+    for s in (2040, 2041, 2042, 2043, 2044):          # held out (last 20 % of 2048 seeds)
+        atoms = geometry(s)
+        b = pa.read_basis("6-31Gss", atoms)
+        ctx = pa.Context(b, schwarz_tol=1e-12)
+        print(f"case: learned guess seed {s} (H2O){len(atoms) // 3}", flush=True)
+        P = model.predict(ctx, b, atoms, S=ctx.one_electron()[0])
+        runs = {g: ctx.self_consistent_field(scfconv=1e-12, diis=2, scfcycles=200, guess=v)
+                for g, v in (("diag", None), ("sad", G.sad_guess(b, "6-31Gss")), ("learned", P))}
+        e_ref = runs["diag"]["e_total"]
+        for g, r in runs.items():
+            assert r["converged"]
+            assert abs(r["e_total"] - e_ref) < 1e-8
+            its[g].append(energy_iterations(r["trace"], e_ref))
+        print("iterations to 1e-8 Eh: " + ", ".join(f"{g} {v[-1]}" for g, v in its.items()), flush=True)
+    print({g: float(np.mean(v)) for g, v in its.items()}, flush=True)
+    assert np.mean(its["learned"]) < np.mean(its["diag"])
