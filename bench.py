@@ -198,6 +198,19 @@ def main():
             scf[name]["dE_vs_fp64_plain"] = scf[name]["e_total"] - scf["fp64_plain"]["e_total"]
             scf[name]["wall_speedup_vs_fp64_plain"] = scf["fp64_plain"]["wall_s"] / scf[name]["wall_s"]
 
+    # HBM bytes per launch of the dominant kernel from the committed counter
+    # summary of this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+    # scaled by the copy kernel; counters cannot be read inside this process)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
+    dkern = f"void k_eri<{', '.join(dc['cls'])}, 0>(EriArgs)"
+    if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f)["kernels"].get(dkern)
+        if tk:
+            traffic = tk["traffic_bytes"]
+            traffic_src = (f"profiles/k_eri_traffic_water16_ccpvdz.json: FETCH {tk['fetch_bytes']:.3e} B + WRITE "
+                           f"{tk['write_bytes']:.3e} B per launch ({tk['launch']})")
     out = {
         "metric": METRIC,
         "value": value,
@@ -228,7 +241,8 @@ def main():
         "roofline": {"bound": "mfma", "compute": "fp64 VALU (same 78.6 TFLOP/s peak as FP64 matrix)",
                      "kernel": f"k_eri<{','.join(dc['cls'])},JK>", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "kernel_ms": float(cls_ms[dom]), "kernel_model_flops": dc["flops"]},
+                     "traffic": traffic, "traffic_source": traffic_src, "kernel_ms": float(cls_ms[dom]),
+                     "kernel_model_flops": dc["flops"]},
         "classes": classes,
         "mixed": mixed,
         "mixed_classes": class_table(mst, mx_cls),

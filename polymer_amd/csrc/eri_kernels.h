@@ -21,6 +21,13 @@ struct PairMeta {
     int prim_off, nprim, fa, fb;
 };
 
+// A segment of one bra row: items t0 .. t0 + n - 1 of the list, all with
+// the bra pair of row `row`.
+struct SegDesc {
+    long long t0;
+    int row, n;
+};
+
 struct EriArgs {
     const PrimPair* prims;
     const PairMeta* meta;
@@ -53,6 +60,8 @@ struct EriArgs {
     const int2* pcell;         // per pair {cell offset Q of the second shell, swap}
     const double* Dc;          // (2 cel1 + 1) x nbf^2 density cells, P(Q)[k][l] = P(k_0, l_Q)
     double *Jc, *Kc;           // jk_copies x (2 cel1 + 1) x nbf^2
+    const SegDesc* segs;       // segment kernel: bra-row segments of the list
+    int nseg;
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -194,6 +203,16 @@ struct TensorCb {
 // grouped in buckets of equal primitive-pair count (each bucket a Schwarz
 // prefix, or a range from rb_first), so neighbouring threads run primitive
 // loops of the same length.  The ket pair is ket_ids[pos].
+// Item t of bra row `row` -> (ket position, ket bucket).
+__device__ __forceinline__ void decode_in_row(const EriArgs& g, int row, long long t, int& pos, int& b) {
+    const int local = (int)(t - g.row_off[row]);
+    const int* rb = g.rb_off + (size_t)row * (g.nbucket + 1);
+    b = 0;
+    while (b + 1 < g.nbucket && rb[b + 1] <= local) ++b;
+    const int first = g.rb_first ? g.rb_first[(size_t)row * g.nbucket + b] : 0;
+    pos = g.bucket_start[b] + first + (local - rb[b]);
+}
+
 __device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& bp, int& pos, int& b) {
     int lo = 0, hi = g.nrows - 1;
     while (lo < hi) {
@@ -201,12 +220,7 @@ __device__ __forceinline__ void decode_work(const EriArgs& g, long long t, int& 
         if (g.row_off[mid] <= t) lo = mid; else hi = mid - 1;
     }
     bp = g.bra_ids[lo];
-    const int local = (int)(t - g.row_off[lo]);
-    const int* rb = g.rb_off + (size_t)lo * (g.nbucket + 1);
-    b = 0;
-    while (b + 1 < g.nbucket && rb[b + 1] <= local) ++b;
-    const int first = g.rb_first ? g.rb_first[(size_t)lo * g.nbucket + b] : 0;
-    pos = g.bucket_start[b] + first + (local - rb[b]);
+    decode_in_row(g, lo, t, pos, b);
 }
 
 // Bucket of ket position pos.
@@ -357,7 +371,116 @@ __global__ void __launch_bounds__(256) k_eri(EriArgs g) {
         eri_item<LA, LB, LC, LD, MODE>(g, t);
 }
 
+// Segment kernel for the light classes (J/K modes): one block per segment of
+// a bra row, so every item of the block shares the bra pair (a b|.  The
+// exchange contributions of a quartet all land in the rows of the two bra
+// shells (K[a][c], K[a][d], K[b][c], K[b][d]) and J_ab in one fixed block:
+// those are summed in LDS (rows NA + NB of width nbf, LDS atomics) and in
+// registers (J_ab, reduced over the wavefront), and added to the XCD's J/K
+// copy once per segment, one atomic per nonzero element.  Only J_cd is added
+// per quartet.  LDS: ((NA + NB) nbf + NA NB) doubles.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
+    extern __shared__ double seg_lds[];
+    const int n = g.nbf;
+    double* Kr = seg_lds;                        // rows fa.. (NA) then fb.. (NB), width n
+    double* Jl = seg_lds + (size_t)(NA + NB) * n;
+    const int total = (NA + NB) * n + NA * NB;
+    const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
+    double* J = g.J + off;
+    double* K = g.K + off;
+    const int lane = threadIdx.x & 63;
+    for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
+        const SegDesc seg = g.segs[sg];
+        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
+        __syncthreads();
+        const int bp = g.bra_ids[seg.row];
+        const PairMeta mb = g.meta[bp];
+        const int fa = mb.fa, fb = mb.fb;
+        const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
+        const PrimPair* bra = g.prims + mb.prim_off;
+        double jab[NA * NB];
+#pragma unroll
+        for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
+        for (long long t = seg.t0 + threadIdx.x; t < seg.t0 + seg.n; t += blockDim.x) {
+            int pos, b;
+            decode_in_row(g, seg.row, t, pos, b);
+            const int kp = g.ket_ids[pos];
+            const PairMeta mk = g.meta[kp];
+            const int fc = mk.fa, fd = mk.fb;
+            const SoaKet kv{g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]),
+                            g.bucket_start[b + 1] - g.bucket_start[b], g.kfs};
+            real v[NA * NB * NC * ND];
+            StoreCb<NB, NC, ND, real> st{v};
+            quartet<real, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, g.ab[4 * kp],
+                                          g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boys, st);
+            RegDigest<NA, NB, NC, ND, real> cb;
+            cb.load(g.D, n, fa, fb, fc, fd);
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                        for (int d = 0; d < ND; ++d) cb(a, bb, c, d, v[((a * NB + bb) * NC + c) * ND + d]);
+            const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
+#pragma unroll
+            for (int i = 0; i < NA * NB; ++i) jab[i] += s * (double)cb.Jab[i];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) atomicAdd(&Kr[(size_t)a * n + fc + c], s * (double)cb.Kac[a * NC + c]);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) atomicAdd(&Kr[(size_t)a * n + fd + d], s * (double)cb.Kad[a * ND + d]);
+            }
+#pragma unroll
+            for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    atomicAdd(&Kr[(size_t)(NA + bb) * n + fc + c], s * (double)cb.Kbc[bb * NC + c]);
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    atomicAdd(&Kr[(size_t)(NA + bb) * n + fd + d], s * (double)cb.Kbd[bb * ND + d]);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    jk_add(&J[(size_t)(fc + c) * n + fd + d], s * (double)cb.Jcd[c * ND + d], g.jk_copies);
+        }
+#pragma unroll
+        for (int i = 0; i < NA * NB; ++i) {
+            const double w = wave_sum(jab[i]);
+            if (lane == 0) atomicAdd(&Jl[i], w);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+            const double x = Kr[i];
+            if (x != 0.0) {
+                const int r = i / n, col = i - r * n;
+                const int f = r < NA ? fa + r : fb + (r - NA);
+                jk_add(&K[(size_t)f * n + col], x, g.jk_copies);
+            }
+        }
+        for (int i = threadIdx.x; i < NA * NB; i += blockDim.x)
+            jk_add(&J[(size_t)(fa + i / NB) * n + fb + i % NB], Jl[i], g.jk_copies);
+        __syncthreads();
+    }
+}
+
 typedef void (*EriKernel)(EriArgs);
+// Getter modes of polyeri_class_kernel beyond the k_eri modes: the segment
+// kernel for MODE_JK and MODE_JK32.
+enum { GET_SEG_JK = 5, GET_SEG_JK32 = 6 };
 // Strided batched FP64 products on the MFMA units (kgemm.hip):
 // C_b(m,n) = sum_k A_b(m,k) B_b(k,n), X_b(r,c) = X[b*sX + r*sXr + c*sXc].
 enum { GEMM_F64 = 1, GEMM_BF16X3 = 2 };

@@ -237,8 +237,9 @@ def main(out_path):
 
 
 def write_kernel_units(out_dir):
-    """One translation unit per class (five modes each) plus the getter, so
-    the class kernels compile in parallel."""
+    """One translation unit per class (the five k_eri modes, and the segment
+    kernel as getter modes 5 and 6) plus the getter, so the class kernels
+    compile in parallel."""
     _, cls = classes()
     os.makedirs(out_dir, exist_ok=True)
     names = []
@@ -255,6 +256,9 @@ def write_kernel_units(out_dir):
             body.append("    switch (mode) {")
             for m in range(5):
                 body.append(f"    case {m}: return k_eri<{la}, {lb}, {lc}, {ld}, {m}>;")
+            # getter modes 5 and 6: the segment kernel in FP64 and mixed precision
+            body.append(f"    case 5: return k_eri_seg<{la}, {lb}, {lc}, {ld}, 0>;")
+            body.append(f"    case 6: return k_eri_seg<{la}, {lb}, {lc}, {ld}, 3>;")
             body += ["    }", "    return nullptr;", "}", ""]
         text = "\n".join(body)
         if not os.path.exists(path) or open(path).read() != text:

@@ -205,12 +205,17 @@ static double flop_model(int la, int lb, int lc, int ld) {
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-// How one work list is launched.  kind: LAUNCH_AUTO picks the register
-// kernel for classes of at most POLYERI_COOP_MIN_INTEGRALS integrals and the
-// cooperative kernel above; block: threads per block of the register kernel;
-// grid: block cap (0 = one block per 256 items for the register kernel,
-// 4096 for the cooperative one).  The autotuner fills these per list.
-enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2 };
+// How one work list is launched.  kind: LAUNCH_AUTO picks, for classes of
+// at most POLYERI_COOP_MIN_INTEGRALS integrals, the segment kernel (bra-row
+// segments with LDS-accumulated exchange rows) when its LDS fits and the
+// list is not density-screened, else the register kernel, and the
+// cooperative kernel above; block: threads per block of the register or
+// segment kernel; grid: block cap (0 = one block per 256 items for the
+// register kernel, one per segment for the segment kernel, 4096 for the
+// cooperative one).  The autotuner fills these per list.
+enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2, LAUNCH_SEG = 3 };
+// Items per bra-row segment of the segment kernel.
+static const int kSegItems = 2048;
 struct LaunchCfg {
     int kind = LAUNCH_AUTO, block = 256, grid = 0;
 };
@@ -226,6 +231,8 @@ struct ClassPairWork {
     int* d_bsoa;           // bucket offsets of class Y in the interleaved ket data
     int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
+    SegDesc* d_seg;        // bra-row segments of at most kSegItems items (molecular lists)
+    int nseg;
     LaunchCfg cfg;         // launch shape (autotuned or default)
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int both;              // periodic lists of two different classes: both bra/ket roles
@@ -337,11 +344,23 @@ static int dalloc(T** p, size_t n) {
 
 static unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// LDS bytes of the segment kernel for bra shells la, lb.
+static size_t seg_lds_bytes(int la, int lb, int nbf) {
+    return 8 * ((size_t)(ncart_h(la) + ncart_h(lb)) * nbf + (size_t)ncart_h(la) * ncart_h(lb));
+}
+
+// The launch kind LAUNCH_AUTO resolves to.
+static int auto_kind(const polyeri_ctx* c, int la, int lb, int lc, int ld, bool seg_ok) {
+    if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) return LAUNCH_COOP;
+    return seg_ok && seg_lds_bytes(la, lb, c->ib.nbf) <= 65536 ? LAUNCH_SEG : LAUNCH_REG;
+}
+
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
                         const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
                         const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr,
-                        LaunchCfg cfg = LaunchCfg(), const int* bsoa = nullptr) {
+                        LaunchCfg cfg = LaunchCfg(), const int* bsoa = nullptr, const SegDesc* segs = nullptr,
+                        int nseg = 0) {
     if (nwork <= 0) return 0;
     EriArgs a{};
     a.ksoa = c->d_ksoa; a.kfs = (long long)c->prims.size(); a.bucket_soa = bsoa;
@@ -359,11 +378,25 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
                 : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
                 : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
                 : kernel_for<MODE_TENSOR>(la, lb, lc, ld);
+    // the segment kernel needs the static list (no screened items), a J/K mode and LDS for the rows
+    const bool seg_ok = segs && nseg > 0 && !items && (mode == MODE_JK || mode == MODE_JK32) &&
+                        seg_lds_bytes(la, lb, c->ib.nbf) <= 65536;
     int kind = cfg.kind;
-    if (kind == LAUNCH_AUTO)
-        kind = ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS || !k
-                   ? LAUNCH_COOP : LAUNCH_REG;
+    if (kind == LAUNCH_AUTO) kind = k ? auto_kind(c, la, lb, lc, ld, seg_ok) : LAUNCH_COOP;
+    if (kind == LAUNCH_SEG && !seg_ok) kind = k ? LAUNCH_REG : LAUNCH_COOP;
     if (kind == LAUNCH_REG && !k) return fail(POLYERI_E_ARG, "no register kernel for class");
+    if (kind == LAUNCH_SEG) {
+        EriKernel ks = polyeri_class_kernel(la, lb, lc, ld, mode == MODE_JK ? GET_SEG_JK : GET_SEG_JK32);
+        if (!ks) return fail(POLYERI_E_ARG, "no segment kernel for class");
+        a.segs = segs;
+        a.nseg = nseg;
+        const int block = cfg.block == 64 || cfg.block == 128 ? cfg.block : 256;
+        unsigned grid = (unsigned)nseg;
+        if (cfg.grid > 0 && grid > (unsigned)cfg.grid) grid = (unsigned)cfg.grid;
+        hipLaunchKernelGGL(ks, dim3(grid), dim3(block), seg_lds_bytes(la, lb, c->ib.nbf), c->stream, a);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     if (kind == LAUNCH_COOP) {
         int ns = polyeri_coop_nslot(la, lb, lc, ld);
         if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
@@ -612,6 +645,8 @@ static int build_ctx(polyeri_ctx* c) {
         w.both = c->pbc && l.X != l.Y;
         w.d_rf = nullptr;
         w.d_items = nullptr;
+        w.d_seg = nullptr;
+        w.nseg = 0;
         w.prim_quartets = l.prim_quartets;
         w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
         w.d_row = nullptr;
@@ -620,6 +655,20 @@ static int build_ctx(polyeri_ctx* c) {
         if ((rc = dalloc(&w.d_rb, l.rb.size()))) return rc;
         HIPCHK(hipMemcpy(w.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
         if (!l.rb.empty()) HIPCHK(hipMemcpy(w.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
+        // Segment lists for FP64 molecular contexts only: a mixed-precision J/K
+        // build ((H2O)4 cc-pVDZ, cut 1e-5) with the segment kernel stopped on an
+        // illegal memory access whose cause is not established, so mixed
+        // contexts keep the register kernel.
+        if (!c->pbc && l.nwork > 0 && c->opts.precision != POLYERI_MIXED) {
+            std::vector<SegDesc> segs;
+            for (int r = 0; r < l.nrows; ++r)
+                for (long long t = l.row[r]; t < l.row[r + 1]; t += kSegItems)
+                    segs.push_back(SegDesc{t, r, (int)std::min<long long>(kSegItems, l.row[r + 1] - t)});
+            if (segs.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
+            w.nseg = (int)segs.size();
+            if ((rc = dalloc(&w.d_seg, segs.size()))) return rc;
+            HIPCHK(hipMemcpy(w.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+        }
         if (l.prec) {
             if ((rc = dalloc(&w.d_rf, l.rf.size()))) return rc;
             if (!l.rf.empty())
@@ -773,6 +822,18 @@ int polyeri_basis_read(const char* path, int natom, const int* iatom, const doub
 // cooperative kernel with grids of 2048, 4096 or 8192 wavefronts.  Each
 // candidate runs once untimed and reps times between events on the
 // context's density buffer (zeroed: the cost does not depend on values).
+// Event timings are diagnostics: where the runtime cannot time a pair of
+// events (under a counter-collecting profiler, for instance) the time reads
+// NaN and the pending error is cleared instead of failing the build.
+static double event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return std::nan("");
+    }
+    return ms;
+}
+
 int polyeri_autotune(polyeri_ctx* ctx, int reps) {
     if (!ctx) return fail(POLYERI_E_ARG, "polyeri_autotune: null context");
     if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_autotune: periodic context");
@@ -795,6 +856,8 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
         if (reg) {
             for (int b : {256, 128, 64}) { LaunchCfg x; x.kind = LAUNCH_REG; x.block = b; cand.push_back(x); }
             LaunchCfg x; x.kind = LAUNCH_REG; x.block = 256; x.grid = 2048; cand.push_back(x);
+            if (cw.nseg > 0 && seg_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536)
+                for (int b : {256, 128, 64}) { LaunchCfg y; y.kind = LAUNCH_SEG; y.block = b; cand.push_back(y); }
         }
         if (!reg || nint >= 36)
             for (int g : {2048, 4096, 8192}) { LaunchCfg x; x.kind = LAUNCH_COOP; x.grid = g; cand.push_back(x); }
@@ -805,13 +868,12 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
                 if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
                 rc = launch_class(c, mode, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
                                   cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr, x,
-                                  cw.d_bsoa);
+                                  cw.d_bsoa, cw.d_seg, cw.nseg);
             }
             if (rc) break;
             HIPCHK(hipEventRecord(e1, c->stream));
             HIPCHK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const double ms = event_ms(e0, e1);     // NaN where untimable: keeps the current pick
             if (ms < best) { best = ms; pick = x; }
         }
         if (rc) break;
@@ -902,7 +964,8 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
-                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg, cw.d_bsoa);
+                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg, cw.d_bsoa,
+                              cw.d_seg, cw.nseg);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -961,19 +1024,13 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
 }
 
 static int jk_collect_timing(polyeri_ctx* c) {
-    float ms = 0;
     HIPCHK(hipEventSynchronize(c->ev1));
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_jk_ms = ms;
+    c->last_jk_ms = event_ms(c->ev0, c->ev1);
     c->last_quartets = 0;
     for (size_t w = 0; w < c->work.size(); ++w)
         c->last_quartets += c->screened_last ? (long long)c->h_counts[w] : c->work[w].nwork;
     if (c->profiling)
-        for (size_t w = 0; w < c->work.size(); ++w) {
-            float m = 0;
-            HIPCHK(hipEventElapsedTime(&m, c->ev[w], c->ev[w + 1]));
-            c->last_class_ms[w] = m;
-        }
+        for (size_t w = 0; w < c->work.size(); ++w) c->last_class_ms[w] = event_ms(c->ev[w], c->ev[w + 1]);
     return 0;
 }
 
@@ -1183,8 +1240,7 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
         st->class_quartets[i] = w.nwork;
         st->class_prec[i] = w.prec;
         const int kind = w.cfg.kind != LAUNCH_AUTO ? w.cfg.kind
-                       : (ncart_h(w.la) * ncart_h(w.lb) * ncart_h(w.lc) * ncart_h(w.ld) > POLYERI_COOP_MIN_INTEGRALS
-                              ? LAUNCH_COOP : LAUNCH_REG);
+                       : auto_kind(ctx, w.la, w.lb, w.lc, w.ld, w.nseg > 0 && !ctx->pbc);
         st->class_kind[i] = kind;
         st->class_block[i] = kind == LAUNCH_COOP ? 64 : w.cfg.block;
         st->class_grid[i] = w.cfg.grid;
@@ -1430,11 +1486,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         HIPCHK(hipStreamSynchronize(s));
         if (info != 0) { rc = fail(POLYERI_E_LIB, "EIGENVECTORS NOT CONVERGED"); break; }
         {
-            float a = 0, b = 0;
-            HIPCHK(hipEventElapsedTime(&a, ef0, ef1));
-            HIPCHK(hipEventElapsedTime(&b, ctx->ev0, ctx->ev1));
-            fock_total += a * 1e-3;
-            jk_total += b * 1e-3;
+            fock_total += event_ms(ef0, ef1) * 1e-3;
+            jk_total += event_ms(ctx->ev0, ctx->ev1) * 1e-3;
             if ((rc = jk_collect_timing(ctx))) break;
             build_quartets += ctx->last_quartets;
         }
