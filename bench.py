@@ -203,7 +203,9 @@ def main():
     # scaled by the copy kernel; counters cannot be read inside this process)
     traffic, traffic_src = None, None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
-    dkern = f"void k_eri<{', '.join(dc['cls'])}, 0>(EriArgs)"
+    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri"}[dc["kernel"]]
+    dkern = (f"void {kfn}<{', '.join(dc['cls'])}, 0>(EriArgs)" if kfn != "k_eri_coop"
+             else "void k_eri_coop<0>(EriArgs, int, int, int, int, int)")
     if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
         with open(tpath) as f:
             tk = json.load(f)["kernels"].get(dkern)
@@ -239,7 +241,7 @@ def main():
         "model_tflops_achieved": st["model_flops"] / s_iter / 1e12,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "compute": "fp64 VALU (same 78.6 TFLOP/s peak as FP64 matrix)",
-                     "kernel": f"k_eri<{','.join(dc['cls'])},JK>", "achieved": achieved,
+                     "kernel": f"{kfn}<{','.join(dc['cls'])},JK>", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic, "traffic_source": traffic_src, "kernel_ms": float(cls_ms[dom]),
                      "kernel_model_flops": dc["flops"]},

@@ -470,6 +470,7 @@ static void free_ctx(polyeri_ctx* c) {
         hipFree(w.d_row); hipFree(w.d_rb);
         if (w.d_rf) hipFree(w.d_rf);
         if (w.d_items) hipFree(w.d_items);
+        if (w.d_seg) hipFree(w.d_seg);
     }
     hipFree(c->d_pcell); hipFree(c->d_Dc); hipFree(c->d_Jc); hipFree(c->d_Kc);
     hipFree(c->d_pcc); hipFree(c->d_jcc); hipFree(c->d_kcc);
