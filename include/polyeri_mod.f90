@@ -12,6 +12,7 @@ MODULE POLYERI
    IMPLICIT NONE
 
    INTEGER(C_INT), PARAMETER :: POLYERI_FP64 = 0, POLYERI_MIXED = 1
+   INTEGER(C_INT), PARAMETER :: POLYERI_GEMM_ROCBLAS = 0, POLYERI_GEMM_MFMA_F64 = 1, POLYERI_GEMM_MFMA_BF16X3 = 2
 
    TYPE, BIND(C) :: POLYERI_OPTS
       REAL(C_DOUBLE) :: SCHWARZ_TOL
@@ -98,6 +99,79 @@ MODULE POLYERI
          TYPE(C_PTR), VALUE :: CTX
          REAL(C_DOUBLE) :: CO(*), EPS(*), E2S, E2T
          INTEGER(C_INT), VALUE :: ICORE, IOCC, NALL, IVIRTCORE
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_MP2_ENERGY_EX(CTX, CO, EPS, ICORE, IOCC, NALL, IVIRTCORE, TRANSFORM, &
+            E2S, E2T) BIND(C, NAME="polyeri_mp2_energy_ex")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         REAL(C_DOUBLE) :: CO(*), EPS(*), E2S, E2T
+         INTEGER(C_INT), VALUE :: ICORE, IOCC, NALL, IVIRTCORE, TRANSFORM
+      END FUNCTION
+
+      ! periodic (1-D, HELIX = 0): cells -CEL1..CEL1 of P_C, C_C, X_C are
+      ! contiguous NCGS x NCGS blocks, k-point arrays hold IX = -KVC..KVC-1
+      INTEGER(C_INT) FUNCTION POLYERI_CTX_CREATE_PERIODIC(NPGS, NCGS, PAX, PAY, PAZ, PGSX, PGSY, PGSZ, ZT, &
+            CC, NATOM, IATOM, XYZ, PERIOD, CEL1, CEL2, OPTS, CTX) BIND(C, NAME="polyeri_ctx_create_periodic")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR, POLYERI_OPTS
+         INTEGER(C_INT), VALUE :: NPGS, NCGS, NATOM, CEL1, CEL2
+         INTEGER(C_INT) :: PAX(*), PAY(*), PAZ(*), IATOM(*)
+         REAL(C_DOUBLE) :: PGSX(*), PGSY(*), PGSZ(*), ZT(*), CC(*), XYZ(*)
+         REAL(C_DOUBLE), VALUE :: PERIOD
+         TYPE(POLYERI_OPTS) :: OPTS
+         TYPE(C_PTR) :: CTX
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_JK_BUILD_CELLS(CTX, P_C, C_C, X_C) BIND(C, NAME="polyeri_jk_build_cells")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         REAL(C_DOUBLE) :: P_C(*), C_C(*), X_C(*)
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_SCF_PERIODIC(CTX, OPTS, KVC, P_C, EPS, TRACE, RES, CO, NALL) &
+            BIND(C, NAME="polyeri_scf_periodic")
+         IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, C_PTR, POLYERI_SCF_OPTS, POLYERI_SCF_RESULT
+         TYPE(C_PTR), VALUE :: CTX
+         TYPE(POLYERI_SCF_OPTS) :: OPTS
+         INTEGER(C_INT), VALUE :: KVC
+         REAL(C_DOUBLE) :: P_C(*), EPS(*), TRACE(*)
+         TYPE(POLYERI_SCF_RESULT) :: RES
+         COMPLEX(C_DOUBLE_COMPLEX) :: CO(*)
+         INTEGER(C_INT) :: NALL(*)
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_MP2_ENERGY_PERIODIC(CTX, KVC, CO, EPS, NALL, ICORE, IOCC, IVIRTCORE, &
+            TRANSFORM, E2S, E2T, SECONDS) BIND(C, NAME="polyeri_mp2_energy_periodic")
+         IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         INTEGER(C_INT), VALUE :: KVC, ICORE, IOCC, IVIRTCORE, TRANSFORM
+         COMPLEX(C_DOUBLE_COMPLEX) :: CO(*)
+         REAL(C_DOUBLE) :: EPS(*), E2S, E2T, SECONDS
+         INTEGER(C_INT) :: NALL(*)
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_REDUCED_CEL1(CTX) BIND(C, NAME="polyeri_reduced_cel1")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+      END FUNCTION
+
+      ! .ao integral files; PATH is NUL-terminated (TRIM(NAME)//C_NULL_CHAR)
+      INTEGER(C_INT) FUNCTION POLYERI_STORE_FOURINDEX_ERI(CTX, PATH, INTTOL, STORED) &
+            BIND(C, NAME="polyeri_store_fourindex_eri")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR, C_CHAR, C_LONG_LONG
+         TYPE(C_PTR), VALUE :: CTX
+         CHARACTER(KIND=C_CHAR) :: PATH(*)
+         REAL(C_DOUBLE), VALUE :: INTTOL
+         INTEGER(C_LONG_LONG) :: STORED
+      END FUNCTION
+
+      INTEGER(C_INT) FUNCTION POLYERI_RESTORE_FOURINDEX_ERI(PATH, NCGS, FERI, RESTORED) &
+            BIND(C, NAME="polyeri_restore_fourindex_eri")
+         IMPORT :: C_INT, C_DOUBLE, C_CHAR, C_LONG_LONG
+         CHARACTER(KIND=C_CHAR) :: PATH(*)
+         INTEGER(C_INT), VALUE :: NCGS
+         REAL(C_DOUBLE) :: FERI(*)
+         INTEGER(C_LONG_LONG) :: RESTORED
       END FUNCTION
 
       FUNCTION POLYERI_LAST_ERROR() BIND(C, NAME="polyeri_last_error")
