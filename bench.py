@@ -186,7 +186,8 @@ def main():
         scf = {}
         plain = pa.Context(b, schwarz_tol=1e-12)
         plain.set_graph(False)
-        runs = (("fp64_plain", plain, {}), ("fp64_batched", ctx, {}), ("mixed_batched", cmx, {}),
+        runs = (("fp64_plain", plain, {}), ("fp64_batched", ctx, {}),
+                ("fp64_incremental", ctx, {"incremental": 8, "dens_tol": 1e-11}), ("mixed_batched", cmx, {}),
                 ("mixed_incremental", cmx, {"incremental": 8, "dens_tol": 1e-11}))
         for name, c, kw in runs:
             t0 = time.perf_counter()
@@ -194,7 +195,7 @@ def main():
             scf[name] = {"e_total": r["e_total"], "iterations": r["iterations"], "converged": bool(r["converged"]),
                          "wall_s": time.perf_counter() - t0, "jk_s": r["jk_seconds"], "quartets": r["quartets"],
                          **kw}
-        for name in ("fp64_batched", "mixed_batched", "mixed_incremental"):
+        for name in ("fp64_batched", "fp64_incremental", "mixed_batched", "mixed_incremental"):
             scf[name]["dE_vs_fp64_plain"] = scf[name]["e_total"] - scf["fp64_plain"]["e_total"]
             scf[name]["wall_speedup_vs_fp64_plain"] = scf["fp64_plain"]["wall_s"] / scf[name]["wall_s"]
 

@@ -62,6 +62,14 @@ struct EriArgs {
     double *Jc, *Kc;           // jk_copies x (2 cel1 + 1) x nbf^2
     const SegDesc* segs;       // segment kernel: bra-row segments of the list
     int nseg;
+    // segment kernel, density screening in the kernel (dtol > 0): skip items
+    // with Q_ab Q_cd max|D| (six shell blocks) < dtol, count the rest in kept
+    double dtol;
+    const double* Qs;
+    const int2* psh;           // shells of each pair
+    const double* dsh;         // nsh x nsh shell-block maxima of |D|
+    int nsh;
+    unsigned* kept;
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -410,10 +418,28 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
         double jab[NA * NB];
 #pragma unroll
         for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
-        for (long long t = seg.t0 + threadIdx.x; t < seg.t0 + seg.n; t += blockDim.x) {
-            int pos, b;
-            decode_in_row(g, seg.row, t, pos, b);
-            const int kp = g.ket_ids[pos];
+        const bool screen = g.dtol > 0.0;
+        for (long long t0 = seg.t0; t0 < seg.t0 + seg.n; t0 += blockDim.x) {
+            const long long t = t0 + threadIdx.x;
+            int pos = 0, b = 0, kp = 0;
+            bool live = t < seg.t0 + seg.n;
+            if (live) {
+                decode_in_row(g, seg.row, t, pos, b);
+                kp = g.ket_ids[pos];
+                if (screen) {
+                    const int2 ab = g.psh[bp], cd = g.psh[kp];
+                    const size_t ns = g.nsh, a = ab.x, bb = ab.y, c = cd.x, d = cd.y;
+                    double m = fmax(g.dsh[c * ns + d], g.dsh[a * ns + bb]);
+                    m = fmax(m, fmax(g.dsh[bb * ns + d], g.dsh[bb * ns + c]));
+                    m = fmax(m, fmax(g.dsh[a * ns + d], g.dsh[a * ns + c]));
+                    live = g.Qs[bp] * g.Qs[kp] * m >= g.dtol;
+                }
+            }
+            if (screen) {
+                const unsigned long long mask = __ballot(live);
+                if (lane == 0 && mask) atomicAdd(g.kept, (unsigned)__popcll(mask));
+            }
+            if (!live) continue;
             const PairMeta mk = g.meta[kp];
             const int fc = mk.fa, fd = mk.fb;
             const SoaKet kv{g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]),

@@ -355,12 +355,31 @@ static int auto_kind(const polyeri_ctx* c, int la, int lb, int lc, int ld, bool 
     return seg_ok && seg_lds_bytes(la, lb, c->ib.nbf) <= 65536 ? LAUNCH_SEG : LAUNCH_REG;
 }
 
+// Density screening done inside the segment kernel.
+struct SegScreen {
+    const double* Q;
+    const int2* psh;
+    const double* dsh;
+    int nsh;
+    double tol;
+    unsigned* kept;
+};
+
+// Whether a list's J/K launch (unscreened) runs the segment kernel.
+static bool list_uses_seg(const polyeri_ctx* c, const ClassPairWork& cw) {
+    if (cw.nseg <= 0 || seg_lds_bytes(cw.la, cw.lb, c->ib.nbf) > 65536) return false;
+    const int mode = cw.prec ? MODE_JK32 : MODE_JK;
+    if (!polyeri_class_kernel(cw.la, cw.lb, cw.lc, cw.ld, mode)) return false;
+    const int kind = cw.cfg.kind == LAUNCH_AUTO ? auto_kind(c, cw.la, cw.lb, cw.lc, cw.ld, true) : cw.cfg.kind;
+    return kind == LAUNCH_SEG;
+}
+
 static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld, const int* bra,
                         const int* ket, const long long* row, int nrows, long long nwork, double* out,
                         const int* rb = nullptr, const int* bstart = nullptr, int nbucket = 0,
                         const int* rf = nullptr, const int2* items = nullptr, const unsigned* count = nullptr,
                         LaunchCfg cfg = LaunchCfg(), const int* bsoa = nullptr, const SegDesc* segs = nullptr,
-                        int nseg = 0) {
+                        int nseg = 0, const SegScreen* scr = nullptr) {
     if (nwork <= 0) return 0;
     EriArgs a{};
     a.ksoa = c->d_ksoa; a.kfs = (long long)c->prims.size(); a.bucket_soa = bsoa;
@@ -390,6 +409,10 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
         if (!ks) return fail(POLYERI_E_ARG, "no segment kernel for class");
         a.segs = segs;
         a.nseg = nseg;
+        if (scr) {
+            a.dtol = scr->tol; a.Qs = scr->Q; a.psh = scr->psh; a.dsh = scr->dsh; a.nsh = scr->nsh;
+            a.kept = scr->kept;
+        }
         const int block = cfg.block == 64 || cfg.block == 128 ? cfg.block : 256;
         unsigned grid = (unsigned)nseg;
         if (cfg.grid > 0 && grid > (unsigned)cfg.grid) grid = (unsigned)cfg.grid;
@@ -951,7 +974,7 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
             return fail(POLYERI_E_HIP, "shell density maxima launch failed");
         for (size_t w = 0; w < c->work.size(); ++w) {
             const ClassPairWork& cw = c->work[w];
-            if (cw.nwork <= 0) continue;
+            if (cw.nwork <= 0 || list_uses_seg(c, cw)) continue;   // segment lists screen in the kernel
             EriArgs a{};
             a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = cw.nwork;
             a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket; a.rb_first = cw.d_rf;
@@ -963,10 +986,12 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
+        const bool seg = list_uses_seg(c, cw);
+        const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
-                              screen ? cw.d_items : nullptr, screen ? c->d_counts + w : nullptr, cw.cfg, cw.d_bsoa,
-                              cw.d_seg, cw.nseg);
+                              screen && !seg ? cw.d_items : nullptr, screen && !seg ? c->d_counts + w : nullptr,
+                              cw.cfg, cw.d_bsoa, cw.d_seg, cw.nseg, screen && seg ? &scr : nullptr);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -986,7 +1011,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     const bool screen = dens_tol > 0.0;
     if (screen)
         for (ClassPairWork& w : c->work)
-            if (!w.d_items && w.nwork > 0) {
+            if (!w.d_items && w.nwork > 0 && !list_uses_seg(c, w)) {   // segment lists keep no items
                 int rc = dalloc(&w.d_items, (size_t)w.nwork);
                 if (rc) return rc;
             }
