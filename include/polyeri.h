@@ -246,6 +246,15 @@ int polyeri_mp2_energy(polyeri_ctx *ctx, const double *co, const double *eps, in
 #define POLYERI_GEMM_MFMA_BF16X3 2
 int polyeri_mp2_energy_ex(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc,
                           int nall, int ivirtcore, int transform, double *e2s, double *e2t);
+/* MP2_ENERGY_ALG1 with the quasi-particle bands (IOPTN(44) = nqp > 0,
+ * src/pmp2energy.f90:445-522, 549-558) at one k-point: E(2) as
+ * polyeri_mp2_energy_ex and QEPSILON for bands max(iocc - nqp, icore) ..
+ * min(iocc + nqp, nall - ivirtcore) - 1 (0-based) in qeps (ncgs entries;
+ * the other bands copy eps), the second-order diagonal, frequency-
+ * independent band energies the reference prints as "MP2 QUASI-PARTICLE
+ * ENERGY BANDS". */
+int polyeri_mp2_energy_qp(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc, int nall,
+                          int ivirtcore, int nqp, int transform, double *e2s, double *e2t, double *qeps);
 /* One product C = A B on those kernels (transform = POLYERI_GEMM_MFMA_F64 or
  * _BF16X3); A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn],
  * C(m,n) = c[m*scm + n*scn]; device pointers. */

@@ -109,6 +109,14 @@ MODULE POLYERI
          INTEGER(C_INT), VALUE :: ICORE, IOCC, NALL, IVIRTCORE, TRANSFORM
       END FUNCTION
 
+      INTEGER(C_INT) FUNCTION POLYERI_MP2_ENERGY_QP(CTX, CO, EPS, ICORE, IOCC, NALL, IVIRTCORE, NQP, TRANSFORM, &
+            E2S, E2T, QEPS) BIND(C, NAME="polyeri_mp2_energy_qp")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         REAL(C_DOUBLE) :: CO(*), EPS(*), E2S, E2T, QEPS(*)
+         INTEGER(C_INT), VALUE :: ICORE, IOCC, NALL, IVIRTCORE, NQP, TRANSFORM
+      END FUNCTION
+
       ! periodic (1-D, HELIX = 0): cells -CEL1..CEL1 of P_C, C_C, X_C are
       ! contiguous NCGS x NCGS blocks, k-point arrays hold IX = -KVC..KVC-1
       INTEGER(C_INT) FUNCTION POLYERI_CTX_CREATE_PERIODIC(NPGS, NCGS, PAX, PAY, PAZ, PGSX, PGSY, PGSZ, ZT, &

@@ -261,3 +261,41 @@ def mp2(feri, co, eps, icore, iocc, nall, ivirtcore=0):
     lib().or_mp2(nc, _ptr(feri), _ptr(co), _ptr(eps), icore, iocc, nall, ivirtcore,
                  ctypes.byref(s), ctypes.byref(t))
     return s.value, t.value
+
+
+def mp2_qp(feri, co, eps, icore, iocc, nall, nqp, ivirtcore=0):
+    """MP2 quasi-particle band energies of MP2_ENERGY_ALG1 at one k-point
+    (IOPTN(44) = nqp > 0; src/pmp2energy.f90:445-522, normalisation :549-558):
+    for every band g from max(iocc - nqp, icore) to min(iocc + nqp, nall -
+    ivirtcore) - 1 (0-based),
+
+      QEPSILON(g) = eps(g) + sum_{b,j,a} [2 (ga|jb)^2 - (ga|jb)(ja|gb)] / (e_g + e_j - e_a - e_b)
+                           + sum_{b,j,i} [2 (ig|jb)^2 - (ig|jb)(jg|ib)] / (e_g + e_b - e_i - e_j)
+
+    i, j over icore..iocc-1, a, b over iocc..nall-ivirtcore-1; (pq|rs) in
+    chemists' notation from the AO tensor feri(I,J,K,L) and the real
+    orbitals co (columns).  Returns (band indices, QEPSILON)."""
+    feri = np.asarray(feri, dtype=np.float64)
+    co = np.asarray(co, dtype=np.float64)
+    eps = np.asarray(eps, dtype=np.float64)
+    hi = nall - ivirtcore
+    g0, g1 = max(iocc - nqp, icore), min(iocc + nqp, hi)
+    occ, vir = np.arange(icore, iocc), np.arange(iocc, hi)
+    mo = np.arange(icore, hi)
+    c = co[:, mo]
+    t = np.einsum("pqrs,pi,qa,rj,sb->iajb", feri, c, c, c, c, optimize=True)   # (ia|jb) over icore..hi-1
+    o, v = occ - icore, vir - icore
+    e = eps[mo]
+    out = []
+    for g in range(g0, g1):
+        x = g - icore
+        tg = t[x][np.ix_(v, o, v)]                       # (ga|jb)[a, j, b]
+        ts = t[np.ix_(o, v)][:, :, x][:, :, v]           # (ja|gb)[j, a, b]
+        d1 = e[x] + e[o][None, :, None] - e[v][:, None, None] - e[v][None, None, :]
+        q = np.sum((2.0 * tg ** 2 - tg * np.transpose(ts, (1, 0, 2))) / d1)
+        ti = t[np.ix_(o, [x], o, v)][:, 0]               # (ig|jb)[i, j, b]
+        tj = np.transpose(ti, (1, 0, 2))                 # (jg|ib)[i, j, b] = ti[j, i, b]
+        d2 = e[x] + e[v][None, None, :] - e[o][:, None, None] - e[o][None, :, None]
+        q += np.sum((2.0 * ti ** 2 - ti * tj) / d2)
+        out.append(eps[g] + q)
+    return list(range(g0, g1)), np.array(out)

@@ -278,6 +278,19 @@ class Context:
             return s.value, t.value, sec.value
         return s.value, t.value
 
+    def mp2_quasiparticle(self, co, eps, icore, iocc, nall, nqp, ivirtcore=0, transform="rocblas"):
+        """E(2) singlet, triplet and the MP2 quasi-particle band energies
+        QEPSILON (IOPTN(44) = nqp): returns (e2s, e2t, qeps) with qeps of
+        length ncgs (bands outside the corrected range copy eps)."""
+        co = np.asfortranarray(co, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        q = np.zeros(self.ncgs)
+        s, t = ctypes.c_double(), ctypes.c_double()
+        mode = {"rocblas": 0, "f64": 1, "bf16x3": 2}[transform]
+        check(lib().polyeri_mp2_energy_qp(self.h, _p(co), _p(eps), icore, iocc, nall, ivirtcore, int(nqp), mode,
+                                          ctypes.byref(s), ctypes.byref(t), _p(q)))
+        return s.value, t.value, q
+
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
         """E(2) singlet and triplet parts.  transform: None (rocBLAS dgemm),
         "f64" (own FP64 MFMA kernel) or "bf16x3" (own BF16-split MFMA kernel)."""

@@ -21,7 +21,7 @@ EXPORTS = [
     "polyeri_mlp_create", "polyeri_mlp_destroy", "polyeri_mlp_nparam", "polyeri_mlp_params",
     "polyeri_mlp_grads", "polyeri_mlp_forward", "polyeri_mlp_train_step", "polyeri_gemm_bf16",
     "polyeri_mp2_energy_ex", "polyeri_gemm_f64", "polyeri_ctx_create_periodic", "polyeri_jk_build_cells",
-    "polyeri_scf_periodic", "polyeri_mp2_energy_periodic", "polyeri_store_fourindex_eri", "polyeri_restore_fourindex_eri",
+    "polyeri_scf_periodic", "polyeri_mp2_energy_periodic", "polyeri_mp2_energy_qp", "polyeri_store_fourindex_eri", "polyeri_restore_fourindex_eri",
     "polyeri_restore_fourindex_eri_cells", "polyeri_eri_tensor_cells", "polyeri_reduced_cel1",
 ]
 
@@ -93,6 +93,7 @@ def lib():
     L.polyeri_jk_build_cells.argtypes = [p, p, p, p]
     L.polyeri_scf_periodic.argtypes = [p, p, i, p, p, p, p, p, p]
     L.polyeri_mp2_energy_periodic.argtypes = [p, i, p, p, p, i, i, i, i, p, p, p]
+    L.polyeri_mp2_energy_qp.argtypes = [p, p, p, i, i, i, i, i, i, p, p, p]
     L.polyeri_store_fourindex_eri.argtypes = [p, ctypes.c_char_p, d, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri.argtypes = [ctypes.c_char_p, i, p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri_cells.argtypes = [ctypes.c_char_p, i, i, i, p, ctypes.POINTER(ctypes.c_longlong)]

@@ -144,20 +144,26 @@ __global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ ds
 }
 
 // E(2) from T(b, j, a, i) = (ia|jb): src/pmp2energy.f90:395-444 at one k-point
-__global__ void k_mp2_energy(const double* __restrict__ T, const double* __restrict__ eps, int no,
-                             int nv, int icore, int iocc, double* __restrict__ acc) {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t tot = (size_t)no * nv * no * nv;
+// E(2) singlet and triplet (src/pmp2energy.f90:395-433 at one k-point) from
+// T[i'][a'][j'][b] = (ia|jb), b fastest: b over the nv virtuals, j' and i'
+// over the "occupied" range icore.. (nop wide), a' over the "virtual" range
+// starting aoff below iocc (nvp wide); the energy uses i, j < no and the
+// true virtuals a' = aoff + a.
+__global__ void k_mp2_energy(const double* __restrict__ T, const double* __restrict__ eps, int no, int nv,
+                             int nop, int nvp, int aoff, int icore, int iocc, double* __restrict__ acc) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)no * nv * no * nv;
     double es = 0.0, et = 0.0;
     if (t < tot) {
-        int b = (int)(t % nv);
+        const int b = (int)(t % nv);
         size_t r = t / nv;
-        int j = (int)(r % no); r /= no;
-        int a = (int)(r % nv);
-        int i = (int)(r / nv);
-        double x = T[t];
-        double xp = T[b + (size_t)nv * (i + (size_t)no * (a + (size_t)nv * j))];  // (ja|ib)
-        double den = eps[icore + i] + eps[icore + j] - eps[iocc + a] - eps[iocc + b];
+        const int j = (int)(r % no); r /= no;
+        const int a = (int)(r % nv);
+        const int i = (int)(r / nv);
+        const size_t ap = (size_t)aoff + a;
+        const double x = T[b + (size_t)nv * (j + (size_t)nop * (ap + (size_t)nvp * i))];
+        const double xp = T[b + (size_t)nv * (i + (size_t)nop * (ap + (size_t)nvp * j))];   // (ja|ib)
+        const double den = eps[icore + i] + eps[icore + j] - eps[iocc + a] - eps[iocc + b];
         es = (0.5 * x * x + 0.5 * x * xp) / den;
         et = (1.5 * x * x - 1.5 * x * xp) / den;
     }
@@ -176,6 +182,44 @@ __global__ void k_mp2_energy(const double* __restrict__ T, const double* __restr
         atomicAdd(&acc[0], ss[0]);
         atomicAdd(&acc[1], st[0]);
     }
+}
+
+// MP2 quasi-particle sums of band g = g0 + blockIdx.y (src/pmp2energy.f90:445-522
+// at one k-point), same T layout; g sits in the i' slot (first sum) and in the
+// a' slot (second sum):
+//   q(g) = sum_{b,j,a} [2 (ga|jb)^2 - (ga|jb)(ja|gb)] / (e_g + e_j - e_a - e_b)
+//        + sum_{b,j,i} [2 (ig|jb)^2 - (ig|jb)(jg|ib)] / (e_g + e_b - e_i - e_j)
+__global__ void k_mp2_qp(const double* __restrict__ T, const double* __restrict__ eps, int no, int nv, int nop,
+                         int nvp, int aoff, int icore, int iocc, int vlo, int g0, double* __restrict__ q) {
+    const int g = g0 + blockIdx.y;
+    const int gi = g - icore, ga = g - vlo;
+    const size_t n1 = (size_t)nv * no * nv, n2 = (size_t)nv * no * no;
+    double s = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2; t += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(t % nv);
+        if (t < n1) {
+            const size_t r = t / nv;
+            const int j = (int)(r % no), a = (int)(r / no);
+            const size_t ap = (size_t)aoff + a;
+            const double x = T[b + (size_t)nv * (j + (size_t)nop * (ap + (size_t)nvp * gi))];
+            const double xs = T[b + (size_t)nv * (gi + (size_t)nop * (ap + (size_t)nvp * j))];
+            s += (2.0 * x * x - x * xs) / (eps[g] + eps[icore + j] - eps[iocc + a] - eps[iocc + b]);
+        } else {
+            const size_t r = (t - n1) / nv;
+            const int j = (int)(r % no), i = (int)(r / no);
+            const double x = T[b + (size_t)nv * (j + (size_t)nop * (ga + (size_t)nvp * i))];
+            const double xs = T[b + (size_t)nv * (i + (size_t)nop * (ga + (size_t)nvp * j))];
+            s += (2.0 * x * x - x * xs) / (eps[g] + eps[iocc + b] - eps[icore + i] - eps[icore + j]);
+        }
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&q[blockIdx.y], sh[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1626,13 +1670,21 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
 // ---------------------------------------------------------------------------
 // MP2 (one k-point): AO integrals on the device, four quarter transforms
 // ---------------------------------------------------------------------------
+// nqp > 0: also the quasi-particle bands (IOPTN(44)): the occupied-side
+// transforms run over icore .. min(iocc + nqp, nall - ivirtcore) - 1 and the
+// virtual side of the third quarter from max(iocc - nqp, icore), as the
+// reference's TOK / TVOKK / TOVOKK ranges (src/pmp2energy.f90:140-154).
 static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
-                      int ivirtcore, int gemm, double* e2s, double* e2t) {
+                      int ivirtcore, int gemm, double* e2s, double* e2t, int nqp = 0, double* qeps = nullptr) {
     if (!ctx || !co || !eps || !e2s || !e2t) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: null argument");
-    int no = iocc - icore, nv = nall - ivirtcore - iocc;
+    const int hi = nall - ivirtcore;
+    int no = iocc - icore, nv = hi - iocc;
     if (no <= 0 || nv <= 0 || icore < 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: empty orbital space");
+    if (nqp < 0 || (nqp > 0 && !qeps)) return fail(POLYERI_E_ARG, "polyeri_mp2_energy: bad quasi-particle request");
     if (gemm != POLYERI_GEMM_ROCBLAS && gemm != POLYERI_GEMM_MFMA_F64 && gemm != POLYERI_GEMM_MFMA_BF16X3)
         return fail(POLYERI_E_ARG, "polyeri_mp2_energy: unknown transform precision");
+    const int ohi = std::min(iocc + nqp, hi), vlo = std::max(iocc - nqp, icore);
+    const int nop = ohi - icore, nvp = hi - vlo, aoff = iocc - vlo;
     HIPCHK(hipSetDevice(ctx->device));
     int rc;
     if ((rc = ensure_blas(ctx))) return rc;
@@ -1641,57 +1693,75 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     int n = ctx->ib.nbf, nc = ctx->rb.ncgs;
     size_t n4 = (size_t)n * n * n * n;
     DevBuf buf;
-    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc;
-    size_t x1 = (size_t)nv * n * n * n, x2 = (size_t)nv * no * n * n, x3 = (size_t)nv * no * nv * n;
-    size_t x4 = (size_t)nv * no * nv * no;
+    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc, *dQ;
+    size_t x1 = (size_t)nv * n * n * n, x2 = (size_t)nv * nop * n * n, x3 = (size_t)nv * nop * nvp * n;
+    size_t x4 = (size_t)nv * nop * nvp * nop;
     if ((rc = buf.get(&dTen, n4)) || (rc = buf.get(&dCO, (size_t)nc * nc)) || (rc = buf.get(&dC, (size_t)n * nc)) ||
         (rc = buf.get(&dX1, x1)) || (rc = buf.get(&dX2, x2)) || (rc = buf.get(&dX3, x3)) || (rc = buf.get(&dT4, x4)) ||
-        (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)))
+        (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)) || (rc = buf.get(&dQ, (size_t)nc)))
         return rc;
     if ((rc = tensor_device(ctx, dTen))) return rc;
     HIPCHK(hipMemcpyAsync(dCO, co, (size_t)nc * nc * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dEps, eps, (size_t)nc * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_mo_int, dim3(nblk((size_t)n * nc)), dim3(256), 0, s, dCO, nc, ctx->d_ref, ctx->d_scale, n, nc, dC);
     const double one = 1.0, zero = 0.0;
-    const double* Cv = dC + (size_t)n * iocc;    // virtual columns
-    const double* Co = dC + (size_t)n * icore;   // active occupied columns
+    const double* Cv = dC + (size_t)n * iocc;    // virtual columns (b)
+    const double* Co = dC + (size_t)n * icore;   // occupied-side columns (j', i')
+    const double* Ca = dC + (size_t)n * vlo;     // virtual-side columns (a')
     if (gemm == POLYERI_GEMM_ROCBLAS) {
         // tensor A(sigma, lambda, nu, mu) column-major; X1(b, lam nu mu) = Cv^T A
         RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, n * n * n, n, &one, Cv, n, dTen,
                             n, &zero, dX1, nv));
         // X2(b, j, nu mu) = X1(b, lam, nu mu) Co(lam, j)  [batched over nu mu]
-        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, no, n, &one, dX1, nv,
-                                            (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv, (rocblas_stride)nv * no,
-                                            n * n));
-        // X3(bj, a, mu) = X2(bj, nu, mu) Cv(nu, a)  [batched over mu]
-        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * no, nv, n, &one,
-                                            dX2, nv * no, (rocblas_stride)nv * no * n, Cv, n, 0, &zero, dX3, nv * no,
-                                            (rocblas_stride)nv * no * nv, n));
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, nop, n, &one, dX1,
+                                            nv, (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv,
+                                            (rocblas_stride)nv * nop, n * n));
+        // X3(bj, a, mu) = X2(bj, nu, mu) Ca(nu, a)  [batched over mu]
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * nop, nvp, n, &one,
+                                            dX2, nv * nop, (rocblas_stride)nv * nop * n, Ca, n, 0, &zero, dX3, nv * nop,
+                                            (rocblas_stride)nv * nop * nvp, n));
         // T(bja, i) = X3(bja, mu) Co(mu, i)
-        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * no * nv, no, n, &one, dX3,
-                            nv * no * nv, Co, n, &zero, dT4, nv * no * nv));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * nop * nvp, nop, n, &one, dX3,
+                            nv * nop * nvp, Co, n, &zero, dT4, nv * nop * nvp));
     } else {
         // the same four quarter transforms on this library's MFMA kernels
         const int mode = gemm == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
-        const long N = n, NV = nv, NO = no;
+        const long N = n, NV = nv, NO = nop, NA = nvp;
         int r = polyeri_gemm_launch(mode, nv, n * n * n, n, Cv, N, 1, 0, dTen, 1, N, 0, dX1, 1, NV, 0, 1, s);
-        if (!r) r = polyeri_gemm_launch(mode, nv, no, n, dX1, 1, NV, NV * N, Co, 1, N, 0, dX2, 1, NV, NV * NO, n * n, s);
+        if (!r) r = polyeri_gemm_launch(mode, nv, nop, n, dX1, 1, NV, NV * N, Co, 1, N, 0, dX2, 1, NV, NV * NO, n * n, s);
         if (!r)
-            r = polyeri_gemm_launch(mode, nv * no, nv, n, dX2, 1, NV * NO, NV * NO * N, Cv, 1, N, 0, dX3, 1, NV * NO,
-                                    NV * NO * NV, n, s);
-        if (!r) r = polyeri_gemm_launch(mode, nv * no * nv, no, n, dX3, 1, NV * NO * NV, 0, Co, 1, N, 0, dT4, 1,
-                                        NV * NO * NV, 0, 1, s);
+            r = polyeri_gemm_launch(mode, nv * nop, nvp, n, dX2, 1, NV * NO, NV * NO * N, Ca, 1, N, 0, dX3, 1, NV * NO,
+                                    NV * NO * NA, n, s);
+        if (!r) r = polyeri_gemm_launch(mode, nv * nop * nvp, nop, n, dX3, 1, NV * NO * NA, 0, Co, 1, N, 0, dT4, 1,
+                                        NV * NO * NA, 0, 1, s);
         if (r) return fail(POLYERI_E_HIP, "MP2 transform launch failed");
     }
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
-    hipLaunchKernelGGL(k_mp2_energy, dim3(nblk(x4)), dim3(256), 0, s, dT4, dEps, no, nv, icore, iocc, dAcc);
+    hipLaunchKernelGGL(k_mp2_energy, dim3(nblk((size_t)no * nv * no * nv)), dim3(256), 0, s, dT4, dEps, no, nv, nop,
+                       nvp, aoff, icore, iocc, dAcc);
     HIPCHK(hipGetLastError());
+    const int ng = nqp > 0 ? ohi - vlo : 0;
+    if (ng > 0) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dQ, (size_t)ng, 0.0);
+        const size_t terms = (size_t)nv * no * (nv + no);
+        const unsigned gx = (unsigned)std::min<size_t>(nblk(terms), 1024);
+        hipLaunchKernelGGL(k_mp2_qp, dim3(gx, (unsigned)ng), dim3(256), 0, s, dT4, dEps, no, nv, nop, nvp, aoff, icore,
+                           iocc, vlo, vlo, dQ);
+        HIPCHK(hipGetLastError());
+    }
     double acc[2];
+    std::vector<double> q((size_t)std::max(ng, 1), 0.0);
     HIPCHK(hipMemcpyAsync(acc, dAcc, 16, hipMemcpyDeviceToHost, s));
+    if (ng > 0) HIPCHK(hipMemcpyAsync(q.data(), dQ, (size_t)ng * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     *e2s = acc[0];
     *e2t = acc[1];
     if (!std::isfinite(acc[0]) || !std::isfinite(acc[1])) return fail(POLYERI_E_NONFINITE, "non-finite E(2)");
+    if (qeps) {
+        // QEPSILON = EPSILON + correction (src/pmp2energy.f90:549-558; one k-point: no k normalisation)
+        for (int k = 0; k < nc; ++k) qeps[k] = eps[k];
+        for (int k = 0; k < ng; ++k) qeps[vlo + k] = eps[vlo + k] + q[k];
+    }
     return 0;
 }
 
@@ -1703,6 +1773,12 @@ int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, in
 int polyeri_mp2_energy_ex(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
                           int ivirtcore, int transform, double* e2s, double* e2t) {
     return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, transform, e2s, e2t);
+}
+
+int polyeri_mp2_energy_qp(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
+                          int ivirtcore, int nqp, int transform, double* e2s, double* e2t, double* qeps) {
+    if (nqp < 1) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_qp: nqp must be positive");
+    return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, transform, e2s, e2t, nqp, qeps);
 }
 
 // ---------------------------------------------------------------------------

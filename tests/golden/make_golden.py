@@ -66,6 +66,27 @@ def parse(path):
             out["max_overlap_line"] = n + 2
             break
     out["trace"] = trace  # ITR, DIIS, DENSITY CHANGE, TOTAL ENERGY
+    # band tables (PRINT_ENERGYBANDS): the KX = 0 block after the HF / MP2
+    # quasi-particle headers, rows "label value" from the top band down
+    for key, head in (("hf_bands", "HF/DFT ENERGY BANDS"),
+                      ("qp_bands", "MP2 QUASI-PARTICLE ENERGY BANDS (DIAGONAL, FREQ-INDEP)")):
+        for n, line in enumerate(lines):
+            if line.startswith(head):
+                rows, started = [], False
+                for line2 in lines[n + 1:]:
+                    if line2.startswith("KX ="):
+                        started = True
+                        continue
+                    if not started or line2.startswith("-----"):
+                        if started and rows:
+                            break
+                        continue
+                    m = re.match(r"^\s*(\S+)\s+(-?\d+\.\d+)\s*$", line2)
+                    if m:
+                        rows.append([m.group(1), float(m.group(2))])
+                out[key] = rows
+                out[key + "_line"] = n + 1
+                break
     return out
 
 

@@ -81,3 +81,20 @@ def test_planted_mistake_is_caught(golden):
     b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
     r = b.scf(inttol=0.0, scfconv=1e-11, direct=False, hfexchange=0.98)
     assert abs(r["e_total"] - g["e_total"]) > 1e-3
+
+
+def test_bh_mp2_quasiparticle_bands_golden(golden):
+    """MP2 quasi-particle bands (IOPTN(44) = 2, frozen core) of the shipped
+    BH/STO-3G output (data/BH_EOM_out:289-297, printed to 4 decimals): the
+    oracle's restatement of src/pmp2energy.f90:445-522 for bands 2..5.
+    Dropping the second sum, flipping the exchange sign, the sign of e_g in
+    the second denominator or the factor 2 each moves a band by >= 7e-3."""
+    g = golden["BH_STO3G"]
+    b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11)
+    bands, q = oracle.mp2_qp(b.eri_tensor(), r["co"], r["eps"], 1, r["iocc"], r["n_indep"], 2)
+    gold = [v for _, v in g["qp_bands"]][::-1]          # printed from the top band down
+    assert bands == [1, 2, 3, 4]
+    assert np.max(np.abs(np.array(q) - np.array(gold))) <= 5.001e-5
+    hf = [v for _, v in g["hf_bands"]][::-1]
+    assert np.max(np.abs(np.array(r["eps"][:6]) - np.array(hf))) <= 5.001e-5
