@@ -134,42 +134,58 @@ struct ZGemmArgs {
     long scm, scn, sC;
 };
 
+// Each wavefront computes NT column tiles of one 16-row strip, so the A
+// value it loads per k step feeds NT (complex: 2 NT or 4 NT) products.
+template <int NT>
 __global__ void __launch_bounds__(256) k_zgemm_f64(ZGemmArgs g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tiles_n = (g.N + 15) >> 4;
+    const int groups = (g.N + 16 * NT - 1) / (16 * NT);
     const long tile = (long)blockIdx.x * 4 + wave;
-    const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
+    const int tm = (int)(tile / groups), tg = (int)(tile % groups);
     if ((long)tm * 16 >= g.M) return;
     const long b = blockIdx.y;
     const bool cplx = g.Ai != nullptr;
     const long oa = b * g.sA, ob = b * g.sB;
     const int r = lane & 15, q = lane >> 4;
-    const int m = tm * 16 + r, n = tn * 16 + r;
-    doublex4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    const int m = tm * 16 + r;
+    doublex4 cr[NT], ci[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        cr[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+        ci[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    }
     for (int k0 = 0; k0 < g.K; k0 += 4) {
         const int k = k0 + q;
-        const bool va = m < g.M && k < g.K, vb = n < g.N && k < g.K;
-        const long ia = oa + m * g.sam + k * g.sak, ib = ob + k * g.sbk + n * g.sbn;
+        const bool va = m < g.M && k < g.K;
+        const long ia = oa + m * g.sam + k * g.sak;
         const double ar = va ? g.Ar[ia] : 0.0;
-        const double br = vb ? g.Br[ib] : 0.0, bi = vb ? g.Bi[ib] : 0.0;
-        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
-        if (cplx) {
-            const double ai = va ? g.Ai[ia] : 0.0;
-            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cr, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
+        const double ai = (cplx && va) ? g.Ai[ia] : 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int n = (tg * NT + t) * 16 + r;
+            const bool vb = n < g.N && k < g.K;
+            const long ib = ob + k * g.sbk + n * g.sbn;
+            const double br = vb ? g.Br[ib] : 0.0, bi = vb ? g.Bi[ib] : 0.0;
+            cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr[t], 0, 0, 0);
+            ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci[t], 0, 0, 0);
+            if (cplx) {
+                cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cr[t], 0, 0, 0);
+                ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci[t], 0, 0, 0);
+            }
         }
     }
     const long oc = b * g.sC;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int row = tm * 16 + q + 4 * j, col = tn * 16 + r;
-        if (row < g.M && col < g.N) {
-            const long ic = oc + row * g.scm + col * g.scn;
-            g.Cr[ic] = cr[j];
-            g.Ci[ic] = ci[j];
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = tm * 16 + q + 4 * j, col = (tg * NT + t) * 16 + r;
+            if (row < g.M && col < g.N) {
+                const long ic = oc + row * g.scm + col * g.scn;
+                g.Cr[ic] = cr[t][j];
+                g.Ci[ic] = ci[t][j];
+            }
         }
-    }
 }
 
 // acc += x y with x, y given as three BF16 pieces each (six products, small first)
@@ -183,25 +199,31 @@ __device__ __forceinline__ floatx4 mfma_split3(const bf16x4* x, const bf16x4* y,
     return acc;
 }
 
+template <int NT>
 __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tiles_n = (g.N + 15) >> 4;
+    const int groups = (g.N + 16 * NT - 1) / (16 * NT);
     const long tile = (long)blockIdx.x * 4 + wave;
-    const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
+    const int tm = (int)(tile / groups), tg = (int)(tile % groups);
     if ((long)tm * 16 >= g.M) return;
     const long b = blockIdx.y;
     const bool cplx = g.Ai != nullptr;
     const long oa = b * g.sA, ob = b * g.sB;
     const int r = lane & 15, q = lane >> 4;
-    const int m = tm * 16 + r, n = tn * 16 + r;
-    floatx4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    const int m = tm * 16 + r;
+    floatx4 cr[NT], ci[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        cr[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        ci[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
     for (int k0 = 0; k0 < g.K; k0 += 16) {
-        bf16x4 ar[3], an[3], ai[3], br[3], bi[3];
+        bf16x4 ar[3], an[3], ai[3];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int k = k0 + 4 * q + j;
-            const bool va = m < g.M && k < g.K, vb = n < g.N && k < g.K;
-            const long ia = oa + m * g.sam + k * g.sak, ib = ob + k * g.sbk + n * g.sbn;
+            const bool va = m < g.M && k < g.K;
+            const long ia = oa + m * g.sam + k * g.sak;
             short x0, x1, x2;
             split3(va ? g.Ar[ia] : 0.0, x0, x1, x2);
             ar[0][j] = x0; ar[1][j] = x1; ar[2][j] = x2;
@@ -210,28 +232,42 @@ __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
             ai[0][j] = x0; ai[1][j] = x1; ai[2][j] = x2;
             split3(-aiv, x0, x1, x2);
             an[0][j] = x0; an[1][j] = x1; an[2][j] = x2;
-            split3(vb ? g.Br[ib] : 0.0, x0, x1, x2);
-            br[0][j] = x0; br[1][j] = x1; br[2][j] = x2;
-            split3(vb ? g.Bi[ib] : 0.0, x0, x1, x2);
-            bi[0][j] = x0; bi[1][j] = x1; bi[2][j] = x2;
         }
-        cr = mfma_split3(ar, br, cr);
-        ci = mfma_split3(ar, bi, ci);
-        if (cplx) {
-            cr = mfma_split3(an, bi, cr);
-            ci = mfma_split3(ai, br, ci);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            bf16x4 br[3], bi[3];
+            const int n = (tg * NT + t) * 16 + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + 4 * q + j;
+                const bool vb = n < g.N && k < g.K;
+                const long ib = ob + k * g.sbk + n * g.sbn;
+                short x0, x1, x2;
+                split3(vb ? g.Br[ib] : 0.0, x0, x1, x2);
+                br[0][j] = x0; br[1][j] = x1; br[2][j] = x2;
+                split3(vb ? g.Bi[ib] : 0.0, x0, x1, x2);
+                bi[0][j] = x0; bi[1][j] = x1; bi[2][j] = x2;
+            }
+            cr[t] = mfma_split3(ar, br, cr[t]);
+            ci[t] = mfma_split3(ar, bi, ci[t]);
+            if (cplx) {
+                cr[t] = mfma_split3(an, bi, cr[t]);
+                ci[t] = mfma_split3(ai, br, ci[t]);
+            }
         }
     }
     const long oc = b * g.sC;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int row = tm * 16 + 4 * q + j, col = tn * 16 + r;
-        if (row < g.M && col < g.N) {
-            const long ic = oc + row * g.scm + col * g.scn;
-            g.Cr[ic] = (double)cr[j];
-            g.Ci[ic] = (double)ci[j];
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = tm * 16 + 4 * q + j, col = (tg * NT + t) * 16 + r;
+            if (row < g.M && col < g.N) {
+                const long ic = oc + row * g.scm + col * g.scn;
+                g.Cr[ic] = (double)cr[t][j];
+                g.Ci[ic] = (double)ci[t][j];
+            }
         }
-    }
 }
 
 }  // namespace
@@ -241,17 +277,27 @@ int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const 
                          double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream) {
     if (M < 1 || N < 1 || K < 1 || batch < 1 || !Ar || !Br || !Bi || !Cr || !Ci) return -1;
     if (mode != GEMM_F64 && mode != GEMM_BF16X3) return -1;
-    const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+    // column tiles per wavefront: enough for N (up to 4)
+    const int ntile = (N + 15) / 16;
+    const int nt = ntile >= 4 ? 4 : ntile;
+    const long tiles = (long)((M + 15) / 16) * ((N + 16 * nt - 1) / (16 * nt));
     if ((tiles + 3) / 4 > 0x7fffffffL) return -1;
     for (int b0 = 0; b0 < batch; b0 += 65535) {       // grid.y limit
         const int nb = batch - b0 < 65535 ? batch - b0 : 65535;
         ZGemmArgs g{M, N, K, Ar + b0 * sA, Ai ? Ai + b0 * sA : nullptr, sam, sak, sA, Br + b0 * sB, Bi + b0 * sB,
                     sbk, sbn, sB, Cr + b0 * sC, Ci + b0 * sC, scm, scn, sC};
         const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
-        if (mode == GEMM_F64)
-            hipLaunchKernelGGL(k_zgemm_f64, grid, dim3(256), 0, stream, g);
-        else
-            hipLaunchKernelGGL(k_zgemm_bf16x3, grid, dim3(256), 0, stream, g);
+        if (mode == GEMM_F64) {
+            if (nt == 1) hipLaunchKernelGGL(k_zgemm_f64<1>, grid, dim3(256), 0, stream, g);
+            else if (nt == 2) hipLaunchKernelGGL(k_zgemm_f64<2>, grid, dim3(256), 0, stream, g);
+            else if (nt == 3) hipLaunchKernelGGL(k_zgemm_f64<3>, grid, dim3(256), 0, stream, g);
+            else hipLaunchKernelGGL(k_zgemm_f64<4>, grid, dim3(256), 0, stream, g);
+        } else {
+            if (nt == 1) hipLaunchKernelGGL(k_zgemm_bf16x3<1>, grid, dim3(256), 0, stream, g);
+            else if (nt == 2) hipLaunchKernelGGL(k_zgemm_bf16x3<2>, grid, dim3(256), 0, stream, g);
+            else if (nt == 3) hipLaunchKernelGGL(k_zgemm_bf16x3<3>, grid, dim3(256), 0, stream, g);
+            else hipLaunchKernelGGL(k_zgemm_bf16x3<4>, grid, dim3(256), 0, stream, g);
+        }
         if (hipGetLastError() != hipSuccess) return -2;
     }
     return 0;

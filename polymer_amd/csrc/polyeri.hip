@@ -2349,20 +2349,21 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
         const int nb = plim[kb] - iocc;
         if (nb <= 0) continue;
         const long NBk = nb;
-        // Q1 quarter: TN[(i,Q1,j,Q3,k)][b] = sum_{Q2,l} T CO(l,b,kb) e^{i kb Q2} (:253-258;
-        // the e^{i kb Q3} factor moves into the next operand)
+        // Q1 quarter: TN[b][(i,Q1,j,Q3,k)] = sum_{Q2,l} T CO(l,b,kb) e^{i kb Q2} (:253-258;
+        // the e^{i kb Q3} factor moves into the next operand); b outermost so the
+        // next quarter reads its contraction index (Q3,k) contiguously
         hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB1)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, kb, 1,
                            (int)n1, ctx->rc1, nb, iocc, 1, 0, 0, 0, dPlim, dB1r, dB1i);
         // B2[kj][(Q3,k)][jo] = conj(CO(k,jo,kj)) e^{i (kb - kj) Q3} (:294-299)
         hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB2)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, nk,
                            (int)n3, ctx->cel2, no, icore, -1, kb, 1, 1, dPlim, dB2r, dB2i);
         HIPCHK(hipGetLastError());
-        if (gemm((int)M1, nb, (int)(n1 * N), dT, nullptr, n1 * N, 1, 0, dB1r, dB1i, NBk, 1, 0, dTNr, dTNi, NBk, 1, 0,
+        if (gemm((int)M1, nb, (int)(n1 * N), dT, nullptr, n1 * N, 1, 0, dB1r, dB1i, NBk, 1, 0, dTNr, dTNi, 1, M1, 0,
                  1))
             return fail(POLYERI_E_HIP, "k-point MP2: first quarter launch failed");
         // Q2 quarter, per kj: TOK[kj][b][jo][(i,Q1,j)] = sum_{Q3,k} TN B2[kj]
         for (int kj = 0; kj < nk; ++kj)
-            if (gemm((int)M2, no, (int)(n3 * N), dTNr, dTNi, n3 * N * NBk, NBk, 1, dB2r + kj * n3 * N * NO,
+            if (gemm((int)M2, no, (int)(n3 * N), dTNr, dTNi, n3 * N, 1, M1, dB2r + kj * n3 * N * NO,
                      dB2i + kj * n3 * N * NO, NO, 1, 0, dTOKr + kj * NBk * NO * M2, dTOKi + kj * NBk * NO * M2, 1,
                      M2, NO * M2, nb))
                 return fail(POLYERI_E_HIP, "k-point MP2: second quarter launch failed");
