@@ -26,6 +26,12 @@
 #define BOYS_NT 801
 #define BOYS_NM 16
 #define BOYS_TMAX 40.0
+// The thread kernels read a second layout of the same values: for each
+// L < BOYS_LA a table of 64-byte rows F_L(T_i) .. F_{L+6}(T_i) (and one pad),
+// so the seven values of one lookup arrive in four 16-byte loads.  It follows
+// the BOYS_NT x BOYS_NM table in the same allocation (BOYS_AOFF doubles in).
+#define BOYS_LA 9
+#define BOYS_AOFF (BOYS_NT * BOYS_NM)
 
 // Primitive pair of a shell pair (a,b): exponent sum, prefactor with both
 // contraction coefficients, centre P and P - A.  64 bytes.
@@ -91,20 +97,28 @@ struct SoaKet {
 // nearest grid point, then the downward recursion
 // F_{m-1} = (2T F_m + e^-T) / (2m-1); large T: asymptotic F_0 and the
 // upward recursion without the (negligible) e^-T term.
+// Two table values loaded as one 16-byte word.
+struct alignas(16) BoysPair {
+    double x, y;
+};
+
+// tab: the aligned per-L tables (see BOYS_LA).
 template <int L>
 POLY_HD void boys_fp64(double T, double pref, const double* __restrict__ tab,
                        double* __restrict__ F) {
+    static_assert(L < BOYS_LA, "Boys table too short for this class");
     if (T < BOYS_TMAX) {
         const int ts = (int)(T * 20.0 + 0.5);
         const double h = 0.05 * ts - T;
-        const double* g = tab + ts * BOYS_NM + L;
-        double f = g[6];
-        f = g[5] + h * (1.0 / 6.0) * f;
-        f = g[4] + h * (1.0 / 5.0) * f;
-        f = g[3] + h * (1.0 / 4.0) * f;
-        f = g[2] + h * (1.0 / 3.0) * f;
-        f = g[1] + h * (1.0 / 2.0) * f;
-        f = g[0] + h * f;
+        const BoysPair* g = reinterpret_cast<const BoysPair*>(tab + ((size_t)L * BOYS_NT + ts) * 8);
+        const BoysPair g01 = g[0], g23 = g[1], g45 = g[2];
+        double f = tab[((size_t)L * BOYS_NT + ts) * 8 + 6];
+        f = g45.y + h * (1.0 / 6.0) * f;
+        f = g45.x + h * (1.0 / 5.0) * f;
+        f = g23.y + h * (1.0 / 4.0) * f;
+        f = g23.x + h * (1.0 / 3.0) * f;
+        f = g01.y + h * (1.0 / 2.0) * f;
+        f = g01.x + h * f;
         F[L] = f;
         if (L > 0) {
             const double e = exp(-T);

@@ -42,7 +42,8 @@ struct EriArgs {
     const int* rb_first;       // nrows x nbucket: first ket of the row's range inside each
                                // bucket (null: every range starts at the bucket start)
     int nbucket;
-    const double* boys;
+    const double* boys;        // BOYS_NT x BOYS_NM (wavefront kernel)
+    const double* boysA;       // aligned per-L tables (thread kernels)
     const double* D;
     double* J;
     double* K;
@@ -295,7 +296,7 @@ __device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
     StoreCb<NB, NC, ND, double> st{v};
     const ShiftKet kv{g.prims + mk.prim_off, s * g.period};
     quartet<double, LA, LB, LC, LD>(g.prims + mb.prim_off, mb.nprim, kv, mk.nprim, g.ab[4 * bp], g.ab[4 * bp + 1],
-                                    g.ab[4 * bp + 2], g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boys, st);
+                                    g.ab[4 * bp + 2], g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boysA, st);
     const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * g.cel1 + 1) * g.nbf * g.nbf;
     double* J = g.Jc + off;
     double* K = g.Kc + off;
@@ -337,7 +338,7 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
     const int n = g.nbf;
     if (MODE == MODE_SCHWARZ) {
         SchwarzCb cb;
-        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+        quartet_fp64<LA, LB, LC, LD>(bra, mb.nprim, ket, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boysA, cb);
         g.out[bp] = sqrt(cb.m);
         return;
     }
@@ -345,7 +346,7 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
                     g.kfs};
     if (MODE == MODE_TENSOR) {
         TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
-        quartet<double, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, cb);
+        quartet<double, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boysA, cb);
     } else {
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
         // light classes only (heavier ones run in kcoop.hip): the (ab|cd)
@@ -353,7 +354,7 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
         using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
         real v[NA * NB * NC * ND];
         StoreCb<NB, NC, ND, real> st{v};
-        quartet<real, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boys, st);
+        quartet<real, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boysA, st);
         RegDigest<NA, NB, NC, ND, real> cb;
         cb.load(g.D, n, fa, fb, fc, fd);
         const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
@@ -447,7 +448,7 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
             real v[NA * NB * NC * ND];
             StoreCb<NB, NC, ND, real> st{v};
             quartet<real, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, g.ab[4 * kp],
-                                          g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boys, st);
+                                          g.ab[4 * kp + 1], g.ab[4 * kp + 2], g.boysA, st);
             RegDigest<NA, NB, NC, ND, real> cb;
             cb.load(g.D, n, fa, fb, fc, fd);
 #pragma unroll

@@ -435,7 +435,8 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     a.bra_ids = bra; a.ket_ids = ket; a.row_off = row; a.nrows = nrows; a.nwork = nwork;
     a.rb_off = rb; a.bucket_start = bstart; a.nbucket = nbucket; a.rb_first = rf;
     a.items = items; a.count = count;
-    a.boys = c->d_boys; a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
+    a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF;
+    a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     a.jk_copies = c->jk_copies;
     EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
                 : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
@@ -490,7 +491,7 @@ static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long n
     a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab;
     a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = nwork;
     a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket;
-    a.boys = c->d_boys; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
+    a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
     a.n3 = 2 * c->cel2 + 1; a.cel1 = c->cel1; a.cel2 = c->cel2; a.period = c->period; a.both = cw.both;
     a.pcell = c->d_pcell; a.Dc = c->d_Dc; a.Jc = c->d_Jc; a.Kc = c->d_Kc;
     if (tensor) { a.out = tensor; a.cel1 = c->rc1; }
@@ -609,7 +610,15 @@ static int build_ctx(polyeri_ctx* c) {
     if ((rc = dalloc(&c->d_prims, c->prims.size()))) return rc;
     if ((rc = dalloc(&c->d_meta, np))) return rc;
     if ((rc = dalloc(&c->d_ab, 4 * (size_t)np))) return rc;
-    if ((rc = dalloc(&c->d_boys, boys.size()))) return rc;
+    // device copy: the BOYS_NT x BOYS_NM table, then the aligned per-L tables
+    // of the thread kernels (BOYS_LA x BOYS_NT rows of F_L .. F_{L+6}, pad)
+    std::vector<double> boys_dev(boys);
+    boys_dev.resize((size_t)BOYS_AOFF + (size_t)BOYS_LA * BOYS_NT * 8, 0.0);
+    for (int L = 0; L < BOYS_LA; ++L)
+        for (int i = 0; i < BOYS_NT; ++i)
+            for (int j = 0; j < 7; ++j)
+                boys_dev[(size_t)BOYS_AOFF + ((size_t)L * BOYS_NT + i) * 8 + j] = boys[(size_t)i * BOYS_NM + L + j];
+    if ((rc = dalloc(&c->d_boys, boys_dev.size()))) return rc;
     if ((rc = dalloc(&c->d_Q, np))) return rc;
     if ((rc = dalloc(&c->d_ref, nbf))) return rc;
     if ((rc = dalloc(&c->d_scale, nbf))) return rc;
@@ -629,7 +638,7 @@ static int build_ctx(polyeri_ctx* c) {
     HIPCHK(hipMemcpy(c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimPair), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_meta, meta.data(), np * sizeof(PairMeta), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_ab, ab.data(), ab.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_boys, boys.data(), boys.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_boys, boys_dev.data(), boys_dev.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_ref, c->ib.ref_of_int.data(), nbf * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_scale, c->ib.scale.data(), nbf * sizeof(double), hipMemcpyHostToDevice));
     // Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), one class at a time
