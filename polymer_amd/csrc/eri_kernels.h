@@ -283,6 +283,125 @@ __device__ __forceinline__ void pbc_integral(const EriArgs& g, double* J, double
     if (g.both) pbc_add(g, J, K, ck.x, cb.x, -s, fk, fl, fi, fj, v);
 }
 
+// Periodic digestion of one (quartet, Q3) in registers, one atomic per
+// matrix element (the same sums as pbc_integral over the quartet's
+// integrals).  SB / SK: the stored bra / ket pair is swapped, so the
+// reference's i (cell 0) and j (cell Q1) functions are b and a (and l, k
+// are c, d).  Blocks: J(Q1)[i][j] += v D(Q2)[k][l]; K(Q3+Q2)[i][l] += v
+// D(Q1-Q3)[k][j]; with both roles also J(Q2)[k][l] += v D(Q1)[i][j] and
+// K(Q1-Q3)[k][j] += v D(Q2+Q3)[i][l].
+template <int NA, int NB, int NC, int ND, bool SB, bool SK>
+__device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, int fa, int fb, int fc, int fd, int q1,
+                                           int q2, int s, double* J, double* K) {
+    constexpr int NI = SB ? NB : NA, NJ = SB ? NA : NB, NK = SK ? ND : NC, NL = SK ? NC : ND;
+    const int fi = SB ? fb : fa, fj = SB ? fa : fb, fk = SK ? fd : fc, fl = SK ? fc : fd;
+    const size_t n = g.nbf, n2 = n * n;
+    const int c1 = g.cel1, cp = g.jk_copies;
+    auto V = [&](int i, int j, int k, int l) -> double {
+        const int a = SB ? j : i, b = SB ? i : j, c = SK ? l : k, d = SK ? k : l;
+        return v[((a * NB + b) * NC + c) * ND + d];
+    };
+    {
+        const double* D = g.Dc + (size_t)(q2 + c1) * n2;
+        double dkl[NK * NL], acc[NI * NJ];
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) dkl[k * NL + l] = D[(size_t)(fk + k) * n + fl + l];
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                double x = 0.0;
+#pragma unroll
+                for (int k = 0; k < NK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) x += V(i, j, k, l) * dkl[k * NL + l];
+                acc[i * NJ + j] = x;
+            }
+        double* Jq = J + (size_t)(q1 + c1) * n2;
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) jk_add(&Jq[(size_t)(fi + i) * n + fj + j], acc[i * NJ + j], cp);
+    }
+    const int qk = s + q2, qd = q1 - s;
+    if (qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1) {
+        const double* D = g.Dc + (size_t)(qd + c1) * n2;
+        double dkj[NK * NJ], acc[NI * NL];
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) dkj[k * NJ + j] = D[(size_t)(fk + k) * n + fj + j];
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                double x = 0.0;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) x += V(i, j, k, l) * dkj[k * NJ + j];
+                acc[i * NL + l] = x;
+            }
+        double* Kq = K + (size_t)(qk + c1) * n2;
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) jk_add(&Kq[(size_t)(fi + i) * n + fl + l], acc[i * NL + l], cp);
+    }
+    if (!g.both) return;
+    {
+        const double* D = g.Dc + (size_t)(q1 + c1) * n2;
+        double dij[NI * NJ], acc[NK * NL];
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) dij[i * NJ + j] = D[(size_t)(fi + i) * n + fj + j];
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) x += V(i, j, k, l) * dij[i * NJ + j];
+                acc[k * NL + l] = x;
+            }
+        double* Jq = J + (size_t)(q2 + c1) * n2;
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) jk_add(&Jq[(size_t)(fk + k) * n + fl + l], acc[k * NL + l], cp);
+    }
+    const int qk2 = q1 - s, qd2 = q2 + s;
+    if (qk2 >= -c1 && qk2 <= c1 && qd2 >= -c1 && qd2 <= c1) {
+        const double* D = g.Dc + (size_t)(qd2 + c1) * n2;
+        double dil[NI * NL], acc[NK * NJ];
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) dil[i * NL + l] = D[(size_t)(fi + i) * n + fl + l];
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) x += V(i, j, k, l) * dil[i * NL + l];
+                acc[k * NJ + j] = x;
+            }
+        double* Kq = K + (size_t)(qk2 + c1) * n2;
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) jk_add(&Kq[(size_t)(fk + k) * n + fj + j], acc[k * NJ + j], cp);
+    }
+}
+
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
     const long long u = t / g.n3;
@@ -301,12 +420,21 @@ __device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
     double* J = g.Jc + off;
     double* K = g.Kc + off;
     const int2 cb = g.pcell[bp], ck = g.pcell[kp];
-    for (int a = 0; a < NA; ++a)
-        for (int bb = 0; bb < NB; ++bb)
-            for (int c = 0; c < NC; ++c)
-                for (int d = 0; d < ND; ++d)
-                    pbc_integral(g, J, K, cb, ck, s, mb.fa, mb.fb, mk.fa, mk.fb, a, bb, c, d,
-                                 v[((a * NB + bb) * NC + c) * ND + d]);
+    if (g.out) {   // lattice tensor: one store per integral
+        for (int a = 0; a < NA; ++a)
+            for (int bb = 0; bb < NB; ++bb)
+                for (int c = 0; c < NC; ++c)
+                    for (int d = 0; d < ND; ++d)
+                        pbc_integral(g, J, K, cb, ck, s, mb.fa, mb.fb, mk.fa, mk.fb, a, bb, c, d,
+                                     v[((a * NB + bb) * NC + c) * ND + d]);
+        return;
+    }
+    switch (cb.y * 2 + ck.y) {
+    case 0: pbc_digest<NA, NB, NC, ND, false, false>(g, v, mb.fa, mb.fb, mk.fa, mk.fb, cb.x, ck.x, s, J, K); break;
+    case 1: pbc_digest<NA, NB, NC, ND, false, true>(g, v, mb.fa, mb.fb, mk.fa, mk.fb, cb.x, ck.x, s, J, K); break;
+    case 2: pbc_digest<NA, NB, NC, ND, true, false>(g, v, mb.fa, mb.fb, mk.fa, mk.fb, cb.x, ck.x, s, J, K); break;
+    default: pbc_digest<NA, NB, NC, ND, true, true>(g, v, mb.fa, mb.fb, mk.fa, mk.fb, cb.x, ck.x, s, J, K); break;
+    }
 }
 
 template <int LA, int LB, int LC, int LD, int MODE>
