@@ -204,8 +204,9 @@ def main():
     # scaled by the copy kernel; counters cannot be read inside this process)
     traffic, traffic_src = None, None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
-    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri"}[dc["kernel"]]
-    dkern = (f"void {kfn}<{', '.join(dc['cls'])}, 0>(EriArgs)" if kfn != "k_eri_coop"
+    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri", "gc": "k_eri_gc"}[dc["kernel"]]
+    dkern = (f"void k_eri_gc<{dc['cls'][0]}, {dc['cls'][2]}>(EriArgs)" if kfn == "k_eri_gc"
+             else f"void {kfn}<{', '.join(dc['cls'])}, 0>(EriArgs)" if kfn != "k_eri_coop"
              else "void k_eri_coop<0>(EriArgs, int, int, int, int, int)")
     if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
         with open(tpath) as f:

@@ -278,7 +278,8 @@ typedef struct polyeri_stats {
     int class_l[POLYERI_MAX_LISTS];     /* la*1000+lb*100+lc*10+ld           */
     int class_prec[POLYERI_MAX_LISTS];  /* 0: FP64, 1: FP32                  */
     int class_kind[POLYERI_MAX_LISTS];  /* 1: thread per quartet, 2: wavefront per quartet,
-                                           3: thread per quartet, bra-row segments (LDS exchange rows) */
+                                           3: thread per quartet, bra-row segments (LDS exchange rows),
+                                           4: as 3 over generally contracted pairs (s shells sharing exponents) */
     int class_block[POLYERI_MAX_LISTS]; /* threads per block                 */
     int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass           */
     int nclass;

@@ -367,7 +367,7 @@ class Context:
         d["classes"] = [dict(cls=f"{st.class_l[i]:04d}", ms=st.last_class_ms[i],
                              prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i],
                              quartets=st.class_quartets[i], prec="fp32" if st.class_prec[i] else "fp64",
-                             kernel={2: "wave", 3: "seg"}.get(st.class_kind[i], "thread"), block=st.class_block[i],
+                             kernel={2: "wave", 3: "seg", 4: "gc"}.get(st.class_kind[i], "thread"), block=st.class_block[i],
                              grid=st.class_grid[i])
                         for i in range(n)]
         return d

@@ -42,6 +42,22 @@ struct PrimPair {
     double PAx, PAy, PAz;
 };
 
+// Generally contracted pair (classes (ls|), s shells that share exponents):
+// the pair's primitive pairs carry K without contraction coefficients, and
+// each of its m (<= 4) member shell pairs weighs primitive pair p with
+// w[j] = c_a(p) c_b(p) of its own two shells.  Member j pairs shell
+// ((code >> 2j) & 1) of the first group (first functions fa0 / fa1) with
+// shell ((code >> 2j+1) & 1) of the second group (fb0 / fb1).
+struct GcMeta {
+    int prim_off, nprim, m, code;
+    int fa0, fa1, fb0, fb1;
+};
+struct GcPrim {
+    PrimPair pp;   // K = sqrt(2) pi^{5/4} exp(-ab/zeta |AB|^2) / zeta
+    double w[4];
+};
+#define GC_MAXM 4
+
 POLY_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
 // sqrt(POLY_PI) rounded to double
@@ -181,6 +197,31 @@ POLY_CLASS(2, 2, 2, 1)
 POLY_CLASS(2, 2, 2, 2)
 #undef POLY_CLASS
 
+// One primitive quartet: the VRR's [e0|f0] values, prefactors K_ab K_cd
+// included, added to acc.
+template <typename real, int LA, int LB, int LC, int LD>
+POLY_HD void prim_vrr(const PrimPair& bp, const PrimPair& kq, const double* __restrict__ boys_tab, real* acc) {
+    constexpr int L = LA + LB + LC + LD;
+    const double ze = bp.zeta + kq.zeta;
+    const double rs = rsqrt_fp64(ze);
+    const double ize = rs * rs;
+    const double rho = bp.zeta * kq.zeta * ize;
+    const double PQx = bp.Px - kq.Px, PQy = bp.Py - kq.Py, PQz = bp.Pz - kq.Pz;
+    const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const double pref = bp.K * kq.K * rs;
+    double F[L + 1];
+    boys_fp64<L>(T, pref, boys_tab, F);
+    real Fr[L + 1];
+#pragma unroll
+    for (int m = 0; m <= L; ++m) Fr[m] = (real)F[m];
+    const double roz = kq.zeta * ize;   // rho / zeta
+    const double roe = bp.zeta * ize;   // rho / eta
+    ClassCode<LA, LB, LC, LD, real>::vrr(
+        (real)bp.PAx, (real)bp.PAy, (real)bp.PAz, (real)(-roz * PQx), (real)(-roz * PQy), (real)(-roz * PQz),
+        (real)kq.PAx, (real)kq.PAy, (real)kq.PAz, (real)(roe * PQx), (real)(roe * PQy), (real)(roe * PQz),
+        (real)(0.5 / bp.zeta), (real)(0.5 / kq.zeta), (real)(0.5 * ize), (real)roz, (real)roe, Fr, acc);
+}
+
 // Contracted (ab|cd) for one shell quartet.  bra/ket point at the
 // primitive pairs of the two shell pairs; AB = A - B, CD = C - D.  Every
 // final integral is passed to cb(ia, ib, ic, id, value) with the component
@@ -192,7 +233,6 @@ template <typename real, int LA, int LB, int LC, int LD, typename Ket, typename 
 POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const Ket& ket, int nket, double ABx,
                      double ABy, double ABz, double CDx, double CDy, double CDz,
                      const double* __restrict__ boys_tab, Cb& cb) {
-    constexpr int L = LA + LB + LC + LD;
     // components of every l in [LA, LA+LB] (and [LC, LC+LD])
     constexpr int NE = ((LA + LB + 1) * (LA + LB + 2) * (LA + LB + 3) - LA * (LA + 1) * (LA + 2)) / 6;
     constexpr int NF = ((LC + LD + 1) * (LC + LD + 2) * (LC + LD + 3) - LC * (LC + 1) * (LC + 2)) / 6;
@@ -201,28 +241,7 @@ POLY_HD void quartet(const PrimPair* __restrict__ bra, int nbra, const Ket& ket,
     for (int i = 0; i < NE * NF; ++i) acc[i] = real(0);
     for (int p = 0; p < nbra; ++p) {
         const PrimPair bp = bra[p];
-        for (int q = 0; q < nket; ++q) {
-            const PrimPair kq = ket.load(q);
-            const double ze = bp.zeta + kq.zeta;
-            const double rs = rsqrt_fp64(ze);
-            const double ize = rs * rs;
-            const double rho = bp.zeta * kq.zeta * ize;
-            const double PQx = bp.Px - kq.Px, PQy = bp.Py - kq.Py, PQz = bp.Pz - kq.Pz;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = bp.K * kq.K * rs;
-            double F[L + 1];
-            boys_fp64<L>(T, pref, boys_tab, F);
-            real Fr[L + 1];
-#pragma unroll
-            for (int m = 0; m <= L; ++m) Fr[m] = (real)F[m];
-            const double roz = kq.zeta * ize;   // rho / zeta
-            const double roe = bp.zeta * ize;   // rho / eta
-            ClassCode<LA, LB, LC, LD, real>::vrr(
-                (real)bp.PAx, (real)bp.PAy, (real)bp.PAz, (real)(-roz * PQx), (real)(-roz * PQy),
-                (real)(-roz * PQz), (real)kq.PAx, (real)kq.PAy, (real)kq.PAz, (real)(roe * PQx),
-                (real)(roe * PQy), (real)(roe * PQz), (real)(0.5 / bp.zeta), (real)(0.5 / kq.zeta),
-                (real)(0.5 * ize), (real)roz, (real)roe, Fr, acc);
-        }
+        for (int q = 0; q < nket; ++q) prim_vrr<real, LA, LB, LC, LD>(bp, ket.load(q), boys_tab, acc);
     }
     ClassCode<LA, LB, LC, LD, real>::hrr((real)ABx, (real)ABy, (real)ABz, (real)CDx, (real)CDy, (real)CDz,
                                          acc, cb);

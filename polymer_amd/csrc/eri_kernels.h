@@ -71,6 +71,10 @@ struct EriArgs {
     const double* dsh;         // nsh x nsh shell-block maxima of |D|
     int nsh;
     unsigned* kept;
+    // generally contracted lists (k_eri_gc): pair data, bra primitive pairs,
+    // ket fields 8 .. 11 of ksoa = the member weights
+    const GcMeta* gmeta;
+    const GcPrim* gprims;
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -632,6 +636,136 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
     }
 }
 
+// Segment kernel of the generally contracted lists, classes (ls|l's): as
+// k_eri_seg, but an item is a pair of generally contracted pairs (GcMeta)
+// and computes all member quartets from one primitive loop (member weights
+// applied to each primitive quartet's [a0|c0] block).  Each member quartet
+// is digested with its own degeneracy weight; a list row whose bra equals
+// the ket visits both orders of two different members, each at half
+// weight.  LDS: the exchange rows of the up to two shells of each bra group
+// ((2 NA + 2) nbf doubles) and J_ab of every bra member (MB NA doubles).
+template <int LA, int LC>
+__global__ void __launch_bounds__(256) k_eri_gc(EriArgs g) {
+    constexpr int NA = ncart(LA), NC = ncart(LC), NI = NA * NC;
+    constexpr int MB = LA == 0 ? 4 : 2, MK = LC == 0 ? 4 : 2;
+    extern __shared__ double seg_lds[];
+    const int n = g.nbf;
+    double* Kr = seg_lds;                        // rows fa0.. fa1.. (NA each), fb0, fb1
+    double* Jl = seg_lds + (size_t)(2 * NA + 2) * n;
+    const int total = (2 * NA + 2) * n + MB * NA;
+    const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
+    double* J = g.J + off;
+    double* K = g.K + off;
+    const int lane = threadIdx.x & 63;
+    const long long fs = g.kfs;
+    for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
+        const SegDesc seg = g.segs[sg];
+        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
+        __syncthreads();
+        const int bp = g.bra_ids[seg.row];
+        const GcMeta mb = g.gmeta[bp];
+        const GcPrim* bra = g.gprims + mb.prim_off;
+        double jab[MB * NA];
+#pragma unroll
+        for (int i = 0; i < MB * NA; ++i) jab[i] = 0.0;
+        for (long long t0 = seg.t0; t0 < seg.t0 + seg.n; t0 += blockDim.x) {
+            const long long t = t0 + threadIdx.x;
+            if (t >= seg.t0 + seg.n) continue;
+            int pos, b;
+            decode_in_row(g, seg.row, t, pos, b);
+            const int kp = g.ket_ids[pos];
+            const GcMeta mk = g.gmeta[kp];
+            const int nk = g.bucket_start[b + 1] - g.bucket_start[b];
+            const double* kb = g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]);
+            double acc[MB * MK * NI];
+#pragma unroll
+            for (int i = 0; i < MB * MK * NI; ++i) acc[i] = 0.0;
+            for (int p = 0; p < mb.nprim; ++p) {
+                const GcPrim bq = bra[p];
+                for (int q = 0; q < mk.nprim; ++q) {
+                    const double* x = kb + (long long)q * nk;
+                    PrimPair kq;
+                    kq.zeta = x[0]; kq.K = x[fs]; kq.Px = x[2 * fs]; kq.Py = x[3 * fs]; kq.Pz = x[4 * fs];
+                    kq.PAx = x[5 * fs]; kq.PAy = x[6 * fs]; kq.PAz = x[7 * fs];
+                    double wk[MK];
+#pragma unroll
+                    for (int j = 0; j < MK; ++j) wk[j] = x[(8 + j) * fs];
+                    double tv[NI];
+#pragma unroll
+                    for (int i = 0; i < NI; ++i) tv[i] = 0.0;
+                    prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
+#pragma unroll
+                    for (int j = 0; j < MK; ++j)
+#pragma unroll
+                        for (int i = 0; i < MB; ++i) {
+                            const double w = bq.w[i] * wk[j];
+#pragma unroll
+                            for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += w * tv[e];
+                        }
+                }
+            }
+            const double sq = bp == kp ? 0.5 : 1.0;
+#pragma unroll
+            for (int i = 0; i < MB; ++i) {
+                if (i >= mb.m) break;
+                const int ia = (mb.code >> (2 * i)) & 1, ib = (mb.code >> (2 * i + 1)) & 1;
+                const int fa = ia ? mb.fa1 : mb.fa0, fb = ib ? mb.fb1 : mb.fb0;
+#pragma unroll
+                for (int j = 0; j < MK; ++j) {
+                    if (j >= mk.m) break;
+                    const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
+                    const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
+                    double v[NI];
+                    StoreCb<1, NC, 1, double> st{v};
+                    ClassCode<LA, 0, LC, 0, double>::hrr(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, acc + (i * MK + j) * NI, st);
+                    RegDigest<NA, 1, NC, 1, double> cb;
+                    cb.load(g.D, n, fa, fb, fc, fd);
+#pragma unroll
+                    for (int a = 0; a < NA; ++a)
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) cb(a, 0, c, 0, v[a * NC + c]);
+                    const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * sq;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        jab[i * NA + a] += s * cb.Jab[a];
+                        double* row = Kr + (size_t)(ia * NA + a) * n;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) atomicAdd(&row[fc + c], s * cb.Kac[a * NC + c]);
+                        atomicAdd(&row[fd], s * cb.Kad[a]);
+                    }
+                    double* rowb = Kr + (size_t)(2 * NA + ib) * n;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) atomicAdd(&rowb[fc + c], s * cb.Kbc[c]);
+                    atomicAdd(&rowb[fd], s * cb.Kbd[0]);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) jk_add(&J[(size_t)(fc + c) * n + fd], s * cb.Jcd[c], g.jk_copies);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MB * NA; ++i) {
+            const double w = wave_sum(jab[i]);
+            if (lane == 0 && i < mb.m * NA) atomicAdd(&Jl[i], w);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (2 * NA + 2) * n; i += blockDim.x) {
+            const double x = Kr[i];
+            if (x != 0.0) {
+                const int r = i / n, col = i - r * n;
+                const int f = r < 2 * NA ? (r < NA ? mb.fa0 + r : mb.fa1 + (r - NA)) : (r == 2 * NA ? mb.fb0 : mb.fb1);
+                jk_add(&K[(size_t)f * n + col], x, g.jk_copies);
+            }
+        }
+        for (int i = threadIdx.x; i < mb.m * NA; i += blockDim.x) {
+            const int j = i / NA, a = i - j * NA;
+            const int fa = ((mb.code >> (2 * j)) & 1) ? mb.fa1 : mb.fa0;
+            const int fb = ((mb.code >> (2 * j + 1)) & 1) ? mb.fb1 : mb.fb0;
+            jk_add(&J[(size_t)(fa + a) * n + fb], Jl[i], g.jk_copies);
+        }
+        __syncthreads();
+    }
+}
+
 typedef void (*EriKernel)(EriArgs);
 // Getter modes of polyeri_class_kernel beyond the k_eri modes: the segment
 // kernel for MODE_JK and MODE_JK32.
@@ -658,6 +792,11 @@ int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shel
 int polyeri_coop_nslot(int la, int lb, int lc, int ld);
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
                         hipStream_t stream);
+// Generally contracted segment kernel (kgc.hip): LDS bytes, whether the
+// class (la s|lc s) has one, and the launch over a.segs (grid blocks).
+size_t polyeri_gc_lds_bytes(int la, int nbf);
+bool polyeri_gc_class(int la, int lc);
+int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).

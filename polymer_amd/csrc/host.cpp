@@ -335,6 +335,104 @@ void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pair
         }
 }
 
+int build_gc_pairs(const IntBasis& ib, double primtol, const std::vector<PairInfo>& pairs,
+                   std::vector<PairInfo>& gpairs, std::vector<GcMeta>& gmeta, std::vector<GcPrim>& gprims,
+                   std::vector<int>& member_pair) {
+    gpairs.clear(); gmeta.clear(); gprims.clear(); member_pair.clear();
+    const int ns = (int)ib.shells.size();
+    // groups, with each member's coefficients over the first shell's exponents
+    std::vector<std::vector<int>> groups;
+    std::vector<std::vector<std::vector<double>>> gcoef;
+    std::vector<int> gof(ns, -1);
+    int ngc = 0;
+    for (int a = 0; a < ns; ++a) {
+        if (gof[a] >= 0) continue;
+        const Shell& A = ib.shells[a];
+        gof[a] = (int)groups.size();
+        groups.push_back({a});
+        gcoef.push_back({A.coef});
+        if (A.L != 0) continue;
+        bool distinct = true;
+        for (size_t i = 0; i < A.alpha.size(); ++i)
+            for (size_t j = 0; j < i; ++j) distinct = distinct && A.alpha[i] != A.alpha[j];
+        if (!distinct) continue;
+        for (int b = a + 1; b < ns; ++b) {
+            const Shell& B = ib.shells[b];
+            if (gof[b] >= 0 || B.L != 0 || B.x != A.x || B.y != A.y || B.z != A.z) continue;
+            std::vector<double> cb(A.alpha.size(), 0.0);
+            bool sub = true;
+            for (size_t j = 0; j < B.alpha.size() && sub; ++j) {
+                const auto it = std::find(A.alpha.begin(), A.alpha.end(), B.alpha[j]);
+                if (it == A.alpha.end()) sub = false;
+                else cb[it - A.alpha.begin()] += B.coef[j];
+            }
+            if (!sub) continue;
+            gof[b] = gof[a];
+            groups.back().push_back(b);
+            gcoef.back().push_back(cb);
+            ++ngc;
+            break;
+        }
+    }
+    if (ngc == 0) return 0;
+    std::vector<int> pidx((size_t)ns * ns, -1);
+    for (size_t i = 0; i < pairs.size(); ++i) {
+        pidx[(size_t)pairs[i].a * ns + pairs[i].b] = (int)i;
+        pidx[(size_t)pairs[i].b * ns + pairs[i].a] = (int)i;
+    }
+    const double c0 = std::sqrt(2.0) * std::pow(POLY_PI, 1.25);
+    const int ng = (int)groups.size();
+    for (int g1 = 0; g1 < ng; ++g1)
+        for (int g2 = g1; g2 < ng; ++g2) {
+            const std::vector<int>& G1 = groups[g1];
+            const std::vector<int>& G2 = groups[g2];
+            const Shell& A = ib.shells[G1[0]];
+            const Shell& B = ib.shells[G2[0]];
+            if (B.L != 0) continue;
+            int mem[GC_MAXM][2], m = 0;
+            for (size_t i = 0; i < G1.size(); ++i)
+                for (size_t j = 0; j < G2.size(); ++j)
+                    if (g1 != g2 || j >= i) { mem[m][0] = (int)i; mem[m][1] = (int)j; ++m; }
+            PairInfo pi;
+            pi.a = G1[0]; pi.b = G2[0];
+            pi.cls = pair_class(A.L, 0);
+            pi.ABx = A.x - B.x; pi.ABy = A.y - B.y; pi.ABz = A.z - B.z;
+            const double r2 = pi.ABx * pi.ABx + pi.ABy * pi.ABy + pi.ABz * pi.ABz;
+            pi.prim_off = (int)gprims.size();
+            for (size_t i = 0; i < A.alpha.size(); ++i)
+                for (size_t j = 0; j < B.alpha.size(); ++j) {
+                    const double al = A.alpha[i], be = B.alpha[j], z = al + be;
+                    const double e = std::exp(-al * be / z * r2);
+                    GcPrim gp{};
+                    double wmax = 0.0;
+                    for (int k = 0; k < m; ++k) {
+                        gp.w[k] = gcoef[g1][mem[k][0]][i] * gcoef[g2][mem[k][1]][j];
+                        wmax = std::max(wmax, std::fabs(gp.w[k]));
+                    }
+                    if (wmax * e < primtol) continue;
+                    gp.pp.zeta = z;
+                    gp.pp.K = e * c0 / z;
+                    gp.pp.Px = (al * A.x + be * B.x) / z;
+                    gp.pp.Py = (al * A.y + be * B.y) / z;
+                    gp.pp.Pz = (al * A.z + be * B.z) / z;
+                    gp.pp.PAx = gp.pp.Px - A.x; gp.pp.PAy = gp.pp.Py - A.y; gp.pp.PAz = gp.pp.Pz - A.z;
+                    gprims.push_back(gp);
+                }
+            pi.nprim = (int)gprims.size() - pi.prim_off;
+            if (pi.nprim == 0) continue;
+            GcMeta gm{};
+            gm.prim_off = pi.prim_off; gm.nprim = pi.nprim; gm.m = m;
+            for (int k = 0; k < m; ++k) gm.code |= (mem[k][0] << (2 * k)) | (mem[k][1] << (2 * k + 1));
+            gm.fa0 = ib.shells[G1[0]].f0; gm.fa1 = G1.size() > 1 ? ib.shells[G1[1]].f0 : -1;
+            gm.fb0 = ib.shells[G2[0]].f0; gm.fb1 = G2.size() > 1 ? ib.shells[G2[1]].f0 : -1;
+            for (int k = 0; k < GC_MAXM; ++k)
+                member_pair.push_back(k < m ? pidx[(size_t)G1[mem[k][0]] * ns + G2[mem[k][1]]] : -1);
+            gpairs.push_back(pi);
+            gmeta.push_back(gm);
+        }
+    return ngc;
+}
+
 // Periodic pairs: for every cell offset Q in [-rc1, rc1] and every ordered
 // shell pair (s1, s2), s1 in cell 0 and s2 in cell Q (translated by Q*period
 // along x).  Stored with the higher angular momentum first, so the pair is

@@ -285,6 +285,19 @@ struct ClassPairWork {
     long long nwork;
     double prim_quartets;
     double flops;
+    // The same list over generally contracted pairs (classes (ls|l's) of a
+    // basis with two-shell s groups, FP64 molecular contexts): used by the
+    // unscreened builds while gc_on (gc_block threads per block).
+    struct {
+        long long nwork = 0;
+        int nrows = 0, nbucket = 1, nseg = 0;
+        const int *d_bra = nullptr, *d_ket = nullptr, *d_bstart = nullptr, *d_bsoa = nullptr;
+        long long* d_row = nullptr;
+        int* d_rb = nullptr;
+        SegDesc* d_seg = nullptr;
+        double prim_quartets = 0;
+    } gc;
+    int gc_on = 0, gc_block = 256;
 };
 
 struct polyeri_ctx {
@@ -313,6 +326,16 @@ struct polyeri_ctx {
     double* d_ksoa = nullptr;
     int* d_bsoa[NPAIRCLASS] = {};
     std::vector<ClassPairWork> work;
+    // generally contracted pairs: metadata, bra primitive pairs, ket data
+    // bucket-interleaved as d_ksoa with the four member weights as fields 8-11
+    GcMeta* d_gmeta = nullptr;
+    GcPrim* d_gprims = nullptr;
+    double* d_gksoa = nullptr;
+    long long gkfs = 0;
+    int* d_gbra[NPAIRCLASS] = {};
+    int* d_gket[NPAIRCLASS] = {};
+    int* d_gbstart[NPAIRCLASS] = {};
+    int* d_gbsoa[NPAIRCLASS] = {};
     int* d_ref = nullptr;
     double* d_scale = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
@@ -484,6 +507,24 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     return 0;
 }
 
+// J/K launch of a list's generally contracted form (one block per segment).
+static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block) {
+    const auto& g = cw.gc;
+    if (g.nwork <= 0) return 0;
+    EriArgs a{};
+    a.bra_ids = g.d_bra; a.ket_ids = g.d_ket; a.row_off = g.d_row; a.nrows = g.nrows; a.nwork = g.nwork;
+    a.rb_off = g.d_rb; a.bucket_start = g.d_bstart; a.nbucket = g.nbucket; a.rb_first = nullptr;
+    a.ksoa = c->d_gksoa; a.kfs = c->gkfs; a.bucket_soa = g.d_bsoa;
+    a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF;
+    a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
+    a.segs = g.d_seg; a.nseg = g.nseg;
+    a.gmeta = c->d_gmeta; a.gprims = c->d_gprims;
+    if (block != 64 && block != 128) block = 256;
+    if (polyeri_gc_launch(a, cw.la, cw.lc, block, g.nseg, c->stream))
+        return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
+    return 0;
+}
+
 // tensor: store the integrals T[i][Q1][j][Q3][k][Q2][l] instead of digesting
 // them into J/K (pair cells in the reduced range [-rc1, rc1]).
 static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long nwork, double* tensor = nullptr) {
@@ -539,6 +580,13 @@ static void free_ctx(polyeri_ctx* c) {
         if (w.d_rf) hipFree(w.d_rf);
         if (w.d_items) hipFree(w.d_items);
         if (w.d_seg) hipFree(w.d_seg);
+        if (w.gc.d_row) hipFree(w.gc.d_row);
+        if (w.gc.d_rb) hipFree(w.gc.d_rb);
+        if (w.gc.d_seg) hipFree(w.gc.d_seg);
+    }
+    hipFree(c->d_gmeta); hipFree(c->d_gprims); hipFree(c->d_gksoa);
+    for (int i = 0; i < NPAIRCLASS; ++i) {
+        hipFree(c->d_gbra[i]); hipFree(c->d_gket[i]); hipFree(c->d_gbstart[i]); hipFree(c->d_gbsoa[i]);
     }
     hipFree(c->d_pcell); hipFree(c->d_Dc); hipFree(c->d_Jc); hipFree(c->d_Kc);
     hipFree(c->d_pcc); hipFree(c->d_jcc); hipFree(c->d_kcc);
@@ -577,6 +625,100 @@ static int reduced_cel1(const IntBasis& ib, double period, int cel1, const std::
         if (mx > 1.0e-15) rc1 = q;
     }
     return rc1;
+}
+
+// Generally contracted lists (build_gc_pairs): Schwarz bound of a pair =
+// the largest of its members', class lists in that order, the work lists
+// of host.cpp, and each list with a generally contracted kernel attached to
+// the FP64 list of the same class pair (segments as for the plain lists).
+// Quartet counts stay those of the plain lists.
+static int build_gc_lists(polyeri_ctx* c) {
+    std::vector<PairInfo> gp;
+    std::vector<GcMeta> gm;
+    std::vector<GcPrim> gprim;
+    std::vector<int> mp;
+    if (build_gc_pairs(c->ib, c->opts.prim_tol, c->pairs, gp, gm, gprim, mp) == 0) return 0;
+    const int ng = (int)gp.size();
+    std::vector<double> gQ(ng, 0.0);
+    for (int i = 0; i < ng; ++i)
+        for (int k = 0; k < GC_MAXM; ++k)
+            if (mp[(size_t)GC_MAXM * i + k] >= 0) gQ[i] = std::max(gQ[i], c->Q[mp[(size_t)GC_MAXM * i + k]]);
+    std::vector<int> cls[NPAIRCLASS], kets[NPAIRCLASS], bst[NPAIRCLASS];
+    for (int i = 0; i < ng; ++i) cls[gp[i].cls].push_back(i);
+    for (int X = 0; X < NPAIRCLASS; ++X)
+        std::stable_sort(cls[X].begin(), cls[X].end(), [&](int a, int b) { return gQ[a] > gQ[b]; });
+    std::vector<WorkList> wl;
+    build_work_lists(gp, gQ, c->opts.schwarz_tol, 0.0, cls, kets, bst, wl);
+    int rc;
+    const size_t P = gprim.size();
+    if ((rc = dalloc(&c->d_gmeta, gm.size())) || (rc = dalloc(&c->d_gprims, P))) return rc;
+    HIPCHK(hipMemcpy(c->d_gmeta, gm.data(), gm.size() * sizeof(GcMeta), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_gprims, gprim.data(), P * sizeof(GcPrim), hipMemcpyHostToDevice));
+    std::vector<double> soa(12 * P, 0.0);
+    size_t off = 0;
+    for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+        const auto& kl = kets[Y];
+        const auto& bs = bst[Y];
+        std::vector<int> bo(std::max<size_t>(bs.size(), 1), 0);
+        for (size_t b = 0; b + 1 < bs.size(); ++b) {
+            const int nk = bs[b + 1] - bs[b];
+            const int np_b = gp[kl[bs[b]]].nprim;
+            bo[b] = (int)off;
+            for (int i = 0; i < nk; ++i) {
+                const PairInfo& pi = gp[kl[bs[b] + i]];
+                for (int q = 0; q < np_b; ++q) {
+                    const GcPrim& x = gprim[pi.prim_off + q];
+                    const size_t at = off + (size_t)q * nk + i;
+                    const double f[12] = {x.pp.zeta, x.pp.K, x.pp.Px, x.pp.Py, x.pp.Pz, x.pp.PAx, x.pp.PAy, x.pp.PAz,
+                                          x.w[0], x.w[1], x.w[2], x.w[3]};
+                    for (int k = 0; k < 12; ++k) soa[k * P + at] = f[k];
+                }
+            }
+            off += (size_t)np_b * nk;
+        }
+        if ((rc = dalloc(&c->d_gbra[Y], cls[Y].size())) || (rc = dalloc(&c->d_gket[Y], kl.size())) ||
+            (rc = dalloc(&c->d_gbstart[Y], bs.size())) || (rc = dalloc(&c->d_gbsoa[Y], bo.size())))
+            return rc;
+        if (!cls[Y].empty()) {
+            HIPCHK(hipMemcpy(c->d_gbra[Y], cls[Y].data(), cls[Y].size() * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(c->d_gket[Y], kl.data(), kl.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        HIPCHK(hipMemcpy(c->d_gbstart[Y], bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_gbsoa[Y], bo.data(), bo.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (off != P) return fail(POLYERI_E_BASIS, "generally contracted ket layout does not cover every primitive pair");
+    if ((rc = dalloc(&c->d_gksoa, soa.size()))) return rc;
+    HIPCHK(hipMemcpy(c->d_gksoa, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->gkfs = (long long)P;
+    for (const WorkList& l : wl) {
+        if (l.nwork <= 0) continue;
+        const int la = kPairL[l.X][0], lc = kPairL[l.Y][0];
+        if (kPairL[l.X][1] != 0 || kPairL[l.Y][1] != 0 || !polyeri_gc_class(la, lc) ||
+            polyeri_gc_lds_bytes(la, c->ib.nbf) > 65536)
+            continue;
+        ClassPairWork* w = nullptr;
+        for (ClassPairWork& x : c->work)
+            if (x.X == l.X && x.Y == l.Y && x.prec == 0) w = &x;
+        if (!w) continue;
+        std::vector<SegDesc> segs;
+        for (int r = 0; r < l.nrows; ++r)
+            for (long long t = l.row[r]; t < l.row[r + 1]; t += kSegItems)
+                segs.push_back(SegDesc{t, r, (int)std::min<long long>(kSegItems, l.row[r + 1] - t)});
+        if (segs.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
+        auto& g = w->gc;
+        g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket; g.nseg = (int)segs.size();
+        g.d_bra = c->d_gbra[l.X]; g.d_ket = c->d_gket[l.Y]; g.d_bstart = c->d_gbstart[l.Y];
+        g.d_bsoa = c->d_gbsoa[l.Y];
+        g.prim_quartets = l.prim_quartets;
+        if ((rc = dalloc(&g.d_row, l.row.size())) || (rc = dalloc(&g.d_rb, l.rb.size())) ||
+            (rc = dalloc(&g.d_seg, segs.size())))
+            return rc;
+        HIPCHK(hipMemcpy(g.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
+        if (!l.rb.empty()) HIPCHK(hipMemcpy(g.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+        w->gc_on = 1;
+    }
+    return 0;
 }
 
 static int build_ctx(polyeri_ctx* c) {
@@ -756,6 +898,7 @@ static int build_ctx(polyeri_ctx* c) {
         c->model_flops += w.flops;
         c->work.push_back(w);
     }
+    if (!c->pbc && c->opts.precision == POLYERI_FP64 && (rc = build_gc_lists(c))) return rc;
     // density-screening tables
     {
         const int nsh = (int)c->ib.shells.size();
@@ -955,6 +1098,25 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
         }
         if (rc) break;
         cw.cfg = pick;
+        // the generally contracted form, kept where it beats the best plain launch
+        if (cw.gc.nwork > 0) {
+            double gbest = 1e300;
+            int gblock = cw.gc_block;
+            for (int blk : {256, 128, 64}) {
+                for (int r = 0; r <= reps && !rc; ++r) {
+                    if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
+                    rc = launch_gc(c, cw, blk);
+                }
+                if (rc) break;
+                HIPCHK(hipEventRecord(e1, c->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                const double ms = event_ms(e0, e1);
+                if (ms < gbest) { gbest = ms; gblock = blk; }
+            }
+            if (rc) break;
+            if (gbest < best) { cw.gc_on = 1; cw.gc_block = gblock; }
+            else if (best < 1e300) cw.gc_on = 0;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1039,6 +1201,11 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
+        if (!screen && cw.gc_on && cw.gc.nwork > 0) {
+            const int rc = launch_gc(c, cw, cw.gc_block);
+            if (rc) return rc;
+            continue;
+        }
         const bool seg = list_uses_seg(c, cw);
         const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
@@ -1318,13 +1485,22 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
         st->last_class_ms[i] = ctx->last_class_ms[i];
         st->class_quartets[i] = w.nwork;
         st->class_prec[i] = w.prec;
-        const int kind = w.cfg.kind != LAUNCH_AUTO ? w.cfg.kind
+        const int kind = w.gc_on && w.gc.nwork > 0 ? 4
+                       : w.cfg.kind != LAUNCH_AUTO ? w.cfg.kind
                        : auto_kind(ctx, w.la, w.lb, w.lc, w.ld, w.nseg > 0 && !ctx->pbc);
         st->class_kind[i] = kind;
-        st->class_block[i] = kind == LAUNCH_COOP ? 64 : w.cfg.block;
+        st->class_block[i] = kind == LAUNCH_COOP ? 64 : kind == 4 ? w.gc_block : w.cfg.block;
         st->class_grid[i] = w.cfg.grid;
         st->class_prim_quartets[i] = w.prim_quartets;
         st->class_flops[i] = w.flops;
+        if (kind == 4) {   // executed arithmetic: every member combination per primitive quartet
+            const int ni = ncart_h(w.la) * ncart_h(w.lc);
+            const int mm = (w.la == 0 ? 4 : 2) * (w.lc == 0 ? 4 : 2);
+            st->class_prim_quartets[i] = w.gc.prim_quartets;
+            st->class_flops[i] = w.gc.prim_quartets * (flop_model(w.la, 0, w.lc, 0) + 2.0 * mm * ni);
+            st->prim_quartets += w.gc.prim_quartets - w.prim_quartets;
+            st->model_flops += st->class_flops[i] - w.flops;
+        }
         st->class_l[i] = w.la * 1000 + w.lb * 100 + w.lc * 10 + w.ld;
     }
     return 0;

@@ -82,6 +82,19 @@ int extract_shells(const RefBasis& rb, IntBasis& ib, std::string& err);
 void build_pairs(const IntBasis& ib, double primtol, std::vector<PairInfo>& pairs,
                  std::vector<PrimPair>& prims);
 void boys_table(std::vector<double>& tab);       // BOYS_NT x BOYS_NM
+// Generally contracted pairs for the classes (ss), (ps), (ds).  Shell
+// groups: an s shell whose exponents all occur among those of an earlier s
+// shell on the same centre joins that shell's group (two shells at most);
+// every other shell is a group of its own.  A pair of groups g1 <= g2 (the
+// second of L 0) has as members the shell pairs of the two groups
+// (unordered when g1 == g2), primitive pairs over the exponents of the
+// groups' first shells (K without coefficients) and one weight per member
+// and primitive pair.  member_pair[4 i + j]: index in pairs of member j of
+// gpairs[i] (-1: no such pair, or j >= m).  Returns the number of two-shell
+// groups; with none the outputs stay empty.
+int build_gc_pairs(const IntBasis& ib, double primtol, const std::vector<PairInfo>& pairs,
+                   std::vector<PairInfo>& gpairs, std::vector<GcMeta>& gmeta, std::vector<GcPrim>& gprims,
+                   std::vector<int>& member_pair);
 // class_ids[X] must hold the pairs of class X in Schwarz order (bra rows);
 // fills ket_ids / bstart (bucketed ket order) and the lists.  cut > tol
 // splits every class pair into an FP64 list and an FP32 list (see WorkList).
