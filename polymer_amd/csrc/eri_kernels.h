@@ -680,28 +680,33 @@ __global__ void __launch_bounds__(256) k_eri_gc(EriArgs g) {
             double acc[MB * MK * NI];
 #pragma unroll
             for (int i = 0; i < MB * MK * NI; ++i) acc[i] = 0.0;
-            for (int p = 0; p < mb.nprim; ++p) {
-                const GcPrim bq = bra[p];
-                for (int q = 0; q < mk.nprim; ++q) {
-                    const double* x = kb + (long long)q * nk;
-                    PrimPair kq;
-                    kq.zeta = x[0]; kq.K = x[fs]; kq.Px = x[2 * fs]; kq.Py = x[3 * fs]; kq.Pz = x[4 * fs];
-                    kq.PAx = x[5 * fs]; kq.PAy = x[6 * fs]; kq.PAz = x[7 * fs];
-                    double wk[MK];
+            // ket primitive outer (its fields stay in registers), bra inner
+            // (the same for the whole block: scalar loads)
+            for (int q = 0; q < mk.nprim; ++q) {
+                const double* x = kb + (long long)q * nk;
+                PrimPair kq;
+                kq.zeta = x[0]; kq.K = x[fs]; kq.Px = x[2 * fs]; kq.Py = x[3 * fs]; kq.Pz = x[4 * fs];
+                kq.PAx = x[5 * fs]; kq.PAy = x[6 * fs]; kq.PAz = x[7 * fs];
+                double wk[MK];
 #pragma unroll
-                    for (int j = 0; j < MK; ++j) wk[j] = x[(8 + j) * fs];
+                for (int j = 0; j < MK; ++j) wk[j] = x[(8 + j) * fs];
+                for (int p = 0; p < mb.nprim; ++p) {
+                    const GcPrim bq = bra[p];
                     double tv[NI];
 #pragma unroll
                     for (int i = 0; i < NI; ++i) tv[i] = 0.0;
                     prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
+                    double u[MB * NI];
+#pragma unroll
+                    for (int i = 0; i < MB; ++i)
+#pragma unroll
+                        for (int e = 0; e < NI; ++e) u[i * NI + e] = bq.w[i] * tv[e];
 #pragma unroll
                     for (int j = 0; j < MK; ++j)
 #pragma unroll
-                        for (int i = 0; i < MB; ++i) {
-                            const double w = bq.w[i] * wk[j];
+                        for (int i = 0; i < MB; ++i)
 #pragma unroll
-                            for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += w * tv[e];
-                        }
+                            for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += wk[j] * u[i * NI + e];
                 }
             }
             const double sq = bp == kp ? 0.5 : 1.0;
