@@ -294,6 +294,14 @@ int polyeri_set_profiling(polyeri_ctx *ctx, int on);
  * eagerly).  No counterpart in the reference. */
 int polyeri_set_graph(polyeri_ctx *ctx, int on);
 
+/* Generally contracted evaluation of the (ls|l's) classes on or off.  With a
+ * basis whose s shells on one centre share exponents (cc-pVDZ 1s/2s), those
+ * lists can run over pairs of shell groups, one primitive loop for all member
+ * quartets (on by default where such groups exist; polyeri_autotune keeps it
+ * per list where it is faster).  Same quartets, same results to rounding.
+ * No counterpart in the reference. */
+int polyeri_set_general_contraction(polyeri_ctx *ctx, int on);
+
 /* Autotuner (no counterpart in the reference): times every work list with
  * each candidate launch shape (thread-per-quartet kernel with 256/128/64
  * threads per block, one pass or a persistent grid; wavefront-per-quartet

@@ -352,6 +352,12 @@ class Context:
         eagerly."""
         check(lib().polyeri_set_graph(self.h, int(bool(on))))
 
+    def set_general_contraction(self, on):
+        """Evaluate the (ls|l's) classes over generally contracted s-shell
+        groups (default where the basis has them) or shell pair by shell
+        pair."""
+        check(lib().polyeri_set_general_contraction(self.h, int(bool(on))))
+
     def autotune(self, reps=2):
         """Time every work list with each candidate launch shape and keep the
         fastest (results do not change)."""

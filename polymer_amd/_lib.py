@@ -17,7 +17,7 @@ EXPORTS = [
     "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
     "polyeri_fock_build_device", "polyeri_device_copy", "polyeri_jk_build_screened",
-    "polyeri_autotune", "polyeri_set_graph",
+    "polyeri_autotune", "polyeri_set_graph", "polyeri_set_general_contraction",
     "polyeri_mlp_create", "polyeri_mlp_destroy", "polyeri_mlp_nparam", "polyeri_mlp_params",
     "polyeri_mlp_grads", "polyeri_mlp_forward", "polyeri_mlp_train_step", "polyeri_gemm_bf16",
     "polyeri_mp2_energy_ex", "polyeri_gemm_f64", "polyeri_ctx_create_periodic", "polyeri_jk_build_cells",
@@ -104,6 +104,7 @@ def lib():
     L.polyeri_jk_build_screened.argtypes = [p, p, ctypes.c_double, p, p]
     L.polyeri_autotune.argtypes = [p, ctypes.c_int]
     L.polyeri_set_graph.argtypes = [p, ctypes.c_int]
+    L.polyeri_set_general_contraction.argtypes = [p, ctypes.c_int]
     L.polyeri_jk_build_device.argtypes = [p, p, p, p]
     L.polyeri_sync.argtypes = [p]
     L.polyeri_one_electron.argtypes = [p, p, p, p, p]

@@ -47,10 +47,13 @@ struct PrimPair {
 // each of its m (<= 4) member shell pairs weighs primitive pair p with
 // w[j] = c_a(p) c_b(p) of its own two shells.  Member j pairs shell
 // ((code >> 2j) & 1) of the first group (first functions fa0 / fa1) with
-// shell ((code >> 2j+1) & 1) of the second group (fb0 / fb1).
+// shell ((code >> 2j+1) & 1) of the second group (fb0 / fb1).  A member
+// quartet is digested only when q_bra q_ket reaches the Schwarz threshold,
+// the same test the plain lists apply.
 struct GcMeta {
     int prim_off, nprim, m, code;
     int fa0, fa1, fb0, fb1;
+    double q[4];   // Schwarz bound of each member pair (0 where the pair has no primitive pairs)
 };
 struct GcPrim {
     PrimPair pp;   // K = sqrt(2) pi^{5/4} exp(-ab/zeta |AB|^2) / zeta

@@ -75,6 +75,7 @@ struct EriArgs {
     // ket fields 8 .. 11 of ksoa = the member weights
     const GcMeta* gmeta;
     const GcPrim* gprims;
+    double qtol;               // Schwarz threshold of the member quartets
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -640,12 +641,13 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
 // k_eri_seg, but an item is a pair of generally contracted pairs (GcMeta)
 // and computes all member quartets from one primitive loop (member weights
 // applied to each primitive quartet's [a0|c0] block).  Each member quartet
-// is digested with its own degeneracy weight; a list row whose bra equals
+// that passes the Schwarz test is digested with its own degeneracy weight; a list row whose bra equals
 // the ket visits both orders of two different members, each at half
 // weight.  LDS: the exchange rows of the up to two shells of each bra group
 // ((2 NA + 2) nbf doubles) and J_ab of every bra member (MB NA doubles).
 template <int LA, int LC>
-__global__ void __launch_bounds__(256) k_eri_gc(EriArgs g) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LA == 1 && LC == 0 ? 3 : 1)))
+k_eri_gc(EriArgs g) {
     constexpr int NA = ncart(LA), NC = ncart(LC), NI = NA * NC;
     constexpr int MB = LA == 0 ? 4 : 2, MK = LC == 0 ? 4 : 2;
     extern __shared__ double seg_lds[];
@@ -718,6 +720,7 @@ __global__ void __launch_bounds__(256) k_eri_gc(EriArgs g) {
 #pragma unroll
                 for (int j = 0; j < MK; ++j) {
                     if (j >= mk.m) break;
+                    if (mb.q[i] * mk.q[j] < g.qtol) continue;
                     const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
                     const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
                     double v[NI];

@@ -518,7 +518,7 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block) {
     a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF;
     a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
     a.segs = g.d_seg; a.nseg = g.nseg;
-    a.gmeta = c->d_gmeta; a.gprims = c->d_gprims;
+    a.gmeta = c->d_gmeta; a.gprims = c->d_gprims; a.qtol = c->opts.schwarz_tol;
     if (block != 64 && block != 128) block = 256;
     if (polyeri_gc_launch(a, cw.la, cw.lc, block, g.nseg, c->stream))
         return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
@@ -641,8 +641,11 @@ static int build_gc_lists(polyeri_ctx* c) {
     const int ng = (int)gp.size();
     std::vector<double> gQ(ng, 0.0);
     for (int i = 0; i < ng; ++i)
-        for (int k = 0; k < GC_MAXM; ++k)
-            if (mp[(size_t)GC_MAXM * i + k] >= 0) gQ[i] = std::max(gQ[i], c->Q[mp[(size_t)GC_MAXM * i + k]]);
+        for (int k = 0; k < GC_MAXM; ++k) {
+            const int pk = mp[(size_t)GC_MAXM * i + k];
+            gm[i].q[k] = pk >= 0 ? c->Q[pk] : 0.0;
+            gQ[i] = std::max(gQ[i], gm[i].q[k]);
+        }
     std::vector<int> cls[NPAIRCLASS], kets[NPAIRCLASS], bst[NPAIRCLASS];
     for (int i = 0; i < ng; ++i) cls[gp[i].cls].push_back(i);
     for (int X = 0; X < NPAIRCLASS; ++X)
@@ -1130,6 +1133,13 @@ int polyeri_set_graph(polyeri_ctx* ctx, int on) {
     if (!ctx) return fail(POLYERI_E_ARG, "null context");
     ctx->use_graph = on != 0;
     if (!ctx->use_graph) drop_graphs(ctx);
+    return 0;
+}
+
+int polyeri_set_general_contraction(polyeri_ctx* ctx, int on) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    for (ClassPairWork& w : ctx->work) w.gc_on = on && w.gc.nwork > 0;
+    drop_graphs(ctx);
     return 0;
 }
 
