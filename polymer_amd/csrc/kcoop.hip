@@ -55,6 +55,8 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             kp = g.ket_ids[pos];
         }
         const PairMeta mb = g.meta[bp], mk = g.meta[kp];
+        // per-primitive VRR coefficients, indexed by the ops' (side, direction)
+        real* cf = sl + d.nslot + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
         for (int k = lane; k <= d.nacc; k += 64) sl[k] = real(0);
         for (int p = 0; p < mb.nprim; ++p) {
             const PrimPair bq = g.prims[mb.prim_off + p];
@@ -67,22 +69,34 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
                 const double pref = bq.K * kq.K * sqrt(ize);
                 const double roz = kq.zeta * ize, roe = bq.zeta * ize;
-                const double oo2z = 0.5 / bq.zeta, oo2e = 0.5 / kq.zeta, oo2ze = 0.5 * ize;
+                const real oo2ze = (real)(0.5 * ize);
+                // cf[3 typ + i]: P - A (bra) / Q - C (ket); cf[6 + 3 typ + i]:
+                // W - P / W - Q; cf[12 + typ]: 1/2zeta, 1/2eta; cf[14 + typ]:
+                // rho/zeta, rho/eta.  Read from LDS so that no select of the
+                // two reciprocals (which the compiler turns into a division
+                // per op) appears in the op loop.
+                if (lane < 16) {
+                    const int typ = lane >= 12 ? lane & 1 : (lane % 6) / 3, i = lane % 3;
+                    const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
+                    double v;
+                    if (lane < 6) v = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
+                                          : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
+                    else if (lane < 12) v = typ ? roe * pq : -roz * pq;
+                    else if (lane < 14) v = 0.5 / (typ ? kq.zeta : bq.zeta);
+                    else v = typ ? roe : roz;
+                    cf[lane] = (real)v;
+                }
                 if (lane <= d.L) sl[d.base + lane] = (real)(pref * boys_taylor(lane, T, g.boys));
                 __syncthreads();
                 for (int lv = 0; lv < d.nvrr_lv; ++lv) {
                     for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
                         const uint4 op = g_prog_vrr[o];
-                        const int typ = op.w & 1, i = (op.w >> 1) & 3;
+                        const int typ = op.w & 1, ti = 3 * typ + ((op.w >> 1) & 3);
                         const real n3 = (real)((op.w >> 3) & 7), n5 = (real)((op.w >> 6) & 7);
-                        const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
-                        const real c1 = (real)(typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
-                                                   : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz)));
-                        const real c2 = (real)(typ ? roe * pq : -roz * pq);
-                        const real x3 = (real)(typ ? oo2e : oo2z), r3 = (real)(typ ? roe : roz);
+                        const real c1 = cf[ti], c2 = cf[6 + ti], x3 = cf[12 + typ], r3 = cf[14 + typ];
                         sl[op.x & 0xffff] = c1 * sl[op.x >> 16] + c2 * sl[op.y & 0xffff] +
                                             n3 * x3 * (sl[op.y >> 16] - r3 * sl[op.z & 0xffff]) +
-                                            n5 * (real)oo2ze * sl[op.z >> 16];
+                                            n5 * oo2ze * sl[op.z >> 16];
                     }
                     __syncthreads();
                 }
@@ -201,12 +215,13 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
 }
 
 // LDS doubles one wavefront of the class needs: the program's node slots,
-// plus, for J/K, the dense integral block and the six density blocks.
+// the dense integral block and the six density blocks (J/K), and the 16
+// VRR coefficients of the current primitive quartet.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
     const int p = prog_index(la, lb, lc, ld);
     if (p < 0) return -1;
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
+    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + 16;
 }
 
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
