@@ -205,9 +205,10 @@ def main():
     traffic, traffic_src = None, None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
     kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri", "gc": "k_eri_gc"}[dc["kernel"]]
+    # the wavefront kernel is one symbol for every heavy class: its entries carry the class
     dkern = (f"void k_eri_gc<{dc['cls'][0]}, {dc['cls'][2]}>(EriArgs)" if kfn == "k_eri_gc"
              else f"void {kfn}<{', '.join(dc['cls'])}, 0>(EriArgs)" if kfn != "k_eri_coop"
-             else "void k_eri_coop<0>(EriArgs, int, int, int, int, int)")
+             else f"void k_eri_coop<0>(EriArgs, int, int, int, int, int) {cname(dc)}")
     if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
         with open(tpath) as f:
             tk = json.load(f)["kernels"].get(dkern)
