@@ -53,6 +53,7 @@ struct PrimPair {
 struct GcMeta {
     int prim_off, nprim, m, code;
     int fa0, fa1, fb0, fb1;
+    int sa0, sa1, sb0, sb1;   // shell indices of the same four shells (-1: none)
     double q[4];   // Schwarz bound of each member pair (0 where the pair has no primitive pairs)
 };
 struct GcPrim {

@@ -645,6 +645,25 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
 // the ket visits both orders of two different members, each at half
 // weight.  LDS: the exchange rows of the up to two shells of each bra group
 // ((2 NA + 2) nbf doubles) and J_ab of every bra member (MB NA doubles).
+// Largest shell-block density maximum between shells {x0, x1} and {y0, y1}
+// (x1 / y1 < 0: absent).
+__device__ __forceinline__ double gc_dmax(const EriArgs& g, int x0, int x1, int y0, int y1) {
+    const size_t ns = g.nsh;
+    double m = g.dsh[x0 * ns + y0];
+    if (y1 >= 0) m = fmax(m, g.dsh[x0 * ns + y1]);
+    if (x1 >= 0) {
+        m = fmax(m, g.dsh[x1 * ns + y0]);
+        if (y1 >= 0) m = fmax(m, g.dsh[x1 * ns + y1]);
+    }
+    return m;
+}
+
+// With density screening (dtol > 0) an item is skipped when the group
+// bound (largest member Q of each side times the largest density block over
+// the groups' shells) is below dtol, and a member quartet is digested only
+// when its own bound Q_ab Q_cd max|D| (six shell blocks) reaches dtol, the
+// test the shell-pair lists apply; digested member quartets are counted in
+// kept.
 template <int LA, int LC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LA == 1 && LC == 0 ? 3 : 1)))
 k_eri_gc(EriArgs g) {
@@ -670,6 +689,8 @@ k_eri_gc(EriArgs g) {
         double jab[MB * NA];
 #pragma unroll
         for (int i = 0; i < MB * NA; ++i) jab[i] = 0.0;
+        const bool screen = g.dtol > 0.0;
+        unsigned nkept = 0;
         for (long long t0 = seg.t0; t0 < seg.t0 + seg.n; t0 += blockDim.x) {
             const long long t = t0 + threadIdx.x;
             if (t >= seg.t0 + seg.n) continue;
@@ -677,6 +698,18 @@ k_eri_gc(EriArgs g) {
             decode_in_row(g, seg.row, t, pos, b);
             const int kp = g.ket_ids[pos];
             const GcMeta mk = g.gmeta[kp];
+            if (screen) {
+                double qb = 0.0, qk = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { qb = fmax(qb, mb.q[i]); qk = fmax(qk, mk.q[i]); }
+                double m = gc_dmax(g, mk.sa0, mk.sa1, mk.sb0, mk.sb1);
+                m = fmax(m, gc_dmax(g, mb.sa0, mb.sa1, mb.sb0, mb.sb1));
+                m = fmax(m, gc_dmax(g, mb.sb0, mb.sb1, mk.sb0, mk.sb1));
+                m = fmax(m, gc_dmax(g, mb.sb0, mb.sb1, mk.sa0, mk.sa1));
+                m = fmax(m, gc_dmax(g, mb.sa0, mb.sa1, mk.sb0, mk.sb1));
+                m = fmax(m, gc_dmax(g, mb.sa0, mb.sa1, mk.sa0, mk.sa1));
+                if (qb * qk * m < g.dtol) continue;
+            }
             const int nk = g.bucket_start[b + 1] - g.bucket_start[b];
             const double* kb = g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]);
             double acc[MB * MK * NI];
@@ -723,6 +756,16 @@ k_eri_gc(EriArgs g) {
                     if (mb.q[i] * mk.q[j] < g.qtol) continue;
                     const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
                     const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
+                    if (screen) {
+                        const size_t ns = g.nsh;
+                        const int a = ia ? mb.sa1 : mb.sa0, bb = ib ? mb.sb1 : mb.sb0;
+                        const int c = kc ? mk.sa1 : mk.sa0, d = kd ? mk.sb1 : mk.sb0;
+                        double m = fmax(g.dsh[c * ns + d], g.dsh[a * ns + bb]);
+                        m = fmax(m, fmax(g.dsh[bb * ns + d], g.dsh[bb * ns + c]));
+                        m = fmax(m, fmax(g.dsh[a * ns + d], g.dsh[a * ns + c]));
+                        if (mb.q[i] * mk.q[j] * m < g.dtol) continue;
+                        nkept += bp != kp || j <= i;   // both orders of a bra == ket row are one quartet
+                    }
                     double v[NI];
                     StoreCb<1, NC, 1, double> st{v};
                     ClassCode<LA, 0, LC, 0, double>::hrr(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, acc + (i * MK + j) * NI, st);
@@ -754,6 +797,11 @@ k_eri_gc(EriArgs g) {
         for (int i = 0; i < MB * NA; ++i) {
             const double w = wave_sum(jab[i]);
             if (lane == 0 && i < mb.m * NA) atomicAdd(&Jl[i], w);
+        }
+        if (screen) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) nkept += __shfl_xor(nkept, o, 64);
+            if (lane == 0 && nkept) atomicAdd(g.kept, nkept);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < (2 * NA + 2) * n; i += blockDim.x) {

@@ -425,6 +425,8 @@ int build_gc_pairs(const IntBasis& ib, double primtol, const std::vector<PairInf
             for (int k = 0; k < m; ++k) gm.code |= (mem[k][0] << (2 * k)) | (mem[k][1] << (2 * k + 1));
             gm.fa0 = ib.shells[G1[0]].f0; gm.fa1 = G1.size() > 1 ? ib.shells[G1[1]].f0 : -1;
             gm.fb0 = ib.shells[G2[0]].f0; gm.fb1 = G2.size() > 1 ? ib.shells[G2[1]].f0 : -1;
+            gm.sa0 = G1[0]; gm.sa1 = G1.size() > 1 ? G1[1] : -1;
+            gm.sb0 = G2[0]; gm.sb1 = G2.size() > 1 ? G2[1] : -1;
             for (int k = 0; k < GC_MAXM; ++k)
                 member_pair.push_back(k < m ? pidx[(size_t)G1[mem[k][0]] * ns + G2[mem[k][1]]] : -1);
             gpairs.push_back(pi);

@@ -507,8 +507,12 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     return 0;
 }
 
-// J/K launch of a list's generally contracted form (one block per segment).
-static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block) {
+// Whether a list's J/K launches run its generally contracted form.
+static bool list_uses_gc(const ClassPairWork& cw) { return cw.gc_on && cw.gc.nwork > 0; }
+
+// J/K launch of a list's generally contracted form (one block per segment),
+// density-screened inside the kernel when scr is given.
+static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const SegScreen* scr = nullptr) {
     const auto& g = cw.gc;
     if (g.nwork <= 0) return 0;
     EriArgs a{};
@@ -519,6 +523,7 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block) {
     a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
     a.segs = g.d_seg; a.nseg = g.nseg;
     a.gmeta = c->d_gmeta; a.gprims = c->d_gprims; a.qtol = c->opts.schwarz_tol;
+    if (scr) { a.dtol = scr->tol; a.dsh = scr->dsh; a.nsh = scr->nsh; a.kept = scr->kept; }
     if (block != 64 && block != 128) block = 256;
     if (polyeri_gc_launch(a, cw.la, cw.lc, block, g.nseg, c->stream))
         return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
@@ -1199,7 +1204,7 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
             return fail(POLYERI_E_HIP, "shell density maxima launch failed");
         for (size_t w = 0; w < c->work.size(); ++w) {
             const ClassPairWork& cw = c->work[w];
-            if (cw.nwork <= 0 || list_uses_seg(c, cw)) continue;   // segment lists screen in the kernel
+            if (cw.nwork <= 0 || list_uses_seg(c, cw) || list_uses_gc(cw)) continue;   // these screen in the kernel
             EriArgs a{};
             a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = cw.nwork;
             a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket; a.rb_first = cw.d_rf;
@@ -1211,13 +1216,13 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
     for (size_t w = 0; w < c->work.size(); ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
-        if (!screen && cw.gc_on && cw.gc.nwork > 0) {
-            const int rc = launch_gc(c, cw, cw.gc_block);
+        const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
+        if (list_uses_gc(cw)) {
+            const int rc = launch_gc(c, cw, cw.gc_block, screen ? &scr : nullptr);
             if (rc) return rc;
             continue;
         }
         const bool seg = list_uses_seg(c, cw);
-        const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
                               screen && !seg ? cw.d_items : nullptr, screen && !seg ? c->d_counts + w : nullptr,
@@ -1241,7 +1246,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     const bool screen = dens_tol > 0.0;
     if (screen)
         for (ClassPairWork& w : c->work)
-            if (!w.d_items && w.nwork > 0 && !list_uses_seg(c, w)) {   // segment lists keep no items
+            if (!w.d_items && w.nwork > 0 && !list_uses_seg(c, w) && !list_uses_gc(w)) {   // these keep no items
                 int rc = dalloc(&w.d_items, (size_t)w.nwork);
                 if (rc) return rc;
             }

@@ -48,6 +48,29 @@ def test_gc_matches_pair_lists(tol):
     assert ctx.stats()["quartets"] == q1
 
 
+@pytest.mark.parametrize("dens_tol", [1e-10, 1e-7])
+def test_gc_density_screened_matches_pair_lists(dens_tol):
+    """Density-screened builds: the generally contracted kernel skips group
+    quartets by a group bound and digests the member quartets that pass the
+    shell-pair lists' own test, so both forms keep the same quartets and
+    give the same J/K (1e-12)."""
+    atoms = G.water_cluster(4, 6)
+    print(f"case: screened generally contracted vs pair lists (H2O)4 dens_tol {dens_tol}", flush=True)
+    b = pa.read_basis("cc-pVDZ", atoms)
+    ctx = pa.Context(b, schwarz_tol=1e-12)
+    p = _density(b, 8)
+    p[np.abs(p) < 1e-3] *= 1e-6            # a sparse-ish density, so the screen drops quartets
+    j1, k1 = ctx.fourindex_eri(p, dens_tol=dens_tol)
+    kept1 = ctx.stats()["last_quartets"]
+    ctx.set_general_contraction(False)
+    j0, k0 = ctx.fourindex_eri(p, dens_tol=dens_tol)
+    kept0 = ctx.stats()["last_quartets"]
+    print(f"kept {kept1} vs {kept0} of {ctx.stats()['quartets']}; max |dJ| {np.max(np.abs(j1 - j0)):.2e} "
+          f"|dK| {np.max(np.abs(k1 - k0)):.2e}", flush=True)
+    assert kept1 == kept0 < ctx.stats()["quartets"]
+    assert np.max(np.abs(j1 - j0)) < 1e-12 and np.max(np.abs(k1 - k0)) < 1e-12
+
+
 def test_gc_h2o_vs_oracle():
     atoms = G.water()
     print("case: generally contracted J/K H2O cc-pVDZ vs oracle", flush=True)
