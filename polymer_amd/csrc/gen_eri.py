@@ -410,6 +410,13 @@ def write_programs(path):
         p = class_program(*c)
         d = {}
         # VRR ops grouped by level
+        # VRR op words carry LDS byte offsets of 8-byte slots (slot 0 = the
+        # zero slot for absent terms); w: typ | i << 1 | n3 << 3 | n5 << 6,
+        # bit 9 set when the op's node is a contracted target, whose
+        # accumulator's byte offset is then in w >> 16
+        dsts = {op[1] for op in p["ops"]}
+        acc_of = {s: a for a, s in p["accmap"] if s in dsts}
+        assert 8 * p["nslot"] < 65536
         lv_off = []
         cur = None
         for op in p["ops"]:
@@ -417,7 +424,10 @@ def write_programs(path):
                 cur = op[0]
                 lv_off.append(len(vops))
             _, dst, s1, s2, s3, s4, s5, typ, i, n3, n5 = op
-            vops.append((dst | (s1 << 16), s2 | (s3 << 16), s4 | (s5 << 16), typ | (i << 1) | (n3 << 3) | (n5 << 6)))
+            w = typ | (i << 1) | (n3 << 3) | (n5 << 6)
+            if dst in acc_of:
+                w |= (1 << 9) | ((8 * acc_of[dst]) << 16)
+            vops.append((8 * dst | (8 * s1 << 16), 8 * s2 | (8 * s3 << 16), 8 * s4 | (8 * s5 << 16), w))
         lv_off.append(len(vops))
         h_off = []
         cur = None
@@ -428,13 +438,16 @@ def write_programs(path):
             _, dst, s1, s2, typ, i = op
             hops.append((dst | (s1 << 16), s2, 0, typ | (i << 1)))
         h_off.append(len(hops))
+        # the accumulation map keeps only the targets no VRR op produces
+        # ((00|00)'s F_0, a Boys value): the others accumulate in their op
         acc_off = len(aux)
-        for a, s in p["accmap"]:
+        accrest = [(a, s) for a, s in p["accmap"] if s not in dsts]
+        for a, s in accrest:
             aux.append(a | (s << 16))
         fin_off = len(aux)
         for s, a, b, cc, dd in p["finals"]:
             aux.append(s | (a << 16) | (b << 20) | (cc << 24) | (dd << 28))
-        descs.append((c, lv_off, h_off, acc_off, len(p["accmap"]), fin_off, len(p["finals"]), p["nslot"], p["L"], p["nacc"], p["base"]))
+        descs.append((c, lv_off, h_off, acc_off, len(accrest), fin_off, len(p["finals"]), p["nslot"], p["L"], p["nacc"], p["base"]))
     maxlv = max(max(len(d[1]), len(d[2])) for d in descs)
     out = ["// Generated by gen_eri.py -- do not edit.", "#pragma once", "#include <stdint.h>", "",
            "// The tables have external linkage (no const at namespace scope): the HIP",

@@ -69,42 +69,54 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
                 const double pref = bq.K * kq.K * sqrt(ize);
                 const double roz = kq.zeta * ize, roe = bq.zeta * ize;
-                const real oo2ze = (real)(0.5 * ize);
                 // cf[3 typ + i]: P - A (bra) / Q - C (ket); cf[6 + 3 typ + i]:
-                // W - P / W - Q; cf[12 + typ]: 1/2zeta, 1/2eta; cf[14 + typ]:
-                // rho/zeta, rho/eta.  Read from LDS so that no select of the
-                // two reciprocals (which the compiler turns into a division
-                // per op) appears in the op loop.
-                if (lane < 16) {
-                    const int typ = lane >= 12 ? lane & 1 : (lane % 6) / 3, i = lane % 3;
+                // W - P / W - Q; cf[14 + typ]: rho/zeta, rho/eta;
+                // cf[16 + 8 typ + n]: n/2zeta, n/2eta; cf[32 + n]: n/2(zeta+eta).
+                // Read from LDS so that the op loop has no select of the two
+                // reciprocals (which the compiler turns into a division per
+                // op) and no integer-to-double conversions.
+                if (lane < 40) {
+                    const int typ = lane >= 16 ? (lane >> 3) & 1 : lane >= 12 ? lane & 1 : (lane % 6) / 3;
+                    const int i = lane % 3;
                     const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
                     double v;
                     if (lane < 6) v = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
                                           : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
                     else if (lane < 12) v = typ ? roe * pq : -roz * pq;
-                    else if (lane < 14) v = 0.5 / (typ ? kq.zeta : bq.zeta);
-                    else v = typ ? roe : roz;
+                    else if (lane < 14) v = 0.0;
+                    else if (lane < 16) v = typ ? roe : roz;
+                    else if (lane < 32) v = (lane & 7) * 0.5 / (typ ? kq.zeta : bq.zeta);
+                    else v = (lane - 32) * 0.5 * ize;
                     cf[lane] = (real)v;
                 }
                 if (lane <= d.L) sl[d.base + lane] = (real)(pref * boys_taylor(lane, T, g.boys));
                 __syncthreads();
+                // op words hold LDS byte offsets of 8-byte slots (halved for
+                // FP32 slots); a target node is also added to its accumulator
+                char* lb = reinterpret_cast<char*>(sl);
+                constexpr int SH = sizeof(real) == 8 ? 0 : 1;
+#define SLOT(off) (*reinterpret_cast<real*>(lb + ((off) >> SH)))
                 for (int lv = 0; lv < d.nvrr_lv; ++lv) {
                     for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
                         const uint4 op = g_prog_vrr[o];
                         const int typ = op.w & 1, ti = 3 * typ + ((op.w >> 1) & 3);
-                        const real n3 = (real)((op.w >> 3) & 7), n5 = (real)((op.w >> 6) & 7);
-                        const real c1 = cf[ti], c2 = cf[6 + ti], x3 = cf[12 + typ], r3 = cf[14 + typ];
-                        sl[op.x & 0xffff] = c1 * sl[op.x >> 16] + c2 * sl[op.y & 0xffff] +
-                                            n3 * x3 * (sl[op.y >> 16] - r3 * sl[op.z & 0xffff]) +
-                                            n5 * oo2ze * sl[op.z >> 16];
+                        const real k3 = cf[16 + 8 * typ + ((op.w >> 3) & 7)], k5 = cf[32 + ((op.w >> 6) & 7)];
+                        const real v = cf[ti] * SLOT(op.x >> 16) + cf[6 + ti] * SLOT(op.y & 0xffff) +
+                                       k3 * (SLOT(op.y >> 16) - cf[14 + typ] * SLOT(op.z & 0xffff)) +
+                                       k5 * SLOT(op.z >> 16);
+                        SLOT(op.x & 0xffff) = v;
+                        if (op.w & 512) SLOT(op.w >> 16) += v;
                     }
                     __syncthreads();
                 }
-                for (int k = lane; k < d.nacc_map; k += 64) {
-                    const uint32_t a = g_prog_aux[d.acc_off + k];
-                    sl[a & 0xffff] += sl[a >> 16];
+#undef SLOT
+                if (d.nacc_map > 0) {   // targets no VRR op produces
+                    for (int k = lane; k < d.nacc_map; k += 64) {
+                        const uint32_t a = g_prog_aux[d.acc_off + k];
+                        sl[a & 0xffff] += sl[a >> 16];
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
         }
         const real ab[3] = {(real)g.ab[4 * bp], (real)g.ab[4 * bp + 1], (real)g.ab[4 * bp + 2]};
@@ -215,13 +227,13 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
 }
 
 // LDS doubles one wavefront of the class needs: the program's node slots,
-// the dense integral block and the six density blocks (J/K), and the 16
+// the dense integral block and the six density blocks (J/K), and the 40
 // VRR coefficients of the current primitive quartet.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
     const int p = prog_index(la, lb, lc, ld);
     if (p < 0) return -1;
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + 16;
+    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + 40;
 }
 
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
