@@ -281,7 +281,7 @@ typedef struct polyeri_stats {
                                            3: thread per quartet, bra-row segments (LDS exchange rows),
                                            4: as 3 over generally contracted pairs (s shells sharing exponents) */
     int class_block[POLYERI_MAX_LISTS]; /* threads per block                 */
-    int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass           */
+    int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass; kinds 3 and 4: segments (blocks) launched */
     int nclass;
 } polyeri_stats;
 int polyeri_get_stats(polyeri_ctx *ctx, polyeri_stats *st);

@@ -639,8 +639,10 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
 
 // Segment kernel of the generally contracted lists, classes (ls|l's): as
 // k_eri_seg, but an item is a pair of generally contracted pairs (GcMeta)
-// and computes all member quartets from one primitive loop (member weights
-// applied to each primitive quartet's [a0|c0] block).  Each member quartet
+// and computes all member quartets from one primitive loop: for each ket
+// primitive, the primitive quartets' [a0|c0] blocks are summed over the bra
+// primitives with each bra member's weights, and that sum is added to every
+// member quartet with the ket members' weights.  Each member quartet
 // that passes the Schwarz test is digested with its own degeneracy weight; a list row whose bra equals
 // the ket visits both orders of two different members, each at half
 // weight.  LDS: the exchange rows of the up to two shells of each bra group
@@ -669,6 +671,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LA == 
 k_eri_gc(EriArgs g) {
     constexpr int NA = ncart(LA), NC = ncart(LC), NI = NA * NC;
     constexpr int MB = LA == 0 ? 4 : 2, MK = LC == 0 ? 4 : 2;
+    // factored member weights for (ss|ss) (measured faster there, no change
+    // for the other classes, whose bra sums cost registers)
+    constexpr bool GC_FACTOR = LA == 0 && LC == 0;
     extern __shared__ double seg_lds[];
     const int n = g.nbf;
     double* Kr = seg_lds;                        // rows fa0.. fa1.. (NA each), fb0, fb1
@@ -722,26 +727,54 @@ k_eri_gc(EriArgs g) {
                 PrimPair kq;
                 kq.zeta = x[0]; kq.K = x[fs]; kq.Px = x[2 * fs]; kq.Py = x[3 * fs]; kq.Pz = x[4 * fs];
                 kq.PAx = x[5 * fs]; kq.PAy = x[6 * fs]; kq.PAz = x[7 * fs];
-                double wk[MK];
+                if (GC_FACTOR) {
+                    // sum over the bra primitives with the bra members'
+                    // weights, then apply the ket members' weights once per
+                    // ket primitive
+                    double sb[MB * NI];
 #pragma unroll
-                for (int j = 0; j < MK; ++j) wk[j] = x[(8 + j) * fs];
-                for (int p = 0; p < mb.nprim; ++p) {
-                    const GcPrim bq = bra[p];
-                    double tv[NI];
+                    for (int i = 0; i < MB * NI; ++i) sb[i] = 0.0;
+                    for (int p = 0; p < mb.nprim; ++p) {
+                        const GcPrim bq = bra[p];
+                        double tv[NI];
 #pragma unroll
-                    for (int i = 0; i < NI; ++i) tv[i] = 0.0;
-                    prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
-                    double u[MB * NI];
-#pragma unroll
-                    for (int i = 0; i < MB; ++i)
-#pragma unroll
-                        for (int e = 0; e < NI; ++e) u[i * NI + e] = bq.w[i] * tv[e];
-#pragma unroll
-                    for (int j = 0; j < MK; ++j)
+                        for (int i = 0; i < NI; ++i) tv[i] = 0.0;
+                        prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
 #pragma unroll
                         for (int i = 0; i < MB; ++i)
 #pragma unroll
-                            for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += wk[j] * u[i * NI + e];
+                            for (int e = 0; e < NI; ++e) sb[i * NI + e] += bq.w[i] * tv[e];
+                    }
+#pragma unroll
+                    for (int j = 0; j < MK; ++j) {
+                        const double wk = x[(8 + j) * fs];
+#pragma unroll
+                        for (int i = 0; i < MB; ++i)
+#pragma unroll
+                            for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += wk * sb[i * NI + e];
+                    }
+                } else {
+                    double wk[MK];
+#pragma unroll
+                    for (int j = 0; j < MK; ++j) wk[j] = x[(8 + j) * fs];
+                    for (int p = 0; p < mb.nprim; ++p) {
+                        const GcPrim bq = bra[p];
+                        double tv[NI];
+#pragma unroll
+                        for (int i = 0; i < NI; ++i) tv[i] = 0.0;
+                        prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
+                        double u[MB * NI];
+#pragma unroll
+                        for (int i = 0; i < MB; ++i)
+#pragma unroll
+                            for (int e = 0; e < NI; ++e) u[i * NI + e] = bq.w[i] * tv[e];
+#pragma unroll
+                        for (int j = 0; j < MK; ++j)
+#pragma unroll
+                            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                                for (int e = 0; e < NI; ++e) acc[(i * MK + j) * NI + e] += wk[j] * u[i * NI + e];
+                    }
                 }
             }
             const double sq = bp == kp ? 0.5 : 1.0;

@@ -258,10 +258,68 @@ static double flop_model(int la, int lb, int lc, int ld) {
 // register kernel, one per segment for the segment kernel, 4096 for the
 // cooperative one).  The autotuner fills these per list.
 enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2, LAUNCH_SEG = 3 };
-// Items per bra-row segment of the segment kernel.
-static const int kSegItems = 2048;
+// Items per bra-row segment of the segment kernel (at most), and the number
+// of segments a list's primitive quartets are spread over (at least).
+static const int kSegItems = 2048, kSegMin = 256;
+static const double kSegTarget = 4096.0;
+
+// Bra-row segments of a work list for the segment kernels: each segment is
+// part of one row, at most kSegItems items and about `target` primitive
+// quartets (the row's bra primitive count times each ket's), with target =
+// the list's primitive quartets / kSegTarget, but at least kSegMin items (one
+// pass of a block: shorter segments pay more for their row flush than they
+// gain in balance) where the row has them; ordered by
+// primitive quartets, largest first, so that the blocks launched last are
+// the short ones.  pairs / bra / kets / bstart: the list's pair data, bra
+// ids by row, ket ids and bucket starts of the ket class.  balanced = false:
+// rows cut every kSegItems items, in row order (the autotuner keeps
+// whichever form is faster per list).
+static std::vector<SegDesc> make_segments(const WorkList& l, const std::vector<PairInfo>& pairs,
+                                          const std::vector<int>& bra, const std::vector<int>& kets,
+                                          const std::vector<int>& bstart, bool balanced = true) {
+    const int nb = l.nbucket;
+    if (!balanced) {
+        std::vector<SegDesc> out;
+        for (int r = 0; r < l.nrows; ++r)
+            for (long long t = l.row[r]; t < l.row[r + 1]; t += kSegItems)
+                out.push_back(SegDesc{t, r, (int)std::min<long long>(kSegItems, l.row[r + 1] - t)});
+        return out;
+    }
+    std::vector<double> knp(nb, 0.0);
+    for (int b = 0; b + 1 < (int)bstart.size() && b < nb; ++b) knp[b] = pairs[kets[bstart[b]]].nprim;
+    const double target = l.prim_quartets / kSegTarget;
+    std::vector<std::pair<double, SegDesc>> segs;
+    for (int r = 0; r < l.nrows; ++r) {
+        const double bp = pairs[bra[r]].nprim;
+        const int* rb = l.rb.data() + (size_t)r * (nb + 1);
+        long long t = l.row[r];
+        int n = 0;
+        double cost = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            int cnt = rb[b + 1] - rb[b];
+            const double ci = bp * knp[b];
+            while (cnt > 0) {
+                int take = std::min(cnt, kSegItems - n);
+                if (ci > 0.0)
+                    take = (int)std::min<double>(take, std::max<double>(kSegMin - n, std::ceil((target - cost) / ci)));
+                take = std::max(take, 1);
+                n += take; cost += take * ci; cnt -= take;
+                if (n >= kSegItems || (n >= kSegMin && cost >= target)) {
+                    segs.push_back({cost, SegDesc{t, r, n}});
+                    t += n; n = 0; cost = 0.0;
+                }
+            }
+        }
+        if (n > 0) segs.push_back({cost, SegDesc{t, r, n}});
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<SegDesc> out(segs.size());
+    for (size_t i = 0; i < segs.size(); ++i) out[i] = segs[i].second;
+    return out;
+}
 struct LaunchCfg {
     int kind = LAUNCH_AUTO, block = 256, grid = 0;
+    int segv = 0;   // segment kernels: 0 cost-balanced segments, 1 segments of kSegItems items
 };
 
 struct ClassPairWork {
@@ -275,8 +333,8 @@ struct ClassPairWork {
     int* d_bsoa;           // bucket offsets of class Y in the interleaved ket data
     int* d_rf;             // FP32 list: nrows x nbucket first kets (else null)
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
-    SegDesc* d_seg;        // bra-row segments of at most kSegItems items (molecular lists)
-    int nseg;
+    SegDesc* d_seg;        // bra-row segments (molecular lists): nseg cost-balanced ones, then
+    int nseg, nsegc;       // nsegc of kSegItems items each (make_segments)
     LaunchCfg cfg;         // launch shape (autotuned or default)
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int both;              // periodic lists of two different classes: both bra/ket roles
@@ -290,14 +348,14 @@ struct ClassPairWork {
     // unscreened builds while gc_on (gc_block threads per block).
     struct {
         long long nwork = 0;
-        int nrows = 0, nbucket = 1, nseg = 0;
+        int nrows = 0, nbucket = 1, nseg = 0, nsegc = 0;
         const int *d_bra = nullptr, *d_ket = nullptr, *d_bstart = nullptr, *d_bsoa = nullptr;
         long long* d_row = nullptr;
         int* d_rb = nullptr;
         SegDesc* d_seg = nullptr;
         double prim_quartets = 0;
     } gc;
-    int gc_on = 0, gc_block = 256;
+    int gc_on = 0, gc_block = 256, gc_segv = 0;
 };
 
 struct polyeri_ctx {
@@ -521,11 +579,12 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const S
     a.ksoa = c->d_gksoa; a.kfs = c->gkfs; a.bucket_soa = g.d_bsoa;
     a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF;
     a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.jk_copies = c->jk_copies;
-    a.segs = g.d_seg; a.nseg = g.nseg;
+    a.segs = cw.gc_segv ? g.d_seg + g.nseg : g.d_seg;
+    a.nseg = cw.gc_segv ? g.nsegc : g.nseg;
     a.gmeta = c->d_gmeta; a.gprims = c->d_gprims; a.qtol = c->opts.schwarz_tol;
     if (scr) { a.dtol = scr->tol; a.dsh = scr->dsh; a.nsh = scr->nsh; a.kept = scr->kept; }
     if (block != 64 && block != 128) block = 256;
-    if (polyeri_gc_launch(a, cw.la, cw.lc, block, g.nseg, c->stream))
+    if (polyeri_gc_launch(a, cw.la, cw.lc, block, a.nseg, c->stream))
         return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
     return 0;
 }
@@ -708,13 +767,13 @@ static int build_gc_lists(polyeri_ctx* c) {
         for (ClassPairWork& x : c->work)
             if (x.X == l.X && x.Y == l.Y && x.prec == 0) w = &x;
         if (!w) continue;
-        std::vector<SegDesc> segs;
-        for (int r = 0; r < l.nrows; ++r)
-            for (long long t = l.row[r]; t < l.row[r + 1]; t += kSegItems)
-                segs.push_back(SegDesc{t, r, (int)std::min<long long>(kSegItems, l.row[r + 1] - t)});
-        if (segs.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
+        std::vector<SegDesc> segs = make_segments(l, gp, cls[l.X], kets[l.Y], bst[l.Y]);
+        const std::vector<SegDesc> segc = make_segments(l, gp, cls[l.X], kets[l.Y], bst[l.Y], false);
+        if (segs.size() + segc.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
         auto& g = w->gc;
-        g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket; g.nseg = (int)segs.size();
+        g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket;
+        g.nseg = (int)segs.size(); g.nsegc = (int)segc.size();
+        segs.insert(segs.end(), segc.begin(), segc.end());
         g.d_bra = c->d_gbra[l.X]; g.d_ket = c->d_gket[l.Y]; g.d_bstart = c->d_gbstart[l.Y];
         g.d_bsoa = c->d_gbsoa[l.Y];
         g.prim_quartets = l.prim_quartets;
@@ -873,7 +932,7 @@ static int build_ctx(polyeri_ctx* c) {
         w.d_rf = nullptr;
         w.d_items = nullptr;
         w.d_seg = nullptr;
-        w.nseg = 0;
+        w.nseg = w.nsegc = 0;
         w.prim_quartets = l.prim_quartets;
         w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
         w.d_row = nullptr;
@@ -887,12 +946,14 @@ static int build_ctx(polyeri_ctx* c) {
         // illegal memory access whose cause is not established, so mixed
         // contexts keep the register kernel.
         if (!c->pbc && l.nwork > 0 && c->opts.precision != POLYERI_MIXED) {
-            std::vector<SegDesc> segs;
-            for (int r = 0; r < l.nrows; ++r)
-                for (long long t = l.row[r]; t < l.row[r + 1]; t += kSegItems)
-                    segs.push_back(SegDesc{t, r, (int)std::min<long long>(kSegItems, l.row[r + 1] - t)});
-            if (segs.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
+            std::vector<SegDesc> segs =
+                make_segments(l, c->pairs, c->class_ids[l.X], c->ket_ids[l.Y], c->bstart[l.Y]);
+            const std::vector<SegDesc> segc =
+                make_segments(l, c->pairs, c->class_ids[l.X], c->ket_ids[l.Y], c->bstart[l.Y], false);
+            if (segs.size() + segc.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
             w.nseg = (int)segs.size();
+            w.nsegc = (int)segc.size();
+            segs.insert(segs.end(), segc.begin(), segc.end());
             if ((rc = dalloc(&w.d_seg, segs.size()))) return rc;
             HIPCHK(hipMemcpy(w.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
         }
@@ -1085,7 +1146,10 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
             for (int b : {256, 128, 64}) { LaunchCfg x; x.kind = LAUNCH_REG; x.block = b; cand.push_back(x); }
             LaunchCfg x; x.kind = LAUNCH_REG; x.block = 256; x.grid = 2048; cand.push_back(x);
             if (cw.nseg > 0 && seg_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536)
-                for (int b : {256, 128, 64}) { LaunchCfg y; y.kind = LAUNCH_SEG; y.block = b; cand.push_back(y); }
+                for (int v : {0, 1})
+                    for (int b : {256, 128, 64}) {
+                        LaunchCfg y; y.kind = LAUNCH_SEG; y.block = b; y.segv = v; cand.push_back(y);
+                    }
         }
         if (!reg || nint >= 36)
             for (int g : {2048, 4096, 8192}) { LaunchCfg x; x.kind = LAUNCH_COOP; x.grid = g; cand.push_back(x); }
@@ -1096,7 +1160,7 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
                 if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
                 rc = launch_class(c, mode, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row, cw.nrows,
                                   cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr, x,
-                                  cw.d_bsoa, cw.d_seg, cw.nseg);
+                                  cw.d_bsoa, x.segv ? cw.d_seg + cw.nseg : cw.d_seg, x.segv ? cw.nsegc : cw.nseg);
             }
             if (rc) break;
             HIPCHK(hipEventRecord(e1, c->stream));
@@ -1109,18 +1173,21 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
         // the generally contracted form, kept where it beats the best plain launch
         if (cw.gc.nwork > 0) {
             double gbest = 1e300;
-            int gblock = cw.gc_block;
-            for (int blk : {256, 128, 64}) {
-                for (int r = 0; r <= reps && !rc; ++r) {
-                    if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
-                    rc = launch_gc(c, cw, blk);
+            int gblock = cw.gc_block, gsegv = cw.gc_segv;
+            for (int v : {0, 1})
+                for (int blk : {256, 128, 64}) {
+                    cw.gc_segv = v;
+                    for (int r = 0; r <= reps && !rc; ++r) {
+                        if (r == 1) HIPCHK(hipEventRecord(e0, c->stream));
+                        rc = launch_gc(c, cw, blk);
+                    }
+                    if (rc) break;
+                    HIPCHK(hipEventRecord(e1, c->stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    const double ms = event_ms(e0, e1);
+                    if (ms < gbest) { gbest = ms; gblock = blk; gsegv = v; }
                 }
-                if (rc) break;
-                HIPCHK(hipEventRecord(e1, c->stream));
-                HIPCHK(hipEventSynchronize(e1));
-                const double ms = event_ms(e0, e1);
-                if (ms < gbest) { gbest = ms; gblock = blk; }
-            }
+            cw.gc_segv = gsegv;
             if (rc) break;
             if (gbest < best) { cw.gc_on = 1; cw.gc_block = gblock; }
             else if (best < 1e300) cw.gc_on = 0;
@@ -1226,7 +1293,8 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
         int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
                               screen && !seg ? cw.d_items : nullptr, screen && !seg ? c->d_counts + w : nullptr,
-                              cw.cfg, cw.d_bsoa, cw.d_seg, cw.nseg, screen && seg ? &scr : nullptr);
+                              cw.cfg, cw.d_bsoa, cw.cfg.segv ? cw.d_seg + cw.nseg : cw.d_seg,
+                              cw.cfg.segv ? cw.nsegc : cw.nseg, screen && seg ? &scr : nullptr);
         if (rc) return rc;
     }
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
@@ -1505,7 +1573,8 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
                        : auto_kind(ctx, w.la, w.lb, w.lc, w.ld, w.nseg > 0 && !ctx->pbc);
         st->class_kind[i] = kind;
         st->class_block[i] = kind == LAUNCH_COOP ? 64 : kind == 4 ? w.gc_block : w.cfg.block;
-        st->class_grid[i] = w.cfg.grid;
+        st->class_grid[i] = kind == 4 ? (w.gc_segv ? w.gc.nsegc : w.gc.nseg)
+                          : kind == LAUNCH_SEG ? (w.cfg.segv ? w.nsegc : w.nseg) : w.cfg.grid;
         st->class_prim_quartets[i] = w.prim_quartets;
         st->class_flops[i] = w.flops;
         if (kind == 4) {   // executed arithmetic: every member combination per primitive quartet
