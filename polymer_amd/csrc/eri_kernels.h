@@ -855,6 +855,172 @@ k_eri_gc(EriArgs g) {
     }
 }
 
+// Segment kernel for classes (ab|c s) whose bra is a shell pair and whose
+// kets are generally contracted pairs ((pp|ss) (pp|ps) (ds|ps) (dp|ss)):
+// bra and ket classes differ, so no bra-ket symmetry is involved.  For each
+// ket primitive the VRR sums over the bra primitives (the block's shared
+// bra), and that block is added to every ket member with its weight; after
+// the loops the bra HRR runs per ket member, whose quartet is digested as in
+// k_eri_seg when it passes Q_ab q_cd >= qtol (and, screened, the density
+// test of the shell-pair lists).  LDS as k_eri_seg.
+// One member quartet of k_eri_gck: bra HRR of its [e0|c0] block, then the
+// segment kernel's digestion (exchange rows of the bra shells in LDS, J_ab
+// in jab, J_cd to the global copy).
+template <int LA, int LB, int LC>
+__device__ __forceinline__ void gck_digest(const EriArgs& g, const double* acc, double ABx, double ABy, double ABz,
+                                           int fa, int fb, int fc, int fd, double* Kr, double* J, double* jab) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC);
+    const int n = g.nbf;
+    double v[NA * NB * NC];
+    StoreCb<NB, NC, 1, double> st{v};
+    ClassCode<LA, LB, LC, 0, double>::hrr(ABx, ABy, ABz, 0.0, 0.0, 0.0, acc, st);
+    RegDigest<NA, NB, NC, 1, double> cb;
+    cb.load(g.D, n, fa, fb, fc, fd);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cb(a, bb, c, 0, v[(a * NB + bb) * NC + c]);
+    const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0);
+#pragma unroll
+    for (int i = 0; i < NA * NB; ++i) jab[i] += s * cb.Jab[i];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) atomicAdd(&Kr[(size_t)a * n + fc + c], s * cb.Kac[a * NC + c]);
+        atomicAdd(&Kr[(size_t)a * n + fd], s * cb.Kad[a]);
+    }
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) atomicAdd(&Kr[(size_t)(NA + bb) * n + fc + c], s * cb.Kbc[bb * NC + c]);
+        atomicAdd(&Kr[(size_t)(NA + bb) * n + fd], s * cb.Kbd[bb]);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) jk_add(&J[(size_t)(fc + c) * n + fd], s * cb.Jcd[c], g.jk_copies);
+}
+
+template <int LA, int LB, int LC>
+__global__ void __launch_bounds__(256) k_eri_gck(EriArgs g) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC);
+    constexpr int NE = ((LA + LB + 1) * (LA + LB + 2) * (LA + LB + 3) - LA * (LA + 1) * (LA + 2)) / 6;
+    constexpr int NF = ncart(LC);
+    constexpr int MK = LC == 0 ? 4 : 2;
+    extern __shared__ double seg_lds[];
+    const int n = g.nbf;
+    double* Kr = seg_lds;                        // rows fa.. (NA) then fb.. (NB), width n
+    double* Jl = seg_lds + (size_t)(NA + NB) * n;
+    const int total = (NA + NB) * n + NA * NB;
+    const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
+    double* J = g.J + off;
+    double* K = g.K + off;
+    const int lane = threadIdx.x & 63;
+    const long long fs = g.kfs;
+    for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
+        const SegDesc seg = g.segs[sg];
+        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
+        __syncthreads();
+        const int bp = g.bra_ids[seg.row];
+        const PairMeta mb = g.meta[bp];
+        const int fa = mb.fa, fb = mb.fb;
+        const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
+        const PrimPair* bra = g.prims + mb.prim_off;
+        const double qb = g.Qs[bp];
+        const bool screen = g.dtol > 0.0;
+        double jab[NA * NB];
+#pragma unroll
+        for (int i = 0; i < NA * NB; ++i) jab[i] = 0.0;
+        unsigned nkept = 0;
+        for (long long t0 = seg.t0; t0 < seg.t0 + seg.n; t0 += blockDim.x) {
+            const long long t = t0 + threadIdx.x;
+            if (t >= seg.t0 + seg.n) continue;
+            int pos, b;
+            decode_in_row(g, seg.row, t, pos, b);
+            const int kp = g.ket_ids[pos];
+            const GcMeta mk = g.gmeta[kp];
+            const int2 ab = screen ? g.psh[bp] : make_int2(0, 0);
+            if (screen) {
+                double qk = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) qk = fmax(qk, mk.q[i]);
+                double m = gc_dmax(g, mk.sa0, mk.sa1, mk.sb0, mk.sb1);
+                m = fmax(m, g.dsh[(size_t)ab.x * g.nsh + ab.y]);
+                m = fmax(m, gc_dmax(g, ab.y, -1, mk.sb0, mk.sb1));
+                m = fmax(m, gc_dmax(g, ab.y, -1, mk.sa0, mk.sa1));
+                m = fmax(m, gc_dmax(g, ab.x, -1, mk.sb0, mk.sb1));
+                m = fmax(m, gc_dmax(g, ab.x, -1, mk.sa0, mk.sa1));
+                if (qb * qk * m < g.dtol) continue;
+            }
+            const int nk = g.bucket_start[b + 1] - g.bucket_start[b];
+            const double* kb = g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]);
+            double acc[MK * NE * NF];
+#pragma unroll
+            for (int i = 0; i < MK * NE * NF; ++i) acc[i] = 0.0;
+            for (int q = 0; q < mk.nprim; ++q) {
+                const double* x = kb + (long long)q * nk;
+                PrimPair kq;
+                kq.zeta = x[0]; kq.K = x[fs]; kq.Px = x[2 * fs]; kq.Py = x[3 * fs]; kq.Pz = x[4 * fs];
+                kq.PAx = x[5 * fs]; kq.PAy = x[6 * fs]; kq.PAz = x[7 * fs];
+                double sb[NE * NF];
+#pragma unroll
+                for (int i = 0; i < NE * NF; ++i) sb[i] = 0.0;
+                for (int p = 0; p < mb.nprim; ++p) prim_vrr<double, LA, LB, LC, 0>(bra[p], kq, g.boysA, sb);
+#pragma unroll
+                for (int j = 0; j < MK; ++j) {
+                    const double wk = x[(8 + j) * fs];
+#pragma unroll
+                    for (int e = 0; e < NE * NF; ++e) acc[j * NE * NF + e] += wk * sb[e];
+                }
+            }
+            // one digestion instance: member j is always in acc[0 ..), the
+            // accumulators move down one member after each step
+#pragma unroll 1
+            for (int j = 0; j < mk.m; ++j) {
+                const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
+                const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
+                bool live = qb * mk.q[j] >= g.qtol;
+                if (screen && live) {
+                    const size_t ns = g.nsh;
+                    const int c = kc ? mk.sa1 : mk.sa0, d = kd ? mk.sb1 : mk.sb0;
+                    double m = fmax(g.dsh[c * ns + d], g.dsh[ab.x * ns + ab.y]);
+                    m = fmax(m, fmax(g.dsh[ab.y * ns + d], g.dsh[ab.y * ns + c]));
+                    m = fmax(m, fmax(g.dsh[ab.x * ns + d], g.dsh[ab.x * ns + c]));
+                    live = qb * mk.q[j] * m >= g.dtol;
+                    nkept += live;
+                }
+                if (live) gck_digest<LA, LB, LC>(g, acc, ABx, ABy, ABz, fa, fb, fc, fd, Kr, J, jab);
+#pragma unroll
+                for (int jj = 0; jj + 1 < MK; ++jj)
+#pragma unroll
+                    for (int e = 0; e < NE * NF; ++e) acc[jj * NE * NF + e] = acc[(jj + 1) * NE * NF + e];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NA * NB; ++i) {
+            const double w = wave_sum(jab[i]);
+            if (lane == 0) atomicAdd(&Jl[i], w);
+        }
+        if (screen) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) nkept += __shfl_xor(nkept, o, 64);
+            if (lane == 0 && nkept) atomicAdd(g.kept, nkept);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+            const double x = Kr[i];
+            if (x != 0.0) {
+                const int r = i / n, col = i - r * n;
+                const int f = r < NA ? fa + r : fb + (r - NA);
+                jk_add(&K[(size_t)f * n + col], x, g.jk_copies);
+            }
+        }
+        for (int i = threadIdx.x; i < NA * NB; i += blockDim.x)
+            jk_add(&J[(size_t)(fa + i / NB) * n + fb + i % NB], Jl[i], g.jk_copies);
+        __syncthreads();
+    }
+}
+
 typedef void (*EriKernel)(EriArgs);
 // Getter modes of polyeri_class_kernel beyond the k_eri modes: the segment
 // kernel for MODE_JK and MODE_JK32.
@@ -886,6 +1052,10 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
 size_t polyeri_gc_lds_bytes(int la, int nbf);
 bool polyeri_gc_class(int la, int lc);
 int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream);
+// The same for k_eri_gck (shell-pair bra (la lb|, generally contracted kets |lc s)).
+size_t polyeri_gck_lds_bytes(int la, int lb, int nbf);
+bool polyeri_gck_class(int la, int lb, int lc);
+int polyeri_gck_launch(const EriArgs& a, int la, int lb, int lc, int block, int grid, hipStream_t stream);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).

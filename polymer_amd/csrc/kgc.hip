@@ -1,5 +1,6 @@
-// kgc.hip -- launches of the generally contracted segment kernel
-// (k_eri_gc, eri_kernels.h) for the classes (ss|ss) (ps|ss) (ps|ps) (ds|ss).
+// kgc.hip -- launches of the generally contracted segment kernels
+// (eri_kernels.h): k_eri_gc for the classes (ss|ss) (ps|ss) (ps|ps) (ds|ss),
+// k_eri_gck for (pp|ss) (pp|ps) (ds|ps) (dp|ss).
 #include "eri_kernels.h"
 
 // LDS bytes of k_eri_gc for bra class (la s|.
@@ -22,6 +23,27 @@ int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hip
     else if (la == 1 && lc == 0) k = k_eri_gc<1, 0>;
     else if (la == 1) k = k_eri_gc<1, 1>;
     else k = k_eri_gc<2, 0>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), lds, stream, a);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+size_t polyeri_gck_lds_bytes(int la, int lb, int nbf) {
+    return 8 * ((size_t)(ncart(la) + ncart(lb)) * nbf + (size_t)ncart(la) * ncart(lb));
+}
+
+bool polyeri_gck_class(int la, int lb, int lc) {
+    return (la == 1 && lb == 1 && lc <= 1) || (la == 2 && lb == 0 && lc == 1) || (la == 2 && lb == 1 && lc == 0);
+}
+
+int polyeri_gck_launch(const EriArgs& a, int la, int lb, int lc, int block, int grid, hipStream_t stream) {
+    if (!polyeri_gck_class(la, lb, lc) || a.nseg <= 0 || grid <= 0) return -1;
+    const size_t lds = polyeri_gck_lds_bytes(la, lb, a.nbf);
+    if (lds > 65536) return -1;
+    void (*k)(EriArgs) = nullptr;
+    if (la == 1 && lc == 0) k = k_eri_gck<1, 1, 0>;
+    else if (la == 1) k = k_eri_gck<1, 1, 1>;
+    else if (lb == 0) k = k_eri_gck<2, 0, 1>;
+    else k = k_eri_gck<2, 1, 0>;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), lds, stream, a);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

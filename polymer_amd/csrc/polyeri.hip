@@ -354,6 +354,7 @@ struct ClassPairWork {
         int* d_rb = nullptr;
         SegDesc* d_seg = nullptr;
         double prim_quartets = 0;
+        int kind = 0;   // 0: group pairs on both sides (k_eri_gc); 1: shell-pair bra, group-pair kets (k_eri_gck)
     } gc;
     int gc_on = 0, gc_block = 256, gc_segv = 0;
 };
@@ -584,6 +585,12 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const S
     a.gmeta = c->d_gmeta; a.gprims = c->d_gprims; a.qtol = c->opts.schwarz_tol;
     if (scr) { a.dtol = scr->tol; a.dsh = scr->dsh; a.nsh = scr->nsh; a.kept = scr->kept; }
     if (block != 64 && block != 128) block = 256;
+    if (g.kind == 1) {   // shell-pair bra
+        a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab; a.Qs = c->d_Q; a.psh = c->d_psh;
+        if (polyeri_gck_launch(a, cw.la, cw.lb, cw.lc, block, a.nseg, c->stream))
+            return fail(POLYERI_E_HIP, "generally contracted ket kernel launch failed");
+        return 0;
+    }
     if (polyeri_gc_launch(a, cw.la, cw.lc, block, a.nseg, c->stream))
         return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
     return 0;
@@ -784,6 +791,55 @@ static int build_gc_lists(polyeri_ctx* c) {
         if (!l.rb.empty()) HIPCHK(hipMemcpy(g.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(g.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
         w->gc_on = 1;
+    }
+    // Shell-pair bras of the classes (pp) (ds) (dp) against group-pair kets
+    // of (ss) (ps): one list set over the shell pairs followed by the group
+    // pairs, so the ket order, buckets and interleaved ket data are the
+    // group-pair lists' own.  Off until the autotuner finds it faster.
+    const int np = (int)c->pairs.size();
+    std::vector<PairInfo> cp(c->pairs);
+    cp.insert(cp.end(), gp.begin(), gp.end());
+    std::vector<double> cq(c->Q);
+    cq.insert(cq.end(), gQ.begin(), gQ.end());
+    std::vector<int> cid[NPAIRCLASS], ck[NPAIRCLASS], cb[NPAIRCLASS];
+    for (int X : {2, 3, 4}) cid[X] = c->class_ids[X];
+    for (int Y : {0, 1})
+        for (int i : cls[Y]) cid[Y].push_back(np + i);
+    std::vector<WorkList> wk;
+    build_work_lists(cp, cq, c->opts.schwarz_tol, 0.0, cid, ck, cb, wk);
+    for (int Y : {0, 1}) {
+        if (ck[Y].size() != kets[Y].size() || cb[Y] != bst[Y])
+            return fail(POLYERI_E_BASIS, "generally contracted ket order differs between list sets");
+        for (size_t i = 0; i < ck[Y].size(); ++i)
+            if (ck[Y][i] != np + kets[Y][i])
+                return fail(POLYERI_E_BASIS, "generally contracted ket order differs between list sets");
+    }
+    for (const WorkList& l : wk) {
+        if (l.nwork <= 0 || l.Y > 1 || l.X < 2) continue;
+        const int la = kPairL[l.X][0], lb = kPairL[l.X][1], lc = kPairL[l.Y][0];
+        if (!polyeri_gck_class(la, lb, lc) || polyeri_gck_lds_bytes(la, lb, c->ib.nbf) > 65536) continue;
+        ClassPairWork* w = nullptr;
+        for (ClassPairWork& x : c->work)
+            if (x.X == l.X && x.Y == l.Y && x.prec == 0) w = &x;
+        if (!w || w->gc.nwork > 0) continue;
+        std::vector<SegDesc> segs = make_segments(l, cp, cid[l.X], ck[l.Y], cb[l.Y]);
+        const std::vector<SegDesc> segc = make_segments(l, cp, cid[l.X], ck[l.Y], cb[l.Y], false);
+        if (segs.size() + segc.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
+        auto& g = w->gc;
+        g.kind = 1;
+        g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket;
+        g.nseg = (int)segs.size(); g.nsegc = (int)segc.size();
+        segs.insert(segs.end(), segc.begin(), segc.end());
+        g.d_bra = c->d_class_ids[l.X]; g.d_ket = c->d_gket[l.Y]; g.d_bstart = c->d_gbstart[l.Y];
+        g.d_bsoa = c->d_gbsoa[l.Y];
+        g.prim_quartets = l.prim_quartets;
+        if ((rc = dalloc(&g.d_row, l.row.size())) || (rc = dalloc(&g.d_rb, l.rb.size())) ||
+            (rc = dalloc(&g.d_seg, segs.size())))
+            return rc;
+        HIPCHK(hipMemcpy(g.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
+        if (!l.rb.empty()) HIPCHK(hipMemcpy(g.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(g.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+        w->gc_on = 0;
     }
     return 0;
 }
@@ -1581,7 +1637,8 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
             const int ni = ncart_h(w.la) * ncart_h(w.lc);
             const int mm = (w.la == 0 ? 4 : 2) * (w.lc == 0 ? 4 : 2);
             st->class_prim_quartets[i] = w.gc.prim_quartets;
-            st->class_flops[i] = w.gc.prim_quartets * (flop_model(w.la, 0, w.lc, 0) + 2.0 * mm * ni);
+            st->class_flops[i] = w.gc.kind == 1 ? w.gc.prim_quartets * flop_model(w.la, w.lb, w.lc, 0)
+                                                : w.gc.prim_quartets * (flop_model(w.la, 0, w.lc, 0) + 2.0 * mm * ni);
             st->prim_quartets += w.gc.prim_quartets - w.prim_quartets;
             st->model_flops += st->class_flops[i] - w.flops;
         }

@@ -4,7 +4,9 @@ cc-pVDZ's O 1s/2s and H 1s/2s shells share exponents, so the classes
 (ss|ss) (ps|ss) (ps|ps) (ds|ss) run by default over pairs of shell groups:
 one primitive loop per group quartet, each member quartet weighted by its
 own coefficients and digested when it passes the same Schwarz test as in
-the shell-pair lists.  Both forms evaluate the same quartets, so J/K agree
+the shell-pair lists.  (pp|ss) (pp|ps) (ds|ps) (dp|ss) can run with
+shell-pair bras against group-pair kets (on where the autotuner finds it
+faster; switched on here for all).  Both forms evaluate the same quartets, so J/K agree
 to the rounding of the FP64 atomics (1e-12 absolute); against the oracle
 the bar is the usual 1e-10."""
 import numpy as np
@@ -18,6 +20,7 @@ from tests.conftest import basis_path
 pytestmark = pytest.mark.gpu
 
 GC_CLASSES = {"0000", "1000", "1010", "2000"}
+GCK_CLASSES = {"1100", "1110", "2010", "2100"}     # shell-pair bra, group-pair kets
 
 
 def _density(b, seed, rank=20):
@@ -39,12 +42,18 @@ def test_gc_matches_pair_lists(tol):
     assert all(kinds[c] == "gc" for c in GC_CLASSES if c in kinds), kinds
     j1, k1 = ctx.fourindex_eri(p)
     q1 = ctx.stats()["quartets"]
+    ctx.set_general_contraction(True)        # also the shell-pair-bra lists
+    kinds = {c["cls"]: c["kernel"] for c in ctx.stats()["classes"] if c["quartets"]}
+    assert all(kinds[c] == "gc" for c in GC_CLASSES | GCK_CLASSES if c in kinds), kinds
+    j2, k2 = ctx.fourindex_eri(p)
     ctx.set_general_contraction(False)
     kinds = {c["cls"]: c["kernel"] for c in ctx.stats()["classes"] if c["quartets"]}
     assert "gc" not in kinds.values(), kinds
     j0, k0 = ctx.fourindex_eri(p)
-    print(f"max |dJ| {np.max(np.abs(j1 - j0)):.2e} |dK| {np.max(np.abs(k1 - k0)):.2e}", flush=True)
+    print(f"max |dJ| {np.max(np.abs(j1 - j0)):.2e} |dK| {np.max(np.abs(k1 - k0)):.2e}; all lists: "
+          f"{np.max(np.abs(j2 - j0)):.2e} {np.max(np.abs(k2 - k0)):.2e}", flush=True)
     assert np.max(np.abs(j1 - j0)) < 1e-12 and np.max(np.abs(k1 - k0)) < 1e-12
+    assert np.max(np.abs(j2 - j0)) < 1e-12 and np.max(np.abs(k2 - k0)) < 1e-12
     assert ctx.stats()["quartets"] == q1
 
 
@@ -60,6 +69,7 @@ def test_gc_density_screened_matches_pair_lists(dens_tol):
     ctx = pa.Context(b, schwarz_tol=1e-12)
     p = _density(b, 8)
     p[np.abs(p) < 1e-3] *= 1e-6            # a sparse-ish density, so the screen drops quartets
+    ctx.set_general_contraction(True)
     j1, k1 = ctx.fourindex_eri(p, dens_tol=dens_tol)
     kept1 = ctx.stats()["last_quartets"]
     ctx.set_general_contraction(False)
