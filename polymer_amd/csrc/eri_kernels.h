@@ -76,6 +76,7 @@ struct EriArgs {
     const GcMeta* gmeta;
     const GcPrim* gprims;
     double qtol;               // Schwarz threshold of the member quartets
+    int qloop;                 // MODE_PBC J/K, thread kernel: Q3 values per thread (0: one, t = item x n3 + Q3)
 };
 
 // Number of work items of a launch: the device count of a screened list, or
@@ -407,6 +408,151 @@ __device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, in
     }
 }
 
+// Periodic J/K of one (bra, ket) item over Q3 in [s0, s1] in one thread:
+// J(Q1) (and, with both roles, J(Q2)) is the same matrix block for every
+// Q3, so it is summed in registers and added once; the exchange blocks go
+// to a different cell per Q3 and are added per Q3.  Same sums as
+// pbc_digest.
+template <int LA, int LB, int LC, int LD, bool SB, bool SK>
+__device__ __forceinline__ void pbc_loop(const EriArgs& g, int bp, int kp, const PairMeta& mb, const PairMeta& mk,
+                                         int q1, int q2, int s0, int s1) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NI = SB ? NB : NA, NJ = SB ? NA : NB, NK = SK ? ND : NC, NL = SK ? NC : ND;
+    const int fi = SB ? mb.fb : mb.fa, fj = SB ? mb.fa : mb.fb, fk = SK ? mk.fb : mk.fa, fl = SK ? mk.fa : mk.fb;
+    const size_t n = g.nbf, n2 = n * n;
+    const int c1 = g.cel1, cp = g.jk_copies;
+    const size_t off = (size_t)xcd_slot(cp) * (2 * c1 + 1) * n2;
+    double* J = g.Jc + off;
+    double* K = g.Kc + off;
+    double dkl[NK * NL], jij[NI * NJ], dij[NI * NJ], jkl[NK * NL];
+    {
+        const double* D = g.Dc + (size_t)(q2 + c1) * n2;
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) dkl[k * NL + l] = D[(size_t)(fk + k) * n + fl + l];
+        const double* E = g.Dc + (size_t)(q1 + c1) * n2;
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                dij[i * NJ + j] = g.both ? E[(size_t)(fi + i) * n + fj + j] : 0.0;
+                jij[i * NJ + j] = 0.0;
+            }
+#pragma unroll
+        for (int i = 0; i < NK * NL; ++i) jkl[i] = 0.0;
+    }
+    const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
+    const double CDx = g.ab[4 * kp], CDy = g.ab[4 * kp + 1], CDz = g.ab[4 * kp + 2];
+    for (int s = s0; s <= s1; ++s) {
+        double v[NA * NB * NC * ND];
+        StoreCb<NB, NC, ND, double> st{v};
+        const ShiftKet kv{g.prims + mk.prim_off, s * g.period};
+        quartet<double, LA, LB, LC, LD>(g.prims + mb.prim_off, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz,
+                                        g.boysA, st);
+        auto V = [&](int i, int j, int k, int l) -> double {
+            const int a = SB ? j : i, b = SB ? i : j, c = SK ? l : k, d = SK ? k : l;
+            return v[((a * NB + b) * NC + c) * ND + d];
+        };
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                double x = 0.0;
+#pragma unroll
+                for (int k = 0; k < NK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) x += V(i, j, k, l) * dkl[k * NL + l];
+                jij[i * NJ + j] += x;
+            }
+        const int qk = s + q2, qd = q1 - s;
+        if (qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1) {
+            const double* D = g.Dc + (size_t)(qd + c1) * n2;
+            double dkj[NK * NJ];
+#pragma unroll
+            for (int k = 0; k < NK; ++k)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) dkj[k * NJ + j] = D[(size_t)(fk + k) * n + fj + j];
+            double* Kq = K + (size_t)(qk + c1) * n2;
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    double x = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) x += V(i, j, k, l) * dkj[k * NJ + j];
+                    jk_add(&Kq[(size_t)(fi + i) * n + fl + l], x, cp);
+                }
+        }
+        if (!g.both) continue;
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) x += V(i, j, k, l) * dij[i * NJ + j];
+                jkl[k * NL + l] += x;
+            }
+        const int qk2 = q1 - s, qd2 = q2 + s;
+        if (qk2 >= -c1 && qk2 <= c1 && qd2 >= -c1 && qd2 <= c1) {
+            const double* D = g.Dc + (size_t)(qd2 + c1) * n2;
+            double dil[NI * NL];
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+#pragma unroll
+                for (int l = 0; l < NL; ++l) dil[i * NL + l] = D[(size_t)(fi + i) * n + fl + l];
+            double* Kq = K + (size_t)(qk2 + c1) * n2;
+#pragma unroll
+            for (int k = 0; k < NK; ++k)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    double x = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NI; ++i)
+#pragma unroll
+                        for (int l = 0; l < NL; ++l) x += V(i, j, k, l) * dil[i * NL + l];
+                    jk_add(&Kq[(size_t)(fk + k) * n + fj + j], x, cp);
+                }
+        }
+    }
+    double* J1 = J + (size_t)(q1 + c1) * n2;
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) jk_add(&J1[(size_t)(fi + i) * n + fj + j], jij[i * NJ + j], cp);
+    if (!g.both) return;
+    double* J2 = J + (size_t)(q2 + c1) * n2;
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+#pragma unroll
+        for (int l = 0; l < NL; ++l) jk_add(&J2[(size_t)(fk + k) * n + fl + l], jkl[k * NL + l], cp);
+}
+
+// Work index t = item x nchunk + chunk, chunk of g.qloop Q3 values.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void pbc_item_loop(const EriArgs& g, long long t) {
+    const int nchunk = (g.n3 + g.qloop - 1) / g.qloop;
+    const long long u = t / nchunk;
+    const int chunk = (int)(t - u * nchunk);
+    const int s0 = -g.cel2 + chunk * g.qloop, s1 = min(s0 + g.qloop - 1, g.cel2);
+    int bp, pos, b;
+    decode_work(g, u, bp, pos, b);
+    const int kp = g.ket_ids[pos];
+    const PairMeta mb = g.meta[bp], mk = g.meta[kp];
+    const int2 cb = g.pcell[bp], ck = g.pcell[kp];
+    switch (cb.y * 2 + ck.y) {
+    case 0: pbc_loop<LA, LB, LC, LD, false, false>(g, bp, kp, mb, mk, cb.x, ck.x, s0, s1); break;
+    case 1: pbc_loop<LA, LB, LC, LD, false, true>(g, bp, kp, mb, mk, cb.x, ck.x, s0, s1); break;
+    case 2: pbc_loop<LA, LB, LC, LD, true, false>(g, bp, kp, mb, mk, cb.x, ck.x, s0, s1); break;
+    default: pbc_loop<LA, LB, LC, LD, true, true>(g, bp, kp, mb, mk, cb.x, ck.x, s0, s1); break;
+    }
+}
+
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
     const long long u = t / g.n3;
@@ -445,7 +591,8 @@ __device__ __forceinline__ void pbc_item(const EriArgs& g, long long t) {
 template <int LA, int LB, int LC, int LD, int MODE>
 __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
     if (MODE == MODE_PBC) {
-        pbc_item<LA, LB, LC, LD>(g, t);
+        if (g.qloop) pbc_item_loop<LA, LB, LC, LD>(g, t);
+        else pbc_item<LA, LB, LC, LD>(g, t);
         return;
     }
     int bp, kp, pos = 0, b = 0;

@@ -142,13 +142,83 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             }
             for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
             if (lane == 0) g.out[bp] = sqrt(m);
-        } else if (MODE == MODE_PBC) {
+        } else if (MODE == MODE_PBC && g.out) {   // lattice tensor: one store per integral
             const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * g.cel1 + 1) * n * n;
             const int2 cb = g.pcell[bp], ck = g.pcell[kp];
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 pbc_integral(g, g.Jc + off, g.Kc + off, cb, ck, s3, fa, fb, fc, fd, (f >> 16) & 15, (f >> 20) & 15,
                              (f >> 24) & 15, f >> 28, (double)sl[f & 0xffff]);
+            }
+        } else if (MODE == MODE_PBC) {
+            // Gather digestion of the lattice sums (the blocks of pbc_digest):
+            // integrals in LDS in the reference's (i j|k l) roles (i in cell 0,
+            // j in cell Q1, k in cell Q3, l in cell Q3+Q2; a stored pair with
+            // its swap bit set has its shells the other way round), the
+            // density blocks in LDS, then one atomic per output element.
+            const int2 cb = g.pcell[bp], ck = g.pcell[kp];
+            const int q1 = cb.x, q2 = ck.x, c1 = g.cel1;
+            const int ni = cb.y ? nb : na, nj = cb.y ? na : nb, nk = ck.y ? nd : nc, nl = ck.y ? nc : nd;
+            const int fi = cb.y ? fb : fa, fj = cb.y ? fa : fb, fk = ck.y ? fd : fc, fl = ck.y ? fc : fd;
+            const int qk = s3 + q2, qd = q1 - s3, qk2 = q1 - s3, qd2 = q2 + s3;
+            const bool kin = qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1;
+            const bool kin2 = g.both && qk2 >= -c1 && qk2 <= c1 && qd2 >= -c1 && qd2 <= c1;
+            const size_t n2 = (size_t)n * n;
+            real* dv = sl + d.nslot;
+            real* D1 = dv + d.nfin;          // D(Q2)[k][l]       nk x nl
+            real* D2 = D1 + nk * nl;         // D(Q1-Q3)[k][j]    nk x nj
+            real* D3 = D2 + nk * nj;         // D(Q1)[i][j]       ni x nj
+            real* D4 = D3 + ni * nj;         // D(Q2+Q3)[i][l]    ni x nl
+            for (int k = lane; k < d.nfin; k += 64) {
+                const uint32_t f = g_prog_aux[d.fin_off + k];
+                const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
+                const int i = cb.y ? b : a, j = cb.y ? a : b, kk = ck.y ? e : c, l = ck.y ? c : e;
+                dv[((i * nj + j) * nk + kk) * nl + l] = sl[f & 0xffff];
+            }
+            const double* Dq2 = g.Dc + (size_t)(q2 + c1) * n2;
+            for (int k = lane; k < nk * nl; k += 64) D1[k] = Dq2[(size_t)(fk + k / nl) * n + fl + k % nl];
+            if (kin) {
+                const double* Dqd = g.Dc + (size_t)(qd + c1) * n2;
+                for (int k = lane; k < nk * nj; k += 64) D2[k] = Dqd[(size_t)(fk + k / nj) * n + fj + k % nj];
+            }
+            if (g.both) {
+                const double* Dq1 = g.Dc + (size_t)(q1 + c1) * n2;
+                for (int k = lane; k < ni * nj; k += 64) D3[k] = Dq1[(size_t)(fi + k / nj) * n + fj + k % nj];
+            }
+            if (kin2) {
+                const double* Dqd2 = g.Dc + (size_t)(qd2 + c1) * n2;
+                for (int k = lane; k < ni * nl; k += 64) D4[k] = Dqd2[(size_t)(fi + k / nl) * n + fl + k % nl];
+            }
+            __syncthreads();
+            const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * c1 + 1) * n2;
+            double* J = g.Jc + off;
+            double* K = g.Kc + off;
+            const int o1 = ni * nj, o2 = o1 + (kin ? ni * nl : 0), o3 = o2 + (g.both ? nk * nl : 0),
+                      o4 = o3 + (kin2 ? nk * nj : 0);
+            for (int o = lane; o < o4; o += 64) {
+                double acc = 0.0;
+                double* dst;
+                if (o < o1) {                          // J(Q1)[i][j] += v D(Q2)[k][l]
+                    const int i = o / nj, j = o % nj;
+                    const real* v = dv + (i * nj + j) * nk * nl;
+                    for (int k = 0; k < nk * nl; ++k) acc += v[k] * D1[k];
+                    dst = &J[(size_t)(q1 + c1) * n2 + (size_t)(fi + i) * n + fj + j];
+                } else if (o < o2) {                   // K(Q3+Q2)[i][l] += v D(Q1-Q3)[k][j]
+                    const int i = (o - o1) / nl, l = (o - o1) % nl;
+                    for (int j = 0; j < nj; ++j)
+                        for (int k = 0; k < nk; ++k) acc += dv[((i * nj + j) * nk + k) * nl + l] * D2[k * nj + j];
+                    dst = &K[(size_t)(qk + c1) * n2 + (size_t)(fi + i) * n + fl + l];
+                } else if (o < o3) {                   // J(Q2)[k][l] += v D(Q1)[i][j]
+                    const int k0 = o - o2;
+                    for (int ij = 0; ij < ni * nj; ++ij) acc += dv[ij * nk * nl + k0] * D3[ij];
+                    dst = &J[(size_t)(q2 + c1) * n2 + (size_t)(fk + k0 / nl) * n + fl + k0 % nl];
+                } else {                               // K(Q1-Q3)[k][j] += v D(Q2+Q3)[i][l]
+                    const int k = (o - o3) / nj, j = (o - o3) % nj;
+                    for (int i = 0; i < ni; ++i)
+                        for (int l = 0; l < nl; ++l) acc += dv[((i * nj + j) * nk + k) * nl + l] * D4[i * nl + l];
+                    dst = &K[(size_t)(qk2 + c1) * n2 + (size_t)(fk + k) * n + fj + j];
+                }
+                jk_add(dst, acc, g.jk_copies);
             }
         } else if (MODE == MODE_TENSOR) {
             TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};

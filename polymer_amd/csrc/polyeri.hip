@@ -616,7 +616,15 @@ static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long n
             return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
         return 0;
     }
-    hipLaunchKernelGGL(k, dim3(nblk((size_t)nwork)), dim3(256), 0, c->stream, a);
+    if (!tensor) {
+        // J/K: each thread loops over a chunk of Q3 values and adds its J
+        // block once; (ss|ss), whose one-integral digestion gains little
+        // and whose primitive loops are long, keeps more and shorter chunks
+        const int nint = ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld);
+        a.qloop = nint == 1 ? 3 : a.n3;
+        a.nwork = nwork / a.n3 * ((a.n3 + a.qloop - 1) / a.qloop);
+    }
+    hipLaunchKernelGGL(k, dim3(nblk((size_t)a.nwork)), dim3(256), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return 0;
 }
