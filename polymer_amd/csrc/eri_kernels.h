@@ -1179,10 +1179,13 @@ int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam
                         long sbk, long sbn, long sB, double* C, long scm, long scn, long sC, int batch,
                         hipStream_t stream);
 // Planar complex products C_b = A_b B_b (A real when Ai is null), same
-// strides for the real and imaginary planes (kgemm.hip).
+// strides for the real and imaginary planes (kgemm.hip).  nci > 0 splits
+// the columns of B and C as (n / nci, n % nci) with outer strides sbo / sco
+// (and inner strides sbn / scn).
 int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const double* Ai, long sam, long sak,
                          long sA, const double* Br, const double* Bi, long sbk, long sbn, long sB, double* Cr,
-                         double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream);
+                         double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream, int nci = 0,
+                         long sbo = 0, long sco = 0);
 // Density screening (kscreen.hip): shell-block maxima of |D|, and the
 // compaction of one work list to the items whose bound
 // Q_ab Q_cd max(|D| over the six blocks the quartet reads) reaches tol.

@@ -124,6 +124,9 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmArgs g) {
 //   C = Cr + i Ci, all planes sharing one set of strides.
 // Real and imaginary parts accumulate in two MFMA accumulators:
 //   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br.
+// Column n of B and C is (n / nci, n % nci), at offset
+// (n / nci) sbo + (n % nci) sbn in B and (n / nci) sco + (n % nci) scn in C,
+// so that several products sharing A run as one (nci = N: plain columns).
 struct ZGemmArgs {
     int M, N, K;
     const double *Ar, *Ai;
@@ -132,6 +135,8 @@ struct ZGemmArgs {
     long sbk, sbn, sB;
     double *Cr, *Ci;
     long scm, scn, sC;
+    int nci;
+    long sbo, sco;
 };
 
 // Each wavefront computes NT column tiles of one 16-row strip, so the A
@@ -149,10 +154,13 @@ __global__ void __launch_bounds__(256) k_zgemm_f64(ZGemmArgs g) {
     const int r = lane & 15, q = lane >> 4;
     const int m = tm * 16 + r;
     doublex4 cr[NT], ci[NT];
+    long bcol[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         cr[t] = doublex4{0.0, 0.0, 0.0, 0.0};
         ci[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+        const int n = (tg * NT + t) * 16 + r;
+        bcol[t] = (long)(n / g.nci) * g.sbo + (long)(n % g.nci) * g.sbn;
     }
     for (int k0 = 0; k0 < g.K; k0 += 4) {
         const int k = k0 + q;
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(256) k_zgemm_f64(ZGemmArgs g) {
         for (int t = 0; t < NT; ++t) {
             const int n = (tg * NT + t) * 16 + r;
             const bool vb = n < g.N && k < g.K;
-            const long ib = ob + k * g.sbk + n * g.sbn;
+            const long ib = ob + k * g.sbk + bcol[t];
             const double br = vb ? g.Br[ib] : 0.0, bi = vb ? g.Bi[ib] : 0.0;
             cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr[t], 0, 0, 0);
             ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci[t], 0, 0, 0);
@@ -181,7 +189,7 @@ __global__ void __launch_bounds__(256) k_zgemm_f64(ZGemmArgs g) {
         for (int j = 0; j < 4; ++j) {
             const int row = tm * 16 + q + 4 * j, col = (tg * NT + t) * 16 + r;
             if (row < g.M && col < g.N) {
-                const long ic = oc + row * g.scm + col * g.scn;
+                const long ic = oc + row * g.scm + (long)(col / g.nci) * g.sco + (long)(col % g.nci) * g.scn;
                 g.Cr[ic] = cr[t][j];
                 g.Ci[ic] = ci[t][j];
             }
@@ -212,10 +220,13 @@ __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
     const int r = lane & 15, q = lane >> 4;
     const int m = tm * 16 + r;
     floatx4 cr[NT], ci[NT];
+    long bcol[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         cr[t] = floatx4{0.f, 0.f, 0.f, 0.f};
         ci[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const int n = (tg * NT + t) * 16 + r;
+        bcol[t] = (long)(n / g.nci) * g.sbo + (long)(n % g.nci) * g.sbn;
     }
     for (int k0 = 0; k0 < g.K; k0 += 16) {
         bf16x4 ar[3], an[3], ai[3];
@@ -241,7 +252,7 @@ __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
             for (int j = 0; j < 4; ++j) {
                 const int k = k0 + 4 * q + j;
                 const bool vb = n < g.N && k < g.K;
-                const long ib = ob + k * g.sbk + n * g.sbn;
+                const long ib = ob + k * g.sbk + bcol[t];
                 short x0, x1, x2;
                 split3(vb ? g.Br[ib] : 0.0, x0, x1, x2);
                 br[0][j] = x0; br[1][j] = x1; br[2][j] = x2;
@@ -263,7 +274,7 @@ __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
         for (int j = 0; j < 4; ++j) {
             const int row = tm * 16 + 4 * q + j, col = (tg * NT + t) * 16 + r;
             if (row < g.M && col < g.N) {
-                const long ic = oc + row * g.scm + col * g.scn;
+                const long ic = oc + row * g.scm + (long)(col / g.nci) * g.sco + (long)(col % g.nci) * g.scn;
                 g.Cr[ic] = (double)cr[t][j];
                 g.Ci[ic] = (double)ci[t][j];
             }
@@ -274,8 +285,10 @@ __global__ void __launch_bounds__(256) k_zgemm_bf16x3(ZGemmArgs g) {
 
 int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const double* Ai, long sam, long sak,
                          long sA, const double* Br, const double* Bi, long sbk, long sbn, long sB, double* Cr,
-                         double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream) {
+                         double* Ci, long scm, long scn, long sC, int batch, hipStream_t stream, int nci, long sbo,
+                         long sco) {
     if (M < 1 || N < 1 || K < 1 || batch < 1 || !Ar || !Br || !Bi || !Cr || !Ci) return -1;
+    if (nci <= 0) nci = N;
     if (mode != GEMM_F64 && mode != GEMM_BF16X3) return -1;
     // column tiles per wavefront: enough for N (up to 4)
     const int ntile = (N + 15) / 16;
@@ -285,7 +298,7 @@ int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const 
     for (int b0 = 0; b0 < batch; b0 += 65535) {       // grid.y limit
         const int nb = batch - b0 < 65535 ? batch - b0 : 65535;
         ZGemmArgs g{M, N, K, Ar + b0 * sA, Ai ? Ai + b0 * sA : nullptr, sam, sak, sA, Br + b0 * sB, Bi + b0 * sB,
-                    sbk, sbn, sB, Cr + b0 * sC, Ci + b0 * sC, scm, scn, sC};
+                    sbk, sbn, sB, Cr + b0 * sC, Ci + b0 * sC, scm, scn, sC, nci, sbo, sco};
         const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
         if (mode == GEMM_F64) {
             if (nt == 1) hipLaunchKernelGGL(k_zgemm_f64<1>, grid, dim3(256), 0, stream, g);

@@ -2675,9 +2675,9 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
     HIPCHK(hipGetLastError());
     auto gemm = [&](int M, int Nn, int K, const double* Ar, const double* Ai, long sam, long sak, long sA,
                     const double* Br, const double* Bi, long sbk, long sbn, long sB, double* Cr, double* Ci, long scm,
-                    long scn, long sC, int batch) {
+                    long scn, long sC, int batch, int nci = 0, long sbo = 0, long sco = 0) {
         return polyeri_zgemm_launch(mode, M, Nn, K, Ar, Ai, sam, sak, sA, Br, Bi, sbk, sbn, sB, Cr, Ci, scm, scn, sC,
-                                    batch, s);
+                                    batch, s, nci, sbo, sco);
     };
     for (int kb = 0; kb < nk; ++kb) {
         const int nb = plim[kb] - iocc;
@@ -2695,18 +2695,16 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
         if (gemm((int)M1, nb, (int)(n1 * N), dT, nullptr, n1 * N, 1, 0, dB1r, dB1i, NBk, 1, 0, dTNr, dTNi, 1, M1, 0,
                  1))
             return fail(POLYERI_E_HIP, "k-point MP2: first quarter launch failed");
-        // Q2 quarter, per kj: TOK[kj][b][jo][(i,Q1,j)] = sum_{Q3,k} TN B2[kj]
-        for (int kj = 0; kj < nk; ++kj)
-            if (gemm((int)M2, no, (int)(n3 * N), dTNr, dTNi, n3 * N, 1, M1, dB2r + kj * n3 * N * NO,
-                     dB2i + kj * n3 * N * NO, NO, 1, 0, dTOKr + kj * NBk * NO * M2, dTOKi + kj * NBk * NO * M2, 1,
-                     M2, NO * M2, nb))
-                return fail(POLYERI_E_HIP, "k-point MP2: second quarter launch failed");
-        // Q3 quarter, per ka: TV[ka][(kj,b,jo)][a][i] = sum_{Q1,j} TOK B3[ka]
-        for (int ka = 0; ka < nk; ++ka)
-            if (gemm((int)N, nvmax, (int)(n1 * N), dTOKr, dTOKi, n1 * N, 1, M2, dB3r + ka * n1 * N * NV,
-                     dB3i + ka * n1 * N * NV, NV, 1, 0, dTVr + ka * NK * NBk * NO * NV * N,
-                     dTVi + ka * NK * NBk * NO * NV * N, 1, N, NV * N, nk * nb * no))
-                return fail(POLYERI_E_HIP, "k-point MP2: third quarter launch failed");
+        // Q2 quarter, every kj in one product (columns (kj, jo)), so TN is
+        // read once: TOK[kj][b][jo][(i,Q1,j)] = sum_{Q3,k} TN B2[kj]
+        if (gemm((int)M2, nk * no, (int)(n3 * N), dTNr, dTNi, n3 * N, 1, M1, dB2r, dB2i, NO, 1, 0, dTOKr, dTOKi, 1,
+                 M2, NO * M2, nb, no, n3 * N * NO, NBk * NO * M2))
+            return fail(POLYERI_E_HIP, "k-point MP2: second quarter launch failed");
+        // Q3 quarter, every ka in one product (columns (ka, a)):
+        // TV[ka][(kj,b,jo)][a][i] = sum_{Q1,j} TOK B3[ka]
+        if (gemm((int)N, nk * nvmax, (int)(n1 * N), dTOKr, dTOKi, n1 * N, 1, M2, dB3r, dB3i, NV, 1, 0, dTVr, dTVi, 1,
+                 N, NV * N, nk * nb * no, nvmax, n1 * N * NV, NK * NBk * NO * NV * N))
+            return fail(POLYERI_E_HIP, "k-point MP2: third quarter launch failed");
         // Q4 quarter, per kj, batched over ka with ki = ka + (kb - kj mod nk):
         // TOV[kj][ka][(b,jo,a)][io] = sum_i TV[ka][kj] conj(CO(i,io,ki))
         for (int kj = 0; kj < nk; ++kj) {
