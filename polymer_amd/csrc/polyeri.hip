@@ -647,34 +647,34 @@ static hipStream_t shared_stream(int device, int* rc) {
 
 static void free_ctx(polyeri_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->d_prims); hipFree(c->d_meta); hipFree(c->d_ab); hipFree(c->d_boys); hipFree(c->d_Q);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_prims); (void)hipFree(c->d_meta); (void)hipFree(c->d_ab); (void)hipFree(c->d_boys); (void)hipFree(c->d_Q);
     for (int i = 0; i < NPAIRCLASS; ++i) {
-        hipFree(c->d_class_ids[i]); hipFree(c->d_ket_ids[i]); hipFree(c->d_bstart[i]); hipFree(c->d_bsoa[i]);
+        (void)hipFree(c->d_class_ids[i]); (void)hipFree(c->d_ket_ids[i]); (void)hipFree(c->d_bstart[i]); (void)hipFree(c->d_bsoa[i]);
     }
-    hipFree(c->d_ksoa);
+    (void)hipFree(c->d_ksoa);
     for (auto& w : c->work) {
-        hipFree(w.d_row); hipFree(w.d_rb);
-        if (w.d_rf) hipFree(w.d_rf);
-        if (w.d_items) hipFree(w.d_items);
-        if (w.d_seg) hipFree(w.d_seg);
-        if (w.gc.d_row) hipFree(w.gc.d_row);
-        if (w.gc.d_rb) hipFree(w.gc.d_rb);
-        if (w.gc.d_seg) hipFree(w.gc.d_seg);
+        (void)hipFree(w.d_row); (void)hipFree(w.d_rb);
+        if (w.d_rf) (void)hipFree(w.d_rf);
+        if (w.d_items) (void)hipFree(w.d_items);
+        if (w.d_seg) (void)hipFree(w.d_seg);
+        if (w.gc.d_row) (void)hipFree(w.gc.d_row);
+        if (w.gc.d_rb) (void)hipFree(w.gc.d_rb);
+        if (w.gc.d_seg) (void)hipFree(w.gc.d_seg);
     }
-    hipFree(c->d_gmeta); hipFree(c->d_gprims); hipFree(c->d_gksoa);
+    (void)hipFree(c->d_gmeta); (void)hipFree(c->d_gprims); (void)hipFree(c->d_gksoa);
     for (int i = 0; i < NPAIRCLASS; ++i) {
-        hipFree(c->d_gbra[i]); hipFree(c->d_gket[i]); hipFree(c->d_gbstart[i]); hipFree(c->d_gbsoa[i]);
+        (void)hipFree(c->d_gbra[i]); (void)hipFree(c->d_gket[i]); (void)hipFree(c->d_gbstart[i]); (void)hipFree(c->d_gbsoa[i]);
     }
-    hipFree(c->d_pcell); hipFree(c->d_Dc); hipFree(c->d_Jc); hipFree(c->d_Kc);
-    hipFree(c->d_pcc); hipFree(c->d_jcc); hipFree(c->d_kcc);
-    hipFree(c->d_psh); hipFree(c->d_sh_f0); hipFree(c->d_sh_nf); hipFree(c->d_dsh); hipFree(c->d_counts);
-    hipFree(c->d_ref); hipFree(c->d_scale); hipFree(c->d_D); hipFree(c->d_J); hipFree(c->d_K);
-    hipFree(c->d_pc); hipFree(c->d_jc); hipFree(c->d_kc); hipFree(c->d_T); hipFree(c->d_V);
-    for (auto e : c->ev) hipEventDestroy(e);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
+    (void)hipFree(c->d_pcell); (void)hipFree(c->d_Dc); (void)hipFree(c->d_Jc); (void)hipFree(c->d_Kc);
+    (void)hipFree(c->d_pcc); (void)hipFree(c->d_jcc); (void)hipFree(c->d_kcc);
+    (void)hipFree(c->d_psh); (void)hipFree(c->d_sh_f0); (void)hipFree(c->d_sh_nf); (void)hipFree(c->d_dsh); (void)hipFree(c->d_counts);
+    (void)hipFree(c->d_ref); (void)hipFree(c->d_scale); (void)hipFree(c->d_D); (void)hipFree(c->d_J); (void)hipFree(c->d_K);
+    (void)hipFree(c->d_pc); (void)hipFree(c->d_jc); (void)hipFree(c->d_kc); (void)hipFree(c->d_T); (void)hipFree(c->d_V);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->blas) rocblas_destroy_handle(c->blas);
     drop_graphs(c);
     delete c;
@@ -1600,7 +1600,7 @@ int polyeri_eri_tensor(polyeri_ctx* ctx, double* feri) {
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = fail(POLYERI_E_HIP, hipGetErrorString(e));
     }
-    hipFree(d_t);
+    (void)hipFree(d_t);
     if (rc) return rc;
     std::fill(feri, feri + nc * nc * nc * nc, 0.0);
     const auto& ref = ctx->ib.ref_of_int;
@@ -1741,7 +1741,7 @@ static int form_c2s(const RefBasis& rb, int spherical, std::vector<double>& c2s)
 
 struct DevBuf {
     std::vector<void*> ptrs;
-    ~DevBuf() { for (void* p : ptrs) hipFree(p); }
+    ~DevBuf() { for (void* p : ptrs) (void)hipFree(p); }
     template <typename T>
     int get(T** p, size_t n) {
         int rc = dalloc(p, n);
@@ -1982,8 +1982,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         if (trace) { trace[2 * (iscf - 1)] = dchg; trace[2 * (iscf - 1) + 1] = etot; }
         if (dchg < o->scfconv) { converged = 1; break; }
     }
-    hipEventDestroy(ef0);
-    hipEventDestroy(ef1);
+    (void)hipEventDestroy(ef0);
+    (void)hipEventDestroy(ef1);
     if (rc) return rc;
     if (iscf > o->scfcycles) iscf = o->scfcycles;
     HIPCHK(hipMemcpyAsync(p_c, dP, n2 * 8, hipMemcpyDeviceToHost, s));
@@ -2755,7 +2755,7 @@ int polyeri_eri_tensor_cells(polyeri_ctx* ctx, double* t_c) {
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = fail(POLYERI_E_HIP, hipGetErrorString(e));
     }
-    hipFree(d_t);
+    (void)hipFree(d_t);
     if (rc) return rc;
     std::fill(t_c, t_c + nc4 * n1 * n1 * n3, 0.0);
     const auto& ref = ctx->ib.ref_of_int;
