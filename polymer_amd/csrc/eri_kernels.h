@@ -1193,10 +1193,14 @@ int polyeri_shell_dmax(const double* D, int nbf, const int* shell_f0, const int*
                        double* dsh, hipStream_t stream);
 int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shells, const double* dsh,
                         int nsh, double tol, int2* items, unsigned* count, hipStream_t stream);
-// Cooperative (one wavefront per quartet, LDS-resident recursion) kernels.
-int polyeri_coop_nslot(int la, int lb, int lc, int ld);
-int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
+// Cooperative (one wavefront per quartet, LDS-resident recursion) kernels;
+// primitive quartets are prepared in batches of 2^pbs (pbs 0..3).
+int polyeri_coop_nslot(int la, int lb, int lc, int ld, int pbs);
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int pbs, int max_grid,
                         hipStream_t stream);
+// default batch: eight for the classes whose LDS is small enough that the
+// records cost no occupancy, one otherwise
+inline int polyeri_coop_default_pbs(int nint) { return nint < 200 ? 3 : 0; }
 // Generally contracted segment kernel (kgc.hip): LDS bytes, whether the
 // class (la s|lc s) has one, and the launch over a.segs (grid blocks).
 size_t polyeri_gc_lds_bytes(int la, int nbf);

@@ -8,6 +8,10 @@
 #include "eri_kernels.h"
 #include "eri_programs.h"
 
+// LDS record of one prepared primitive quartet: 40 VRR coefficients, then
+// F_0 .. F_L
+constexpr int COOP_PREC = 40 + 4 * POLYERI_LMAX + 1;
+
 __device__ __forceinline__ double boys_taylor(int m, double T, const double* __restrict__ tab) {
     if (T < BOYS_TMAX) {
         const int ts = (int)(T * 20.0 + 0.5);
@@ -28,7 +32,7 @@ __device__ __forceinline__ double boys_taylor(int m, double T, const double* __r
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd) {
+__global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
     // node values in the recursion's precision (FP32 for MODE_JK32)
     using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
     extern __shared__ double sl_raw[];
@@ -55,41 +59,56 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             kp = g.ket_ids[pos];
         }
         const PairMeta mb = g.meta[bp], mk = g.meta[kp];
-        // per-primitive VRR coefficients, indexed by the ops' (side, direction)
-        real* cf = sl + d.nslot + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
+        // Primitive quartets are prepared 2^pbs at a time, 64 >> pbs lanes
+        // each (the pair data and Boys lookups of a batch in flight together):
+        // record r holds the VRR coefficients, indexed by the ops' (side,
+        // direction) -- [3 typ + i]: P - A (bra) / Q - C (ket); [6 + 3 typ + i]:
+        // W - P / W - Q; [14 + typ]: rho/zeta, rho/eta; [16 + 8 typ + n]:
+        // n/2zeta, n/2eta; [32 + n]: n/2(zeta+eta) -- then F_0 .. F_L times the
+        // prefactor.  Read from LDS so that the op loop has no select of the
+        // two reciprocals (which the compiler turns into a division per op)
+        // and no integer-to-double conversions.
+        real* pbuf = sl + d.nslot + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
         for (int k = lane; k <= d.nacc; k += 64) sl[k] = real(0);
-        for (int p = 0; p < mb.nprim; ++p) {
-            const PrimPair bq = g.prims[mb.prim_off + p];
-            for (int q = 0; q < mk.nprim; ++q) {
-                PrimPair kq = g.prims[mk.prim_off + q];
-                if (MODE == MODE_PBC) kq.Px += s3 * g.period;
-                const double ze = bq.zeta + kq.zeta, ize = 1.0 / ze;
-                const double rho = bq.zeta * kq.zeta * ize;
-                const double PQx = bq.Px - kq.Px, PQy = bq.Py - kq.Py, PQz = bq.Pz - kq.Pz;
-                const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-                const double pref = bq.K * kq.K * sqrt(ize);
-                const double roz = kq.zeta * ize, roe = bq.zeta * ize;
-                // cf[3 typ + i]: P - A (bra) / Q - C (ket); cf[6 + 3 typ + i]:
-                // W - P / W - Q; cf[14 + typ]: rho/zeta, rho/eta;
-                // cf[16 + 8 typ + n]: n/2zeta, n/2eta; cf[32 + n]: n/2(zeta+eta).
-                // Read from LDS so that the op loop has no select of the two
-                // reciprocals (which the compiler turns into a division per
-                // op) and no integer-to-double conversions.
-                if (lane < 40) {
-                    const int typ = lane >= 16 ? (lane >> 3) & 1 : lane >= 12 ? lane & 1 : (lane % 6) / 3;
-                    const int i = lane % 3;
-                    const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
-                    double v;
-                    if (lane < 6) v = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
-                                          : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
-                    else if (lane < 12) v = typ ? roe * pq : -roz * pq;
-                    else if (lane < 14) v = 0.0;
-                    else if (lane < 16) v = typ ? roe : roz;
-                    else if (lane < 32) v = (lane & 7) * 0.5 / (typ ? kq.zeta : bq.zeta);
-                    else v = (lane - 32) * 0.5 * ize;
-                    cf[lane] = (real)v;
+        const int npq = mb.nprim * mk.nprim;
+        const int pb = 1 << pbs, lpr = 64 >> pbs;
+        for (int p0 = 0; p0 < npq; p0 += pb) {
+            const int nbt = npq - p0 < pb ? npq - p0 : pb;
+            {
+                const int j = lane >> (6 - pbs), part = lane & (lpr - 1);
+                if (j < nbt) {
+                    const int pp = p0 + j;
+                    const PrimPair bq = g.prims[mb.prim_off + pp / mk.nprim];
+                    PrimPair kq = g.prims[mk.prim_off + pp % mk.nprim];
+                    if (MODE == MODE_PBC) kq.Px += s3 * g.period;
+                    const double ze = bq.zeta + kq.zeta, ize = 1.0 / ze;
+                    const double rho = bq.zeta * kq.zeta * ize;
+                    const double PQx = bq.Px - kq.Px, PQy = bq.Py - kq.Py, PQz = bq.Pz - kq.Pz;
+                    const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+                    const double pref = bq.K * kq.K * sqrt(ize);
+                    const double roz = kq.zeta * ize, roe = bq.zeta * ize;
+                    real* rec = pbuf + j * COOP_PREC;
+                    for (int e = part; e < 40; e += lpr) {
+                        const int typ = e >= 16 ? (e >> 3) & 1 : e >= 12 ? e & 1 : (e % 6) / 3;
+                        const int i = e % 3;
+                        const double pq = i == 0 ? PQx : (i == 1 ? PQy : PQz);
+                        double v;
+                        if (e < 6) v = typ ? (i == 0 ? kq.PAx : (i == 1 ? kq.PAy : kq.PAz))
+                                           : (i == 0 ? bq.PAx : (i == 1 ? bq.PAy : bq.PAz));
+                        else if (e < 12) v = typ ? roe * pq : -roz * pq;
+                        else if (e < 14) v = 0.0;
+                        else if (e < 16) v = typ ? roe : roz;
+                        else if (e < 32) v = (e & 7) * 0.5 / (typ ? kq.zeta : bq.zeta);
+                        else v = (e - 32) * 0.5 * ize;
+                        rec[e] = (real)v;
+                    }
+                    for (int m = part; m <= d.L; m += lpr) rec[40 + m] = (real)(pref * boys_taylor(m, T, g.boys));
                 }
-                if (lane <= d.L) sl[d.base + lane] = (real)(pref * boys_taylor(lane, T, g.boys));
+            }
+            __syncthreads();
+            for (int q = 0; q < nbt; ++q) {
+                const real* cf = pbuf + q * COOP_PREC;
+                if (lane <= d.L) sl[d.base + lane] = cf[40 + lane];
                 __syncthreads();
                 // op words hold LDS byte offsets of 8-byte slots (halved for
                 // FP32 slots); a target node is also added to its accumulator
@@ -297,31 +316,33 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
 }
 
 // LDS doubles one wavefront of the class needs: the program's node slots,
-// the dense integral block and the six density blocks (J/K), and the 40
-// VRR coefficients of the current primitive quartet.
-int polyeri_coop_nslot(int la, int lb, int lc, int ld) {
+// the dense integral block and the six density blocks (J/K), and the
+// records of one batch of 2^pbs primitive quartets.
+int polyeri_coop_nslot(int la, int lb, int lc, int ld, int pbs) {
     const int p = prog_index(la, lb, lc, ld);
     if (p < 0) return -1;
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + 40;
+    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + (COOP_PREC << pbs);
 }
 
-int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int nslot, int max_grid,
+int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int pbs, int max_grid,
                         hipStream_t stream) {
     const int p = prog_index(la, lb, lc, ld);
-    if (p < 0 || a.nwork <= 0 || max_grid <= 0) return p < 0 || max_grid <= 0 ? -1 : 0;
+    if (p < 0 || pbs < 0 || pbs > 3 || a.nwork <= 0 || max_grid <= 0)
+        return p < 0 || pbs < 0 || pbs > 3 || max_grid <= 0 ? -1 : 0;
+    const int nslot = polyeri_coop_nslot(la, lb, lc, ld, pbs);
     const long long grid = a.nwork < max_grid ? a.nwork : max_grid;
     const size_t lds = (size_t)nslot * sizeof(double);
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     if (mode == MODE_JK)
-        hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+        hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     else if (mode == MODE_JK32)
-        hipLaunchKernelGGL(k_eri_coop<MODE_JK32>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+        hipLaunchKernelGGL(k_eri_coop<MODE_JK32>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     else if (mode == MODE_PBC)
-        hipLaunchKernelGGL(k_eri_coop<MODE_PBC>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+        hipLaunchKernelGGL(k_eri_coop<MODE_PBC>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     else if (mode == MODE_SCHWARZ)
-        hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+        hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     else
-        hipLaunchKernelGGL(k_eri_coop<MODE_TENSOR>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd);
+        hipLaunchKernelGGL(k_eri_coop<MODE_TENSOR>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

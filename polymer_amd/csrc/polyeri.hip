@@ -320,6 +320,7 @@ static std::vector<SegDesc> make_segments(const WorkList& l, const std::vector<P
 struct LaunchCfg {
     int kind = LAUNCH_AUTO, block = 256, grid = 0;
     int segv = 0;   // segment kernels: 0 cost-balanced segments, 1 segments of kSegItems items
+    int pbs = -1;   // cooperative kernel: log2 of the primitive batch (-1: the class default)
 };
 
 struct ClassPairWork {
@@ -548,10 +549,10 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
         return 0;
     }
     if (kind == LAUNCH_COOP) {
-        int ns = polyeri_coop_nslot(la, lb, lc, ld);
-        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
+        const int pbs = cfg.pbs >= 0 ? cfg.pbs
+                                     : polyeri_coop_default_pbs(ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld));
         const int grid = cfg.grid > 0 ? cfg.grid : 4096;
-        if (polyeri_coop_launch(a, mode, la, lb, lc, ld, ns, grid, c->stream))
+        if (polyeri_coop_launch(a, mode, la, lb, lc, ld, pbs, grid, c->stream))
             return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
         return 0;
     }
@@ -609,10 +610,10 @@ static int launch_class_pbc(polyeri_ctx* c, const ClassPairWork& cw, long long n
     if (tensor) { a.out = tensor; a.cel1 = c->rc1; }
     if (nwork <= 0) return 0;
     EriKernel k = kernel_for<MODE_PBC>(cw.la, cw.lb, cw.lc, cw.ld);
-    if (!k || ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld) > POLYERI_COOP_MIN_INTEGRALS) {
-        const int ns = polyeri_coop_nslot(cw.la, cw.lb, cw.lc, cw.ld);
-        if (ns <= 0) return fail(POLYERI_E_HIP, "cooperative program lookup failed");
-        if (polyeri_coop_launch(a, MODE_PBC, cw.la, cw.lb, cw.lc, cw.ld, ns, 4096, c->stream))
+    const int nint4 = ncart_h(cw.la) * ncart_h(cw.lb) * ncart_h(cw.lc) * ncart_h(cw.ld);
+    if (!k || nint4 > POLYERI_COOP_MIN_INTEGRALS) {
+        if (polyeri_coop_launch(a, MODE_PBC, cw.la, cw.lb, cw.lc, cw.ld, polyeri_coop_default_pbs(nint4), 4096,
+                                c->stream))
             return fail(POLYERI_E_HIP, "cooperative kernel launch failed");
         return 0;
     }
@@ -1216,7 +1217,10 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
                     }
         }
         if (!reg || nint >= 36)
-            for (int g : {2048, 4096, 8192}) { LaunchCfg x; x.kind = LAUNCH_COOP; x.grid = g; cand.push_back(x); }
+            for (int g : {2048, 4096, 8192})
+                for (int pbs : {0, 1, 2, 3}) {
+                    LaunchCfg x; x.kind = LAUNCH_COOP; x.grid = g; x.pbs = pbs; cand.push_back(x);
+                }
         double best = 1e300;
         LaunchCfg pick = cw.cfg;
         for (const LaunchCfg& x : cand) {
