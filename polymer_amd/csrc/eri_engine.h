@@ -32,6 +32,16 @@
 // the BOYS_NT x BOYS_NM table in the same allocation (BOYS_AOFF doubles in).
 #define BOYS_LA 9
 #define BOYS_AOFF (BOYS_NT * BOYS_NM)
+// A third layout, small enough for LDS, for the kernels that copy it there:
+// for each L < BOYS_LC, rows F_L(T_j) .. F_{L+9}(T_j) on the coarse grid
+// T_j = j/5 (every fourth point of the main grid), j = 0..BOYS_CNT-1, read
+// with a ten-term Taylor step (|h| <= 0.1, truncation below 1e-18).  It
+// follows the aligned tables (BOYS_COFF doubles in); BOYS_CROWS doubles
+// (16 080 bytes) per L.
+#define BOYS_LC 7
+#define BOYS_CNT 201
+#define BOYS_CROWS (BOYS_CNT * 10)
+#define BOYS_COFF (BOYS_AOFF + BOYS_LA * BOYS_NT * 8)
 
 // Primitive pair of a shell pair (a,b): exponent sum, prefactor with both
 // contraction coefficients, centre P and P - A.  64 bytes.
@@ -156,6 +166,43 @@ POLY_HD void boys_fp64(double T, double pref, const double* __restrict__ tab,
     for (int m = 0; m <= L; ++m) F[m] *= pref;
 }
 
+// The same from the coarse rows of one L (row0: BOYS_CNT rows of ten
+// values, in LDS), same downward recursion and asymptotic form.
+template <int L>
+POLY_HD void boys_coarse(double T, double pref, const double* __restrict__ row0, double* __restrict__ F) {
+    static_assert(L < BOYS_LC, "coarse Boys table too short for this class");
+    if (T < BOYS_TMAX) {
+        const int ts = (int)(T * 5.0 + 0.5);
+        const double h = 0.2 * ts - T;
+        const BoysPair* g = reinterpret_cast<const BoysPair*>(row0 + ts * 10);
+        const BoysPair g01 = g[0], g23 = g[1], g45 = g[2], g67 = g[3], g89 = g[4];
+        double f = g89.y;
+        f = g89.x + h * (1.0 / 9.0) * f;
+        f = g67.y + h * (1.0 / 8.0) * f;
+        f = g67.x + h * (1.0 / 7.0) * f;
+        f = g45.y + h * (1.0 / 6.0) * f;
+        f = g45.x + h * (1.0 / 5.0) * f;
+        f = g23.y + h * (1.0 / 4.0) * f;
+        f = g23.x + h * (1.0 / 3.0) * f;
+        f = g01.y + h * (1.0 / 2.0) * f;
+        f = g01.x + h * f;
+        F[L] = f;
+        if (L > 0) {
+            const double e = exp(-T);
+            const double t2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double r = rsqrt_fp64(T), it = r * r;
+        F[0] = 0.5 * POLY_SQRT_PI * r;
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = F[m] * (m + 0.5) * it;
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] *= pref;
+}
+
 #include "eri_generated.h"
 
 // Dispatch to the generated per-class recursion.
@@ -202,8 +249,8 @@ POLY_CLASS(2, 2, 2, 2)
 #undef POLY_CLASS
 
 // One primitive quartet: the VRR's [e0|f0] values, prefactors K_ab K_cd
-// included, added to acc.
-template <typename real, int LA, int LB, int LC, int LD>
+// included, added to acc.  COARSE: boys_tab is the coarse rows of L.
+template <typename real, int LA, int LB, int LC, int LD, bool COARSE = false>
 POLY_HD void prim_vrr(const PrimPair& bp, const PrimPair& kq, const double* __restrict__ boys_tab, real* acc) {
     constexpr int L = LA + LB + LC + LD;
     const double ze = bp.zeta + kq.zeta;
@@ -214,7 +261,8 @@ POLY_HD void prim_vrr(const PrimPair& bp, const PrimPair& kq, const double* __re
     const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
     const double pref = bp.K * kq.K * rs;
     double F[L + 1];
-    boys_fp64<L>(T, pref, boys_tab, F);
+    if constexpr (COARSE) boys_coarse<L>(T, pref, boys_tab, F);
+    else boys_fp64<L>(T, pref, boys_tab, F);
     real Fr[L + 1];
 #pragma unroll
     for (int m = 0; m <= L; ++m) Fr[m] = (real)F[m];

@@ -812,8 +812,10 @@ __device__ __forceinline__ double gc_dmax(const EriArgs& g, int x0, int x1, int 
 // the groups' shells) is below dtol, and a member quartet is digested only
 // when its own bound Q_ab Q_cd max|D| (six shell blocks) reaches dtol, the
 // test the shell-pair lists apply; digested member quartets are counted in
-// kept.
-template <int LA, int LC>
+// kept.  BT: the block copies the coarse Boys rows of its L into LDS (in
+// front of the rows) and reads them there instead of the aligned global
+// table, whose row gathers miss the L1 on most primitive iterations.
+template <int LA, int LC, bool BT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LA == 1 && LC == 0 ? 3 : 1)))
 k_eri_gc(EriArgs g) {
     constexpr int NA = ncart(LA), NC = ncart(LC), NI = NA * NC;
@@ -821,7 +823,15 @@ k_eri_gc(EriArgs g) {
     // factored member weights for (ss|ss) (measured faster there, no change
     // for the other classes, whose bra sums cost registers)
     constexpr bool GC_FACTOR = LA == 0 && LC == 0;
-    extern __shared__ double seg_lds[];
+    extern __shared__ double lds_all[];
+    const double* btab = g.boysA;
+    double* seg_lds = lds_all;
+    if (BT) {
+        const double* src = g.boys + BOYS_COFF + (size_t)(LA + LC) * BOYS_CROWS;
+        for (int i = threadIdx.x; i < BOYS_CROWS; i += blockDim.x) lds_all[i] = src[i];
+        btab = lds_all;
+        seg_lds = lds_all + BOYS_CROWS;      // the first zeroing pass below is followed by a barrier
+    }
     const int n = g.nbf;
     double* Kr = seg_lds;                        // rows fa0.. fa1.. (NA each), fb0, fb1
     double* Jl = seg_lds + (size_t)(2 * NA + 2) * n;
@@ -886,7 +896,7 @@ k_eri_gc(EriArgs g) {
                         double tv[NI];
 #pragma unroll
                         for (int i = 0; i < NI; ++i) tv[i] = 0.0;
-                        prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
+                        prim_vrr<double, LA, 0, LC, 0, BT>(bq.pp, kq, btab, tv);
 #pragma unroll
                         for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -909,7 +919,7 @@ k_eri_gc(EriArgs g) {
                         double tv[NI];
 #pragma unroll
                         for (int i = 0; i < NI; ++i) tv[i] = 0.0;
-                        prim_vrr<double, LA, 0, LC, 0>(bq.pp, kq, g.boysA, tv);
+                        prim_vrr<double, LA, 0, LC, 0, BT>(bq.pp, kq, btab, tv);
                         double u[MB * NI];
 #pragma unroll
                         for (int i = 0; i < MB; ++i)
@@ -1048,13 +1058,22 @@ __device__ __forceinline__ void gck_digest(const EriArgs& g, const double* acc, 
     for (int c = 0; c < NC; ++c) jk_add(&J[(size_t)(fc + c) * n + fd], s * cb.Jcd[c], g.jk_copies);
 }
 
-template <int LA, int LB, int LC>
+// BT: the coarse Boys rows in LDS, as in k_eri_gc.
+template <int LA, int LB, int LC, bool BT>
 __global__ void __launch_bounds__(256) k_eri_gck(EriArgs g) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC);
     constexpr int NE = ((LA + LB + 1) * (LA + LB + 2) * (LA + LB + 3) - LA * (LA + 1) * (LA + 2)) / 6;
     constexpr int NF = ncart(LC);
     constexpr int MK = LC == 0 ? 4 : 2;
-    extern __shared__ double seg_lds[];
+    extern __shared__ double lds_all[];
+    const double* btab = g.boysA;
+    double* seg_lds = lds_all;
+    if (BT) {
+        const double* src = g.boys + BOYS_COFF + (size_t)(LA + LB + LC) * BOYS_CROWS;
+        for (int i = threadIdx.x; i < BOYS_CROWS; i += blockDim.x) lds_all[i] = src[i];
+        btab = lds_all;
+        seg_lds = lds_all + BOYS_CROWS;      // the first zeroing pass below is followed by a barrier
+    }
     const int n = g.nbf;
     double* Kr = seg_lds;                        // rows fa.. (NA) then fb.. (NB), width n
     double* Jl = seg_lds + (size_t)(NA + NB) * n;
@@ -1112,7 +1131,7 @@ __global__ void __launch_bounds__(256) k_eri_gck(EriArgs g) {
                 double sb[NE * NF];
 #pragma unroll
                 for (int i = 0; i < NE * NF; ++i) sb[i] = 0.0;
-                for (int p = 0; p < mb.nprim; ++p) prim_vrr<double, LA, LB, LC, 0>(bra[p], kq, g.boysA, sb);
+                for (int p = 0; p < mb.nprim; ++p) prim_vrr<double, LA, LB, LC, 0, BT>(bra[p], kq, btab, sb);
 #pragma unroll
                 for (int j = 0; j < MK; ++j) {
                     const double wk = x[(8 + j) * fs];

@@ -16,13 +16,16 @@ bool polyeri_gc_class(int la, int lc) {
 
 int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream) {
     if (!polyeri_gc_class(la, lc) || a.nseg <= 0 || grid <= 0) return -1;
-    const size_t lds = polyeri_gc_lds_bytes(la, a.nbf);
+    size_t lds = polyeri_gc_lds_bytes(la, a.nbf);
     if (lds > 65536) return -1;
+    // the coarse Boys rows go into LDS where they fit beside the rows
+    const bool bt = lds + BOYS_CROWS * sizeof(double) <= 65536;
+    if (bt) lds += BOYS_CROWS * sizeof(double);
     void (*k)(EriArgs) = nullptr;
-    if (la == 0) k = k_eri_gc<0, 0>;
-    else if (la == 1 && lc == 0) k = k_eri_gc<1, 0>;
-    else if (la == 1) k = k_eri_gc<1, 1>;
-    else k = k_eri_gc<2, 0>;
+    if (la == 0) k = bt ? k_eri_gc<0, 0, true> : k_eri_gc<0, 0, false>;
+    else if (la == 1 && lc == 0) k = bt ? k_eri_gc<1, 0, true> : k_eri_gc<1, 0, false>;
+    else if (la == 1) k = bt ? k_eri_gc<1, 1, true> : k_eri_gc<1, 1, false>;
+    else k = bt ? k_eri_gc<2, 0, true> : k_eri_gc<2, 0, false>;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), lds, stream, a);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -37,13 +40,15 @@ bool polyeri_gck_class(int la, int lb, int lc) {
 
 int polyeri_gck_launch(const EriArgs& a, int la, int lb, int lc, int block, int grid, hipStream_t stream) {
     if (!polyeri_gck_class(la, lb, lc) || a.nseg <= 0 || grid <= 0) return -1;
-    const size_t lds = polyeri_gck_lds_bytes(la, lb, a.nbf);
+    size_t lds = polyeri_gck_lds_bytes(la, lb, a.nbf);
     if (lds > 65536) return -1;
+    const bool bt = lds + BOYS_CROWS * sizeof(double) <= 65536;
+    if (bt) lds += BOYS_CROWS * sizeof(double);
     void (*k)(EriArgs) = nullptr;
-    if (la == 1 && lc == 0) k = k_eri_gck<1, 1, 0>;
-    else if (la == 1) k = k_eri_gck<1, 1, 1>;
-    else if (lb == 0) k = k_eri_gck<2, 0, 1>;
-    else k = k_eri_gck<2, 1, 0>;
+    if (la == 1 && lc == 0) k = bt ? k_eri_gck<1, 1, 0, true> : k_eri_gck<1, 1, 0, false>;
+    else if (la == 1) k = bt ? k_eri_gck<1, 1, 1, true> : k_eri_gck<1, 1, 1, false>;
+    else if (lb == 0) k = bt ? k_eri_gck<2, 0, 1, true> : k_eri_gck<2, 0, 1, false>;
+    else k = bt ? k_eri_gck<2, 1, 0, true> : k_eri_gck<2, 1, 0, false>;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(block), lds, stream, a);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -885,13 +885,20 @@ static int build_ctx(polyeri_ctx* c) {
     if ((rc = dalloc(&c->d_meta, np))) return rc;
     if ((rc = dalloc(&c->d_ab, 4 * (size_t)np))) return rc;
     // device copy: the BOYS_NT x BOYS_NM table, then the aligned per-L tables
-    // of the thread kernels (BOYS_LA x BOYS_NT rows of F_L .. F_{L+6}, pad)
+    // of the thread kernels (BOYS_LA x BOYS_NT rows of F_L .. F_{L+6}, pad),
+    // then the coarse per-L rows (BOYS_LC x BOYS_CNT rows of F_L .. F_{L+9}
+    // at every fourth grid point)
+    static_assert(BOYS_LC - 1 + 9 < BOYS_NM && 4 * (BOYS_CNT - 1) < BOYS_NT, "coarse Boys rows out of the table");
     std::vector<double> boys_dev(boys);
-    boys_dev.resize((size_t)BOYS_AOFF + (size_t)BOYS_LA * BOYS_NT * 8, 0.0);
+    boys_dev.resize((size_t)BOYS_COFF + (size_t)BOYS_LC * BOYS_CROWS, 0.0);
     for (int L = 0; L < BOYS_LA; ++L)
         for (int i = 0; i < BOYS_NT; ++i)
             for (int j = 0; j < 7; ++j)
                 boys_dev[(size_t)BOYS_AOFF + ((size_t)L * BOYS_NT + i) * 8 + j] = boys[(size_t)i * BOYS_NM + L + j];
+    for (int L = 0; L < BOYS_LC; ++L)
+        for (int i = 0; i < BOYS_CNT; ++i)
+            for (int j = 0; j < 10; ++j)
+                boys_dev[(size_t)BOYS_COFF + (size_t)L * BOYS_CROWS + i * 10 + j] = boys[(size_t)4 * i * BOYS_NM + L + j];
     if ((rc = dalloc(&c->d_boys, boys_dev.size()))) return rc;
     if ((rc = dalloc(&c->d_Q, np))) return rc;
     if ((rc = dalloc(&c->d_ref, nbf))) return rc;
