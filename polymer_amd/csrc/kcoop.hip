@@ -31,6 +31,11 @@ __device__ __forceinline__ double boys_taylor(int m, double T, const double* __r
     return f;
 }
 
+// floor(t / d) for 0 <= t < 4096 and d <= 64 from r = 1/d (FP32): t/d is
+// at least 1/(2d) away from the next integer after the +1/2 shift, far
+// beyond the rounding of the product.
+__device__ __forceinline__ int qdiv(int t, float r) { return (int)(((float)t + 0.5f) * r); }
+
 template <int MODE>
 __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
     // node values in the recursion's precision (FP32 for MODE_JK32)
@@ -39,6 +44,7 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
     real* sl = reinterpret_cast<real*>(sl_raw);
     const ProgDesc& d = g_prog_desc[prog];
     const int lane = threadIdx.x;
+    const float rb = 1.0f / nb, rc = 1.0f / nc, rd = 1.0f / nd, rcd = 1.0f / (nc * nd);
     const long long nw = work_count(g);
     for (long long t = blockIdx.x; t < nw; t += gridDim.x) {
         int bp, kp, s3 = 0;
@@ -247,9 +253,13 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             }
         } else {
             // Gather digestion: the quartet's integrals are copied into a dense
-            // (a,b,c,d)-ordered array and the six density blocks into LDS; then
-            // each lane forms whole output elements of the six J/K blocks, so
-            // one atomic per output element replaces six per integral.
+            // (a,b,c,d)-ordered array and the six density blocks into LDS.  Each
+            // output element of the six J/K blocks is a two-index sum; a task is
+            // one output element and one value of the outer index (the inner
+            // index summed in registers), the tasks of a block are spread over
+            // the lanes, and the partial sums meet in an LDS buffer (LDS
+            // atomics), so every lane runs the same short loop; then one global
+            // atomic per output element.
             const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
             real* dv = sl + d.nslot;
             real* Dab = dv + d.nfin;
@@ -258,6 +268,9 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             real* Dad = Dac + na * nc;
             real* Dbc = Dad + na * nd;
             real* Dbd = Dbc + nb * nc;
+            const int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
+                      o6 = o5 + nb * nd;
+            real* out = Dbd + nb * nd + (COOP_PREC << pbs);
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
@@ -269,46 +282,76 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             for (int k = lane; k < na * nd; k += 64) Dad[k] = (real)g.D[(size_t)(fa + k / nd) * n + fd + k % nd];
             for (int k = lane; k < nb * nc; k += 64) Dbc[k] = (real)g.D[(size_t)(fb + k / nc) * n + fc + k % nc];
             for (int k = lane; k < nb * nd; k += 64) Dbd[k] = (real)g.D[(size_t)(fb + k / nd) * n + fd + k % nd];
+            for (int k = lane; k < o6; k += 64) out[k] = real(0);
+            __syncthreads();
+            const int nbcd = nb * nc * nd, ncd = nc * nd;
+            // J[a,b] += sum_cd v(ab|cd) D[c,d]: task (ab, c), inner d
+            for (int t = lane; t < na * nb * nc; t += 64) {
+                const int ab = qdiv(t, rc), c = t - ab * nc;
+                const real* v = dv + t * nd;
+                const real* Dp = Dcd + c * nd;
+                real acc = 0;
+                for (int e = 0; e < nd; ++e) acc += v[e] * Dp[e];
+                atomicAdd(&out[ab], acc);
+            }
+            // J[c,d] += sum_ab v(ab|cd) D[a,b]: task (a, cd), inner b
+            for (int t = lane; t < na * ncd; t += 64) {
+                const int a = qdiv(t, rcd), cd = t - a * ncd;
+                const real* v = dv + a * nbcd + cd;
+                const real* Dp = Dab + a * nb;
+                real acc = 0;
+                for (int b = 0; b < nb; ++b) acc += v[b * ncd] * Dp[b];
+                atomicAdd(&out[o1 + cd], acc);
+            }
+            // K[a,c] += sum_bd v(ab|cd) D[b,d]: task (a, b, c), inner d
+            for (int t = lane; t < na * nbcd / nd; t += 64) {
+                const int ab = qdiv(t, rc), c = t - ab * nc, a = qdiv(ab, rb), b = ab - a * nb;
+                const real* v = dv + t * nd;
+                const real* Dp = Dbd + b * nd;
+                real acc = 0;
+                for (int e = 0; e < nd; ++e) acc += v[e] * Dp[e];
+                atomicAdd(&out[o2 + a * nc + c], acc);
+            }
+            // K[a,d] += sum_bc v(ab|cd) D[b,c]: task (a, b, d), inner c
+            for (int t = lane; t < na * nb * nd; t += 64) {
+                const int ab = qdiv(t, rd), e = t - ab * nd, a = qdiv(ab, rb), b = ab - a * nb;
+                const real* v = dv + ab * ncd + e;
+                const real* Dp = Dbc + b * nc;
+                real acc = 0;
+                for (int c = 0; c < nc; ++c) acc += v[c * nd] * Dp[c];
+                atomicAdd(&out[o3 + a * nd + e], acc);
+            }
+            // K[b,c] += sum_ad v(ab|cd) D[a,d]: task (a, b, c), inner d
+            for (int t = lane; t < na * nbcd / nd; t += 64) {
+                const int ab = qdiv(t, rc), c = t - ab * nc, a = qdiv(ab, rb), b = ab - a * nb;
+                const real* v = dv + t * nd;
+                const real* Dp = Dad + a * nd;
+                real acc = 0;
+                for (int e = 0; e < nd; ++e) acc += v[e] * Dp[e];
+                atomicAdd(&out[o4 + b * nc + c], acc);
+            }
+            // K[b,d] += sum_ac v(ab|cd) D[a,c]: task (a, b, d), inner c
+            for (int t = lane; t < na * nb * nd; t += 64) {
+                const int ab = qdiv(t, rd), e = t - ab * nd, a = qdiv(ab, rb), b = ab - a * nb;
+                const real* v = dv + ab * ncd + e;
+                const real* Dp = Dac + a * nc;
+                real acc = 0;
+                for (int c = 0; c < nc; ++c) acc += v[c * nd] * Dp[c];
+                atomicAdd(&out[o5 + b * nd + e], acc);
+            }
             __syncthreads();
             const size_t off = (size_t)xcd_slot(g.jk_copies) * n * n;
             double* J = g.J + off;
             double* K = g.K + off;
-            const int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
-                      o6 = o5 + nb * nd;
             for (int o = lane; o < o6; o += 64) {
-                real acc = 0;
                 double* dst;
-                if (o < o1) {                          // J[a,b] += v(ab|cd) D[c,d]
-                    const int a = o / nb, b = o % nb;
-                    const real* v = dv + (a * nb + b) * nc * nd;
-                    for (int k = 0; k < nc * nd; ++k) acc += v[k] * Dcd[k];
-                    dst = &J[(size_t)(fa + a) * n + fb + b];
-                } else if (o < o2) {                   // J[c,d] += v(ab|cd) D[a,b]
-                    const int k0 = o - o1;
-                    for (int k = 0; k < na * nb; ++k) acc += dv[k * nc * nd + k0] * Dab[k];
-                    dst = &J[(size_t)(fc + k0 / nd) * n + fd + k0 % nd];
-                } else if (o < o3) {                   // K[a,c] += v(ab|cd) D[b,d]
-                    const int a = (o - o2) / nc, c = (o - o2) % nc;
-                    for (int b = 0; b < nb; ++b)
-                        for (int e = 0; e < nd; ++e) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dbd[b * nd + e];
-                    dst = &K[(size_t)(fa + a) * n + fc + c];
-                } else if (o < o4) {                   // K[a,d] += v(ab|cd) D[b,c]
-                    const int a = (o - o3) / nd, e = (o - o3) % nd;
-                    for (int b = 0; b < nb; ++b)
-                        for (int c = 0; c < nc; ++c) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dbc[b * nc + c];
-                    dst = &K[(size_t)(fa + a) * n + fd + e];
-                } else if (o < o5) {                   // K[b,c] += v(ab|cd) D[a,d]
-                    const int b = (o - o4) / nc, c = (o - o4) % nc;
-                    for (int a = 0; a < na; ++a)
-                        for (int e = 0; e < nd; ++e) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dad[a * nd + e];
-                    dst = &K[(size_t)(fb + b) * n + fc + c];
-                } else {                               // K[b,d] += v(ab|cd) D[a,c]
-                    const int b = (o - o5) / nd, e = (o - o5) % nd;
-                    for (int a = 0; a < na; ++a)
-                        for (int c = 0; c < nc; ++c) acc += dv[((a * nb + b) * nc + c) * nd + e] * Dac[a * nc + c];
-                    dst = &K[(size_t)(fb + b) * n + fd + e];
-                }
-                jk_add(dst, s * (double)acc, g.jk_copies);
+                if (o < o1) dst = &J[(size_t)(fa + o / nb) * n + fb + o % nb];
+                else if (o < o2) dst = &J[(size_t)(fc + (o - o1) / nd) * n + fd + (o - o1) % nd];
+                else if (o < o3) dst = &K[(size_t)(fa + (o - o2) / nc) * n + fc + (o - o2) % nc];
+                else if (o < o4) dst = &K[(size_t)(fa + (o - o3) / nd) * n + fd + (o - o3) % nd];
+                else if (o < o5) dst = &K[(size_t)(fb + (o - o4) / nc) * n + fc + (o - o4) % nc];
+                else dst = &K[(size_t)(fb + (o - o5) / nd) * n + fd + (o - o5) % nd];
+                jk_add(dst, s * (double)out[o], g.jk_copies);
             }
         }
         __syncthreads();
@@ -316,13 +359,14 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
 }
 
 // LDS doubles one wavefront of the class needs: the program's node slots,
-// the dense integral block and the six density blocks (J/K), and the
-// records of one batch of 2^pbs primitive quartets.
+// the dense integral block and the six density blocks (J/K), the records of
+// one batch of 2^pbs primitive quartets, and the J/K output elements.
 int polyeri_coop_nslot(int la, int lb, int lc, int ld, int pbs) {
     const int p = prog_index(la, lb, lc, ld);
     if (p < 0) return -1;
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + (COOP_PREC << pbs);
+    return g_prog_nslot[p] + na * nb * nc * nd + na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd + (COOP_PREC << pbs) +
+           na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
 }
 
 int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int pbs, int max_grid,
