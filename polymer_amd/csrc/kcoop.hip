@@ -218,32 +218,59 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
             const size_t off = (size_t)xcd_slot(g.jk_copies) * (2 * c1 + 1) * n2;
             double* J = g.Jc + off;
             double* K = g.Kc + off;
+            // the blocks' sums as tasks over the lanes (one output element and
+            // one outer index each) meeting in an LDS buffer, as in the
+            // molecular digestion below
             const int o1 = ni * nj, o2 = o1 + (kin ? ni * nl : 0), o3 = o2 + (g.both ? nk * nl : 0),
                       o4 = o3 + (kin2 ? nk * nj : 0);
-            for (int o = lane; o < o4; o += 64) {
-                double acc = 0.0;
-                double* dst;
-                if (o < o1) {                          // J(Q1)[i][j] += v D(Q2)[k][l]
-                    const int i = o / nj, j = o % nj;
-                    const real* v = dv + (i * nj + j) * nk * nl;
-                    for (int k = 0; k < nk * nl; ++k) acc += v[k] * D1[k];
-                    dst = &J[(size_t)(q1 + c1) * n2 + (size_t)(fi + i) * n + fj + j];
-                } else if (o < o2) {                   // K(Q3+Q2)[i][l] += v D(Q1-Q3)[k][j]
-                    const int i = (o - o1) / nl, l = (o - o1) % nl;
-                    for (int j = 0; j < nj; ++j)
-                        for (int k = 0; k < nk; ++k) acc += dv[((i * nj + j) * nk + k) * nl + l] * D2[k * nj + j];
-                    dst = &K[(size_t)(qk + c1) * n2 + (size_t)(fi + i) * n + fl + l];
-                } else if (o < o3) {                   // J(Q2)[k][l] += v D(Q1)[i][j]
-                    const int k0 = o - o2;
-                    for (int ij = 0; ij < ni * nj; ++ij) acc += dv[ij * nk * nl + k0] * D3[ij];
-                    dst = &J[(size_t)(q2 + c1) * n2 + (size_t)(fk + k0 / nl) * n + fl + k0 % nl];
-                } else {                               // K(Q1-Q3)[k][j] += v D(Q2+Q3)[i][l]
-                    const int k = (o - o3) / nj, j = (o - o3) % nj;
-                    for (int i = 0; i < ni; ++i)
-                        for (int l = 0; l < nl; ++l) acc += dv[((i * nj + j) * nk + k) * nl + l] * D4[i * nl + l];
-                    dst = &K[(size_t)(qk2 + c1) * n2 + (size_t)(fk + k) * n + fj + j];
+            real* out = D1 + (na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd) + (COOP_PREC << pbs);
+            const int nkl = nk * nl;
+            const float rnj = 1.0f / nj, rnk = 1.0f / nk, rnl = 1.0f / nl, rnkl = 1.0f / nkl;
+            for (int k = lane; k < o4; k += 64) out[k] = real(0);
+            __syncthreads();
+            // J(Q1)[i][j] += sum_kl v D(Q2)[k][l]: task (ij, k), inner l
+            for (int t = lane; t < ni * nj * nk; t += 64) {
+                const int ij = qdiv(t, rnk), k = t - ij * nk;
+                const real* v = dv + t * nl;
+                real acc = 0;
+                for (int l = 0; l < nl; ++l) acc += v[l] * D1[k * nl + l];
+                atomicAdd(&out[ij], acc);
+            }
+            // K(Q3+Q2)[i][l] += sum_jk v D(Q1-Q3)[k][j]: task (i, j, l), inner k
+            if (kin)
+                for (int t = lane; t < ni * nj * nl; t += 64) {
+                    const int ij = qdiv(t, rnl), l = t - ij * nl, i = qdiv(ij, rnj), j = ij - i * nj;
+                    const real* v = dv + ij * nkl + l;
+                    real acc = 0;
+                    for (int k = 0; k < nk; ++k) acc += v[k * nl] * D2[k * nj + j];
+                    atomicAdd(&out[o1 + i * nl + l], acc);
                 }
-                jk_add(dst, acc, g.jk_copies);
+            // J(Q2)[k][l] += sum_ij v D(Q1)[i][j]: task (i, kl), inner j
+            if (g.both)
+                for (int t = lane; t < ni * nkl; t += 64) {
+                    const int i = qdiv(t, rnkl), kl = t - i * nkl;
+                    const real* v = dv + i * nj * nkl + kl;
+                    real acc = 0;
+                    for (int j = 0; j < nj; ++j) acc += v[j * nkl] * D3[i * nj + j];
+                    atomicAdd(&out[o2 + kl], acc);
+                }
+            // K(Q1-Q3)[k][j] += sum_il v D(Q2+Q3)[i][l]: task (i, j, k), inner l
+            if (kin2)
+                for (int t = lane; t < ni * nj * nk; t += 64) {
+                    const int ij = qdiv(t, rnk), k = t - ij * nk, i = qdiv(ij, rnj), j = ij - i * nj;
+                    const real* v = dv + t * nl;
+                    real acc = 0;
+                    for (int l = 0; l < nl; ++l) acc += v[l] * D4[i * nl + l];
+                    atomicAdd(&out[o3 + k * nj + j], acc);
+                }
+            __syncthreads();
+            for (int o = lane; o < o4; o += 64) {
+                double* dst;
+                if (o < o1) dst = &J[(size_t)(q1 + c1) * n2 + (size_t)(fi + o / nj) * n + fj + o % nj];
+                else if (o < o2) dst = &K[(size_t)(qk + c1) * n2 + (size_t)(fi + (o - o1) / nl) * n + fl + (o - o1) % nl];
+                else if (o < o3) dst = &J[(size_t)(q2 + c1) * n2 + (size_t)(fk + (o - o2) / nl) * n + fl + (o - o2) % nl];
+                else dst = &K[(size_t)(qk2 + c1) * n2 + (size_t)(fk + (o - o3) / nj) * n + fj + (o - o3) % nj];
+                jk_add(dst, (double)out[o], g.jk_copies);
             }
         } else if (MODE == MODE_TENSOR) {
             TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
