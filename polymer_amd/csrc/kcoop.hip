@@ -36,8 +36,14 @@ __device__ __forceinline__ double boys_taylor(int m, double T, const double* __r
 // beyond the rounding of the product.
 __device__ __forceinline__ int qdiv(int t, float r) { return (int)(((float)t + 0.5f) * r); }
 
-template <int MODE>
-__global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
+// REGOPS: each lane loads its VRR op words once per wavefront into
+// registers (at most COOP_REGIT wavefront passes over the levels, which
+// holds for the classes with L <= 6), so no op load sits on a level's
+// critical path.
+constexpr int COOP_REGIT = 12;
+
+template <int MODE, bool REGOPS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(REGOPS ? 4 : 1))) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
     // node values in the recursion's precision (FP32 for MODE_JK32)
     using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
     extern __shared__ double sl_raw[];
@@ -45,6 +51,30 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
     const ProgDesc& d = g_prog_desc[prog];
     const int lane = threadIdx.x;
     const float rb = 1.0f / nb, rc = 1.0f / nc, rd = 1.0f / nd, rcd = 1.0f / (nc * nd);
+    // pass k of the VRR: op word of this lane (vmask bit k: the lane has an
+    // op), a barrier after it (brk bit k: the level ends)
+    uint4 opw[REGOPS ? COOP_REGIT : 1];
+    unsigned vmask = 0, brk = 0;
+    int nit = 0;
+    if (REGOPS) {
+        int lv = 0, base = d.vrr_lv[0];
+#pragma unroll
+        for (int k = 0; k < COOP_REGIT; ++k) {
+            const bool live = lv < d.nvrr_lv;
+            const int end = live ? d.vrr_lv[lv + 1] : 0;
+            const bool has = live && base + lane < end;
+            opw[k] = has ? g_prog_vrr[base + lane] : make_uint4(0u, 0u, 0u, 0u);
+            vmask |= (unsigned)has << k;
+            nit += live;
+            if (live && base + 64 >= end) {
+                brk |= 1u << k;
+                ++lv;
+                base = lv < d.nvrr_lv ? d.vrr_lv[lv] : 0;
+            } else {
+                base += 64;
+            }
+        }
+    }
     const long long nw = work_count(g);
     for (long long t = blockIdx.x; t < nw; t += gridDim.x) {
         int bp, kp, s3 = 0;
@@ -121,19 +151,39 @@ __global__ void __launch_bounds__(64) k_eri_coop(EriArgs g, int prog, int na, in
                 char* lb = reinterpret_cast<char*>(sl);
                 constexpr int SH = sizeof(real) == 8 ? 0 : 1;
 #define SLOT(off) (*reinterpret_cast<real*>(lb + ((off) >> SH)))
-                for (int lv = 0; lv < d.nvrr_lv; ++lv) {
-                    for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
-                        const uint4 op = g_prog_vrr[o];
-                        const int typ = op.w & 1, ti = 3 * typ + ((op.w >> 1) & 3);
-                        const real k3 = cf[16 + 8 * typ + ((op.w >> 3) & 7)], k5 = cf[32 + ((op.w >> 6) & 7)];
-                        const real v = cf[ti] * SLOT(op.x >> 16) + cf[6 + ti] * SLOT(op.y & 0xffff) +
-                                       k3 * (SLOT(op.y >> 16) - cf[14 + typ] * SLOT(op.z & 0xffff)) +
-                                       k5 * SLOT(op.z >> 16);
-                        SLOT(op.x & 0xffff) = v;
-                        if (op.w & 512) SLOT(op.w >> 16) += v;
+#define VRR_OP(op)                                                                                         \
+    {                                                                                                      \
+        const int typ = (op).w & 1, ti = 3 * typ + (((op).w >> 1) & 3);                                    \
+        const real k3 = cf[16 + 8 * typ + (((op).w >> 3) & 7)], k5 = cf[32 + (((op).w >> 6) & 7)];         \
+        const real v = cf[ti] * SLOT((op).x >> 16) + cf[6 + ti] * SLOT((op).y & 0xffff) +                  \
+                       k3 * (SLOT((op).y >> 16) - cf[14 + typ] * SLOT((op).z & 0xffff)) +                  \
+                       k5 * SLOT((op).z >> 16);                                                            \
+        SLOT((op).x & 0xffff) = v;                                                                         \
+        if ((op).w & 512) SLOT((op).w >> 16) += v;                                                         \
+    }
+                if (REGOPS) {
+#pragma unroll
+                    for (int k = 0; k < COOP_REGIT; ++k) {
+                        if (k < nit) {
+                            // the empty asm keeps the compiler from hoisting
+                            // the decoded fields of all passes out of the
+                            // primitive loop (which would cost ~150 VGPRs)
+                            uint4 op = opw[k];
+                            asm volatile("" : "+v"(op.x), "+v"(op.y), "+v"(op.z), "+v"(op.w));
+                            if ((vmask >> k) & 1) VRR_OP(op);
+                            if ((brk >> k) & 1) __syncthreads();
+                        }
                     }
-                    __syncthreads();
+                } else {
+                    for (int lv = 0; lv < d.nvrr_lv; ++lv) {
+                        for (int o = d.vrr_lv[lv] + lane; o < d.vrr_lv[lv + 1]; o += 64) {
+                            const uint4 op = g_prog_vrr[o];
+                            VRR_OP(op);
+                        }
+                        __syncthreads();
+                    }
                 }
+#undef VRR_OP
 #undef SLOT
                 if (d.nacc_map > 0) {   // targets no VRR op produces
                     for (int k = lane; k < d.nacc_map; k += 64) {
@@ -405,15 +455,15 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
     const long long grid = a.nwork < max_grid ? a.nwork : max_grid;
     const size_t lds = (size_t)nslot * sizeof(double);
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    if (mode == MODE_JK)
-        hipLaunchKernelGGL(k_eri_coop<MODE_JK>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
-    else if (mode == MODE_JK32)
-        hipLaunchKernelGGL(k_eri_coop<MODE_JK32>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
-    else if (mode == MODE_PBC)
-        hipLaunchKernelGGL(k_eri_coop<MODE_PBC>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
-    else if (mode == MODE_SCHWARZ)
-        hipLaunchKernelGGL(k_eri_coop<MODE_SCHWARZ>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
-    else
-        hipLaunchKernelGGL(k_eri_coop<MODE_TENSOR>, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
+    // every class with L <= 6 needs at most COOP_REGIT wavefront passes
+    // over its VRR levels (checked against the generated programs)
+    const bool ro = la + lb + lc + ld <= 6;
+    void (*k)(EriArgs, int, int, int, int, int, int) = nullptr;
+    if (mode == MODE_JK) k = ro ? k_eri_coop<MODE_JK, true> : k_eri_coop<MODE_JK, false>;
+    else if (mode == MODE_JK32) k = ro ? k_eri_coop<MODE_JK32, true> : k_eri_coop<MODE_JK32, false>;
+    else if (mode == MODE_PBC) k = ro ? k_eri_coop<MODE_PBC, true> : k_eri_coop<MODE_PBC, false>;
+    else if (mode == MODE_SCHWARZ) k = k_eri_coop<MODE_SCHWARZ, false>;
+    else k = k_eri_coop<MODE_TENSOR, false>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
