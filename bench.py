@@ -204,14 +204,10 @@ def main():
     # scaled by the copy kernel; counters cannot be read inside this process)
     traffic, traffic_src = None, None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
-    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri", "gc": "k_eri_gc"}[dc["kernel"]]
-    # the wavefront kernel is one symbol for every heavy class: its entries carry the class
-    dkern = (f"void k_eri_gc<{dc['cls'][0]}, {dc['cls'][2]}>(EriArgs)" if kfn == "k_eri_gc"
-             else f"void {kfn}<{', '.join(dc['cls'])}, 0>(EriArgs)" if kfn != "k_eri_coop"
-             else f"void k_eri_coop<0>(EriArgs, int, int, int, int, int) {cname(dc)}")
     if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
+        # entries are keyed by kernel symbol and carry their class
         with open(tpath) as f:
-            tk = json.load(f)["kernels"].get(dkern)
+            tk = next((v for v in json.load(f)["kernels"].values() if v["class"] == cname(dc)), None)
         if tk:
             traffic = tk["traffic_bytes"]
             traffic_src = (f"profiles/k_eri_traffic_water16_ccpvdz.json: FETCH {tk['fetch_bytes']:.3e} B + WRITE "
