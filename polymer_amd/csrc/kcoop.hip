@@ -36,14 +36,13 @@ __device__ __forceinline__ double boys_taylor(int m, double T, const double* __r
 // beyond the rounding of the product.
 __device__ __forceinline__ int qdiv(int t, float r) { return (int)(((float)t + 0.5f) * r); }
 
-// REGOPS: each lane loads its VRR op words once per wavefront into
-// registers (at most COOP_REGIT wavefront passes over the levels, which
-// holds for the classes with L <= 6), so no op load sits on a level's
+// NIT > 0: each lane loads its VRR op words once per wavefront into
+// registers (at most NIT wavefront passes over the levels: 12 for the
+// classes with L <= 6, 20 for L = 7), so no op load sits on a level's
 // critical path.
-constexpr int COOP_REGIT = 12;
-
-template <int MODE, bool REGOPS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(REGOPS ? 4 : 1))) k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
+template <int MODE, int NIT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NIT == 0 ? 1 : NIT <= 12 ? 4 : 3)))
+k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
     // node values in the recursion's precision (FP32 for MODE_JK32)
     using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
     extern __shared__ double sl_raw[];
@@ -53,13 +52,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(REGOPS 
     const float rb = 1.0f / nb, rc = 1.0f / nc, rd = 1.0f / nd, rcd = 1.0f / (nc * nd);
     // pass k of the VRR: op word of this lane (vmask bit k: the lane has an
     // op), a barrier after it (brk bit k: the level ends)
-    uint4 opw[REGOPS ? COOP_REGIT : 1];
+    constexpr bool REGOPS = NIT > 0;
+    uint4 opw[REGOPS ? NIT : 1];
     unsigned vmask = 0, brk = 0;
     int nit = 0;
     if (REGOPS) {
         int lv = 0, base = d.vrr_lv[0];
 #pragma unroll
-        for (int k = 0; k < COOP_REGIT; ++k) {
+        for (int k = 0; k < NIT; ++k) {
             const bool live = lv < d.nvrr_lv;
             const int end = live ? d.vrr_lv[lv + 1] : 0;
             const bool has = live && base + lane < end;
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(REGOPS 
     }
                 if (REGOPS) {
 #pragma unroll
-                    for (int k = 0; k < COOP_REGIT; ++k) {
+                    for (int k = 0; k < NIT; ++k) {
                         if (k < nit) {
                             // the empty asm keeps the compiler from hoisting
                             // the decoded fields of all passes out of the
@@ -455,15 +455,15 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
     const long long grid = a.nwork < max_grid ? a.nwork : max_grid;
     const size_t lds = (size_t)nslot * sizeof(double);
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
-    // every class with L <= 6 needs at most COOP_REGIT wavefront passes
-    // over its VRR levels (checked against the generated programs)
-    const bool ro = la + lb + lc + ld <= 6;
+    // every class with L <= 6 needs at most 12 wavefront passes over its
+    // VRR levels, L = 7 20 (checked against the generated programs)
+    const int L = la + lb + lc + ld;
     void (*k)(EriArgs, int, int, int, int, int, int) = nullptr;
-    if (mode == MODE_JK) k = ro ? k_eri_coop<MODE_JK, true> : k_eri_coop<MODE_JK, false>;
-    else if (mode == MODE_JK32) k = ro ? k_eri_coop<MODE_JK32, true> : k_eri_coop<MODE_JK32, false>;
-    else if (mode == MODE_PBC) k = ro ? k_eri_coop<MODE_PBC, true> : k_eri_coop<MODE_PBC, false>;
-    else if (mode == MODE_SCHWARZ) k = k_eri_coop<MODE_SCHWARZ, false>;
-    else k = k_eri_coop<MODE_TENSOR, false>;
+    if (mode == MODE_JK) k = L <= 6 ? k_eri_coop<MODE_JK, 12> : L == 7 ? k_eri_coop<MODE_JK, 20> : k_eri_coop<MODE_JK, 0>;
+    else if (mode == MODE_JK32) k = L <= 6 ? k_eri_coop<MODE_JK32, 12> : k_eri_coop<MODE_JK32, 0>;
+    else if (mode == MODE_PBC) k = L <= 6 ? k_eri_coop<MODE_PBC, 12> : k_eri_coop<MODE_PBC, 0>;
+    else if (mode == MODE_SCHWARZ) k = k_eri_coop<MODE_SCHWARZ, 0>;
+    else k = k_eri_coop<MODE_TENSOR, 0>;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64), lds, stream, a, p, na, nb, nc, nd, pbs);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
