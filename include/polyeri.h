@@ -303,10 +303,13 @@ int polyeri_set_graph(polyeri_ctx *ctx, int on);
 int polyeri_set_general_contraction(polyeri_ctx *ctx, int on);
 
 /* Autotuner (no counterpart in the reference): times every work list with
- * each candidate launch shape (thread-per-quartet kernel with 256/128/64
- * threads per block, one pass or a persistent grid; wavefront-per-quartet
- * kernel with three grid sizes) reps times and keeps the fastest.  Results
- * are unchanged; later builds use the chosen shapes (polyeri_stats). */
+ * each candidate launch shape (thread-per-quartet and bra-row segment
+ * kernels with 256/128/64 threads per block, one pass or a persistent grid,
+ * two segmentations; wavefront-per-quartet kernel with three grid sizes and
+ * primitive batches of 1/2/4/8; the generally contracted form with
+ * 256/128/64 threads and two segmentations) reps times and keeps the
+ * fastest.  Results are unchanged to rounding; later builds use the chosen
+ * shapes (polyeri_stats). */
 int polyeri_autotune(polyeri_ctx *ctx, int reps);
 
 const char *polyeri_last_error(void);
