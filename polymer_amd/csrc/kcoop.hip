@@ -39,8 +39,10 @@ __device__ __forceinline__ int qdiv(int t, float r) { return (int)(((float)t + 0
 // NIT > 0: each lane loads its VRR op words once per wavefront into
 // registers (at most NIT wavefront passes over the levels: 12 for the
 // classes with L <= 6, 20 for L = 7), so no op load sits on a level's
-// critical path.
-template <int MODE, int NIT>
+// critical path.  NH > 0 (J/K, L = 5: at most 7 VRR passes): the HRR op
+// words (at most NH passes) and the final-integral map (at most 3 passes)
+// are held in registers too.
+template <int MODE, int NIT, int NH = 0>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NIT == 0 ? 1 : NIT <= 12 ? 4 : 3)))
 k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
     // node values in the recursion's precision (FP32 for MODE_JK32)
@@ -74,6 +76,33 @@ k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
                 base += 64;
             }
         }
+    }
+    // HRR passes (x, y, w of the op word) and final-integral map entries
+    constexpr int NHW = NH > 0 ? NH : 1, NFM = NH > 0 ? 3 : 1;
+    uint32_t hx[NHW], hy[NHW], hw[NHW], fmap[NFM];
+    unsigned hmask = 0, hbrk = 0;
+    int nhit = 0;
+    if (NH > 0) {
+        int lv = 0, base = d.hrr_lv[0];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const bool live = lv < d.nhrr_lv;
+            const int end = live ? d.hrr_lv[lv + 1] : 0;
+            const bool has = live && base + lane < end;
+            const uint4 op = has ? g_prog_hrr[base + lane] : make_uint4(0u, 0u, 0u, 0u);
+            hx[k] = op.x; hy[k] = op.y; hw[k] = op.w;
+            hmask |= (unsigned)has << k;
+            nhit += live;
+            if (live && base + 64 >= end) {
+                hbrk |= 1u << k;
+                ++lv;
+                base = lv < d.nhrr_lv ? d.hrr_lv[lv] : 0;
+            } else {
+                base += 64;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NFM; ++j) fmap[j] = lane + 64 * j < d.nfin ? g_prog_aux[d.fin_off + lane + 64 * j] : 0u;
     }
     const long long nw = work_count(g);
     for (long long t = blockIdx.x; t < nw; t += gridDim.x) {
@@ -196,16 +225,33 @@ k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
         }
         const real ab[3] = {(real)g.ab[4 * bp], (real)g.ab[4 * bp + 1], (real)g.ab[4 * bp + 2]};
         const real cd[3] = {(real)g.ab[4 * kp], (real)g.ab[4 * kp + 1], (real)g.ab[4 * kp + 2]};
-        for (int lv = 0; lv < d.nhrr_lv; ++lv) {
-            for (int o = d.hrr_lv[lv] + lane; o < d.hrr_lv[lv + 1]; o += 64) {
-                const uint4 op = g_prog_hrr[o];
-                const int typ = op.w & 1, i = (op.w >> 1) & 3;
-                const real gv = typ ? (i == 0 ? cd[0] : (i == 1 ? cd[1] : cd[2]))
-                                    : (i == 0 ? ab[0] : (i == 1 ? ab[1] : ab[2]));
-                sl[op.x & 0xffff] = sl[op.x >> 16] + gv * sl[op.y];
+#define HRR_OP(ox, oy, ow)                                                          \
+    {                                                                               \
+        const int typ = (ow) & 1, i = ((ow) >> 1) & 3;                              \
+        const real gv = typ ? (i == 0 ? cd[0] : (i == 1 ? cd[1] : cd[2]))           \
+                            : (i == 0 ? ab[0] : (i == 1 ? ab[1] : ab[2]));          \
+        sl[(ox) & 0xffff] = sl[(ox) >> 16] + gv * sl[(oy)];                         \
+    }
+        if (NH > 0) {
+#pragma unroll
+            for (int k = 0; k < NHW; ++k) {
+                if (k < nhit) {
+                    uint32_t ox = hx[k], oy = hy[k], ow = hw[k];
+                    asm volatile("" : "+v"(ox), "+v"(oy), "+v"(ow));
+                    if ((hmask >> k) & 1) HRR_OP(ox, oy, ow);
+                    if ((hbrk >> k) & 1) __syncthreads();
+                }
             }
-            __syncthreads();
+        } else {
+            for (int lv = 0; lv < d.nhrr_lv; ++lv) {
+                for (int o = d.hrr_lv[lv] + lane; o < d.hrr_lv[lv + 1]; o += 64) {
+                    const uint4 op = g_prog_hrr[o];
+                    HRR_OP(op.x, op.y, op.w);
+                }
+                __syncthreads();
+            }
         }
+#undef HRR_OP
         const int fa = mb.fa, fb = mb.fb, fc = mk.fa, fd = mk.fb;
         const int n = g.nbf;
         if (MODE == MODE_SCHWARZ) {
@@ -348,10 +394,22 @@ k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
             const int o1 = na * nb, o2 = o1 + nc * nd, o3 = o2 + na * nc, o4 = o3 + na * nd, o5 = o4 + nb * nc,
                       o6 = o5 + nb * nd;
             real* out = Dbd + nb * nd + (COOP_PREC << pbs);
-            for (int k = lane; k < d.nfin; k += 64) {
-                const uint32_t f = g_prog_aux[d.fin_off + k];
-                const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
-                dv[((a * nb + b) * nc + c) * nd + e] = sl[f & 0xffff];
+            if (NH > 0) {
+#pragma unroll
+                for (int j = 0; j < NFM; ++j) {
+                    if (lane + 64 * j < d.nfin) {
+                        uint32_t f = fmap[j];
+                        asm volatile("" : "+v"(f));
+                        const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
+                        dv[((a * nb + b) * nc + c) * nd + e] = sl[f & 0xffff];
+                    }
+                }
+            } else {
+                for (int k = lane; k < d.nfin; k += 64) {
+                    const uint32_t f = g_prog_aux[d.fin_off + k];
+                    const int a = (f >> 16) & 15, b = (f >> 20) & 15, c = (f >> 24) & 15, e = f >> 28;
+                    dv[((a * nb + b) * nc + c) * nd + e] = sl[f & 0xffff];
+                }
             }
             for (int k = lane; k < na * nb; k += 64) Dab[k] = (real)g.D[(size_t)(fa + k / nb) * n + fb + k % nb];
             for (int k = lane; k < nc * nd; k += 64) Dcd[k] = (real)g.D[(size_t)(fc + k / nd) * n + fd + k % nd];
@@ -459,7 +517,10 @@ int polyeri_coop_launch(const EriArgs& a, int mode, int la, int lb, int lc, int 
     // VRR levels, L = 7 20 (checked against the generated programs)
     const int L = la + lb + lc + ld;
     void (*k)(EriArgs, int, int, int, int, int, int) = nullptr;
-    if (mode == MODE_JK) k = L <= 6 ? k_eri_coop<MODE_JK, 12> : L == 7 ? k_eri_coop<MODE_JK, 20> : k_eri_coop<MODE_JK, 0>;
+    // L = 5: at most 7 VRR passes, 6 HRR passes and 192 final integrals
+    if (mode == MODE_JK)
+        k = L == 5 ? k_eri_coop<MODE_JK, 7, 6> : L <= 6 ? k_eri_coop<MODE_JK, 12> : L == 7 ? k_eri_coop<MODE_JK, 20>
+                                                                                           : k_eri_coop<MODE_JK, 0>;
     else if (mode == MODE_JK32) k = L <= 6 ? k_eri_coop<MODE_JK32, 12> : k_eri_coop<MODE_JK32, 0>;
     else if (mode == MODE_PBC) k = L <= 6 ? k_eri_coop<MODE_PBC, 12> : k_eri_coop<MODE_PBC, 0>;
     else if (mode == MODE_SCHWARZ) k = k_eri_coop<MODE_SCHWARZ, 0>;
