@@ -4,7 +4,7 @@
 // class's VRR / HRR recursion is executed level by level from generated op
 // tables (eri_programs.h); every node value lives in LDS, so no per-lane
 // array has to fit in registers or spill to scratch.  Same integrals and
-// the same digestion as k_eri.
+// the same J/K contributions as k_eri, gathered per output element.
 #include "eri_kernels.h"
 #include "eri_programs.h"
 
