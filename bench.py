@@ -240,7 +240,7 @@ def main():
         "eri_prim_quartets": st["prim_quartets"],
         "model_tflops_achieved": st["model_flops"] / s_iter / 1e12,
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "compute": "fp64 VALU (same 78.6 TFLOP/s peak as FP64 matrix)",
+        "roofline": {"bound": "valu_fp64", "compute": "fp64 VALU (MI355X FP64 vector peak 78.6 TFLOP/s)",
                      "kernel": f"{kfn}<{','.join(dc['cls'])},JK>", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic, "traffic_source": traffic_src, "kernel_ms": float(cls_ms[dom]),

@@ -485,7 +485,8 @@ void build_pairs_cells(const IntBasis& ib, double primtol, double period, int rc
 
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
                       double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
-                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym) {
+                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym,
+                      uint64_t split_mask) {
     out.clear();
     std::vector<int> rank(pairs.size(), 0);
     for (int X = 0; X < NPAIRCLASS; ++X)
@@ -503,9 +504,9 @@ void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<doub
             if (j == 0 || pairs[kl[j]].nprim != pairs[kl[j - 1]].nprim) bs.push_back((int)j);
         bs.push_back((int)kl.size());
     }
-    const bool split = cut > tol;
     for (int X = 0; X < NPAIRCLASS; ++X)
-        for (int Y = 0; Y <= X; ++Y)
+        for (int Y = 0; Y <= X; ++Y) {
+            const bool split = cut > tol && ((split_mask >> (X * NPAIRCLASS + Y)) & 1);
             for (int prec = 0; prec <= (split ? 1 : 0); ++prec) {
                 const auto& bx = class_ids[X];
                 const auto& ky = ket_ids[Y];
@@ -560,6 +561,7 @@ void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<doub
                 w.nwork = w.row[bx.size()];
                 out.push_back(std::move(w));
             }
+        }
 }
 
 // F_m(T) on the grid by the convergent series

@@ -532,6 +532,8 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     if (kind == LAUNCH_AUTO) kind = k ? auto_kind(c, la, lb, lc, ld, seg_ok) : LAUNCH_COOP;
     if (kind == LAUNCH_SEG && !seg_ok) kind = k ? LAUNCH_REG : LAUNCH_COOP;
     if (kind == LAUNCH_REG && !k) return fail(POLYERI_E_ARG, "no register kernel for class");
+    if (kind == LAUNCH_SEG && mode == MODE_JK32)
+        return fail(POLYERI_E_ARG, "the FP32 segment kernel is switched off (unexplained fault)");
     if (kind == LAUNCH_SEG) {
         EriKernel ks = polyeri_class_kernel(la, lb, lc, ld, mode == MODE_JK ? GET_SEG_JK : GET_SEG_JK32);
         if (!ks) return fail(POLYERI_E_ARG, "no segment kernel for class");
@@ -947,11 +949,22 @@ static int build_ctx(polyeri_ctx* c) {
         if (!ids.empty())
             HIPCHK(hipMemcpy(c->d_class_ids[X], ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
     }
-    // class-pair work lists (host.cpp build_work_lists)
+    // class-pair work lists (host.cpp build_work_lists).  Mixed precision
+    // splits the class pairs of the wavefront kernel only: the light classes
+    // keep one FP64 list each and run the segment and generally contracted
+    // kernels, as in FP64 contexts (an FP32 recursion issues at the FP64 rate
+    // on gfx950 without packed FP32 math, so it saves them nothing, and the
+    // FP32 segment kernel stays switched off; see launch_class).
     std::vector<WorkList> wl;
     const double cut = c->opts.precision == POLYERI_MIXED ? c->opts.mixed_cut : 0.0;
+    uint64_t split = 0;
+    for (int X = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y <= X; ++Y)
+            if (ncart_h(kPairL[X][0]) * ncart_h(kPairL[X][1]) * ncart_h(kPairL[Y][0]) * ncart_h(kPairL[Y][1]) >
+                POLYERI_COOP_MIN_INTEGRALS)
+                split |= 1ull << (X * NPAIRCLASS + Y);
     build_work_lists(c->pairs, c->Q, c->opts.schwarz_tol, c->pbc ? 0.0 : cut, c->class_ids, c->ket_ids, c->bstart,
-                     wl, c->pbc);
+                     wl, c->pbc, split);
     for (int Y = 0; Y < NPAIRCLASS; ++Y) {
         if ((rc = dalloc(&c->d_ket_ids[Y], c->ket_ids[Y].size()))) return rc;
         if ((rc = dalloc(&c->d_bstart[Y], c->bstart[Y].size()))) return rc;
@@ -1013,11 +1026,12 @@ static int build_ctx(polyeri_ctx* c) {
         if ((rc = dalloc(&w.d_rb, l.rb.size()))) return rc;
         HIPCHK(hipMemcpy(w.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
         if (!l.rb.empty()) HIPCHK(hipMemcpy(w.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
-        // Segment lists for FP64 molecular contexts only: a mixed-precision J/K
-        // build ((H2O)4 cc-pVDZ, cut 1e-5) with the segment kernel stopped on an
-        // illegal memory access whose cause is not established, so mixed
-        // contexts keep the register kernel.
-        if (!c->pbc && l.nwork > 0 && c->opts.precision != POLYERI_MIXED) {
+        // Segment lists for the FP64 lists of molecular contexts.  FP32 lists
+        // get none: the FP32 segment kernel once stopped on an illegal memory
+        // access ((H2O)4 cc-pVDZ, cut 1e-5) whose cause is not established (a
+        // host replay of its index path over those lists stays in bounds), so
+        // it is not launched.
+        if (!c->pbc && l.nwork > 0 && !l.prec) {
             std::vector<SegDesc> segs =
                 make_segments(l, c->pairs, c->class_ids[l.X], c->ket_ids[l.Y], c->bstart[l.Y]);
             const std::vector<SegDesc> segc =
@@ -1039,7 +1053,7 @@ static int build_ctx(polyeri_ctx* c) {
         c->model_flops += w.flops;
         c->work.push_back(w);
     }
-    if (!c->pbc && c->opts.precision == POLYERI_FP64 && (rc = build_gc_lists(c))) return rc;
+    if (!c->pbc && (rc = build_gc_lists(c))) return rc;
     // density-screening tables
     {
         const int nsh = (int)c->ib.shells.size();

@@ -97,12 +97,14 @@ int build_gc_pairs(const IntBasis& ib, double primtol, const std::vector<PairInf
                    std::vector<int>& member_pair);
 // class_ids[X] must hold the pairs of class X in Schwarz order (bra rows);
 // fills ket_ids / bstart (bucketed ket order) and the lists.  cut > tol
-// splits every class pair into an FP64 list and an FP32 list (see WorkList).
+// splits a class pair into an FP64 list and an FP32 list (see WorkList) when
+// bit X * NPAIRCLASS + Y of split_mask is set (every class pair by default).
 // nosym: every ket of class Y for every bra of class X (no rank_j <= rank_i
 // restriction), for the periodic lattice sums.
 void build_work_lists(const std::vector<PairInfo>& pairs, const std::vector<double>& Q, double tol,
                       double cut, const std::vector<int> class_ids[], std::vector<int> ket_ids[],
-                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym = false);
+                      std::vector<int> bstart[], std::vector<WorkList>& out, bool nosym = false,
+                      uint64_t split_mask = ~0ull);
 struct int2_host {
     int x, y;
 };

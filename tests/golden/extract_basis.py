@@ -6,7 +6,7 @@ Fortran 'D' exponents), which is what the basis reader parses
 (src/pread.f90:611-1037 is the reference reader of that format).
 
 Run once in a checkout that has the reference tree:
-    python polymer_amd/basis/extract_basis.py /root/reference/basisset
+    python tests/golden/extract_basis.py /root/reference/basisset polymer_amd/basis
 """
 import os
 import sys
@@ -41,6 +41,7 @@ def extract(src_dir, name, out_dir):
 if __name__ == "__main__":
     src = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/basisset"
     here = os.path.dirname(os.path.abspath(__file__))
+    out = sys.argv[2] if len(sys.argv) > 2 else os.path.join(here, "..", "..", "polymer_amd", "basis")
     for b in BASES:
-        n = extract(src, b, here)
+        n = extract(src, b, out)
         print(b, n, "elements")
