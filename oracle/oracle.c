@@ -11,6 +11,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -724,16 +725,27 @@ long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int 
                                  const double *p_p, double inttol, double *c_p, double *x_p) {
     boys_init();
     pairtab_init();
-    int ns = b->npshell, np = b->npgs;
+    int ns = b->npshell, np = b->npgs, ncell = 2 * cel1 + 1, n1 = 2 * rc1 + 1;
     size_t np2 = (size_t)np * np, nsq = (size_t)ns * ns;
     double *maxc = maxc_make(b);
-    cpair *kl0 = (cpair *)malloc(sizeof(cpair) * nsq);
-    cpair ij, q;
-    double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
     const double c1f = 2.0 / sqrt(PI_REF);
     long long nq = 0;
-    for (int q1 = -rc1; q1 <= rc1; ++q1)
-        for (int q2 = -rc1; q2 <= rc1; ++q2) {
+    /* (Q1, Q2) tickets spread over the host threads as the reference spreads
+     * them over MPI ranks (:224-240); each thread sums into its own J/K
+     * cells, added in thread order afterwards (its MPI_ALLREDUCE, :757-764) */
+    int nt = omp_get_max_threads();
+    double *cpt = (double *)calloc((size_t)nt * ncell * np2, sizeof(double));
+    double *xpt = (double *)calloc((size_t)nt * ncell * np2, sizeof(double));
+#pragma omp parallel num_threads(nt) reduction(+ : nq)
+    {
+        double *cpl = cpt + (size_t)omp_get_thread_num() * ncell * np2;
+        double *xpl = xpt + (size_t)omp_get_thread_num() * ncell * np2;
+        cpair *kl0 = (cpair *)malloc(sizeof(cpair) * nsq);
+        cpair ij, q;
+        double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
+#pragma omp for schedule(static, 1)
+        for (int tk = 0; tk < n1 * n1; ++tk) {
+            const int q1 = tk / n1 - rc1, q2 = tk % n1 - rc1;
             /* KLKL with K in cell 0 and L in cell Q2 (:258-364) */
             for (int k = 0; k < ns; ++k)
                 for (int l = 0; l < ns; ++l) {
@@ -746,11 +758,11 @@ long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int 
                     cpair_make_shift(b, I, 0.0, J, q1 * period, &ij);
                     cpair_bound(&ij, maxc, e);
                     double c1 = ij.ov * c1f;
-                    double *cq1 = c_p + (size_t)(q1 + cel1) * np2;
+                    double *cq1 = cpl + (size_t)(q1 + cel1) * np2;
                     for (int q3 = -cel2; q3 <= cel2; ++q3) {
                         /* LCEL1 (:247-259): exchange only into stored cells */
                         int lx = (q3 + q2 >= -cel1 && q3 + q2 <= cel1 && q1 - q3 >= -cel1 && q1 - q3 <= cel1);
-                        double *xq = lx ? x_p + (size_t)(q3 + q2 + cel1) * np2 : NULL;
+                        double *xq = lx ? xpl + (size_t)(q3 + q2 + cel1) * np2 : NULL;
                         const double *pq13 = lx ? p_p + (size_t)(q1 - q3 + cel1) * np2 : NULL;
                         for (size_t skl = 0; skl < nsq; ++skl) {
                             if (ij.bound * kl0[skl].bound < inttol) continue;
@@ -777,7 +789,14 @@ long long or_fourindex_eri_cells(const or_basis *b, double period, int rc1, int 
                     }
                 }
         }
-    free(e); free(kl0); free(maxc);
+        free(e); free(kl0);
+    }
+    for (int th = 0; th < nt; ++th)
+        for (size_t i = 0; i < (size_t)ncell * np2; ++i) {
+            c_p[i] += cpt[(size_t)th * ncell * np2 + i];
+            x_p[i] += xpt[(size_t)th * ncell * np2 + i];
+        }
+    free(cpt); free(xpt); free(maxc);
     return nq;
 }
 

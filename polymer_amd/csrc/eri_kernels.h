@@ -102,6 +102,24 @@ __device__ __forceinline__ void jk_add(double* p, double v, int copies) {
         atomicAdd(p, v);
 }
 
+// Fixed-point accumulation (periodic J/K): the element holds a two's
+// complement integer in units of 2^-JK_FX_BITS, each contribution is rounded
+// to that grid and added with an integer atomic.  Integer addition is
+// associative, so the sum (and the sum of the per-XCD copies) does not depend
+// on the order in which the atomics arrive: two builds of the same density
+// give bit-identical matrices.  Resolution 2^-50 = 8.9e-16 per contribution;
+// the final elements must lie within +-2^13 (intermediate sums may wrap).
+#define JK_FX_BITS 50
+__device__ __forceinline__ void jk_add_fx(double* p, double v, int copies) {
+    const unsigned long long x = (unsigned long long)__double2ll_rn(v * 0x1p50);
+    if (copies > 1)
+        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), x, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p), x, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Digestion with the quartet's six density / Fock blocks held in registers
 // (light classes).  With s the permutational degeneracy weight:
 // Jacc[ab] += s v D[cd], Jacc[cd] += s v D[ab], Kacc[ac] += s v D[bd],
@@ -252,10 +270,10 @@ __device__ __forceinline__ void pbc_add(const EriArgs& g, double* J, double* K, 
                                         int fj, int fk, int fl, double v) {
     const size_t n = g.nbf, n2 = n * n;
     const int c1 = g.cel1;
-    jk_add(&J[(size_t)(q1 + c1) * n2 + fi * n + fj], v * g.Dc[(size_t)(q2 + c1) * n2 + fk * n + fl], g.jk_copies);
+    jk_add_fx(&J[(size_t)(q1 + c1) * n2 + fi * n + fj], v * g.Dc[(size_t)(q2 + c1) * n2 + fk * n + fl], g.jk_copies);
     const int qk = q3 + q2, qd = q1 - q3;
     if (qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1)
-        jk_add(&K[(size_t)(qk + c1) * n2 + fi * n + fl], v * g.Dc[(size_t)(qd + c1) * n2 + fk * n + fj],
+        jk_add_fx(&K[(size_t)(qk + c1) * n2 + fi * n + fl], v * g.Dc[(size_t)(qd + c1) * n2 + fk * n + fj],
                g.jk_copies);
 }
 
@@ -329,7 +347,7 @@ __device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, in
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) jk_add(&Jq[(size_t)(fi + i) * n + fj + j], acc[i * NJ + j], cp);
+            for (int j = 0; j < NJ; ++j) jk_add_fx(&Jq[(size_t)(fi + i) * n + fj + j], acc[i * NJ + j], cp);
     }
     const int qk = s + q2, qd = q1 - s;
     if (qk >= -c1 && qk <= c1 && qd >= -c1 && qd <= c1) {
@@ -354,7 +372,7 @@ __device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, in
 #pragma unroll
         for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int l = 0; l < NL; ++l) jk_add(&Kq[(size_t)(fi + i) * n + fl + l], acc[i * NL + l], cp);
+            for (int l = 0; l < NL; ++l) jk_add_fx(&Kq[(size_t)(fi + i) * n + fl + l], acc[i * NL + l], cp);
     }
     if (!g.both) return;
     {
@@ -379,7 +397,7 @@ __device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, in
 #pragma unroll
         for (int k = 0; k < NK; ++k)
 #pragma unroll
-            for (int l = 0; l < NL; ++l) jk_add(&Jq[(size_t)(fk + k) * n + fl + l], acc[k * NL + l], cp);
+            for (int l = 0; l < NL; ++l) jk_add_fx(&Jq[(size_t)(fk + k) * n + fl + l], acc[k * NL + l], cp);
     }
     const int qk2 = q1 - s, qd2 = q2 + s;
     if (qk2 >= -c1 && qk2 <= c1 && qd2 >= -c1 && qd2 <= c1) {
@@ -404,7 +422,7 @@ __device__ __forceinline__ void pbc_digest(const EriArgs& g, const double* v, in
 #pragma unroll
         for (int k = 0; k < NK; ++k)
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) jk_add(&Kq[(size_t)(fk + k) * n + fj + j], acc[k * NJ + j], cp);
+            for (int j = 0; j < NJ; ++j) jk_add_fx(&Kq[(size_t)(fk + k) * n + fj + j], acc[k * NJ + j], cp);
     }
 }
 
@@ -483,7 +501,7 @@ __device__ __forceinline__ void pbc_loop(const EriArgs& g, int bp, int kp, const
                     for (int j = 0; j < NJ; ++j)
 #pragma unroll
                         for (int k = 0; k < NK; ++k) x += V(i, j, k, l) * dkj[k * NJ + j];
-                    jk_add(&Kq[(size_t)(fi + i) * n + fl + l], x, cp);
+                    jk_add_fx(&Kq[(size_t)(fi + i) * n + fl + l], x, cp);
                 }
         }
         if (!g.both) continue;
@@ -516,7 +534,7 @@ __device__ __forceinline__ void pbc_loop(const EriArgs& g, int bp, int kp, const
                     for (int i = 0; i < NI; ++i)
 #pragma unroll
                         for (int l = 0; l < NL; ++l) x += V(i, j, k, l) * dil[i * NL + l];
-                    jk_add(&Kq[(size_t)(fk + k) * n + fj + j], x, cp);
+                    jk_add_fx(&Kq[(size_t)(fk + k) * n + fj + j], x, cp);
                 }
         }
     }
@@ -524,13 +542,13 @@ __device__ __forceinline__ void pbc_loop(const EriArgs& g, int bp, int kp, const
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) jk_add(&J1[(size_t)(fi + i) * n + fj + j], jij[i * NJ + j], cp);
+        for (int j = 0; j < NJ; ++j) jk_add_fx(&J1[(size_t)(fi + i) * n + fj + j], jij[i * NJ + j], cp);
     if (!g.both) return;
     double* J2 = J + (size_t)(q2 + c1) * n2;
 #pragma unroll
     for (int k = 0; k < NK; ++k)
 #pragma unroll
-        for (int l = 0; l < NL; ++l) jk_add(&J2[(size_t)(fk + k) * n + fl + l], jkl[k * NL + l], cp);
+        for (int l = 0; l < NL; ++l) jk_add_fx(&J2[(size_t)(fk + k) * n + fl + l], jkl[k * NL + l], cp);
 }
 
 // Work index t = item x nchunk + chunk, chunk of g.qloop Q3 values.

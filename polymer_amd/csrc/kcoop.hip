@@ -366,7 +366,7 @@ k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
                 else if (o < o2) dst = &K[(size_t)(qk + c1) * n2 + (size_t)(fi + (o - o1) / nl) * n + fl + (o - o1) % nl];
                 else if (o < o3) dst = &J[(size_t)(q2 + c1) * n2 + (size_t)(fk + (o - o2) / nl) * n + fl + (o - o2) % nl];
                 else dst = &K[(size_t)(qk2 + c1) * n2 + (size_t)(fk + (o - o3) / nj) * n + fj + (o - o3) % nj];
-                jk_add(dst, (double)out[o], g.jk_copies);
+                jk_add_fx(dst, (double)out[o], g.jk_copies);
             }
         } else if (MODE == MODE_TENSOR) {
             TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};

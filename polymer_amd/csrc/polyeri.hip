@@ -442,7 +442,8 @@ struct polyeri_ctx {
     int cel1 = 0, cel2 = 0, rc1 = 0;
     std::vector<int2_host> pcell;
     int2* d_pcell = nullptr;
-    double *d_Dc = nullptr, *d_Jc = nullptr, *d_Kc = nullptr;
+    double *d_Dc = nullptr, *d_Jc = nullptr, *d_Kc = nullptr;   // J/K cells: fixed-point integers (jk_add_fx)
+    unsigned* d_fx_overflow = nullptr;                            // set when a J/K cell sum left the fixed-point range
     double *d_pcc = nullptr, *d_jcc = nullptr, *d_kcc = nullptr;
 };
 
@@ -671,6 +672,7 @@ static void free_ctx(polyeri_ctx* c) {
         (void)hipFree(c->d_gbra[i]); (void)hipFree(c->d_gket[i]); (void)hipFree(c->d_gbstart[i]); (void)hipFree(c->d_gbsoa[i]);
     }
     (void)hipFree(c->d_pcell); (void)hipFree(c->d_Dc); (void)hipFree(c->d_Jc); (void)hipFree(c->d_Kc);
+    (void)hipFree(c->d_fx_overflow);
     (void)hipFree(c->d_pcc); (void)hipFree(c->d_jcc); (void)hipFree(c->d_kcc);
     (void)hipFree(c->d_psh); (void)hipFree(c->d_sh_f0); (void)hipFree(c->d_sh_nf); (void)hipFree(c->d_dsh); (void)hipFree(c->d_counts);
     (void)hipFree(c->d_ref); (void)hipFree(c->d_scale); (void)hipFree(c->d_D); (void)hipFree(c->d_J); (void)hipFree(c->d_K);
@@ -1074,9 +1076,10 @@ static int build_ctx(polyeri_ctx* c) {
         if ((rc = dalloc(&c->d_pcell, c->pcell.size())) || (rc = dalloc(&c->d_Dc, ncell * n2)) ||
             (rc = dalloc(&c->d_Jc, ncell * n2 * c->jk_copies)) || (rc = dalloc(&c->d_Kc, ncell * n2 * c->jk_copies)) ||
             (rc = dalloc(&c->d_pcc, ncell * c2)) || (rc = dalloc(&c->d_jcc, ncell * c2)) ||
-            (rc = dalloc(&c->d_kcc, ncell * c2)))
+            (rc = dalloc(&c->d_kcc, ncell * c2)) || (rc = dalloc(&c->d_fx_overflow, 1)))
             return rc;
         HIPCHK(hipMemcpy(c->d_pcell, c->pcell.data(), c->pcell.size() * sizeof(int2_host), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(c->d_fx_overflow, 0, sizeof(unsigned)));
         // work counts: every (bra, ket) item runs for 2 cel2 + 1 cells Q3
         c->quartets = 0;
         for (ClassPairWork& w : c->work) c->quartets += w.nwork * (2LL * c->cel2 + 1);
@@ -1312,24 +1315,33 @@ int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
 
 // J/K from the context's device density (reference layout, ncgs^2).
 // Periodic: C_C(ref i, ref j, Q) = s_i s_j J(Q)[i][j] summed over the per-XCD
-// copies (no symmetrisation: the lattice sums run over every quartet).
+// copies (no symmetrisation: the lattice sums run over every quartet).  The
+// copies hold fixed-point integers (jk_add_fx): they are summed as integers
+// (wrapping, so intermediate overflow cancels) and converted once, so the
+// result does not depend on the order of the atomics.  A sum beyond the
+// fixed-point range is reported through *overflow.
 __global__ void k_unpack_cells(const double* __restrict__ J, const double* __restrict__ K, int nbf, int ncell,
                                int copies, const int* __restrict__ ref, const double* __restrict__ sc, int nc,
-                               double* __restrict__ jc, double* __restrict__ kc) {
+                               double* __restrict__ jc, double* __restrict__ kc, unsigned* __restrict__ overflow) {
     const size_t n2 = (size_t)nbf * nbf;
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n2 * ncell) return;
     const int cell = (int)(t / n2);
     const size_t e = t % n2;
     const int mu = (int)(e / nbf), nu = (int)(e % nbf);
-    double js = 0.0, ks = 0.0;
+    const unsigned long long* Ji = reinterpret_cast<const unsigned long long*>(J);
+    const unsigned long long* Ki = reinterpret_cast<const unsigned long long*>(K);
+    unsigned long long js = 0, ks = 0;
     for (int x = 0; x < copies; ++x) {
-        js += J[((size_t)x * ncell + cell) * n2 + e];
-        ks += K[((size_t)x * ncell + cell) * n2 + e];
+        js += Ji[((size_t)x * ncell + cell) * n2 + e];
+        ks += Ki[((size_t)x * ncell + cell) * n2 + e];
     }
+    const double jv = (double)(long long)js * 0x1p-50, kv = (double)(long long)ks * 0x1p-50;
+    // a sum near the edge of the +-2^13 range has most likely wrapped
+    if (fabs(jv) > 0x1p12 || fabs(kv) > 0x1p12) *overflow = 1u;
     const size_t r = (size_t)cell * nc * nc + ref[mu] + (size_t)nc * ref[nu];
-    jc[r] = sc[mu] * sc[nu] * js;
-    kc[r] = sc[mu] * sc[nu] * ks;
+    jc[r] = sc[mu] * sc[nu] * jv;
+    kc[r] = sc[mu] * sc[nu] * kv;
 }
 
 __global__ void k_fill_u32(unsigned* p, size_t n) {
@@ -1503,11 +1515,21 @@ static int jk_device_cells(polyeri_ctx* c, const double* d_pc, double* d_jc, dou
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2 * ncell)), dim3(256), 0, c->stream, d_jc, c2 * ncell, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2 * ncell)), dim3(256), 0, c->stream, d_kc, c2 * ncell, 0.0);
     hipLaunchKernelGGL(k_unpack_cells, dim3(nblk(n2 * ncell)), dim3(256), 0, c->stream, c->d_Jc, c->d_Kc, nbf, ncell,
-                       c->jk_copies, c->d_ref, c->d_scale, nc, d_jc, d_kc);
+                       c->jk_copies, c->d_ref, c->d_scale, nc, d_jc, d_kc, c->d_fx_overflow);
     HIPCHK(hipGetLastError());
     c->screened_last = false;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return 0;
+}
+
+// Fails when a periodic J/K cell sum has left the fixed-point range
+// (|element| >= 2^12 Eh); the flag is cleared for the next build.
+static int check_fx_overflow(polyeri_ctx* c) {
+    unsigned f = 0;
+    HIPCHK(hipMemcpy(&f, c->d_fx_overflow, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (!f) return 0;
+    HIPCHK(hipMemset(c->d_fx_overflow, 0, sizeof(unsigned)));
+    return fail(POLYERI_E_NONFINITE, "periodic J/K element beyond the fixed-point accumulation range");
 }
 
 int polyeri_jk_build_cells(polyeri_ctx* ctx, const double* p_c, double* c_c, double* x_c) {
@@ -1522,6 +1544,7 @@ int polyeri_jk_build_cells(polyeri_ctx* ctx, const double* p_c, double* c_c, dou
     HIPCHK(hipMemcpyAsync(x_c, ctx->d_kcc, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if ((rc = jk_collect_timing(ctx))) return rc;
+    if ((rc = check_fx_overflow(ctx))) return rc;
     for (size_t i = 0; i < n; ++i)
         if (!std::isfinite(c_c[i]) || !std::isfinite(x_c[i])) return fail(POLYERI_E_NONFINITE, "non-finite J/K");
     return 0;
@@ -2496,6 +2519,7 @@ int polyeri_scf_periodic(polyeri_ctx* ctx, const polyeri_scf_opts* o, int kvc, d
         if (dchg < o->scfconv) { converged = 1; break; }
     }
     if (rc) return rc;
+    if ((rc = check_fx_overflow(ctx))) return rc;
     if (iscf > o->scfcycles) iscf = o->scfcycles;
     HIPCHK(hipMemcpyAsync(p_c, dP, nv * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
