@@ -279,7 +279,8 @@ typedef struct polyeri_stats {
     int class_prec[POLYERI_MAX_LISTS];  /* 0: FP64, 1: FP32                  */
     int class_kind[POLYERI_MAX_LISTS];  /* 1: thread per quartet, 2: wavefront per quartet,
                                            3: thread per quartet, bra-row segments (LDS exchange rows),
-                                           4: as 3 over generally contracted pairs (s shells sharing exponents) */
+                                           4: as 3 over generally contracted pairs (s shells sharing exponents),
+                                           5: Rys quadrature, lanes sharing a quartet, bra-row segments */
     int class_block[POLYERI_MAX_LISTS]; /* threads per block                 */
     int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass; kinds 3 and 4: segments (blocks) launched */
     int nclass;
@@ -311,6 +312,16 @@ int polyeri_set_general_contraction(polyeri_ctx *ctx, int on);
  * fastest.  Results are unchanged to rounding; later builds use the chosen
  * shapes (polyeri_stats). */
 int polyeri_autotune(polyeri_ctx *ctx, int reps);
+
+/* Nodes u[0..n-1] (ascending, in (0, 1)) and weights w[0..n-1] of the
+ * n-point Rys quadrature at Boys argument T >= 0, n = 1..5: the Gauss rule
+ * of the weight exp(-T t^2) on t in [0, 1] in the variable u = t^2, so that
+ * sum_r w_r u_r^m = F_m(T) for m < 2n.  These are the nodes the library's
+ * Rys-quadrature kernels use for the d-heavy quartet classes; host
+ * evaluation, no device needed.  No counterpart in the reference, whose
+ * integrals come from F_m(T) through the Obara-Saika recursion
+ * (src/pfourindexdirect.f90:543-690). */
+int polyeri_rys_nodes(int n, double T, double *u, double *w);
 
 const char *polyeri_last_error(void);
 

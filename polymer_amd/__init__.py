@@ -70,6 +70,15 @@ def restore_fourindex_eri_cells(path, ncgs, cel1, cel2):
     return np.ascontiguousarray(t.transpose(0, 1, 2, 6, 5, 4, 3)), n.value
 
 
+def rys_nodes(n, T):
+    """Nodes u (ascending) and weights w of the n-point Rys rule at T
+    (polyeri_rys_nodes): sum_r w_r u_r^m = F_m(T) for m < 2n."""
+    u = np.zeros(n)
+    w = np.zeros(n)
+    check(lib().polyeri_rys_nodes(int(n), ctypes.c_double(T), _p(u), _p(w)))
+    return u, w
+
+
 def read_basis(basis, atoms):
     """READ3 (src/pread.f90:584-1482): atoms = [(Z, x, y, z)] in bohr."""
     L = lib()
@@ -373,7 +382,7 @@ class Context:
         d["classes"] = [dict(cls=f"{st.class_l[i]:04d}", ms=st.last_class_ms[i],
                              prim_quartets=st.class_prim_quartets[i], flops=st.class_flops[i],
                              quartets=st.class_quartets[i], prec="fp32" if st.class_prec[i] else "fp64",
-                             kernel={2: "wave", 3: "seg", 4: "gc"}.get(st.class_kind[i], "thread"), block=st.class_block[i],
+                             kernel={2: "wave", 3: "seg", 4: "gc", 5: "rys"}.get(st.class_kind[i], "thread"), block=st.class_block[i],
                              grid=st.class_grid[i])
                         for i in range(n)]
         return d

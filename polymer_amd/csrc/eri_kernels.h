@@ -1247,6 +1247,14 @@ int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hip
 size_t polyeri_gck_lds_bytes(int la, int lb, int nbf);
 bool polyeri_gck_class(int la, int lb, int lc);
 int polyeri_gck_launch(const EriArgs& a, int la, int lb, int lc, int block, int grid, hipStream_t stream);
+// Rys-quadrature segment kernels (krys.hip) for the classes (dp|pp) (dp|dp)
+// (dd|pp) (dd|ds) (dd|dp) (dd|dd), FP64 J/K over bra-row segments.
+bool polyeri_rys_class(int la, int lb, int lc, int ld);
+size_t polyeri_rys_lds_bytes(int la, int lb, int nbf);
+int polyeri_rys_launch(const EriArgs& a, int la, int lb, int lc, int ld, int block, int max_grid, hipStream_t stream);
+// Nodes u[0..n-1] and weights w[0..n-1] of the n-point Rys rule at T, n <= 5
+// (host evaluation of the kernels' tables).
+int polyeri_rys_nodes_host(int n, double T, double* u, double* w);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).

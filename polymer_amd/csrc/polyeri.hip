@@ -257,7 +257,8 @@ static double flop_model(int la, int lb, int lc, int ld) {
 // segment kernel; grid: block cap (0 = one block per 256 items for the
 // register kernel, one per segment for the segment kernel, 4096 for the
 // cooperative one).  The autotuner fills these per list.
-enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2, LAUNCH_SEG = 3 };
+enum { LAUNCH_AUTO = 0, LAUNCH_REG = 1, LAUNCH_COOP = 2, LAUNCH_SEG = 3, LAUNCH_RYS = 5 };
+static const int RYS_NMAX_EXPORT = 5;
 // Items per bra-row segment of the segment kernel (at most), and the number
 // of segments a list's primitive quartets are spread over (at least).
 static const int kSegItems = 2048, kSegMin = 256;
@@ -479,7 +480,10 @@ static size_t seg_lds_bytes(int la, int lb, int nbf) {
 
 // The launch kind LAUNCH_AUTO resolves to.
 static int auto_kind(const polyeri_ctx* c, int la, int lb, int lc, int ld, bool seg_ok) {
-    if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS) return LAUNCH_COOP;
+    if (ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld) > POLYERI_COOP_MIN_INTEGRALS)
+        return seg_ok && polyeri_rys_class(la, lb, lc, ld) && polyeri_rys_lds_bytes(la, lb, c->ib.nbf) <= 65536
+                   ? LAUNCH_RYS
+                   : LAUNCH_COOP;
     return seg_ok && seg_lds_bytes(la, lb, c->ib.nbf) <= 65536 ? LAUNCH_SEG : LAUNCH_REG;
 }
 
@@ -494,11 +498,16 @@ struct SegScreen {
 };
 
 // Whether a list's J/K launch (unscreened) runs the segment kernel.
+// (The Rys kernel is a segment kernel too: both screen inside the kernel and
+// keep no item lists.)
 static bool list_uses_seg(const polyeri_ctx* c, const ClassPairWork& cw) {
     if (cw.nseg <= 0 || seg_lds_bytes(cw.la, cw.lb, c->ib.nbf) > 65536) return false;
+    const int kind = cw.cfg.kind == LAUNCH_AUTO ? auto_kind(c, cw.la, cw.lb, cw.lc, cw.ld, true) : cw.cfg.kind;
+    if (kind == LAUNCH_RYS)
+        return !cw.prec && polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) &&
+               polyeri_rys_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536;
     const int mode = cw.prec ? MODE_JK32 : MODE_JK;
     if (!polyeri_class_kernel(cw.la, cw.lb, cw.lc, cw.ld, mode)) return false;
-    const int kind = cw.cfg.kind == LAUNCH_AUTO ? auto_kind(c, cw.la, cw.lb, cw.lc, cw.ld, true) : cw.cfg.kind;
     return kind == LAUNCH_SEG;
 }
 
@@ -535,6 +544,23 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     if (kind == LAUNCH_REG && !k) return fail(POLYERI_E_ARG, "no register kernel for class");
     if (kind == LAUNCH_SEG && mode == MODE_JK32)
         return fail(POLYERI_E_ARG, "the FP32 segment kernel is switched off (unexplained fault)");
+    if (kind == LAUNCH_RYS) {
+        const bool rys_ok = seg_ok && mode == MODE_JK && polyeri_rys_class(la, lb, lc, ld) &&
+                            polyeri_rys_lds_bytes(la, lb, c->ib.nbf) <= 65536;
+        if (!rys_ok) {
+            kind = LAUNCH_COOP;
+        } else {
+            a.segs = segs;
+            a.nseg = nseg;
+            if (scr) {
+                a.dtol = scr->tol; a.Qs = scr->Q; a.psh = scr->psh; a.dsh = scr->dsh; a.nsh = scr->nsh;
+                a.kept = scr->kept;
+            }
+            if (polyeri_rys_launch(a, la, lb, lc, ld, cfg.block, cfg.grid, c->stream))
+                return fail(POLYERI_E_HIP, "Rys kernel launch failed");
+            return 0;
+        }
+    }
     if (kind == LAUNCH_SEG) {
         EriKernel ks = polyeri_class_kernel(la, lb, lc, ld, mode == MODE_JK ? GET_SEG_JK : GET_SEG_JK32);
         if (!ks) return fail(POLYERI_E_ARG, "no segment kernel for class");
@@ -1175,6 +1201,12 @@ int polyeri_ctx_create_periodic(int npgs, int ncgs, const int* pax, const int* p
     return 0;
 }
 
+int polyeri_rys_nodes(int n, double T, double* u, double* w) {
+    if (n < 1 || n > RYS_NMAX_EXPORT || !(T >= 0.0) || !u || !w)
+        return fail(POLYERI_E_ARG, "polyeri_rys_nodes: n must be 1..5, T >= 0 and u, w non-null");
+    return polyeri_rys_nodes_host(n, T, u, w) ? fail(POLYERI_E_ARG, "polyeri_rys_nodes: bad argument") : 0;
+}
+
 int polyeri_basis_read(const char* path, int natom, const int* iatom, const double* xyz, int* npgs,
                        int* ncgs, int* npshell, int* pax, int* pay, int* paz, int* pgs_atom,
                        double* pgsx, double* pgsy, double* pgsz, double* zt, double* cc, double* cgsx,
@@ -1240,6 +1272,12 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
                         LaunchCfg y; y.kind = LAUNCH_SEG; y.block = b; y.segv = v; cand.push_back(y);
                     }
         }
+        if (!cw.prec && cw.nseg > 0 && polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) &&
+            polyeri_rys_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536)
+            for (int v : {0, 1})
+                for (int b : {256, 128, 64}) {
+                    LaunchCfg y; y.kind = LAUNCH_RYS; y.block = b; y.segv = v; cand.push_back(y);
+                }
         if (!reg || nint >= 36)
             for (int g : {2048, 4096, 8192})
                 for (int pbs : {0, 1, 2, 3}) {
@@ -1686,7 +1724,7 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
         st->class_kind[i] = kind;
         st->class_block[i] = kind == LAUNCH_COOP ? 64 : kind == 4 ? w.gc_block : w.cfg.block;
         st->class_grid[i] = kind == 4 ? (w.gc_segv ? w.gc.nsegc : w.gc.nseg)
-                          : kind == LAUNCH_SEG ? (w.cfg.segv ? w.nsegc : w.nseg) : w.cfg.grid;
+                          : kind == LAUNCH_SEG || kind == LAUNCH_RYS ? (w.cfg.segv ? w.nsegc : w.nseg) : w.cfg.grid;
         st->class_prim_quartets[i] = w.prim_quartets;
         st->class_flops[i] = w.flops;
         if (kind == 4) {   // executed arithmetic: every member combination per primitive quartet
