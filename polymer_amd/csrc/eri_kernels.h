@@ -1248,10 +1248,12 @@ size_t polyeri_gck_lds_bytes(int la, int lb, int nbf);
 bool polyeri_gck_class(int la, int lb, int lc);
 int polyeri_gck_launch(const EriArgs& a, int la, int lb, int lc, int block, int grid, hipStream_t stream);
 // Rys-quadrature segment kernels (krys.hip) for the classes (dp|pp) (dp|dp)
-// (dd|pp) (dd|ds) (dd|dp) (dd|dd), FP64 J/K over bra-row segments.
+// (dd|pp) (dd|ds) (dd|dp) (dd|dd): J/K over bra-row segments, mode MODE_JK
+// or MODE_JK32 (FP32 recursion).
 bool polyeri_rys_class(int la, int lb, int lc, int ld);
 size_t polyeri_rys_lds_bytes(int la, int lb, int nbf);
-int polyeri_rys_launch(const EriArgs& a, int la, int lb, int lc, int ld, int block, int max_grid, hipStream_t stream);
+int polyeri_rys_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int block, int max_grid,
+                       hipStream_t stream);
 // Nodes u[0..n-1] and weights w[0..n-1] of the n-point Rys rule at T, n <= 5
 // (host evaluation of the kernels' tables).
 int polyeri_rys_nodes_host(int n, double T, double* u, double* w);

@@ -55,13 +55,16 @@ __device__ __forceinline__ int glob_comp(int k, const int* p) {
 __device__ __forceinline__ double sel3(const double* v, int i) { return i == 0 ? v[0] : i == 1 ? v[1] : v[2]; }
 
 // One variant of the class over the items of one segment that this wave
-// owns: chunk c of the segment's items goes to wave (c + base) % nw.
-template <int LA, int LB, int LC, int LD, int V>
+// owns: chunk c of the segment's items goes to wave (c + base) % nw.  MODE:
+// MODE_JK (FP64) or MODE_JK32 (recursion and per-lane integrals in FP32;
+// nodes, weights, pair data and the digestion in FP64).
+template <int LA, int LB, int LC, int LD, int V, int MODE>
 __device__ __forceinline__ void rys_variant(const EriArgs& g, const SegDesc& seg, int bp, int fa, int fb,
                                             const PrimPair* __restrict__ bra, int nbra, double* __restrict__ Kr,
                                             double* __restrict__ Jl, const double* __restrict__ Dl, double* J,
                                             int wave, int nw, int& base) {
     using R = RysVar<LA, LB, LC, LD, V>;
+    using real = typename std::conditional<MODE == MODE_JK32, float, double>::type;
     constexpr int NA = ncart(LA), NB = ncart(LB);
     const int lane = threadIdx.x & 63;
     constexpr int IPW = 64 / R::NMEMB;
@@ -113,9 +116,9 @@ __device__ __forceinline__ void rys_variant(const EriArgs& g, const SegDesc& seg
                         g.bucket_start[b + 1] - g.bucket_start[b], g.kfs};
         const double CDg[3] = {g.ab[4 * kp], g.ab[4 * kp + 1], g.ab[4 * kp + 2]};
         const double CDl0 = sel3(CDg, p[0]), CDl1 = sel3(CDg, p[1]), CDl2 = sel3(CDg, p[2]);
-        double acc[R::NOUT];
+        real accr[R::NOUT];
 #pragma unroll
-        for (int i = 0; i < R::NOUT; ++i) acc[i] = 0.0;
+        for (int i = 0; i < R::NOUT; ++i) accr[i] = real(0);
         for (int pi = 0; pi < nbra; ++pi) {
             const PrimPair bq = bra[pi];
             const double PAg[3] = {bq.PAx, bq.PAy, bq.PAz};
@@ -138,17 +141,20 @@ __device__ __forceinline__ void rys_variant(const EriArgs& g, const SegDesc& seg
                     double u, w;
                     rys_node<R::NROOT>(T, r, rys_tab<R::NROOT>(), rys_herm<R::NROOT>(), u, w);
                     const double uz = roz * u, ue = roe * u;
-                    R::node(w * pref, u * h2ze, h2z * (1.0 - uz), h2e * (1.0 - ue), PA0 - uz * PQ0, PA1 - uz * PQ1,
-                            PA2 - uz * PQ2, QC0 + ue * PQ0, QC1 + ue * PQ1, QC2 + ue * PQ2, ABl0, ABl1, ABl2, CDl0,
-                            CDl1, CDl2, acc);
+                    R::template node<real>(
+                        (real)(w * pref), (real)(u * h2ze), (real)(h2z * (1.0 - uz)), (real)(h2e * (1.0 - ue)),
+                        (real)(PA0 - uz * PQ0), (real)(PA1 - uz * PQ1), (real)(PA2 - uz * PQ2), (real)(QC0 + ue * PQ0),
+                        (real)(QC1 + ue * PQ1), (real)(QC2 + ue * PQ2), (real)ABl0, (real)ABl1, (real)ABl2,
+                        (real)CDl0, (real)CDl1, (real)CDl2, accr);
                 }
             }
         }
         // digestion of the lane's integrals v(a, b, c, d) (local components)
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
         const double* D = g.D;
+        double acc[R::NOUT];
 #pragma unroll
-        for (int i = 0; i < R::NOUT; ++i) acc[i] *= s;
+        for (int i = 0; i < R::NOUT; ++i) acc[i] = s * (double)accr[i];
         // J_ab += v D_cd (registers, per lane); J_cd += v D_ab (one atomic per element)
 #pragma unroll
         for (int ic = 0; ic < R::NC; ++ic)
@@ -230,19 +236,19 @@ __device__ __forceinline__ void rys_variant(const EriArgs& g, const SegDesc& seg
     base += chunks;
 }
 
-template <int LA, int LB, int LC, int LD, int V>
+template <int LA, int LB, int LC, int LD, int V, int MODE>
 __device__ __forceinline__ void rys_variants(const EriArgs& g, const SegDesc& seg, int bp, int fa, int fb,
                                              const PrimPair* bra, int nbra, double* Kr, double* Jl, const double* Dl,
                                              double* J, int wave, int nw, int& base) {
     if constexpr (V < RysClass<LA, LB, LC, LD>::NVAR) {
-        rys_variant<LA, LB, LC, LD, V>(g, seg, bp, fa, fb, bra, nbra, Kr, Jl, Dl, J, wave, nw, base);
-        rys_variants<LA, LB, LC, LD, V + 1>(g, seg, bp, fa, fb, bra, nbra, Kr, Jl, Dl, J, wave, nw, base);
+        rys_variant<LA, LB, LC, LD, V, MODE>(g, seg, bp, fa, fb, bra, nbra, Kr, Jl, Dl, J, wave, nw, base);
+        rys_variants<LA, LB, LC, LD, V + 1, MODE>(g, seg, bp, fa, fb, bra, nbra, Kr, Jl, Dl, J, wave, nw, base);
     }
 }
 
 // LDS: exchange rows of the two bra shells ((NA + NB) nbf doubles), J_ab and
 // the bra's density block D_ab (NA NB doubles each).
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int MODE>
 __global__ void __launch_bounds__(256) k_eri_rys(EriArgs g) {
     constexpr int NA = ncart(LA), NB = ncart(LB);
     extern __shared__ double rys_lds[];
@@ -264,8 +270,8 @@ __global__ void __launch_bounds__(256) k_eri_rys(EriArgs g) {
         for (int i = threadIdx.x; i < NA * NB; i += blockDim.x) Dl[i] = g.D[(size_t)(fa + i / NB) * n + fb + i % NB];
         __syncthreads();
         int base = 0;
-        rys_variants<LA, LB, LC, LD, 0>(g, seg, bp, fa, fb, g.prims + mb.prim_off, mb.nprim, Kr, Jl, Dl, J, wave, nw,
-                                        base);
+        rys_variants<LA, LB, LC, LD, 0, MODE>(g, seg, bp, fa, fb, g.prims + mb.prim_off, mb.nprim, Kr, Jl, Dl, J,
+                                              wave, nw, base);
         __syncthreads();
         for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
             const double x = Kr[i];
@@ -293,18 +299,25 @@ bool polyeri_rys_class(int la, int lb, int lc, int ld) {
     return false;
 }
 
-int polyeri_rys_launch(const EriArgs& a, int la, int lb, int lc, int ld, int block, int max_grid, hipStream_t stream) {
-    if (a.nseg <= 0) return 0;
-    void (*k)(EriArgs) = nullptr;
+template <int MODE>
+static void (*rys_kernel(int la, int lb, int lc, int ld))(EriArgs) {
     switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 2111: k = k_eri_rys<2, 1, 1, 1>; break;
-    case 2121: k = k_eri_rys<2, 1, 2, 1>; break;
-    case 2211: k = k_eri_rys<2, 2, 1, 1>; break;
-    case 2220: k = k_eri_rys<2, 2, 2, 0>; break;
-    case 2221: k = k_eri_rys<2, 2, 2, 1>; break;
-    case 2222: k = k_eri_rys<2, 2, 2, 2>; break;
-    default: return -1;
+    case 2111: return k_eri_rys<2, 1, 1, 1, MODE>;
+    case 2121: return k_eri_rys<2, 1, 2, 1, MODE>;
+    case 2211: return k_eri_rys<2, 2, 1, 1, MODE>;
+    case 2220: return k_eri_rys<2, 2, 2, 0, MODE>;
+    case 2221: return k_eri_rys<2, 2, 2, 1, MODE>;
+    case 2222: return k_eri_rys<2, 2, 2, 2, MODE>;
     }
+    return nullptr;
+}
+
+int polyeri_rys_launch(const EriArgs& a, int mode, int la, int lb, int lc, int ld, int block, int max_grid,
+                       hipStream_t stream) {
+    if (a.nseg <= 0) return 0;
+    void (*k)(EriArgs) = mode == MODE_JK ? rys_kernel<MODE_JK>(la, lb, lc, ld)
+                       : mode == MODE_JK32 ? rys_kernel<MODE_JK32>(la, lb, lc, ld) : nullptr;
+    if (!k) return -1;
     if (block != 64 && block != 128 && block != 256) block = 256;
     unsigned grid = (unsigned)a.nseg;
     if (max_grid > 0 && grid > (unsigned)max_grid) grid = (unsigned)max_grid;

@@ -504,8 +504,7 @@ static bool list_uses_seg(const polyeri_ctx* c, const ClassPairWork& cw) {
     if (cw.nseg <= 0 || seg_lds_bytes(cw.la, cw.lb, c->ib.nbf) > 65536) return false;
     const int kind = cw.cfg.kind == LAUNCH_AUTO ? auto_kind(c, cw.la, cw.lb, cw.lc, cw.ld, true) : cw.cfg.kind;
     if (kind == LAUNCH_RYS)
-        return !cw.prec && polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) &&
-               polyeri_rys_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536;
+        return polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) && polyeri_rys_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536;
     const int mode = cw.prec ? MODE_JK32 : MODE_JK;
     if (!polyeri_class_kernel(cw.la, cw.lb, cw.lc, cw.ld, mode)) return false;
     return kind == LAUNCH_SEG;
@@ -545,7 +544,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     if (kind == LAUNCH_SEG && mode == MODE_JK32)
         return fail(POLYERI_E_ARG, "the FP32 segment kernel is switched off (unexplained fault)");
     if (kind == LAUNCH_RYS) {
-        const bool rys_ok = seg_ok && mode == MODE_JK && polyeri_rys_class(la, lb, lc, ld) &&
+        const bool rys_ok = seg_ok && polyeri_rys_class(la, lb, lc, ld) &&
                             polyeri_rys_lds_bytes(la, lb, c->ib.nbf) <= 65536;
         if (!rys_ok) {
             kind = LAUNCH_COOP;
@@ -556,7 +555,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
                 a.dtol = scr->tol; a.Qs = scr->Q; a.psh = scr->psh; a.dsh = scr->dsh; a.nsh = scr->nsh;
                 a.kept = scr->kept;
             }
-            if (polyeri_rys_launch(a, la, lb, lc, ld, cfg.block, cfg.grid, c->stream))
+            if (polyeri_rys_launch(a, mode, la, lb, lc, ld, cfg.block, cfg.grid, c->stream))
                 return fail(POLYERI_E_HIP, "Rys kernel launch failed");
             return 0;
         }
@@ -1054,12 +1053,14 @@ static int build_ctx(polyeri_ctx* c) {
         if ((rc = dalloc(&w.d_rb, l.rb.size()))) return rc;
         HIPCHK(hipMemcpy(w.d_row, l.row.data(), l.row.size() * sizeof(long long), hipMemcpyHostToDevice));
         if (!l.rb.empty()) HIPCHK(hipMemcpy(w.d_rb, l.rb.data(), l.rb.size() * sizeof(int), hipMemcpyHostToDevice));
-        // Segment lists for the FP64 lists of molecular contexts.  FP32 lists
-        // get none: the FP32 segment kernel once stopped on an illegal memory
-        // access ((H2O)4 cc-pVDZ, cut 1e-5) whose cause is not established (a
-        // host replay of its index path over those lists stays in bounds), so
-        // it is not launched.
-        if (!c->pbc && l.nwork > 0 && !l.prec) {
+        // Segment lists for the FP64 lists of molecular contexts, and for the
+        // FP32 lists of the Rys classes (their own kernel).  The light FP32
+        // lists get none: the FP32 light-class segment kernel once stopped on
+        // an illegal memory access ((H2O)4 cc-pVDZ, cut 1e-5) whose cause is
+        // not established (a host replay of its index path over those lists
+        // stays in bounds), so it is not launched, and mixed contexts do not
+        // split the light classes.
+        if (!c->pbc && l.nwork > 0 && (!l.prec || polyeri_rys_class(w.la, w.lb, w.lc, w.ld))) {
             std::vector<SegDesc> segs =
                 make_segments(l, c->pairs, c->class_ids[l.X], c->ket_ids[l.Y], c->bstart[l.Y]);
             const std::vector<SegDesc> segc =
@@ -1272,7 +1273,7 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
                         LaunchCfg y; y.kind = LAUNCH_SEG; y.block = b; y.segv = v; cand.push_back(y);
                     }
         }
-        if (!cw.prec && cw.nseg > 0 && polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) &&
+        if (cw.nseg > 0 && polyeri_rys_class(cw.la, cw.lb, cw.lc, cw.ld) &&
             polyeri_rys_lds_bytes(cw.la, cw.lb, c->ib.nbf) <= 65536)
             for (int v : {0, 1})
                 for (int b : {256, 128, 64}) {

@@ -5,9 +5,10 @@ src/pmp2energy.f90:1-548).
 
 * Lattice J/K cells of a seeded density against the oracle's cells at the
   same cut-offs (tests/golden/pe_jk_cutoff5_10.npz, written by
-  tests/golden/make_pe_fixture.py), 1e-10 absolute.  The oracle screens
-  composite primitive pairs at INTTOL 1e-12, the library contracted shell
-  pairs at Schwarz 1e-12.
+  tests/golden/make_pe_fixture.py), 1e-10 absolute.  The library screens
+  contracted shell pairs at Schwarz 1e-12; the oracle's composite-primitive
+  screen is taken at INTTOL 1e-14, because at 1e-12 it drops lattice-sum
+  terms worth 3e-10 (J) and 4e-10 (K) that the contracted screen keeps.
 * The J/K cells are accumulated in fixed point (integer atomics), so two
   builds of one density are bit-identical, and J/K are linear in P.
 * Two SCF runs give bit-identical energies and iteration counts (every
