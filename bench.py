@@ -203,7 +203,8 @@ def main():
     # summary of this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
     # scaled by the copy kernel; counters cannot be read inside this process)
     traffic, traffic_src = None, None
-    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri", "gc": "k_eri_gc"}[dc["kernel"]]
+    kfn = {"seg": "k_eri_seg", "wave": "k_eri_coop", "thread": "k_eri", "gc": "k_eri_gc",
+           "rys": "k_eri_rys"}.get(dc["kernel"], dc["kernel"])
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_eri_traffic_water16_ccpvdz.json")
     if args.nwater == 16 and args.seed == 0 and os.path.exists(tpath):
         # entries are keyed by kernel symbol and carry their class

@@ -323,6 +323,19 @@ int polyeri_autotune(polyeri_ctx *ctx, int reps);
  * (src/pfourindexdirect.f90:543-690). */
 int polyeri_rys_nodes(int n, double T, double *u, double *w);
 
+/* The Cartesian block (ab|cd) of one primitive quartet of unnormalised
+ * Gaussians (centres xyz[0..11] = A, B, C, D; exponents ex[0..3]), for a
+ * class (la lb|lc ld) that the Rys kernels evaluate ((dp|pp) (dp|dp) (dd|pp)
+ * (dd|ds) (dd|dp) (dd|dd)), computed on the host by the same per-lane code
+ * as the kernels: every lane variant and axis-permutation member, mapped to
+ * the global components.  out: ncart(la) ncart(lb) ncart(lc) ncart(ld)
+ * doubles, nested a, b, c, d, component order xx xy xz yy yz zz / x y z.
+ * Fails when the variants do not produce every integral exactly once.  Host
+ * only (tests of the generated recursions); no counterpart in the reference,
+ * whose integrals come from the Obara-Saika recursion
+ * (src/pfourindexdirect.f90:543-690). */
+int polyeri_rys_block(int la, int lb, int lc, int ld, const double *xyz, const double *ex, double *out);
+
 const char *polyeri_last_error(void);
 
 /* ---- Learned SCF guess: multilayer perceptron on the device ------------

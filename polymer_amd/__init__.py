@@ -79,6 +79,18 @@ def rys_nodes(n, T):
     return u, w
 
 
+def rys_block(la, lb, lc, ld, xyz, ex):
+    """(ab|cd) of one primitive quartet of unnormalised Cartesian Gaussians
+    (centres xyz (4, 3), exponents ex (4)) from the Rys kernels' per-lane
+    code on the host (polyeri_rys_block); shape (ncart(la), .., ncart(ld))."""
+    nc = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
+    out = np.zeros(nc)
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(12)
+    e = np.ascontiguousarray(ex, dtype=np.float64).reshape(4)
+    check(lib().polyeri_rys_block(la, lb, lc, ld, _p(x), _p(e), _p(out)))
+    return out
+
+
 def read_basis(basis, atoms):
     """READ3 (src/pread.f90:584-1482): atoms = [(Z, x, y, z)] in bohr."""
     L = lib()

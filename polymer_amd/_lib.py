@@ -24,6 +24,7 @@ EXPORTS = [
     "polyeri_scf_periodic", "polyeri_mp2_energy_periodic", "polyeri_mp2_energy_qp", "polyeri_store_fourindex_eri", "polyeri_restore_fourindex_eri",
     "polyeri_restore_fourindex_eri_cells", "polyeri_eri_tensor_cells", "polyeri_reduced_cel1",
     "polyeri_rys_nodes",
+    "polyeri_rys_block",
 ]
 
 
@@ -89,6 +90,7 @@ def lib():
     p, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     L.polyeri_basis_read.argtypes = [ctypes.c_char_p, i, p, p] + [p] * 17
     L.polyeri_rys_nodes.argtypes = [i, ctypes.c_double, p, p]
+    L.polyeri_rys_block.argtypes = [i, i, i, i, p, p, p]
     L.polyeri_ctx_create.argtypes = [i, i, p, p, p, p, p, p, p, p, i, p, p, p, ctypes.POINTER(p)]
     L.polyeri_ctx_destroy.argtypes = [p]
     L.polyeri_ctx_create_periodic.argtypes = [i, i, p, p, p, p, p, p, p, p, i, p, p, d, i, i, p, ctypes.POINTER(p)]

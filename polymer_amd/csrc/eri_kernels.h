@@ -1257,6 +1257,10 @@ int polyeri_rys_launch(const EriArgs& a, int mode, int la, int lb, int lc, int l
 // Nodes u[0..n-1] and weights w[0..n-1] of the n-point Rys rule at T, n <= 5
 // (host evaluation of the kernels' tables).
 int polyeri_rys_nodes_host(int n, double T, double* u, double* w);
+// One primitive quartet's Cartesian block through the kernels' per-lane Rys
+// code on the host (0; -1 bad class or pointer; -2 an integral not produced
+// exactly once).
+int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, const double* ex, double* out);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).

@@ -42,7 +42,7 @@ __device__ __forceinline__ const double* rys_herm() {
 // L, in the frame whose local axis l is global axis p[l] (component order
 // xx, xy, xz, yy, yz, zz / x, y, z).
 template <int L>
-__device__ __forceinline__ int glob_comp(int k, const int* p) {
+POLY_HD int glob_comp(int k, const int* p) {
     if (L == 0) return 0;
     if (L == 1) return p[k];
     const int i = k < 3 ? 0 : k < 5 ? 1 : 2;               // local axes of the d component
@@ -52,7 +52,39 @@ __device__ __forceinline__ int glob_comp(int k, const int* p) {
     return a * (5 - a) / 2 + b;
 }
 
-__device__ __forceinline__ double sel3(const double* v, int i) { return i == 0 ? v[0] : i == 1 ? v[1] : v[2]; }
+POLY_HD double sel3(const double* v, int i) { return i == 0 ? v[0] : i == 1 ? v[1] : v[2]; }
+
+// One primitive quartet's contribution to a lane's integrals: the bra
+// primitive pair bq (PA in the lane's frame: PA0..PA2), the ket primitive
+// pair kq, the lane's axis permutation p, A - B and C - D in that frame;
+// tab / herm: the N-point node tables of the class (R::NROOT).
+template <typename R, typename real>
+POLY_HD void rys_prim(const PrimPair& bq, double PA0, double PA1, double PA2, const PrimPair& kq, const int* p,
+                      double ABl0, double ABl1, double ABl2, double CDl0, double CDl1, double CDl2,
+                      const double* __restrict__ tab, const double* __restrict__ herm, real* __restrict__ accr) {
+    const double ze = bq.zeta + kq.zeta;
+    const double rs = rsqrt_fp64(ze);
+    const double ize = rs * rs;
+    const double PQg[3] = {bq.Px - kq.Px, bq.Py - kq.Py, bq.Pz - kq.Pz};
+    const double T = bq.zeta * kq.zeta * ize * (PQg[0] * PQg[0] + PQg[1] * PQg[1] + PQg[2] * PQg[2]);
+    const double pref = bq.K * kq.K * rs;
+    const double QCg[3] = {kq.PAx, kq.PAy, kq.PAz};
+    const double PQ0 = sel3(PQg, p[0]), PQ1 = sel3(PQg, p[1]), PQ2 = sel3(PQg, p[2]);
+    const double QC0 = sel3(QCg, p[0]), QC1 = sel3(QCg, p[1]), QC2 = sel3(QCg, p[2]);
+    const double roz = kq.zeta * ize, roe = bq.zeta * ize;   // rho/zeta, rho/eta
+    const double h2z = 0.5 / bq.zeta, h2e = 0.5 / kq.zeta, h2ze = 0.5 * ize;
+#pragma unroll 1
+    for (int r = 0; r < R::NROOT; ++r) {
+        double u, w;
+        rys_node<R::NROOT>(T, r, tab, herm, u, w);
+        const double uz = roz * u, ue = roe * u;
+        R::template node<real>((real)(w * pref), (real)(u * h2ze), (real)(h2z * (1.0 - uz)),
+                               (real)(h2e * (1.0 - ue)), (real)(PA0 - uz * PQ0), (real)(PA1 - uz * PQ1),
+                               (real)(PA2 - uz * PQ2), (real)(QC0 + ue * PQ0), (real)(QC1 + ue * PQ1),
+                               (real)(QC2 + ue * PQ2), (real)ABl0, (real)ABl1, (real)ABl2, (real)CDl0, (real)CDl1,
+                               (real)CDl2, accr);
+    }
+}
 
 // One variant of the class over the items of one segment that this wave
 // owns: chunk c of the segment's items goes to wave (c + base) % nw.  MODE:
@@ -123,31 +155,9 @@ __device__ __forceinline__ void rys_variant(const EriArgs& g, const SegDesc& seg
             const PrimPair bq = bra[pi];
             const double PAg[3] = {bq.PAx, bq.PAy, bq.PAz};
             const double PA0 = sel3(PAg, p[0]), PA1 = sel3(PAg, p[1]), PA2 = sel3(PAg, p[2]);
-            for (int qi = 0; qi < mk.nprim; ++qi) {
-                const PrimPair kq = kv.load(qi);
-                const double ze = bq.zeta + kq.zeta;
-                const double rs = rsqrt_fp64(ze);
-                const double ize = rs * rs;
-                const double PQg[3] = {bq.Px - kq.Px, bq.Py - kq.Py, bq.Pz - kq.Pz};
-                const double T = bq.zeta * kq.zeta * ize * (PQg[0] * PQg[0] + PQg[1] * PQg[1] + PQg[2] * PQg[2]);
-                const double pref = bq.K * kq.K * rs;
-                const double QCg[3] = {kq.PAx, kq.PAy, kq.PAz};
-                const double PQ0 = sel3(PQg, p[0]), PQ1 = sel3(PQg, p[1]), PQ2 = sel3(PQg, p[2]);
-                const double QC0 = sel3(QCg, p[0]), QC1 = sel3(QCg, p[1]), QC2 = sel3(QCg, p[2]);
-                const double roz = kq.zeta * ize, roe = bq.zeta * ize;   // rho/zeta, rho/eta
-                const double h2z = 0.5 / bq.zeta, h2e = 0.5 / kq.zeta, h2ze = 0.5 * ize;
-#pragma unroll 1
-                for (int r = 0; r < R::NROOT; ++r) {
-                    double u, w;
-                    rys_node<R::NROOT>(T, r, rys_tab<R::NROOT>(), rys_herm<R::NROOT>(), u, w);
-                    const double uz = roz * u, ue = roe * u;
-                    R::template node<real>(
-                        (real)(w * pref), (real)(u * h2ze), (real)(h2z * (1.0 - uz)), (real)(h2e * (1.0 - ue)),
-                        (real)(PA0 - uz * PQ0), (real)(PA1 - uz * PQ1), (real)(PA2 - uz * PQ2), (real)(QC0 + ue * PQ0),
-                        (real)(QC1 + ue * PQ1), (real)(QC2 + ue * PQ2), (real)ABl0, (real)ABl1, (real)ABl2,
-                        (real)CDl0, (real)CDl1, (real)CDl2, accr);
-                }
-            }
+            for (int qi = 0; qi < mk.nprim; ++qi)
+                rys_prim<R, real>(bq, PA0, PA1, PA2, kv.load(qi), p, ABl0, ABl1, ABl2, CDl0, CDl1, CDl2,
+                                  rys_tab<R::NROOT>(), rys_herm<R::NROOT>(), accr);
         }
         // digestion of the lane's integrals v(a, b, c, d) (local components)
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);
@@ -343,5 +353,82 @@ int polyeri_rys_nodes_host(int n, double T, double* u, double* w) {
         default: return -1;
         }
     }
+    return 0;
+}
+
+// Host evaluation of one primitive quartet's Cartesian block through the
+// same per-lane code the kernels run (every variant, every member), for the
+// CPU tests of the generated recursions.
+static const double* host_tab(int n) {
+    return n == 1 ? h_rys_tab1 : n == 2 ? h_rys_tab2 : n == 3 ? h_rys_tab3 : n == 4 ? h_rys_tab4 : h_rys_tab5;
+}
+static const double* host_herm(int n) {
+    return n == 1 ? h_rys_herm1 : n == 2 ? h_rys_herm2 : n == 3 ? h_rys_herm3 : n == 4 ? h_rys_herm4 : h_rys_herm5;
+}
+
+template <int LA, int LB, int LC, int LD, int V>
+static void rys_block_host_var(const PrimPair& bq, const PrimPair& kq, const double* AB, const double* CD, double* out,
+                               int* hits) {
+    if constexpr (V < RysClass<LA, LB, LC, LD>::NVAR) {
+        using R = RysVar<LA, LB, LC, LD, V>;
+        constexpr int NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+        for (int m = 0; m < R::NMEMB; ++m) {
+            int p[3];
+            R::perm(m, p[0], p[1], p[2]);
+            const double PAg[3] = {bq.PAx, bq.PAy, bq.PAz};
+            double acc[R::NOUT];
+            for (int i = 0; i < R::NOUT; ++i) acc[i] = 0.0;
+            rys_prim<R, double>(bq, sel3(PAg, p[0]), sel3(PAg, p[1]), sel3(PAg, p[2]), kq, p, sel3(AB, p[0]),
+                                sel3(AB, p[1]), sel3(AB, p[2]), sel3(CD, p[0]), sel3(CD, p[1]), sel3(CD, p[2]),
+                                host_tab(R::NROOT), host_herm(R::NROOT), acc);
+            for (int ia = 0; ia < R::NA; ++ia)
+                for (int ib = 0; ib < R::NB; ++ib)
+                    for (int ic = 0; ic < R::NC; ++ic)
+                        for (int id = 0; id < R::ND; ++id) {
+                            const int o = ((glob_comp<LA>(R::CA[ia], p) * NB + glob_comp<LB>(R::CB[ib], p)) * NC +
+                                           glob_comp<LC>(R::CC[ic], p)) * ND + glob_comp<LD>(R::CD[id], p);
+                            out[o] = acc[((ia * R::NB + ib) * R::NC + ic) * R::ND + id];
+                            ++hits[o];
+                        }
+        }
+        rys_block_host_var<LA, LB, LC, LD, V + 1>(bq, kq, AB, CD, out, hits);
+    }
+}
+
+// Primitive pair of exponents a (centre A) and b (centre B), unit
+// coefficients: K = sqrt(2) pi^{5/4} exp(-ab/zeta |AB|^2) / zeta.
+static PrimPair host_prim_pair(double a, const double* A, double b, const double* B) {
+    PrimPair q;
+    q.zeta = a + b;
+    const double d2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+    q.K = std::sqrt(2.0) * std::pow(3.141592653589793, 1.25) * std::exp(-a * b / q.zeta * d2) / q.zeta;
+    q.Px = (a * A[0] + b * B[0]) / q.zeta;
+    q.Py = (a * A[1] + b * B[1]) / q.zeta;
+    q.Pz = (a * A[2] + b * B[2]) / q.zeta;
+    q.PAx = q.Px - A[0];
+    q.PAy = q.Py - A[1];
+    q.PAz = q.Pz - A[2];
+    return q;
+}
+
+int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, const double* ex, double* out) {
+    if (!polyeri_rys_class(la, lb, lc, ld) || !xyz || !ex || !out) return -1;
+    const PrimPair bq = host_prim_pair(ex[0], xyz, ex[1], xyz + 3);
+    const PrimPair kq = host_prim_pair(ex[2], xyz + 6, ex[3], xyz + 9);
+    const double AB[3] = {xyz[0] - xyz[3], xyz[1] - xyz[4], xyz[2] - xyz[5]};
+    const double CD[3] = {xyz[6] - xyz[9], xyz[7] - xyz[10], xyz[8] - xyz[11]};
+    const int nout = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+    std::vector<int> hits(nout, 0);
+    switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 2111: rys_block_host_var<2, 1, 1, 1, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    case 2121: rys_block_host_var<2, 1, 2, 1, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    case 2211: rys_block_host_var<2, 2, 1, 1, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    case 2220: rys_block_host_var<2, 2, 2, 0, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    case 2221: rys_block_host_var<2, 2, 2, 1, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    case 2222: rys_block_host_var<2, 2, 2, 2, 0>(bq, kq, AB, CD, out, hits.data()); break;
+    }
+    // every integral of the block produced by exactly one (variant, member, output)
+    for (int i = 0; i < nout; ++i)
+        if (hits[i] != 1) return -2;
     return 0;
 }

@@ -1208,6 +1208,13 @@ int polyeri_rys_nodes(int n, double T, double* u, double* w) {
     return polyeri_rys_nodes_host(n, T, u, w) ? fail(POLYERI_E_ARG, "polyeri_rys_nodes: bad argument") : 0;
 }
 
+int polyeri_rys_block(int la, int lb, int lc, int ld, const double* xyz, const double* ex, double* out) {
+    const int rc = polyeri_rys_block_host(la, lb, lc, ld, xyz, ex, out);
+    if (rc == -1) return fail(POLYERI_E_ARG, "polyeri_rys_block: not a Rys class, or a null pointer");
+    if (rc) return fail(POLYERI_E_ARG, "polyeri_rys_block: the lane variants do not cover the block exactly once");
+    return 0;
+}
+
 int polyeri_basis_read(const char* path, int natom, const int* iatom, const double* xyz, int* npgs,
                        int* ncgs, int* npshell, int* pax, int* pay, int* paz, int* pgs_atom,
                        double* pgsx, double* pgsy, double* pgsz, double* zt, double* cc, double* cgsx,
