@@ -158,6 +158,20 @@ int polyeri_restore_fourindex_eri(const char *path, int ncgs, double *feri, long
  * cel1; a cell outside the ranges is "INTEGRAL FILE HAS BEEN DEGRADED". */
 int polyeri_restore_fourindex_eri_cells(const char *path, int ncgs, int cel1, int cel2, double *t_c,
                                         long long *restored);
+/* DIRECT .F. (RESTORE_FOURINDEX_ERI as the SCF's J/K, src/pfourindexstore.f90:
+ * 805-1004, called at src/pselfconsistentfield.f90:45): appends the records
+ * of the .ao file at path to the context (kept in device memory; call once
+ * per file .ao.00000, .ao.00001, ...) and switches every J/K build of the
+ * context -- polyeri_jk_build*, polyeri_fock_build_device, polyeri_scf,
+ * polyeri_jk_build_cells, polyeri_scf_periodic -- and the molecular MP2
+ * tensor to them: C(I,J,Q1) += P(K,L,Q2) v, X(I,L,Q3+Q2) += P(K,J,Q1-Q3) v
+ * per record.  A periodic context reads the cells with its own CUTOFF1 /
+ * CUTOFF2 and rejects a Q1 or Q2 beyond REDUCED_CEL1X ("INTEGRAL FILE HAS
+ * BEEN DEGRADED"). restored = the records of this file. */
+int polyeri_load_fourindex_eri(polyeri_ctx *ctx, const char *path, long long *restored);
+/* direct = 1: J/K builds evaluate the integrals again (the default);
+ * direct = 0: they use the loaded records (error if none are loaded). */
+int polyeri_set_direct(polyeri_ctx *ctx, int direct);
 
 typedef struct polyeri_scf_opts {
     double scfconv;     /* DOPTN(14) RMS density change                     */

@@ -379,6 +379,19 @@ class Context:
         pair."""
         check(lib().polyeri_set_general_contraction(self.h, int(bool(on))))
 
+    def load_fourindex_eri(self, path):
+        """DIRECT .F.: append the records of an .ao file (store_fourindex_eri)
+        and switch this context's J/K builds, SCFs and MP2 tensor to them.
+        Returns the number of records read."""
+        n = ctypes.c_longlong()
+        check(lib().polyeri_load_fourindex_eri(self.h, str(path).encode(), ctypes.byref(n)))
+        return n.value
+
+    def set_direct(self, direct):
+        """True: evaluate the integrals in every J/K build (default); False:
+        use the loaded .ao records."""
+        check(lib().polyeri_set_direct(self.h, int(bool(direct))))
+
     def autotune(self, reps=2):
         """Time every work list with each candidate launch shape and keep the
         fastest (results do not change)."""

@@ -1261,6 +1261,18 @@ int polyeri_rys_nodes_host(int n, double T, double* u, double* w);
 // code on the host (0; -1 bad class or pointer; -2 an integral not produced
 // exactly once).
 int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, const double* ex, double* out);
+// One record of the reference's .ao files (kstored.hip): packed cells,
+// packed 1-based indices I J K L, the integral.
+struct StoredEri {
+    int32_t cells, ijkl;
+    double v;
+};
+// J/K blocks (reference layout, 2 cel1 + 1 cells or 1) from n records.
+int polyeri_jk_stored_launch(const StoredEri* rec, long long n, int nc, int cel1, int cel2, const double* P, double* J,
+                             double* K, hipStream_t s);
+// Internal-function AO tensor of a molecule from n records.
+int polyeri_tensor_stored_launch(const StoredEri* rec, long long n, const int* inv, const double* scale, int nbf,
+                                 double* T, hipStream_t s);
 // Classes with more than this many integrals per quartet run cooperatively.
 #define POLYERI_COOP_MIN_INTEGRALS 108
 // Kernel of class (la lb|lc ld) and mode, or nullptr (generated getters).

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -359,6 +360,7 @@ struct ClassPairWork {
         int kind = 0;   // 0: group pairs on both sides (k_eri_gc); 1: shell-pair bra, group-pair kets (k_eri_gck)
     } gc;
     int gc_on = 0, gc_block = 256, gc_segv = 0;
+    double est_ms = 0;     // launch time of the chosen form (autotuner), else 0: model FLOPs order the lists
 };
 
 struct polyeri_ctx {
@@ -422,7 +424,7 @@ struct polyeri_ctx {
     double* d_dsh = nullptr;
     unsigned* d_counts = nullptr;
     std::vector<unsigned> h_counts;
-    bool screened_last = false;
+    bool screened_last = false, stored_last = false;
     long long last_quartets = 0;
     // J/K builds replayed from captured HIP graphs, one per (density, J, K,
     // screening) combination; dropped when the launch shapes change
@@ -434,6 +436,16 @@ struct polyeri_ctx {
     };
     bool use_graph = true;
     std::vector<GraphEntry> graphs;
+    // captured builds fork the class launches into kBranches streams (the
+    // context's own and kBranches - 1 side streams), joined before unpacking
+    hipStream_t side[3] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {};
+    // stored integrals (DIRECT .F.): .ao records in HBM, used by the J/K
+    // builds and the MP2 tensor while use_stored; reference -> internal map
+    StoredEri* d_ao = nullptr;
+    long long n_ao = 0;
+    int* d_inv = nullptr;
+    bool use_stored = false;
     bool tuned = false;
     // periodic lattice sums along x (HELIX = 0): cells of the density and
     // of J/K in [-cel1, cel1], Coulomb sum over Q3 in [-cel2, cel2], pair
@@ -674,6 +686,24 @@ static hipStream_t shared_stream(int device, int* rc) {
     return streams[device];
 }
 
+// Side streams of the captured J/K builds (default priority, shared by the
+// contexts of a device like the main stream).
+static const int kBranches = 4;
+static int shared_side_streams(int device, hipStream_t* out) {
+    static std::mutex mu;
+    static std::vector<std::array<hipStream_t, kBranches - 1>> streams;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)streams.size() <= device) streams.resize(device + 1, std::array<hipStream_t, kBranches - 1>{});
+    for (int i = 0; i < kBranches - 1; ++i) {
+        if (!streams[device][i]) {
+            hipError_t e = hipStreamCreateWithFlags(&streams[device][i], hipStreamNonBlocking);
+            if (e != hipSuccess) return fail(POLYERI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        out[i] = streams[device][i];
+    }
+    return 0;
+}
+
 static void free_ctx(polyeri_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -705,6 +735,10 @@ static void free_ctx(polyeri_ctx* c) {
     for (auto e : c->ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    (void)hipFree(c->d_ao); (void)hipFree(c->d_inv);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    for (auto e : c->ev_join)
+        if (e) (void)hipEventDestroy(e);
     if (c->blas) rocblas_destroy_handle(c->blas);
     drop_graphs(c);
     delete c;
@@ -899,6 +933,9 @@ static int build_ctx(polyeri_ctx* c) {
     if (src) return src;
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    if ((src = shared_side_streams(c->device, c->side))) return src;
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    for (auto& e : c->ev_join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     int np = (int)c->pairs.size(), nbf = c->ib.nbf, nc = c->rb.ncgs;
     std::vector<PairMeta> meta(np);
     std::vector<double> ab(4 * (size_t)np);
@@ -1329,7 +1366,9 @@ int polyeri_autotune(polyeri_ctx* ctx, int reps) {
             if (rc) break;
             if (gbest < best) { cw.gc_on = 1; cw.gc_block = gblock; }
             else if (best < 1e300) cw.gc_on = 0;
+            if (gbest < best) best = gbest;
         }
+        cw.est_ms = best < 1e300 ? best / std::max(reps, 1) : 0.0;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1402,7 +1441,8 @@ __global__ void k_fill_u32(unsigned* p, size_t n) {
 // quartets whose density-weighted Schwarz bound is below dens_tol
 // (kscreen.hip); the surviving items of every list are compacted on the
 // device and the class kernels stride over them.
-static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol) {
+static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol,
+                      bool capturing = false) {
     int nbf = c->ib.nbf, nc = c->rb.ncgs;
     size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
     const bool screen = dens_tol > 0.0;
@@ -1427,23 +1467,56 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
                 return fail(POLYERI_E_HIP, "screening launch failed");
         }
     }
-    for (size_t w = 0; w < c->work.size(); ++w) {
+    // Branches: the class launches only add into J/K (the additions commute),
+    // so a captured build runs them on kBranches streams, the lists assigned
+    // largest first to the least loaded branch (autotuned times, else model
+    // FLOPs), and joins them before unpacking.  Profiled builds stay on one
+    // stream (per-list events between the launches).
+    const bool fork = capturing && !c->profiling && c->work.size() > 1;
+    std::vector<int> branch(c->work.size(), 0);
+    hipStream_t main = c->stream;
+    if (fork) {
+        std::vector<int> order(c->work.size());
+        for (size_t w = 0; w < order.size(); ++w) order[w] = (int)w;
+        const bool timed = std::all_of(c->work.begin(), c->work.end(),
+                                       [](const ClassPairWork& x) { return x.nwork <= 0 || x.est_ms > 0; });
+        auto cost = [&](int w) { return timed ? c->work[w].est_ms : c->work[w].flops; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        double load[kBranches] = {};
+        for (int w : order) {
+            int bmin = 0;
+            for (int i = 1; i < kBranches; ++i)
+                if (load[i] < load[bmin]) bmin = i;
+            branch[w] = bmin;
+            load[bmin] += cost(w);
+        }
+        HIPCHK(hipEventRecord(c->ev_fork, main));
+        for (int i = 0; i < kBranches - 1; ++i) HIPCHK(hipStreamWaitEvent(c->side[i], c->ev_fork, 0));
+    }
+    int rc = 0;
+    for (size_t w = 0; w < c->work.size() && !rc; ++w) {
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
+        c->stream = branch[w] ? c->side[branch[w] - 1] : main;   // every launch below goes to c->stream
         const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
         if (list_uses_gc(cw)) {
-            const int rc = launch_gc(c, cw, cw.gc_block, screen ? &scr : nullptr);
-            if (rc) return rc;
-            continue;
-        }
-        const bool seg = list_uses_seg(c, cw);
-        int rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
+            rc = launch_gc(c, cw, cw.gc_block, screen ? &scr : nullptr);
+        } else {
+            const bool seg = list_uses_seg(c, cw);
+            rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
                               screen && !seg ? cw.d_items : nullptr, screen && !seg ? c->d_counts + w : nullptr,
                               cw.cfg, cw.d_bsoa, cw.cfg.segv ? cw.d_seg + cw.nseg : cw.d_seg,
                               cw.cfg.segv ? cw.nsegc : cw.nseg, screen && seg ? &scr : nullptr);
-        if (rc) return rc;
+        }
+        c->stream = main;
     }
+    if (fork)   // join every branch, also after a failed launch (capture must end joined)
+        for (int i = 0; i < kBranches - 1; ++i) {
+            HIPCHK(hipEventRecord(c->ev_join[i], c->side[i]));
+            HIPCHK(hipStreamWaitEvent(main, c->ev_join[i], 0));
+        }
+    if (rc) return rc;
     if (c->profiling) HIPCHK(hipEventRecord(c->ev[c->work.size()], c->stream));
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_jc, c2, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_kc, c2, 0.0);
@@ -1456,8 +1529,28 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
 // One J/K build.  Without per-class profiling the kernel sequence is
 // captured once per (density, J, K, screening) combination into a HIP graph
 // and replayed; the timing events stay outside the graph.
+// J/K from the stored records (DIRECT .F.) into the reference-layout
+// blocks d_jc / d_kc (ncell blocks: 1 for a molecule).
+static int jk_stored(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc) {
+    const int nc = c->rb.ncgs, ncell = c->pbc ? 2 * c->cel1 + 1 : 1;
+    const size_t c2 = (size_t)nc * nc * ncell;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_jc, c2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c2)), dim3(256), 0, c->stream, d_kc, c2, 0.0);
+    HIPCHK(hipGetLastError());
+    if (polyeri_jk_stored_launch(c->d_ao, c->n_ao, nc, c->pbc ? c->cel1 : 0, c->pbc ? c->cel2 : 0, d_pc, d_jc, d_kc,
+                                 c->stream))
+        return fail(POLYERI_E_HIP, "stored-integral J/K launch failed");
+    c->screened_last = false;
+    c->stored_last = true;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return 0;
+}
+
 static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc, double dens_tol = 0.0) {
     if (c->pbc) return fail(POLYERI_E_ARG, "periodic context: use polyeri_jk_build_cells");
+    if (c->use_stored) return jk_stored(c, d_pc, d_jc, d_kc);   // every stored record, no density screening
+    c->stored_last = false;
     const bool screen = dens_tol > 0.0;
     if (screen)
         for (ClassPairWork& w : c->work)
@@ -1476,7 +1569,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
                 c->graphs.erase(c->graphs.begin());
             }
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = jk_enqueue(c, d_pc, d_jc, d_kc, dens_tol);
+            int rc = jk_enqueue(c, d_pc, d_jc, d_kc, dens_tol, true);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -1503,6 +1596,10 @@ static int jk_collect_timing(polyeri_ctx* c) {
     HIPCHK(hipEventSynchronize(c->ev1));
     c->last_jk_ms = event_ms(c->ev0, c->ev1);
     c->last_quartets = 0;
+    if (c->stored_last) {   // records digested (no class launches)
+        c->last_quartets = c->n_ao;
+        return 0;
+    }
     for (size_t w = 0; w < c->work.size(); ++w)
         c->last_quartets += c->screened_last ? (long long)c->h_counts[w] : c->work[w].nwork;
     if (c->profiling)
@@ -1540,6 +1637,8 @@ static int jk_build_host(polyeri_ctx* ctx, const double* p_c, double dens_tol, d
 // Periodic J/K of the contracted density cells (reference layout, cell Q at
 // block Q + cel1) into the J/K cells.
 static int jk_device_cells(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d_kc) {
+    if (c->use_stored) return jk_stored(c, d_pc, d_jc, d_kc);
+    c->stored_last = false;
     const int nbf = c->ib.nbf, nc = c->rb.ncgs, ncell = 2 * c->cel1 + 1;
     const size_t n2 = (size_t)nbf * nbf, c2 = (size_t)nc * nc;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1670,6 +1769,11 @@ static int tensor_device(polyeri_ctx* c, double* d_t) {
     size_t n4 = n * n * n * n;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n4)), dim3(256), 0, c->stream, d_t, n4, 0.0);
     HIPCHK(hipGetLastError());
+    if (c->use_stored) {   // DIRECT .F.: the tensor from the stored records
+        if (polyeri_tensor_stored_launch(c->d_ao, c->n_ao, c->d_inv, c->d_scale, (int)n, d_t, c->stream))
+            return fail(POLYERI_E_HIP, "stored-integral tensor launch failed");
+        return 0;
+    }
     for (const ClassPairWork& cw : c->work) {
         int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
                               cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr,
@@ -2968,19 +3072,17 @@ int polyeri_restore_fourindex_eri_cells(const char* path, int ncgs, int cel1, in
     return restore_ao(path, ncgs, cel1, cel2, t_c, restored);
 }
 
-// cel1 < 0: molecular file (every record in cell 0); else Q1, Q2 in
-// [-cel1, cel1], Q3 in [-cel2, cel2] into (2 cel1 + 1)^2 (2 cel2 + 1) blocks.
-static int restore_ao(const char* path, int ncgs, int cel1, int cel2, double* feri, long long* restored) {
-    if (!path || !feri || ncgs <= 0 || ncgs >= 256) return fail(POLYERI_E_ARG, "polyeri_restore_fourindex_eri: bad argument");
+// Records of one .ao file, appended to out.  cel1 < 0: a molecular file
+// (every record in cell 0); else the file's cells are packed with cel1 /
+// cel2 and must satisfy Q1, Q2 in [-q12, q12], Q3 in [-cel2, cel2] (the
+// reference's degraded-file test, src/pfourindexstore.f90:901-906), and
+// every index lies in 1 .. ncgs.
+static int read_ao_records(const char* path, int ncgs, int cel1, int q12, int cel2, std::vector<StoredEri>& out) {
     const bool cells = cel1 >= 0;
-    const size_t n1 = cells ? 2 * cel1 + 1 : 1, n3 = cells ? 2 * cel2 + 1 : 1;
     FILE* f = fopen(path, "rb");
     if (!f) return fail(POLYERI_E_ARG, std::string("cannot open ") + path);
-    const size_t nc = ncgs, nc4 = nc * nc * nc * nc;
-    std::fill(feri, feri + nc4 * n1 * n1 * n3, 0.0);
     std::vector<int32_t> x(kAoCache), y(kAoCache), z(kAoCache), ijkl(kAoCache);
     std::vector<double> v(kAoCache);
-    long long count = 0;
     int rc = 0;
     for (;;) {
         int32_t len = 0, len2 = 0;
@@ -2993,26 +3095,89 @@ static int restore_ao(const char* path, int ncgs, int cel1, int cel2, double* fe
         if (!ok) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
         for (int e = 0; e < kAoCache; ++e) {
             if (x[e] == -1) break;
-            size_t blk = 0;
             if (!cells) {
                 if (x[e] != 0 || y[e] != 0 || z[e] != 0) { rc = fail(POLYERI_E_ARG, "periodic .ao records in a molecular read"); break; }
             } else {
                 // src/pmp2energy.f90:221-252: unpack and range-check the cells
                 const int q1 = ((x[e] >> 16) & 255) - cel1, q2 = ((x[e] >> 8) & 255) - cel1, q3 = (x[e] & 255) - cel2;
-                if (y[e] != 0 || z[e] != 0 || q1 < -cel1 || q1 > cel1 || q2 < -cel1 || q2 > cel1 || q3 < -cel2 ||
-                    q3 > cel2) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
-                blk = (((size_t)(q1 + cel1) * n1 + (q2 + cel1)) * n3 + (q3 + cel2)) * nc4;
+                if ((x[e] >> 24) != 0 || y[e] != 0 || z[e] != 0 || q1 < -q12 || q1 > q12 || q2 < -q12 || q2 > q12 ||
+                    q3 < -cel2 || q3 > cel2) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
             }
-            const size_t i = ((ijkl[e] >> 24) & 255) - 1, j = ((ijkl[e] >> 16) & 255) - 1, k = ((ijkl[e] >> 8) & 255) - 1,
-                         l = (ijkl[e] & 255) - 1;
-            if (i >= nc || j >= nc || k >= nc || l >= nc) { rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED"); break; }
-            feri[blk + i + nc * (j + nc * (k + nc * l))] = v[e];
-            ++count;
+            const int i = ((ijkl[e] >> 24) & 255), j = ((ijkl[e] >> 16) & 255), k = ((ijkl[e] >> 8) & 255),
+                      l = (ijkl[e] & 255);
+            if (i < 1 || j < 1 || k < 1 || l < 1 || i > ncgs || j > ncgs || k > ncgs || l > ncgs) {
+                rc = fail(POLYERI_E_ARG, "INTEGRAL FILE HAS BEEN DEGRADED");
+                break;
+            }
+            out.push_back(StoredEri{x[e], ijkl[e], v[e]});
         }
         if (rc) break;
     }
     fclose(f);
+    return rc;
+}
+
+// cel1 < 0: molecular file (every record in cell 0); else Q1, Q2 in
+// [-cel1, cel1], Q3 in [-cel2, cel2] into (2 cel1 + 1)^2 (2 cel2 + 1) blocks.
+static int restore_ao(const char* path, int ncgs, int cel1, int cel2, double* feri, long long* restored) {
+    if (!path || !feri || ncgs <= 0 || ncgs >= 256) return fail(POLYERI_E_ARG, "polyeri_restore_fourindex_eri: bad argument");
+    const bool cells = cel1 >= 0;
+    const size_t n1 = cells ? 2 * cel1 + 1 : 1, n3 = cells ? 2 * cel2 + 1 : 1;
+    const size_t nc = ncgs, nc4 = nc * nc * nc * nc;
+    std::vector<StoredEri> rec;
+    int rc = read_ao_records(path, ncgs, cel1, cel1, cel2, rec);
     if (rc) return rc;
-    if (restored) *restored = count;
+    std::fill(feri, feri + nc4 * n1 * n1 * n3, 0.0);
+    for (const StoredEri& r : rec) {
+        size_t blk = 0;
+        if (cells) {
+            const int q1 = ((r.cells >> 16) & 255) - cel1, q2 = ((r.cells >> 8) & 255) - cel1, q3 = (r.cells & 255) - cel2;
+            blk = (((size_t)(q1 + cel1) * n1 + (q2 + cel1)) * n3 + (q3 + cel2)) * nc4;
+        }
+        const size_t i = ((r.ijkl >> 24) & 255) - 1, j = ((r.ijkl >> 16) & 255) - 1, k = ((r.ijkl >> 8) & 255) - 1,
+                     l = (r.ijkl & 255) - 1;
+        feri[blk + i + nc * (j + nc * (k + nc * l))] = r.v;
+    }
+    if (restored) *restored = (long long)rec.size();
+    return 0;
+}
+
+// DIRECT .F.: the records of an .ao file into the context (appended to
+// those already loaded), and the context's J/K builds switched to them.
+int polyeri_load_fourindex_eri(polyeri_ctx* ctx, const char* path, long long* restored) {
+    if (!ctx || !path) return fail(POLYERI_E_ARG, "polyeri_load_fourindex_eri: null argument");
+    const int nc = ctx->rb.ncgs;
+    if (nc >= 256) return fail(POLYERI_E_ARG, "polyeri_load_fourindex_eri: the .ao format holds at most 255 functions");
+    if (ctx->pbc && (2 * ctx->cel1 + 1 > 255 || 2 * ctx->cel2 + 1 > 255))
+        return fail(POLYERI_E_ARG, "polyeri_load_fourindex_eri: cell range beyond the .ao format");
+    std::vector<StoredEri> rec;
+    int rc = read_ao_records(path, nc, ctx->pbc ? ctx->cel1 : -1, ctx->pbc ? ctx->rc1 : 0, ctx->pbc ? ctx->cel2 : 0,
+                             rec);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t tot = (size_t)ctx->n_ao + rec.size();
+    StoredEri* d = nullptr;
+    if ((rc = dalloc(&d, tot))) return rc;
+    if (ctx->n_ao) HIPCHK(hipMemcpy(d, ctx->d_ao, ctx->n_ao * sizeof(StoredEri), hipMemcpyDeviceToDevice));
+    if (!rec.empty())
+        HIPCHK(hipMemcpy(d + ctx->n_ao, rec.data(), rec.size() * sizeof(StoredEri), hipMemcpyHostToDevice));
+    (void)hipFree(ctx->d_ao);
+    ctx->d_ao = d;
+    ctx->n_ao = (long long)tot;
+    if (!ctx->d_inv) {
+        std::vector<int> inv(nc);
+        for (int c = 0; c < nc; ++c) inv[c] = ctx->ib.int_of_ref[c];
+        if ((rc = dalloc(&ctx->d_inv, nc))) return rc;
+        HIPCHK(hipMemcpy(ctx->d_inv, inv.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    }
+    ctx->use_stored = true;
+    if (restored) *restored = (long long)rec.size();
+    return 0;
+}
+
+int polyeri_set_direct(polyeri_ctx* ctx, int direct) {
+    if (!ctx) return fail(POLYERI_E_ARG, "null context");
+    if (!direct && ctx->n_ao <= 0) return fail(POLYERI_E_ARG, "polyeri_set_direct: no stored integrals loaded");
+    ctx->use_stored = !direct;
     return 0;
 }

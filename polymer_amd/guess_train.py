@@ -2,11 +2,12 @@
 
     python -m polymer_amd.guess_train --n 2048 --mode train --out polymer_amd/guess_water_631gss.npz
     python -m polymer_amd.guess_train --n 2048 --mode eval --out polymer_amd/guess_water_631gss.npz
+    python -m polymer_amd.guess_train --basis cc-pVDZ --n 1024 --out polymer_amd/guess_water_ccpvdz.npz
 
 Data: n seeded geometries of (H2O)4 .. (H2O)8 (seed s has 4 + s % 5
 waters), each built like the benchmark clusters (rigid placement, random
 orientations, O-O >= 2.6 A, O inside a ball of radius 2.9 (n/4)^(1/3) A)
-with 0.02 A bond and 2 degree angle jitter, 6-31G**.  Converged densities
+with 0.02 A bond and 2 degree angle jitter, 6-31G** (or --basis cc-pVDZ).  Converged densities
 come from this library's FP64 SCF (RMS density change 1e-8).  The first
 80 % of the seeds train; the last 20 % are held out.
 
@@ -156,8 +157,9 @@ def main():
     ap.add_argument("--core-subset", type=int, default=100)
     ap.add_argument("--eval-limit", type=int, default=None)
     ap.add_argument("--mode", choices=("all", "train", "eval"), default="all")
+    ap.add_argument("--basis", default="6-31Gss", choices=("6-31Gss", "cc-pVDZ"))
     args = ap.parse_args()
-    basis = "6-31Gss"
+    basis = args.basis
     ntrain = int(round(0.8 * args.n))
     if args.mode in ("all", "train"):
         train(args, basis, ntrain)
