@@ -92,6 +92,15 @@ int polyeri_jk_build(polyeri_ctx *ctx, const double *p_c, double *c_c, double *x
 /* Same on device pointers (ncgs x ncgs each), enqueued on the context's
  * stream; returns after the kernels are enqueued.  polyeri_sync waits. */
 int polyeri_jk_build_device(polyeri_ctx *ctx, const double *d_p_c, double *d_c_c, double *d_x_c);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The same
+ * molecular J/K build as polyeri_jk_build_device, but with the density
+ * staging and the J/K accumulation copies in a fixed, library-wide device
+ * workspace of 2^22 doubles (32 MiB per device, held until the process
+ * ends) and eager launches: it calls no allocator.  Limit: nbf <= 1182
+ * (larger contexts get POLYERI_E_ARG).  Every call on a device shares that
+ * device's workspace, so calls that overlap on one device race. */
+int polyeri_jk_build_device_noalloc(polyeri_ctx *ctx, const double *d_p_c, double *d_c_c, double *d_x_c);
 
 /* polyeri_jk_build with density screening (no counterpart in the reference,
  * whose screen is IJIJ*KLKL < INTTOL only, src/pfourindexdirect.f90:529):

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libpolyeri.so")
 # every entry point include/polyeri.h declares
 EXPORTS = [
     "polyeri_basis_read", "polyeri_ctx_create", "polyeri_ctx_destroy", "polyeri_jk_build",
-    "polyeri_jk_build_device", "polyeri_sync", "polyeri_one_electron", "polyeri_eri_tensor",
+    "polyeri_jk_build_device", "polyeri_jk_build_device_noalloc", "polyeri_sync", "polyeri_one_electron", "polyeri_eri_tensor",
     "polyeri_scf", "polyeri_mp2_energy", "polyeri_get_stats", "polyeri_set_profiling",
     "polyeri_last_error", "polyeri_device_alloc", "polyeri_device_free", "polyeri_memcpy",
     "polyeri_fock_build_device", "polyeri_device_copy", "polyeri_jk_build_screened",
@@ -113,6 +113,7 @@ def lib():
     L.polyeri_set_graph.argtypes = [p, ctypes.c_int]
     L.polyeri_set_general_contraction.argtypes = [p, ctypes.c_int]
     L.polyeri_jk_build_device.argtypes = [p, p, p, p]
+    L.polyeri_jk_build_device_noalloc.argtypes = [p, p, p, p]
     L.polyeri_sync.argtypes = [p]
     L.polyeri_one_electron.argtypes = [p, p, p, p, p]
     L.polyeri_eri_tensor.argtypes = [p, p]

@@ -446,6 +446,7 @@ struct polyeri_ctx {
     long long n_ao = 0;
     int* d_inv = nullptr;
     bool use_stored = false;
+    double* d_noalloc = nullptr;   // this device's address of `scratch`
     bool tuned = false;
     // periodic lattice sums along x (HELIX = 0): cells of the density and
     // of J/K in [-cel1, cel1], Coulomb sum over Q3 in [-cel2, cel2], pair
@@ -684,6 +685,31 @@ static hipStream_t shared_stream(int device, int* rc) {
     }
     *rc = 0;
     return streams[device];
+}
+
+// Fixed device workspace of polyeri_jk_build_device_noalloc: the packed
+// density and the per-XCD J/K accumulation copies of one build (instead of
+// the context's own), so that entry point allocates nothing.  Each device
+// holds its own copy of the array; host code uses only the address that
+// hipGetSymbolAddress returns on that device (kept per device below).
+static constexpr size_t kNoAllocDoubles = size_t(1) << 22;   // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static int noalloc_scratch(int device, double** out) {
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (device < 0) return fail(POLYERI_E_ARG, "bad device");
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {   // the caller has made `device` current
+        double* p = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(POLYERI_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return 0;
 }
 
 // Side streams of the captured J/K builds (default priority, shared by the
@@ -1611,6 +1637,41 @@ int polyeri_jk_build_device(polyeri_ctx* ctx, const double* d_p_c, double* d_c_c
     if (!ctx || !d_p_c || !d_c_c || !d_x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build_device: null argument");
     HIPCHK(hipSetDevice(ctx->device));
     return jk_device(ctx, d_p_c, d_c_c, d_x_c);
+}
+
+// The same build with the density staging and the J/K accumulation copies
+// in the fixed workspace `scratch` (8 copies while 17 nbf^2 doubles fit,
+// else one copy with device-scope atomics), eager launches (no graph), no
+// allocation.  Every call on a device shares that device's workspace:
+// calls that overlap on one device race.
+int polyeri_jk_build_device_noalloc(polyeri_ctx* ctx, const double* d_p_c, double* d_c_c, double* d_x_c) {
+    if (!ctx || !d_p_c || !d_c_c || !d_x_c) return fail(POLYERI_E_ARG, "polyeri_jk_build_device_noalloc: null argument");
+    if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_jk_build_device_noalloc: periodic context");
+    const size_t n2 = (size_t)ctx->ib.nbf * ctx->ib.nbf;
+    const int copies = 17 * n2 <= kNoAllocDoubles ? 8 : 3 * n2 <= kNoAllocDoubles ? 1 : 0;
+    if (!copies)
+        return fail(POLYERI_E_ARG, "polyeri_jk_build_device_noalloc: 3 nbf^2 doubles exceed the fixed workspace of "
+                                   "2^22 doubles (nbf <= 1182)");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->d_noalloc) {   // first call on this context: the device's workspace address (once per device)
+        int rc = noalloc_scratch(ctx->device, &ctx->d_noalloc);
+        if (rc) return rc;
+    }
+    if (ctx->use_stored) return jk_stored(ctx, d_p_c, d_c_c, d_x_c);   // uses no workspace
+    ctx->stored_last = false;
+    double *sd = ctx->d_D, *sj = ctx->d_J, *sk = ctx->d_K;
+    const int sc = ctx->jk_copies;
+    ctx->d_D = ctx->d_noalloc;
+    ctx->d_J = ctx->d_noalloc + n2;
+    ctx->d_K = ctx->d_noalloc + n2 + copies * n2;
+    ctx->jk_copies = copies;
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    int rc = e == hipSuccess ? jk_enqueue(ctx, d_p_c, d_c_c, d_x_c, 0.0) : fail(POLYERI_E_HIP, hipGetErrorString(e));
+    ctx->d_D = sd; ctx->d_J = sj; ctx->d_K = sk; ctx->jk_copies = sc;   // launches took their arguments by value
+    if (rc) return rc;
+    ctx->screened_last = false;
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    return 0;
 }
 
 int polyeri_sync(polyeri_ctx* ctx) {
