@@ -201,6 +201,17 @@ typedef struct polyeri_scf_opts {
      * (difference)| they touch is below dens_tol; 0 = no density screening. */
     int incremental;
     double dens_tol;
+    /* Solver choices (no counterpart in the reference; 0 = the reference's):
+     * solver = 1: after the first iteration's diagonalisation, and while
+     * the HOMO-LUMO gap exceeds 1 mEh at TEMPHF <= 1 K, the density is the
+     * trace-correcting purification of the orthonormal-basis Fock matrix
+     * (FP64 products only); the orbital energies and coefficients returned
+     * come from one diagonalisation of the last Fock matrix.
+     * diis_mode = 1: commutator DIIS (F P S - S P F) over the Fock matrices
+     * of the last 8 iterations, every iteration, in place of the density
+     * DIIS / relaxation of IOPTN(16) / DOPTN(15). */
+    int solver;
+    int diis_mode;
 } polyeri_scf_opts;
 
 typedef struct polyeri_scf_result {

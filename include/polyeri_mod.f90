@@ -27,6 +27,7 @@ MODULE POLYERI
       INTEGER(C_INT) :: SCFCYCLES, DIIS, CHARGE, SPHERICAL, GUESS
       INTEGER(C_INT) :: INCREMENTAL
       REAL(C_DOUBLE) :: DENS_TOL
+      INTEGER(C_INT) :: SOLVER, DIIS_MODE
    END TYPE
 
    TYPE, BIND(C) :: POLYERI_SCF_RESULT

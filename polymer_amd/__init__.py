@@ -198,13 +198,17 @@ class Context:
 
     def self_consistent_field(self, scfconv=1e-8, scfrelax=1.0, hfexchange=1.0, lindep=1e-6,
                               temphf=0.0, scfcycles=50, diis=5, charge=0, spherical=False,
-                              guess=None, incremental=0, dens_tol=0.0):
+                              guess=None, incremental=0, dens_tol=0.0, solver=0, diis_mode=0):
         """Defaults are the reference's keyword defaults (src/pread.f90:65-172).
         incremental = n > 0 builds J/K from the density difference with a
         full build every n iterations; dens_tol > 0 screens quartets by their
-        Schwarz bound times the density (difference) they touch."""
+        Schwarz bound times the density (difference) they touch; solver = 1
+        forms the zero-temperature density by purification instead of
+        diagonalisation while the gap is open; diis_mode = 1 uses commutator
+        DIIS on the Fock matrices every iteration."""
         o = ScfOpts(scfconv, scfrelax, hfexchange, lindep, temphf, scfcycles, diis, charge,
-                    int(spherical), 0 if guess is None else 1, int(incremental), float(dens_tol))
+                    int(spherical), 0 if guess is None else 1, int(incremental), float(dens_tol),
+                    int(solver), int(diis_mode))
         p = self._mat() if guess is None else np.asfortranarray(guess, dtype=np.float64).copy()
         eps = np.zeros(self.ncgs)
         co = self._mat()

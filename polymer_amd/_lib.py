@@ -47,7 +47,8 @@ class ScfOpts(ctypes.Structure):
                 ("temphf", ctypes.c_double), ("scfcycles", ctypes.c_int),
                 ("diis", ctypes.c_int), ("charge", ctypes.c_int),
                 ("spherical", ctypes.c_int), ("guess", ctypes.c_int),
-                ("incremental", ctypes.c_int), ("dens_tol", ctypes.c_double)]
+                ("incremental", ctypes.c_int), ("dens_tol", ctypes.c_double),
+                ("solver", ctypes.c_int), ("diis_mode", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):

@@ -551,7 +551,10 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     const bool seg_ok = segs && nseg > 0 && !items && (mode == MODE_JK || mode == MODE_JK32) &&
                         seg_lds_bytes(la, lb, c->ib.nbf) <= 65536;
     int kind = cfg.kind;
-    if (kind == LAUNCH_AUTO) kind = k ? auto_kind(c, la, lb, lc, ld, seg_ok) : LAUNCH_COOP;
+    if (kind == LAUNCH_AUTO) {
+        kind = auto_kind(c, la, lb, lc, ld, seg_ok);   // Rys, wavefront, segment or register kernel
+        if ((kind == LAUNCH_REG || kind == LAUNCH_SEG) && !k) kind = LAUNCH_COOP;
+    }
     if (kind == LAUNCH_SEG && !seg_ok) kind = k ? LAUNCH_REG : LAUNCH_COOP;
     if (kind == LAUNCH_REG && !k) return fail(POLYERI_E_ARG, "no register kernel for class");
     if (kind == LAUNCH_SEG && mode == MODE_JK32)
@@ -2017,6 +2020,118 @@ static int ensure_blas(polyeri_ctx* c) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Zero-temperature density without diagonalisation: trace-correcting
+// purification (TC2; A. M. N. Niklasson, Phys. Rev. B 66, 155115 (2002)).
+// X_0 = (e_max I - F') / (e_max - e_min) from Gershgorin bounds of the
+// orthonormal-basis Fock matrix F' maps the spectrum into [0, 1] with the
+// occupied levels near 1; each step replaces X by X^2 or by 2X - X^2,
+// whichever brings the trace closer to the number of occupied orbitals, and
+// the sequence converges quadratically to the projector on the nocc lowest
+// eigenvectors, so D' = 2 X equals the aufbau density of the diagonalisation
+// whenever the HOMO-LUMO gap is open.  Every step is one FP64 product and
+// two small kernels; the step decision stays on the device.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gershgorin(const double* __restrict__ A, int n, double* __restrict__ out) {
+    __shared__ double lo[256], hi[256];
+    double mn = 1e300, mx = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double* col = A + (size_t)n * i;   // column i = row i (symmetric)
+        double r = 0.0;
+        for (int j = 0; j < n; ++j) r += j == i ? 0.0 : fabs(col[j]);
+        mn = fmin(mn, col[i] - r);
+        mx = fmax(mx, col[i] + r);
+    }
+    lo[threadIdx.x] = mn;
+    hi[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + h]);
+            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = lo[0]; out[1] = hi[0]; }
+}
+
+__global__ void k_tc2_init(const double* __restrict__ F, int n, const double* __restrict__ b, double* __restrict__ X) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const double w = b[1] - b[0];
+    X[t] = ((t % n == t / n ? b[1] : 0.0) - F[t]) / (w > 0.0 ? w : 1.0);
+}
+
+// st[0] = 1: X <- X^2, 0: X <- 2X - X^2; st[1] = tr X - tr X^2 (the
+// idempotency error of X before the step).  Y = X X.
+__global__ void __launch_bounds__(256) k_tc2_trace(const double* __restrict__ X, const double* __restrict__ Y, int n,
+                                                   double nocc, double* __restrict__ st) {
+    __shared__ double sx[256], sy[256];
+    double a = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a += X[(size_t)i * (n + 1)];
+        c += Y[(size_t)i * (n + 1)];
+    }
+    sx[threadIdx.x] = a;
+    sy[threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sx[threadIdx.x] += sx[threadIdx.x + h];
+            sy[threadIdx.x] += sy[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double tx = sx[0], ty = sy[0];
+        st[0] = fabs(ty - nocc) < fabs(2.0 * tx - ty - nocc) ? 1.0 : 0.0;
+        st[1] = tx - ty;
+    }
+}
+
+__global__ void k_tc2_update(double* __restrict__ X, const double* __restrict__ Y, size_t n2,
+                             const double* __restrict__ st) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
+    X[t] = st[0] != 0.0 ? Y[t] : 2.0 * X[t] - Y[t];
+}
+
+// E = B - B^T (n x n): the commutator F P S - S P F from B = F P S.
+__global__ void k_antisym(const double* __restrict__ B, int n, double* __restrict__ E) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const size_t i = t % n, j = t / n;
+    E[t] = B[t] - B[i * n + j];
+}
+
+// X (n x n) <- the projector on the nocc lowest eigenvectors of Fp; Y and
+// st (4 doubles) are work space.  *ok = 0 when tr X - tr X^2 did not fall
+// below tol within the step limit (a closed or tiny gap): the caller then
+// diagonalises.  The host reads the idempotency error every 8 steps.
+static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, int nocc, double* X, double* Y,
+                      double* st, int* ok) {
+    const size_t n2 = (size_t)n * n;
+    const double one = 1.0, zero = 0.0, tol = 1e-11;
+    *ok = 0;
+    hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, s, Fp, n, st + 2);
+    hipLaunchKernelGGL(k_tc2_init, dim3(nblk(n2)), dim3(256), 0, s, Fp, n, st + 2, X);
+    HIPCHK(hipGetLastError());
+    double err = 0.0;
+    for (int it = 0; it < 160 && !*ok; it += 8) {
+        for (int k = 0; k < 8; ++k) {
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, X, n, &zero, Y,
+                                n));
+            hipLaunchKernelGGL(k_tc2_trace, dim3(1), dim3(256), 0, s, X, Y, n, (double)nocc, st);
+            hipLaunchKernelGGL(k_tc2_update, dim3(nblk(n2)), dim3(256), 0, s, X, Y, n2, st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&err, st + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (std::isfinite(err) && std::fabs(err) < tol) *ok = 1;
+    }
+    return 0;
+}
+
 int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double* eps_out, double* co_out,
                 double* trace, polyeri_scf_result* res) {
     if (!ctx || !o || !p_c || !res) return fail(POLYERI_E_ARG, "polyeri_scf: null argument");
@@ -2077,6 +2192,20 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     nel -= o->charge;
     if (nel % 2) return fail(POLYERI_E_ARG, "ODD NUMBER OF ELECTRONS");
     int iocc = nel / 2;
+    // solver options (no counterpart in the reference): purification of the
+    // zero-temperature density instead of the diagonalisation after the
+    // first iteration, commutator DIIS on the Fock matrices every iteration
+    const bool want_purify = o->solver == 1 && (o->temphf > 0 ? o->temphf : 1.0e-4) <= 1.0 && iocc > 0 && iocc < nind;
+    const bool cdiis = o->diis_mode == 1;
+    double *dXp = nullptr, *dYp = nullptr, *dSt = nullptr, *dFpLast = nullptr, *dHF = nullptr, *dHE = nullptr,
+           *dDots = nullptr;
+    if (want_purify &&
+        ((rc = buf.get(&dXp, n2)) || (rc = buf.get(&dYp, n2)) || (rc = buf.get(&dSt, 4)) || (rc = buf.get(&dFpLast, n2))))
+        return rc;
+    if (cdiis && ((rc = buf.get(&dHF, n2 * 8)) || (rc = buf.get(&dHE, n2 * 8)) || (rc = buf.get(&dDots, 8))))
+        return rc;
+    bool gap_open = false, last_purified = false;
+    std::vector<double> cB(64 * 64, 0.0);   // commutator DIIS: <e_i, e_j> of the last 8 iterations (ring)
     // guess
     std::vector<double> P(n2, 0.0);
     if (o->guess == 1) {
@@ -2112,6 +2241,45 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     hipEvent_t ef0, ef1;
     HIPCHK(hipEventCreate(&ef0));
     HIPCHK(hipEventCreate(&ef1));
+    // eigenvalues, coefficients, Fermi occupations and the output density
+    // of F' (dFp, overwritten): the reference's FOCK_BUILD path
+    auto diagonalise = [&]() -> int {
+        RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nind, dFp, nind, dE, dWk, dInfo));
+        // C = X C', CO = C2S^T C
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &one, dX, nc, dFp, nind, &zero, dW1, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, nind, nc, &one, dC2S, nc, dW1, nc, &zero, dCO, nc));
+        HIPCHK(hipMemcpyAsync(eps.data(), dE, nind * 8, hipMemcpyDeviceToHost, s));
+        int info = 0;
+        HIPCHK(hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (info != 0) return fail(POLYERI_E_LIB, "EIGENVECTORS NOT CONVERGED");
+        for (int j = nind; j < nc; ++j) eps[j] = 1.0e99;
+        // Fermi level by bisection (src/pfock.f90:339-423)
+        double mine = 1e99, maxe = -1e99;
+        for (int j = 0; j < nc; ++j) { maxe = std::max(maxe, eps[j]); mine = std::min(mine, eps[j]); }
+        double f1 = mine, f2 = maxe, f12 = 0.5 * (mine + maxe);
+        for (int it = 0; it < 100000; ++it) {
+            double n1 = 0, nn2 = 0, n12 = 0;
+            for (int j = 0; j < nc; ++j) {
+                n1 += 2.0 / (std::exp((eps[j] - f1) / kBoltzmann / temphf) + 1.0);
+                nn2 += 2.0 / (std::exp((eps[j] - f2) / kBoltzmann / temphf) + 1.0);
+                n12 += 2.0 / (std::exp((eps[j] - f12) / kBoltzmann / temphf) + 1.0);
+            }
+            n1 -= nel; nn2 -= nel; n12 -= nel;
+            if (std::fabs(n12) < 1e-10) { fermi = f12; break; }
+            else if (n1 > 0 && nn2 > 0) { f2 = f1; f1 -= 0.1; }
+            else if (n1 < 0 && nn2 < 0) { f1 = f2; f2 += 0.1; }
+            else if (n1 < 0 && n12 >= 0) f2 = f12;
+            else if (n12 <= 0 && nn2 > 0) f1 = f12;
+            else break;
+            f12 = 0.5 * (f1 + f2);
+        }
+        for (int j = 0; j < nc; ++j) occ[j] = j < nind ? 2.0 / (std::exp((eps[j] - fermi) / kBoltzmann / temphf) + 1.0) : 0.0;
+        HIPCHK(hipMemcpyAsync(dOcc, occ.data(), nind * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_scale_cols, dim3(nblk((size_t)nc * nind)), dim3(256), 0, s, dCO, dOcc, nc, nind, dW1);
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nind, &one, dCO, nc, dW1, nc, &zero, dPout, nc));
+        return 0;
+    };
     for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
         HIPCHK(hipEventRecord(ef0, s));
         const bool inc_now = o->incremental > 0 && iscf > 1 && since_full < o->incremental;
@@ -2139,46 +2307,82 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dW2, nc));
         RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nc, &one, dW2, nc, dX, nc, &zero, dW1, nc));
         RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nind, nind, nc, &one, dX, nc, dW1, nc, &zero, dFp, nind));
-        RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nind, dFp, nind, dE, dWk, dInfo));
-        // C = X C', CO = C2S^T C
-        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &one, dX, nc, dFp, nind, &zero, dW1, nc));
-        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, nind, nc, &one, dC2S, nc, dW1, nc, &zero, dCO, nc));
-        HIPCHK(hipMemcpyAsync(eps.data(), dE, nind * 8, hipMemcpyDeviceToHost, s));
-        int info = 0;
-        HIPCHK(hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (info != 0) { rc = fail(POLYERI_E_LIB, "EIGENVECTORS NOT CONVERGED"); break; }
+        // commutator DIIS (diis_mode 1): error F P S - S P F of this
+        // iteration's input density, in the orthonormal basis; F' is replaced
+        // by the extrapolation over the last (up to 8) iterations
+        if (cdiis) {
+            const int slot8 = (iscf - 1) % 8;
+            double* he = dHE + n2 * slot8;
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dF, nc, dP, nc, &zero, dW1, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dW1, nc, dS, nc, &zero, dW2, nc));
+            hipLaunchKernelGGL(k_antisym, dim3(nblk(n2)), dim3(256), 0, s, dW2, nc, dW1);
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dW1, nc, &zero, dW2, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW2, nc, dC2S, nc, &zero, dW1, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nc, &one, dW1, nc, dX, nc, &zero, dW2, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nind, nind, nc, &one, dX, nc, dW2, nc, &zero, he, nind));
+            HIPCHK(hipMemcpyAsync(dHF + n2 * slot8, dFp, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s));
+            const int nt = std::min(8, iscf);
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
+            for (int i = 0; i < nt; ++i)
+                RBCHK(rocblas_ddot(h, nind * nind, dHE + n2 * ((iscf - 1 - i) % 8), 1, he, 1, dDots + i));
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+            double dots[8];
+            HIPCHK(hipMemcpyAsync(dots, dDots, nt * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            // B[a][b] over iterations a, b (ring of 64 x 64 by iteration mod 64)
+            for (int i = 0; i < nt; ++i) {
+                const int a = (iscf - 1) % 64, b = (iscf - 1 - i) % 64;
+                cB[a * 64 + b] = cB[b * 64 + a] = dots[i];
+            }
+            if (nt > 1) {
+                const int m = nt + 1;
+                std::vector<double> B((size_t)m * m, 0.0), rhs(m, 0.0), cv;
+                for (int i = 0; i < nt; ++i) {
+                    for (int j = 0; j < nt; ++j) B[j + (size_t)m * i] = cB[((iscf - 1 - i) % 64) * 64 + (iscf - 1 - j) % 64];
+                    B[i + (size_t)m * nt] = -1.0;
+                    B[nt + (size_t)m * i] = -1.0;
+                }
+                rhs[nt] = -1.0;
+                lu_solve(m, B, rhs, cv);
+                bool finite = true;
+                for (int i = 0; i < nt; ++i) finite = finite && std::isfinite(cv[i]);
+                if (finite) {
+                    hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)nind * nind)), dim3(256), 0, s, dFp, (size_t)nind * nind, 0.0);
+                    for (int i = 0; i < nt; ++i) {
+                        const double w = cv[i];
+                        RBCHK(rocblas_daxpy(h, nind * nind, &w, dHF + n2 * ((iscf - 1 - i) % 8), 1, dFp, 1));
+                    }
+                }
+            }
+        }
+        // the density: purification while the gap is open (after the first
+        // iteration's diagonalisation), else the diagonalisation
+        const bool purify_now = want_purify && gap_open && iscf > 1;
+        int purified = 0;
+        if (purify_now) {
+            HIPCHK(hipMemcpyAsync(dFpLast, dFp, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s));
+            if ((rc = purify_tc2(h, s, dFp, nind, iocc, dXp, dYp, dSt, &purified))) break;
+        }
+        if (purified) {
+            // P = C2S^T X (2 X_p) X^T C2S
+            const double two = 2.0;
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &two, dX, nc, dXp, nind, &zero, dW1, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nind, &one, dW1, nc, dX, nc, &zero, dW2, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dW2, nc, &zero, dW1, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dPout, nc));
+        } else {
+            if ((rc = diagonalise())) break;
+            if (want_purify && iscf == 1) {   // eps ascending: open when the HOMO-LUMO gap exceeds 1 mEh
+                gap_open = std::isfinite(eps[iocc]) && eps[iocc] < 1e90 && eps[iocc] - eps[iocc - 1] > 1e-3;
+            }
+        }
+        last_purified = purified != 0;
         {
             fock_total += event_ms(ef0, ef1) * 1e-3;
             jk_total += event_ms(ctx->ev0, ctx->ev1) * 1e-3;
             if ((rc = jk_collect_timing(ctx))) break;
             build_quartets += ctx->last_quartets;
         }
-        for (int j = nind; j < nc; ++j) eps[j] = 1.0e99;
-        // Fermi level by bisection (src/pfock.f90:339-423)
-        double mine = 1e99, maxe = -1e99;
-        for (int j = 0; j < nc; ++j) { maxe = std::max(maxe, eps[j]); mine = std::min(mine, eps[j]); }
-        double f1 = mine, f2 = maxe, f12 = 0.5 * (mine + maxe);
-        for (int it = 0; it < 100000; ++it) {
-            double n1 = 0, nn2 = 0, n12 = 0;
-            for (int j = 0; j < nc; ++j) {
-                n1 += 2.0 / (std::exp((eps[j] - f1) / kBoltzmann / temphf) + 1.0);
-                nn2 += 2.0 / (std::exp((eps[j] - f2) / kBoltzmann / temphf) + 1.0);
-                n12 += 2.0 / (std::exp((eps[j] - f12) / kBoltzmann / temphf) + 1.0);
-            }
-            n1 -= nel; nn2 -= nel; n12 -= nel;
-            if (std::fabs(n12) < 1e-10) { fermi = f12; break; }
-            else if (n1 > 0 && nn2 > 0) { f2 = f1; f1 -= 0.1; }
-            else if (n1 < 0 && nn2 < 0) { f1 = f2; f2 += 0.1; }
-            else if (n1 < 0 && n12 >= 0) f2 = f12;
-            else if (n12 <= 0 && nn2 > 0) f1 = f12;
-            else break;
-            f12 = 0.5 * (f1 + f2);
-        }
-        for (int j = 0; j < nc; ++j) occ[j] = j < nind ? 2.0 / (std::exp((eps[j] - fermi) / kBoltzmann / temphf) + 1.0) : 0.0;
-        HIPCHK(hipMemcpyAsync(dOcc, occ.data(), nind * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_scale_cols, dim3(nblk((size_t)nc * nind)), dim3(256), 0, s, dCO, dOcc, nc, nind, dW1);
-        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nind, &one, dCO, nc, dW1, nc, &zero, dPout, nc));
         // STORE_DENSITYMATRIX: Delta P, RMS change, Pulay row (src/pdensitymatrix.f90:131-208)
         int slot = (iscf - 1) % HMAX;
         double* hp = dHP + n2 * slot;
@@ -2191,7 +2395,7 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         double dchg = 0.0;
         RBCHK(rocblas_ddot(h, (int)n2, hd, 1, hd, 1, &dchg));
         pulay[iscf + (size_t)ld * iscf] = dchg;
-        for (int i = std::max(1, iscf - HMAX + 1); i < iscf; ++i) {
+        for (int i = std::max(1, iscf - HMAX + 1); i < iscf && !cdiis; ++i) {
             double v = 0.0;
             RBCHK(rocblas_ddot(h, (int)n2, dHD + n2 * ((i - 1) % HMAX), 1, hd, 1, &v));
             pulay[i + (size_t)ld * iscf] = v;
@@ -2199,7 +2403,10 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         }
         dchg = std::sqrt(dchg / (double)n2);
         // DIIS every other iteration from IOPTN(16) on, else relaxation
-        if (o->diis != 0 && iscf >= o->diis && ((iscf - o->diis) % 2) == 0) {
+        // (commutator DIIS: the next input density is P_out of the
+        // extrapolated Fock matrix, taken after the energy below)
+        if (cdiis) {
+        } else if (o->diis != 0 && iscf >= o->diis && ((iscf - o->diis) % 2) == 0) {
             int nt = std::min(8, iscf), n = nt + 1;
             std::vector<double> B((size_t)n * n, 0.0), rhs(n, 0.0), cv;
             for (int i = 0; i < nt; ++i) {
@@ -2229,7 +2436,10 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             jk_total += ctx->last_jk_ms * 1e-3;
             build_quartets += ctx->last_quartets;
         }
-        // ENERGY_CALC with the updated density (src/pselfconsistentfield.f90:487-574)
+        // ENERGY_CALC with the updated density (src/pselfconsistentfield.f90:487-574);
+        // with commutator DIIS the input density is idempotent (the previous
+        // P_out) and the energy is that of the input density and its own J/K,
+        // stationary to second order
         RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, &ek));
         RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, &ev));
         RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_jc, 1, &ej));
@@ -2240,6 +2450,11 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         if (!std::isfinite(etot)) { rc = fail(POLYERI_E_NONFINITE, "non-finite SCF energy"); break; }
         if (trace) { trace[2 * (iscf - 1)] = dchg; trace[2 * (iscf - 1) + 1] = etot; }
         if (dchg < o->scfconv) { converged = 1; break; }
+        if (cdiis) HIPCHK(hipMemcpyAsync(dP, dPout, n2 * 8, hipMemcpyDeviceToDevice, s));
+    }
+    if (!rc && last_purified) {   // orbital energies and coefficients of the last Fock matrix
+        hipError_t e = hipMemcpyAsync(dFp, dFpLast, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s);
+        rc = e == hipSuccess ? diagonalise() : fail(POLYERI_E_HIP, hipGetErrorString(e));
     }
     (void)hipEventDestroy(ef0);
     (void)hipEventDestroy(ef1);
