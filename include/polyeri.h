@@ -346,6 +346,16 @@ int polyeri_set_general_contraction(polyeri_ctx *ctx, int on);
  * fastest.  Results are unchanged to rounding; later builds use the chosen
  * shapes (polyeri_stats). */
 int polyeri_autotune(polyeri_ctx *ctx, int reps);
+/* The launch configuration of work list `list` (0 .. nclass - 1 of
+ * polyeri_get_stats) as 9 ints: kind (0 default, 1 thread per quartet,
+ * 2 wavefront per quartet, 3 bra-row segments, 5 Rys segments), threads per
+ * block, grid cap, segment variant, primitive batch (-1 default), generally
+ * contracted form on, its block, its segment variant, autotuned time in
+ * microseconds.  set restores such a record (e.g. the autotuner's picks saved
+ * by an earlier process, so that a counter run times the same shapes).  No
+ * counterpart in the reference. */
+int polyeri_get_launch_config(const polyeri_ctx *ctx, int list, int *cfg);
+int polyeri_set_launch_config(polyeri_ctx *ctx, int list, const int *cfg);
 
 /* Nodes u[0..n-1] (ascending, in (0, 1)) and weights w[0..n-1] of the
  * n-point Rys quadrature at Boys argument T >= 0, n = 1..5: the Gauss rule

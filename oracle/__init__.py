@@ -47,6 +47,8 @@ def lib():
         L.or_fourindex_eri_range.restype = ll
         L.or_fourindex_eri_rows.argtypes = [p, p, p, d, i, p, ll, p, p, p, p, p, p]
         L.or_fourindex_eri_rows.restype = ll
+        L.or_fourindex_eri_rows_mt.argtypes = [p, p, d, i, p, ll, i, p, p, p, p, p]
+        L.or_fourindex_eri_rows_mt.restype = ll
         L.or_eri_tensor.argtypes = [p, p, p]
         L.or_eri_tensor_cells.argtypes = [p, d, i, i, p]
         L.or_fourindex_eri_cells_c.argtypes = [p, d, i, i, i, p, d, p, p]
@@ -195,6 +197,20 @@ class Basis:
                                          _ptr(r), len(r), _ptr(c_p), _ptr(x_p), ctypes.byref(nreal),
                                          ctypes.byref(fl), ctypes.byref(ts), ctypes.byref(tr))
         return nq, nreal.value, ts.value, tr.value
+
+    def fourindex_eri_rows_mt(self, p_c, inttol, rows, nthreads, recompute_kl=True):
+        """fourindex_eri_rows over nthreads threads: (quartets, real quartets,
+        set-up seconds, wall seconds of the rows, C_P, X_P)."""
+        p_p = self.decontract(p_c)
+        c_p = np.zeros((self.npgs, self.npgs), order="F")
+        x_p = np.zeros((self.npgs, self.npgs), order="F")
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        nreal = ctypes.c_longlong()
+        ts, tr = ctypes.c_double(), ctypes.c_double()
+        nq = lib().or_fourindex_eri_rows_mt(self.h, _ptr(p_p), float(inttol), int(recompute_kl), _ptr(r), len(r),
+                                            int(nthreads), _ptr(c_p), _ptr(x_p), ctypes.byref(nreal),
+                                            ctypes.byref(ts), ctypes.byref(tr))
+        return nq, nreal.value, ts.value, tr.value, c_p, x_p
 
     def eri_tensor(self):
         m = self.ncgs

@@ -614,6 +614,60 @@ static int shell_real_mask(const or_basis *b, int s, const double *maxc) {
 /* ------------------------------------------------------------------ */
 /* Direct J/K: src/pfourindexdirect.f90:1-814 (molecule: Q1=Q2=Q3=0)  */
 /* ------------------------------------------------------------------ */
+/* One bra row (I,J) of the loop nest: every KL of the table against the
+ * row's composite pair, digested into the primitive-basis C and X. */
+static void eri_row(const or_basis *b, long long sij, const double *p_p, double inttol, int recompute_kl,
+                    const double *maxc, const int *treal, cpair *kl, double *e, double *c_p, double *x_p,
+                    long long *nq, long long *nreal, double *flops, int want_flops) {
+    int ns = b->npshell, np = b->npgs;
+    size_t nsq = (size_t)ns * ns;
+    if (sij < 0 || sij >= (long long)nsq) return;
+    const double c1f = 2.0 / sqrt(PI_REF);
+    int I = (int)(sij / ns), J = (int)(sij % ns);
+    cpair ij;
+    cpair_make(b, I, J, &ij);
+    cpair_bound(&ij, maxc, e);
+    if (recompute_kl) {
+        /* the reference rebuilds the KL pair table inside every (I,J,Q3) */
+        for (int k = 0; k < ns; ++k)
+            for (int l = 0; l < ns; ++l) {
+                double bd = kl[(size_t)k * ns + l].bound;
+                cpair_make(b, k, l, &kl[(size_t)k * ns + l]);
+                kl[(size_t)k * ns + l].bound = bd;
+            }
+    }
+    double c1 = ij.ov * c1f;
+    for (size_t skl = 0; skl < nsq; ++skl) {
+        const cpair *q = &kl[skl];
+        if (ij.bound * q->bound < inttol) continue;
+        ++*nq;
+        int K = (int)(skl / ns), Lq = (int)(skl % ns);
+        for (int la = 0; la <= OR_LMAX; ++la) if (treal[I] >> la & 1)
+        for (int lb = 0; lb <= OR_LMAX; ++lb) if (treal[J] >> lb & 1)
+        for (int lc = 0; lc <= OR_LMAX; ++lc) if (treal[K] >> lc & 1)
+        for (int ld = 0; ld <= OR_LMAX; ++ld) if (treal[Lq] >> ld & 1) {
+            ++*nreal;
+            if (want_flops) *flops += or_flop_model(la, lb, lc, ld);
+        }
+        double ze = ij.zeta + q->zeta;
+        double rho = ij.zeta * q->zeta / ze;
+        double c2 = c1 * sqrt(rho) * q->ov;
+        double dx = q->px - ij.px, dy = q->py - ij.py, dz = q->pz - ij.pz;
+        double t = rho * (dx * dx + dy * dy + dz * dz);
+        os_quartet(&ij, q, c2, t, 0, e);
+        int nij = ij.t->n, nk = q->t->n, ms = ij.lang + q->lang + 1;
+        for (int a = 0; a < nij; ++a) {
+            int pi = ij.pi[a], pj = ij.pj[a];
+            for (int c = 0; c < nk; ++c) {
+                double v = e[((size_t)a * nk + c) * ms];
+                int pk = q->pi[c], pl = q->pj[c];
+                c_p[pi + (size_t)np * pj] += p_p[pk + (size_t)np * pl] * v;
+                x_p[pi + (size_t)np * pl] += p_p[pk + (size_t)np * pj] * v;
+            }
+        }
+    }
+}
+
 long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const double *p_p,
                                 double inttol, int recompute_kl, const long long *rows,
                                 long long nrows, double *c_p, double *x_p,
@@ -622,13 +676,12 @@ long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const doub
     (void)s_p;
     boys_init();
     pairtab_init();
-    int ns = b->npshell, np = b->npgs;
+    int ns = b->npshell;
     double *maxc = maxc_make(b);
     int *treal = (int *)malloc(sizeof(int) * ns);
     for (int s = 0; s < ns; ++s) treal[s] = shell_real_mask(b, s, maxc);
     size_t nsq = (size_t)ns * ns;
     cpair *kl = (cpair *)malloc(sizeof(cpair) * nsq);
-    cpair ij;
     double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
     double t0 = wall_seconds();
     for (int k = 0; k < ns; ++k)
@@ -640,57 +693,70 @@ long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const doub
     if (setup_seconds) *setup_seconds = t1 - t0;
     long long nq = 0, nreal = 0;
     double flops = 0;
-    const double c1f = 2.0 / sqrt(PI_REF);
-    for (long long r = 0; r < nrows; ++r) {
-        long long sij = rows[r];
-        if (sij < 0 || sij >= (long long)nsq) continue;
-        int I = (int)(sij / ns), J = (int)(sij % ns);
-        cpair_make(b, I, J, &ij);
-        cpair_bound(&ij, maxc, e);
-        if (recompute_kl) {
-            /* the reference rebuilds the KL pair table inside every (I,J,Q3) */
-            for (int k = 0; k < ns; ++k)
-                for (int l = 0; l < ns; ++l) {
-                    double bd = kl[(size_t)k * ns + l].bound;
-                    cpair_make(b, k, l, &kl[(size_t)k * ns + l]);
-                    kl[(size_t)k * ns + l].bound = bd;
-                }
-        }
-        double c1 = ij.ov * c1f;
-        for (size_t skl = 0; skl < nsq; ++skl) {
-            const cpair *q = &kl[skl];
-            if (ij.bound * q->bound < inttol) continue;
-            ++nq;
-            int K = (int)(skl / ns), Lq = (int)(skl % ns);
-            for (int la = 0; la <= OR_LMAX; ++la) if (treal[I] >> la & 1)
-            for (int lb = 0; lb <= OR_LMAX; ++lb) if (treal[J] >> lb & 1)
-            for (int lc = 0; lc <= OR_LMAX; ++lc) if (treal[K] >> lc & 1)
-            for (int ld = 0; ld <= OR_LMAX; ++ld) if (treal[Lq] >> ld & 1) {
-                ++nreal;
-                if (flop_model) flops += or_flop_model(la, lb, lc, ld);
-            }
-            double ze = ij.zeta + q->zeta;
-            double rho = ij.zeta * q->zeta / ze;
-            double c2 = c1 * sqrt(rho) * q->ov;
-            double dx = q->px - ij.px, dy = q->py - ij.py, dz = q->pz - ij.pz;
-            double t = rho * (dx * dx + dy * dy + dz * dz);
-            os_quartet(&ij, q, c2, t, 0, e);
-            int nij = ij.t->n, nk = q->t->n, ms = ij.lang + q->lang + 1;
-            for (int a = 0; a < nij; ++a) {
-                int pi = ij.pi[a], pj = ij.pj[a];
-                for (int c = 0; c < nk; ++c) {
-                    double v = e[((size_t)a * nk + c) * ms];
-                    int pk = q->pi[c], pl = q->pj[c];
-                    c_p[pi + (size_t)np * pj] += p_p[pk + (size_t)np * pl] * v;
-                    x_p[pi + (size_t)np * pl] += p_p[pk + (size_t)np * pj] * v;
-                }
-            }
-        }
-    }
+    for (long long r = 0; r < nrows; ++r)
+        eri_row(b, rows[r], p_p, inttol, recompute_kl, maxc, treal, kl, e, c_p, x_p, &nq, &nreal, &flops,
+                flop_model != NULL);
     if (row_seconds) *row_seconds = wall_seconds() - t1;
     free(e); free(kl); free(maxc); free(treal);
     if (real_quartets) *real_quartets = nreal;
     if (flop_model) *flop_model = flops;
+    return nq;
+}
+
+/* The same loop nest with the rows spread over nthreads OpenMP threads
+ * (dynamic schedule, one row at a time), each thread with its own KL
+ * table, work buffer and C / X accumulators (summed at the end), so every
+ * thread does exactly the single-threaded per-row work.  setup_seconds: the
+ * tables' construction (concurrent); row_seconds: the wall time of the
+ * rows. */
+long long or_fourindex_eri_rows_mt(const or_basis *b, const double *p_p, double inttol, int recompute_kl,
+                                   const long long *rows, long long nrows, int nthreads, double *c_p,
+                                   double *x_p, long long *real_quartets, double *setup_seconds,
+                                   double *row_seconds) {
+    boys_init();
+    pairtab_init();
+    int ns = b->npshell, np = b->npgs;
+    double *maxc = maxc_make(b);
+    int *treal = (int *)malloc(sizeof(int) * ns);
+    for (int s = 0; s < ns; ++s) treal[s] = shell_real_mask(b, s, maxc);
+    size_t nsq = (size_t)ns * ns, np2 = (size_t)np * np;
+    long long nq = 0, nreal = 0;
+    double t0 = 0, t1 = 0, t2 = 0;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : nq, nreal)
+    {
+        cpair *kl = (cpair *)malloc(sizeof(cpair) * nsq);
+        double *e = (double *)malloc(sizeof(double) * 400 * 400 * (4 * OR_LMAX + 2));
+        double *cl = (double *)calloc(np2, sizeof(double)), *xl = (double *)calloc(np2, sizeof(double));
+        double fl = 0;
+#pragma omp barrier
+#pragma omp master
+        t0 = wall_seconds();
+        for (int k = 0; k < ns; ++k)
+            for (int l = 0; l < ns; ++l) {
+                cpair_make(b, k, l, &kl[(size_t)k * ns + l]);
+                cpair_bound(&kl[(size_t)k * ns + l], maxc, e);
+            }
+#pragma omp barrier
+#pragma omp master
+        t1 = wall_seconds();
+#pragma omp barrier
+#pragma omp for schedule(dynamic, 1)
+        for (long long r = 0; r < nrows; ++r)
+            eri_row(b, rows[r], p_p, inttol, recompute_kl, maxc, treal, kl, e, cl, xl, &nq, &nreal, &fl, 0);
+#pragma omp master
+        t2 = wall_seconds();
+#pragma omp critical
+        for (size_t i = 0; i < np2; ++i) {
+            c_p[i] += cl[i];
+            x_p[i] += xl[i];
+        }
+        free(cl); free(xl); free(e); free(kl);
+    }
+    free(maxc); free(treal);
+    if (setup_seconds) *setup_seconds = t1 - t0;
+    if (row_seconds) *row_seconds = t2 - t1;
+    if (real_quartets) *real_quartets = nreal;
     return nq;
 }
 

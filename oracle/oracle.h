@@ -84,6 +84,12 @@ long long or_fourindex_eri_rows(const or_basis *b, const double *s_p, const doub
                                 long long nrows, double *c_p, double *x_p,
                                 long long *real_quartets, double *flop_model,
                                 double *setup_seconds, double *row_seconds);
+/* The same over nthreads OpenMP threads, each with its own tables and
+ * accumulators; row_seconds is the wall time of all rows. */
+long long or_fourindex_eri_rows_mt(const or_basis *b, const double *p_p, double inttol, int recompute_kl,
+                                   const long long *rows, long long nrows, int nthreads, double *c_p,
+                                   double *x_p, long long *real_quartets, double *setup_seconds,
+                                   double *row_seconds);
 
 /* Contracted ERI tensor FERI(i,j,k,l) = sum CC.CC.CC.CC (pq|rs), the
  * quantity src/pfourindexstore.f90:704-708 accumulates (no screening). */

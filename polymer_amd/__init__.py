@@ -396,6 +396,22 @@ class Context:
         use the loaded .ao records."""
         check(lib().polyeri_set_direct(self.h, int(bool(direct))))
 
+    def launch_configs(self):
+        """Per work list: the 9-int launch record of polyeri_get_launch_config."""
+        n = self.stats()["nclass"]
+        out = []
+        for i in range(n):
+            c = (ctypes.c_int * 9)()
+            check(lib().polyeri_get_launch_config(self.h, i, c))
+            out.append(list(c))
+        return out
+
+    def set_launch_configs(self, cfgs):
+        """Restore records from launch_configs() (same geometry and basis)."""
+        for i, c in enumerate(cfgs):
+            a = (ctypes.c_int * 9)(*[int(x) for x in c])
+            check(lib().polyeri_set_launch_config(self.h, i, a))
+
     def autotune(self, reps=2):
         """Time every work list with each candidate launch shape and keep the
         fastest (results do not change)."""

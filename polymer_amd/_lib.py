@@ -23,7 +23,7 @@ EXPORTS = [
     "polyeri_mp2_energy_ex", "polyeri_gemm_f64", "polyeri_ctx_create_periodic", "polyeri_jk_build_cells",
     "polyeri_scf_periodic", "polyeri_mp2_energy_periodic", "polyeri_mp2_energy_qp", "polyeri_store_fourindex_eri", "polyeri_restore_fourindex_eri",
     "polyeri_restore_fourindex_eri_cells", "polyeri_eri_tensor_cells", "polyeri_reduced_cel1",
-    "polyeri_load_fourindex_eri", "polyeri_set_direct",
+    "polyeri_load_fourindex_eri", "polyeri_set_direct", "polyeri_get_launch_config", "polyeri_set_launch_config",
     "polyeri_rys_nodes",
     "polyeri_rys_block",
 ]
@@ -105,6 +105,8 @@ def lib():
     L.polyeri_restore_fourindex_eri_cells.argtypes = [ctypes.c_char_p, i, i, i, p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_load_fourindex_eri.argtypes = [p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_set_direct.argtypes = [p, i]
+    L.polyeri_get_launch_config.argtypes = [p, i, p]
+    L.polyeri_set_launch_config.argtypes = [p, i, p]
     L.polyeri_eri_tensor_cells.argtypes = [p, p]
     L.polyeri_reduced_cel1.argtypes = [p]
     L.polyeri_ctx_destroy.restype = None

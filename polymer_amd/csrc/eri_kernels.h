@@ -71,6 +71,11 @@ struct EriArgs {
     const double* dsh;         // nsh x nsh shell-block maxima of |D|
     int nsh;
     unsigned* kept;
+    // with dtol > 0: per segment the largest ket bound (segq, indexed like
+    // segs) and the largest |D| block (*dmax); a segment with bra bound x
+    // segq x dmax < dtol holds no surviving item and is skipped
+    const double* segq;
+    const double* dmax;
     // generally contracted lists (k_eri_gc): pair data, bra primitive pairs,
     // ket fields 8 .. 11 of ksoa = the member weights
     const GcMeta* gmeta;
@@ -707,9 +712,10 @@ __global__ void __launch_bounds__(256) k_eri_seg(EriArgs g) {
     const int lane = threadIdx.x & 63;
     for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
         const SegDesc seg = g.segs[sg];
+        const int bp = g.bra_ids[seg.row];
+        if (g.dtol > 0.0 && g.segq && g.Qs[bp] * g.segq[sg] * g.dmax[0] < g.dtol) continue;   // block-uniform
         for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
         __syncthreads();
-        const int bp = g.bra_ids[seg.row];
         const PairMeta mb = g.meta[bp];
         const int fa = mb.fa, fb = mb.fb;
         const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
@@ -861,10 +867,13 @@ k_eri_gc(EriArgs g) {
     const long long fs = g.kfs;
     for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
         const SegDesc seg = g.segs[sg];
-        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
-        __syncthreads();
         const int bp = g.bra_ids[seg.row];
         const GcMeta mb = g.gmeta[bp];
+        if (g.dtol > 0.0 && g.segq &&
+            fmax(fmax(mb.q[0], mb.q[1]), fmax(mb.q[2], mb.q[3])) * g.segq[sg] * g.dmax[0] < g.dtol)
+            continue;   // block-uniform: no item of the segment survives
+        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
+        __syncthreads();
         const GcPrim* bra = g.gprims + mb.prim_off;
         double jab[MB * NA];
 #pragma unroll
@@ -1103,9 +1112,10 @@ __global__ void __launch_bounds__(256) k_eri_gck(EriArgs g) {
     const long long fs = g.kfs;
     for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
         const SegDesc seg = g.segs[sg];
+        const int bp = g.bra_ids[seg.row];
+        if (g.dtol > 0.0 && g.segq && g.Qs[bp] * g.segq[sg] * g.dmax[0] < g.dtol) continue;   // block-uniform
         for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
         __syncthreads();
-        const int bp = g.bra_ids[seg.row];
         const PairMeta mb = g.meta[bp];
         const int fa = mb.fa, fb = mb.fb;
         const double ABx = g.ab[4 * bp], ABy = g.ab[4 * bp + 1], ABz = g.ab[4 * bp + 2];
@@ -1228,6 +1238,8 @@ int polyeri_zgemm_launch(int mode, int M, int N, int K, const double* Ar, const 
 // Q_ab Q_cd max(|D| over the six blocks the quartet reads) reaches tol.
 int polyeri_shell_dmax(const double* D, int nbf, const int* shell_f0, const int* shell_nf, int nsh,
                        double* dsh, hipStream_t stream);
+// *out = the largest of the n values x (one block).
+int polyeri_max_reduce(const double* x, size_t n, double* out, hipStream_t stream);
 int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shells, const double* dsh,
                         int nsh, double tol, int2* items, unsigned* count, hipStream_t stream);
 // Cooperative (one wavefront per quartet, LDS-resident recursion) kernels;

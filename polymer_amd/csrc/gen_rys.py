@@ -19,8 +19,10 @@ rys_generated.h -- per shell-quartet class and lane variant, one primitive
   quartet's contribution to a lane's share of the contracted integrals (see
   emit_class below).
 
-Run by __graft_entry__.build(); files are rewritten only when their text
-changes.
+Run by __graft_entry__.build(), which regenerates rys_generated.h (files
+are rewritten only when their text changes); the tables, whose 90-digit
+reference values take minutes, only with --tables or when rys_tables.h is
+missing.
 """
 import math
 import os

@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(256) k_eri_rys(EriArgs g) {
     for (int sg = blockIdx.x; sg < g.nseg; sg += gridDim.x) {
         const SegDesc seg = g.segs[sg];
         const int bp = g.bra_ids[seg.row];
+        if (g.dtol > 0.0 && g.segq && g.Qs[bp] * g.segq[sg] * g.dmax[0] < g.dtol) continue;   // block-uniform
         const PairMeta mb = g.meta[bp];
         const int fa = mb.fa, fb = mb.fb;
         for (int i = threadIdx.x; i < total; i += blockDim.x) rys_lds[i] = 0.0;

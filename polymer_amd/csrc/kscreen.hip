@@ -64,6 +64,24 @@ int polyeri_shell_dmax(const double* D, int nbf, const int* shell_f0, const int*
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+__global__ void __launch_bounds__(256) k_max_reduce(const double* __restrict__ x, size_t n, double* __restrict__ out) {
+    __shared__ double m[256];
+    double v = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, x[i]);
+    m[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = m[0];
+}
+
+int polyeri_max_reduce(const double* x, size_t n, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, stream, x, n, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int polyeri_screen_list(const EriArgs& a, const double* Q, const int2* pair_shells, const double* dsh,
                         int nsh, double tol, int2* items, unsigned* count, hipStream_t stream) {
     if (a.nwork <= 0) return 0;

@@ -319,6 +319,43 @@ static std::vector<SegDesc> make_segments(const WorkList& l, const std::vector<P
     for (size_t i = 0; i < segs.size(); ++i) out[i] = segs[i].second;
     return out;
 }
+// Largest ket Schwarz bound kq over the items of each segment.  A row's
+// items cover one contiguous range of ket positions per bucket (the
+// decode_in_row mapping), so a sparse table of range maxima over the ket
+// order answers each range.  With a density-screened build a segment whose
+// bra bound x this x the largest |D| block is below the threshold holds no
+// surviving item and is skipped whole.
+static std::vector<double> segment_ket_bounds(const WorkList& l, const std::vector<SegDesc>& segs,
+                                              const std::vector<int>& kets, const std::vector<int>& bstart,
+                                              const std::vector<double>& kq) {
+    const int P = (int)kets.size();
+    std::vector<double> out(segs.size(), 0.0);
+    if (P == 0) return out;
+    std::vector<std::vector<double>> sp(1, std::vector<double>(P));
+    for (int i = 0; i < P; ++i) sp[0][i] = kq[kets[i]];
+    for (int k = 1; (1 << k) <= P; ++k) {
+        sp.emplace_back(P - (1 << k) + 1);
+        for (int i = 0; i + (1 << k) <= P; ++i) sp[k][i] = std::max(sp[k - 1][i], sp[k - 1][i + (1 << (k - 1))]);
+    }
+    const int nb = l.nbucket;
+    for (size_t s = 0; s < segs.size(); ++s) {
+        const SegDesc& sg = segs[s];
+        const int* rb = l.rb.data() + (size_t)sg.row * (nb + 1);
+        const long long l0 = sg.t0 - l.row[sg.row], l1 = l0 + sg.n;
+        double m = 0.0;
+        for (int b = 0; b < nb && b + 1 < (int)bstart.size(); ++b) {
+            const long long lo = std::max<long long>(l0, rb[b]), hi = std::min<long long>(l1, rb[b + 1]);
+            if (lo >= hi) continue;
+            const int first = l.rf.empty() ? 0 : l.rf[(size_t)sg.row * nb + b];
+            const int p0 = bstart[b] + first + (int)(lo - rb[b]), p1 = p0 + (int)(hi - lo);
+            const int k = 31 - __builtin_clz((unsigned)(p1 - p0));
+            m = std::max(m, std::max(sp[k][p0], sp[k][p1 - (1 << k)]));
+        }
+        out[s] = m;
+    }
+    return out;
+}
+
 struct LaunchCfg {
     int kind = LAUNCH_AUTO, block = 256, grid = 0;
     int segv = 0;   // segment kernels: 0 cost-balanced segments, 1 segments of kSegItems items
@@ -338,6 +375,7 @@ struct ClassPairWork {
     int2* d_items;         // density-screened items (capacity nwork), allocated on first use
     SegDesc* d_seg;        // bra-row segments (molecular lists): nseg cost-balanced ones, then
     int nseg, nsegc;       // nsegc of kSegItems items each (make_segments)
+    double* d_segq;        // per segment of d_seg: the largest ket bound (segment_ket_bounds)
     LaunchCfg cfg;         // launch shape (autotuned or default)
     int prec;              // 0: FP64, 1: FP32 (mixed precision)
     int both;              // periodic lists of two different classes: both bra/ket roles
@@ -356,6 +394,7 @@ struct ClassPairWork {
         long long* d_row = nullptr;
         int* d_rb = nullptr;
         SegDesc* d_seg = nullptr;
+        double* d_segq = nullptr;
         double prim_quartets = 0;
         int kind = 0;   // 0: group pairs on both sides (k_eri_gc); 1: shell-pair bra, group-pair kets (k_eri_gck)
     } gc;
@@ -422,6 +461,7 @@ struct polyeri_ctx {
     int2* d_psh = nullptr;
     int *d_sh_f0 = nullptr, *d_sh_nf = nullptr;
     double* d_dsh = nullptr;
+    double* d_dmax = nullptr;   // largest entry of d_dsh (screened builds)
     unsigned* d_counts = nullptr;
     std::vector<unsigned> h_counts;
     bool screened_last = false, stored_last = false;
@@ -508,6 +548,8 @@ struct SegScreen {
     int nsh;
     double tol;
     unsigned* kept;
+    const double* segq;   // per segment of the launch: largest ket bound (null: no segment test)
+    const double* dmax;   // device scalar: largest |D| block
 };
 
 // Whether a list's J/K launch (unscreened) runs the segment kernel.
@@ -569,7 +611,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
             a.nseg = nseg;
             if (scr) {
                 a.dtol = scr->tol; a.Qs = scr->Q; a.psh = scr->psh; a.dsh = scr->dsh; a.nsh = scr->nsh;
-                a.kept = scr->kept;
+                a.kept = scr->kept; a.segq = scr->segq; a.dmax = scr->dmax;
             }
             if (polyeri_rys_launch(a, mode, la, lb, lc, ld, cfg.block, cfg.grid, c->stream))
                 return fail(POLYERI_E_HIP, "Rys kernel launch failed");
@@ -583,7 +625,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
         a.nseg = nseg;
         if (scr) {
             a.dtol = scr->tol; a.Qs = scr->Q; a.psh = scr->psh; a.dsh = scr->dsh; a.nsh = scr->nsh;
-            a.kept = scr->kept;
+            a.kept = scr->kept; a.segq = scr->segq; a.dmax = scr->dmax;
         }
         const int block = cfg.block == 64 || cfg.block == 128 ? cfg.block : 256;
         unsigned grid = (unsigned)nseg;
@@ -628,7 +670,10 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const S
     a.segs = cw.gc_segv ? g.d_seg + g.nseg : g.d_seg;
     a.nseg = cw.gc_segv ? g.nsegc : g.nseg;
     a.gmeta = c->d_gmeta; a.gprims = c->d_gprims; a.qtol = c->opts.schwarz_tol;
-    if (scr) { a.dtol = scr->tol; a.dsh = scr->dsh; a.nsh = scr->nsh; a.kept = scr->kept; }
+    if (scr) {
+        a.dtol = scr->tol; a.dsh = scr->dsh; a.nsh = scr->nsh; a.kept = scr->kept; a.dmax = scr->dmax;
+        a.segq = g.d_segq ? g.d_segq + (cw.gc_segv ? g.nseg : 0) : nullptr;
+    }
     if (block != 64 && block != 128) block = 256;
     if (g.kind == 1) {   // shell-pair bra
         a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab; a.Qs = c->d_Q; a.psh = c->d_psh;
@@ -747,6 +792,7 @@ static void free_ctx(polyeri_ctx* c) {
         if (w.d_rf) (void)hipFree(w.d_rf);
         if (w.d_items) (void)hipFree(w.d_items);
         if (w.d_seg) (void)hipFree(w.d_seg);
+        (void)hipFree(w.d_segq); (void)hipFree(w.gc.d_segq);
         if (w.gc.d_row) (void)hipFree(w.gc.d_row);
         if (w.gc.d_rb) (void)hipFree(w.gc.d_rb);
         if (w.gc.d_seg) (void)hipFree(w.gc.d_seg);
@@ -758,7 +804,7 @@ static void free_ctx(polyeri_ctx* c) {
     (void)hipFree(c->d_pcell); (void)hipFree(c->d_Dc); (void)hipFree(c->d_Jc); (void)hipFree(c->d_Kc);
     (void)hipFree(c->d_fx_overflow);
     (void)hipFree(c->d_pcc); (void)hipFree(c->d_jcc); (void)hipFree(c->d_kcc);
-    (void)hipFree(c->d_psh); (void)hipFree(c->d_sh_f0); (void)hipFree(c->d_sh_nf); (void)hipFree(c->d_dsh); (void)hipFree(c->d_counts);
+    (void)hipFree(c->d_psh); (void)hipFree(c->d_sh_f0); (void)hipFree(c->d_sh_nf); (void)hipFree(c->d_dsh); (void)hipFree(c->d_counts); (void)hipFree(c->d_dmax);
     (void)hipFree(c->d_ref); (void)hipFree(c->d_scale); (void)hipFree(c->d_D); (void)hipFree(c->d_J); (void)hipFree(c->d_K);
     (void)hipFree(c->d_pc); (void)hipFree(c->d_jc); (void)hipFree(c->d_kc); (void)hipFree(c->d_T); (void)hipFree(c->d_V);
     for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -882,6 +928,11 @@ static int build_gc_lists(polyeri_ctx* c) {
         g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket;
         g.nseg = (int)segs.size(); g.nsegc = (int)segc.size();
         segs.insert(segs.end(), segc.begin(), segc.end());
+        {
+            const std::vector<double> sq = segment_ket_bounds(l, segs, kets[l.Y], bst[l.Y], gQ);
+            if ((rc = dalloc(&g.d_segq, sq.size()))) return rc;
+            HIPCHK(hipMemcpy(g.d_segq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
         g.d_bra = c->d_gbra[l.X]; g.d_ket = c->d_gket[l.Y]; g.d_bstart = c->d_gbstart[l.Y];
         g.d_bsoa = c->d_gbsoa[l.Y];
         g.prim_quartets = l.prim_quartets;
@@ -928,6 +979,13 @@ static int build_gc_lists(polyeri_ctx* c) {
         if (segs.size() + segc.size() > 0x7fffffffu) return fail(POLYERI_E_ARG, "too many segments");
         auto& g = w->gc;
         g.kind = 1;
+        {
+            std::vector<SegDesc> all(segs);
+            all.insert(all.end(), segc.begin(), segc.end());
+            const std::vector<double> sq = segment_ket_bounds(l, all, ck[l.Y], cb[l.Y], cq);
+            if ((rc = dalloc(&g.d_segq, sq.size()))) return rc;
+            HIPCHK(hipMemcpy(g.d_segq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
         g.nwork = l.nwork; g.nrows = l.nrows; g.nbucket = l.nbucket;
         g.nseg = (int)segs.size(); g.nsegc = (int)segc.size();
         segs.insert(segs.end(), segc.begin(), segc.end());
@@ -1110,6 +1168,7 @@ static int build_ctx(polyeri_ctx* c) {
         w.d_rf = nullptr;
         w.d_items = nullptr;
         w.d_seg = nullptr;
+        w.d_segq = nullptr;
         w.nseg = w.nsegc = 0;
         w.prim_quartets = l.prim_quartets;
         w.flops = l.prim_quartets * flop_model(w.la, w.lb, w.lc, w.ld);
@@ -1137,6 +1196,9 @@ static int build_ctx(polyeri_ctx* c) {
             segs.insert(segs.end(), segc.begin(), segc.end());
             if ((rc = dalloc(&w.d_seg, segs.size()))) return rc;
             HIPCHK(hipMemcpy(w.d_seg, segs.data(), segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+            const std::vector<double> sq = segment_ket_bounds(l, segs, c->ket_ids[l.Y], c->bstart[l.Y], c->Q);
+            if ((rc = dalloc(&w.d_segq, sq.size()))) return rc;
+            HIPCHK(hipMemcpy(w.d_segq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
         }
         if (l.prec) {
             if ((rc = dalloc(&w.d_rf, l.rf.size()))) return rc;
@@ -1157,7 +1219,8 @@ static int build_ctx(polyeri_ctx* c) {
         std::vector<int> f0(nsh), nf(nsh);
         for (int i = 0; i < nsh; ++i) { f0[i] = c->ib.shells[i].f0; nf[i] = ncart(c->ib.shells[i].L); }
         if ((rc = dalloc(&c->d_psh, np)) || (rc = dalloc(&c->d_sh_f0, nsh)) || (rc = dalloc(&c->d_sh_nf, nsh)) ||
-            (rc = dalloc(&c->d_dsh, (size_t)nsh * nsh)) || (rc = dalloc(&c->d_counts, c->work.size())))
+            (rc = dalloc(&c->d_dsh, (size_t)nsh * nsh)) || (rc = dalloc(&c->d_counts, c->work.size())) ||
+            (rc = dalloc(&c->d_dmax, 1)))
             return rc;
         HIPCHK(hipMemcpy(c->d_psh, psh.data(), np * sizeof(int2), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_sh_f0, f0.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
@@ -1421,6 +1484,48 @@ int polyeri_set_general_contraction(polyeri_ctx* ctx, int on) {
     return 0;
 }
 
+// Launch configuration of work list `list` as 9 ints: kind (0 auto, 1
+// register, 2 wavefront, 3 segment, 5 Rys), block, grid, segment variant,
+// primitive batch (-1 default), generally contracted form on, its block,
+// its segment variant, and the autotuned time in microseconds (0 if none).
+int polyeri_get_launch_config(const polyeri_ctx* ctx, int list, int* cfg) {
+    if (!ctx || !cfg || list < 0 || list >= (int)ctx->work.size())
+        return fail(POLYERI_E_ARG, "polyeri_get_launch_config: bad context, list or pointer");
+    const ClassPairWork& w = ctx->work[list];
+    const int v[9] = {w.cfg.kind, w.cfg.block, w.cfg.grid, w.cfg.segv, w.cfg.pbs, w.gc_on, w.gc_block, w.gc_segv,
+                      (int)std::lround(w.est_ms * 1e3)};
+    std::copy(v, v + 9, cfg);
+    return 0;
+}
+
+// Sets what polyeri_get_launch_config returns (e.g. the autotuner's picks
+// of an earlier process); rejects kinds that the list cannot run.
+int polyeri_set_launch_config(polyeri_ctx* ctx, int list, const int* cfg) {
+    if (!ctx || !cfg || list < 0 || list >= (int)ctx->work.size())
+        return fail(POLYERI_E_ARG, "polyeri_set_launch_config: bad context, list or pointer");
+    ClassPairWork& w = ctx->work[list];
+    const int kind = cfg[0];
+    if (kind != LAUNCH_AUTO && kind != LAUNCH_REG && kind != LAUNCH_COOP && kind != LAUNCH_SEG && kind != LAUNCH_RYS)
+        return fail(POLYERI_E_ARG, "polyeri_set_launch_config: unknown launch kind");
+    const int mode = w.prec ? MODE_JK32 : MODE_JK;
+    const bool reg = polyeri_class_kernel(w.la, w.lb, w.lc, w.ld, mode) != nullptr;
+    if ((kind == LAUNCH_REG && !reg) || (kind == LAUNCH_SEG && (!reg || w.nseg <= 0 || w.prec)) ||
+        (kind == LAUNCH_RYS && (w.nseg <= 0 || !polyeri_rys_class(w.la, w.lb, w.lc, w.ld))) ||
+        (cfg[5] && w.gc.nwork <= 0))
+        return fail(POLYERI_E_ARG, "polyeri_set_launch_config: the list cannot run this launch kind");
+    w.cfg.kind = kind;
+    w.cfg.block = cfg[1];
+    w.cfg.grid = std::max(0, cfg[2]);
+    w.cfg.segv = cfg[3] ? 1 : 0;
+    w.cfg.pbs = std::max(-1, std::min(3, cfg[4]));
+    w.gc_on = cfg[5] ? 1 : 0;
+    w.gc_block = cfg[6];
+    w.gc_segv = cfg[7] ? 1 : 0;
+    w.est_ms = std::max(0, cfg[8]) * 1e-3;
+    drop_graphs(ctx);
+    return 0;
+}
+
 int polyeri_set_profiling(polyeri_ctx* ctx, int on) {
     if (!ctx) return fail(POLYERI_E_ARG, "null context");
     ctx->profiling = on != 0;
@@ -1483,7 +1588,8 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
     if (screen) {
         const int nsh = (int)c->ib.shells.size();
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(c->work.size())), dim3(256), 0, c->stream, c->d_counts, c->work.size());
-        if (polyeri_shell_dmax(c->d_D, nbf, c->d_sh_f0, c->d_sh_nf, nsh, c->d_dsh, c->stream))
+        if (polyeri_shell_dmax(c->d_D, nbf, c->d_sh_f0, c->d_sh_nf, nsh, c->d_dsh, c->stream) ||
+            polyeri_max_reduce(c->d_dsh, (size_t)nsh * nsh, c->d_dmax, c->stream))
             return fail(POLYERI_E_HIP, "shell density maxima launch failed");
         for (size_t w = 0; w < c->work.size(); ++w) {
             const ClassPairWork& cw = c->work[w];
@@ -1527,7 +1633,8 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
         const ClassPairWork& cw = c->work[w];
         if (c->profiling) HIPCHK(hipEventRecord(c->ev[w], c->stream));
         c->stream = branch[w] ? c->side[branch[w] - 1] : main;   // every launch below goes to c->stream
-        const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w};
+        const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w,
+                            cw.d_segq ? cw.d_segq + (cw.cfg.segv ? cw.nseg : 0) : nullptr, c->d_dmax};
         if (list_uses_gc(cw)) {
             rc = launch_gc(c, cw, cw.gc_block, screen ? &scr : nullptr);
         } else {
@@ -2106,10 +2213,12 @@ __global__ void k_antisym(const double* __restrict__ B, int n, double* __restric
 
 // X (n x n) <- the projector on the nocc lowest eigenvectors of Fp; Y and
 // st (4 doubles) are work space.  *ok = 0 when tr X - tr X^2 did not fall
-// below tol within the step limit (a closed or tiny gap): the caller then
-// diagonalises.  The host reads the idempotency error every 8 steps.
+// below tol within 96 steps (a closed gap): the caller then diagonalises.
+// The host reads the idempotency error first after *steps - 4 steps (the
+// count the previous call needed; at least 8), then every 4 steps, and
+// returns the count used in *steps.
 static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, int nocc, double* X, double* Y,
-                      double* st, int* ok) {
+                      double* st, int* ok, int* steps) {
     const size_t n2 = (size_t)n * n;
     const double one = 1.0, zero = 0.0, tol = 1e-11;
     *ok = 0;
@@ -2117,8 +2226,9 @@ static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, 
     hipLaunchKernelGGL(k_tc2_init, dim3(nblk(n2)), dim3(256), 0, s, Fp, n, st + 2, X);
     HIPCHK(hipGetLastError());
     double err = 0.0;
-    for (int it = 0; it < 160 && !*ok; it += 8) {
-        for (int k = 0; k < 8; ++k) {
+    int done = 0, next = std::max(8, *steps - 4);
+    while (!*ok && done < 96) {
+        for (; done < next; ++done) {
             RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, X, n, &zero, Y,
                                 n));
             hipLaunchKernelGGL(k_tc2_trace, dim3(1), dim3(256), 0, s, X, Y, n, (double)nocc, st);
@@ -2128,7 +2238,9 @@ static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, 
         HIPCHK(hipMemcpyAsync(&err, st + 1, sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (std::isfinite(err) && std::fabs(err) < tol) *ok = 1;
+        next = done + 4;
     }
+    *steps = done;
     return 0;
 }
 
@@ -2199,12 +2311,15 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     const bool cdiis = o->diis_mode == 1;
     double *dXp = nullptr, *dYp = nullptr, *dSt = nullptr, *dFpLast = nullptr, *dHF = nullptr, *dHE = nullptr,
            *dDots = nullptr;
+    double* dEn = nullptr;
+    if ((rc = buf.get(&dEn, 4))) return rc;
     if (want_purify &&
         ((rc = buf.get(&dXp, n2)) || (rc = buf.get(&dYp, n2)) || (rc = buf.get(&dSt, 4)) || (rc = buf.get(&dFpLast, n2))))
         return rc;
     if (cdiis && ((rc = buf.get(&dHF, n2 * 8)) || (rc = buf.get(&dHE, n2 * 8)) || (rc = buf.get(&dDots, 8))))
         return rc;
-    bool gap_open = false, last_purified = false;
+    bool purify_ok = want_purify, last_purified = false;
+    int tc2_steps = 24;
     std::vector<double> cB(64 * 64, 0.0);   // commutator DIIS: <e_i, e_j> of the last 8 iterations (ring)
     // guess
     std::vector<double> P(n2, 0.0);
@@ -2355,13 +2470,13 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
                 }
             }
         }
-        // the density: purification while the gap is open (after the first
-        // iteration's diagonalisation), else the diagonalisation
-        const bool purify_now = want_purify && gap_open && iscf > 1;
+        // the density: purification while it converges (an open gap), else
+        // the diagonalisation; after a diagonalisation purification resumes
+        // only if the HOMO-LUMO gap exceeds 1 mEh
         int purified = 0;
-        if (purify_now) {
+        if (purify_ok) {
             HIPCHK(hipMemcpyAsync(dFpLast, dFp, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s));
-            if ((rc = purify_tc2(h, s, dFp, nind, iocc, dXp, dYp, dSt, &purified))) break;
+            if ((rc = purify_tc2(h, s, dFp, nind, iocc, dXp, dYp, dSt, &purified, &tc2_steps))) break;
         }
         if (purified) {
             // P = C2S^T X (2 X_p) X^T C2S
@@ -2372,9 +2487,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dPout, nc));
         } else {
             if ((rc = diagonalise())) break;
-            if (want_purify && iscf == 1) {   // eps ascending: open when the HOMO-LUMO gap exceeds 1 mEh
-                gap_open = std::isfinite(eps[iocc]) && eps[iocc] < 1e90 && eps[iocc] - eps[iocc - 1] > 1e-3;
-            }
+            if (want_purify)   // eps ascending
+                purify_ok = std::isfinite(eps[iocc]) && eps[iocc] < 1e90 && eps[iocc] - eps[iocc - 1] > 1e-3;
         }
         last_purified = purified != 0;
         {
@@ -2440,10 +2554,18 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         // with commutator DIIS the input density is idempotent (the previous
         // P_out) and the energy is that of the input density and its own J/K,
         // stationary to second order
-        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, &ek));
-        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, &ev));
-        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_jc, 1, &ej));
-        RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_kc, 1, &ex));
+        {   // four products, one synchronisation
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, dEn));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, dEn + 1));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_jc, 1, dEn + 2));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_kc, 1, dEn + 3));
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+            double en[4];
+            HIPCHK(hipMemcpyAsync(en, dEn, sizeof en, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            ek = en[0]; ev = en[1]; ej = en[2]; ex = en[3];
+        }
         ej *= 0.5;
         ex *= -0.25 * o->hfexchange;
         etot = ctx->e_nuc + ek + ev + ej + ex;

@@ -88,32 +88,43 @@ def pair_list(atoms):
     otherwise, distance below CUTOFF (diagonal pairs included)."""
     xyz = np.array([a[1:] for a in atoms])
     z = np.array([a[0] for a in atoms])
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=2)
+    names = {(8, 8): "OO", (8, 1): "OH", (1, 1): "HH"}
     out = []
-    for i in range(len(atoms)):
-        for j in range(i, len(atoms)):
-            if np.linalg.norm(xyz[i] - xyz[j]) >= CUTOFF:
-                continue
-            a, b = (i, j) if not (z[i] == 1 and z[j] == 8) else (j, i)
-            t = {(8, 8): "OO", (8, 1): "OH", (1, 1): "HH"}[(int(z[a]), int(z[b]))]
-            out.append((a, b, t))
+    for i, j in zip(*np.nonzero(np.triu(d < CUTOFF))):
+        i, j = int(i), int(j)
+        a, b = (i, j) if not (z[i] == 1 and z[j] == 8) else (j, i)
+        out.append((a, b, names[(int(z[a]), int(z[b]))]))
     return out
 
 
 def pair_features(atoms, pairs):
+    """Invariant features of every pair (one row each, NFEAT columns): same
+    molecule, same atom, the position of j in i's frame, the distance and
+    its radial-basis expansion, the relative orientation of the two frames,
+    and the neighbour densities of both atoms."""
     xyz = np.array([a[1:] for a in atoms])
     z = np.array([a[0] for a in atoms])
-    fr = frames(atoms)
+    fr = np.array(frames(atoms))                       # (molecules, 3, 3)
     env = _env(xyz, z)
     rbf = np.linspace(0.0, 6.0, 8)
     X = np.zeros((len(pairs), NFEAT), dtype=np.float32)
-    for r, (i, j, _) in enumerate(pairs):
-        Fi, Fj = fr[i // 3], fr[j // 3]
-        d = xyz[j] - xyz[i]
-        dl = Fi.T @ d / ANG
-        dist = np.linalg.norm(d) / ANG
-        f = [float(i // 3 == j // 3), float(i == j), *(dl / 5.0), dist / 5.0,
-             *np.exp(-((dist - rbf) / 0.75) ** 2), *(Fi.T @ Fj).ravel(), *env[i], *env[j]]
-        X[r, :len(f)] = f
+    if not pairs:
+        return X
+    i = np.array([p[0] for p in pairs])
+    j = np.array([p[1] for p in pairs])
+    Fi, Fj = fr[i // 3], fr[j // 3]
+    d = xyz[j] - xyz[i]
+    dl = np.einsum("pxa,px->pa", Fi, d) / ANG
+    dist = np.linalg.norm(d, axis=1) / ANG
+    X[:, 0] = i // 3 == j // 3
+    X[:, 1] = i == j
+    X[:, 2:5] = dl / 5.0
+    X[:, 5] = dist / 5.0
+    X[:, 6:14] = np.exp(-((dist[:, None] - rbf[None, :]) / 0.75) ** 2)
+    X[:, 14:23] = np.einsum("pxa,pxb->pab", Fi, Fj).reshape(-1, 9)
+    X[:, 23:35] = env[i]
+    X[:, 35:47] = env[j]
     return X
 
 
@@ -127,13 +138,15 @@ def _dfact(n):
 def atom_functions(basis):
     """Per atom: reference indices of its real functions and their Cartesian
     exponents, in the reference order."""
-    cen = np.stack([basis.cgsx, basis.cgsy, basis.cgsz], axis=1)
+    real = np.asarray(basis.real_functions, dtype=np.int64)
+    cen = np.stack([basis.cgsx, basis.cgsy, basis.cgsz], axis=1)[real]
+    axyz = np.array([at[1:] for at in basis.atoms], dtype=np.float64)
+    owner = np.argmin(np.linalg.norm(cen[:, None, :] - axyz[None, :, :], axis=2), axis=1)
+    first = np.argmax(np.asarray(basis.cc)[real] != 0, axis=1)      # first primitive of each function
+    lx, ly, lz = (np.asarray(a)[first] for a in (basis.pax, basis.pay, basis.paz))
     funcs = [[] for _ in basis.atoms]
-    for c in basis.real_functions:
-        p = int(np.nonzero(basis.cc[c])[0][0])
-        lxyz = (int(basis.pax[p]), int(basis.pay[p]), int(basis.paz[p]))
-        a = int(np.argmin([np.linalg.norm(cen[c] - np.array(at[1:])) for at in basis.atoms]))
-        funcs[a].append((int(c), lxyz))
+    for c, a, x, y, z in zip(real.tolist(), owner.tolist(), lx.tolist(), ly.tolist(), lz.tolist()):
+        funcs[a].append((c, (x, y, z)))
     return funcs
 
 
@@ -309,23 +322,36 @@ class GuessModel:
 
     # -- inference --------------------------------------------------------
     def predict(self, ctx, basis, atoms, S=None):
-        """Guess density (ncgs x ncgs, reference layout, Fortran order)."""
-        smp, fl, Tm = self.samples(atoms, basis)
-        P = np.zeros((basis.ncgs, basis.ncgs), order="F")
+        """Guess density (ncgs x ncgs, reference layout, Fortran order).
+        Per type, the blocks of all pairs are rotated back and scattered in
+        batched array operations."""
+        pairs = pair_list(atoms)
+        X = pair_features(atoms, pairs)
+        fl = atom_functions(basis)
+        fr = frames(atoms)
+        Tm = [frame_transform(fr[a // 3], fl[a]) for a in range(len(atoms))]
+        idx = [np.array([c for c, _ in f], dtype=np.int64) for f in fl]
+        P = np.zeros((basis.ncgs, basis.ncgs))
         for t in TYPES:
-            X, _, ij = smp[t]
-            if not X:
+            rows = [r for r, p in enumerate(pairs) if p[2] == t]
+            if not rows:
                 continue
-            Xn = (np.array(X) - self.xmu[t]) / self.xsd[t]
+            Xn = (X[rows] - self.xmu[t]) / self.xsd[t]
             Y = mlp_forward(self.nets[t], ctx, Xn) * self.ysd[t] + self.ymu[t]
-            for (i, j), y in zip(ij, Y):
-                ii = [c for c, _ in fl[i]]
-                jj = [c for c, _ in fl[j]]
-                blk = Tm[i] @ y.reshape(len(ii), len(jj)).astype(np.float64) @ Tm[j].T
-                if i == j:
-                    blk = 0.5 * (blk + blk.T)
-                P[np.ix_(ii, jj)] = blk
-                P[np.ix_(jj, ii)] = blk.T
+            i = np.array([pairs[r][0] for r in rows])
+            j = np.array([pairs[r][1] for r in rows])
+            ni, nj = len(fl[i[0]]), len(fl[j[0]])
+            Ti = np.stack([Tm[a] for a in i])
+            Tj = np.stack([Tm[a] for a in j])
+            blk = Ti @ Y.reshape(-1, ni, nj).astype(np.float64) @ np.transpose(Tj, (0, 2, 1))
+            same = i == j
+            if same.any():    # diagonal blocks (types OO, HH only)
+                blk[same] = 0.5 * (blk[same] + np.transpose(blk[same], (0, 2, 1)))
+            I = np.stack([idx[a] for a in i])[:, :, None]
+            J = np.stack([idx[a] for a in j])[:, None, :]
+            P[I, J] = blk
+            P[J.transpose(0, 2, 1), I.transpose(0, 2, 1)] = np.transpose(blk, (0, 2, 1))
+        P = np.asfortranarray(P)
         if S is not None:
             nel = sum(a[0] for a in atoms)
             tr = float(np.sum(P * S))

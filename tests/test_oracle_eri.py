@@ -115,3 +115,22 @@ def test_screening_drops_only_small_quartets(h2o):
     j1, k1, i1 = b.fourindex_eri(p, 1e-10)
     assert i1["composite_quartets"] < i0["composite_quartets"]
     assert np.max(np.abs(j1 - j0)) < 1e-8 and np.max(np.abs(k1 - k0)) < 1e-8
+
+
+def test_threaded_rows_equal_single_thread():
+    """The all-cores CPU baseline runs the same loop nest over threads: the
+    primitive-basis C and X equal the single-threaded ones (summation order
+    only) and every quartet is counted once."""
+    import oracle
+    from polymer_amd import geometry as G
+    atoms = G.water_cluster(2, 1)
+    ob = oracle.Basis(basis_path("6-31Gss"), atoms)
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((ob.ncgs, ob.ncgs))
+    p = p + p.T
+    rows = list(range(0, ob.npshell ** 2, 5))
+    a = ob.fourindex_eri_rows_mt(p, 1e-12, rows, 1)
+    b = ob.fourindex_eri_rows_mt(p, 1e-12, rows, 4)
+    nq, nreal, _, _ = ob.fourindex_eri_rows(p, 1e-12, rows)
+    assert a[0] == b[0] == nq and a[1] == b[1] == nreal
+    assert np.max(np.abs(a[4] - b[4])) < 1e-11 and np.max(np.abs(a[5] - b[5])) < 1e-11
