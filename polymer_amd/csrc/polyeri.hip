@@ -402,6 +402,20 @@ struct ClassPairWork {
     double est_ms = 0;     // launch time of the chosen form (autotuner), else 0: model FLOPs order the lists
 };
 
+struct ScfArena {
+    std::vector<std::pair<void*, size_t>> bufs;
+    size_t next = 0;
+    int get_bytes(void** p, size_t bytes);
+    template <typename T>
+    int get(T** p, size_t n) {
+        void* q = nullptr;
+        const int rc = get_bytes(&q, n * sizeof(T));
+        *p = static_cast<T*>(q);
+        return rc;
+    }
+    void release();
+};
+
 struct polyeri_ctx {
     RefBasis rb;
     IntBasis ib;
@@ -487,6 +501,13 @@ struct polyeri_ctx {
     int* d_inv = nullptr;
     bool use_stored = false;
     double* d_noalloc = nullptr;   // this device's address of `scratch`
+    // polyeri_scf: work buffers, and the one-electron matrices, C2S and the
+    // canonical orthogonaliser of this geometry (kept per (spherical, lindep))
+    ScfArena scf_arena;
+    bool scf_orth = false;
+    int scf_spherical = 0, scf_nind = 0;
+    double scf_lindep = 0.0;
+    double *d_scfT = nullptr, *d_scfV = nullptr, *d_scfS = nullptr, *d_scfC2S = nullptr, *d_scfX = nullptr;
     bool tuned = false;
     // periodic lattice sums along x (HELIX = 0): cells of the density and
     // of J/K in [-cel1, cel1], Coulomb sum over Q3 in [-cel2, cel2], pair
@@ -656,6 +677,16 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
 // Whether a list's J/K launches run its generally contracted form.
 static bool list_uses_gc(const ClassPairWork& cw) { return cw.gc_on && cw.gc.nwork > 0; }
 
+// In a density-screened build: whether the list screens inside its segment
+// (or generally contracted, or Rys) kernel; the others have their surviving
+// items compacted first (k_screen) for the thread or wavefront kernel.  (A
+// compacting variant for every list with a thread kernel measured 3x
+// slower on the (H2O)16 incremental builds: the compaction pass and the
+// per-quartet J/K atomics cost more than the idle lanes it removes.)
+static bool screens_in_kernel(const polyeri_ctx* c, const ClassPairWork& cw) {
+    return list_uses_seg(c, cw) || list_uses_gc(cw);
+}
+
 // J/K launch of a list's generally contracted form (one block per segment),
 // density-screened inside the kernel when scr is given.
 static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const SegScreen* scr = nullptr) {
@@ -811,6 +842,9 @@ static void free_ctx(polyeri_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     (void)hipFree(c->d_ao); (void)hipFree(c->d_inv);
+    c->scf_arena.release();
+    (void)hipFree(c->d_scfT); (void)hipFree(c->d_scfV); (void)hipFree(c->d_scfS); (void)hipFree(c->d_scfC2S);
+    (void)hipFree(c->d_scfX);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     for (auto e : c->ev_join)
         if (e) (void)hipEventDestroy(e);
@@ -1593,7 +1627,7 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
             return fail(POLYERI_E_HIP, "shell density maxima launch failed");
         for (size_t w = 0; w < c->work.size(); ++w) {
             const ClassPairWork& cw = c->work[w];
-            if (cw.nwork <= 0 || list_uses_seg(c, cw) || list_uses_gc(cw)) continue;   // these screen in the kernel
+            if (cw.nwork <= 0 || screens_in_kernel(c, cw)) continue;   // these screen in the kernel
             EriArgs a{};
             a.bra_ids = cw.d_bra; a.ket_ids = cw.d_ket; a.row_off = cw.d_row; a.nrows = cw.nrows; a.nwork = cw.nwork;
             a.rb_off = cw.d_rb; a.bucket_start = cw.d_bstart; a.nbucket = cw.nbucket; a.rb_first = cw.d_rf;
@@ -1635,10 +1669,11 @@ static int jk_enqueue(polyeri_ctx* c, const double* d_pc, double* d_jc, double* 
         c->stream = branch[w] ? c->side[branch[w] - 1] : main;   // every launch below goes to c->stream
         const SegScreen scr{c->d_Q, c->d_psh, c->d_dsh, (int)c->ib.shells.size(), dens_tol, c->d_counts + w,
                             cw.d_segq ? cw.d_segq + (cw.cfg.segv ? cw.nseg : 0) : nullptr, c->d_dmax};
-        if (list_uses_gc(cw)) {
+        const bool compact = screen && !screens_in_kernel(c, cw);
+        if (list_uses_gc(cw) && !compact) {
             rc = launch_gc(c, cw, cw.gc_block, screen ? &scr : nullptr);
         } else {
-            const bool seg = list_uses_seg(c, cw);
+            const bool seg = list_uses_seg(c, cw) && !compact;
             rc = launch_class(c, cw.prec ? MODE_JK32 : MODE_JK, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket,
                               cw.d_row, cw.nrows, cw.nwork, nullptr, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf,
                               screen && !seg ? cw.d_items : nullptr, screen && !seg ? c->d_counts + w : nullptr,
@@ -1690,7 +1725,7 @@ static int jk_device(polyeri_ctx* c, const double* d_pc, double* d_jc, double* d
     const bool screen = dens_tol > 0.0;
     if (screen)
         for (ClassPairWork& w : c->work)
-            if (!w.d_items && w.nwork > 0 && !list_uses_seg(c, w) && !list_uses_gc(w)) {   // these keep no items
+            if (!w.d_items && w.nwork > 0 && !screens_in_kernel(c, w)) {   // the others keep no items
                 int rc = dalloc(&w.d_items, (size_t)w.nwork);
                 if (rc) return rc;
             }
@@ -2119,6 +2154,36 @@ struct DevBuf {
     }
 };
 
+// Device buffers of polyeri_scf kept by the context between calls: the k-th
+// request of a call gets the k-th buffer (re-allocated only when it must
+// grow), so repeated SCFs of one geometry allocate nothing after the first.
+// Every call initialises what it reads.
+int ScfArena::get_bytes(void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (next < bufs.size() && bufs[next].second >= bytes) {
+        *p = bufs[next++].first;
+        return 0;
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(POLYERI_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (next < bufs.size()) {
+        (void)hipFree(bufs[next].first);
+        bufs[next] = {q, bytes};
+    } else {
+        bufs.push_back({q, bytes});
+    }
+    ++next;
+    *p = q;
+    return 0;
+}
+
+void ScfArena::release() {
+    for (auto& b : bufs) (void)hipFree(b.first);
+    bufs.clear();
+    next = 0;
+}
+
 static int ensure_blas(polyeri_ctx* c) {
     if (!c->blas) {
         RBCHK(rocblas_create_handle(&c->blas));
@@ -2139,40 +2204,62 @@ static int ensure_blas(polyeri_ctx* c) {
 // whenever the HOMO-LUMO gap is open.  Every step is one FP64 product and
 // two small kernels; the step decision stays on the device.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_gershgorin(const double* __restrict__ A, int n, double* __restrict__ out) {
-    __shared__ double lo[256], hi[256];
+// Gershgorin interval of column i of a symmetric n x n matrix (one
+// wavefront per column): lohi[2i] = a_ii - r_i, lohi[2i+1] = a_ii + r_i.
+__global__ void __launch_bounds__(64) k_gershgorin(const double* __restrict__ A, int n, double* __restrict__ lohi) {
+    const int i = blockIdx.x;
+    const double* col = A + (size_t)n * i;   // column i = row i
+    double r = 0.0;
+    for (int j = threadIdx.x; j < n; j += 64) r += j == i ? 0.0 : fabs(col[j]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (threadIdx.x == 0) {
+        lohi[2 * i] = col[i] - r;
+        lohi[2 * i + 1] = col[i] + r;
+    }
+}
+
+// Block-wide min / max of the n Gershgorin intervals (every block reduces
+// them itself, 2n loads from L2).
+__device__ void gersh_bounds(const double* __restrict__ lohi, int n, double* lo, double* hi) {
+    __shared__ double sl[256], sh[256];
     double mn = 1e300, mx = -1e300;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double* col = A + (size_t)n * i;   // column i = row i (symmetric)
-        double r = 0.0;
-        for (int j = 0; j < n; ++j) r += j == i ? 0.0 : fabs(col[j]);
-        mn = fmin(mn, col[i] - r);
-        mx = fmax(mx, col[i] + r);
+        mn = fmin(mn, lohi[2 * i]);
+        mx = fmax(mx, lohi[2 * i + 1]);
     }
-    lo[threadIdx.x] = mn;
-    hi[threadIdx.x] = mx;
+    sl[threadIdx.x] = mn;
+    sh[threadIdx.x] = mx;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) {
-            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + h]);
-            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + h]);
+            sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + h]);
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + h]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[0] = lo[0]; out[1] = hi[0]; }
+    *lo = sl[0];
+    *hi = sh[0];
 }
 
-__global__ void k_tc2_init(const double* __restrict__ F, int n, const double* __restrict__ b, double* __restrict__ X) {
+// X = (e_max I - F) / (e_max - e_min); blocks of 256 threads.
+__global__ void __launch_bounds__(256) k_tc2_init(const double* __restrict__ F, int n, const double* __restrict__ lohi,
+                                                  double* __restrict__ X) {
+    double lo, hi;
+    gersh_bounds(lohi, n, &lo, &hi);
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n * n) return;
-    const double w = b[1] - b[0];
-    X[t] = ((t % n == t / n ? b[1] : 0.0) - F[t]) / (w > 0.0 ? w : 1.0);
+    const double w = hi - lo;
+    X[t] = ((t % n == t / n ? hi : 0.0) - F[t]) / (w > 0.0 ? w : 1.0);
 }
 
-// st[0] = 1: X <- X^2, 0: X <- 2X - X^2; st[1] = tr X - tr X^2 (the
-// idempotency error of X before the step).  Y = X X.
-__global__ void __launch_bounds__(256) k_tc2_trace(const double* __restrict__ X, const double* __restrict__ Y, int n,
-                                                   double nocc, double* __restrict__ st) {
+// One TC2 step after Y = X X, into Z (X and Y are only read, so no block
+// overwrites a diagonal element another block still sums): every block
+// sums the two diagonals itself (2n loads), takes Z = X^2 when tr X^2 is
+// nearer to nocc than 2 tr X - tr X^2, else Z = 2X - X^2; block 0 records
+// the idempotency error tr X - tr X^2 of X in st[1].  Blocks of 256.
+__global__ void __launch_bounds__(256) k_tc2_step(const double* __restrict__ X, const double* __restrict__ Y, int n,
+                                                  double nocc, double* __restrict__ st, double* __restrict__ Z) {
     __shared__ double sx[256], sy[256];
     double a = 0.0, c = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -2189,18 +2276,12 @@ __global__ void __launch_bounds__(256) k_tc2_trace(const double* __restrict__ X,
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double tx = sx[0], ty = sy[0];
-        st[0] = fabs(ty - nocc) < fabs(2.0 * tx - ty - nocc) ? 1.0 : 0.0;
-        st[1] = tx - ty;
-    }
-}
-
-__global__ void k_tc2_update(double* __restrict__ X, const double* __restrict__ Y, size_t n2,
-                             const double* __restrict__ st) {
+    const double tx = sx[0], ty = sy[0];
+    const bool square = fabs(ty - nocc) < fabs(2.0 * tx - ty - nocc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[1] = tx - ty;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n2) return;
-    X[t] = st[0] != 0.0 ? Y[t] : 2.0 * X[t] - Y[t];
+    if (t >= (size_t)n * n) return;
+    Z[t] = square ? Y[t] : 2.0 * X[t] - Y[t];
 }
 
 // E = B - B^T (n x n): the commutator F P S - S P F from B = F P S.
@@ -2211,19 +2292,19 @@ __global__ void k_antisym(const double* __restrict__ B, int n, double* __restric
     E[t] = B[t] - B[i * n + j];
 }
 
-// X (n x n) <- the projector on the nocc lowest eigenvectors of Fp; Y and
-// st (4 doubles) are work space.  *ok = 0 when tr X - tr X^2 did not fall
+// *result <- the projector on the nocc lowest eigenvectors of Fp (one of
+// the n x n buffers X, Z); Y, st (4 doubles) and lohi (2n) are work space.  *ok = 0 when tr X - tr X^2 did not fall
 // below tol within 96 steps (a closed gap): the caller then diagonalises.
 // The host reads the idempotency error first after *steps - 4 steps (the
 // count the previous call needed; at least 8), then every 4 steps, and
 // returns the count used in *steps.
 static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, int nocc, double* X, double* Y,
-                      double* st, int* ok, int* steps) {
+                      double* Z, double* st, double* lohi, int* ok, int* steps, double** result) {
     const size_t n2 = (size_t)n * n;
     const double one = 1.0, zero = 0.0, tol = 1e-11;
     *ok = 0;
-    hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, s, Fp, n, st + 2);
-    hipLaunchKernelGGL(k_tc2_init, dim3(nblk(n2)), dim3(256), 0, s, Fp, n, st + 2, X);
+    hipLaunchKernelGGL(k_gershgorin, dim3(n), dim3(64), 0, s, Fp, n, lohi);
+    hipLaunchKernelGGL(k_tc2_init, dim3(nblk(n2)), dim3(256), 0, s, Fp, n, lohi, X);
     HIPCHK(hipGetLastError());
     double err = 0.0;
     int done = 0, next = std::max(8, *steps - 4);
@@ -2231,8 +2312,8 @@ static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, 
         for (; done < next; ++done) {
             RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X, n, X, n, &zero, Y,
                                 n));
-            hipLaunchKernelGGL(k_tc2_trace, dim3(1), dim3(256), 0, s, X, Y, n, (double)nocc, st);
-            hipLaunchKernelGGL(k_tc2_update, dim3(nblk(n2)), dim3(256), 0, s, X, Y, n2, st);
+            hipLaunchKernelGGL(k_tc2_step, dim3(nblk(n2)), dim3(256), 0, s, X, Y, n, (double)nocc, st, Z);
+            std::swap(X, Z);   // the step's result is the next X
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(&err, st + 1, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2241,6 +2322,7 @@ static int purify_tc2(rocblas_handle h, hipStream_t s, const double* Fp, int n, 
         next = done + 4;
     }
     *steps = done;
+    *result = X;
     return 0;
 }
 
@@ -2257,11 +2339,26 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     int nc = rb.ncgs;
     size_t n2 = (size_t)nc * nc;
     hipStream_t s = ctx->stream;
-    DevBuf buf;
-    double *dT, *dV, *dS, *dC2S, *dF, *dW1, *dW2, *dX, *dE, *dWk, *dCO, *dP, *dPout, *dOcc, *dFp;
+    // POLYERI_SCF_PHASES=1: wall time per phase (the stream drained at each
+    // boundary), printed to stderr at the end -- a diagnostic
+    static const bool phases = std::getenv("POLYERI_SCF_PHASES") != nullptr;
+    double ph[6] = {0, 0, 0, 0, 0, 0};   // J/K + Fock, F', DIIS, density, bookkeeping + energy, final orbitals
+    auto tick = [&]() {
+        if (phases) (void)hipStreamSynchronize(s);
+        return std::chrono::steady_clock::now();
+    };
+    auto tnow = tick();
+    auto lap = [&](int k) {
+        if (!phases) return;
+        const auto t = tick();
+        ph[k] += std::chrono::duration<double>(t - tnow).count();
+        tnow = t;
+    };
+    ScfArena& buf = ctx->scf_arena;
+    buf.next = 0;
+    double *dF, *dW1, *dW2, *dE, *dWk, *dCO, *dP, *dPout, *dOcc, *dFp;
     int* dInfo;
-    if ((rc = buf.get(&dT, n2)) || (rc = buf.get(&dV, n2)) || (rc = buf.get(&dS, n2)) || (rc = buf.get(&dC2S, n2)) ||
-        (rc = buf.get(&dF, n2)) || (rc = buf.get(&dW1, n2)) || (rc = buf.get(&dW2, n2)) || (rc = buf.get(&dX, n2)) ||
+    if ((rc = buf.get(&dF, n2)) || (rc = buf.get(&dW1, n2)) || (rc = buf.get(&dW2, n2)) ||
         (rc = buf.get(&dE, nc)) || (rc = buf.get(&dWk, nc)) || (rc = buf.get(&dCO, n2)) || (rc = buf.get(&dP, n2)) ||
         (rc = buf.get(&dPout, n2)) || (rc = buf.get(&dOcc, nc)) || (rc = buf.get(&dFp, n2)) || (rc = buf.get(&dInfo, 1)))
         return rc;
@@ -2276,28 +2373,46 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     const int HMAX = 9;   // DIIS uses at most 8 previous vectors
     double *dHP, *dHD;
     if ((rc = buf.get(&dHP, n2 * HMAX)) || (rc = buf.get(&dHD, n2 * HMAX))) return rc;
-    std::vector<double> c2s;
-    if ((rc = form_c2s(rb, o->spherical, c2s))) return rc;
-    HIPCHK(hipMemcpy(dT, ctx->T_C.data(), n2 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dV, ctx->N_C.data(), n2 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dS, ctx->S_C.data(), n2 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dC2S, c2s.data(), n2 * 8, hipMemcpyHostToDevice));
     const double one = 1.0, zero = 0.0;
-    // canonical orthogonaliser of C2S S C2S^T (src/pfock.f90:270-281)
-    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dS, nc, &zero, dW1, nc));
-    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dW2, nc));
-    RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nc, dW2, nc, dE, dWk, dInfo));
-    std::vector<double> e(nc);
-    HIPCHK(hipMemcpyAsync(e.data(), dE, nc * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::vector<int> keep;
-    for (int j = 0; j < nc; ++j) if (e[j] > o->lindep) keep.push_back(j);
-    int nind = (int)keep.size();
-    if (nind == 0) return fail(POLYERI_E_ARG, "no linearly independent functions");
-    int* dKeep;
-    if ((rc = buf.get(&dKeep, nind))) return rc;
-    HIPCHK(hipMemcpy(dKeep, keep.data(), nind * sizeof(int), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_orth, dim3(nblk((size_t)nc * nind)), dim3(256), 0, s, dW2, dE, dKeep, nc, nind, dX);
+    // T, N, S, C2S and the canonical orthogonaliser of C2S S C2S^T
+    // (src/pfock.f90:270-281): computed once per geometry and (spherical,
+    // LINDEP), kept by the context for later SCFs
+    if (!ctx->scf_orth || ctx->scf_spherical != o->spherical || ctx->scf_lindep != o->lindep) {
+        ctx->scf_orth = false;
+        for (double** d : {&ctx->d_scfT, &ctx->d_scfV, &ctx->d_scfS, &ctx->d_scfC2S, &ctx->d_scfX})
+            if (!*d && (rc = dalloc(d, n2))) return rc;
+        std::vector<double> c2s;
+        if ((rc = form_c2s(rb, o->spherical, c2s))) return rc;
+        HIPCHK(hipMemcpy(ctx->d_scfT, ctx->T_C.data(), n2 * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_scfV, ctx->N_C.data(), n2 * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_scfS, ctx->S_C.data(), n2 * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_scfC2S, c2s.data(), n2 * 8, hipMemcpyHostToDevice));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, ctx->d_scfC2S, nc,
+                            ctx->d_scfS, nc, &zero, dW1, nc));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc,
+                            ctx->d_scfC2S, nc, &zero, dW2, nc));
+        RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nc, dW2, nc, dE, dWk, dInfo));
+        std::vector<double> e(nc);
+        HIPCHK(hipMemcpyAsync(e.data(), dE, nc * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<int> keep;
+        for (int j = 0; j < nc; ++j) if (e[j] > o->lindep) keep.push_back(j);
+        if (keep.empty()) return fail(POLYERI_E_ARG, "no linearly independent functions");
+        DevBuf once;   // not from the arena, so its slots keep their sizes from call to call
+        int* dKeep;
+        if ((rc = once.get(&dKeep, keep.size()))) return rc;
+        HIPCHK(hipMemcpy(dKeep, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_orth, dim3(nblk((size_t)nc * keep.size())), dim3(256), 0, s, dW2, dE, dKeep, nc,
+                           (int)keep.size(), ctx->d_scfX);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));   // before `once` frees dKeep
+        ctx->scf_nind = (int)keep.size();
+        ctx->scf_spherical = o->spherical;
+        ctx->scf_lindep = o->lindep;
+        ctx->scf_orth = true;
+    }
+    double *dT = ctx->d_scfT, *dV = ctx->d_scfV, *dS = ctx->d_scfS, *dC2S = ctx->d_scfC2S, *dX = ctx->d_scfX;
+    const int nind = ctx->scf_nind;
     // electrons (src/porbitals.f90:15-19)
     int nel = 0;
     for (int z : rb.iatom) nel += z;
@@ -2312,9 +2427,11 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
     double *dXp = nullptr, *dYp = nullptr, *dSt = nullptr, *dFpLast = nullptr, *dHF = nullptr, *dHE = nullptr,
            *dDots = nullptr;
     double* dEn = nullptr;
-    if ((rc = buf.get(&dEn, 4))) return rc;
+    if ((rc = buf.get(&dEn, 5))) return rc;
+    double *dZp = nullptr, *dLohi = nullptr;
     if (want_purify &&
-        ((rc = buf.get(&dXp, n2)) || (rc = buf.get(&dYp, n2)) || (rc = buf.get(&dSt, 4)) || (rc = buf.get(&dFpLast, n2))))
+        ((rc = buf.get(&dXp, n2)) || (rc = buf.get(&dYp, n2)) || (rc = buf.get(&dSt, 4)) || (rc = buf.get(&dFpLast, n2)) ||
+         (rc = buf.get(&dZp, n2)) || (rc = buf.get(&dLohi, 2 * (size_t)nc))))
         return rc;
     if (cdiis && ((rc = buf.get(&dHF, n2 * 8)) || (rc = buf.get(&dHE, n2 * 8)) || (rc = buf.get(&dDots, 8))))
         return rc;
@@ -2396,6 +2513,7 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         return 0;
     };
     for (iscf = 1; iscf <= o->scfcycles; ++iscf) {
+        lap(4);
         HIPCHK(hipEventRecord(ef0, s));
         const bool inc_now = o->incremental > 0 && iscf > 1 && since_full < o->incremental;
         if (inc_now) {
@@ -2417,11 +2535,13 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
 
         hipLaunchKernelGGL(k_fock, dim3(nblk(n2)), dim3(256), 0, s, dT, dV, ctx->d_jc, ctx->d_kc, o->hfexchange, nc, dF);
         HIPCHK(hipEventRecord(ef1, s));
+        lap(0);
         // C2S F C2S^T, X^T (.) X
         RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dF, nc, &zero, dW1, nc));
         RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dW2, nc));
         RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nc, &one, dW2, nc, dX, nc, &zero, dW1, nc));
         RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nind, nind, nc, &one, dX, nc, dW1, nc, &zero, dFp, nind));
+        lap(1);
         // commutator DIIS (diis_mode 1): error F P S - S P F of this
         // iteration's input density, in the orthonormal basis; F' is replaced
         // by the extrapolation over the last (up to 8) iterations
@@ -2470,18 +2590,21 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
                 }
             }
         }
+        lap(2);
         // the density: purification while it converges (an open gap), else
         // the diagonalisation; after a diagonalisation purification resumes
         // only if the HOMO-LUMO gap exceeds 1 mEh
         int purified = 0;
+        double* dProj = nullptr;
         if (purify_ok) {
             HIPCHK(hipMemcpyAsync(dFpLast, dFp, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s));
-            if ((rc = purify_tc2(h, s, dFp, nind, iocc, dXp, dYp, dSt, &purified, &tc2_steps))) break;
+            if ((rc = purify_tc2(h, s, dFp, nind, iocc, dXp, dYp, dZp, dSt, dLohi, &purified, &tc2_steps, &dProj)))
+                break;
         }
         if (purified) {
             // P = C2S^T X (2 X_p) X^T C2S
             const double two = 2.0;
-            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &two, dX, nc, dXp, nind, &zero, dW1, nc));
+            RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nind, nind, &two, dX, nc, dProj, nind, &zero, dW1, nc));
             RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nc, nc, nind, &one, dW1, nc, dX, nc, &zero, dW2, nc));
             RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nc, nc, nc, &one, dC2S, nc, dW2, nc, &zero, dW1, nc));
             RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, dW1, nc, dC2S, nc, &zero, dPout, nc));
@@ -2491,6 +2614,7 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
                 purify_ok = std::isfinite(eps[iocc]) && eps[iocc] < 1e90 && eps[iocc] - eps[iocc - 1] > 1e-3;
         }
         last_purified = purified != 0;
+        lap(3);
         {
             fock_total += event_ms(ef0, ef1) * 1e-3;
             jk_total += event_ms(ctx->ev0, ctx->ev1) * 1e-3;
@@ -2498,24 +2622,44 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             build_quartets += ctx->last_quartets;
         }
         // STORE_DENSITYMATRIX: Delta P, RMS change, Pulay row (src/pdensitymatrix.f90:131-208)
-        int slot = (iscf - 1) % HMAX;
-        double* hp = dHP + n2 * slot;
-        double* hd = dHD + n2 * slot;
-        HIPCHK(hipMemcpyAsync(hp, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(hd, dPout, n2 * 8, hipMemcpyDeviceToDevice, s));
-        const double mone = -1.0;
-        RBCHK(rocblas_daxpy(h, (int)n2, &mone, dP, 1, hd, 1));
-        RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
         double dchg = 0.0;
-        RBCHK(rocblas_ddot(h, (int)n2, hd, 1, hd, 1, &dchg));
-        pulay[iscf + (size_t)ld * iscf] = dchg;
-        for (int i = std::max(1, iscf - HMAX + 1); i < iscf && !cdiis; ++i) {
-            double v = 0.0;
-            RBCHK(rocblas_ddot(h, (int)n2, dHD + n2 * ((i - 1) % HMAX), 1, hd, 1, &v));
-            pulay[i + (size_t)ld * iscf] = v;
-            pulay[iscf + (size_t)ld * i] = v;
+        bool have_energy = false;
+        if (cdiis) {
+            // no density history: the change and this iteration's energy
+            // (input density, its own J/K) in one synchronisation
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(n2)), dim3(256), 0, s, dPout, dP, -1.0, n2, dW1);
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
+            RBCHK(rocblas_ddot(h, (int)n2, dW1, 1, dW1, 1, dEn + 4));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, dEn));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, dEn + 1));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_jc, 1, dEn + 2));
+            RBCHK(rocblas_ddot(h, (int)n2, dP, 1, ctx->d_kc, 1, dEn + 3));
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+            double en[5];
+            HIPCHK(hipMemcpyAsync(en, dEn, sizeof en, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            ek = en[0]; ev = en[1]; ej = en[2]; ex = en[3];
+            dchg = std::sqrt(en[4] / (double)n2);
+            have_energy = true;
+        } else {
+            int slot = (iscf - 1) % HMAX;
+            double* hp = dHP + n2 * slot;
+            double* hd = dHD + n2 * slot;
+            HIPCHK(hipMemcpyAsync(hp, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(hd, dPout, n2 * 8, hipMemcpyDeviceToDevice, s));
+            const double mone = -1.0;
+            RBCHK(rocblas_daxpy(h, (int)n2, &mone, dP, 1, hd, 1));
+            RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+            RBCHK(rocblas_ddot(h, (int)n2, hd, 1, hd, 1, &dchg));
+            pulay[iscf + (size_t)ld * iscf] = dchg;
+            for (int i = std::max(1, iscf - HMAX + 1); i < iscf; ++i) {
+                double v = 0.0;
+                RBCHK(rocblas_ddot(h, (int)n2, dHD + n2 * ((i - 1) % HMAX), 1, hd, 1, &v));
+                pulay[i + (size_t)ld * iscf] = v;
+                pulay[iscf + (size_t)ld * i] = v;
+            }
+            dchg = std::sqrt(dchg / (double)n2);
         }
-        dchg = std::sqrt(dchg / (double)n2);
         // DIIS every other iteration from IOPTN(16) on, else relaxation
         // (commutator DIIS: the next input density is P_out of the
         // extrapolated Fock matrix, taken after the energy below)
@@ -2549,12 +2693,13 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             if ((rc = jk_collect_timing(ctx))) break;
             jk_total += ctx->last_jk_ms * 1e-3;
             build_quartets += ctx->last_quartets;
+            have_energy = false;
         }
         // ENERGY_CALC with the updated density (src/pselfconsistentfield.f90:487-574);
         // with commutator DIIS the input density is idempotent (the previous
         // P_out) and the energy is that of the input density and its own J/K,
         // stationary to second order
-        {   // four products, one synchronisation
+        if (!have_energy) {   // four products, one synchronisation
             RBCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
             RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dT, 1, dEn));
             RBCHK(rocblas_ddot(h, (int)n2, dP, 1, dV, 1, dEn + 1));
@@ -2574,10 +2719,15 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         if (dchg < o->scfconv) { converged = 1; break; }
         if (cdiis) HIPCHK(hipMemcpyAsync(dP, dPout, n2 * 8, hipMemcpyDeviceToDevice, s));
     }
+    lap(4);
     if (!rc && last_purified) {   // orbital energies and coefficients of the last Fock matrix
         hipError_t e = hipMemcpyAsync(dFp, dFpLast, (size_t)nind * nind * 8, hipMemcpyDeviceToDevice, s);
         rc = e == hipSuccess ? diagonalise() : fail(POLYERI_E_HIP, hipGetErrorString(e));
     }
+    lap(5);
+    if (phases)
+        fprintf(stderr, "polyeri_scf phases (s): J/K+Fock %.4f  F' %.4f  DIIS %.4f  density %.4f  rest %.4f  final %.4f\n",
+                ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]);
     (void)hipEventDestroy(ef0);
     (void)hipEventDestroy(ef1);
     if (rc) return rc;

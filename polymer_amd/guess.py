@@ -36,8 +36,10 @@ from ._lib import check, lib
 
 ANG = 1.0 / 0.529177249          # bohr per angstrom
 CUTOFF = 6.0 * ANG                # pairs beyond this distance get a zero block
-NFEAT = 48
+NFEAT = 64                        # feature columns computed; a model uses the first nfeat of them
+NFEAT_GEOM = 48                   # geometric features only (columns 0..47)
 TYPES = ("OO", "OH", "HH")
+QO, QH = -0.834, 0.417            # point charges of the other molecules for the field features (TIP3P)
 
 
 # ---------------------------------------------------------------------------
@@ -98,11 +100,30 @@ def pair_list(atoms):
     return out
 
 
+def _fields(xyz, z):
+    """Per atom: the electric field (bohr units) of the point charges of the
+    other molecules, in the atom's molecular frame, and their potential."""
+    n = len(z)
+    q = np.where(z == 8, QO, QH)
+    mol = np.arange(n) // 3
+    d = xyz[:, None, :] - xyz[None, :, :]                   # r_a - r_b
+    r = np.linalg.norm(d, axis=2)
+    other = mol[:, None] != mol[None, :]
+    inv = np.where(other, 1.0 / np.where(other, r, 1.0), 0.0)
+    E = np.einsum("ab,abx->ax", q[None, :] * inv ** 3, d)
+    phi = (q[None, :] * inv).sum(axis=1)
+    fr = np.array(frames([(int(zz), *x) for zz, x in zip(z, xyz)]))
+    El = np.einsum("axk,ax->ak", fr[mol], E)               # frame components
+    return El, phi
+
+
 def pair_features(atoms, pairs):
     """Invariant features of every pair (one row each, NFEAT columns): same
     molecule, same atom, the position of j in i's frame, the distance and
     its radial-basis expansion, the relative orientation of the two frames,
-    and the neighbour densities of both atoms."""
+    the neighbour densities of both atoms (columns 0..46), then the field of
+    the other molecules' point charges at i and at j in their own frames and
+    the potential at both (columns 48..55)."""
     xyz = np.array([a[1:] for a in atoms])
     z = np.array([a[0] for a in atoms])
     fr = np.array(frames(atoms))                       # (molecules, 3, 3)
@@ -125,6 +146,11 @@ def pair_features(atoms, pairs):
     X[:, 14:23] = np.einsum("pxa,pxb->pab", Fi, Fj).reshape(-1, 9)
     X[:, 23:35] = env[i]
     X[:, 35:47] = env[j]
+    El, phi = _fields(xyz, z)
+    X[:, 48:51] = 10.0 * El[i]
+    X[:, 51:54] = 10.0 * El[j]
+    X[:, 54] = phi[i]
+    X[:, 55] = phi[j]
     return X
 
 
@@ -290,13 +316,15 @@ def _block_dims(basis_name):
 class GuessModel:
     HIDDEN = (128, 128)
 
-    def __init__(self, basis_name="6-31Gss", seed=0):
+    def __init__(self, basis_name="6-31Gss", seed=0, nfeat=NFEAT, hidden=None):
         self.basis_name = basis_name
+        self.nfeat = int(nfeat)
+        self.hidden = tuple(int(h) for h in (hidden or self.HIDDEN))
         self.nf = _block_dims(basis_name)
         self.out = {"OO": self.nf["O"] ** 2, "OH": self.nf["O"] * self.nf["H"], "HH": self.nf["H"] ** 2}
-        self.nets = {t: MLP([NFEAT, *self.HIDDEN, self.out[t]], seed=seed + k) for k, t in enumerate(TYPES)}
-        self.xmu = {t: np.zeros(NFEAT, np.float32) for t in TYPES}
-        self.xsd = {t: np.ones(NFEAT, np.float32) for t in TYPES}
+        self.nets = {t: MLP([self.nfeat, *self.hidden, self.out[t]], seed=seed + k) for k, t in enumerate(TYPES)}
+        self.xmu = {t: np.zeros(self.nfeat, np.float32) for t in TYPES}
+        self.xsd = {t: np.ones(self.nfeat, np.float32) for t in TYPES}
         self.ymu = {t: np.zeros(self.out[t], np.float32) for t in TYPES}
         self.ysd = {t: np.ones(self.out[t], np.float32) for t in TYPES}
 
@@ -304,7 +332,7 @@ class GuessModel:
     def samples(self, atoms, basis, P=None):
         """Features per type, and (with P) the frame-local target blocks."""
         pairs = pair_list(atoms)
-        X = pair_features(atoms, pairs)
+        X = pair_features(atoms, pairs)[:, :self.nfeat]
         fl = atom_functions(basis)
         fr = frames(atoms)
         Tm = [frame_transform(fr[a // 3], fl[a]) for a in range(len(atoms))]
@@ -326,7 +354,7 @@ class GuessModel:
         Per type, the blocks of all pairs are rotated back and scattered in
         batched array operations."""
         pairs = pair_list(atoms)
-        X = pair_features(atoms, pairs)
+        X = pair_features(atoms, pairs)[:, :self.nfeat]
         fl = atom_functions(basis)
         fr = frames(atoms)
         Tm = [frame_transform(fr[a // 3], fl[a]) for a in range(len(atoms))]
@@ -391,7 +419,7 @@ class GuessModel:
                         bs = min(batch, n - s0)
                         loss = ctypes.c_float()
                         check(lib().polyeri_mlp_train_step(
-                            net.h, ctypes.c_void_p(dx.value + 4 * s0 * NFEAT),
+                            net.h, ctypes.c_void_p(dx.value + 4 * s0 * self.nfeat),
                             ctypes.c_void_p(dy.value + 4 * s0 * Yn.shape[1]),
                             ctypes.c_void_p(dw.value + 4 * s0 * Yn.shape[1]), bs, ctypes.c_float(rate), 1,
                             ctypes.byref(loss)))
@@ -406,7 +434,7 @@ class GuessModel:
 
     # -- persistence ------------------------------------------------------
     def save(self, path):
-        d = {"basis": np.array(self.basis_name), "hidden": np.array(self.HIDDEN)}
+        d = {"basis": np.array(self.basis_name), "hidden": np.array(self.hidden), "nfeat": np.array(self.nfeat)}
         for t in TYPES:
             d[f"{t}_w"] = self.nets[t].params()
             d[f"{t}_xmu"], d[f"{t}_xsd"] = self.xmu[t], self.xsd[t]
@@ -416,7 +444,9 @@ class GuessModel:
     @classmethod
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
-        m = cls(str(z["basis"]))
+        # models saved before the field features use the 48 geometric columns
+        nfeat = int(z["nfeat"]) if "nfeat" in z.files else NFEAT_GEOM
+        m = cls(str(z["basis"]), nfeat=nfeat, hidden=tuple(int(h) for h in z["hidden"]))
         for t in TYPES:
             m.nets[t].set_params(z[f"{t}_w"])
             m.xmu[t], m.xsd[t] = z[f"{t}_xmu"], z[f"{t}_xsd"]

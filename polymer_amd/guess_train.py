@@ -2,7 +2,8 @@
 
     python -m polymer_amd.guess_train --n 2048 --mode train --out polymer_amd/guess_water_631gss.npz
     python -m polymer_amd.guess_train --n 2048 --mode eval --out polymer_amd/guess_water_631gss.npz
-    python -m polymer_amd.guess_train --basis cc-pVDZ --n 1024 --out polymer_amd/guess_water_ccpvdz.npz
+    python -m polymer_amd.guess_train --basis cc-pVDZ --n 2048 --nfeat 64 --hidden 128,128 --eval-limit 60 \
+        --core-subset 10 --out polymer_amd/guess_water_ccpvdz.npz
 
 Data: n seeded geometries of (H2O)4 .. (H2O)8 (seed s has 4 + s % 5
 waters), each built like the benchmark clusters (rigid placement, random
@@ -23,6 +24,7 @@ weights and a JSON report are written next to --out.
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
@@ -60,24 +62,34 @@ def rms_iterations(trace, conv=1e-8):
 
 
 def train(args, basis, ntrain):
-    model = G.GuessModel(basis)
-    data = {t: ([], []) for t in G.TYPES}
+    model = G.GuessModel(basis, nfeat=args.nfeat, hidden=tuple(int(h) for h in args.hidden.split(",")))
+    full = G.GuessModel(basis, nfeat=G.NFEAT, hidden=(16,))      # all feature columns, for the data
     t0 = time.perf_counter()
-    for s in range(ntrain):
-        atoms = geometry(s)
-        b = read_basis(basis, atoms)
-        ctx = Context(b, schwarz_tol=1e-12)
-        # the converged density does not depend on the solver: purification
-        # and commutator DIIS reach it in about half the iterations
-        r = ctx.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=100, solver=1, diis_mode=1)
-        if not r["converged"]:
-            continue
-        smp, _, _ = model.samples(atoms, b, r["P"])
-        for t in G.TYPES:
-            data[t][0].extend(smp[t][0])
-            data[t][1].extend(smp[t][1])
-        if s % 100 == 0:
-            log(f"data {s}/{ntrain} ({time.perf_counter() - t0:.0f} s)")
+    if args.data and os.path.exists(args.data):
+        z = np.load(args.data, allow_pickle=False)
+        data = {t: (z[f"{t}_X"], z[f"{t}_Y"]) for t in G.TYPES}
+        log(f"data from {args.data}")
+    else:
+        data = {t: ([], []) for t in G.TYPES}
+        for s in range(ntrain):
+            atoms = geometry(s)
+            b = read_basis(basis, atoms)
+            ctx = Context(b, schwarz_tol=1e-12)
+            # the converged density does not depend on the solver: purification
+            # and commutator DIIS reach it in about half the iterations
+            r = ctx.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=100, solver=1, diis_mode=1)
+            if not r["converged"]:
+                continue
+            smp, _, _ = full.samples(atoms, b, r["P"])
+            for t in G.TYPES:
+                data[t][0].extend(smp[t][0])
+                data[t][1].extend(smp[t][1])
+            if s % 100 == 0:
+                log(f"data {s}/{ntrain} ({time.perf_counter() - t0:.0f} s)")
+        if args.data:
+            np.savez(args.data, **{f"{t}_{k}": np.array(data[t][i], np.float32)
+                                   for t in G.TYPES for i, k in enumerate("XY")})
+    data = {t: (np.asarray(data[t][0], np.float32)[:, :model.nfeat], data[t][1]) for t in G.TYPES}
     arrays = {t: (np.array(data[t][0], np.float32), np.array(data[t][1], np.float32)) for t in G.TYPES}
     log("pairs per type: " + ", ".join(f"{t} {len(arrays[t][0])}" for t in G.TYPES))
     t1 = time.perf_counter()
@@ -86,7 +98,8 @@ def train(args, basis, ntrain):
     train_s = time.perf_counter() - t1
     model.save(args.out)
     log(f"trained in {train_s:.1f} s: " + ", ".join(f"{t} {v:.3e}" for t, v in losses.items()))
-    rep = {"basis": basis, "n_geometries": args.n, "n_train": ntrain,
+    rep = {"basis": basis, "n_geometries": args.n, "n_train": ntrain, "nfeat": model.nfeat,
+           "hidden": list(model.hidden),
            "train_pairs": {t: int(len(arrays[t][0])) for t in G.TYPES}, "train_loss": losses,
            "train_seconds": train_s, "data_seconds": t1 - t0, "epochs": args.epochs}
     with open(args.out.replace(".npz", "_train.json"), "w") as f:
@@ -174,6 +187,11 @@ def main():
     ap.add_argument("--eval-limit", type=int, default=None)
     ap.add_argument("--mode", choices=("all", "train", "eval"), default="all")
     ap.add_argument("--basis", default="6-31Gss", choices=("6-31Gss", "cc-pVDZ"))
+    ap.add_argument("--nfeat", type=int, default=G.NFEAT,
+                    help=f"feature columns ({G.NFEAT_GEOM}: geometry only; {G.NFEAT}: with the field features)")
+    ap.add_argument("--hidden", default="128,128", help="hidden layer widths")
+    ap.add_argument("--data", default=None, help="cache of the training pairs (all feature columns): read if it "
+                                                "exists, else written after the SCFs")
     args = ap.parse_args()
     basis = args.basis
     ntrain = int(round(0.8 * args.n))
