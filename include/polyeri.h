@@ -318,6 +318,10 @@ typedef struct polyeri_stats {
     int class_block[POLYERI_MAX_LISTS]; /* threads per block                 */
     int class_grid[POLYERI_MAX_LISTS];  /* block cap, 0 = one pass; kinds 3 and 4: segments (blocks) launched */
     int nclass;
+    /* the last molecular MP2 (polyeri_mp2_energy*): device time of the AO
+     * tensor, of the four quarter transforms, and their algorithmic FLOPs
+     * 2VN^4 + 2OVN^3 + 2OV^2N^2 + 2O^2V^2N */
+    double last_mp2_tensor_ms, last_mp2_transform_ms, last_mp2_flops;
 } polyeri_stats;
 int polyeri_get_stats(polyeri_ctx *ctx, polyeri_stats *st);
 

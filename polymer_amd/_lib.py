@@ -76,7 +76,9 @@ class Stats(ctypes.Structure):
                 ("class_quartets", ctypes.c_longlong * MAX_LISTS),
                 ("class_l", ctypes.c_int * MAX_LISTS), ("class_prec", ctypes.c_int * MAX_LISTS),
                 ("class_kind", ctypes.c_int * MAX_LISTS), ("class_block", ctypes.c_int * MAX_LISTS),
-                ("class_grid", ctypes.c_int * MAX_LISTS), ("nclass", ctypes.c_int)]
+                ("class_grid", ctypes.c_int * MAX_LISTS), ("nclass", ctypes.c_int),
+                ("last_mp2_tensor_ms", ctypes.c_double), ("last_mp2_transform_ms", ctypes.c_double),
+                ("last_mp2_flops", ctypes.c_double)]
 
 
 _LIB = None

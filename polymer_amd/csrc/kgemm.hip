@@ -9,12 +9,13 @@
 // i + j <= 2 on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation, the
 // result converted to FP64 (about FP32 accuracy at BF16 MFMA rate).
 //
-// One wavefront per 16 x 16 tile of C (four per block); lane l holds row
-// l % 16 of the A slice and column l % 16 of the B slice (k = l / 16 for the
-// FP64 instruction, k = 4 (l / 16) .. +3 for BF16) and receives column l % 16
-// of the result: rows q, q+4, q+8, q+12 (FP64) or 4q .. 4q+3 (BF16),
-// q = l / 16.  Rows, columns and k out of range are zero, so any size is
-// accepted.
+// BF16x3: one wavefront per 16 x 16 tile of C (four per block); lane l
+// holds row l % 16 of the A slice and column l % 16 of the B slice (k =
+// 4 (l / 16) .. +3) and receives rows 4q .. 4q+3 of column l % 16 of the
+// result, q = l / 16.  The v_mfma_f64_16x16x4f64 lane layout of the FP64
+// kernel: A row l % 16, B column l % 16, k = l / 16; result rows q, q+4,
+// q+8, q+12 of column l % 16.  Rows, columns and k out of range are zero,
+// so any size is accepted.
 #include <hip/hip_runtime.h>
 
 #include "eri_kernels.h"
@@ -35,31 +36,74 @@ struct GemmArgs {
     long scm, scn, sC;
 };
 
-__global__ void __launch_bounds__(256) k_gemm_f64(GemmArgs g) {
+// LDS-tiled FP64 product: a block of 256 threads computes a 64 x 128 tile
+// of C; the four wavefronts take 32 x 64 quarters, each 2 x 4 tiles of
+// 16 x 16 on v_mfma_f64_16x16x4f64, so every A fragment a lane reads from LDS
+// feeds four products and every B fragment two.  K advances 16 at a time,
+// the A (64 x 16) and B (16 x 128) slices staged in LDS by all threads with
+// the unit-stride index fastest (AK1 / BK1: sak / sbk == 1), out-of-range
+// rows, columns and k read as zero.  Block order: the blocks of one column
+// block (all row blocks) are 8 launch slots apart, i.e. on one XCD, so the
+// B slice they share is fetched into one L2.
+constexpr int TM = 64, TN = 128, TK = 16;
+
+template <bool AK1, bool BK1>
+__global__ void __launch_bounds__(256) k_gemm_f64_lds(GemmArgs g, int mblocks, int nblocks) {
+    __shared__ double As[TK][TM + 1];
+    __shared__ double Bs[TK][TN + 1];
+    const int bid = blockIdx.x, xcd = bid & 7, slot = bid >> 3;
+    const int mb = slot % mblocks, nb = (slot / mblocks) * 8 + xcd;
+    if (nb >= nblocks) return;                       // block-uniform, before any barrier
+    const long bat = blockIdx.y;
+    const double* A = g.A + bat * g.sA;
+    const double* B = g.B + bat * g.sB;
+    const int m0 = mb * TM, n0 = nb * TN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tiles_n = (g.N + 15) >> 4;
-    const long tile = (long)blockIdx.x * 4 + wave;
-    const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
-    if ((long)tm * 16 >= g.M) return;
-    const long b = blockIdx.y;
-    const double* A = g.A + b * g.sA;
-    const double* B = g.B + b * g.sB;
+    const int wr = wave & 1, wc = wave >> 1;
     const int r = lane & 15, q = lane >> 4;
-    const int m = tm * 16 + r, n = tn * 16 + r;
-    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += 4) {
-        const int k = k0 + q;
-        const double a = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.0;
-        const double bb = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
-    }
-    // the FP64 instruction returns rows q, q+4, q+8, q+12 of column l % 16
-    double* C = g.C + b * g.sC;
+    doublex4 acc[2][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int row = tm * 16 + q + 4 * j, col = tn * 16 + r;
-        if (row < g.M && col < g.N) C[row * g.scm + col * g.scn] = acc[j];
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < g.K; k0 += TK) {
+#pragma unroll
+        for (int e = threadIdx.x; e < TM * TK; e += 256) {
+            const int kk = AK1 ? e % TK : e / TM, mm = AK1 ? e / TK : e % TM;
+            const int m = m0 + mm, k = k0 + kk;
+            As[kk][mm] = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.0;
+        }
+#pragma unroll
+        for (int e = threadIdx.x; e < TN * TK; e += 256) {
+            const int kk = BK1 ? e % TK : e / TN, nn = BK1 ? e / TK : e % TN;
+            const int n = n0 + nn, k = k0 + kk;
+            Bs[kk][nn] = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 4) {
+            double a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[ks + q][wr * 32 + i * 16 + r];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[ks + q][wc * 64 + j * 16 + r];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
     }
+    double* C = g.C + bat * g.sC;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {   // rows q, q + 4, q + 8, q + 12 of the tile, column r
+                const int row = m0 + wr * 32 + i * 16 + q + 4 * x, col = n0 + wc * 64 + j * 16 + r;
+                if (row < g.M && col < g.N) C[row * g.scm + col * g.scn] = acc[i][j][x];
+            }
 }
 
 __device__ __forceinline__ short bf16_bits(float x) {
@@ -326,10 +370,17 @@ int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam
         const int nb = batch - b0 < 65535 ? batch - b0 : 65535;
         GemmArgs g{M, N, K, A + b0 * sA, sam, sak, sA, B + b0 * sB, sbk, sbn, sB, C + b0 * sC, scm, scn, sC};
         const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
-        if (mode == GEMM_F64)
-            hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, stream, g);
-        else
+        if (mode == GEMM_F64) {
+            const int mbk = (M + TM - 1) / TM, nbk = (N + TN - 1) / TN, nb8 = (nbk + 7) / 8 * 8;
+            if ((long)mbk * nb8 > 0x7fffffffL) return -1;
+            const dim3 tg((unsigned)(mbk * nb8), (unsigned)nb);
+            if (sak == 1 && sbk == 1) hipLaunchKernelGGL((k_gemm_f64_lds<true, true>), tg, dim3(256), 0, stream, g, mbk, nbk);
+            else if (sak == 1) hipLaunchKernelGGL((k_gemm_f64_lds<true, false>), tg, dim3(256), 0, stream, g, mbk, nbk);
+            else if (sbk == 1) hipLaunchKernelGGL((k_gemm_f64_lds<false, true>), tg, dim3(256), 0, stream, g, mbk, nbk);
+            else hipLaunchKernelGGL((k_gemm_f64_lds<false, false>), tg, dim3(256), 0, stream, g, mbk, nbk);
+        } else {
             hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, stream, g);
+        }
         if (hipGetLastError() != hipSuccess) return -2;
     }
     return 0;

@@ -501,6 +501,9 @@ struct polyeri_ctx {
     int* d_inv = nullptr;
     bool use_stored = false;
     double* d_noalloc = nullptr;   // this device's address of `scratch`
+    // the last molecular MP2: device time of the AO tensor and of the four
+    // quarter transforms (incl. the MO coefficient set-up), their FLOPs
+    double last_mp2_tensor_ms = 0, last_mp2_transform_ms = 0, last_mp2_flops = 0;
     // polyeri_scf: work buffers, and the one-electron matrices, C2S and the
     // canonical orthogonaliser of this geometry (kept per (spherical, lindep))
     ScfArena scf_arena;
@@ -2031,6 +2034,9 @@ int polyeri_get_stats(polyeri_ctx* ctx, polyeri_stats* st) {
     st->last_jk_ms = ctx->last_jk_ms;
     st->last_quartets = ctx->last_quartets;
     st->nclass = (int)std::min<size_t>(POLYERI_MAX_LISTS, ctx->work.size());
+    st->last_mp2_tensor_ms = ctx->last_mp2_tensor_ms;
+    st->last_mp2_transform_ms = ctx->last_mp2_transform_ms;
+    st->last_mp2_flops = ctx->last_mp2_flops;
     for (int i = 0; i < st->nclass; ++i) {
         const ClassPairWork& w = ctx->work[i];
         st->last_class_ms[i] = ctx->last_class_ms[i];
@@ -2780,7 +2786,12 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
         (rc = buf.get(&dX1, x1)) || (rc = buf.get(&dX2, x2)) || (rc = buf.get(&dX3, x3)) || (rc = buf.get(&dT4, x4)) ||
         (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)) || (rc = buf.get(&dQ, (size_t)nc)))
         return rc;
+    HIPCHK(hipEventRecord(ctx->ev0, s));
     if ((rc = tensor_device(ctx, dTen))) return rc;
+    hipEvent_t et = nullptr;
+    HIPCHK(hipEventCreate(&et));
+    struct EvGuard { hipEvent_t e; ~EvGuard() { if (e) (void)hipEventDestroy(e); } } eg{et};
+    HIPCHK(hipEventRecord(et, s));
     HIPCHK(hipMemcpyAsync(dCO, co, (size_t)nc * nc * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dEps, eps, (size_t)nc * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_mo_int, dim3(nblk((size_t)n * nc)), dim3(256), 0, s, dCO, nc, ctx->d_ref, ctx->d_scale, n, nc, dC);
@@ -2816,6 +2827,12 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
                                         NV * NO * NA, 0, 1, s);
         if (r) return fail(POLYERI_E_HIP, "MP2 transform launch failed");
     }
+    HIPCHK(hipEventRecord(ctx->ev1, s));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    ctx->last_mp2_tensor_ms = event_ms(ctx->ev0, et);
+    ctx->last_mp2_transform_ms = event_ms(et, ctx->ev1);
+    ctx->last_mp2_flops = 2.0 * nv * (double)n * n * n * n + 2.0 * nv * (double)nop * n * n * n +
+                          2.0 * nv * (double)nop * nvp * n * n + 2.0 * nv * (double)nop * nvp * nop * n;
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
     hipLaunchKernelGGL(k_mp2_energy, dim3(nblk((size_t)no * nv * no * nv)), dim3(256), 0, s, dT4, dEps, no, nv, nop,
                        nvp, aoff, icore, iocc, dAcc);

@@ -1,7 +1,7 @@
 """Train and evaluate the learned SCF guess for water clusters.
 
-    python -m polymer_amd.guess_train --n 2048 --mode train --out polymer_amd/guess_water_631gss.npz
-    python -m polymer_amd.guess_train --n 2048 --mode eval --out polymer_amd/guess_water_631gss.npz
+    python -m polymer_amd.guess_train --n 2048 --nfeat 48 --mode train --out polymer_amd/guess_water_631gss.npz
+    python -m polymer_amd.guess_train --n 2048 --nfeat 48 --mode eval --out polymer_amd/guess_water_631gss.npz
     python -m polymer_amd.guess_train --basis cc-pVDZ --n 2048 --nfeat 64 --hidden 128,128 --eval-limit 60 \
         --core-subset 10 --out polymer_amd/guess_water_ccpvdz.npz
 

@@ -45,13 +45,13 @@ def _run(ctx, mode, A, B, ta, tb):
 
 
 @pytest.mark.parametrize("mode,tol", [(1, 1e-13), (2, 1e-6)])
-@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True)])
-def test_gemm_f64_kernels(ctx, mode, tol, ta, tb):
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("M,K,N", [(35, 19, 21), (150, 50, 300)])
+def test_gemm_f64_kernels(ctx, mode, tol, ta, tb, M, K, N):
     rng = np.random.default_rng(7)
-    M, K, N = 35, 19, 21
     A = rng.standard_normal((K, M) if ta else (M, K))
     B = rng.standard_normal((N, K) if tb else (K, N))
-    print(f"case: gemm mode {mode} transA {ta} transB {tb}", flush=True)
+    print(f"case: gemm mode {mode} transA {ta} transB {tb} M {M} K {K} N {N}", flush=True)
     C = _run(ctx, mode, A, B, ta, tb)
     ref = (A.T if ta else A) @ (B.T if tb else B)
     where = [int(np.argmin(np.max(np.abs(ref - C[r]), axis=1))) for r in range(M)]
