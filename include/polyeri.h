@@ -264,7 +264,8 @@ int polyeri_mp2_energy_periodic(polyeri_ctx *ctx, int kvc, const double *co, con
 
 /* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
  * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the
- * device (no .ao files), four quarter transforms, singlet and triplet E(2).
+ * device (no .ao files), four quarter transforms (occupied indices first) on
+ * this library's FP64 MFMA kernel, singlet and triplet E(2).
  * co: ncgs x ncgs Cartesian MO coefficients (columns = orbitals); orbitals
  * icore..iocc-1 active occupied, iocc..nall-1-ivirtcore virtual. */
 int polyeri_mp2_energy(polyeri_ctx *ctx, const double *co, const double *eps, int icore,

@@ -303,7 +303,7 @@ class Context:
             return s.value, t.value, sec.value
         return s.value, t.value
 
-    def mp2_quasiparticle(self, co, eps, icore, iocc, nall, nqp, ivirtcore=0, transform="rocblas"):
+    def mp2_quasiparticle(self, co, eps, icore, iocc, nall, nqp, ivirtcore=0, transform="f64"):
         """E(2) singlet, triplet and the MP2 quasi-particle band energies
         QEPSILON (IOPTN(44) = nqp): returns (e2s, e2t, qeps) with qeps of
         length ncgs (bands outside the corrected range copy eps)."""
@@ -317,8 +317,9 @@ class Context:
         return s.value, t.value, q
 
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
-        """E(2) singlet and triplet parts.  transform: None (rocBLAS dgemm),
-        "f64" (own FP64 MFMA kernel) or "bf16x3" (own BF16-split MFMA kernel)."""
+        """E(2) singlet and triplet parts.  transform: None (the library's
+        default, its FP64 MFMA kernel), "f64" (the same), "rocblas" (rocBLAS
+        dgemm) or "bf16x3" (own BF16-split MFMA kernel)."""
         co = np.asfortranarray(co, dtype=np.float64)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         s, t = ctypes.c_double(), ctypes.c_double()

@@ -36,74 +36,209 @@ struct GemmArgs {
     long scm, scn, sC;
 };
 
-// LDS-tiled FP64 product: a block of 256 threads computes a 64 x 128 tile
-// of C; the four wavefronts take 32 x 64 quarters, each 2 x 4 tiles of
-// 16 x 16 on v_mfma_f64_16x16x4f64, so every A fragment a lane reads from LDS
-// feeds four products and every B fragment two.  K advances 16 at a time,
-// the A (64 x 16) and B (16 x 128) slices staged in LDS by all threads with
-// the unit-stride index fastest (AK1 / BK1: sak / sbk == 1), out-of-range
-// rows, columns and k read as zero.  Block order: the blocks of one column
-// block (all row blocks) are 8 launch slots apart, i.e. on one XCD, so the
-// B slice they share is fetched into one L2.
-constexpr int TM = 64, TN = 128, TK = 16;
+// LDS-tiled FP64 product.  A block of 256 threads (four wavefronts in a
+// WM x WN grid) computes a BM x BN tile of C, each wavefront TMW x TNW tiles
+// of 16 x 16 on v_mfma_f64_16x16x4f64, so an A fragment read from LDS feeds
+// TNW products and a B fragment TMW.  The tile shape follows the product: a
+// short M (up to 64 rows, the first quarter transform's occupied orbitals)
+// takes one row of wavefronts over 128 columns; a narrow N (up to 64
+// columns) one column of wavefronts over 128 rows; otherwise 64 x 128, or
+// 32 x 160 when N fits 160 (one column block, so A is streamed once).
+//
+// K advances 16 at a time through two LDS buffers: the slices one and two
+// steps ahead are loaded from global memory into registers while the current
+// one feeds the MFMA units, the next one then stored into the other buffer,
+// one barrier per step.  Staging runs along the unit-stride index of each
+// operand (see the staging map in the kernel).  The loads are buffer loads
+// relative to a wavefront-uniform base with 32-bit byte offsets fixed per
+// thread; a row, column or k out of range gets an offset past the buffer's
+// record count, which the hardware returns as zero without a memory access,
+// so the loop has no branches.  LDS rows are padded to 16 mod 32 doubles so
+// that the four k rows of a fragment read fall on different bank halves.  At
+// most 80 KB of LDS and 256 registers per wavefront, so two blocks share a
+// CU: one block's barrier and load waits are covered by the other's products.
+constexpr unsigned kOob = 0x80000000u;          // byte offset past kRecords: reads as zero
+constexpr int kRecords = 0x7fffffff;
+constexpr int kRsrcWord3 = 0x00020000;          // raw buffer, 32-bit data format (gfx9 family)
 
-template <bool AK1, bool BK1>
-__global__ void __launch_bounds__(256) k_gemm_f64_lds(GemmArgs g, int mblocks, int nblocks) {
-    __shared__ double As[TK][TM + 1];
-    __shared__ double Bs[TK][TN + 1];
-    const int bid = blockIdx.x, xcd = bid & 7, slot = bid >> 3;
-    const int mb = slot % mblocks, nb = (slot / mblocks) * 8 + xcd;
-    if (nb >= nblocks) return;                       // block-uniform, before any barrier
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+template <int WM, int WN, int TMW, int TNW, bool AKC, bool BKC>
+__global__ void __launch_bounds__(64 * WM * WN, 2) k_gemm_f64_tiled(GemmArgs g, int mblocks) {
+    constexpr int NT = 64 * WM * WN, BM = WM * TMW * 16, BN = WN * TNW * 16, TK = 16;
+    constexpr int LA = BM + (48 - BM % 32) % 32, LB = BN + (48 - BN % 32) % 32;
+    constexpr int SA = BM * TK, SB = BN * TK;                     // slice elements
+    constexpr int NA = (SA + NT - 1) / NT, NB = (SB + NT - 1) / NT;  // per thread (the last one partial)
+    static_assert(2 * TK * (LA + LB) * 8 <= 80 * 1024, "two blocks per CU");
+    __shared__ double As[2][TK][LA];
+    __shared__ double Bs[2][TK][LB];
+    const int mb = (int)(blockIdx.x % (unsigned)mblocks), nb = (int)(blockIdx.x / (unsigned)mblocks);
     const long bat = blockIdx.y;
-    const double* A = g.A + bat * g.sA;
-    const double* B = g.B + bat * g.sB;
-    const int m0 = mb * TM, n0 = nb * TN;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = wave & 1, wc = wave >> 1;
+    const int m0 = mb * BM, n0 = nb * BN;
+    const double* A = g.A + bat * g.sA + (long)m0 * g.sam;
+    const double* B = g.B + bat * g.sB + (long)n0 * g.sbn;
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    static_assert(NT == 256 && TK == 16, "four wavefronts, 16-deep slices");
+    // Staging map of the A slice (TK x BM): element p of this thread has k
+    // index a_kk(p) and row a_mm(p).  Three forms:
+    //  AMODE 1 (sak == 1): k = t & 15, rows t / 16 + 16 p: 16 lanes read one
+    //          row's 16 consecutive k;
+    //  AMODE 2 (rows contiguous, BM a multiple of 64): wavefront w stages k
+    //          rows 4w .. 4w + 3, each in BM / 64 runs of 64 consecutive rows;
+    //          the buffer base sits at row 4w, so the per-lane offsets span
+    //          three k strides at most;
+    //  AMODE 3 (otherwise): element t + 256 p in k-major order.
+    // Likewise for B (column b_nn).
+    constexpr int AMODE = AKC ? 1 : (BM % 64 == 0 ? 2 : 3);
+    constexpr int BMODE = BKC ? 1 : (BN % 64 == 0 ? 2 : 3);
+    auto a_kk = [&](int p) -> int {
+        if constexpr (AMODE == 1) return t & 15;
+        else if constexpr (AMODE == 2) return 4 * wave + p / (BM / 64);
+        else return (t + NT * p) / BM;
+    };
+    auto a_mm = [&](int p) -> int {
+        if constexpr (AMODE == 1) return (t >> 4) + p * (NT / 16);
+        else if constexpr (AMODE == 2) return (p % (BM / 64)) * 64 + lane;
+        else return (t + NT * p) % BM;
+    };
+    auto b_kk = [&](int p) -> int {
+        if constexpr (BMODE == 1) return t & 15;
+        else if constexpr (BMODE == 2) return 4 * wave + p / (BN / 64);
+        else return (t + NT * p) / BN;
+    };
+    auto b_nn = [&](int p) -> int {
+        if constexpr (BMODE == 1) return (t >> 4) + p * (NT / 16);
+        else if constexpr (BMODE == 2) return (p % (BN / 64)) * 64 + lane;
+        else return (t + NT * p) % BN;
+    };
+    const int ka0 = AMODE == 2 ? 4 * wave : 0, kb0 = BMODE == 2 ? 4 * wave : 0;   // k row of the buffer base
+    unsigned aoff[NA], boff[NB];
+#pragma unroll
+    for (int p = 0; p < NA; ++p)
+        aoff[p] = t + NT * p < SA && m0 + a_mm(p) < g.M
+                      ? (unsigned)(8 * ((long)a_mm(p) * g.sam + (long)(a_kk(p) - ka0) * g.sak)) : kOob;
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+        boff[p] = t + NT * p < SB && n0 + b_nn(p) < g.N
+                      ? (unsigned)(8 * ((long)b_nn(p) * g.sbn + (long)(b_kk(p) - kb0) * g.sbk)) : kOob;
+    // two register sets: the slice two steps ahead is in flight while the
+    // next one is stored
+    double ra0[NA], rb0[NB], ra1[NA], rb1[NB];
+    auto load = [&](int k0, double (&ra)[NA], double (&rb)[NB]) {
+        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(A + (long)(k0 + ka0) * g.sak), (short)0, kRecords, kRsrcWord3);
+        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(B + (long)(k0 + kb0) * g.sbk), (short)0, kRecords, kRsrcWord3);
+#pragma unroll
+        for (int p = 0; p < NA; ++p) ra[p] = buf_load(rsa, k0 + a_kk(p) < g.K ? aoff[p] : kOob);
+#pragma unroll
+        for (int p = 0; p < NB; ++p) rb[p] = buf_load(rsb, k0 + b_kk(p) < g.K ? boff[p] : kOob);
+    };
+    auto store = [&](int buf, const double (&ra)[NA], const double (&rb)[NB]) {
+#pragma unroll
+        for (int p = 0; p < NA; ++p)
+            if (SA % NT == 0 || t + NT * p < SA) As[buf][a_kk(p)][a_mm(p)] = ra[p];
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
+            if (SB % NT == 0 || t + NT * p < SB) Bs[buf][b_kk(p)][b_nn(p)] = rb[p];
+    };
     const int r = lane & 15, q = lane >> 4;
-    doublex4 acc[2][4];
+    const int wr = (wave % WM) * TMW * 16, wc = (wave / WM) * TNW * 16;
+    doublex4 acc[TMW][TNW];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TMW; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += TK) {
-#pragma unroll
-        for (int e = threadIdx.x; e < TM * TK; e += 256) {
-            const int kk = AK1 ? e % TK : e / TM, mm = AK1 ? e / TK : e % TM;
-            const int m = m0 + mm, k = k0 + kk;
-            As[kk][mm] = (m < g.M && k < g.K) ? A[m * g.sam + k * g.sak] : 0.0;
-        }
-#pragma unroll
-        for (int e = threadIdx.x; e < TN * TK; e += 256) {
-            const int kk = BK1 ? e % TK : e / TN, nn = BK1 ? e / TK : e % TN;
-            const int n = n0 + nn, k = k0 + kk;
-            Bs[kk][nn] = (n < g.N && k < g.K) ? B[k * g.sbk + n * g.sbn] : 0.0;
-        }
-        __syncthreads();
+        for (int j = 0; j < TNW; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    auto compute = [&](int buf) {
 #pragma unroll
         for (int ks = 0; ks < TK; ks += 4) {
-            double a[2], b[4];
+            double a[TMW], b[TNW];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[ks + q][wr * 32 + i * 16 + r];
+            for (int i = 0; i < TMW; ++i) a[i] = As[buf][ks + q][wr + i * 16 + r];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[ks + q][wc * 64 + j * 16 + r];
+            for (int j = 0; j < TNW; ++j) b[j] = Bs[buf][ks + q][wc + j * 16 + r];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < TMW; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < TNW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
+    };
+    // Slices at k >= K load as zero without a memory access, so the steps
+    // past the end need no test.  Step s computes LDS buffer s & 1, stores
+    // the slice of step s + 1 (register set (s + 1) & 1) into the other
+    // buffer, which every wavefront finished reading before the previous
+    // barrier, and loads the slice of step s + 2 into set s & 1.
+    const int nk = (g.K + TK - 1) / TK;
+    load(0, ra0, rb0);
+    load(TK, ra1, rb1);
+    store(0, ra0, rb0);
+    __syncthreads();
+    for (int it = 0; it < nk; it += 2) {
+        load((it + 2) * TK, ra0, rb0);
+        compute(0);
+        store(1, ra1, rb1);
+        __syncthreads();
+        if (it + 1 >= nk) break;                       // block-uniform
+        load((it + 3) * TK, ra1, rb1);
+        compute(1);
+        store(0, ra0, rb0);
         __syncthreads();
     }
     double* C = g.C + bat * g.sC;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TMW; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < TNW; ++j)
 #pragma unroll
             for (int x = 0; x < 4; ++x) {   // rows q, q + 4, q + 8, q + 12 of the tile, column r
-                const int row = m0 + wr * 32 + i * 16 + q + 4 * x, col = n0 + wc * 64 + j * 16 + r;
-                if (row < g.M && col < g.N) C[row * g.scm + col * g.scn] = acc[i][j][x];
+                const int row = m0 + wr + i * 16 + q + 4 * x, col = n0 + wc + j * 16 + r;
+                if (row < g.M && col < g.N) C[(long)row * g.scm + (long)col * g.scn] = acc[i][j][x];
             }
+}
+
+template <int WM, int WN, int TMW, int TNW>
+static int launch_f64_tiled(const GemmArgs& g, int nbat, hipStream_t stream) {
+    const bool akc = g.sak == 1, bkc = g.sbk == 1;
+    constexpr int BM = WM * TMW * 16, BN = WN * TNW * 16;
+    const long mbk = (g.M + BM - 1) / BM, nbk = (g.N + BN - 1) / BN;
+    if (mbk * nbk > 0x7fffffffL) return -1;
+    // per-thread byte offsets within one K slice must stay below the record count
+    const long ka = akc ? 15 : (BM % 64 == 0 ? 3 : 15), kb = bkc ? 15 : (BN % 64 == 0 ? 3 : 15);   // k rows a lane spans
+    const long amax = 8 * ((long)(BM - 1) * g.sam + ka * g.sak), bmax = 8 * ((long)(BN - 1) * g.sbn + kb * g.sbk);
+    if (g.sam < 0 || g.sak < 0 || g.sbk < 0 || g.sbn < 0 || amax >= (long)kRecords - 8 || bmax >= (long)kRecords - 8)
+        return -1;
+    const dim3 grid((unsigned)(mbk * nbk), (unsigned)nbat), block(64 * WM * WN);
+    if (akc && bkc) hipLaunchKernelGGL((k_gemm_f64_tiled<WM, WN, TMW, TNW, true, true>), grid, block, 0, stream, g, (int)mbk);
+    else if (akc) hipLaunchKernelGGL((k_gemm_f64_tiled<WM, WN, TMW, TNW, true, false>), grid, block, 0, stream, g, (int)mbk);
+    else if (bkc) hipLaunchKernelGGL((k_gemm_f64_tiled<WM, WN, TMW, TNW, false, true>), grid, block, 0, stream, g, (int)mbk);
+    else hipLaunchKernelGGL((k_gemm_f64_tiled<WM, WN, TMW, TNW, false, false>), grid, block, 0, stream, g, (int)mbk);
+    return 0;
+}
+
+// Tile shape for an M x N product (see k_gemm_f64_tiled).
+static int launch_f64(const GemmArgs& g, int nbat, hipStream_t stream) {
+    const int tm = (g.M + 15) / 16, tn = (g.N + 15) / 16;
+    if (tm <= 4) {
+        switch (tm) {
+            case 1: return launch_f64_tiled<1, 4, 1, 2>(g, nbat, stream);
+            case 2: return launch_f64_tiled<1, 4, 2, 2>(g, nbat, stream);
+            case 3: return launch_f64_tiled<1, 4, 3, 2>(g, nbat, stream);
+            default: return launch_f64_tiled<1, 4, 4, 2>(g, nbat, stream);
+        }
+    }
+    if (tn <= 4) {
+        switch (tn) {
+            case 1: return launch_f64_tiled<4, 1, 2, 1>(g, nbat, stream);
+            case 2: return launch_f64_tiled<4, 1, 2, 2>(g, nbat, stream);
+            case 3: return launch_f64_tiled<4, 1, 2, 3>(g, nbat, stream);
+            default: return launch_f64_tiled<4, 1, 2, 4>(g, nbat, stream);
+        }
+    }
+    if (tn > 8 && tn <= 10) return launch_f64_tiled<2, 2, 1, 5>(g, nbat, stream);
+    return launch_f64_tiled<2, 2, 2, 4>(g, nbat, stream);
 }
 
 __device__ __forceinline__ short bf16_bits(float x) {
@@ -371,13 +506,7 @@ int polyeri_gemm_launch(int mode, int M, int N, int K, const double* A, long sam
         GemmArgs g{M, N, K, A + b0 * sA, sam, sak, sA, B + b0 * sB, sbk, sbn, sB, C + b0 * sC, scm, scn, sC};
         const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
         if (mode == GEMM_F64) {
-            const int mbk = (M + TM - 1) / TM, nbk = (N + TN - 1) / TN, nb8 = (nbk + 7) / 8 * 8;
-            if ((long)mbk * nb8 > 0x7fffffffL) return -1;
-            const dim3 tg((unsigned)(mbk * nb8), (unsigned)nb);
-            if (sak == 1 && sbk == 1) hipLaunchKernelGGL((k_gemm_f64_lds<true, true>), tg, dim3(256), 0, stream, g, mbk, nbk);
-            else if (sak == 1) hipLaunchKernelGGL((k_gemm_f64_lds<true, false>), tg, dim3(256), 0, stream, g, mbk, nbk);
-            else if (sbk == 1) hipLaunchKernelGGL((k_gemm_f64_lds<false, true>), tg, dim3(256), 0, stream, g, mbk, nbk);
-            else hipLaunchKernelGGL((k_gemm_f64_lds<false, false>), tg, dim3(256), 0, stream, g, mbk, nbk);
+            if (launch_f64(g, nb, stream)) return -1;
         } else {
             hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, stream, g);
         }

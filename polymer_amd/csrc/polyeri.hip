@@ -144,29 +144,40 @@ __global__ void k_copy(const double2* __restrict__ src, double2* __restrict__ ds
         dst[i] = src[i];
 }
 
-// E(2) from T(b, j, a, i) = (ia|jb): src/pmp2energy.f90:395-444 at one k-point
 // E(2) singlet and triplet (src/pmp2energy.f90:395-433 at one k-point) from
-// T[i'][a'][j'][b] = (ia|jb), b fastest: b over the nv virtuals, j' and i'
-// over the "occupied" range icore.. (nop wide), a' over the "virtual" range
-// starting aoff below iocc (nvp wide); the energy uses i, j < no and the
+// T[b][a'][i'][j'] = (ia|jb), j' fastest: j' and i' over the "occupied"
+// range icore.. (nop wide), a' over the "virtual" range starting aoff below
+// iocc (nvp wide), b over the nv virtuals; the energy uses i, j < no and the
 // true virtuals a' = aoff + a.
-__global__ void k_mp2_energy(const double* __restrict__ T, const double* __restrict__ eps, int no, int nv,
-                             int nop, int nvp, int aoff, int icore, int iocc, double* __restrict__ acc) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t tot = (size_t)no * nv * no * nv;
+// One block per virtual a and a stride of virtuals b: the no x no block of
+// (ia|jb) of each pair (a, b) is read once, coalesced, into LDS (TILE; else
+// straight from global memory) so that (ja|ib) is its transpose.
+template <bool TILE>
+__global__ void __launch_bounds__(256) k_mp2_energy(const double* __restrict__ T, const double* __restrict__ eps,
+                                                    int no, int nv, int nop, int nvp, int aoff, int icore, int iocc,
+                                                    double* __restrict__ acc) {
+    extern __shared__ double tile[];                     // no x (no + 1)
+    const int a = blockIdx.x, ld = no + 1, nn = no * no;
     double es = 0.0, et = 0.0;
-    if (t < tot) {
-        const int b = (int)(t % nv);
-        size_t r = t / nv;
-        const int j = (int)(r % no); r /= no;
-        const int a = (int)(r % nv);
-        const int i = (int)(r / nv);
-        const size_t ap = (size_t)aoff + a;
-        const double x = T[b + (size_t)nv * (j + (size_t)nop * (ap + (size_t)nvp * i))];
-        const double xp = T[b + (size_t)nv * (i + (size_t)nop * (ap + (size_t)nvp * j))];   // (ja|ib)
-        const double den = eps[icore + i] + eps[icore + j] - eps[iocc + a] - eps[iocc + b];
-        es = (0.5 * x * x + 0.5 * x * xp) / den;
-        et = (1.5 * x * x - 1.5 * x * xp) / den;
+    for (int b = blockIdx.y; b < nv; b += gridDim.y) {
+        const double* __restrict__ Tab = T + (size_t)nop * nop * ((size_t)aoff + a + (size_t)nvp * b);
+        const double eab = eps[iocc + a] + eps[iocc + b];
+        if (TILE) {
+            __syncthreads();                                 // the previous pair's reads are done
+            for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+                const int i = e / no, j = e - i * no;
+                tile[i * ld + j] = Tab[j + (size_t)nop * i];
+            }
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const int i = e / no, j = e - i * no;
+            const double x = TILE ? tile[i * ld + j] : Tab[j + (size_t)nop * i];     // (ia|jb)
+            const double xp = TILE ? tile[j * ld + i] : Tab[i + (size_t)nop * j];    // (ja|ib)
+            const double den = eps[icore + i] + eps[icore + j] - eab;
+            es += (0.5 * x * x + 0.5 * x * xp) / den;
+            et += (1.5 * x * x - 1.5 * x * xp) / den;
+        }
     }
     __shared__ double ss[256], st[256];
     ss[threadIdx.x] = es;
@@ -196,20 +207,24 @@ __global__ void k_mp2_qp(const double* __restrict__ T, const double* __restrict_
     const int gi = g - icore, ga = g - vlo;
     const size_t n1 = (size_t)nv * no * nv, n2 = (size_t)nv * no * no;
     double s = 0.0;
+    const size_t nop2 = (size_t)nop * nop;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2; t += (size_t)gridDim.x * blockDim.x) {
-        const int b = (int)(t % nv);
         if (t < n1) {
-            const size_t r = t / nv;
-            const int j = (int)(r % no), a = (int)(r / no);
-            const size_t ap = (size_t)aoff + a;
-            const double x = T[b + (size_t)nv * (j + (size_t)nop * (ap + (size_t)nvp * gi))];
-            const double xs = T[b + (size_t)nv * (gi + (size_t)nop * (ap + (size_t)nvp * j))];
+            const int j = (int)(t % no);
+            const size_t r = t / no;
+            const int a = (int)(r % nv), b = (int)(r / nv);
+            const size_t base = nop2 * ((size_t)aoff + a + (size_t)nvp * b);
+            const double x = T[base + j + (size_t)nop * gi];
+            const double xs = T[base + gi + (size_t)nop * j];
             s += (2.0 * x * x - x * xs) / (eps[g] + eps[icore + j] - eps[iocc + a] - eps[iocc + b]);
         } else {
-            const size_t r = (t - n1) / nv;
-            const int j = (int)(r % no), i = (int)(r / no);
-            const double x = T[b + (size_t)nv * (j + (size_t)nop * (ga + (size_t)nvp * i))];
-            const double xs = T[b + (size_t)nv * (i + (size_t)nop * (ga + (size_t)nvp * j))];
+            const size_t u = t - n1;
+            const int j = (int)(u % no);
+            const size_t r = u / no;
+            const int i = (int)(r % no), b = (int)(r / no);
+            const size_t base = nop2 * ((size_t)ga + (size_t)nvp * b);
+            const double x = T[base + j + (size_t)nop * i];
+            const double xs = T[base + i + (size_t)nop * j];
             s += (2.0 * x * x - x * xs) / (eps[g] + eps[iocc + b] - eps[icore + i] - eps[icore + j]);
         }
     }
@@ -2780,8 +2795,9 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     size_t n4 = (size_t)n * n * n * n;
     DevBuf buf;
     double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc, *dQ;
-    size_t x1 = (size_t)nv * n * n * n, x2 = (size_t)nv * nop * n * n, x3 = (size_t)nv * nop * nvp * n;
-    size_t x4 = (size_t)nv * nop * nvp * nop;
+    // occupied-side indices first: the largest intermediate is nop n^3
+    size_t x1 = (size_t)nop * n * n * n, x2 = (size_t)nop * nop * n * n, x3 = (size_t)nop * nop * nvp * n;
+    size_t x4 = (size_t)nop * nop * nvp * nv;
     if ((rc = buf.get(&dTen, n4)) || (rc = buf.get(&dCO, (size_t)nc * nc)) || (rc = buf.get(&dC, (size_t)n * nc)) ||
         (rc = buf.get(&dX1, x1)) || (rc = buf.get(&dX2, x2)) || (rc = buf.get(&dX3, x3)) || (rc = buf.get(&dT4, x4)) ||
         (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)) || (rc = buf.get(&dQ, (size_t)nc)))
@@ -2799,43 +2815,56 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     const double* Cv = dC + (size_t)n * iocc;    // virtual columns (b)
     const double* Co = dC + (size_t)n * icore;   // occupied-side columns (j', i')
     const double* Ca = dC + (size_t)n * vlo;     // virtual-side columns (a')
+    // Tensor A(s, l, u, m) = (sl|um) at s + n (l + n (u + n m)).  Its 8-fold
+    // symmetry makes the reversed reading A[m + n (u + n (l + n s))] the same
+    // tensor, so the first transform reads s as the slowest index: rows of n^3
+    // consecutive elements, streamed in long runs.  Quarter transforms:
+    //   X1(i, m, u, l) = sum_s Co(s, i) A(s, l, u, m)       X1[i + nop (m + n (u + n l))]
+    //   X2(j, i, m, l) = sum_u X1(i, m, u, l) Co(u, j)      [batched over l]
+    //   X3(j, i, a, m) = sum_l X2(j, i, m, l) Ca(l, a)      [batched over m]
+    //   T(j, i, a, b)  = sum_m X3(j, i, a, m) Cv(m, b)  = (ia|jb)
+    const long N = n, NO = nop, NA = nvp, NO2 = (long)nop * nop, N3 = (long)n * n * n;
     if (gemm == POLYERI_GEMM_ROCBLAS) {
-        // tensor A(sigma, lambda, nu, mu) column-major; X1(b, lam nu mu) = Cv^T A
-        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nv, n * n * n, n, &one, Cv, n, dTen,
-                            n, &zero, dX1, nv));
-        // X2(b, j, nu mu) = X1(b, lam, nu mu) Co(lam, j)  [batched over nu mu]
-        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv, nop, n, &one, dX1,
-                                            nv, (rocblas_stride)nv * n, Co, n, 0, &zero, dX2, nv,
-                                            (rocblas_stride)nv * nop, n * n));
-        // X3(bj, a, mu) = X2(bj, nu, mu) Ca(nu, a)  [batched over mu]
-        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nv * nop, nvp, n, &one,
-                                            dX2, nv * nop, (rocblas_stride)nv * nop * n, Ca, n, 0, &zero, dX3, nv * nop,
-                                            (rocblas_stride)nv * nop * nvp, n));
-        // T(bja, i) = X3(bja, mu) Co(mu, i)
-        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nv * nop * nvp, nop, n, &one, dX3,
-                            nv * nop * nvp, Co, n, &zero, dT4, nv * nop * nvp));
+        // column-major views: X1 is nop x n^3, X2 (as its transpose) nop x (nop n) per l, X3 nop^2 x nvp per m
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, nop, n * n * n, n, &one, Co, n,
+                            dTen, n * n * n, &zero, dX1, nop));
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_transpose, nop, nop * n, n,
+                                            &one, Co, n, 0, dX1, nop * n, (rocblas_stride)nop * n * n, &zero, dX2, nop,
+                                            (rocblas_stride)NO2 * n, n));
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, nop * nop, nvp, n, &one,
+                                            dX2, nop * nop * n, (rocblas_stride)NO2, Ca, n, 0, &zero, dX3, nop * nop,
+                                            (rocblas_stride)NO2 * nvp, n));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nop * nop * nvp, nv, n, &one, dX3,
+                            nop * nop * nvp, Cv, n, &zero, dT4, nop * nop * nvp));
     } else {
         // the same four quarter transforms on this library's MFMA kernels
         const int mode = gemm == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
-        const long N = n, NV = nv, NO = nop, NA = nvp;
-        int r = polyeri_gemm_launch(mode, nv, n * n * n, n, Cv, N, 1, 0, dTen, 1, N, 0, dX1, 1, NV, 0, 1, s);
-        if (!r) r = polyeri_gemm_launch(mode, nv, nop, n, dX1, 1, NV, NV * N, Co, 1, N, 0, dX2, 1, NV, NV * NO, n * n, s);
-        if (!r)
-            r = polyeri_gemm_launch(mode, nv * nop, nvp, n, dX2, 1, NV * NO, NV * NO * N, Ca, 1, N, 0, dX3, 1, NV * NO,
-                                    NV * NO * NA, n, s);
-        if (!r) r = polyeri_gemm_launch(mode, nv * nop * nvp, nop, n, dX3, 1, NV * NO * NA, 0, Co, 1, N, 0, dT4, 1,
-                                        NV * NO * NA, 0, 1, s);
+        int r = polyeri_gemm_launch(mode, nop, n * n * n, n, Co, N, 1, 0, dTen, N3, 1, 0, dX1, 1, NO, 0, 1, s);
+        if (!r) r = polyeri_gemm_launch(mode, nop * n, nop, n, dX1, 1, NO * N, NO * N * N, Co, 1, N, 0, dX2, NO, 1,
+                                        NO2 * N, n, s);
+        if (!r) r = polyeri_gemm_launch(mode, nop * nop, nvp, n, dX2, 1, NO2 * N, NO2, Ca, 1, N, 0, dX3, 1, NO2,
+                                        NO2 * NA, n, s);
+        if (!r) r = polyeri_gemm_launch(mode, nop * nop * nvp, nv, n, dX3, 1, NO2 * NA, 0, Cv, 1, N, 0, dT4, 1,
+                                        NO2 * NA, 0, 1, s);
         if (r) return fail(POLYERI_E_HIP, "MP2 transform launch failed");
     }
     HIPCHK(hipEventRecord(ctx->ev1, s));
     HIPCHK(hipEventSynchronize(ctx->ev1));
     ctx->last_mp2_tensor_ms = event_ms(ctx->ev0, et);
     ctx->last_mp2_transform_ms = event_ms(et, ctx->ev1);
-    ctx->last_mp2_flops = 2.0 * nv * (double)n * n * n * n + 2.0 * nv * (double)nop * n * n * n +
-                          2.0 * nv * (double)nop * nvp * n * n + 2.0 * nv * (double)nop * nvp * nop * n;
+    ctx->last_mp2_flops = 2.0 * nop * (double)n * n * n * n + 2.0 * nop * (double)nop * n * n * n +
+                          2.0 * nop * (double)nop * nvp * n * n + 2.0 * nop * (double)nop * nvp * nv * n;
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
-    hipLaunchKernelGGL(k_mp2_energy, dim3(nblk((size_t)no * nv * no * nv)), dim3(256), 0, s, dT4, dEps, no, nv, nop,
-                       nvp, aoff, icore, iocc, dAcc);
+    {
+        const size_t tile = (size_t)no * (no + 1) * 8;
+        const dim3 eg((unsigned)nv, (unsigned)std::min(nv, 16));
+        if (tile <= 56 * 1024)
+            hipLaunchKernelGGL(k_mp2_energy<true>, eg, dim3(256), tile, s, dT4, dEps, no, nv, nop, nvp, aoff, icore,
+                               iocc, dAcc);
+        else
+            hipLaunchKernelGGL(k_mp2_energy<false>, eg, dim3(256), 0, s, dT4, dEps, no, nv, nop, nvp, aoff, icore,
+                               iocc, dAcc);
+    }
     HIPCHK(hipGetLastError());
     const int ng = nqp > 0 ? ohi - vlo : 0;
     if (ng > 0) {
@@ -2864,7 +2893,7 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
 
 int polyeri_mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,
                        int ivirtcore, double* e2s, double* e2t) {
-    return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, POLYERI_GEMM_ROCBLAS, e2s, e2t);
+    return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, POLYERI_GEMM_MFMA_F64, e2s, e2t);
 }
 
 int polyeri_mp2_energy_ex(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall,

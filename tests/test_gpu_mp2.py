@@ -1,7 +1,7 @@
 """MP2 AO->MO transform on this library's MFMA kernels.
 
-The four quarter transforms of polyeri_mp2_energy run either on rocBLAS
-dgemm, on the library's FP64 MFMA kernel (v_mfma_f64_16x16x4f64) or on its
+The four quarter transforms of polyeri_mp2_energy run on the library's FP64
+MFMA kernel (v_mfma_f64_16x16x4f64, the default), on rocBLAS dgemm or on its
 BF16x3-split kernel (three BF16 pieces per FP64 operand, six MFMA products,
 FP32 accumulation).  Bars: the shipped goldens (H2O/cc-pVDZ E(2), BH
 frozen-core E(2)) within 1e-8 Eh for FP64 MFMA; the BF16x3 error against
@@ -57,8 +57,9 @@ def test_mp2_water4_transforms_agree():
     r = ctx.self_consistent_field(scfconv=1e-9, diis=2)
     print("case: MP2 (H2O)4 transforms", flush=True)
     e = {k: sum(ctx.mp2_energy(r["co"], r["eps"], 4, r["iocc"], r["n_indep"], transform=k))
-         for k in ("rocblas", "f64", "bf16x3")}
+         for k in (None, "rocblas", "f64", "bf16x3")}
     print({k: f"{v:.12f}" for k, v in e.items()}, flush=True)
+    assert abs(e[None] - e["f64"]) < 1e-12     # the default transform is the FP64 MFMA kernel
     assert abs(e["f64"] - e["rocblas"]) < 1e-10
     assert abs(e["bf16x3"] - e["rocblas"]) < 1e-6
 
