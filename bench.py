@@ -83,6 +83,46 @@ def cpu_baseline(atoms, basis, quartets_per_build, budget_s):
     return one, allc
 
 
+def mp2_line(nwater, seed, transforms):
+    """MP2 of (H2O)nwater / cc-pVDZ: the AO tensor and the four quarter
+    transforms on the device, each transform warmed once then timed."""
+    import polymer_amd as pa
+    from polymer_amd import geometry as G
+    print("bench: MP2 transform", file=sys.stderr, flush=True)
+    basis = "cc-pVDZ"
+    m_atoms = G.water_cluster(nwater, seed)
+    mb = pa.read_basis(basis, m_atoms)
+    mc = pa.Context(mb, schwarz_tol=1e-12)
+    r = mc.self_consistent_field(scfconv=1e-9, diis=2, scfcycles=80, solver=1, diis_mode=1)
+    mp2 = {"workload": f"(H2O){nwater} {basis} MP2, all electrons correlated, seed {seed}",
+           "basis_functions": len(set(mb.real_functions.tolist())), "ncgs_with_ghosts": mb.ncgs,
+           "occupied": r["iocc"], "virtual": r["n_indep"] - r["iocc"]}
+    names = {"f64": "mfma_f64", "rocblas": "rocblas", "bf16x3": "mfma_bf16x3"}
+    for tr in transforms:
+        mc.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], transform=tr)     # warm
+        t0 = time.perf_counter()
+        e_s, e_t = mc.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], transform=tr)
+        wall = time.perf_counter() - t0
+        st = mc.stats()
+        mp2[names[tr]] = {"e2": e_s + e_t, "wall_s": wall, "tensor_ms": st["last_mp2_tensor_ms"],
+                          "transform_ms": st["last_mp2_transform_ms"],
+                          "transform_tflops": st["last_mp2_flops"] / (st["last_mp2_transform_ms"] * 1e9)}
+        mp2["transform_flops"] = st["last_mp2_flops"]
+    mc.close()
+    if "mfma_f64" in mp2:
+        f = mp2["mfma_f64"]
+        if "rocblas" in mp2:
+            mp2["dE2_mfma_f64_vs_rocblas"] = f["e2"] - mp2["rocblas"]["e2"]
+            mp2["speedup_vs_rocblas"] = mp2["rocblas"]["transform_ms"] / f["transform_ms"]
+        if "mfma_bf16x3" in mp2:
+            mp2["dE2_bf16x3_vs_mfma_f64"] = mp2["mfma_bf16x3"]["e2"] - f["e2"]
+        mp2["roofline"] = {"bound": "mfma_fp64", "achieved": f["transform_tflops"], "peak": FP64_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": f["transform_tflops"] / FP64_PEAK_TFLOPS,
+                           "flops_model": "2 no n^4 + 2 no^2 n^3 + 2 no^2 nv n^2 + 2 no^2 nv^2 n (four quarter "
+                                          "transforms, occupied indices first)"}
+    return mp2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +135,8 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="skip the two converged SCF runs")
     ap.add_argument("--no-mp2", action="store_true", help="skip the MP2 transform line")
     ap.add_argument("--mp2-nwater", type=int, default=12)
+    ap.add_argument("--mp2-transforms", default="f64,rocblas", help="comma list of f64, rocblas, bf16x3")
+    ap.add_argument("--mp2-only", action="store_true", help="print the MP2 line alone (for profiling runs)")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="run the library's copy kernel over this many MiB first (counter scaling)")
     ap.add_argument("--save-tuning", default=None, help="write the autotuned launch records to this JSON file")
@@ -105,6 +147,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the J/K build "
               "is not sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
+    if args.mp2_only:
+        print(json.dumps(mp2_line(args.mp2_nwater, args.seed, args.mp2_transforms.split(","))))
+        return
 
     import numpy as np
 
@@ -271,34 +316,7 @@ def main():
                              "dE_vs_fp64_reference": scf["combined"]["dE_vs_fp64_reference"],
                              "target_dE": 1e-6}
 
-    mp2 = None
-    if not args.no_mp2:
-        print("bench: MP2 transform", file=sys.stderr, flush=True)
-        m_atoms = G.water_cluster(args.mp2_nwater, args.seed)
-        mb = pa.read_basis(basis, m_atoms)
-        mc = pa.Context(mb, schwarz_tol=1e-12)
-        r = mc.self_consistent_field(scfconv=1e-9, diis=2, scfcycles=80, solver=1, diis_mode=1)
-        mp2 = {"workload": f"(H2O){args.mp2_nwater} {basis} MP2, all electrons correlated, seed {args.seed}",
-               "basis_functions": len(set(mb.real_functions.tolist())), "ncgs_with_ghosts": mb.ncgs,
-               "occupied": r["iocc"], "virtual": r["n_indep"] - r["iocc"]}
-        for name, tr in (("mfma_f64", None), ("rocblas", "rocblas")):
-            mc.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], transform=tr)     # warm
-            t0 = time.perf_counter()
-            e_s, e_t = mc.mp2_energy(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], transform=tr)
-            wall = time.perf_counter() - t0
-            st2 = mc.stats()
-            mp2[name] = {"e2": e_s + e_t, "wall_s": wall, "tensor_ms": st2["last_mp2_tensor_ms"],
-                         "transform_ms": st2["last_mp2_transform_ms"],
-                         "transform_tflops": st2["last_mp2_flops"] / (st2["last_mp2_transform_ms"] * 1e9)}
-        mp2["transform_flops"] = st2["last_mp2_flops"]
-        mp2["dE2_mfma_f64_vs_rocblas"] = mp2["mfma_f64"]["e2"] - mp2["rocblas"]["e2"]
-        mp2["speedup_vs_rocblas"] = mp2["rocblas"]["transform_ms"] / mp2["mfma_f64"]["transform_ms"]
-        mp2["roofline"] = {"bound": "mfma_fp64", "achieved": mp2["mfma_f64"]["transform_tflops"],
-                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": mp2["mfma_f64"]["transform_tflops"] / FP64_PEAK_TFLOPS,
-                           "flops_model": "2 no n^4 + 2 no^2 n^3 + 2 no^2 nv n^2 + 2 no^2 nv^2 n (four quarter "
-                                          "transforms, occupied indices first)"}
-        mc.close()
+    mp2 = None if args.no_mp2 else mp2_line(args.mp2_nwater, args.seed, args.mp2_transforms.split(","))
 
     # HBM bytes per launch of the dominant kernel from the committed counter
     # summary of this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs

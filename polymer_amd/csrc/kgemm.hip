@@ -65,6 +65,18 @@ __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned of
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
+// LDS column swizzle of k row kk for operands staged k fastest (mode 1):
+// 16 lanes store one column's 16 k rows; with rows 16 mod 32 doubles apart
+// they would fall on two bank pairs.  XOR-ing the column's low three bits
+// with kk / 2 spreads them over 16 (the even rows over the first 16 pairs,
+// the odd ones, shifted by the row pitch, over the other 16), and a
+// fragment read (16 consecutive columns of one row) still covers the same
+// 16 aligned columns.
+template <int MODE>
+__device__ __forceinline__ int swz(int kk) {
+    return MODE == 1 ? (kk >> 1) & 7 : 0;
+}
+
 template <int WM, int WN, int TMW, int TNW, bool AKC, bool BKC>
 __global__ void __launch_bounds__(64 * WM * WN, 2) k_gemm_f64_tiled(GemmArgs g, int mblocks) {
     constexpr int NT = 64 * WM * WN, BM = WM * TMW * 16, BN = WN * TNW * 16, TK = 16;
@@ -140,10 +152,10 @@ __global__ void __launch_bounds__(64 * WM * WN, 2) k_gemm_f64_tiled(GemmArgs g, 
     auto store = [&](int buf, const double (&ra)[NA], const double (&rb)[NB]) {
 #pragma unroll
         for (int p = 0; p < NA; ++p)
-            if (SA % NT == 0 || t + NT * p < SA) As[buf][a_kk(p)][a_mm(p)] = ra[p];
+            if (SA % NT == 0 || t + NT * p < SA) As[buf][a_kk(p)][a_mm(p) ^ swz<AMODE>(a_kk(p))] = ra[p];
 #pragma unroll
         for (int p = 0; p < NB; ++p)
-            if (SB % NT == 0 || t + NT * p < SB) Bs[buf][b_kk(p)][b_nn(p)] = rb[p];
+            if (SB % NT == 0 || t + NT * p < SB) Bs[buf][b_kk(p)][b_nn(p) ^ swz<BMODE>(b_kk(p))] = rb[p];
     };
     const int r = lane & 15, q = lane >> 4;
     const int wr = (wave % WM) * TMW * 16, wc = (wave / WM) * TNW * 16;
@@ -157,9 +169,9 @@ __global__ void __launch_bounds__(64 * WM * WN, 2) k_gemm_f64_tiled(GemmArgs g, 
         for (int ks = 0; ks < TK; ks += 4) {
             double a[TMW], b[TNW];
 #pragma unroll
-            for (int i = 0; i < TMW; ++i) a[i] = As[buf][ks + q][wr + i * 16 + r];
+            for (int i = 0; i < TMW; ++i) a[i] = As[buf][ks + q][(wr + i * 16 + r) ^ swz<AMODE>(ks + q)];
 #pragma unroll
-            for (int j = 0; j < TNW; ++j) b[j] = Bs[buf][ks + q][wc + j * 16 + r];
+            for (int j = 0; j < TNW; ++j) b[j] = Bs[buf][ks + q][(wc + j * 16 + r) ^ swz<BMODE>(ks + q)];
 #pragma unroll
             for (int i = 0; i < TMW; ++i)
 #pragma unroll
