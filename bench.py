@@ -287,7 +287,7 @@ def main():
                 ("mixed_incremental", cmx, {"incremental": 8, "dens_tol": 1e-11}),
                 ("fp64_purify_cdiis", ctx, fast),
                 ("fp64_reference", ctx, {**fast, "scfconv": 1e-10}),
-                ("combined", cmx, {**fast, "incremental": 6, "dens_tol": 1e-10, "guess": learned}))
+                ("combined", cmx, {**fast, "incremental": 80, "dens_tol": 2e-9, "guess": learned}))
         for name, c, kw in runs:
             kw = dict(kw)
             c.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80,

@@ -2383,13 +2383,17 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
         (rc = buf.get(&dE, nc)) || (rc = buf.get(&dWk, nc)) || (rc = buf.get(&dCO, n2)) || (rc = buf.get(&dP, n2)) ||
         (rc = buf.get(&dPout, n2)) || (rc = buf.get(&dOcc, nc)) || (rc = buf.get(&dFp, n2)) || (rc = buf.get(&dInfo, 1)))
         return rc;
-    // incremental builds: density / J / K of the last build, the difference
+    // incremental builds: density / J / K of the last build, the difference;
+    // density-screened builds keep the input density of the last build too
+    // (for the unscreened build at convergence)
     double *dPprev = nullptr, *dJprev = nullptr, *dKprev = nullptr, *dDel = nullptr, *dJd = nullptr, *dKd = nullptr;
-    if (o->incremental > 0)
-        if ((rc = buf.get(&dPprev, n2)) || (rc = buf.get(&dJprev, n2)) || (rc = buf.get(&dKprev, n2)) ||
-            (rc = buf.get(&dDel, n2)) || (rc = buf.get(&dJd, n2)) || (rc = buf.get(&dKd, n2)))
-            return rc;
     const double dens_tol = o->dens_tol > 0 ? o->dens_tol : 0.0;
+    if (o->incremental > 0 || dens_tol > 0.0)
+        if ((rc = buf.get(&dPprev, n2))) return rc;
+    if (o->incremental > 0)
+        if ((rc = buf.get(&dJprev, n2)) || (rc = buf.get(&dKprev, n2)) || (rc = buf.get(&dDel, n2)) ||
+            (rc = buf.get(&dJd, n2)) || (rc = buf.get(&dKd, n2)))
+            return rc;
     int since_full = 0;
     const int HMAX = 9;   // DIIS uses at most 8 previous vectors
     double *dHP, *dHD;
@@ -2548,8 +2552,8 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             if ((rc = jk_device(ctx, dP, ctx->d_jc, ctx->d_kc, dens_tol))) break;
             since_full = 1;
         }
+        if (dPprev) HIPCHK(hipMemcpyAsync(dPprev, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
         if (o->incremental > 0) {
-            HIPCHK(hipMemcpyAsync(dPprev, dP, n2 * 8, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipMemcpyAsync(dJprev, ctx->d_jc, n2 * 8, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipMemcpyAsync(dKprev, ctx->d_kc, n2 * 8, hipMemcpyDeviceToDevice, s));
         }
@@ -2705,10 +2709,10 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
             RBCHK(rocblas_dscal(h, (int)n2, &b, dP, 1));
             RBCHK(rocblas_daxpy(h, (int)n2, &a, dPout, 1, dP, 1));
         }
-        // Converged on an incremental build: the energy below takes J/K of this
-        // iteration's input density from one full, unscreened build, so it
-        // carries no accumulated screening error.
-        if (dchg < o->scfconv && inc_now) {
+        // Converged on an incremental or density-screened build: the energy
+        // below takes J/K of this iteration's input density from one full,
+        // unscreened build, so it carries no screening error.
+        if (dchg < o->scfconv && (inc_now || dens_tol > 0.0)) {
             if ((rc = jk_device(ctx, dPprev, ctx->d_jc, ctx->d_kc, 0.0))) break;
             HIPCHK(hipStreamSynchronize(s));
             if ((rc = jk_collect_timing(ctx))) break;

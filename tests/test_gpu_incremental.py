@@ -66,3 +66,24 @@ def test_incremental_scf_water8(precision):
     assert ref["converged"] and r["converged"]
     assert abs(de) < 1e-6
     assert r["quartets"] < ref["quartets"]
+
+
+@pytest.mark.parametrize("incremental", [0, 80])
+def test_screened_builds_converged_energy_exact(incremental):
+    """Density-screened builds (full or incremental, never a periodic full
+    rebuild) with purification and commutator DIIS, the combined solver's
+    settings: the converged energy comes from one unscreened build of the
+    last input density, so it matches a tightly converged FP64 reference far
+    below 1 uEh."""
+    atoms = G.water_cluster(8, 4)
+    print(f"case: screened SCF (H2O)8/cc-pVDZ incremental {incremental} dens_tol 2e-9", flush=True)
+    b = pa.read_basis("cc-pVDZ", atoms)
+    ctx = pa.Context(b, schwarz_tol=1e-12)
+    ref = ctx.self_consistent_field(scfconv=1e-10, diis=2, scfcycles=80, solver=1, diis_mode=1)
+    r = ctx.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80, solver=1, diis_mode=1,
+                                  incremental=incremental, dens_tol=2e-9)
+    de = r["e_total"] - ref["e_total"]
+    print(f"E reference {ref['e_total']:.10f} screened {r['e_total']:.10f} dE {de:.2e}; iterations "
+          f"{ref['iterations']} / {r['iterations']}", flush=True)
+    assert ref["converged"] and r["converged"]
+    assert abs(de) < 1e-8
