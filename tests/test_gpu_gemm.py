@@ -46,7 +46,10 @@ def _run(ctx, mode, A, B, ta, tb):
 
 @pytest.mark.parametrize("mode,tol", [(1, 1e-13), (2, 1e-6)])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-@pytest.mark.parametrize("M,K,N", [(35, 19, 21), (150, 50, 300)])
+# shapes that take each tile form of the FP64 kernel: short M (1, 3 and 4 row
+# tiles), narrow N, 128-wide and 160-wide column blocks, partial K slices
+@pytest.mark.parametrize("M,K,N", [(35, 19, 21), (150, 50, 300), (16, 300, 9), (64, 33, 130), (150, 37, 40),
+                                   (100, 70, 150)])
 def test_gemm_f64_kernels(ctx, mode, tol, ta, tb, M, K, N):
     rng = np.random.default_rng(7)
     A = rng.standard_normal((K, M) if ta else (M, K))
