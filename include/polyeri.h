@@ -182,6 +182,24 @@ int polyeri_load_fourindex_eri(polyeri_ctx *ctx, const char *path, long long *re
  * direct = 0: they use the loaded records (error if none are loaded). */
 int polyeri_set_direct(polyeri_ctx *ctx, int direct);
 
+/* Two-electron part of the nuclear gradient (FOURINDEX_ERI_GRADIENT,
+ * src/pfourindexgradient.f90:1-1478, called at src/psinglepoint.f90:380;
+ * molecule, HELIX 0): grad[3*atom + axis] = dE2/dR_atom, the sum that
+ * routine adds into FORCEX / FORCEY / FORCEZ
+ * (src/pfourindexgradient.f90:1334-1420), of
+ * E2 = 1/2 sum P P (mu nu|lam sig) - hfexchange/4 sum P P (mu lam|nu sig)
+ * at fixed P_C (ncgs x ncgs column-major, the SCF density; hfexchange =
+ * DOPTN(19)), in Eh/bohr.  Quartets below the context's Schwarz threshold
+ * are skipped; molecular contexts, at most 2048 atoms. */
+int polyeri_eri_gradient(polyeri_ctx *ctx, const double *p_c, double hfexchange, double *grad);
+/* The same sum evaluated on the host by the same code, from the basis
+ * arrays of polyeri_ctx_create and a Schwarz threshold, without a context
+ * or a GPU (slow; for checks).  No counterpart in the reference. */
+int polyeri_eri_gradient_host(int npgs, int ncgs, const int *pax, const int *pay, const int *paz,
+                              const double *pgsx, const double *pgsy, const double *pgsz, const double *zt,
+                              const double *cc, int natom, const int *iatom, const double *xyz, double schwarz_tol,
+                              const double *p_c, double hfexchange, double *grad);
+
 typedef struct polyeri_scf_opts {
     double scfconv;     /* DOPTN(14) RMS density change                     */
     double scfrelax;    /* DOPTN(15)                                        */

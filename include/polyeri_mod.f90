@@ -94,6 +94,14 @@ MODULE POLYERI
          TYPE(POLYERI_SCF_RESULT) :: RES
       END FUNCTION
 
+      INTEGER(C_INT) FUNCTION POLYERI_ERI_GRADIENT(CTX, P_C, HFEXCHANGE, GRAD) &
+            BIND(C, NAME="polyeri_eri_gradient")
+         IMPORT :: C_INT, C_DOUBLE, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         REAL(C_DOUBLE) :: P_C(*), GRAD(*)
+         REAL(C_DOUBLE), VALUE :: HFEXCHANGE
+      END FUNCTION
+
       INTEGER(C_INT) FUNCTION POLYERI_MP2_ENERGY(CTX, CO, EPS, ICORE, IOCC, NALL, IVIRTCORE, E2S, E2T) &
             BIND(C, NAME="polyeri_mp2_energy")
          IMPORT :: C_INT, C_DOUBLE, C_PTR

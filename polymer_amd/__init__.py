@@ -49,6 +49,20 @@ class RefBasis:
         return np.nonzero(np.any(self.cc != 0.0, axis=1))[0]
 
 
+def eri_gradient_host(basis, P, hfexchange=1.0, schwarz_tol=0.0):
+    """The two-electron gradient of Context.eri_gradient evaluated on the
+    host by the same code, without a context or a GPU (slow; for checks)."""
+    b = basis
+    z = np.ascontiguousarray([a[0] for a in b.atoms], dtype=np.int32)
+    xyz = np.ascontiguousarray([[a[1], a[2], a[3]] for a in b.atoms], dtype=np.float64)
+    P = np.asfortranarray(P, dtype=np.float64)
+    g = np.zeros((len(b.atoms), 3))
+    check(lib().polyeri_eri_gradient_host(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz), _p(b.pgsx), _p(b.pgsy),
+                                          _p(b.pgsz), _p(b.zt), _p(b.cc), len(b.atoms), _p(z), _p(xyz),
+                                          float(schwarz_tol), _p(P), float(hfexchange), _p(g)))
+    return g
+
+
 def restore_fourindex_eri(path, ncgs):
     """Read a molecular .ao file (RESTORE_FOURINDEX_ERI) into an (ncgs,)*4
     Fortran-order tensor; returns (tensor, number of integrals read)."""
@@ -396,6 +410,15 @@ class Context:
         """True: evaluate the integrals in every J/K build (default); False:
         use the loaded .ao records."""
         check(lib().polyeri_set_direct(self.h, int(bool(direct))))
+
+    def eri_gradient(self, P, hfexchange=1.0):
+        """Two-electron part of the nuclear gradient at fixed density P_C
+        ((ncgs, ncgs), the SCF density): dE2/dR as an (natom, 3) array in
+        Eh/bohr (FOURINDEX_ERI_GRADIENT)."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        g = np.zeros((len(self.basis.atoms), 3))
+        check(lib().polyeri_eri_gradient(self.h, _p(P), float(hfexchange), _p(g)))
+        return g
 
     def launch_configs(self):
         """Per work list: the 9-int launch record of polyeri_get_launch_config."""

@@ -1273,6 +1273,24 @@ int polyeri_rys_nodes_host(int n, double T, double* u, double* w);
 // code on the host (0; -1 bad class or pointer; -2 an integral not produced
 // exactly once).
 int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, const double* ex, double* out);
+
+// Two-electron part of the nuclear gradient (kgrad.hip): shells of the
+// library's basis (L, atom, first function, first primitive, primitive
+// count, centre), primitive exponents and coefficients, the density in the
+// library's basis (nbf x nbf), the per-pair Schwarz bounds (filled by the
+// launcher), the Schwarz threshold and the exchange scale.
+struct GradArgs {
+    int nsh, natom, nbf;
+    const int *L, *atom, *f0, *p0, *np;
+    const double *xyz, *alpha, *coef;
+    const double* D;
+    const double* Qp;
+    double tol, xfac;
+};
+// grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: nsh (nsh + 1) / 2 doubles
+int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream);
+// the same sum on the host (host pointers), grad overwritten
+int polyeri_grad_host(const GradArgs& g, double* grad);
 // One record of the reference's .ao files (kstored.hip): packed cells,
 // packed 1-based indices I J K L, the integral.
 struct StoredEri {

@@ -2872,7 +2872,7 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     HIPCHK(hipGetLastError());
     const int ng = nqp > 0 ? ohi - vlo : 0;
     if (ng > 0) {
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dQ, (size_t)ng, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)ng)), dim3(256), 0, s, dQ, (size_t)ng, 0.0);
         const size_t terms = (size_t)nv * no * (nv + no);
         const unsigned gx = (unsigned)std::min<size_t>(nblk(terms), 1024);
         hipLaunchKernelGGL(k_mp2_qp, dim3(gx, (unsigned)ng), dim3(256), 0, s, dT4, dEps, no, nv, nop, nvp, aoff, icore,
@@ -3778,4 +3778,114 @@ int polyeri_set_direct(polyeri_ctx* ctx, int direct) {
     if (!direct && ctx->n_ao <= 0) return fail(POLYERI_E_ARG, "polyeri_set_direct: no stored integrals loaded");
     ctx->use_stored = !direct;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Two-electron part of the nuclear gradient (FOURINDEX_ERI_GRADIENT,
+// src/pfourindexgradient.f90:1-1478, molecule): kgrad.hip
+// ---------------------------------------------------------------------------
+struct GradHost {
+    std::vector<int> L, atom, f0, p0, np;
+    std::vector<double> xyz, alpha, coef, D;
+};
+
+// Shells, primitives and the density in the library's basis:
+// D(mu, nu) = s_mu s_nu P_C(ref mu, ref nu).  The reference's arrays carry
+// no atom index per primitive, so each shell's atom is the one at its
+// centre; -1 when no atom is there.
+static int grad_host_arrays(const IntBasis& ib, const RefBasis& rb, const double* p_c, GradHost& h) {
+    for (const Shell& s : ib.shells) {
+        int at = -1;
+        for (int a = 0; a < rb.natom && at < 0; ++a)
+            if (std::fabs(rb.ax[a] - s.x) < 1e-8 && std::fabs(rb.ay[a] - s.y) < 1e-8 && std::fabs(rb.az[a] - s.z) < 1e-8)
+                at = a;
+        if (at < 0) return fail(POLYERI_E_BASIS, "eri gradient: a shell centre is not at an atom");
+        h.L.push_back(s.L);
+        h.atom.push_back(at);
+        h.f0.push_back(s.f0);
+        h.p0.push_back((int)h.alpha.size());
+        h.np.push_back((int)s.alpha.size());
+        h.xyz.insert(h.xyz.end(), {s.x, s.y, s.z});
+        h.alpha.insert(h.alpha.end(), s.alpha.begin(), s.alpha.end());
+        h.coef.insert(h.coef.end(), s.coef.begin(), s.coef.end());
+    }
+    const int nbf = ib.nbf, nc = rb.ncgs;
+    h.D.resize((size_t)nbf * nbf);
+    for (int mu = 0; mu < nbf; ++mu)
+        for (int nu = 0; nu < nbf; ++nu)
+            h.D[(size_t)mu * nbf + nu] =
+                ib.scale[mu] * ib.scale[nu] * p_c[ib.ref_of_int[mu] + (size_t)nc * ib.ref_of_int[nu]];
+    return 0;
+}
+
+static int grad_check(polyeri_ctx* ctx, const double* p_c, double* grad, const char* who) {
+    if (!ctx || !p_c || !grad) return fail(POLYERI_E_ARG, std::string(who) + ": null argument");
+    if (ctx->pbc) return fail(POLYERI_E_ARG, std::string(who) + ": molecular contexts only");
+    if (3 * ctx->rb.natom * 8 > 48 * 1024) return fail(POLYERI_E_ARG, std::string(who) + ": more than 2048 atoms");
+    return 0;
+}
+
+int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange, double* grad) {
+    int rc;
+    if ((rc = grad_check(ctx, p_c, grad, "polyeri_eri_gradient"))) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    GradHost h;
+    if ((rc = grad_host_arrays(ctx->ib, ctx->rb, p_c, h))) return rc;
+    const int nsh = (int)h.L.size(), natom = ctx->rb.natom, nbf = ctx->ib.nbf;
+    const size_t npair = (size_t)nsh * (nsh + 1) / 2;
+    DevBuf buf;
+    int *dL, *dAt, *dF0, *dP0, *dNp;
+    double *dXyz, *dAl, *dCo, *dD, *dQp, *dG;
+    if ((rc = buf.get(&dL, nsh)) || (rc = buf.get(&dAt, nsh)) || (rc = buf.get(&dF0, nsh)) ||
+        (rc = buf.get(&dP0, nsh)) || (rc = buf.get(&dNp, nsh)) || (rc = buf.get(&dXyz, 3 * (size_t)nsh)) ||
+        (rc = buf.get(&dAl, h.alpha.size())) || (rc = buf.get(&dCo, h.coef.size())) ||
+        (rc = buf.get(&dD, h.D.size())) || (rc = buf.get(&dQp, npair)) || (rc = buf.get(&dG, 3 * (size_t)natom)))
+        return rc;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dL, h.L.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dAt, h.atom.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dF0, h.f0.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dP0, h.p0.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dNp, h.np.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dXyz, h.xyz.data(), h.xyz.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dAl, h.alpha.data(), h.alpha.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dCo, h.coef.data(), h.coef.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dD, h.D.data(), h.D.size() * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill, dim3(nblk(3 * (size_t)natom)), dim3(256), 0, s, dG, 3 * (size_t)natom, 0.0);
+    GradArgs g{nsh, natom, nbf, dL, dAt, dF0, dP0, dNp, dXyz, dAl, dCo, dD, nullptr, ctx->opts.schwarz_tol, hfexchange};
+    if (polyeri_grad_launch(g, dQp, dG, s)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
+    HIPCHK(hipMemcpyAsync(grad, dG, 3 * (size_t)natom * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < 3 * natom; ++i)
+        if (!std::isfinite(grad[i])) return fail(POLYERI_E_NONFINITE, "polyeri_eri_gradient: non-finite gradient");
+    return 0;
+}
+
+int polyeri_eri_gradient_host(int npgs, int ncgs, const int* pax, const int* pay, const int* paz,
+                              const double* pgsx, const double* pgsy, const double* pgsz, const double* zt,
+                              const double* cc, int natom, const int* iatom, const double* xyz, double schwarz_tol,
+                              const double* p_c, double hfexchange, double* grad) {
+    if (npgs <= 0 || ncgs <= 0 || !pax || !pay || !paz || !pgsx || !pgsy || !pgsz || !zt || !cc || natom <= 0 ||
+        !iatom || !xyz || !p_c || !grad)
+        return fail(POLYERI_E_ARG, "polyeri_eri_gradient_host: null or empty argument");
+    RefBasis rb;
+    rb.natom = natom;
+    rb.iatom.assign(iatom, iatom + natom);
+    for (int i = 0; i < natom; ++i) {
+        rb.ax.push_back(xyz[3 * i]); rb.ay.push_back(xyz[3 * i + 1]); rb.az.push_back(xyz[3 * i + 2]);
+    }
+    rb.npgs = npgs; rb.ncgs = ncgs;
+    rb.pax.assign(pax, pax + npgs); rb.pay.assign(pay, pay + npgs); rb.paz.assign(paz, paz + npgs);
+    rb.pgsx.assign(pgsx, pgsx + npgs); rb.pgsy.assign(pgsy, pgsy + npgs); rb.pgsz.assign(pgsz, pgsz + npgs);
+    rb.zt.assign(zt, zt + npgs);
+    rb.cc.assign(cc, cc + (size_t)npgs * ncgs);
+    rb.pgs_atom.assign(npgs, 0);
+    IntBasis ib;
+    std::string err;
+    if (extract_shells(rb, ib, err)) return fail(POLYERI_E_BASIS, err);
+    GradHost h;
+    if (int rc = grad_host_arrays(ib, rb, p_c, h)) return rc;
+    GradArgs g{(int)h.L.size(), natom, ib.nbf, h.L.data(), h.atom.data(), h.f0.data(), h.p0.data(), h.np.data(),
+               h.xyz.data(), h.alpha.data(), h.coef.data(), h.D.data(), nullptr, schwarz_tol, hfexchange};
+    return polyeri_grad_host(g, grad) ? fail(POLYERI_E_ARG, "polyeri_eri_gradient_host failed") : 0;
 }

@@ -1,0 +1,60 @@
+"""Two-electron nuclear gradient (FOURINDEX_ERI_GRADIENT,
+src/pfourindexgradient.f90:1334-1420) through the library's host replay of
+the device code (polyeri_eri_gradient_host), pinned by what the mathematics
+fixes:
+
+* central finite differences of the oracle's two-electron energy
+  E2(R) = 1/2 sum P_C C_C - x/4 sum P_C X_C at a fixed contracted density
+  (the basis functions move with their atoms), with the oracle's J/K
+  (itself pinned against McMurchie-Davidson integrals);
+* translational invariance: the gradient sums to zero over the atoms;
+* linearity in the exchange scale x.
+
+The density is a random symmetric matrix (E2 is a quadratic form in P, so
+any P exercises every term)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import polymer_amd as pa
+from tests import molecules as M
+
+
+def _e2(basis, atoms, P, x):
+    ob = oracle.Basis(os.path.join(pa.BASIS_DIR, basis), atoms)
+    c, k, _ = ob.fourindex_eri(P, 0.0)
+    return 0.5 * np.sum(P * c) - 0.25 * x * np.sum(P * k)
+
+
+def _fd(basis, atoms, P, x, h=1e-4):
+    g = np.zeros((len(atoms), 3))
+    for a in range(len(atoms)):
+        for e in range(3):
+            def moved(d):
+                at = [list(t) for t in atoms]
+                at[a][1 + e] += d
+                return [tuple(t) for t in at]
+            g[a, e] = (_e2(basis, moved(h), P, x) - _e2(basis, moved(-h), P, x)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "6-31Gss", "cc-pVDZ"])
+def test_eri_gradient_matches_finite_differences(basis):
+    atoms = M.H2O_GOLDEN
+    b = pa.read_basis(basis, atoms)
+    rng = np.random.default_rng(3)
+    P = rng.standard_normal((b.ncgs, b.ncgs))
+    P = 0.1 * (P + P.T)
+    x = 1.0
+    g = pa.eri_gradient_host(b, P, x)
+    fd = _fd(basis, atoms, P, x)
+    err = np.max(np.abs(g - fd))
+    print(f"{basis}: max |g - fd| = {err:.2e}, max |g| = {np.max(np.abs(g)):.3e}")
+    assert err < 1e-6 * max(1.0, np.max(np.abs(g)))
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-10 * max(1.0, np.max(np.abs(g)))
+    # linear in the exchange scale
+    g0 = pa.eri_gradient_host(b, P, 0.0)
+    g2 = pa.eri_gradient_host(b, P, 2.0)
+    assert np.max(np.abs(g2 - 2 * g + g0)) < 1e-10 * max(1.0, np.max(np.abs(g)))
