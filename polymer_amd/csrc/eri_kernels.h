@@ -1277,17 +1277,23 @@ int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, co
 // Two-electron part of the nuclear gradient (kgrad.hip): shells of the
 // library's basis (L, atom, first function, first primitive, primitive
 // count, centre), primitive exponents and coefficients, the density in the
-// library's basis (nbf x nbf), the per-pair Schwarz bounds (filled by the
-// launcher), the Schwarz threshold and the exchange scale.
+// library's basis (nbf x nbf) and its largest |element| per shell block
+// (nsh x nsh), the per-pair Schwarz bounds (filled by the launcher), the
+// threshold on Q_ab Q_cd and on Q_ab Q_cd max |Gamma|, the exchange scale.
+// Shell pairs (pa[i], pb[i]), pa >= pb, in the order the quartets run
+// (sorted by class and primitive count, so that a wavefront's lanes, which
+// share a bra pair and take consecutive ket pairs, run alike).
 struct GradArgs {
-    int nsh, natom, nbf;
+    int nsh, natom, nbf, npair;
     const int *L, *atom, *f0, *p0, *np;
+    const int *pa, *pb;
     const double *xyz, *alpha, *coef;
     const double* D;
+    const double* dsh;
     const double* Qp;
-    double tol, xfac;
+    double tol, xfac, ptol;
 };
-// grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: nsh (nsh + 1) / 2 doubles
+// grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: npair doubles
 int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream);
 // the same sum on the host (host pointers), grad overwritten
 int polyeri_grad_host(const GradArgs& g, double* grad);

@@ -20,9 +20,11 @@
 // horizontal transfers to I(i, j, k, l) for i + j <= La + Lb + 1 and
 // k + l <= Lc + Ld + 1 -- one level above the quartet, for the derivatives.
 //
-// One thread per unique contracted quartet (grid-stride), quartets with
-// Q_ab Q_cd below the Schwarz threshold skipped; per-atom sums in LDS, one
-// global atomic per atom coordinate per block.  The same code runs on the
+// One thread per unique contracted quartet (grid-stride) over a pair list
+// sorted by class and primitive count; quartets with Q_ab Q_cd, or Q_ab Q_cd
+// times the largest |Gamma| of the quartet, below the threshold skipped, and
+// primitive pairs below the J/K path's primitive threshold; per-atom sums in
+// LDS, one global atomic per atom coordinate per block.  The same code runs on the
 // host (polyeri_grad_host) for the CPU tests.
 #include <hip/hip_runtime.h>
 
@@ -80,10 +82,9 @@ POLY_HD void comp_of(int l, int q, int& lx, int& ly, int& lz) {
     lx = ly = lz = 0;
 }
 
+// binomial coefficients C(n, k), n <= 3 (the transfers go one level above d)
 POLY_HD double binom(int n, int k) {
-    double r = 1.0;
-    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
-    return r;
+    return n == 3 ? (k == 0 || k == 3 ? 1.0 : 3.0) : n == 2 ? (k == 1 ? 2.0 : 1.0) : 1.0;
 }
 
 // One-dimensional factors of one axis for one root: I[i][j][k][l] for
@@ -154,12 +155,14 @@ POLY_HD double pair_bound(const GradArgs& g, const RysTabs& rt, int a, int b) {
             const double al = g.alpha[A.p0 + i], be = g.alpha[B.p0 + j];
             const double ze = al + be, kab = std::exp(-al * be / ze * ab2);
             const double cab = g.coef[A.p0 + i] * g.coef[B.p0 + j] * kab;
+            if (std::fabs(cab) < g.ptol) continue;
             const double P[3] = {(al * A.x + be * B.x) / ze, (al * A.y + be * B.y) / ze, (al * A.z + be * B.z) / ze};
             for (int k = 0; k < A.np; ++k)
                 for (int l = 0; l < B.np; ++l) {
                     const double ga = g.alpha[A.p0 + k], de = g.alpha[B.p0 + l];
                     const double et = ga + de, kcd = std::exp(-ga * de / et * ab2);
                     const double ccd = g.coef[A.p0 + k] * g.coef[B.p0 + l] * kcd;
+                    if (std::fabs(ccd) < g.ptol) continue;
                     const double Q[3] = {(ga * A.x + de * B.x) / et, (ga * A.y + de * B.y) / et,
                                          (ga * A.z + de * B.z) / et};
                     const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
@@ -210,12 +213,14 @@ POLY_HD void grad_quartet(const GradArgs& g, const RysTabs& rt, int a, int b, in
             const double al = g.alpha[A.p0 + i], be = g.alpha[B.p0 + j];
             const double ze = al + be;
             const double cab = g.coef[A.p0 + i] * g.coef[B.p0 + j] * std::exp(-al * be / ze * ab2);
+            if (std::fabs(cab) < g.ptol) continue;          // the J/K pairs' primitive screen
             const double P[3] = {(al * A.x + be * B.x) / ze, (al * A.y + be * B.y) / ze, (al * A.z + be * B.z) / ze};
             for (int k = 0; k < C.np; ++k)
                 for (int l = 0; l < D.np; ++l) {
                     const double ga = g.alpha[C.p0 + k], de = g.alpha[D.p0 + l];
                     const double et = ga + de;
                     const double ccd = g.coef[C.p0 + k] * g.coef[D.p0 + l] * std::exp(-ga * de / et * cd2);
+                    if (std::fabs(ccd) < g.ptol) continue;
                     const double Q[3] = {(ga * C.x + de * D.x) / et, (ga * C.y + de * D.y) / et,
                                          (ga * C.z + de * D.z) / et};
                     const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
@@ -291,11 +296,16 @@ POLY_HD void tri_index(unsigned long long t, int& r, int& c) {
 // gradient through add(atom, axis, value); false when screened out.
 template <typename Add>
 POLY_HD void quartet_task(const GradArgs& g, const RysTabs& rt, unsigned long long t, Add add) {
-    int pij, pkl, a, b, c, d;
+    int pij, pkl;
     tri_index(t, pij, pkl);
-    if (g.Qp[pij] * g.Qp[pkl] < g.tol) return;
-    tri_index((unsigned long long)pij, a, b);
-    tri_index((unsigned long long)pkl, c, d);
+    const double qq = g.Qp[pij] * g.Qp[pkl];
+    if (qq < g.tol) return;
+    const int a = g.pa[pij], b = g.pb[pij], c = g.pa[pkl], d = g.pb[pkl];
+    // the density-weighted bound: Q_ab Q_cd max |Gamma| over the quartet
+    const int ns = g.nsh;
+    const double gmax = 4.0 * g.dsh[a * ns + b] * g.dsh[c * ns + d] +
+                        std::fabs(g.xfac) * (g.dsh[a * ns + c] * g.dsh[b * ns + d] + g.dsh[a * ns + d] * g.dsh[b * ns + c]);
+    if (qq * gmax < g.tol) return;
     const double fdeg = (a == b ? 0.5 : 1.0) * (c == d ? 0.5 : 1.0) * (pij == pkl ? 0.5 : 1.0);
     double gA[3], gB[3], gC[3];
     grad_quartet(g, rt, a, b, c, d, fdeg, gA, gB, gC);
@@ -324,12 +334,10 @@ RysTabs host_tabs() {
     return rt;
 }
 
-__global__ void __launch_bounds__(256) k_grad_schwarz(GradArgs g, double* Qp, int npair) {
+__global__ void __launch_bounds__(256) k_grad_schwarz(GradArgs g, double* Qp) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npair) return;
-    int a, b;
-    tri_index((unsigned long long)t, a, b);
-    Qp[t] = pair_bound(g, device_tabs(), a, b);
+    if (t >= g.npair) return;
+    Qp[t] = pair_bound(g, device_tabs(), g.pa[t], g.pb[t]);
 }
 
 // per-atom sums in LDS (3 natom doubles of dynamic shared memory), one
@@ -352,9 +360,9 @@ __global__ void __launch_bounds__(256) k_eri_grad(GradArgs g, unsigned long long
 }  // namespace
 
 int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream) {
-    const int npair = g.nsh * (g.nsh + 1) / 2;
-    if (g.nsh <= 0 || g.natom <= 0 || 3 * g.natom * 8 > 48 * 1024) return -1;
-    hipLaunchKernelGGL(k_grad_schwarz, dim3((npair + 255) / 256), dim3(256), 0, stream, g, d_Qp, npair);
+    const int npair = g.npair;
+    if (g.nsh <= 0 || npair <= 0 || g.natom <= 0 || 3 * g.natom * 8 > 48 * 1024) return -1;
+    hipLaunchKernelGGL(k_grad_schwarz, dim3((npair + 255) / 256), dim3(256), 0, stream, g, d_Qp);
     if (hipGetLastError() != hipSuccess) return -2;
     GradArgs gq = g;
     gq.Qp = d_Qp;
@@ -367,13 +375,9 @@ int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStre
 
 int polyeri_grad_host(const GradArgs& g, double* grad) {
     const RysTabs rt = host_tabs();
-    const int npair = g.nsh * (g.nsh + 1) / 2;
+    const int npair = g.npair;
     std::vector<double> Qp(npair);
-    for (int t = 0; t < npair; ++t) {
-        int a, b;
-        tri_index((unsigned long long)t, a, b);
-        Qp[t] = pair_bound(g, rt, a, b);
-    }
+    for (int t = 0; t < npair; ++t) Qp[t] = pair_bound(g, rt, g.pa[t], g.pb[t]);
     GradArgs gq = g;
     gq.Qp = Qp.data();
     for (int i = 0; i < 3 * g.natom; ++i) grad[i] = 0.0;

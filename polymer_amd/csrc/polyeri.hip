@@ -3785,8 +3785,8 @@ int polyeri_set_direct(polyeri_ctx* ctx, int direct) {
 // src/pfourindexgradient.f90:1-1478, molecule): kgrad.hip
 // ---------------------------------------------------------------------------
 struct GradHost {
-    std::vector<int> L, atom, f0, p0, np;
-    std::vector<double> xyz, alpha, coef, D;
+    std::vector<int> L, atom, f0, p0, np, pa, pb;
+    std::vector<double> xyz, alpha, coef, D, dsh;
 };
 
 // Shells, primitives and the density in the library's basis:
@@ -3815,6 +3815,29 @@ static int grad_host_arrays(const IntBasis& ib, const RefBasis& rb, const double
         for (int nu = 0; nu < nbf; ++nu)
             h.D[(size_t)mu * nbf + nu] =
                 ib.scale[mu] * ib.scale[nu] * p_c[ib.ref_of_int[mu] + (size_t)nc * ib.ref_of_int[nu]];
+    const int nsh = (int)ib.shells.size();
+    h.dsh.assign((size_t)nsh * nsh, 0.0);
+    for (int a = 0; a < nsh; ++a)
+        for (int b = 0; b < nsh; ++b) {
+            double m = 0.0;
+            for (int p = 0; p < ncart(ib.shells[a].L); ++p)
+                for (int q = 0; q < ncart(ib.shells[b].L); ++q)
+                    m = std::max(m, std::fabs(h.D[(size_t)(ib.shells[a].f0 + p) * nbf + ib.shells[b].f0 + q]));
+            h.dsh[(size_t)a * nsh + b] = m;
+        }
+    // shell pairs a >= b by class, then primitive count (the quartet order)
+    std::vector<std::pair<int, int>> pr;
+    for (int a = 0; a < nsh; ++a)
+        for (int b = 0; b <= a; ++b) pr.push_back({a, b});
+    auto key = [&](const std::pair<int, int>& p) {
+        const int la = h.L[p.first], lb = h.L[p.second];
+        return std::make_tuple(std::max(la, lb), std::min(la, lb), h.np[p.first] * h.np[p.second]);
+    };
+    std::stable_sort(pr.begin(), pr.end(), [&](const auto& x, const auto& y) { return key(x) > key(y); });
+    for (const auto& p : pr) {
+        h.pa.push_back(p.first);
+        h.pb.push_back(p.second);
+    }
     return 0;
 }
 
@@ -3832,15 +3855,18 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
     GradHost h;
     if ((rc = grad_host_arrays(ctx->ib, ctx->rb, p_c, h))) return rc;
     const int nsh = (int)h.L.size(), natom = ctx->rb.natom, nbf = ctx->ib.nbf;
-    const size_t npair = (size_t)nsh * (nsh + 1) / 2;
+    const size_t npair = h.pa.size();
     DevBuf buf;
-    int *dL, *dAt, *dF0, *dP0, *dNp;
-    double *dXyz, *dAl, *dCo, *dD, *dQp, *dG;
+    int *dL, *dAt, *dF0, *dP0, *dNp, *dPa, *dPb;
+    double *dXyz, *dAl, *dCo, *dD, *dDsh, *dQp, *dG;
     if ((rc = buf.get(&dL, nsh)) || (rc = buf.get(&dAt, nsh)) || (rc = buf.get(&dF0, nsh)) ||
         (rc = buf.get(&dP0, nsh)) || (rc = buf.get(&dNp, nsh)) || (rc = buf.get(&dXyz, 3 * (size_t)nsh)) ||
         (rc = buf.get(&dAl, h.alpha.size())) || (rc = buf.get(&dCo, h.coef.size())) ||
-        (rc = buf.get(&dD, h.D.size())) || (rc = buf.get(&dQp, npair)) || (rc = buf.get(&dG, 3 * (size_t)natom)))
+        (rc = buf.get(&dD, h.D.size())) || (rc = buf.get(&dDsh, h.dsh.size())) || (rc = buf.get(&dQp, npair)) ||
+        (rc = buf.get(&dG, 3 * (size_t)natom)) || (rc = buf.get(&dPa, npair)) || (rc = buf.get(&dPb, npair)))
         return rc;
+    HIPCHK(hipMemcpyAsync(dPa, h.pa.data(), npair * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dPb, h.pb.data(), npair * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     hipStream_t s = ctx->stream;
     HIPCHK(hipMemcpyAsync(dL, h.L.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dAt, h.atom.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
@@ -3851,8 +3877,10 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
     HIPCHK(hipMemcpyAsync(dAl, h.alpha.data(), h.alpha.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dCo, h.coef.data(), h.coef.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dD, h.D.data(), h.D.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dDsh, h.dsh.data(), h.dsh.size() * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * (size_t)natom)), dim3(256), 0, s, dG, 3 * (size_t)natom, 0.0);
-    GradArgs g{nsh, natom, nbf, dL, dAt, dF0, dP0, dNp, dXyz, dAl, dCo, dD, nullptr, ctx->opts.schwarz_tol, hfexchange};
+    GradArgs g{nsh, natom, nbf, (int)npair, dL, dAt, dF0, dP0, dNp, dPa, dPb, dXyz, dAl, dCo, dD, dDsh, nullptr,
+               ctx->opts.schwarz_tol, hfexchange, ctx->opts.prim_tol};
     if (polyeri_grad_launch(g, dQp, dG, s)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
     HIPCHK(hipMemcpyAsync(grad, dG, 3 * (size_t)natom * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -3885,7 +3913,8 @@ int polyeri_eri_gradient_host(int npgs, int ncgs, const int* pax, const int* pay
     if (extract_shells(rb, ib, err)) return fail(POLYERI_E_BASIS, err);
     GradHost h;
     if (int rc = grad_host_arrays(ib, rb, p_c, h)) return rc;
-    GradArgs g{(int)h.L.size(), natom, ib.nbf, h.L.data(), h.atom.data(), h.f0.data(), h.p0.data(), h.np.data(),
-               h.xyz.data(), h.alpha.data(), h.coef.data(), h.D.data(), nullptr, schwarz_tol, hfexchange};
+    GradArgs g{(int)h.L.size(), natom, ib.nbf, (int)h.pa.size(), h.L.data(), h.atom.data(), h.f0.data(),
+               h.p0.data(), h.np.data(), h.pa.data(), h.pb.data(), h.xyz.data(), h.alpha.data(), h.coef.data(),
+               h.D.data(), h.dsh.data(), nullptr, schwarz_tol, hfexchange, 1e-20};
     return polyeri_grad_host(g, grad) ? fail(POLYERI_E_ARG, "polyeri_eri_gradient_host failed") : 0;
 }
