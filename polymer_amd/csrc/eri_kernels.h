@@ -1280,11 +1280,13 @@ int polyeri_rys_block_host(int la, int lb, int lc, int ld, const double* xyz, co
 // library's basis (nbf x nbf) and its largest |element| per shell block
 // (nsh x nsh), the per-pair Schwarz bounds (filled by the launcher), the
 // threshold on Q_ab Q_cd and on Q_ab Q_cd max |Gamma|, the exchange scale.
-// Shell pairs (pa[i], pb[i]), pa >= pb, in the order the quartets run
-// (sorted by class and primitive count, so that a wavefront's lanes, which
-// share a bra pair and take consecutive ket pairs, run alike).
+// Shell pairs (pa[i], pb[i]) with L(pa) >= L(pb), grouped by pair class
+// ((ss) (ps) (pp) (ds) (dp) (dd) = 0 .. 5: pairs seg[X] .. seg[X + 1] - 1),
+// by primitive count inside a class; the quartets of class pair X >= Y run
+// as one launch of the kernel for that class.
 struct GradArgs {
     int nsh, natom, nbf, npair;
+    int seg[7];
     const int *L, *atom, *f0, *p0, *np;
     const int *pa, *pb;
     const double *xyz, *alpha, *coef;

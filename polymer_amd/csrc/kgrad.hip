@@ -193,21 +193,97 @@ POLY_HD double pair_bound(const GradArgs& g, const RysTabs& rt, int a, int b) {
     return std::sqrt(m);
 }
 
-// Gradient contributions of one unique contracted quartet to centres A, B, C
-POLY_HD void grad_quartet(const GradArgs& g, const RysTabs& rt, int a, int b, int c, int d, double fdeg,
-                          double (&gA)[3], double (&gB)[3], double (&gC)[3]) {
+// One-dimensional factors of one axis for one root with compile-time
+// ranges: I[i][j][k][l], i + j <= NA, k + l <= NC, i <= LA + 1, j <= LB + 1,
+// k <= LC + 1, l <= LD.
+template <int LA, int LB, int LC, int LD>
+POLY_HD void axis_table_t(double PA, double QC, double PQ, double AB, double CD, double u, double ze, double et,
+                          double rho, double (&I)[LA + 2][LB + 2][LC + 2][LD + 1]) {
+    constexpr int NA = LA + LB + 1, NC = LC + LD + 1;
+    double G[NA + 1][NC + 1];
+    const double c00 = PA - rho / ze * PQ * u, d00 = QC + rho / et * PQ * u;
+    const double b10 = 0.5 / ze * (1.0 - rho * u / ze), b01 = 0.5 / et * (1.0 - rho * u / et);
+    const double b00 = 0.5 * u / (ze + et);
+    G[0][0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) G[i + 1][0] = c00 * G[i][0] + (i ? i * b10 * G[i - 1][0] : 0.0);
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int i = 0; i <= NA; ++i)
+            G[i][k + 1] = d00 * G[i][k] + (k ? k * b01 * G[i][k - 1] : 0.0) + (i ? i * b00 * G[i - 1][k] : 0.0);
+    double H[NA + 1][LC + 2][LD + 1];
+#pragma unroll
+    for (int i = 0; i <= NA; ++i)
+#pragma unroll
+        for (int k = 0; k <= LC + 1; ++k)
+#pragma unroll
+            for (int l = 0; l <= LD; ++l) {
+                if (k + l > NC) continue;
+                double sum = 0.0, pw = 1.0;
+#pragma unroll
+                for (int t = l; t >= 0; --t) {
+                    sum += binom(l, t) * pw * G[i][k + t];
+                    pw *= CD;
+                }
+                H[i][k][l] = sum;
+            }
+#pragma unroll
+    for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB + 1; ++j) {
+            if (i + j > NA) continue;
+#pragma unroll
+            for (int k = 0; k <= LC + 1; ++k)
+#pragma unroll
+                for (int l = 0; l <= LD; ++l) {
+                    if (k + l > NC) continue;
+                    double sum = 0.0, pw = 1.0;
+#pragma unroll
+                    for (int t = j; t >= 0; --t) {
+                        sum += binom(j, t) * pw * H[i + t][k][l];
+                        pw *= AB;
+                    }
+                    I[i][j][k][l] = sum;
+                }
+        }
+}
+
+// Gradient contributions of one unique contracted quartet, shells a, b, c, d
+// of L = LA, LB, LC, LD, to centres A, B, C (D's by invariance).  The
+// quartet's Gamma block is formed once, then the primitive quartets and
+// roots run over it.
+template <int LA, int LB, int LC, int LD>
+POLY_HD void grad_quartet_t(const GradArgs& g, const RysTabs& rt, int a, int b, int c, int d, double fdeg,
+                            double (&gA)[3], double (&gB)[3], double (&gC)[3]) {
+    constexpr int CA = ncart(LA), CB = ncart(LB), CC = ncart(LC), CD_ = ncart(LD);
+    constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
     const Sh A = shell_of(g, a), B = shell_of(g, b), C = shell_of(g, c), D = shell_of(g, d);
-    const int na = A.L + B.L + 1, nc = C.L + D.L + 1;
-    const int n = (A.L + B.L + C.L + D.L + 1) / 2 + 1;
-    const int ca = ncart(A.L), cb = ncart(B.L), cc = ncart(C.L), cd = ncart(D.L);
-    const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z}, CD[3] = {C.x - D.x, C.y - D.y, C.z - D.z};
+    const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z}, CDv[3] = {C.x - D.x, C.y - D.y, C.z - D.z};
     const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-    const double cd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
+    const double cd2 = CDv[0] * CDv[0] + CDv[1] * CDv[1] + CDv[2] * CDv[2];
     const double Ac[3] = {A.x, A.y, A.z}, Cc[3] = {C.x, C.y, C.z};
     const double* Dm = g.D;
-    const int nbf = g.nbf;
+    const size_t nbf = (size_t)g.nbf;
+    double gam[CA][CB][CC][CD_];
+#pragma unroll
+    for (int p = 0; p < CA; ++p)
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const double dab = Dm[(A.f0 + p) * nbf + B.f0 + q];
+#pragma unroll
+            for (int s2 = 0; s2 < CC; ++s2) {
+                const double dac = Dm[(A.f0 + p) * nbf + C.f0 + s2], dbc = Dm[(B.f0 + q) * nbf + C.f0 + s2];
+#pragma unroll
+                for (int t = 0; t < CD_; ++t) {
+                    const double dcd = Dm[(C.f0 + s2) * nbf + D.f0 + t], dbd = Dm[(B.f0 + q) * nbf + D.f0 + t];
+                    const double dad = Dm[(A.f0 + p) * nbf + D.f0 + t];
+                    gam[p][q][s2][t] = fdeg * (4.0 * dab * dcd - g.xfac * (dac * dbd + dad * dbc));
+                }
+            }
+        }
     for (int e = 0; e < 3; ++e) gA[e] = gB[e] = gC[e] = 0.0;
-    double I3[3][IMAX][IMAX][IMAX][LMAXD];
+    double I3[3][LA + 2][LB + 2][LC + 2][LD + 1];
     for (int i = 0; i < A.np; ++i)
         for (int j = 0; j < B.np; ++j) {
             const double al = g.alpha[A.p0 + i], be = g.alpha[B.p0 + j];
@@ -226,36 +302,35 @@ POLY_HD void grad_quartet(const GradArgs& g, const RysTabs& rt, int a, int b, in
                     const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
                     const double rho = ze * et / (ze + et);
                     const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-                    const double pref =
-                        2.0 * POLY_PI * POLY_PI * POLY_SQRT_PI / (ze * et * std::sqrt(ze + et)) * cab * ccd * fdeg;
-                    double u[RYS_NMAX], w[RYS_NMAX];
-                    rys_nodes_n(n, T, rt, u, w);
-                    for (int r = 0; r < n; ++r) {
+                    const double pref = 2.0 * POLY_PI * POLY_PI * POLY_SQRT_PI / (ze * et * std::sqrt(ze + et)) * cab * ccd;
+                    double u[NR], w[NR];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) rys_node<NR>(T, r, rt.tab[NR], rt.herm[NR], u[r], w[r]);
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+#pragma unroll
                         for (int ax = 0; ax < 3; ++ax)
-                            axis_table(P[ax] - Ac[ax], Q[ax] - Cc[ax], PQ[ax], AB[ax], CD[ax], u[r], ze, et, rho, na,
-                                       nc, A.L + 1, B.L + 1, C.L + 1, D.L, I3[ax]);
+                            axis_table_t<LA, LB, LC, LD>(P[ax] - Ac[ax], Q[ax] - Cc[ax], PQ[ax], AB[ax], CDv[ax], u[r],
+                                                         ze, et, rho, I3[ax]);
                         const double wr = w[r] * pref;
-                        for (int p = 0; p < ca; ++p) {
+#pragma unroll
+                        for (int p = 0; p < CA; ++p) {
                             int a3[3];
-                            comp_of(A.L, p, a3[0], a3[1], a3[2]);
-                            for (int q = 0; q < cb; ++q) {
+                            comp_of(LA, p, a3[0], a3[1], a3[2]);
+#pragma unroll
+                            for (int q = 0; q < CB; ++q) {
                                 int b3[3];
-                                comp_of(B.L, q, b3[0], b3[1], b3[2]);
-                                const double dab = Dm[(size_t)(A.f0 + p) * nbf + B.f0 + q];
-                                for (int s = 0; s < cc; ++s) {
+                                comp_of(LB, q, b3[0], b3[1], b3[2]);
+#pragma unroll
+                                for (int s2 = 0; s2 < CC; ++s2) {
                                     int c3[3];
-                                    comp_of(C.L, s, c3[0], c3[1], c3[2]);
-                                    const double dac = Dm[(size_t)(A.f0 + p) * nbf + C.f0 + s];
-                                    const double dbc = Dm[(size_t)(B.f0 + q) * nbf + C.f0 + s];
-                                    for (int t = 0; t < cd; ++t) {
-                                        const double dcd = Dm[(size_t)(C.f0 + s) * nbf + D.f0 + t];
-                                        const double dbd = Dm[(size_t)(B.f0 + q) * nbf + D.f0 + t];
-                                        const double dad = Dm[(size_t)(A.f0 + p) * nbf + D.f0 + t];
-                                        const double gam = 4.0 * dab * dcd - g.xfac * (dac * dbd + dad * dbc);
-                                        if (gam == 0.0) continue;
+                                    comp_of(LC, s2, c3[0], c3[1], c3[2]);
+#pragma unroll
+                                    for (int t = 0; t < CD_; ++t) {
                                         int d3[3];
-                                        comp_of(D.L, t, d3[0], d3[1], d3[2]);
+                                        comp_of(LD, t, d3[0], d3[1], d3[2]);
                                         double v[3], dA[3], dB[3], dC[3];
+#pragma unroll
                                         for (int e = 0; e < 3; ++e) {
                                             const auto& Ie = I3[e];
                                             const int x1 = a3[e], x2 = b3[e], x3 = c3[e], x4 = d3[e];
@@ -264,7 +339,7 @@ POLY_HD void grad_quartet(const GradArgs& g, const RysTabs& rt, int a, int b, in
                                             dB[e] = 2.0 * be * Ie[x1][x2 + 1][x3][x4] - (x2 ? x2 * Ie[x1][x2 - 1][x3][x4] : 0.0);
                                             dC[e] = 2.0 * ga * Ie[x1][x2][x3 + 1][x4] - (x3 ? x3 * Ie[x1][x2][x3 - 1][x4] : 0.0);
                                         }
-                                        const double f = wr * gam;
+                                        const double f = wr * gam[p][q][s2][t];
                                         gA[0] += f * dA[0] * v[1] * v[2];
                                         gA[1] += f * v[0] * dA[1] * v[2];
                                         gA[2] += f * v[0] * v[1] * dA[2];
@@ -292,12 +367,10 @@ POLY_HD void tri_index(unsigned long long t, int& r, int& c) {
     c = (int)(t - (unsigned long long)(rr * (rr + 1) / 2));
 }
 
-// Contribution of quartet index t (over unique pair pairs) to the per-atom
-// gradient through add(atom, axis, value); false when screened out.
-template <typename Add>
-POLY_HD void quartet_task(const GradArgs& g, const RysTabs& rt, unsigned long long t, Add add) {
-    int pij, pkl;
-    tri_index(t, pij, pkl);
+// Contribution of the quartet of sorted pairs pij (class LA LB) and pkl
+// (class LC LD) to the per-atom gradient through add(atom, axis, value).
+template <int LA, int LB, int LC, int LD, typename Add>
+POLY_HD void quartet_cls(const GradArgs& g, const RysTabs& rt, int pij, int pkl, Add add) {
     const double qq = g.Qp[pij] * g.Qp[pkl];
     if (qq < g.tol) return;
     const int a = g.pa[pij], b = g.pb[pij], c = g.pa[pkl], d = g.pb[pkl];
@@ -308,13 +381,24 @@ POLY_HD void quartet_task(const GradArgs& g, const RysTabs& rt, unsigned long lo
     if (qq * gmax < g.tol) return;
     const double fdeg = (a == b ? 0.5 : 1.0) * (c == d ? 0.5 : 1.0) * (pij == pkl ? 0.5 : 1.0);
     double gA[3], gB[3], gC[3];
-    grad_quartet(g, rt, a, b, c, d, fdeg, gA, gB, gC);
+    grad_quartet_t<LA, LB, LC, LD>(g, rt, a, b, c, d, fdeg, gA, gB, gC);
     const int at[4] = {g.atom[a], g.atom[b], g.atom[c], g.atom[d]};
     for (int e = 0; e < 3; ++e) {
         add(at[0], e, gA[e]);
         add(at[1], e, gB[e]);
         add(at[2], e, gC[e]);
         add(at[3], e, -(gA[e] + gB[e] + gC[e]));
+    }
+}
+
+// quartet t of the class pair: (x0 + i, y0 + j), j <= i when the classes
+// are equal (lower triangle), else i = t / ny, j = t % ny
+POLY_HD void class_pair_index(unsigned long long t, int same, int ny, int& i, int& j) {
+    if (same) {
+        tri_index(t, i, j);
+    } else {
+        i = (int)(t / (unsigned long long)ny);
+        j = (int)(t % (unsigned long long)ny);
     }
 }
 
@@ -342,20 +426,76 @@ __global__ void __launch_bounds__(256) k_grad_schwarz(GradArgs g, double* Qp) {
 
 // per-atom sums in LDS (3 natom doubles of dynamic shared memory), one
 // global atomic per element per block
-__global__ void __launch_bounds__(256) k_eri_grad(GradArgs g, unsigned long long nq, double* __restrict__ grad) {
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) k_eri_grad(GradArgs g, int x0, int y0, int ny, int same, unsigned long long nq,
+                                                  double* __restrict__ grad) {
     extern __shared__ double sg[];
     for (int i = threadIdx.x; i < 3 * g.natom; i += blockDim.x) sg[i] = 0.0;
     __syncthreads();
     const RysTabs rt = device_tabs();
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
-         t += (unsigned long long)gridDim.x * blockDim.x)
-        quartet_task(g, rt, t, [&](int at, int e, double v) {
+         t += (unsigned long long)gridDim.x * blockDim.x) {
+        int i, j;
+        class_pair_index(t, same, ny, i, j);
+        quartet_cls<LA, LB, LC, LD>(g, rt, x0 + i, y0 + j, [&](int at, int e, double v) {
             if (v != 0.0) atomicAdd(&sg[3 * at + e], v);
         });
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < 3 * g.natom; i += blockDim.x)
         if (sg[i] != 0.0) atomicAdd(&grad[i], sg[i]);
 }
+
+// Calls f.template operator()<LA, LB, LC, LD>() for the classes of pair
+// classes X >= Y ((ss) (ps) (pp) (ds) (dp) (dd) = 0 .. 5).
+template <typename F>
+void for_class_pair(int X, int Y, F&& f) {
+#define GRAD_CASE(x, y, la, lb, lc, ld) \
+    case x * 6 + y: f.template operator()<la, lb, lc, ld>(); break;
+    switch (X * 6 + Y) {
+        GRAD_CASE(0, 0, 0, 0, 0, 0)
+        GRAD_CASE(1, 0, 1, 0, 0, 0) GRAD_CASE(1, 1, 1, 0, 1, 0)
+        GRAD_CASE(2, 0, 1, 1, 0, 0) GRAD_CASE(2, 1, 1, 1, 1, 0) GRAD_CASE(2, 2, 1, 1, 1, 1)
+        GRAD_CASE(3, 0, 2, 0, 0, 0) GRAD_CASE(3, 1, 2, 0, 1, 0) GRAD_CASE(3, 2, 2, 0, 1, 1) GRAD_CASE(3, 3, 2, 0, 2, 0)
+        GRAD_CASE(4, 0, 2, 1, 0, 0) GRAD_CASE(4, 1, 2, 1, 1, 0) GRAD_CASE(4, 2, 2, 1, 1, 1) GRAD_CASE(4, 3, 2, 1, 2, 0)
+        GRAD_CASE(4, 4, 2, 1, 2, 1)
+        GRAD_CASE(5, 0, 2, 2, 0, 0) GRAD_CASE(5, 1, 2, 2, 1, 0) GRAD_CASE(5, 2, 2, 2, 1, 1) GRAD_CASE(5, 3, 2, 2, 2, 0)
+        GRAD_CASE(5, 4, 2, 2, 2, 1) GRAD_CASE(5, 5, 2, 2, 2, 2)
+        default: break;
+    }
+#undef GRAD_CASE
+}
+
+// one class pair's launch (device) or loop (host)
+struct GradLaunch {
+    const GradArgs& gq;
+    int x0, y0, ny, same;
+    unsigned long long nq;
+    unsigned grid;
+    size_t lds;
+    double* d;
+    hipStream_t s;
+    template <int A, int B, int C, int D>
+    void operator()() const {
+        hipLaunchKernelGGL((k_eri_grad<A, B, C, D>), dim3(grid), dim3(256), lds, s, gq, x0, y0, ny, same, nq, d);
+    }
+};
+
+struct GradHostRun {
+    const GradArgs& gq;
+    const RysTabs& rt;
+    int x0, y0, ny, same;
+    unsigned long long nq;
+    double* grad;
+    template <int A, int B, int C, int D>
+    void operator()() const {
+        for (unsigned long long t = 0; t < nq; ++t) {
+            int i, j;
+            class_pair_index(t, same, ny, i, j);
+            quartet_cls<A, B, C, D>(gq, rt, x0 + i, y0 + j, [&](int at, int e, double v) { grad[3 * at + e] += v; });
+        }
+    }
+};
 
 }  // namespace
 
@@ -366,11 +506,21 @@ int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStre
     if (hipGetLastError() != hipSuccess) return -2;
     GradArgs gq = g;
     gq.Qp = d_Qp;
-    const unsigned long long nq = (unsigned long long)npair * (npair + 1) / 2;
-    const unsigned long long want = (nq + 255) / 256;
-    const unsigned grid = (unsigned)(want < 16384ULL ? want : 16384ULL);
-    hipLaunchKernelGGL(k_eri_grad, dim3(grid), dim3(256), 3 * g.natom * sizeof(double), stream, gq, nq, d_grad);
-    return hipGetLastError() == hipSuccess ? 0 : -2;
+    for (int X = 0; X < 6; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const int nx = g.seg[X + 1] - g.seg[X], ny = g.seg[Y + 1] - g.seg[Y];
+            if (nx <= 0 || ny <= 0) continue;
+            const int same = X == Y;
+            const unsigned long long nq =
+                same ? (unsigned long long)nx * (nx + 1) / 2 : (unsigned long long)nx * (unsigned long long)ny;
+            const unsigned long long want = (nq + 255) / 256;
+            const unsigned grid = (unsigned)(want < 16384ULL ? want : 16384ULL);
+            const size_t lds = 3 * (size_t)g.natom * sizeof(double);
+            GradLaunch launch{gq, g.seg[X], g.seg[Y], ny, same, nq, grid, lds, d_grad, stream};
+            for_class_pair(X, Y, launch);
+            if (hipGetLastError() != hipSuccess) return -2;
+        }
+    return 0;
 }
 
 int polyeri_grad_host(const GradArgs& g, double* grad) {
@@ -381,8 +531,15 @@ int polyeri_grad_host(const GradArgs& g, double* grad) {
     GradArgs gq = g;
     gq.Qp = Qp.data();
     for (int i = 0; i < 3 * g.natom; ++i) grad[i] = 0.0;
-    const unsigned long long nq = (unsigned long long)npair * (npair + 1) / 2;
-    for (unsigned long long t = 0; t < nq; ++t)
-        quartet_task(gq, rt, t, [&](int at, int e, double v) { grad[3 * at + e] += v; });
+    for (int X = 0; X < 6; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const int nx = g.seg[X + 1] - g.seg[X], ny = g.seg[Y + 1] - g.seg[Y];
+            if (nx <= 0 || ny <= 0) continue;
+            const int same = X == Y;
+            const unsigned long long nq =
+                same ? (unsigned long long)nx * (nx + 1) / 2 : (unsigned long long)nx * (unsigned long long)ny;
+            GradHostRun run{gq, rt, g.seg[X], g.seg[Y], ny, same, nq, grad};
+            for_class_pair(X, Y, run);
+        }
     return 0;
 }

@@ -3787,6 +3787,7 @@ int polyeri_set_direct(polyeri_ctx* ctx, int direct) {
 struct GradHost {
     std::vector<int> L, atom, f0, p0, np, pa, pb;
     std::vector<double> xyz, alpha, coef, D, dsh;
+    int seg[NPAIRCLASS + 1];
 };
 
 // Shells, primitives and the density in the library's basis:
@@ -3825,19 +3826,24 @@ static int grad_host_arrays(const IntBasis& ib, const RefBasis& rb, const double
                     m = std::max(m, std::fabs(h.D[(size_t)(ib.shells[a].f0 + p) * nbf + ib.shells[b].f0 + q]));
             h.dsh[(size_t)a * nsh + b] = m;
         }
-    // shell pairs a >= b by class, then primitive count (the quartet order)
+    // every unordered shell pair once, higher L first, grouped by pair class
+    // (ascending) and by primitive count inside a class
     std::vector<std::pair<int, int>> pr;
     for (int a = 0; a < nsh; ++a)
-        for (int b = 0; b <= a; ++b) pr.push_back({a, b});
-    auto key = [&](const std::pair<int, int>& p) {
-        const int la = h.L[p.first], lb = h.L[p.second];
-        return std::make_tuple(std::max(la, lb), std::min(la, lb), h.np[p.first] * h.np[p.second]);
-    };
-    std::stable_sort(pr.begin(), pr.end(), [&](const auto& x, const auto& y) { return key(x) > key(y); });
+        for (int b = 0; b <= a; ++b)
+            pr.push_back(h.L[b] >= h.L[a] ? std::make_pair(b, a) : std::make_pair(a, b));
+    auto cls = [&](const std::pair<int, int>& p) { return pair_class(h.L[p.first], h.L[p.second]); };
+    std::stable_sort(pr.begin(), pr.end(), [&](const auto& x, const auto& y) {
+        if (cls(x) != cls(y)) return cls(x) < cls(y);
+        return h.np[x.first] * h.np[x.second] > h.np[y.first] * h.np[y.second];
+    });
+    for (int X = 0; X <= NPAIRCLASS; ++X) h.seg[X] = 0;
     for (const auto& p : pr) {
         h.pa.push_back(p.first);
         h.pb.push_back(p.second);
+        ++h.seg[cls(p) + 1];
     }
+    for (int X = 0; X < NPAIRCLASS; ++X) h.seg[X + 1] += h.seg[X];
     return 0;
 }
 
@@ -3879,8 +3885,9 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
     HIPCHK(hipMemcpyAsync(dD, h.D.data(), h.D.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dDsh, h.dsh.data(), h.dsh.size() * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * (size_t)natom)), dim3(256), 0, s, dG, 3 * (size_t)natom, 0.0);
-    GradArgs g{nsh, natom, nbf, (int)npair, dL, dAt, dF0, dP0, dNp, dPa, dPb, dXyz, dAl, dCo, dD, dDsh, nullptr,
-               ctx->opts.schwarz_tol, hfexchange, ctx->opts.prim_tol};
+    GradArgs g{nsh, natom, nbf, (int)npair, {0}, dL, dAt, dF0, dP0, dNp, dPa, dPb, dXyz, dAl, dCo, dD, dDsh,
+               nullptr, ctx->opts.schwarz_tol, hfexchange, ctx->opts.prim_tol};
+    std::copy(h.seg, h.seg + NPAIRCLASS + 1, g.seg);
     if (polyeri_grad_launch(g, dQp, dG, s)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
     HIPCHK(hipMemcpyAsync(grad, dG, 3 * (size_t)natom * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -3913,8 +3920,9 @@ int polyeri_eri_gradient_host(int npgs, int ncgs, const int* pax, const int* pay
     if (extract_shells(rb, ib, err)) return fail(POLYERI_E_BASIS, err);
     GradHost h;
     if (int rc = grad_host_arrays(ib, rb, p_c, h)) return rc;
-    GradArgs g{(int)h.L.size(), natom, ib.nbf, (int)h.pa.size(), h.L.data(), h.atom.data(), h.f0.data(),
+    GradArgs g{(int)h.L.size(), natom, ib.nbf, (int)h.pa.size(), {0}, h.L.data(), h.atom.data(), h.f0.data(),
                h.p0.data(), h.np.data(), h.pa.data(), h.pb.data(), h.xyz.data(), h.alpha.data(), h.coef.data(),
                h.D.data(), h.dsh.data(), nullptr, schwarz_tol, hfexchange, 1e-20};
+    std::copy(h.seg, h.seg + NPAIRCLASS + 1, g.seg);
     return polyeri_grad_host(g, grad) ? fail(POLYERI_E_ARG, "polyeri_eri_gradient_host failed") : 0;
 }
