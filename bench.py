@@ -318,6 +318,20 @@ def main():
 
     mp2 = None if args.no_mp2 else mp2_line(args.mp2_nwater, args.seed, args.mp2_transforms.split(","))
 
+    # two-electron nuclear gradient at the converged FP64 density of the workload
+    grad = None
+    if not args.no_scf:
+        print("bench: ERI gradient", file=sys.stderr, flush=True)
+        rg = ctx.self_consistent_field(scfconv=1e-8, diis=2, scfcycles=80, solver=1, diis_mode=1)
+        ctx.eri_gradient(rg["P"])                                   # warm
+        t0 = time.perf_counter()
+        gv = ctx.eri_gradient(rg["P"])
+        gs = time.perf_counter() - t0
+        grad = {"workload": "dE2/dR of the converged (H2O)16 cc-pVDZ FP64 density (FOURINDEX_ERI_GRADIENT), "
+                            "Schwarz 1e-12 on Q_ab Q_cd and on Q_ab Q_cd max|Gamma|",
+                "seconds": gs, "fock_builds_equivalent": gs / s_iter, "max_abs_Eh_per_bohr": float(np.max(np.abs(gv))),
+                "translational_drift": float(np.max(np.abs(gv.sum(axis=0))))}
+
     # HBM bytes per launch of the dominant kernel from the committed counter
     # summary of this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
     # scaled by the copy kernel; counters cannot be read inside this process)
@@ -370,6 +384,7 @@ def main():
         "mixed_classes": class_table(mst, mx_cls),
         "scf": scf,
         "mp2": mp2,
+        "eri_gradient": grad,
         "checksum_abs_F": fsum,
     }
     if not args.no_cpu:
