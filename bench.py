@@ -327,10 +327,15 @@ def main():
         t0 = time.perf_counter()
         gv = ctx.eri_gradient(rg["P"])
         gs = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gt = ctx.scf_gradient(rg)
+        ts = time.perf_counter() - t0
         grad = {"workload": "dE2/dR of the converged (H2O)16 cc-pVDZ FP64 density (FOURINDEX_ERI_GRADIENT), "
                             "Schwarz 1e-12 on Q_ab Q_cd and on Q_ab Q_cd max|Gamma|",
                 "seconds": gs, "fock_builds_equivalent": gs / s_iter, "max_abs_Eh_per_bohr": float(np.max(np.abs(gv))),
-                "translational_drift": float(np.max(np.abs(gv.sum(axis=0))))}
+                "translational_drift": float(np.max(np.abs(gv.sum(axis=0)))),
+                "scf_gradient_seconds": ts, "scf_gradient_max_abs": float(np.max(np.abs(gt))),
+                "scf_gradient_translational_drift": float(np.max(np.abs(gt.sum(axis=0))))}
 
     # HBM bytes per launch of the dominant kernel from the committed counter
     # summary of this workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs

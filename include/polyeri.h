@@ -192,6 +192,23 @@ int polyeri_set_direct(polyeri_ctx *ctx, int direct);
  * DOPTN(19)), in Eh/bohr.  Quartets below the context's Schwarz threshold
  * are skipped; molecular contexts, at most 2048 atoms. */
 int polyeri_eri_gradient(polyeri_ctx *ctx, const double *p_c, double hfexchange, double *grad);
+/* One-electron and nuclear parts of the nuclear gradient, in Eh/bohr into
+ * grad[3*atom + axis]: sum P_C (dT + dV) - W_C dS over the derivatives of the
+ * basis functions (OVERLAP_KINETIC(1), NUCLEAR_ATTRACTION(1),
+ * src/ponee.f90:157-214 and 418-), the derivative of the attraction
+ * operator at each nucleus (HELLMANN_FEYNMAN_ATTRACTION,
+ * src/phellmannfeynman.f90:86-) and the nuclear repulsion
+ * (HELLMANN_FEYNMAN_REPULSION, src/phellmannfeynman.f90:1-85); w_c is the
+ * energy-weighted density W_C = sum_l f_l e_l C_l C_l^T
+ * (src/pfock.f90:478-497).  Host code (a few ms); molecular contexts.
+ * polyeri_one_electron_gradient + polyeri_eri_gradient = dE_SCF/dR. */
+int polyeri_one_electron_gradient(polyeri_ctx *ctx, const double *p_c, const double *w_c, double *grad);
+/* The same from the basis arrays of polyeri_ctx_create, without a context or
+ * a GPU.  No counterpart in the reference. */
+int polyeri_one_electron_gradient_host(int npgs, int ncgs, const int *pax, const int *pay, const int *paz,
+                                       const double *pgsx, const double *pgsy, const double *pgsz, const double *zt,
+                                       const double *cc, int natom, const int *iatom, const double *xyz,
+                                       const double *p_c, const double *w_c, double *grad);
 /* The same sum evaluated on the host by the same code, from the basis
  * arrays of polyeri_ctx_create and a Schwarz threshold, without a context
  * or a GPU (slow; for checks).  No counterpart in the reference. */

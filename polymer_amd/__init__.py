@@ -63,6 +63,33 @@ def eri_gradient_host(basis, P, hfexchange=1.0, schwarz_tol=0.0):
     return g
 
 
+def _basis_args(b):
+    z = np.ascontiguousarray([a[0] for a in b.atoms], dtype=np.int32)
+    xyz = np.ascontiguousarray([[a[1], a[2], a[3]] for a in b.atoms], dtype=np.float64)
+    return z, xyz
+
+
+def energy_weighted_density(co, eps, iocc):
+    """W_C = sum_l f_l e_l C_l C_l^T with f = 2 for the iocc occupied
+    orbitals (src/pfock.f90:478-497 at zero temperature)."""
+    c = np.asarray(co)[:, :iocc]
+    return 2.0 * (c * np.asarray(eps)[:iocc]) @ c.T
+
+
+def one_electron_gradient_host(basis, P, W):
+    """The one-electron and nuclear parts of the gradient of
+    Context.one_electron_gradient, without a context or a GPU."""
+    b = basis
+    z, xyz = _basis_args(b)
+    P = np.asfortranarray(P, dtype=np.float64)
+    W = np.asfortranarray(W, dtype=np.float64)
+    g = np.zeros((len(b.atoms), 3))
+    check(lib().polyeri_one_electron_gradient_host(b.npgs, b.ncgs, _p(b.pax), _p(b.pay), _p(b.paz), _p(b.pgsx),
+                                                   _p(b.pgsy), _p(b.pgsz), _p(b.zt), _p(b.cc), len(b.atoms), _p(z),
+                                                   _p(xyz), _p(P), _p(W), _p(g)))
+    return g
+
+
 def restore_fourindex_eri(path, ncgs):
     """Read a molecular .ao file (RESTORE_FOURINDEX_ERI) into an (ncgs,)*4
     Fortran-order tensor; returns (tensor, number of integrals read)."""
@@ -419,6 +446,24 @@ class Context:
         g = np.zeros((len(self.basis.atoms), 3))
         check(lib().polyeri_eri_gradient(self.h, _p(P), float(hfexchange), _p(g)))
         return g
+
+    def one_electron_gradient(self, P, W):
+        """One-electron and nuclear parts of the nuclear gradient, (natom, 3)
+        in Eh/bohr: sum P (dT + dV) - W dS, the attraction operator's
+        derivative at each nucleus and the nuclear repulsion."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        W = np.asfortranarray(W, dtype=np.float64)
+        g = np.zeros((len(self.basis.atoms), 3))
+        check(lib().polyeri_one_electron_gradient(self.h, _p(P), _p(W), _p(g)))
+        return g
+
+    def scf_gradient(self, result, hfexchange=1.0):
+        """dE_SCF/dR, (natom, 3) in Eh/bohr, of a converged zero-temperature
+        self_consistent_field result: the one-electron part with the
+        energy-weighted density of its orbitals plus the two-electron part
+        (the reference's sequence at src/psinglepoint.f90:358-383)."""
+        W = energy_weighted_density(result["co"], result["eps"], result["iocc"])
+        return self.one_electron_gradient(result["P"], W) + self.eri_gradient(result["P"], hfexchange)
 
     def launch_configs(self):
         """Per work list: the 9-int launch record of polyeri_get_launch_config."""

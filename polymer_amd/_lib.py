@@ -25,7 +25,8 @@ EXPORTS = [
     "polyeri_restore_fourindex_eri_cells", "polyeri_eri_tensor_cells", "polyeri_reduced_cel1",
     "polyeri_load_fourindex_eri", "polyeri_set_direct", "polyeri_get_launch_config", "polyeri_set_launch_config",
     "polyeri_rys_nodes",
-    "polyeri_rys_block", "polyeri_eri_gradient", "polyeri_eri_gradient_host",
+    "polyeri_rys_block", "polyeri_eri_gradient", "polyeri_eri_gradient_host", "polyeri_one_electron_gradient",
+    "polyeri_one_electron_gradient_host",
 ]
 
 
@@ -108,6 +109,8 @@ def lib():
     L.polyeri_load_fourindex_eri.argtypes = [p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_set_direct.argtypes = [p, i]
     L.polyeri_eri_gradient.argtypes = [p, p, d, p]
+    L.polyeri_one_electron_gradient.argtypes = [p, p, p, p]
+    L.polyeri_one_electron_gradient_host.argtypes = [i, i, p, p, p, p, p, p, p, p, i, p, p, p, p, p]
     L.polyeri_eri_gradient_host.argtypes = [i, i, p, p, p, p, p, p, p, p, i, p, p, d, p, d, p]
     L.polyeri_get_launch_config.argtypes = [p, i, p]
     L.polyeri_set_launch_config.argtypes = [p, i, p]

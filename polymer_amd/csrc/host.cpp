@@ -725,6 +725,162 @@ void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vect
     }
 }
 
+// One-electron and nuclear parts of the nuclear gradient at fixed D and W
+// (library basis, nbf x nbf row-major): sum D (dT + dV) - W dS over the
+// basis-function derivatives on both centres (OVERLAP_KINETIC(1),
+// NUCLEAR_ATTRACTION(1), src/ponee.f90:157-214, 418-), the derivative of
+// the attraction operator at each nucleus (HELLMANN_FEYNMAN_ATTRACTION,
+// src/phellmannfeynman.f90:86-), taken through translational invariance of
+// each nucleus' term, -(d/dA + d/dB), and the nuclear repulsion
+// (HELLMANN_FEYNMAN_REPULSION, src/phellmannfeynman.f90:1-85).  The
+// derivative of a primitive Cartesian Gaussian on its centre is
+// 2 alpha [a + 1] - a_x [a - 1].  shell_atom: the atom of each shell.
+void one_electron_gradient(const IntBasis& ib, const std::vector<int>& shell_atom, const std::vector<double>& boys,
+                           const double* D, const double* W, double* grad) {
+    const int n = ib.nbf, ns = (int)ib.shells.size(), natom = (int)ib.natom_z.size();
+    for (int i = 0; i < 3 * natom; ++i) grad[i] = 0.0;
+    std::vector<std::vector<double>> part(ns, std::vector<double>());
+#pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < ns; ++a) {
+        std::vector<double> g(3 * natom, 0.0);
+        std::vector<double> es(kTri.n), ev((size_t)kTri.n * 12), vk((size_t)natom * kTri.n);
+        const Shell& A = ib.shells[a];
+        for (int b = a; b < ns; ++b) {
+            const Shell& B = ib.shells[b];
+            const int la = A.L, lb = B.L, na = ncart(la), nb = ncart(lb);
+            const double ABx = A.x - B.x, ABy = A.y - B.y, ABz = A.z - B.z;
+            const double r2 = ABx * ABx + ABy * ABy + ABz * ABz;
+            const double fac = a == b ? 1.0 : 2.0;
+            const int atA = shell_atom[a], atB = shell_atom[b];
+            for (size_t i = 0; i < A.alpha.size(); ++i)
+                for (size_t j = 0; j < B.alpha.size(); ++j) {
+                    const double al = A.alpha[i], be = B.alpha[j], z = al + be;
+                    const double cc = A.coef[i] * B.coef[j] * std::exp(-al * be / z * r2);
+                    if (std::fabs(cc) < 1e-17) continue;   // overlap factor of a distant pair
+                    const double P[3] = {(al * A.x + be * B.x) / z, (al * A.y + be * B.y) / z,
+                                         (al * A.z + be * B.z) / z};
+                    const double PA[3] = {P[0] - A.x, P[1] - A.y, P[2] - A.z};
+                    const int Le = la + lb + 3;          // +1 derivative, +2 kinetic
+                    es[0] = std::pow(POLY_PI / z, 1.5);
+                    for (int l = 1; l <= Le; ++l)
+                        for (int ix = l; ix >= 0; --ix)
+                            for (int iz = 0; iz <= l - ix; ++iz) {
+                                const int iy = l - ix - iz;
+                                const int d = ix > 0 ? 0 : (iy > 0 ? 1 : 2);
+                                int e1[3] = {ix, iy, iz};
+                                e1[d]--;
+                                double val = PA[d] * es[kTri.idx[e1[0]][e1[1]][e1[2]]];
+                                if (e1[d] > 0) {
+                                    int e2[3] = {e1[0], e1[1], e1[2]};
+                                    e2[d]--;
+                                    val += e1[d] / (2 * z) * es[kTri.idx[e2[0]][e2[1]][e2[2]]];
+                                }
+                                es[kTri.idx[ix][iy][iz]] = val;
+                            }
+                    // attraction ladders of every nucleus, one level above the pair
+                    const int Lv = la + lb + 1;
+                    for (int k = 0; k < natom; ++k) {
+                        const double PC[3] = {P[0] - ib.atom_xyz[3 * k], P[1] - ib.atom_xyz[3 * k + 1],
+                                              P[2] - ib.atom_xyz[3 * k + 2]};
+                        const double TT = z * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                        double F[16];
+                        boys_host(Lv, TT, boys, F);
+                        const double pre = -ib.natom_z[k] * 2.0 * POLY_PI / z;
+                        for (int m = 0; m <= Lv; ++m) ev[(size_t)m * kTri.n] = pre * F[m];
+                        for (int l = 1; l <= Lv; ++l)
+                            for (int ix = l; ix >= 0; --ix)
+                                for (int iz = 0; iz <= l - ix; ++iz) {
+                                    const int iy = l - ix - iz;
+                                    const int d = ix > 0 ? 0 : (iy > 0 ? 1 : 2);
+                                    int e1[3] = {ix, iy, iz};
+                                    e1[d]--;
+                                    const int i1 = kTri.idx[e1[0]][e1[1]][e1[2]];
+                                    int i2 = -1;
+                                    if (e1[d] > 0) {
+                                        int e2[3] = {e1[0], e1[1], e1[2]};
+                                        e2[d]--;
+                                        i2 = kTri.idx[e2[0]][e2[1]][e2[2]];
+                                    }
+                                    const int i0 = kTri.idx[ix][iy][iz];
+                                    for (int m = 0; m <= Lv - l; ++m) {
+                                        double val = PA[d] * ev[(size_t)m * kTri.n + i1] - PC[d] * ev[(size_t)(m + 1) * kTri.n + i1];
+                                        if (i2 >= 0)
+                                            val += e1[d] / (2 * z) * (ev[(size_t)m * kTri.n + i2] - ev[(size_t)(m + 1) * kTri.n + i2]);
+                                        ev[(size_t)m * kTri.n + i0] = val;
+                                    }
+                                }
+                        for (int q = 0; q < kTri.n; ++q) vk[(size_t)k * kTri.n + q] = ev[q];
+                    }
+                    auto S = [&](const int* x, const int* y) {
+                        return hrr2(es.data(), x[0], x[1], x[2], y[0], y[1], y[2], ABx, ABy, ABz);
+                    };
+                    auto T = [&](const int* x, const int* y) {   // kinetic operator on the second function
+                        double t = 0.0;
+                        const double sv = S(x, y);
+                        for (int d = 0; d < 3; ++d) {
+                            int up[3] = {y[0], y[1], y[2]};
+                            up[d] += 2;
+                            double v = 4 * be * be * S(x, up) - 2 * be * (2 * y[d] + 1) * sv;
+                            if (y[d] >= 2) {
+                                int dn[3] = {y[0], y[1], y[2]};
+                                dn[d] -= 2;
+                                v += y[d] * (y[d] - 1) * S(x, dn);
+                            }
+                            t += -0.5 * v;
+                        }
+                        return t;
+                    };
+                    auto V = [&](int k, const int* x, const int* y) {
+                        return hrr2(vk.data() + (size_t)k * kTri.n, x[0], x[1], x[2], y[0], y[1], y[2], ABx, ABy, ABz);
+                    };
+                    for (int p = 0; p < na; ++p) {
+                        int x[3];
+                        cart_comp(la, p, x[0], x[1], x[2]);
+                        for (int q = 0; q < nb; ++q) {
+                            int y[3];
+                            cart_comp(lb, q, y[0], y[1], y[2]);
+                            const size_t ipq = (size_t)(A.f0 + p) * n + B.f0 + q;
+                            const double dpq = fac * cc * D[ipq], wpq = fac * cc * W[ipq];
+                            if (dpq == 0.0 && wpq == 0.0) continue;
+                            for (int e = 0; e < 3; ++e) {
+                                // derivatives of the bra function (centre A) and the ket function (centre B)
+                                int xu[3] = {x[0], x[1], x[2]}, xd[3] = {x[0], x[1], x[2]};
+                                int yu[3] = {y[0], y[1], y[2]}, yd[3] = {y[0], y[1], y[2]};
+                                xu[e]++; xd[e]--; yu[e]++; yd[e]--;
+                                const double sA = 2 * al * S(xu, y) - (x[e] ? x[e] * S(xd, y) : 0.0);
+                                const double sB = 2 * be * S(x, yu) - (y[e] ? y[e] * S(x, yd) : 0.0);
+                                const double tA = 2 * al * T(xu, y) - (x[e] ? x[e] * T(xd, y) : 0.0);
+                                const double tB = 2 * be * T(x, yu) - (y[e] ? y[e] * T(x, yd) : 0.0);
+                                double vA = 0.0, vB = 0.0;
+                                for (int k = 0; k < natom; ++k) {
+                                    const double ak = 2 * al * V(k, xu, y) - (x[e] ? x[e] * V(k, xd, y) : 0.0);
+                                    const double bk = 2 * be * V(k, x, yu) - (y[e] ? y[e] * V(k, x, yd) : 0.0);
+                                    vA += ak;
+                                    vB += bk;
+                                    g[3 * k + e] -= dpq * (ak + bk);          // the operator at nucleus k
+                                }
+                                g[3 * atA + e] += dpq * (tA + vA) - wpq * sA;
+                                g[3 * atB + e] += dpq * (tB + vB) - wpq * sB;
+                            }
+                        }
+                    }
+                }
+        }
+        part[a] = std::move(g);
+    }
+    for (int a = 0; a < ns; ++a)
+        for (int i = 0; i < 3 * natom; ++i) grad[i] += part[a][i];
+    // nuclear repulsion
+    for (int i = 0; i < natom; ++i)
+        for (int j = 0; j < natom; ++j) {
+            if (i == j) continue;
+            const double d[3] = {ib.atom_xyz[3 * i] - ib.atom_xyz[3 * j], ib.atom_xyz[3 * i + 1] - ib.atom_xyz[3 * j + 1],
+                                 ib.atom_xyz[3 * i + 2] - ib.atom_xyz[3 * j + 2]};
+            const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            for (int e = 0; e < 3; ++e) grad[3 * i + e] -= ib.natom_z[i] * ib.natom_z[j] * d[e] / (r * r * r);
+        }
+}
+
 double nuclear_repulsion(const IntBasis& ib) {
     double e = 0.0;
     int n = (int)ib.natom_z.size();

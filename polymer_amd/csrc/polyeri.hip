@@ -3926,3 +3926,64 @@ int polyeri_eri_gradient_host(int npgs, int ncgs, const int* pax, const int* pay
     std::copy(h.seg, h.seg + NPAIRCLASS + 1, g.seg);
     return polyeri_grad_host(g, grad) ? fail(POLYERI_E_ARG, "polyeri_eri_gradient_host failed") : 0;
 }
+
+// One-electron and nuclear parts of the nuclear gradient (OVERLAP_KINETIC(1),
+// NUCLEAR_ATTRACTION(1), HELLMANN_FEYNMAN_ATTRACTION / _REPULSION): host code
+// over the library's basis (host.cpp one_electron_gradient).
+static int one_electron_gradient_ib(const IntBasis& ib, const RefBasis& rb, const double* p_c, const double* w_c,
+                                    double* grad) {
+    std::vector<int> at(ib.shells.size(), -1);
+    for (size_t s = 0; s < ib.shells.size(); ++s) {
+        const Shell& sh = ib.shells[s];
+        for (int a = 0; a < rb.natom && at[s] < 0; ++a)
+            if (std::fabs(rb.ax[a] - sh.x) < 1e-8 && std::fabs(rb.ay[a] - sh.y) < 1e-8 && std::fabs(rb.az[a] - sh.z) < 1e-8)
+                at[s] = a;
+        if (at[s] < 0) return fail(POLYERI_E_BASIS, "one-electron gradient: a shell centre is not at an atom");
+    }
+    const int nbf = ib.nbf, nc = rb.ncgs;
+    std::vector<double> D((size_t)nbf * nbf), W((size_t)nbf * nbf);
+    for (int mu = 0; mu < nbf; ++mu)
+        for (int nu = 0; nu < nbf; ++nu) {
+            const double sc = ib.scale[mu] * ib.scale[nu];
+            const size_t r = ib.ref_of_int[mu] + (size_t)nc * ib.ref_of_int[nu];
+            D[(size_t)mu * nbf + nu] = sc * p_c[r];
+            W[(size_t)mu * nbf + nu] = sc * w_c[r];
+        }
+    std::vector<double> boys;
+    boys_table(boys);
+    one_electron_gradient(ib, at, boys, D.data(), W.data(), grad);
+    for (int i = 0; i < 3 * rb.natom; ++i)
+        if (!std::isfinite(grad[i])) return fail(POLYERI_E_NONFINITE, "one-electron gradient: non-finite value");
+    return 0;
+}
+
+int polyeri_one_electron_gradient(polyeri_ctx* ctx, const double* p_c, const double* w_c, double* grad) {
+    if (!ctx || !p_c || !w_c || !grad) return fail(POLYERI_E_ARG, "polyeri_one_electron_gradient: null argument");
+    if (ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_one_electron_gradient: molecular contexts only");
+    return one_electron_gradient_ib(ctx->ib, ctx->rb, p_c, w_c, grad);
+}
+
+int polyeri_one_electron_gradient_host(int npgs, int ncgs, const int* pax, const int* pay, const int* paz,
+                                       const double* pgsx, const double* pgsy, const double* pgsz, const double* zt,
+                                       const double* cc, int natom, const int* iatom, const double* xyz,
+                                       const double* p_c, const double* w_c, double* grad) {
+    if (npgs <= 0 || ncgs <= 0 || !pax || !pay || !paz || !pgsx || !pgsy || !pgsz || !zt || !cc || natom <= 0 ||
+        !iatom || !xyz || !p_c || !w_c || !grad)
+        return fail(POLYERI_E_ARG, "polyeri_one_electron_gradient_host: null or empty argument");
+    RefBasis rb;
+    rb.natom = natom;
+    rb.iatom.assign(iatom, iatom + natom);
+    for (int i = 0; i < natom; ++i) {
+        rb.ax.push_back(xyz[3 * i]); rb.ay.push_back(xyz[3 * i + 1]); rb.az.push_back(xyz[3 * i + 2]);
+    }
+    rb.npgs = npgs; rb.ncgs = ncgs;
+    rb.pax.assign(pax, pax + npgs); rb.pay.assign(pay, pay + npgs); rb.paz.assign(paz, paz + npgs);
+    rb.pgsx.assign(pgsx, pgsx + npgs); rb.pgsy.assign(pgsy, pgsy + npgs); rb.pgsz.assign(pgsz, pgsz + npgs);
+    rb.zt.assign(zt, zt + npgs);
+    rb.cc.assign(cc, cc + (size_t)npgs * ncgs);
+    rb.pgs_atom.assign(npgs, 0);
+    IntBasis ib;
+    std::string err;
+    if (extract_shells(rb, ib, err)) return fail(POLYERI_E_BASIS, err);
+    return one_electron_gradient_ib(ib, rb, p_c, w_c, grad);
+}

@@ -116,6 +116,10 @@ void build_pairs_cells(const IntBasis& ib, double primtol, double period, int rc
 void one_electron(const IntBasis& ib, const std::vector<double>& boys, std::vector<double>& S,
                   std::vector<double>& T, std::vector<double>& V);
 double nuclear_repulsion(const IntBasis& ib);
+// One-electron and nuclear-repulsion parts of the nuclear gradient at fixed
+// D and W (library basis, nbf x nbf), grad (3 natom) overwritten.
+void one_electron_gradient(const IntBasis& ib, const std::vector<int>& shell_atom, const std::vector<double>& boys,
+                           const double* D, const double* W, double* grad);
 // Cartesian exponents of component k of a shell of angular momentum l.
 void cart_comp(int l, int k, int& lx, int& ly, int& lz);
 

@@ -58,3 +58,65 @@ def test_eri_gradient_matches_finite_differences(basis):
     g0 = pa.eri_gradient_host(b, P, 0.0)
     g2 = pa.eri_gradient_host(b, P, 2.0)
     assert np.max(np.abs(g2 - 2 * g + g0)) < 1e-10 * max(1.0, np.max(np.abs(g)))
+
+
+def _e1(basis, atoms, P, W):
+    ob = oracle.Basis(os.path.join(pa.BASIS_DIR, basis), atoms)
+    _, s, t, n = ob.one_electron()
+    return np.sum(P * (t + n)) - np.sum(W * s) + ob.nuclear_repulsion()
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "cc-pVDZ"])
+def test_one_electron_gradient_matches_finite_differences(basis):
+    """sum P (T + N) - sum W S + V_nn at fixed P, W, differentiated by central
+    differences of the oracle's one-electron matrices and nuclear repulsion."""
+    atoms = M.H2O_GOLDEN
+    b = pa.read_basis(basis, atoms)
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal((b.ncgs, b.ncgs)); P = 0.1 * (P + P.T)
+    W = rng.standard_normal((b.ncgs, b.ncgs)); W = 0.1 * (W + W.T)
+    g = pa.one_electron_gradient_host(b, P, W)
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for a in range(len(atoms)):
+        for e in range(3):
+            def moved(d):
+                at = [list(t) for t in atoms]
+                at[a][1 + e] += d
+                return [tuple(t) for t in at]
+            fd[a, e] = (_e1(basis, moved(h), P, W) - _e1(basis, moved(-h), P, W)) / (2 * h)
+    err = np.max(np.abs(g - fd))
+    print(f"{basis}: one-electron max |g - fd| = {err:.2e}, max |g| = {np.max(np.abs(g)):.3e}")
+    assert err < 1e-6 * max(1.0, np.max(np.abs(g)))
+
+
+def test_scf_gradient_matches_energy_differences():
+    """The total analytic RHF gradient (one-electron part with the
+    energy-weighted density + two-electron part) against central differences
+    of the oracle's converged SCF energy, H2O/6-31G**, bent off symmetry."""
+    basis = "6-31Gss"
+    atoms = [(1, 0.1, 1.079252144093028, 1.474611055780858), (8, 0.0, 0.0, 0.0),
+             (1, -0.05, 1.1, -1.4)]
+
+    def scf(at):
+        ob = oracle.Basis(os.path.join(pa.BASIS_DIR, basis), at)
+        return ob.scf(inttol=0.0, scfconv=1e-11, scfcycles=200, diis=2)
+
+    r = scf(atoms)
+    b = pa.read_basis(basis, atoms)
+    W = pa.energy_weighted_density(r["co"], r["eps"], r["iocc"])
+    g = pa.one_electron_gradient_host(b, r["P"], W) + pa.eri_gradient_host(b, r["P"], 1.0)
+    h = 5e-4
+    fd = np.zeros_like(g)
+    for a in range(len(atoms)):
+        for e in range(3):
+            def moved(d):
+                at = [list(t) for t in atoms]
+                at[a][1 + e] += d
+                return [tuple(t) for t in at]
+            fd[a, e] = (scf(moved(h))["e_total"] - scf(moved(-h))["e_total"]) / (2 * h)
+    err = np.max(np.abs(g - fd))
+    print(f"SCF gradient max |g - fd| = {err:.2e}, max |g| = {np.max(np.abs(g)):.3e}, "
+          f"sum over atoms {np.max(np.abs(g.sum(axis=0))):.1e}")
+    assert err < 2e-6
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-9

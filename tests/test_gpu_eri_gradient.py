@@ -50,3 +50,34 @@ def test_eri_gradient_water16_invariants():
     assert np.max(np.abs(g2 - g)) < 1e-6
     ctx.close()
     c2.close()
+
+
+def test_scf_gradient_matches_energy_differences_gpu():
+    """The total RHF gradient of a GPU SCF (Context.scf_gradient: one-electron
+    part on the host, two-electron part on the device) against central
+    differences of the GPU SCF energy, H2O/cc-pVDZ bent off symmetry."""
+    atoms = [(1, 0.1, 1.079252144093028, 1.474611055780858), (8, 0.0, 0.0, 0.0), (1, -0.05, 1.1, -1.4)]
+    print("case: SCF gradient vs energy differences, H2O/cc-pVDZ", flush=True)
+
+    def scf(at):
+        c = pa.Context(pa.read_basis("cc-pVDZ", at), schwarz_tol=0.0)
+        r = c.self_consistent_field(scfconv=1e-11, diis=2, scfcycles=100, solver=1, diis_mode=1)
+        return c, r
+
+    ctx, r = scf(atoms)
+    g = ctx.scf_gradient(r)
+    h = 5e-4
+    fd = np.zeros_like(g)
+    for a in range(len(atoms)):
+        for e in range(3):
+            def moved(d):
+                at = [list(t) for t in atoms]
+                at[a][1 + e] += d
+                return [tuple(t) for t in at]
+            ep = scf(moved(h))[1]["e_total"]
+            em = scf(moved(-h))[1]["e_total"]
+            fd[a, e] = (ep - em) / (2 * h)
+    err = np.max(np.abs(g - fd))
+    print(f"max |g - fd| = {err:.2e}, max |g| = {np.max(np.abs(g)):.3e}", flush=True)
+    assert err < 2e-6
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-9
