@@ -296,6 +296,13 @@ int polyeri_scf_periodic(polyeri_ctx *ctx, const polyeri_scf_opts *opts, int kvc
 int polyeri_mp2_energy_periodic(polyeri_ctx *ctx, int kvc, const double *co, const double *eps, const int *nall,
                                 int icore, int iocc, int ivirtcore, int transform, double *e2s, double *e2t,
                                 double *seconds);
+/* The same with MODULO = nmod (IOPTN(99), src/pmp2energy.f90:64-80): the
+ * k-point MP2 over every nmod-th k-point of the 2 kvc-point SCF grid, with
+ * the sub-grid's phases and (2 kvc / nmod)^3 normalisation; nmod must divide
+ * kvc ("MODULO MUST EVENLY DIVIDE KPOINTS ALONG X" otherwise). */
+int polyeri_mp2_energy_periodic_mod(polyeri_ctx *ctx, int kvc, int nmod, const double *co, const double *eps,
+                                    const int *nall, int icore, int iocc, int ivirtcore, int transform, double *e2s,
+                                    double *e2t, double *seconds);
 
 /* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
  * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the

@@ -319,12 +319,13 @@ class Context:
         return res
 
     def mp2_energy_periodic(self, kpoints, co, eps, nall, icore, iocc, ivirtcore=0, transform="f64",
-                            timing=False):
+                            timing=False, modulo=1):
         """k-point E(2) singlet and triplet per unit cell (MP2_ENERGY_ALG1 of a
         periodic run) with the lattice integrals and all four quarter
         transforms on the device.  co: (nk, ncgs, ncgs) complex, columns =
         bands; eps: (nk, ncgs); nall: IALL per k.  transform "f64" (FP64 MFMA)
-        or "bf16x3" (BF16-split MFMA, FP32 accumulation)."""
+        or "bf16x3" (BF16-split MFMA, FP32 accumulation).  modulo: MODULO
+        (IOPTN(99)), every modulo-th k-point of the SCF grid."""
         if self.period is None:
             raise PolyeriError(-1, "mp2_energy_periodic needs a periodic context")
         nk = 2 * kpoints if kpoints > 0 else 1
@@ -337,9 +338,10 @@ class Context:
         na = np.ascontiguousarray(nall, dtype=np.int32).reshape(nk)
         mode = {"f64": 1, "bf16x3": 2}[transform]
         s, t, sec = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        check(lib().polyeri_mp2_energy_periodic(self.h, int(kpoints), cf.ctypes.data_as(ctypes.c_void_p), _p(eps),
-                                                na.ctypes.data_as(ctypes.c_void_p), icore, iocc, ivirtcore, mode,
-                                                ctypes.byref(s), ctypes.byref(t), ctypes.byref(sec)))
+        check(lib().polyeri_mp2_energy_periodic_mod(self.h, int(kpoints), int(modulo),
+                                                    cf.ctypes.data_as(ctypes.c_void_p), _p(eps),
+                                                    na.ctypes.data_as(ctypes.c_void_p), icore, iocc, ivirtcore, mode,
+                                                    ctypes.byref(s), ctypes.byref(t), ctypes.byref(sec)))
         if timing:
             return s.value, t.value, sec.value
         return s.value, t.value

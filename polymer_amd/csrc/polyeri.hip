@@ -3987,3 +3987,33 @@ int polyeri_one_electron_gradient_host(int npgs, int ncgs, const int* pax, const
     if (extract_shells(rb, ib, err)) return fail(POLYERI_E_BASIS, err);
     return one_electron_gradient_ib(ib, rb, p_c, w_c, grad);
 }
+
+// MP2_ENERGY_ALG1 with MODULO = nmod > 1 (IOPTN(99), src/pmp2energy.f90:64-80):
+// the k-point MP2 on the sub-grid of every nmod-th k-point of the SCF grid,
+// k = kb nmod for kb in [-kvc / nmod, kvc / nmod - 1], with that sub-grid's
+// phases and its (2 kvc / nmod)^3 normalisation -- the k-point MP2 of the
+// selected bands.
+int polyeri_mp2_energy_periodic_mod(polyeri_ctx* ctx, int kvc, int nmod, const double* co, const double* eps,
+                                    const int* nall, int icore, int iocc, int ivirtcore, int transform, double* e2s,
+                                    double* e2t, double* seconds) {
+    if (!ctx || !co || !eps || !nall || !e2s || !e2t)
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic_mod: null argument");
+    if (nmod < 1) return fail(POLYERI_E_ARG, "ILLEGAL MODULO PARAMETER");
+    if (kvc < 0 || kvc / nmod * nmod != kvc) return fail(POLYERI_E_ARG, "MODULO MUST EVENLY DIVIDE KPOINTS ALONG X");
+    if (nmod == 1)
+        return polyeri_mp2_energy_periodic(ctx, kvc, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t,
+                                           seconds);
+    const int kvc2 = kvc / nmod, nk2 = kvc2 > 0 ? 2 * kvc2 : 1, nc = ctx->rb.ncgs;
+    const size_t cstride = 2 * (size_t)nc * nc;           // complex, column-major per k
+    std::vector<double> co2(cstride * nk2), eps2((size_t)nc * nk2);
+    std::vector<int> nall2(nk2);
+    for (int j = 0; j < nk2; ++j) {
+        const int kb = kvc2 > 0 ? j - kvc2 : 0;
+        const int k = kb * nmod + kvc;                    // index of k-point kb nmod on the SCF grid
+        std::copy(co + cstride * k, co + cstride * (k + 1), co2.begin() + cstride * j);
+        std::copy(eps + (size_t)nc * k, eps + (size_t)nc * (k + 1), eps2.begin() + (size_t)nc * j);
+        nall2[j] = nall[k];
+    }
+    return polyeri_mp2_energy_periodic(ctx, kvc2, co2.data(), eps2.data(), nall2.data(), icore, iocc, ivirtcore,
+                                       transform, e2s, e2t, seconds);
+}

@@ -172,3 +172,34 @@ def test_periodic_store_restore_roundtrip(tmp_path):
     sl = slice(cel1 - rc1, cel1 + rc1 + 1)
     assert m == n
     assert np.array_equal(R[sl, sl], np.where(np.abs(T) > tol, T, 0.0))
+
+
+def test_periodic_mp2_modulo():
+    """MODULO (IOPTN(99), src/pmp2energy.f90:64-80): the k-point MP2 of an
+    SCF on 8 k-points with MODULO 2 is the k-point MP2 of the bands at every
+    second k-point (k = kb * 2, kb in [-2, 1]) with the 4-point grid's phases
+    and normalisation: equal to the device's 4-point MP2 of that subset and
+    to the oracle's restatement on it (1e-10 Eh); MODULO 3 of KPOINTS 4 is
+    rejected with the reference's message."""
+    from oracle import periodic
+    basis, atoms, period, cel1, cel2, _, icore = _case("h2_631gss")
+    kvc, nmod = 4, 2
+    print(f"case: k-point MP2 MODULO {nmod} of {2 * kvc} k-points", flush=True)
+    b = pa.read_basis(basis, atoms)
+    ctx = pa.Context(b, schwarz_tol=0.0, period=period, cel1=cel1, cel2=cel2)
+    r = ctx.self_consistent_field_periodic(kpoints=kvc, scfconv=1e-8, diis=2, scfcycles=60)
+    assert r["converged"]
+    iocc = r["iocc"]
+    s, t = ctx.mp2_energy_periodic(kvc, r["co"], r["eps"], r["nall"], icore, iocc, modulo=nmod)
+    idx = [kb * nmod + kvc for kb in range(-kvc // nmod, kvc // nmod)]
+    co2, eps2, nall2 = r["co"][idx], r["eps"][idx], np.asarray(r["nall"])[idx]
+    s2, t2 = ctx.mp2_energy_periodic(kvc // nmod, co2, eps2, nall2, icore, iocc)
+    rc1 = ctx.reduced_cel1
+    Tr = oracle.Basis(basis_path(basis), atoms).eri_tensor_cells(period, rc1, cel2)
+    sr, tr = periodic.mp2_kpoints(Tr, co2, eps2, rc1, cel2, kvc // nmod, icore, iocc, list(nall2))
+    print(f"MODULO {nmod}: E2 singlet {s:.12f} subset {s2:.12f} oracle {sr:.12f}; triplet {t:.12f} "
+          f"{t2:.12f} {tr:.12f}", flush=True)
+    assert abs(s - s2) < 1e-12 and abs(t - t2) < 1e-12
+    assert abs(s - sr) < 1e-10 and abs(t - tr) < 1e-10
+    with pytest.raises(pa.PolyeriError, match="MODULO MUST EVENLY DIVIDE"):
+        ctx.mp2_energy_periodic(2, r["co"][2:6], r["eps"][2:6], np.asarray(r["nall"])[2:6], icore, iocc, modulo=3)
