@@ -167,6 +167,16 @@ MODULE POLYERI
          INTEGER(C_INT) :: NALL(*)
       END FUNCTION
 
+      INTEGER(C_INT) FUNCTION POLYERI_MP2_ENERGY_PERIODIC_MOD(CTX, KVC, NMOD, CO, EPS, NALL, ICORE, IOCC, &
+            IVIRTCORE, TRANSFORM, E2S, E2T, SECONDS) BIND(C, NAME="polyeri_mp2_energy_periodic_mod")
+         IMPORT :: C_INT, C_DOUBLE, C_DOUBLE_COMPLEX, C_PTR
+         TYPE(C_PTR), VALUE :: CTX
+         INTEGER(C_INT), VALUE :: KVC, NMOD, ICORE, IOCC, IVIRTCORE, TRANSFORM
+         COMPLEX(C_DOUBLE_COMPLEX) :: CO(*)
+         REAL(C_DOUBLE) :: EPS(*), E2S, E2T, SECONDS
+         INTEGER(C_INT) :: NALL(*)
+      END FUNCTION
+
       INTEGER(C_INT) FUNCTION POLYERI_REDUCED_CEL1(CTX) BIND(C, NAME="polyeri_reduced_cel1")
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: CTX
