@@ -303,6 +303,16 @@ int polyeri_mp2_energy_periodic(polyeri_ctx *ctx, int kvc, const double *co, con
 int polyeri_mp2_energy_periodic_mod(polyeri_ctx *ctx, int kvc, int nmod, const double *co, const double *eps,
                                     const int *nall, int icore, int iocc, int ivirtcore, int transform, double *e2s,
                                     double *e2t, double *seconds);
+/* MP2_ENERGY_ALG1 of a periodic run with the quasi-particle bands
+ * (IOPTN(44) = nqp > 0, src/pmp2energy.f90:137-153, 445-522, 549-558; with
+ * MODULO = nmod as polyeri_mp2_energy_periodic_mod): E(2) as that call and
+ * QEPSILON(band, k) in qeps (EPSILON's shape, ncgs per k-point of the SCF
+ * grid) for bands max(iocc - nqp, icore) .. min(iocc + nqp, IALL(k) -
+ * ivirtcore) - 1 (0-based) at the k-points the MP2 samples; every other
+ * entry copies eps. */
+int polyeri_mp2_energy_periodic_qp(polyeri_ctx *ctx, int kvc, int nmod, const double *co, const double *eps,
+                                   const int *nall, int icore, int iocc, int ivirtcore, int nqp, int transform,
+                                   double *e2s, double *e2t, double *qeps, double *seconds);
 
 /* Drop-in for CALL MP2_ENERGY_ALG1 (src/psinglepoint.f90:422,
  * src/pmp2energy.f90:1-571) at one k-point: AO integrals computed on the

@@ -279,7 +279,7 @@ def _kwrap(k, kvc):
     return k
 
 
-def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0):
+def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0, nqp=0):
     """E(2) singlet and triplet per unit cell (src/pmp2energy.f90:187-548).
 
     T[Q1 + rc1, Q2 + rc1, Q3 + cel2, i, j, k, l] = (i_0 j_Q1 | k_Q3 l_Q3+Q2)
@@ -293,7 +293,19 @@ def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0):
       TVOKK(a,j,ka,kj)    = sum_{J,Q1} TOK CO(J,a,ka) e^{i ka Q1}           :323-329
       TOVOKK(i,a,j,ka,kj) = sum_I TVOKK conj(CO(I,i,ki)), ki = ka-kj+kb     :350-386
     then E2S/E2T over i, j (occupied), a, b (virtual) divided by (2 kvc)^3
-    (:400-433, :542-543)."""
+    (:400-433, :542-543).
+
+    nqp > 0 (IOPTN(44)): the j and i ranges reach band min(iocc + nqp,
+    IALL - IVIRTCORE) - 1 and the a range starts at max(iocc - nqp, icore)
+    (:137-153), and the quasi-particle bands g in max(iocc - nqp, icore) ..
+    min(iocc + nqp, IALL(kg) - IVIRTCORE) - 1 at every kg collect (:445-522)
+      sum_{kj,ka; kg = ka-kj+kb} sum_{j,a} [2|T(g,a,j)|^2 - Re T(g,a,j;ka,kj) T*(j,a,g;ka,kg)]
+                                            / (e_g + e_j - e_a - e_b)
+      sum_{kj,kg; ki = kg-kj+kb} sum_{j,i} [2|T(i,g,j)|^2 - Re T(i,g,j;kg,kj) T*(j,g,i;kg,ki)]
+                                            / (e_g + e_b - e_i - e_j)
+    and QEPSILON = eps + that / (2 kvc)^2 (:549-558).  Returns (es, et) for
+    nqp = 0, else (es, et, qeps) with qeps[k] a copy of eps[k] outside the
+    corrected bands."""
     nk = 2 * kvc if kvc > 0 else 1
     ks = list(range(-kvc, kvc)) if kvc > 0 else [0]
 
@@ -301,13 +313,27 @@ def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0):
         return np.exp(1j * np.pi * n / kvc) if kvc > 0 else 1.0 + 0j
     q1s = np.arange(-rc1, rc1 + 1)
     q3s = np.arange(-cel2, cel2 + 1)
-    occ = np.arange(icore, iocc)
     for k in range(nk):
         if nall[k] - ivirtcore < iocc:
             raise ValueError("fewer bands than occupied orbitals at a k-point")
+    plim = [nall[k] - ivirtcore for k in range(nk)]
+    pmax = max(plim)
+    no = iocc - icore
+    op = np.arange(icore, min(iocc + nqp, pmax))         # j and i bands (occupied + QP)
+    vlo = max(iocc - nqp, icore)
+    vv = np.arange(vlo, pmax)                            # a bands (QP + virtual)
+    aoff = iocc - vlo
+    nop, nvp = len(op), len(vv)
+
+    def bands(k_i, idx):
+        """CO columns idx at k, zero past IALL(k) - IVIRTCORE."""
+        c = co[k_i][:, idx].copy()
+        c[:, idx >= plim[k_i]] = 0.0
+        return c
     es, et = 0.0, 0.0
+    dq = np.zeros((nk, pmax))
     for kb_i, kb in enumerate(ks):
-        vb = np.arange(iocc, nall[kb_i] - ivirtcore)
+        vb = np.arange(iocc, plim[kb_i])
         if len(vb) == 0:
             continue
         # TNNNKK[q1, q3, i, j, k, b]
@@ -317,35 +343,63 @@ def mp2_kpoints(T, co, eps, rc1, cel2, kvc, icore, iocc, nall, ivirtcore=0):
             for a2, q2 in enumerate(q1s):
                 tn[:, a3] += np.einsum("qijkl,lb->qijkb", T[:, a2, a3], cb) * ph(kb * (q2 + q3))
         # TOK[q1, i, j, kj, jo, b]
-        tok = np.zeros((len(q1s),) + T.shape[3:5] + (nk, len(occ), len(vb)), dtype=complex)
+        tok = np.zeros((len(q1s),) + T.shape[3:5] + (nk, nop, len(vb)), dtype=complex)
         for kj_i, kj in enumerate(ks):
-            cj = co[kj_i][:, occ].conj()
+            cj = bands(kj_i, op).conj()
             for a3, q3 in enumerate(q3s):
                 tok[:, :, :, kj_i] += np.einsum("qijkb,ko->qijob", tn[:, a3], cj) * ph(-kj * q3)
-        # TVOKK[i, ka, a, kj, jo, b] (a padded to the largest virtual count)
-        nvmax = max(nall) - ivirtcore - iocc
-        tv = np.zeros((T.shape[3], nk, nvmax, nk, len(occ), len(vb)), dtype=complex)
+        # TVOKK[i, ka, a, kj, jo, b]
+        tv = np.zeros((T.shape[3], nk, nvp, nk, nop, len(vb)), dtype=complex)
         for ka_i, ka in enumerate(ks):
-            va = np.arange(iocc, nall[ka_i] - ivirtcore)
+            ca = bands(ka_i, vv)
             for a1, q1 in enumerate(q1s):
-                tv[:, ka_i, :len(va)] += np.einsum("ijkob,ja->iakob", tok[a1], co[ka_i][:, va]) * ph(ka * q1)
+                tv[:, ka_i] += np.einsum("ijkob,ja->iakob", tok[a1], ca) * ph(ka * q1)
         # TOVOKK[io, ka, a, kj, jo, b]
-        tov = np.zeros((len(occ), nk, nvmax, nk, len(occ), len(vb)), dtype=complex)
+        tov = np.zeros((nop, nk, nvp, nk, nop, len(vb)), dtype=complex)
         for ka_i, ka in enumerate(ks):
             for kj_i, kj in enumerate(ks):
                 ki_i = ks.index(_kwrap(ka - kj + kb, kvc))
-                tov[:, ka_i, :, kj_i] = np.einsum("iaob,im->maob", tv[:, ka_i, :, kj_i], co[ki_i][:, occ].conj())
+                tov[:, ka_i, :, kj_i] = np.einsum("iaob,im->maob", tv[:, ka_i, :, kj_i], bands(ki_i, op).conj())
+        eb = eps[kb_i][vb]
         for ka_i, ka in enumerate(ks):
-            nva = nall[ka_i] - ivirtcore - iocc
+            nva = plim[ka_i] - iocc
+            ea = eps[ka_i][iocc:plim[ka_i]]
             for kj_i, kj in enumerate(ks):
                 ki_i = ks.index(_kwrap(ka - kj + kb, kvc))
-                t1 = tov[:, ka_i, :nva, kj_i]                                  # (i, a, j, b)
-                t2 = np.transpose(tov[:, ka_i, :nva, ki_i], (2, 1, 0, 3))     # TOVOKK(j, a, i) at (ka, ki)
-                d = (eps[ki_i][occ][:, None, None, None] + eps[kj_i][occ][None, None, :, None]
-                     - eps[ka_i][iocc:iocc + nva][None, :, None, None] - eps[kb_i][vb][None, None, None, :])
+                t1 = tov[:no, ka_i, aoff:aoff + nva, kj_i, :no]                # (i, a, j, b)
+                t2 = np.transpose(tov[:no, ka_i, aoff:aoff + nva, ki_i, :no], (2, 1, 0, 3))   # TOVOKK(j, a, i) at (ka, ki)
+                d = (eps[ki_i][icore:iocc][:, None, None, None] + eps[kj_i][icore:iocc][None, None, :, None]
+                     - ea[None, :, None, None] - eb[None, None, None, :])
                 aa = np.real(t1 * t1.conj())
                 ab = np.real(t1 * t2.conj())
                 es += float(np.sum((0.5 * aa + 0.5 * ab) / d))
                 et += float(np.sum((1.5 * aa - 1.5 * ab) / d))
+        if nqp <= 0:
+            continue
+        ej = [eps[k][icore:iocc] for k in range(nk)]
+        for kj_i, kj in enumerate(ks):
+            for ka_i, ka in enumerate(ks):              # first sum: g in the i slot (:446-483)
+                kg_i = ks.index(_kwrap(ka - kj + kb, kvc))
+                ea = eps[ka_i][iocc:plim[ka_i]]
+                nva = len(ea)
+                for g in range(vlo, min(iocc + nqp, plim[kg_i])):
+                    t1 = tov[g - icore, ka_i, aoff:aoff + nva, kj_i, :no]           # T(g,a,j)[a, j, b]
+                    t2 = tov[:no, ka_i, aoff:aoff + nva, kg_i, g - icore]           # T(j,a,g)[j, a, b]
+                    t2 = np.transpose(t2, (1, 0, 2))
+                    d = eps[kg_i][g] + ej[kj_i][None, :, None] - ea[:, None, None] - eb[None, None, :]
+                    dq[kg_i, g] += float(np.sum((2.0 * np.real(t1 * t1.conj()) - np.real(t1 * t2.conj())) / d))
+            for kg_i, kg in enumerate(ks):              # second sum: g in the a slot (:484-520)
+                ki_i = ks.index(_kwrap(kg - kj + kb, kvc))
+                for g in range(vlo, min(iocc + nqp, plim[kg_i])):
+                    t1 = tov[:no, kg_i, g - vlo, kj_i, :no]                         # T(i,g,j)[i, j, b]
+                    t2 = np.transpose(tov[:no, kg_i, g - vlo, ki_i, :no], (1, 0, 2))  # T(j,g,i)[i, j, b]
+                    d = eps[kg_i][g] + eb[None, None, :] - ej[ki_i][:, None, None] - ej[kj_i][None, :, None]
+                    dq[kg_i, g] += float(np.sum((2.0 * np.real(t1 * t1.conj()) - np.real(t1 * t2.conj())) / d))
     norm = float(nk) ** 3
-    return es / norm, et / norm
+    if nqp <= 0:
+        return es / norm, et / norm
+    qeps = np.array([np.asarray(eps[k], dtype=np.float64).copy() for k in range(nk)])
+    for k in range(nk):
+        for g in range(vlo, min(iocc + nqp, plim[k])):
+            qeps[k][g] = eps[k][g] + dq[k, g] / float(nk) ** 2
+    return es / norm, et / norm, qeps

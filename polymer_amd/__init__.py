@@ -318,16 +318,9 @@ class Context:
             check(rc)
         return res
 
-    def mp2_energy_periodic(self, kpoints, co, eps, nall, icore, iocc, ivirtcore=0, transform="f64",
-                            timing=False, modulo=1):
-        """k-point E(2) singlet and triplet per unit cell (MP2_ENERGY_ALG1 of a
-        periodic run) with the lattice integrals and all four quarter
-        transforms on the device.  co: (nk, ncgs, ncgs) complex, columns =
-        bands; eps: (nk, ncgs); nall: IALL per k.  transform "f64" (FP64 MFMA)
-        or "bf16x3" (BF16-split MFMA, FP32 accumulation).  modulo: MODULO
-        (IOPTN(99)), every modulo-th k-point of the SCF grid."""
+    def _periodic_mp2_args(self, kpoints, co, eps, nall):
         if self.period is None:
-            raise PolyeriError(-1, "mp2_energy_periodic needs a periodic context")
+            raise PolyeriError(-1, "the k-point MP2 needs a periodic context")
         nk = 2 * kpoints if kpoints > 0 else 1
         n = self.ncgs
         co = np.asarray(co, dtype=np.complex128)
@@ -336,6 +329,17 @@ class Context:
         cf = np.ascontiguousarray(co.transpose(0, 2, 1))      # column-major per k
         eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(nk, n)
         na = np.ascontiguousarray(nall, dtype=np.int32).reshape(nk)
+        return cf, eps, na
+
+    def mp2_energy_periodic(self, kpoints, co, eps, nall, icore, iocc, ivirtcore=0, transform="f64",
+                            timing=False, modulo=1):
+        """k-point E(2) singlet and triplet per unit cell (MP2_ENERGY_ALG1 of a
+        periodic run) with the lattice integrals and all four quarter
+        transforms on the device.  co: (nk, ncgs, ncgs) complex, columns =
+        bands; eps: (nk, ncgs); nall: IALL per k.  transform "f64" (FP64 MFMA)
+        or "bf16x3" (BF16-split MFMA, FP32 accumulation).  modulo: MODULO
+        (IOPTN(99)), every modulo-th k-point of the SCF grid."""
+        cf, eps, na = self._periodic_mp2_args(kpoints, co, eps, nall)
         mode = {"f64": 1, "bf16x3": 2}[transform]
         s, t, sec = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         check(lib().polyeri_mp2_energy_periodic_mod(self.h, int(kpoints), int(modulo),
@@ -345,6 +349,23 @@ class Context:
         if timing:
             return s.value, t.value, sec.value
         return s.value, t.value
+
+    def mp2_quasiparticle_periodic(self, kpoints, co, eps, nall, icore, iocc, nqp, ivirtcore=0, transform="f64",
+                                   modulo=1):
+        """k-point E(2) singlet, triplet and the MP2 quasi-particle bands
+        QEPSILON (IOPTN(44) = nqp, with MODULO = modulo): returns
+        (e2s, e2t, qeps) with qeps of eps's shape (nk, ncgs); bands and
+        k-points outside the corrected range copy eps."""
+        cf, eps, na = self._periodic_mp2_args(kpoints, co, eps, nall)
+        mode = {"f64": 1, "bf16x3": 2}[transform]
+        q = np.zeros_like(eps)
+        s, t, sec = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().polyeri_mp2_energy_periodic_qp(self.h, int(kpoints), int(modulo),
+                                                   cf.ctypes.data_as(ctypes.c_void_p), _p(eps),
+                                                   na.ctypes.data_as(ctypes.c_void_p), icore, iocc, ivirtcore,
+                                                   int(nqp), mode, ctypes.byref(s), ctypes.byref(t), _p(q),
+                                                   ctypes.byref(sec)))
+        return s.value, t.value, q
 
     def mp2_quasiparticle(self, co, eps, icore, iocc, nall, nqp, ivirtcore=0, transform="f64"):
         """E(2) singlet, triplet and the MP2 quasi-particle band energies

@@ -26,7 +26,7 @@ EXPORTS = [
     "polyeri_load_fourindex_eri", "polyeri_set_direct", "polyeri_get_launch_config", "polyeri_set_launch_config",
     "polyeri_rys_nodes",
     "polyeri_rys_block", "polyeri_eri_gradient", "polyeri_eri_gradient_host", "polyeri_one_electron_gradient",
-    "polyeri_one_electron_gradient_host", "polyeri_mp2_energy_periodic_mod",
+    "polyeri_one_electron_gradient_host", "polyeri_mp2_energy_periodic_mod", "polyeri_mp2_energy_periodic_qp",
 ]
 
 
@@ -103,6 +103,7 @@ def lib():
     L.polyeri_scf_periodic.argtypes = [p, p, i, p, p, p, p, p, p]
     L.polyeri_mp2_energy_periodic.argtypes = [p, i, p, p, p, i, i, i, i, p, p, p]
     L.polyeri_mp2_energy_periodic_mod.argtypes = [p, i, i, p, p, p, i, i, i, i, p, p, p]
+    L.polyeri_mp2_energy_periodic_qp.argtypes = [p, i, i, p, p, p, i, i, i, i, i, p, p, p, p]
     L.polyeri_mp2_energy_qp.argtypes = [p, p, p, i, i, i, i, i, i, p, p, p]
     L.polyeri_store_fourindex_eri.argtypes = [p, ctypes.c_char_p, d, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri.argtypes = [ctypes.c_char_p, i, p, ctypes.POINTER(ctypes.c_longlong)]

@@ -3354,11 +3354,14 @@ __global__ void k_mp2k_operand(const double* __restrict__ Cr, const double* __re
 }
 
 // E(2) of one k_b (src/pmp2energy.f90:395-433) from
-// TOVOKK[kj][ka][b][jo][a][io] (planar), T2 = TOVOKK(j, a, i) at (ka, ki),
-// ki = ka - kj + kb (mod nk); eps[k][band]; a < IALL(ka) - IVIRTCORE - IOCC.
+// TOVOKK[kj][ka][b][jo][a'][io] (planar; jo, io < nop from ICORE, a' < nvp
+// from band iocc - aoff), T2 = TOVOKK(j, a, i) at (ka, ki), ki = ka - kj + kb
+// (mod nk); eps[k][band]; the sum runs over jo, io < no and the virtual
+// a < IALL(ka) - IVIRTCORE - IOCC.
 __global__ void k_mp2k_energy(const double* __restrict__ Tr, const double* __restrict__ Ti,
                               const double* __restrict__ eps, const int* __restrict__ plim, int nk, int kb, int nb,
-                              int no, int nv, int nc, int icore, int iocc, double* __restrict__ acc) {
+                              int no, int nv, int nop, int nvp, int aoff, int nc, int icore, int iocc,
+                              double* __restrict__ acc) {
     const size_t tot = (size_t)nk * nk * nb * no * nv * no;
     double es = 0.0, et = 0.0;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
@@ -3371,8 +3374,9 @@ __global__ void k_mp2k_energy(const double* __restrict__ Tr, const double* __res
         const int kj = (int)(r / nk);
         if (iocc + a >= plim[ka]) continue;
         const int ki = (ka - kj + kb + nk) % nk;
-        const double x = Tr[t], y = Ti[t];
-        const size_t t2 = (((((size_t)ki * nk + ka) * nb + b) * no + io) * nv + a) * no + jo;
+        const size_t t1 = (((((size_t)kj * nk + ka) * nb + b) * nop + jo) * nvp + aoff + a) * nop + io;
+        const double x = Tr[t1], y = Ti[t1];
+        const size_t t2 = (((((size_t)ki * nk + ka) * nb + b) * nop + io) * nvp + aoff + a) * nop + jo;
         const double x2 = Tr[t2], y2 = Ti[t2];
         const double aa = x * x + y * y, ab = x * x2 + y * y2;
         const double den = eps[(size_t)ki * nc + icore + io] + eps[(size_t)kj * nc + icore + jo] -
@@ -3397,11 +3401,78 @@ __global__ void k_mp2k_energy(const double* __restrict__ Tr, const double* __res
     }
 }
 
-int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, const double* eps, const int* nall,
-                                int icore, int iocc, int ivirtcore, int transform, double* e2s, double* e2t,
-                                double* seconds) {
+// Quasi-particle sums of one k_b (IOPTN(44), src/pmp2energy.f90:445-522) on
+// the TOVOKK layout of k_mp2k_energy: block (g', kg) adds to dq[kg][g'] the
+// band g = vlo + g' at kg (g < min(iocc + nqp, IALL(kg) - IVIRTCORE))
+//   sum over (kj, b, j, a), ka = kg + kj - kb:
+//     [2 |T(g,a,j;ka,kj)|^2 - Re T(g,a,j;ka,kj) T*(j,a,g;ka,kg)] / (e_g + e_j - e_a - e_b)
+//   sum over (kj, b, j, i), ki = kg - kj + kb:
+//     [2 |T(i,g,j;kg,kj)|^2 - Re T(i,g,j;kg,kj) T*(j,g,i;kg,ki)] / (e_g + e_b - e_i - e_j)
+// One block per output and the k_b launches in stream order: no atomics,
+// the same sum on every run.
+__global__ void __launch_bounds__(256) k_mp2k_qp(const double* __restrict__ Tr, const double* __restrict__ Ti,
+                                                 const double* __restrict__ eps, const int* __restrict__ plim,
+                                                 int nk, int kb, int nb, int no, int nv, int nop, int nvp, int aoff,
+                                                 int nc, int icore, int iocc, int vlo, int ghi, int ng,
+                                                 double* __restrict__ dq) {
+    const int gp = blockIdx.x, kg = blockIdx.y;
+    const int g = vlo + gp;
+    if (g >= min(ghi, plim[kg])) return;                      // whole block
+    const double eg = eps[(size_t)kg * nc + g];
+    const size_t sK = (size_t)nb * nop * nvp * nop;          // one (kj, ka) block of TOVOKK
+    double q = 0.0;
+    const size_t n1 = (size_t)nk * nb * no * nv;
+    for (size_t t = threadIdx.x; t < n1; t += blockDim.x) {
+        const int a = (int)(t % nv);
+        size_t r = t / nv;
+        const int jo = (int)(r % no); r /= no;
+        const int b = (int)(r % nb);
+        const int kj = (int)(r / nb);
+        const int ka = (kg + kj - kb + nk) % nk;
+        if (iocc + a >= plim[ka]) continue;
+        const size_t o1 = ((size_t)kj * nk + ka) * sK + (((size_t)b * nop + jo) * nvp + aoff + a) * nop + (g - icore);
+        const size_t o2 = ((size_t)kg * nk + ka) * sK + (((size_t)b * nop + (g - icore)) * nvp + aoff + a) * nop + jo;
+        const double x = Tr[o1], y = Ti[o1], x2 = Tr[o2], y2 = Ti[o2];
+        const double den = eg + eps[(size_t)kj * nc + icore + jo] - eps[(size_t)ka * nc + iocc + a] -
+                           eps[(size_t)kb * nc + iocc + b];
+        q += (2.0 * (x * x + y * y) - (x * x2 + y * y2)) / den;
+    }
+    const size_t n2 = (size_t)nk * nb * no * no;
+    for (size_t t = threadIdx.x; t < n2; t += blockDim.x) {
+        const int io = (int)(t % no);
+        size_t r = t / no;
+        const int jo = (int)(r % no); r /= no;
+        const int b = (int)(r % nb);
+        const int kj = (int)(r / nb);
+        const int ki = (kg - kj + kb + nk) % nk;
+        const size_t o1 = ((size_t)kj * nk + kg) * sK + (((size_t)b * nop + jo) * nvp + (g - vlo)) * nop + io;
+        const size_t o2 = ((size_t)ki * nk + kg) * sK + (((size_t)b * nop + io) * nvp + (g - vlo)) * nop + jo;
+        const double x = Tr[o1], y = Ti[o1], x2 = Tr[o2], y2 = Ti[o2];
+        const double den = eg + eps[(size_t)kb * nc + iocc + b] - eps[(size_t)ki * nc + icore + io] -
+                           eps[(size_t)kj * nc + icore + jo];
+        q += (2.0 * (x * x + y * y) - (x * x2 + y * y2)) / den;
+    }
+    __shared__ double sq[256];
+    sq[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sq[threadIdx.x] += sq[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dq[(size_t)kg * ng + gp] += sq[0];
+}
+
+// nqp > 0: also the quasi-particle bands (IOPTN(44)) into qeps[k][band]
+// (ncgs per k): the j and i sides of the transforms run over icore ..
+// min(iocc + nqp, max IALL - IVIRTCORE) - 1 and the a side from
+// max(iocc - nqp, icore) (src/pmp2energy.f90:137-153), then k_mp2k_qp.
+static int mp2_periodic(polyeri_ctx* ctx, int kvc, const double* co, const double* eps, const int* nall, int icore,
+                        int iocc, int ivirtcore, int transform, double* e2s, double* e2t, double* seconds, int nqp,
+                        double* qeps) {
     if (!ctx || !co || !eps || !nall || !e2s || !e2t)
         return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: null argument");
+    if (nqp < 0 || (nqp > 0 && !qeps))
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: bad quasi-particle request");
     if (!ctx->pbc) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: not a periodic context");
     if (kvc < 0 || kvc > 512) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: bad KPOINTS");
     if (transform != POLYERI_GEMM_MFMA_F64 && transform != POLYERI_GEMM_MFMA_BF16X3)
@@ -3419,14 +3490,17 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
         nvmax = std::max(nvmax, plim[k] - iocc);
     }
     if (nvmax <= 0) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: empty virtual space");
+    const int pmax = iocc + nvmax;                      // max IALL - IVIRTCORE
+    const int ghi = std::min(iocc + nqp, pmax), vlo = std::max(iocc - nqp, icore);
+    const int nop = ghi - icore, nvp = pmax - vlo, aoff = iocc - vlo, ng = nqp > 0 ? ghi - vlo : 0;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const int mode = transform == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
-    const long N = n, n1 = 2 * ctx->rc1 + 1, n3 = 2 * ctx->cel2 + 1, NK = nk, NO = no, NV = nvmax;
+    const long N = n, n1 = 2 * ctx->rc1 + 1, n3 = 2 * ctx->cel2 + 1, NK = nk, NO = nop, NV = nvp;
     const long NB = nvmax;                              // virtual bands of one k_b, at most
     const long M1 = N * n1 * N * n3 * N, M2 = N * n1 * N;
-    if (M1 > INT_MAX || (long)NK * NB * NO > INT_MAX)
+    if (M1 > INT_MAX || (long)NK * NB * NO > INT_MAX || (long)NB * NO * NV > INT_MAX)
         return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: lattice tensor too large for one pass");
     const size_t nT = (size_t)M1 * n1 * N;
     const size_t nTN = (size_t)M1 * NB, nTOK = (size_t)NK * NB * NO * M2, nTV = (size_t)NK * NK * NB * NO * NV * N,
@@ -3442,7 +3516,7 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
                                          " MiB of transform buffers exceed free device memory");
     DevBuf buf;
     double *dT, *dTNr, *dTNi, *dTOKr, *dTOKi, *dTVr, *dTVi, *dTOVr, *dTOVi, *dB1r, *dB1i, *dB2r, *dB2i, *dB3r, *dB3i,
-        *dB4r, *dB4i, *dCr, *dCi, *dEps, *dAcc;
+        *dB4r, *dB4i, *dCr, *dCi, *dEps, *dAcc, *dQ = nullptr;
     double2* dCO;
     int* dPlim;
     int rc;
@@ -3453,8 +3527,10 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
         (rc = buf.get(&dB2i, nB2)) || (rc = buf.get(&dB3r, nB3)) || (rc = buf.get(&dB3i, nB3)) ||
         (rc = buf.get(&dB4r, nB4)) || (rc = buf.get(&dB4i, nB4)) || (rc = buf.get(&dCr, nC)) ||
         (rc = buf.get(&dCi, nC)) || (rc = buf.get(&dEps, (size_t)nc * nk)) || (rc = buf.get(&dAcc, 2)) ||
-        (rc = buf.get(&dCO, (size_t)nc * nc * nk)) || (rc = buf.get(&dPlim, (size_t)nk)))
+        (rc = buf.get(&dCO, (size_t)nc * nc * nk)) || (rc = buf.get(&dPlim, (size_t)nk)) ||
+        (ng > 0 && (rc = buf.get(&dQ, (size_t)nk * ng))))
         return rc;
+    if (ng > 0) hipLaunchKernelGGL(k_fill, dim3(nblk((size_t)nk * ng)), dim3(256), 0, s, dQ, (size_t)nk * ng, 0.0);
     HIPCHK(hipMemcpyAsync(dCO, co, sizeof(double2) * (size_t)nc * nc * nk, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dEps, eps, 8 * (size_t)nc * nk, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dPlim, plim.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
@@ -3465,9 +3541,9 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
     // k-independent operands: B3[ka][(Q1,j)][a] = CO(j,a,ka) e^{i ka Q1} (:323-329),
     // B4[t][i][io] = conj(CO(i,io,t mod nk)), t < 2 nk (:350-386)
     hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB3)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, nk, (int)n1,
-                       ctx->rc1, nvmax, iocc, 1, 0, 0, 0, dPlim, dB3r, dB3i);
+                       ctx->rc1, nvp, vlo, 1, 0, 0, 0, dPlim, dB3r, dB3i);
     hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB4)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, 2 * nk, 1, 0,
-                       no, icore, 0, 0, 0, 1, dPlim, dB4r, dB4i);
+                       nop, icore, 0, 0, 0, 1, dPlim, dB4r, dB4i);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
     HIPCHK(hipGetLastError());
     auto gemm = [&](int M, int Nn, int K, const double* Ar, const double* Ai, long sam, long sak, long sA,
@@ -3487,26 +3563,26 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
                            (int)n1, ctx->rc1, nb, iocc, 1, 0, 0, 0, dPlim, dB1r, dB1i);
         // B2[kj][(Q3,k)][jo] = conj(CO(k,jo,kj)) e^{i (kb - kj) Q3} (:294-299)
         hipLaunchKernelGGL(k_mp2k_operand, dim3(nblk(nB2)), dim3(256), 0, s, dCr, dCi, nk, n, nc, kvc, 0, nk,
-                           (int)n3, ctx->cel2, no, icore, -1, kb, 1, 1, dPlim, dB2r, dB2i);
+                           (int)n3, ctx->cel2, nop, icore, -1, kb, 1, 1, dPlim, dB2r, dB2i);
         HIPCHK(hipGetLastError());
         if (gemm((int)M1, nb, (int)(n1 * N), dT, nullptr, n1 * N, 1, 0, dB1r, dB1i, NBk, 1, 0, dTNr, dTNi, 1, M1, 0,
                  1))
             return fail(POLYERI_E_HIP, "k-point MP2: first quarter launch failed");
         // Q2 quarter, every kj in one product (columns (kj, jo)), so TN is
         // read once: TOK[kj][b][jo][(i,Q1,j)] = sum_{Q3,k} TN B2[kj]
-        if (gemm((int)M2, nk * no, (int)(n3 * N), dTNr, dTNi, n3 * N, 1, M1, dB2r, dB2i, NO, 1, 0, dTOKr, dTOKi, 1,
-                 M2, NO * M2, nb, no, n3 * N * NO, NBk * NO * M2))
+        if (gemm((int)M2, nk * nop, (int)(n3 * N), dTNr, dTNi, n3 * N, 1, M1, dB2r, dB2i, NO, 1, 0, dTOKr, dTOKi, 1,
+                 M2, NO * M2, nb, nop, n3 * N * NO, NBk * NO * M2))
             return fail(POLYERI_E_HIP, "k-point MP2: second quarter launch failed");
         // Q3 quarter, every ka in one product (columns (ka, a)):
         // TV[ka][(kj,b,jo)][a][i] = sum_{Q1,j} TOK B3[ka]
-        if (gemm((int)N, nk * nvmax, (int)(n1 * N), dTOKr, dTOKi, n1 * N, 1, M2, dB3r, dB3i, NV, 1, 0, dTVr, dTVi, 1,
-                 N, NV * N, nk * nb * no, nvmax, n1 * N * NV, NK * NBk * NO * NV * N))
+        if (gemm((int)N, nk * nvp, (int)(n1 * N), dTOKr, dTOKi, n1 * N, 1, M2, dB3r, dB3i, NV, 1, 0, dTVr, dTVi, 1,
+                 N, NV * N, nk * nb * nop, nvp, n1 * N * NV, NK * NBk * NO * NV * N))
             return fail(POLYERI_E_HIP, "k-point MP2: third quarter launch failed");
         // Q4 quarter, per kj, batched over ka with ki = ka + (kb - kj mod nk):
         // TOV[kj][ka][(b,jo,a)][io] = sum_i TV[ka][kj] conj(CO(i,io,ki))
         for (int kj = 0; kj < nk; ++kj) {
             const long d = ((kb - kj) % nk + nk) % nk;
-            if (gemm(nb * no * nvmax, no, (int)N, dTVr + kj * NBk * NO * NV * N, dTVi + kj * NBk * NO * NV * N, N, 1,
+            if (gemm(nb * nop * nvp, nop, (int)N, dTVr + kj * NBk * NO * NV * N, dTVi + kj * NBk * NO * NV * N, N, 1,
                      NK * NBk * NO * NV * N, dB4r + d * N * NO, dB4i + d * N * NO, NO, 1, N * NO,
                      dTOVr + kj * NK * NBk * NO * NV * NO, dTOVi + kj * NK * NBk * NO * NV * NO, NO, 1,
                      NBk * NO * NV * NO, nk))
@@ -3515,18 +3591,36 @@ int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, con
         const size_t ne = (size_t)nk * nk * nb * no * nvmax * no;
         const size_t grid = std::min<size_t>(nblk(ne), 4096);
         hipLaunchKernelGGL(k_mp2k_energy, dim3((unsigned)grid), dim3(256), 0, s, dTOVr, dTOVi, dEps, dPlim, nk, kb, nb,
-                           no, nvmax, nc, icore, iocc, dAcc);
+                           no, nvmax, nop, nvp, aoff, nc, icore, iocc, dAcc);
+        if (ng > 0)
+            hipLaunchKernelGGL(k_mp2k_qp, dim3(ng, nk), dim3(256), 0, s, dTOVr, dTOVi, dEps, dPlim, nk, kb, nb, no,
+                               nvmax, nop, nvp, aoff, nc, icore, iocc, vlo, ghi, ng, dQ);
         HIPCHK(hipGetLastError());
     }
     double acc[2];
+    std::vector<double> hq((size_t)nk * ng);
     HIPCHK(hipMemcpyAsync(acc, dAcc, 16, hipMemcpyDeviceToHost, s));
+    if (ng > 0) HIPCHK(hipMemcpyAsync(hq.data(), dQ, 8 * hq.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const double norm = (double)nk * nk * nk;
     *e2s = acc[0] / norm;
     *e2t = acc[1] / norm;
+    if (nqp > 0) {
+        // QEPSILON = EPSILON + sum / (2 KVC)^2 (src/pmp2energy.f90:550-558)
+        std::copy(eps, eps + (size_t)nc * nk, qeps);
+        for (int k = 0; k < nk; ++k)
+            for (int g = vlo; g < std::min(ghi, plim[k]); ++g)
+                qeps[(size_t)k * nc + g] = eps[(size_t)k * nc + g] + hq[(size_t)k * ng + g - vlo] / ((double)nk * nk);
+    }
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!std::isfinite(acc[0]) || !std::isfinite(acc[1])) return fail(POLYERI_E_NONFINITE, "non-finite E(2)");
     return 0;
+}
+
+int polyeri_mp2_energy_periodic(polyeri_ctx* ctx, int kvc, const double* co, const double* eps, const int* nall,
+                                int icore, int iocc, int ivirtcore, int transform, double* e2s, double* e2t,
+                                double* seconds) {
+    return mp2_periodic(ctx, kvc, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t, seconds, 0, nullptr);
 }
 
 int polyeri_reduced_cel1(const polyeri_ctx* ctx) {
@@ -3992,28 +4086,53 @@ int polyeri_one_electron_gradient_host(int npgs, int ncgs, const int* pax, const
 // the k-point MP2 on the sub-grid of every nmod-th k-point of the SCF grid,
 // k = kb nmod for kb in [-kvc / nmod, kvc / nmod - 1], with that sub-grid's
 // phases and its (2 kvc / nmod)^3 normalisation -- the k-point MP2 of the
-// selected bands.
-int polyeri_mp2_energy_periodic_mod(polyeri_ctx* ctx, int kvc, int nmod, const double* co, const double* eps,
-                                    const int* nall, int icore, int iocc, int ivirtcore, int transform, double* e2s,
-                                    double* e2t, double* seconds) {
+// selected bands.  Quasi-particle bands (nqp > 0) come back at those
+// k-points of qeps (EPSILON's shape); the other k-points copy eps.
+static int mp2_periodic_mod(polyeri_ctx* ctx, int kvc, int nmod, const double* co, const double* eps,
+                            const int* nall, int icore, int iocc, int ivirtcore, int transform, double* e2s,
+                            double* e2t, double* seconds, int nqp, double* qeps) {
     if (!ctx || !co || !eps || !nall || !e2s || !e2t)
         return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic_mod: null argument");
     if (nmod < 1) return fail(POLYERI_E_ARG, "ILLEGAL MODULO PARAMETER");
     if (kvc < 0 || kvc / nmod * nmod != kvc) return fail(POLYERI_E_ARG, "MODULO MUST EVENLY DIVIDE KPOINTS ALONG X");
     if (nmod == 1)
-        return polyeri_mp2_energy_periodic(ctx, kvc, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t,
-                                           seconds);
+        return mp2_periodic(ctx, kvc, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t, seconds, nqp,
+                            qeps);
+    if (nqp < 0 || (nqp > 0 && !qeps))
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic: bad quasi-particle request");
     const int kvc2 = kvc / nmod, nk2 = kvc2 > 0 ? 2 * kvc2 : 1, nc = ctx->rb.ncgs;
     const size_t cstride = 2 * (size_t)nc * nc;           // complex, column-major per k
-    std::vector<double> co2(cstride * nk2), eps2((size_t)nc * nk2);
-    std::vector<int> nall2(nk2);
+    std::vector<double> co2(cstride * nk2), eps2((size_t)nc * nk2), q2(nqp > 0 ? (size_t)nc * nk2 : 0);
+    std::vector<int> nall2(nk2), kfull(nk2);
     for (int j = 0; j < nk2; ++j) {
         const int kb = kvc2 > 0 ? j - kvc2 : 0;
         const int k = kb * nmod + kvc;                    // index of k-point kb nmod on the SCF grid
+        kfull[j] = k;
         std::copy(co + cstride * k, co + cstride * (k + 1), co2.begin() + cstride * j);
         std::copy(eps + (size_t)nc * k, eps + (size_t)nc * (k + 1), eps2.begin() + (size_t)nc * j);
         nall2[j] = nall[k];
     }
-    return polyeri_mp2_energy_periodic(ctx, kvc2, co2.data(), eps2.data(), nall2.data(), icore, iocc, ivirtcore,
-                                       transform, e2s, e2t, seconds);
+    const int rc = mp2_periodic(ctx, kvc2, co2.data(), eps2.data(), nall2.data(), icore, iocc, ivirtcore, transform,
+                                e2s, e2t, seconds, nqp, nqp > 0 ? q2.data() : nullptr);
+    if (rc || nqp <= 0) return rc;
+    const int nk = kvc > 0 ? 2 * kvc : 1;
+    std::copy(eps, eps + (size_t)nc * nk, qeps);
+    for (int j = 0; j < nk2; ++j)
+        std::copy(q2.begin() + (size_t)nc * j, q2.begin() + (size_t)nc * (j + 1), qeps + (size_t)nc * kfull[j]);
+    return 0;
+}
+
+int polyeri_mp2_energy_periodic_mod(polyeri_ctx* ctx, int kvc, int nmod, const double* co, const double* eps,
+                                    const int* nall, int icore, int iocc, int ivirtcore, int transform, double* e2s,
+                                    double* e2t, double* seconds) {
+    return mp2_periodic_mod(ctx, kvc, nmod, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t, seconds, 0,
+                            nullptr);
+}
+
+int polyeri_mp2_energy_periodic_qp(polyeri_ctx* ctx, int kvc, int nmod, const double* co, const double* eps,
+                                   const int* nall, int icore, int iocc, int ivirtcore, int nqp, int transform,
+                                   double* e2s, double* e2t, double* qeps, double* seconds) {
+    if (nqp < 1) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_periodic_qp: nqp must be positive");
+    return mp2_periodic_mod(ctx, kvc, nmod, co, eps, nall, icore, iocc, ivirtcore, transform, e2s, e2t, seconds,
+                            nqp, qeps);
 }

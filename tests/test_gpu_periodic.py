@@ -203,3 +203,57 @@ def test_periodic_mp2_modulo():
     assert abs(s - sr) < 1e-10 and abs(t - tr) < 1e-10
     with pytest.raises(pa.PolyeriError, match="MODULO MUST EVENLY DIVIDE"):
         ctx.mp2_energy_periodic(2, r["co"][2:6], r["eps"][2:6], np.asarray(r["nall"])[2:6], icore, iocc, modulo=3)
+
+
+@pytest.mark.parametrize("case", ["h2_631gss", "ne_ccpvdz", "pe_sto3g"])
+def test_periodic_mp2_qp_matches_oracle(case):
+    """k-point MP2 quasi-particle bands (IOPTN(44) = 2, src/pmp2energy.f90:
+    445-522, 549-558): the device's wider transforms and its k_mp2k_qp sums
+    vs the oracle's restatement on the same bands (1e-10 Eh); E(2) equals
+    the nqp = 0 call (1e-12); bands outside the corrected range copy eps."""
+    from oracle import periodic
+    basis, atoms, period, cel1, cel2, kvc, icore = _case(case)
+    nqp = 2
+    print(f"case: k-point MP2 QP {case} kpoints {2 * kvc} icore {icore} nqp {nqp}", flush=True)
+    b = pa.read_basis(basis, atoms)
+    ctx = pa.Context(b, schwarz_tol=0.0, period=period, cel1=cel1, cel2=cel2)
+    r = ctx.self_consistent_field_periodic(kpoints=kvc, scfconv=1e-8, diis=2, scfcycles=60)
+    assert r["converged"]
+    iocc = r["iocc"]
+    s, t, q = ctx.mp2_quasiparticle_periodic(kvc, r["co"], r["eps"], r["nall"], icore, iocc, nqp)
+    s0, t0 = ctx.mp2_energy_periodic(kvc, r["co"], r["eps"], r["nall"], icore, iocc)
+    rc1 = ctx.reduced_cel1
+    Tr = oracle.Basis(basis_path(basis), atoms).eri_tensor_cells(period, rc1, cel2)
+    sr, tr, qr = periodic.mp2_kpoints(Tr, r["co"], r["eps"], rc1, cel2, kvc, icore, iocc, list(r["nall"]), nqp=nqp)
+    lo, hi = max(iocc - nqp, icore), iocc + nqp
+    print(f"E2 {s + t:.12f} (nqp 0: {s0 + t0:.12f}, oracle {sr + tr:.12f}); QP bands k=0: "
+          f"{np.array2string(q[0][lo:hi], precision=8)} max |dq| {np.max(np.abs(q - qr)):.2e}", flush=True)
+    assert abs(s - s0) < 1e-12 and abs(t - t0) < 1e-12
+    assert abs(s - sr) < 1e-10 and abs(t - tr) < 1e-10
+    assert np.max(np.abs(q - qr)) < 1e-10
+    assert np.min(np.abs(q[:, lo:hi] - r["eps"][:, lo:hi])) > 1e-5
+    keep = np.ones(q.shape[1], dtype=bool)
+    keep[lo:hi] = False
+    assert np.array_equal(q[:, keep], np.asarray(r["eps"])[:, keep])
+
+
+def test_periodic_mp2_qp_modulo():
+    """Quasi-particle bands with MODULO 2 on an 8-k-point SCF: the bands at
+    k = kb * 2 equal the 4-k-point QP bands of that subset (1e-12), the
+    other k-points copy eps."""
+    basis, atoms, period, cel1, cel2, _, icore = _case("h2_631gss")
+    kvc, nmod, nqp = 4, 2, 1
+    print(f"case: k-point MP2 QP MODULO {nmod} of {2 * kvc} k-points", flush=True)
+    ctx = pa.Context(pa.read_basis(basis, atoms), schwarz_tol=0.0, period=period, cel1=cel1, cel2=cel2)
+    r = ctx.self_consistent_field_periodic(kpoints=kvc, scfconv=1e-8, diis=2, scfcycles=60)
+    assert r["converged"]
+    iocc = r["iocc"]
+    s, t, q = ctx.mp2_quasiparticle_periodic(kvc, r["co"], r["eps"], r["nall"], icore, iocc, nqp, modulo=nmod)
+    idx = [kb * nmod + kvc for kb in range(-kvc // nmod, kvc // nmod)]
+    s2, t2, q2 = ctx.mp2_quasiparticle_periodic(kvc // nmod, r["co"][idx], r["eps"][idx],
+                                                np.asarray(r["nall"])[idx], icore, iocc, nqp)
+    print(f"MODULO QP: max |q - q_subset| {np.max(np.abs(q[idx] - q2)):.2e}", flush=True)
+    assert abs(s - s2) < 1e-12 and abs(t - t2) < 1e-12
+    assert np.max(np.abs(q[idx] - q2)) < 1e-12
+    rest = [k for k in range(2 * kvc) if k not in idx]
+    assert np.array_equal(q[rest], np.asarray(r["eps"])[rest])

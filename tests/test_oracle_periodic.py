@@ -212,6 +212,30 @@ def test_mp2_kpoints_isolated_limit_is_molecular_golden(golden):
     assert abs(s - s0) < 1e-12 and abs(t - t0) < 1e-12
 
 
+def test_mp2_kpoints_qp_isolated_limit_is_molecular_golden(golden):
+    """CUTOFF 0, Gamma only, IOPTN(44) = 2: the k-point quasi-particle bands
+    (src/pmp2energy.f90:445-522, 549-558) are the molecule's -- the shipped
+    BH/STO-3G bands (data/BH_EOM_out:289-297, 4 decimals) and the molecular
+    oracle (1e-12) -- and E(2) is unchanged by the wider j, i and a ranges."""
+    from oracle import periodic
+    from tests import molecules as M
+    g = golden["BH_STO3G"]
+    b = oracle.Basis(basis_path("STO-3G"), M.BH_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11)
+    T = periodic.ao_tensor_cells(basis_path("STO-3G"), M.BH_GOLDEN, 100.0, 0, 0)
+    co = np.asarray(r["co"], dtype=complex)[None]
+    eps = np.asarray(r["eps"])[None]
+    s, t, q = periodic.mp2_kpoints(T, co, eps, 0, 0, 0, 1, r["iocc"], [r["n_indep"]], nqp=2)
+    s0, t0 = periodic.mp2_kpoints(T, co, eps, 0, 0, 0, 1, r["iocc"], [r["n_indep"]])
+    assert abs(s - s0) < 1e-14 and abs(t - t0) < 1e-14
+    bands, qm = oracle.mp2_qp(b.eri_tensor(), r["co"], r["eps"], 1, r["iocc"], r["n_indep"], 2)
+    assert np.max(np.abs(q[0][bands] - qm)) < 1e-12
+    gold = [v for _, v in g["qp_bands"]][::-1]
+    assert np.max(np.abs(q[0][bands] - np.array(gold))) <= 5.001e-5
+    rest = np.setdiff1d(np.arange(len(eps[0])), bands)
+    assert np.array_equal(q[0][rest], eps[0][rest])
+
+
 def _fold_case(seed):
     """Random translation-invariant integrals G(i_0 j_dj | k_dk l_dl),
     |dj|, |dk|, |dl| <= 1, on a 2-function primitive cell; primitive-cell
@@ -292,3 +316,56 @@ def test_mp2_kpoints_band_folding_identity(seed):
     assert abs(sp) > 1e-3 and abs(tp) > 1e-3
     assert abs(sd - 2 * sp) < 1e-11 * max(1.0, abs(sd))
     assert abs(td - 2 * tp) < 1e-11 * max(1.0, abs(td))
+
+
+def _fold_bands(n, kvc_p, c, e):
+    """The primitive bands folded onto the doubled cell's k grid: returns
+    (cd, ed, src) with src[K][band] = (primitive k index, primitive band)."""
+    nk_p = 2 * kvc_p
+    kvc_d = kvc_p // 2
+    nk_d = 2 * kvc_d
+    cd = np.zeros((nk_d, 2 * n, 2 * n), dtype=complex)
+    ed = np.zeros((nk_d, 2 * n))
+    src = []
+    for K in range(nk_d):
+        IXd = K - kvc_d
+        cols, vals, who = [], [], []
+        for kp in range(nk_p):
+            IX = kp - kvc_p
+            if (IX - IXd) % nk_d:
+                continue
+            k = np.pi * IX / kvc_p
+            for band in range(n):
+                cols.append(np.concatenate([c[kp][:, band] * np.exp(1j * k * s) for s in range(2)]) / np.sqrt(2.0))
+                vals.append(e[kp][band])
+                who.append((kp, band))
+        order = np.argsort(vals)
+        cd[K] = np.array(cols).T[:, order]
+        ed[K] = np.array(vals)[order]
+        src.append([who[o] for o in order])
+    return cd, ed, src
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mp2_kpoints_qp_band_folding_identity(seed):
+    """Quasi-particle energies are intensive: the doubled cell's QP band at K
+    equals the primitive QP band it folds from (every band, both sums), and
+    E(2) is unchanged by nqp.  Fails for a wrong kg = ka - kj + kb or
+    ki = kg - kj + kb, a swap term at the wrong k-point, or a QP
+    normalisation other than (2 kvc)^2."""
+    from oracle import periodic
+    n, G, kvc_p, c, e = _fold_case(seed)
+    nk_p = 2 * kvc_p
+    Tp = _tensor_prim(n, G, 2, 1)
+    sp, tp, qp = periodic.mp2_kpoints(Tp, c, e, 2, 1, kvc_p, 0, 1, [n] * nk_p, nqp=1)
+    sp0, tp0 = periodic.mp2_kpoints(Tp, c, e, 2, 1, kvc_p, 0, 1, [n] * nk_p)
+    assert abs(sp - sp0) < 1e-13 and abs(tp - tp0) < 1e-13
+    cd, ed, src = _fold_bands(n, kvc_p, c, e)
+    sd, td, qd = periodic.mp2_kpoints(_tensor_doubled(n, G, 2, 1), cd, ed, 2, 1, kvc_p // 2, 0, 2,
+                                      [2 * n] * (nk_p // 2), nqp=2)
+    assert abs(sd - 2 * sp) < 1e-11 * max(1.0, abs(sd))
+    shift = np.abs(qp - e)
+    assert shift.min() > 1e-4                      # every band is corrected
+    for K, row in enumerate(src):
+        for band, (kp, bp) in enumerate(row):
+            assert abs(qd[K][band] - qp[kp][bp]) < 1e-12 * max(1.0, abs(qp[kp][bp]))
