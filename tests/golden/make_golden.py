@@ -33,6 +33,11 @@ PATTERNS = {
     "e2_alg1": r"^E\(2\)\s+=\s+(\S+) HARTREE",
     "e2_cc": r"^E\(2\) =\s+(\S+) \(",
     "e2_zero_t": r"^ZERO-TEMPERATURE MP2 ENERGY =\s+(\S+) HARTREE",
+    "e3_cc": r"^E\(3\) =\s+(\S+) \(",
+    "e_cc_singles": r"^E\(SINGLES\)\s+=\s+(\S+) HARTREE",
+    "e_cc_doubles": r"^E\(DOUBLES\)\s+=\s+(\S+) HARTREE",
+    "e_cc_corr": r"^E\(CORRELATION\)\s+=\s+(\S+) HARTREE",
+    "e_ccsd": r"^E\(CCSD\)\s+=\s+(\S+) HARTREE",
 }
 
 
@@ -87,6 +92,19 @@ def parse(path):
                 out[key] = rows
                 out[key + "_line"] = n + 1
                 break
+    # coupled-cluster iteration table: rows "ITER RESIDUUM CORRELATION CPU"
+    for n, line in enumerate(lines):
+        if line.startswith("CCSD ITERATION HAS STARTED"):
+            rows = []
+            for line2 in lines[n + 3:]:
+                if line2.startswith("-----"):
+                    break
+                m = re.match(r"^\s*(\d+)\s+(\S+)\s+(\S+)\s+\S+\s*$", line2)
+                if m:
+                    rows.append([int(m.group(1)), float(m.group(2)), float(m.group(3))])
+            out["cc_trace"] = rows
+            out["cc_trace_line"] = n + 1
+            break
     return out
 
 
