@@ -342,6 +342,29 @@ int polyeri_mp2_energy_ex(polyeri_ctx *ctx, const double *co, const double *eps,
  * ENERGY BANDS". */
 int polyeri_mp2_energy_qp(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc, int nall,
                           int ivirtcore, int nqp, int transform, double *e2s, double *e2t, double *qeps);
+/* INTEGRAL_TRANSFORM_DIRECTACCESS (src/pcc.f90:247-436) at one k-point: the
+ * full MO integral set g[p + m (q + m (r + m s))] = (pq|rs) over the MOs
+ * lo .. hi-1 (0-based columns of co, m = hi - lo), from AO integrals
+ * computed on the device and four quarter transforms on this library's FP64
+ * MFMA kernel (no .mo4 file). */
+int polyeri_mo_integrals(polyeri_ctx *ctx, const double *co, int lo, int hi, double *g);
+/* COUPLED_CLUSTER_ENERGY with COPTN(70) = 'CCSD' at one k-point
+ * (src/pcc.f90:1-243; MP2ENERGY_DISK, MP3ENERGY_DISK :503-807): spin-
+ * restricted CCSD on the active MOs icore .. nall - ivirtcore - 1 of canonical
+ * closed-shell HF orbitals (co, eps as polyeri_mp2_energy), every amplitude
+ * contraction on the device's FP64 MFMA GEMM, DIIS over ndiis vectors
+ * (IOPTN(54)), converged when max |t_new - t_old| < conv.  E(SINGLES) and
+ * E(DOUBLES) as the reference prints them (src/pcc.f90:1041-1166). */
+typedef struct {
+    double e2, e3;                /* E(2), E(3) */
+    double e_singles, e_doubles;  /* sum L t1 t1, sum L t2 (L_iajb = 2 (ia|jb) - (ib|ja)) */
+    double e_corr;                /* E(CORRELATION) = e_singles + e_doubles */
+    double residual;              /* max |t_new - t_old| of the last iteration */
+    int iterations, converged;
+    double tensor_ms, transform_ms, iter_ms, seconds;  /* device times; wall time of the call */
+} polyeri_cc_result;
+int polyeri_ccsd(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc, int nall, int ivirtcore,
+                 double conv, int maxiter, int ndiis, polyeri_cc_result *res);
 /* One product C = A B on those kernels (transform = POLYERI_GEMM_MFMA_F64 or
  * _BF16X3); A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn],
  * C(m,n) = c[m*scm + n*scn]; device pointers. */

@@ -380,6 +380,27 @@ class Context:
                                           ctypes.byref(s), ctypes.byref(t), _p(q)))
         return s.value, t.value, q
 
+    def mo_integrals(self, co, lo, hi):
+        """Full MO integral set (pq|rs) over MOs lo .. hi-1 (columns of co),
+        INTEGRAL_TRANSFORM_DIRECTACCESS on the device: an (m, m, m, m) array."""
+        co = np.asfortranarray(co, dtype=np.float64)
+        m = hi - lo
+        g = np.zeros((m, m, m, m))
+        check(lib().polyeri_mo_integrals(self.h, _p(co), int(lo), int(hi), _p(g)))
+        return g
+
+    def ccsd(self, co, eps, icore, iocc, nall, ivirtcore=0, conv=1e-10, maxiter=100, diis=8):
+        """Spin-restricted CCSD (COUPLED_CLUSTER_ENERGY, COPTN(70) = 'CCSD') with
+        E(2) and E(3) on the device: returns a dict with e2, e3, e_singles,
+        e_doubles, e_corr, residual, iterations, converged and timings."""
+        from ._lib import CcResult
+        co = np.asfortranarray(co, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        r = CcResult()
+        check(lib().polyeri_ccsd(self.h, _p(co), _p(eps), int(icore), int(iocc), int(nall), int(ivirtcore),
+                                 float(conv), int(maxiter), int(diis), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in CcResult._fields_}
+
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):
         """E(2) singlet and triplet parts.  transform: None (the library's
         default, its FP64 MFMA kernel), "f64" (the same), "rocblas" (rocBLAS

@@ -27,6 +27,7 @@ EXPORTS = [
     "polyeri_rys_nodes",
     "polyeri_rys_block", "polyeri_eri_gradient", "polyeri_eri_gradient_host", "polyeri_one_electron_gradient",
     "polyeri_one_electron_gradient_host", "polyeri_mp2_energy_periodic_mod", "polyeri_mp2_energy_periodic_qp",
+    "polyeri_mo_integrals", "polyeri_ccsd",
 ]
 
 
@@ -61,6 +62,15 @@ class ScfResult(ctypes.Structure):
                 ("iocc", ctypes.c_int), ("jk_seconds", ctypes.c_double),
                 ("fock_seconds", ctypes.c_double), ("scf_seconds", ctypes.c_double),
                 ("quartets", ctypes.c_longlong)]
+
+
+class CcResult(ctypes.Structure):
+    _fields_ = [("e2", ctypes.c_double), ("e3", ctypes.c_double),
+                ("e_singles", ctypes.c_double), ("e_doubles", ctypes.c_double),
+                ("e_corr", ctypes.c_double), ("residual", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("tensor_ms", ctypes.c_double), ("transform_ms", ctypes.c_double),
+                ("iter_ms", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
 MAX_LISTS = 64   # POLYERI_MAX_LISTS
@@ -105,6 +115,8 @@ def lib():
     L.polyeri_mp2_energy_periodic_mod.argtypes = [p, i, i, p, p, p, i, i, i, i, p, p, p]
     L.polyeri_mp2_energy_periodic_qp.argtypes = [p, i, i, p, p, p, i, i, i, i, i, p, p, p, p]
     L.polyeri_mp2_energy_qp.argtypes = [p, p, p, i, i, i, i, i, i, p, p, p]
+    L.polyeri_mo_integrals.argtypes = [p, p, i, i, p]
+    L.polyeri_ccsd.argtypes = [p, p, p, i, i, i, i, d, i, i, ctypes.POINTER(CcResult)]
     L.polyeri_store_fourindex_eri.argtypes = [p, ctypes.c_char_p, d, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri.argtypes = [ctypes.c_char_p, i, p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri_cells.argtypes = [ctypes.c_char_p, i, i, i, p, ctypes.POINTER(ctypes.c_longlong)]

@@ -56,6 +56,7 @@ int polyeri_record_error(int code, const char* msg) { return fail(code, msg ? ms
 // Kernels
 // ---------------------------------------------------------------------------
 #include "eri_kernels.h"
+#include "kcc.h"
 
 __global__ void k_fill(double* p, size_t n, double v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2909,6 +2910,114 @@ int polyeri_mp2_energy_qp(polyeri_ctx* ctx, const double* co, const double* eps,
                           int ivirtcore, int nqp, int transform, double* e2s, double* e2t, double* qeps) {
     if (nqp < 1) return fail(POLYERI_E_ARG, "polyeri_mp2_energy_qp: nqp must be positive");
     return mp2_energy(ctx, co, eps, icore, iocc, nall, ivirtcore, transform, e2s, e2t, nqp, qeps);
+}
+
+// ---------------------------------------------------------------------------
+// Full MO integral set and coupled-cluster energies (src/pcc.f90)
+// ---------------------------------------------------------------------------
+// g[p][q][r][s] = (pq|rs) from the quarter-transform result
+// G[b][a][i][j] = (ia|jb) of the four products below.
+__global__ void k_mo_order(const double* __restrict__ G, double* __restrict__ g, int m) {
+    const size_t m4 = (size_t)m * m * m * m;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < m4; t += (size_t)gridDim.x * blockDim.x) {
+        const int sidx = (int)(t % m), r = (int)(t / m % m), q = (int)(t / ((size_t)m * m) % m),
+                  p = (int)(t / ((size_t)m * m * m));
+        g[t] = G[(((size_t)sidx * m + q) * m + p) * m + r];
+    }
+}
+
+// INTEGRAL_TRANSFORM_DIRECTACCESS (src/pcc.f90:247-436) at one k-point: the
+// AO tensor on the device and four quarter transforms over the MOs lo .. hi-1
+// on the FP64 MFMA kernel (the MP2 transform with every index over the full
+// active range), then reordered to dG[p][q][r][s] = (pq|rs).  Buffers are
+// owned by buf; *ms_tensor / *ms_transform get the device times.
+static int mo_tensor_device(polyeri_ctx* ctx, const double* co, int lo, int hi, DevBuf& buf, double** dG,
+                            double* ms_tensor, double* ms_transform) {
+    if (ctx->pbc) return fail(POLYERI_E_ARG, "MO integrals: periodic context (molecules only)");
+    const int n = ctx->ib.nbf, nc = ctx->rb.ncgs, m = hi - lo;
+    if (lo < 0 || m < 1 || hi > nc) return fail(POLYERI_E_ARG, "MO integrals: bad orbital range");
+    const long N = n, M = m, N3 = (long)n * n * n, M2 = (long)m * m;
+    if ((double)m * n * n * n * 8 >= 2147483648.0 || (double)M2 * M2 * 8 >= 2147483648.0)
+        return fail(POLYERI_E_ARG, "MO integrals: transform intermediates beyond 2 GiB");
+    hipStream_t s = ctx->stream;
+    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT;
+    int rc;
+    if ((rc = buf.get(&dTen, (size_t)N3 * n)) || (rc = buf.get(&dCO, (size_t)nc * nc)) ||
+        (rc = buf.get(&dC, (size_t)n * nc)) || (rc = buf.get(&dX1, (size_t)m * N3)) ||
+        (rc = buf.get(&dX2, (size_t)M2 * n * n)) || (rc = buf.get(&dX3, (size_t)M2 * m * n)) ||
+        (rc = buf.get(&dT, (size_t)M2 * M2)) || (rc = buf.get(dG, (size_t)M2 * M2)))
+        return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    struct Ev { hipEvent_t a, b, c; ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); } } ev{e0, e1, e2};
+    HIPCHK(hipEventRecord(e0, s));
+    if ((rc = tensor_device(ctx, dTen))) return rc;
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipMemcpyAsync(dCO, co, (size_t)nc * nc * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mo_int, dim3(nblk((size_t)n * nc)), dim3(256), 0, s, dCO, nc, ctx->d_ref, ctx->d_scale, n, nc, dC);
+    const double* Cm = dC + (size_t)n * lo;
+    // X1(i, m, u, l) = sum_s C(s, i) A(s, l, u, m); X2(j, i, m, l); X3(j, i, a, m); T(j, i, a, b) = (ia|jb)
+    int r = polyeri_gemm_launch(GEMM_F64, m, n * n * n, n, Cm, N, 1, 0, dTen, N3, 1, 0, dX1, 1, M, 0, 1, s);
+    if (!r) r = polyeri_gemm_launch(GEMM_F64, m * n, m, n, dX1, 1, M * N, M * N * N, Cm, 1, N, 0, dX2, M, 1, M2 * N, n, s);
+    if (!r) r = polyeri_gemm_launch(GEMM_F64, m * m, m, n, dX2, 1, M2 * N, M2, Cm, 1, N, 0, dX3, 1, M2, M2 * M, n, s);
+    if (!r) r = polyeri_gemm_launch(GEMM_F64, m * m * m, m, n, dX3, 1, M2 * M, 0, Cm, 1, N, 0, dT, 1, M2 * M, 0, 1, s);
+    if (r) return fail(POLYERI_E_HIP, "MO integral transform launch failed");
+    hipLaunchKernelGGL(k_mo_order, dim3(nblk((size_t)M2 * M2)), dim3(256), 0, s, dT, *dG, m);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e2, s));
+    HIPCHK(hipEventSynchronize(e2));
+    if (ms_tensor) *ms_tensor = event_ms(e0, e1);
+    if (ms_transform) *ms_transform = event_ms(e1, e2);
+    return 0;
+}
+
+int polyeri_mo_integrals(polyeri_ctx* ctx, const double* co, int lo, int hi, double* g) {
+    if (!ctx || !co || !g) return fail(POLYERI_E_ARG, "polyeri_mo_integrals: null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    double* dG = nullptr;
+    int rc = mo_tensor_device(ctx, co, lo, hi, buf, &dG, nullptr, nullptr);
+    if (rc) return rc;
+    const size_t m = (size_t)(hi - lo);
+    HIPCHK(hipMemcpyAsync(g, dG, m * m * m * m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall, int ivirtcore,
+                 double conv, int maxiter, int ndiis, polyeri_cc_result* res) {
+    if (!ctx || !co || !eps || !res) return fail(POLYERI_E_ARG, "polyeri_ccsd: null argument");
+    const int hi = nall - ivirtcore, no = iocc - icore;
+    if (icore < 0 || no < 1 || ivirtcore < 0 || hi <= iocc || nall > ctx->rb.ncgs)
+        return fail(POLYERI_E_ARG, "polyeri_ccsd: empty occupied or virtual space");
+    if (!(conv > 0.0) || maxiter < 1) return fail(POLYERI_E_ARG, "polyeri_ccsd: bad convergence request");
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf buf;
+    double* dG = nullptr;
+    double mst = 0.0, msx = 0.0;
+    int rc = mo_tensor_device(ctx, co, icore, hi, buf, &dG, &mst, &msx);
+    if (rc) return rc;
+    CcSolveOut o{};
+    std::string err;
+    rc = polyeri_cc_solve(dG, hi - icore, no, eps + icore, conv, maxiter, ndiis, &o, ctx->stream, &err);
+    if (rc) return fail(rc == -2 ? POLYERI_E_NOMEM : rc == -1 ? POLYERI_E_ARG : POLYERI_E_HIP, "polyeri_ccsd: " + err);
+    res->e2 = o.e2;
+    res->e3 = o.e3;
+    res->e_singles = o.e_singles;
+    res->e_doubles = o.e_doubles;
+    res->e_corr = o.e_singles + o.e_doubles;
+    res->residual = o.residual;
+    res->iterations = o.iterations;
+    res->converged = o.converged;
+    res->tensor_ms = mst;
+    res->transform_ms = msx;
+    res->iter_ms = o.iter_ms;
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!std::isfinite(res->e_corr)) return fail(POLYERI_E_NONFINITE, "polyeri_ccsd: non-finite energy");
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
