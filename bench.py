@@ -123,6 +123,56 @@ def mp2_line(nwater, seed, transforms):
     return mp2
 
 
+def ccsd_line(nwater, seed, cpu_budget, want_cpu):
+    """Spin-restricted CCSD of (H2O)nwater / cc-pVDZ on the device (full MO
+    transform, E(2), E(3), CCSD iterations): one warm call, one timed call.
+    The CPU baseline times the oracle's CCSD residual (numpy) on the same MO
+    integrals for one iteration."""
+    import numpy as np
+
+    import polymer_amd as pa
+    from polymer_amd import geometry as G
+    print("bench: CCSD", file=sys.stderr, flush=True)
+    basis = "cc-pVDZ"
+    atoms = G.water_cluster(nwater, seed)
+    mc = pa.Context(pa.read_basis(basis, atoms), schwarz_tol=1e-12)
+    r = mc.self_consistent_field(scfconv=1e-9, diis=2, scfcycles=80, solver=1, diis_mode=1)
+    n, iocc = r["n_indep"], r["iocc"]
+    mc.ccsd(r["co"], r["eps"], 0, iocc, n, conv=1e-8)                         # warm
+    t0 = time.perf_counter()
+    res = mc.ccsd(r["co"], r["eps"], 0, iocc, n, conv=1e-8)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"(H2O){nwater} {basis} CCSD, all electrons correlated, seed {seed}",
+           "nmo": n, "occupied": iocc, "virtual": n - iocc, "wall_s": wall,
+           "e2": res["e2"], "e3": res["e3"], "e_corr": res["e_corr"], "iterations": res["iterations"],
+           "converged": res["converged"], "tensor_ms": res["tensor_ms"], "transform_ms": res["transform_ms"],
+           "iter_ms": res["iter_ms"], "iter_gflop": res["iter_gflop"],
+           "iter_tflops": res["iter_gflop"] / res["iter_ms"] if res["iter_ms"] > 0 else None}
+    out["roofline"] = {"bound": "mfma_fp64", "achieved": out["iter_tflops"], "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": (out["iter_tflops"] or 0.0) / FP64_PEAK_TFLOPS,
+                       "flops_model": "2 m n k over every contraction of one iteration (GEMM work; transposes "
+                                      "and element-wise kernels are not counted)"}
+    if want_cpu:
+        from oracle import cc
+        g = mc.mo_integrals(r["co"], 0, n)
+        I = cc._Ints(g, iocc)
+        rng = np.random.default_rng(seed)
+        t1 = 1e-3 * rng.standard_normal((iocc, n - iocc))
+        t2 = 1e-2 * rng.standard_normal((iocc, iocc, n - iocc, n - iocc))
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or (time.perf_counter() - t0 < cpu_budget and reps < 3):
+            cc.ccsd_residual(I, t1, t2)
+            reps += 1
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out["cpu_baseline"] = {"value": ms, "unit": "ms per CCSD iteration", "cores": os.cpu_count(),
+                               "kind": "port", "sample": f"{reps} residual evaluation(s) of the oracle's numpy "
+                                                         f"CCSD (BLAS threads as numpy sets them)"}
+        out["speedup_vs_cpu"] = ms / res["iter_ms"] if res["iter_ms"] > 0 else None
+    mc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +187,9 @@ def main():
     ap.add_argument("--mp2-nwater", type=int, default=12)
     ap.add_argument("--mp2-transforms", default="f64,rocblas", help="comma list of f64, rocblas, bf16x3")
     ap.add_argument("--mp2-only", action="store_true", help="print the MP2 line alone (for profiling runs)")
+    ap.add_argument("--no-ccsd", action="store_true", help="skip the CCSD line")
+    ap.add_argument("--ccsd-nwater", type=int, default=4)
+    ap.add_argument("--ccsd-only", action="store_true", help="print the CCSD line alone (for profiling runs)")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="run the library's copy kernel over this many MiB first (counter scaling)")
     ap.add_argument("--save-tuning", default=None, help="write the autotuned launch records to this JSON file")
@@ -149,6 +202,9 @@ def main():
         sys.exit(2)
     if args.mp2_only:
         print(json.dumps(mp2_line(args.mp2_nwater, args.seed, args.mp2_transforms.split(","))))
+        return
+    if args.ccsd_only:
+        print(json.dumps(ccsd_line(args.ccsd_nwater, args.seed, args.cpu_budget, not args.no_cpu)))
         return
 
     import numpy as np
@@ -317,6 +373,8 @@ def main():
                              "target_dE": 1e-6}
 
     mp2 = None if args.no_mp2 else mp2_line(args.mp2_nwater, args.seed, args.mp2_transforms.split(","))
+    ccsd = None if args.no_ccsd else ccsd_line(args.ccsd_nwater, args.seed, min(args.cpu_budget, 10.0),
+                                               not args.no_cpu)
 
     # two-electron nuclear gradient at the converged FP64 density of the workload
     grad = None
@@ -389,6 +447,7 @@ def main():
         "mixed_classes": class_table(mst, mx_cls),
         "scf": scf,
         "mp2": mp2,
+        "ccsd": ccsd,
         "eri_gradient": grad,
         "checksum_abs_F": fsum,
     }

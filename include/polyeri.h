@@ -362,6 +362,7 @@ typedef struct {
     double residual;              /* max |t_new - t_old| of the last iteration */
     int iterations, converged;
     double tensor_ms, transform_ms, iter_ms, seconds;  /* device times; wall time of the call */
+    double iter_gflop;            /* GEMM work of one CCSD iteration (GFLOP) */
 } polyeri_cc_result;
 int polyeri_ccsd(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc, int nall, int ivirtcore,
                  double conv, int maxiter, int ndiis, polyeri_cc_result *res);

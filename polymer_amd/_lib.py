@@ -70,7 +70,8 @@ class CcResult(ctypes.Structure):
                 ("e_corr", ctypes.c_double), ("residual", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("tensor_ms", ctypes.c_double), ("transform_ms", ctypes.c_double),
-                ("iter_ms", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("iter_ms", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("iter_gflop", ctypes.c_double)]
 
 
 MAX_LISTS = 64   # POLYERI_MAX_LISTS

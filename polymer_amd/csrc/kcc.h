@@ -10,6 +10,7 @@ struct CcSolveOut {
     double residual;            // max |t_new - t_old| of the last iteration
     int iterations, converged;
     double iter_ms;             // device time per iteration (average)
+    double iter_gflop;          // GEMM work of one iteration, GFLOP
 };
 
 // CCSD (and MP2, MP3) from the active MO integrals g[p][q][r][s] = (pq|rs)

@@ -19,6 +19,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
+#include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -35,17 +38,30 @@ struct Perm {
 
 // dst[t] = alpha src[off(t)] + beta dst[t], t over dst in row-major order,
 // off the source offset of the same multi-index (gather transpose).
+// Tensors here stay below 2^31 elements (checked by the solver), so the
+// index arithmetic runs in 32-bit unsigned integers.
 __global__ void __launch_bounds__(256) k_cc_permute(const double* __restrict__ src, double* __restrict__ dst, Perm p,
                                                     long long n, double alpha, double beta) {
-    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
-        long long r = t, off = 0;
+    for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < (unsigned)n; t += gridDim.x * 256u) {
+        unsigned r = t, off = 0;
         for (int d = p.rank - 1; d >= 0; --d) {
-            const long long q = r / p.dim[d];
-            off += (r - q * p.dim[d]) * p.sst[d];
+            const unsigned dd = (unsigned)p.dim[d];
+            const unsigned q = r / dd;
+            off += (r - q * dd) * (unsigned)p.sst[d];
             r = q;
         }
         const double v = alpha * src[off];
         dst[t] = beta == 0.0 ? v : beta * dst[t] + v;
+    }
+}
+
+// out[t] = sum_s part[s mn + t], s = 0 .. ns-1 in order (split-K partials)
+__global__ void __launch_bounds__(256) k_cc_sumk(const double* __restrict__ part, int ns, long long mn,
+                                                 double* __restrict__ out) {
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < mn; t += (long long)gridDim.x * 256) {
+        double v = 0.0;
+        for (int q = 0; q < ns; ++q) v += part[q * mn + t];
+        out[t] = v;
     }
 }
 
@@ -82,6 +98,8 @@ __global__ void __launch_bounds__(256) k_cc_denom(const double* __restrict__ r, 
 }
 
 constexpr int kRedBlocks = 256;
+constexpr long long kSplitMax = 512;            // split-K chunks, at most
+constexpr long long kSplitBuf = 1LL << 21;      // doubles of split-K partials
 
 // Per-block partial sums of a . b (mode 0) or of max |a - b| (mode 1);
 // the host adds the kRedBlocks partials in a fixed order.
@@ -122,8 +140,12 @@ struct Dev {
 struct Engine {
     hipStream_t s;
     long long no, nv;
-    double *w0, *w1, *w2, *part;
+    double *w0, *w1, *w2, *part, *wk;
     std::string* err;
+    double flops = 0.0;   // GEMM flops issued (2 m n k per contraction)
+    Dev* dev = nullptr;   // owner of cached layouts
+    std::set<const double*> consts;               // tensors fixed for the whole solve
+    std::map<std::string, double*> cache;         // their transposed copies, by (pointer, layout)
 
     long long dim(char c) const { return (c >= 'i' && c <= 'l') ? no : nv; }
     long long size(const std::string& l) const {
@@ -177,6 +199,20 @@ struct Engine {
             else if (!(ina && !inc)) return bad("cc contract: label " + std::string(1, c) + " of " + lb);
         }
         if (M.size() + N.size() != lc.size()) return bad("cc contract: output labels " + lc);
+        // the contracted labels in A's order or in B's, whichever leaves
+        // fewer elements to transpose this call (cached layouts are free)
+        {
+            std::string KB;
+            for (char c : lb)
+                if (K.find(c) != std::string::npos) KB += c;
+            auto cost = [&](const std::string& Ko) {
+                long long c = 0;
+                if (!(la == M + Ko || la == Ko + M) && !cached(A, la, M + Ko)) c += size(la);
+                if (!(lb == Ko + N || lb == N + Ko) && !cached(B, lb, Ko + N)) c += size(lb);
+                return c;
+            };
+            if (KB != K && cost(KB) < cost(K)) K = KB;
+        }
         const long long m = size(M), n = size(N), k = size(K);
         if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return bad("cc contract: dimension too large");
         const double* Ap = A;
@@ -184,16 +220,39 @@ struct Engine {
         if (la == M + K) { sam = (long)k; sak = 1; }
         else if (la == K + M) { sam = 1; sak = (long)m; }
         else {
-            if (permute(1.0, A, la, 0.0, w0, M + K)) return -1;
-            Ap = w0; sam = (long)k; sak = 1;
+            if (!(Ap = layout(A, la, M + K, w0))) return -1;
+            sam = (long)k; sak = 1;
         }
         const double* Bp = B;
         long sbk, sbn;
         if (lb == K + N) { sbk = (long)n; sbn = 1; }
         else if (lb == N + K) { sbk = 1; sbn = (long)k; }
         else {
-            if (permute(1.0, B, lb, 0.0, w1, K + N)) return -1;
-            Bp = w1; sbk = (long)n; sbn = 1;
+            if (!(Bp = layout(B, lb, K + N, w1))) return -1;
+            sbk = (long)n; sbn = 1;
+        }
+        flops += 2.0 * (double)m * (double)n * (double)k;
+        // Split-K for a small output with a long reduction (the GEMM spreads
+        // only output tiles over the chip): ns chunks of kc as one batched
+        // product into partials, the remainder as one more, summed in order.
+        const long long tiles = ((m + 15) / 16) * ((n + 15) / 16);
+        long long ns = 1;
+        if (k >= 2048 && tiles < 1024) {
+            ns = std::min<long long>(k / 512, std::max<long long>(1, 4096 / tiles));
+            ns = std::min<long long>(ns, kSplitMax);
+            while (ns > 1 && (ns + 1) * m * n > kSplitBuf) --ns;
+        }
+        if (ns > 1) {
+            const long long kc = k / ns, rem = k - ns * kc;
+            if (polyeri_gemm_launch(GEMM_F64, (int)m, (int)n, (int)kc, Ap, sam, sak, (long)(kc * sak), Bp, sbk, sbn,
+                                    (long)(kc * sbk), wk, (long)n, 1, (long)(m * n), (int)ns, s))
+                return bad("cc contract: split-K GEMM launch failed for " + la + "," + lb + "->" + lc);
+            if (rem > 0 && polyeri_gemm_launch(GEMM_F64, (int)m, (int)n, (int)rem, Ap + ns * kc * sak, sam, sak, 0,
+                                               Bp + ns * kc * sbk, sbk, sbn, 0, wk + ns * m * n, (long)n, 1, 0, 1, s))
+                return bad("cc contract: split-K remainder launch failed");
+            hipLaunchKernelGGL(k_cc_sumk, dim3(grid_of(m * n)), dim3(256), 0, s, wk, (int)(ns + (rem > 0)), m * n, w2);
+            if (hipGetLastError() != hipSuccess) return bad("cc split-K sum launch failed");
+            return permute(alpha, w2, M + N, beta, C, lc);
         }
         const bool direct = alpha == 1.0 && beta == 0.0 && (lc == M + N || lc == N + M);
         double* Cp = direct ? C : w2;
@@ -204,6 +263,29 @@ struct Engine {
             return bad("cc contract: GEMM launch failed for " + la + "," + lb + "->" + lc);
         if (!direct) return permute(alpha, w2, M + N, beta, C, lc);
         return 0;
+    }
+
+    bool cached(const double* p, const std::string& from, const std::string& to) const {
+        return consts.count(p) && cache.count(key(p, from, to));
+    }
+    static std::string key(const double* p, const std::string& from, const std::string& to) {
+        return std::to_string((unsigned long long)(uintptr_t)p) + ":" + from + ">" + to;
+    }
+    // p[from] transposed to the order `to`: a cached copy for a constant
+    // tensor (made on first use), else a transpose into the workspace ws.
+    const double* layout(const double* p, const std::string& from, const std::string& to, double* ws) {
+        if (!consts.count(p)) return permute(1.0, p, from, 0.0, ws, to) ? nullptr : ws;
+        const std::string kk = key(p, from, to);
+        auto it = cache.find(kk);
+        if (it != cache.end()) return it->second;
+        double* q = dev->get((size_t)size(to));
+        if (!q) {
+            bad("cc: device allocation of a cached layout failed");
+            return nullptr;
+        }
+        if (permute(1.0, p, from, 0.0, q, to)) return nullptr;
+        cache[kk] = q;
+        return q;
     }
 
     int axpby(const double* x, double* y, long long n, double a, double b) {
@@ -272,12 +354,13 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     ndiis = std::max(1, std::min(ndiis, 16));
     const long long o2v2 = no * no * nv * nv, n1 = no * nv, n12 = n1 + o2v2;
     Dev dv;
-    Engine E{s, no, nv, nullptr, nullptr, nullptr, nullptr, err};
+    Engine E{s, no, nv, nullptr, nullptr, nullptr, nullptr, nullptr, err};
     double *ovov, *oovv, *ovvo, *ovoo, *ovvv, *O2, *V2, *L, *Lp, *dEps;
     double *T, *Tn, *R, *tau, *tk, *tl, *x, *u;
-    double *Foo, *Fvv, *Fov, *Loo, *Lvv, *Xoo, *Xov, *Woooo, *Wvvvv, *Wvoov, *Wvovo, *Vd, *Ed;
+    double *Foo, *Fvv, *Fov, *Loo, *Lvv, *Xoo, *Xov, *Woooo, *Wvoov, *Wvovo, *Vd, *Ed;
     const long long wn = big;
     if (!(E.w0 = dv.get(wn)) || !(E.w1 = dv.get(wn)) || !(E.w2 = dv.get(wn)) || !(E.part = dv.get(kRedBlocks)) ||
+        !(E.wk = dv.get(kSplitBuf)) ||
         !(ovov = dv.get(o2v2)) || !(oovv = dv.get(o2v2)) || !(ovvo = dv.get(o2v2)) ||
         !(ovoo = dv.get(no * no * no * nv)) || !(ovvv = dv.get(no * nv * nv * nv)) ||
         !(O2 = dv.get(no * no * no * no)) || !(V2 = dv.get(nv * nv * nv * nv)) || !(L = dv.get(o2v2)) ||
@@ -285,7 +368,7 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
         !(R = dv.get(n12)) || !(tau = dv.get(o2v2)) || !(tk = dv.get(o2v2)) || !(tl = dv.get(o2v2)) ||
         !(x = dv.get(o2v2)) || !(u = dv.get(wn)) || !(Foo = dv.get(no * no)) || !(Fvv = dv.get(nv * nv)) ||
         !(Fov = dv.get(n1)) || !(Loo = dv.get(no * no)) || !(Lvv = dv.get(nv * nv)) || !(Xoo = dv.get(no * no)) ||
-        !(Xov = dv.get(n1)) || !(Woooo = dv.get(no * no * no * no)) || !(Wvvvv = dv.get(nv * nv * nv * nv)) ||
+        !(Xov = dv.get(n1)) || !(Woooo = dv.get(no * no * no * no)) ||
         !(Wvoov = dv.get(o2v2)) || !(Wvovo = dv.get(o2v2)) || !(Vd = dv.get(ndiis * n12)) ||
         !(Ed = dv.get(ndiis * n12))) {
         if (err) *err = "cc: device allocation failed";
@@ -318,6 +401,11 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
              E.axpby(ovov, L, o2v2, 2.0, 0.0) || E.permute(-1.0, ovov, "ibja", 1.0, L, "iajb") ||
              E.permute(1.0, L, "iajb", 0.0, Lp, "ijab");
     if (rc) return -3;
+    E.dev = &dv;
+    for (const double* c : {(const double*)ovov, (const double*)oovv, (const double*)ovvo, (const double*)ovoo,
+                            (const double*)ovvv, (const double*)O2, (const double*)V2, (const double*)L,
+                            (const double*)Lp})
+        E.consts.insert(c);
     double* t1 = T;
     double* t2 = T + n1;
     double* r1 = R;
@@ -357,7 +445,9 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
+    double f_it = 0.0;
     for (it = 1; it <= maxiter; ++it) {
+        const double f0 = E.flops;
         // tau = t2 + t1 t1; tk[ikcd] = t2 + t1_ic t1_kd; tl[klac] = t2 + t1_ka t1_lc
         rc = E.axpby(t2, tau, o2v2, 1.0, 0.0) || E.contract(1.0, t1, "ia", t1, "jb", 1.0, tau, "ijab") ||
              E.axpby(t2, tk, o2v2, 1.0, 0.0) || E.contract(1.0, t1, "ic", t1, "kd", 1.0, tk, "ikcd") ||
@@ -388,22 +478,19 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
              E.contract(1.0, ovoo, "lcki", t1, "jc", 1.0, Woooo, "klij") ||
              E.contract(1.0, ovoo, "kclj", t1, "ic", 1.0, Woooo, "klij") ||
              E.contract(1.0, ovov, "kcld", tau, "ijcd", 1.0, Woooo, "klij") ||
-             E.axpby(V2, Wvvvv, nv * nv * nv * nv, 1.0, 0.0) ||
-             E.contract(-1.0, ovvv, "kdac", t1, "kb", 1.0, Wvvvv, "abcd") ||
-             E.contract(-1.0, ovvv, "kcbd", t1, "ka", 1.0, Wvvvv, "abcd") ||
              E.permute(1.0, ovvo, "kcai", 0.0, Wvoov, "akic") ||
              E.contract(1.0, ovvv, "kcad", t1, "id", 1.0, Wvoov, "akic") ||
              E.contract(-1.0, ovoo, "kcli", t1, "la", 1.0, Wvoov, "akic") ||
-             E.contract(-0.5, ovov, "ldkc", t2, "ilda", 1.0, Wvoov, "akic") ||
+             // y[ilda] = -t2/2 - t1_id t1_la, z = y + t2[ilad]: the amplitude
+             // combinations the ovov terms of W_voov and W_vovo share
+             E.contract(-1.0, t1, "id", t1, "la", 0.0, tl, "ilda") || E.axpby(t2, tl, o2v2, -0.5, 1.0) ||
+             E.axpby(tl, tk, o2v2, 1.0, 0.0) || E.permute(1.0, t2, "ilad", 1.0, tk, "ilda") ||
+             E.contract(1.0, ovov, "ldkc", tk, "ilda", 1.0, Wvoov, "akic") ||
              E.contract(-0.5, ovov, "lckd", t2, "ilad", 1.0, Wvoov, "akic") ||
-             E.contract(1.0, t1, "id", t1, "la", 0.0, x, "ilda") ||
-             E.contract(-1.0, ovov, "ldkc", x, "ilda", 1.0, Wvoov, "akic") ||
-             E.contract(1.0, ovov, "ldkc", t2, "ilad", 1.0, Wvoov, "akic") ||
              E.permute(1.0, oovv, "kiac", 0.0, Wvovo, "akci") ||
              E.contract(1.0, ovvv, "kdac", t1, "id", 1.0, Wvovo, "akci") ||
              E.contract(-1.0, ovoo, "lcki", t1, "la", 1.0, Wvovo, "akci") ||
-             E.contract(-0.5, ovov, "lckd", t2, "ilda", 1.0, Wvovo, "akci") ||
-             E.contract(-1.0, ovov, "lckd", x, "ilda", 1.0, Wvovo, "akci");
+             E.contract(1.0, ovov, "lckd", tl, "ilda", 1.0, Wvovo, "akci");
         // doubles residual
         rc = rc || E.permute(1.0, ovov, "iajb", 0.0, r2, "ijab") ||
              E.contract(1.0, ovvv, "iacb", t1, "jc", 0.0, x, "ijab") ||
@@ -413,14 +500,20 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
              E.contract(1.0, ovvo, "kcai", t1, "jc", 0.0, u, "kaij") ||
              E.contract(-1.0, u, "kaij", t1, "kb", 1.0, x, "ijab") || E.add_pair_sym(x, r2) ||
              E.contract(1.0, Woooo, "klij", tau, "klab", 1.0, r2, "ijab") ||
-             E.contract(1.0, tau, "ijcd", Wvvvv, "abcd", 1.0, r2, "ijab") ||
+             // W_vvvv = (ac|bd) - sum_k (kd|ac) t_kb - sum_k (kc|bd) t_ka against tau:
+             // the t1 parts as Y[kaij] = sum_cd (kd|ac) tau_ijcd, then -P(t_kb Y)
+             E.contract(1.0, tau, "ijcd", V2, "abcd", 1.0, r2, "ijab") ||
+             E.contract(1.0, ovvv, "kdac", tau, "ijcd", 0.0, u, "kaij") ||
+             E.contract(-1.0, u, "kaij", t1, "kb", 0.0, x, "ijab") || E.add_pair_sym(x, r2) ||
              E.contract(1.0, Lvv, "ac", t2, "ijcb", 0.0, x, "ijab") ||
              E.contract(-1.0, Loo, "ki", t2, "kjab", 1.0, x, "ijab") ||
-             E.contract(2.0, Wvoov, "akic", t2, "kjcb", 1.0, x, "ijab") ||
-             E.contract(-1.0, Wvovo, "akci", t2, "kjcb", 1.0, x, "ijab") ||
+             // 2 W_voov[akic] - W_vovo[akci] against the same amplitudes
+             E.axpby(Wvoov, tk, o2v2, 2.0, 0.0) || E.permute(-1.0, Wvovo, "akci", 1.0, tk, "akic") ||
+             E.contract(1.0, tk, "akic", t2, "kjcb", 1.0, x, "ijab") ||
              E.contract(-1.0, Wvoov, "akic", t2, "kjbc", 1.0, x, "ijab") ||
              E.contract(-1.0, Wvovo, "bkci", t2, "kjac", 1.0, x, "ijab") || E.add_pair_sym(x, r2);
         if (rc || denom(R, Tn, n12)) return -3;
+        f_it = E.flops - f0;
         // DIIS over [t1 | t2]: vector Tn, error Tn - T
         double* vs = Vd + slot * n12;
         double* es = Ed + slot * n12;
@@ -464,6 +557,7 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     out->iterations = std::min(it, maxiter);
     out->iter_ms = ms / std::max(1, out->iterations);
     out->residual = resid;
+    out->iter_gflop = f_it * 1e-9;
     // E(SINGLES) = sum L_iajb t1_ia t1_jb, E(DOUBLES) = sum L t2
     if (E.contract(1.0, L, "iajb", t1, "jb", 0.0, Xov, "ia") || E.reduce(Xov, t1, n1, 0, &out->e_singles) ||
         E.reduce(Lp, t2, o2v2, 0, &out->e_doubles))

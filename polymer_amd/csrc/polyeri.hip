@@ -3015,6 +3015,7 @@ int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icor
     res->tensor_ms = mst;
     res->transform_ms = msx;
     res->iter_ms = o.iter_ms;
+    res->iter_gflop = o.iter_gflop;
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!std::isfinite(res->e_corr)) return fail(POLYERI_E_NONFINITE, "polyeri_ccsd: non-finite energy");
     return 0;
