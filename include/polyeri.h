@@ -366,6 +366,13 @@ typedef struct {
 } polyeri_cc_result;
 int polyeri_ccsd(polyeri_ctx *ctx, const double *co, const double *eps, int icore, int iocc, int nall, int ivirtcore,
                  double conv, int maxiter, int ndiis, polyeri_cc_result *res);
+/* COUPLED_CLUSTER_ENERGY with COPTN(70) = method ("CCSD", "CCD", "LCCD",
+ * "LCCSD" or "QCISD"; src/pcc.f90:39-66): the same solver with the terms of
+ * that method; E(CORRELATION) is singles + doubles for CCSD and the doubles
+ * part for the others (src/pcc.f90:202-207).  Other names fail with
+ * "UNKNOWN OR UNIMPLEMENTED COUPLED-CLUSTER METHOD". */
+int polyeri_cc(polyeri_ctx *ctx, const char *method, const double *co, const double *eps, int icore, int iocc,
+               int nall, int ivirtcore, double conv, int maxiter, int ndiis, polyeri_cc_result *res);
 /* One product C = A B on those kernels (transform = POLYERI_GEMM_MFMA_F64 or
  * _BF16X3); A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn],
  * C(m,n) = c[m*scm + n*scn]; device pointers. */

@@ -12,14 +12,20 @@ COPTN(70) = 'CCSD':
   from the first-order doubles, E(3) = sum L t2(2) with t2(2) the linear
   doubles residual of t2(1) over the denominators (singles enter at fourth
   order for canonical orbitals);
-* ccsd: the spin-restricted CCSD iteration (src/pcc.f90:120-240, the kappa
+* ccsd: the spin-restricted CC iteration (src/pcc.f90:120-240, the kappa
   and amplitude subroutines :1170-3094), here as the spin-adapted closed-shell
   equations with the usual one- and two-body intermediates (F_oo, F_vv, F_ov,
   L_oo, L_vv, W_oooo, W_vvvv, W_voov, W_vovo) for canonical HF orbitals, and
   commutator-free DIIS over the amplitude vectors; E(SINGLES) and
   E(DOUBLES) as ENERGY_T1_CONTRIBUTION / ENERGY_T2_CONTRIBUTION
   (src/pcc.f90:1041-1166): sum L_ijab t_i^a t_j^b and sum L_ijab t_ij^ab with
-  L_ijab = 2 (ia|jb) - (ib|ja).
+  L_ijab = 2 (ia|jb) - (ib|ja);
+* CCD, LCCD, LCCSD, QCISD (COPTN(70), src/pcc.f90:39-66): the components of
+  the CCSD residual of given degrees (p, q) in (t1, t2), extracted exactly
+  from CCSD residuals at integer-scaled amplitudes (residual_components):
+  CCD = CCSD at t1 = 0; LCCD its linear part; LCCSD the linear part in
+  (t1, t2); QCISD singles linear + t1 t2, doubles linear + t2^2.  E(CORRELATION)
+  is the doubles energy for all but CCSD (src/pcc.f90:202-207).
 
 Pinned by the shipped H2O/cc-pVDZ CCSD run (E(2), E(3), E(SINGLES),
 E(DOUBLES), E(CORRELATION); data/H2O_CCSD_ccpVDZ_out:391-451) and by
@@ -151,18 +157,69 @@ def ccsd_residual(I, t1, t2):
     return r1, r2
 
 
-def ccsd(g, eps, no, conv=1e-11, maxiter=200, ndiis=8):
-    """Closed-shell CCSD on the active MO integrals g (occupied first, no
-    occupied) and orbital energies eps.  Returns a dict with e_singles,
-    e_doubles, e_corr, iterations, t1, t2."""
+# Degree components of the CCSD residual kept by each method of COPTN(70)
+# (src/pcc.f90:39-66): (p, q) = powers of (t1, t2); None = no singles.
+METHODS = {
+    "CCSD": (None, None),
+    "CCD": ([], [(0, 0), (0, 1), (0, 2)]),
+    "LCCD": ([], [(0, 0), (0, 1)]),
+    "LCCSD": ([(0, 0), (1, 0), (0, 1)], [(0, 0), (1, 0), (0, 1)]),
+    "QCISD": ([(0, 0), (1, 0), (0, 1), (1, 1)], [(0, 0), (1, 0), (0, 1), (0, 2)]),
+}
+
+
+def residual_components(I, t1, t2):
+    """The CCSD residual as a polynomial in (lambda t1, mu t2): returns
+    C1[p][q], C2[p][q] (p <= 4, q <= 2) with R(lambda t1, mu t2) =
+    sum lambda^p mu^q C[p][q], from 5 x 3 exact integer nodes."""
+    lam = np.array([-2.0, -1.0, 0.0, 1.0, 2.0])
+    mu = np.array([-1.0, 0.0, 1.0])
+    Vl = np.vander(lam, 5, increasing=True)
+    Vm = np.vander(mu, 3, increasing=True)
+    Il, Im = np.linalg.inv(Vl), np.linalg.inv(Vm)
+    R1 = [[None] * 3 for _ in range(5)]
+    R2 = [[None] * 3 for _ in range(5)]
+    for a, x in enumerate(lam):
+        for b, y in enumerate(mu):
+            R1[a][b], R2[a][b] = ccsd_residual(I, x * t1, y * t2)
+    C1 = [[sum(Il[p, a] * Im[q, b] * R1[a][b] for a in range(5) for b in range(3)) for q in range(3)]
+          for p in range(5)]
+    C2 = [[sum(Il[p, a] * Im[q, b] * R2[a][b] for a in range(5) for b in range(3)) for q in range(3)]
+          for p in range(5)]
+    return C1, C2
+
+
+def method_residual(I, t1, t2, method):
+    """The residual of COPTN(70) = method: CCSD in full, the others as the
+    (p, q) components of the CCSD residual listed in METHODS."""
+    if method == "CCSD":
+        return ccsd_residual(I, t1, t2)
+    keep1, keep2 = METHODS[method]
+    C1, C2 = residual_components(I, t1, t2)
+    r1 = sum((C1[p][q] for p, q in keep1), np.zeros_like(t1))
+    r2 = sum((C2[p][q] for p, q in keep2), np.zeros_like(t2))
+    return r1, r2
+
+
+def ccsd(g, eps, no, conv=1e-11, maxiter=200, ndiis=8, method="CCSD"):
+    """Closed-shell coupled cluster (method: CCSD, CCD, LCCD, LCCSD, QCISD)
+    on the active MO integrals g (occupied first, no occupied) and orbital
+    energies eps.  Returns a dict with e_singles, e_doubles, e_corr (CCSD:
+    singles + doubles; the others: doubles, src/pcc.f90:202-207),
+    iterations, t1, t2."""
+    if method not in METHODS:
+        raise ValueError("UNKNOWN OR UNIMPLEMENTED COUPLED-CLUSTER METHOD")
     I = _Ints(np.asarray(g, dtype=np.float64), no)
     eps = np.asarray(eps, dtype=np.float64)
     d1, d2 = _denoms(eps, no)
     t1 = np.zeros_like(d1)
     t2 = np.zeros_like(d2)
     vecs, errs = [], []
+    singles = METHODS[method][0] is None or len(METHODS[method][0]) > 0
     for it in range(1, maxiter + 1):
-        r1, r2 = ccsd_residual(I, t1, t2)
+        r1, r2 = method_residual(I, t1, t2, method)
+        if not singles:
+            r1 = np.zeros_like(t1)
         n1, n2 = r1 / d1, r2 / d2
         dt = np.concatenate([(n1 - t1).ravel(), (n2 - t2).ravel()])
         vec = np.concatenate([n1.ravel(), n2.ravel()])
@@ -187,7 +244,8 @@ def ccsd(g, eps, no, conv=1e-11, maxiter=200, ndiis=8):
         if np.max(np.abs(dt)) < conv:
             break
     es, ed = _energy_parts(I, t1, t2)
-    return {"e_singles": es, "e_doubles": ed, "e_corr": es + ed, "iterations": it, "t1": t1, "t2": t2}
+    ec = es + ed if method == "CCSD" else ed
+    return {"e_singles": es, "e_doubles": ed, "e_corr": ec, "iterations": it, "t1": t1, "t2": t2}
 
 
 def fci_two_electron(g, eps):

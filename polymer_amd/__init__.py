@@ -389,16 +389,17 @@ class Context:
         check(lib().polyeri_mo_integrals(self.h, _p(co), int(lo), int(hi), _p(g)))
         return g
 
-    def ccsd(self, co, eps, icore, iocc, nall, ivirtcore=0, conv=1e-10, maxiter=100, diis=8):
-        """Spin-restricted CCSD (COUPLED_CLUSTER_ENERGY, COPTN(70) = 'CCSD') with
-        E(2) and E(3) on the device: returns a dict with e2, e3, e_singles,
-        e_doubles, e_corr, residual, iterations, converged and timings."""
+    def ccsd(self, co, eps, icore, iocc, nall, ivirtcore=0, conv=1e-10, maxiter=100, diis=8, method="CCSD"):
+        """Spin-restricted coupled cluster (COUPLED_CLUSTER_ENERGY, COPTN(70) =
+        method: CCSD, CCD, LCCD, LCCSD or QCISD) with E(2) and E(3) on the
+        device: returns a dict with e2, e3, e_singles, e_doubles, e_corr,
+        residual, iterations, converged and timings."""
         from ._lib import CcResult
         co = np.asfortranarray(co, dtype=np.float64)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         r = CcResult()
-        check(lib().polyeri_ccsd(self.h, _p(co), _p(eps), int(icore), int(iocc), int(nall), int(ivirtcore),
-                                 float(conv), int(maxiter), int(diis), ctypes.byref(r)))
+        check(lib().polyeri_cc(self.h, str(method).encode(), _p(co), _p(eps), int(icore), int(iocc), int(nall),
+                               int(ivirtcore), float(conv), int(maxiter), int(diis), ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in CcResult._fields_}
 
     def mp2_energy(self, co, eps, icore, iocc, nall, ivirtcore=0, transform=None):

@@ -27,7 +27,7 @@ EXPORTS = [
     "polyeri_rys_nodes",
     "polyeri_rys_block", "polyeri_eri_gradient", "polyeri_eri_gradient_host", "polyeri_one_electron_gradient",
     "polyeri_one_electron_gradient_host", "polyeri_mp2_energy_periodic_mod", "polyeri_mp2_energy_periodic_qp",
-    "polyeri_mo_integrals", "polyeri_ccsd",
+    "polyeri_mo_integrals", "polyeri_ccsd", "polyeri_cc",
 ]
 
 
@@ -118,6 +118,7 @@ def lib():
     L.polyeri_mp2_energy_qp.argtypes = [p, p, p, i, i, i, i, i, i, p, p, p]
     L.polyeri_mo_integrals.argtypes = [p, p, i, i, p]
     L.polyeri_ccsd.argtypes = [p, p, p, i, i, i, i, d, i, i, ctypes.POINTER(CcResult)]
+    L.polyeri_cc.argtypes = [p, ctypes.c_char_p, p, p, i, i, i, i, d, i, i, ctypes.POINTER(CcResult)]
     L.polyeri_store_fourindex_eri.argtypes = [p, ctypes.c_char_p, d, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri.argtypes = [ctypes.c_char_p, i, p, ctypes.POINTER(ctypes.c_longlong)]
     L.polyeri_restore_fourindex_eri_cells.argtypes = [ctypes.c_char_p, i, i, i, p, ctypes.POINTER(ctypes.c_longlong)]

@@ -11,7 +11,9 @@
 // operands are first transposed by a gather kernel.  The equations are the
 // spin-adapted closed-shell CCSD for canonical HF orbitals written with the
 // intermediates F_oo, F_vv, F_ov, L_oo, L_vv, W_oooo, W_vvvv, W_voov, W_vovo;
-// DIIS over the amplitude vectors.  Labels i, j, k, l run over the occupied
+// DIIS over the amplitude vectors.  CCD, LCCD, LCCSD and QCISD run the same
+// code with term switches (no singles; no t2^2 terms; no t1-nonlinear or
+// t1 t2 doubles terms).  Labels i, j, k, l run over the occupied
 // orbitals, a, b, c, d over the virtual ones; integrals are in chemists'
 // notation, (ia|jb) stored [i][a][j][b].
 #include "kcc.h"
@@ -338,9 +340,17 @@ bool solve_dense(std::vector<double> A, std::vector<double>& x, int n) {
 
 }  // namespace
 
-int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, double conv, int maxiter, int ndiis,
-                     CcSolveOut* out, hipStream_t s, std::string* err) {
+int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, int method, double conv, int maxiter,
+                     int ndiis, CcSolveOut* out, hipStream_t s, std::string* err) {
     const long long no = no_, nv = nmo - no_, n = nmo;
+    if (method < CC_CCSD || method > CC_QCISD) {
+        if (err) *err = "UNKNOWN OR UNIMPLEMENTED COUPLED-CLUSTER METHOD";
+        return -1;
+    }
+    // singles; t2^2 terms; t1-nonlinear and t1 t2 doubles terms (src/pcc.f90:39-66)
+    const bool S = method == CC_CCSD || method == CC_LCCSD || method == CC_QCISD;
+    const bool Q2 = method == CC_CCSD || method == CC_CCD || method == CC_QCISD;
+    const bool NL = method == CC_CCSD;
     if (no < 1 || nv < 1 || !out || !eps) {
         if (err) *err = "cc: empty occupied or virtual space";
         return -1;
@@ -356,7 +366,7 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     Dev dv;
     Engine E{s, no, nv, nullptr, nullptr, nullptr, nullptr, nullptr, err};
     double *ovov, *oovv, *ovvo, *ovoo, *ovvv, *O2, *V2, *L, *Lp, *dEps;
-    double *T, *Tn, *R, *tau, *tk, *tl, *x, *u;
+    double *T, *Tn, *R, *tau, *tauF, *tk, *tl, *x, *u;
     double *Foo, *Fvv, *Fov, *Loo, *Lvv, *Xoo, *Xov, *Woooo, *Wvoov, *Wvovo, *Vd, *Ed;
     const long long wn = big;
     if (!(E.w0 = dv.get(wn)) || !(E.w1 = dv.get(wn)) || !(E.w2 = dv.get(wn)) || !(E.part = dv.get(kRedBlocks)) ||
@@ -365,7 +375,7 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
         !(ovoo = dv.get(no * no * no * nv)) || !(ovvv = dv.get(no * nv * nv * nv)) ||
         !(O2 = dv.get(no * no * no * no)) || !(V2 = dv.get(nv * nv * nv * nv)) || !(L = dv.get(o2v2)) ||
         !(Lp = dv.get(o2v2)) || !(dEps = dv.get(n)) || !(T = dv.get(n12)) || !(Tn = dv.get(n12)) ||
-        !(R = dv.get(n12)) || !(tau = dv.get(o2v2)) || !(tk = dv.get(o2v2)) || !(tl = dv.get(o2v2)) ||
+        !(R = dv.get(n12)) || !(tau = dv.get(o2v2)) || !(tauF = dv.get(o2v2)) || !(tk = dv.get(o2v2)) || !(tl = dv.get(o2v2)) ||
         !(x = dv.get(o2v2)) || !(u = dv.get(wn)) || !(Foo = dv.get(no * no)) || !(Fvv = dv.get(nv * nv)) ||
         !(Fov = dv.get(n1)) || !(Loo = dv.get(no * no)) || !(Lvv = dv.get(nv * nv)) || !(Xoo = dv.get(no * no)) ||
         !(Xov = dv.get(n1)) || !(Woooo = dv.get(no * no * no * no)) ||
@@ -448,65 +458,72 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     double f_it = 0.0;
     for (it = 1; it <= maxiter; ++it) {
         const double f0 = E.flops;
-        // tau = t2 + t1 t1; tk[ikcd] = t2 + t1_ic t1_kd; tl[klac] = t2 + t1_ka t1_lc
-        rc = E.axpby(t2, tau, o2v2, 1.0, 0.0) || E.contract(1.0, t1, "ia", t1, "jb", 1.0, tau, "ijab") ||
-             E.axpby(t2, tk, o2v2, 1.0, 0.0) || E.contract(1.0, t1, "ic", t1, "kd", 1.0, tk, "ikcd") ||
-             E.axpby(t2, tl, o2v2, 1.0, 0.0) || E.contract(1.0, t1, "ka", t1, "lc", 1.0, tl, "klac");
+        // Term switches of the method: S singles; q2 the terms quadratic in
+        // t2; nl the terms nonlinear in t1 and the t1 t2 terms of the doubles
+        // (CCSD keeps all).  K skips a term whose coefficient is zero.
+        auto K = [&](double a, const double* A, const char* la, const double* B, const char* lb, double b, double* C,
+                     const char* lc) -> int {
+            if (a != 0.0) return E.contract(a, A, la, B, lb, b, C, lc);
+            return b == 0.0 ? zero(C, E.size(lc)) : 0;
+        };
+        const double q2 = Q2 ? 1.0 : 0.0, nl = NL ? 1.0 : 0.0, fF = (Q2 || NL) ? 1.0 : 0.0;
+        // tau = t2 + nl t1 t1 (against W_oooo and (ac|bd)); tauF = q2 t2 + nl t1 t1
+        // (the F intermediates); tk[ikcd] = t2 + nl t1_ic t1_kd; tl[klac] = t2 + nl t1_ka t1_lc
+        rc = E.axpby(t2, tau, o2v2, 1.0, 0.0) || K(nl, t1, "ia", t1, "jb", 1.0, tau, "ijab") ||
+             E.axpby(t2, tauF, o2v2, q2, 0.0) || K(nl, t1, "ia", t1, "jb", 1.0, tauF, "ijab");
+        if (S)
+            rc = rc || E.axpby(t2, tk, o2v2, 1.0, 0.0) || K(nl, t1, "ic", t1, "kd", 1.0, tk, "ikcd") ||
+                 E.axpby(t2, tl, o2v2, 1.0, 0.0) || K(nl, t1, "ka", t1, "lc", 1.0, tl, "klac");
         // one-body intermediates
-        rc = rc || E.contract(1.0, L, "kcld", tau, "ilcd", 0.0, Foo, "ki") ||
-             E.contract(-1.0, L, "kcld", tau, "klad", 0.0, Fvv, "ac") ||
-             E.contract(1.0, L, "kcld", t1, "ld", 0.0, Fov, "kc") || E.axpby(Foo, Loo, no * no, 1.0, 0.0) ||
-             E.contract(2.0, ovoo, "lcki", t1, "lc", 1.0, Loo, "ki") ||
-             E.contract(-1.0, ovoo, "kcli", t1, "lc", 1.0, Loo, "ki") || E.axpby(Fvv, Lvv, nv * nv, 1.0, 0.0) ||
-             E.contract(2.0, ovvv, "kdac", t1, "kd", 1.0, Lvv, "ac") ||
-             E.contract(-1.0, ovvv, "kcad", t1, "kd", 1.0, Lvv, "ac");
+        rc = rc || K(fF, L, "kcld", tauF, "ilcd", 0.0, Foo, "ki") ||
+             K(-fF, L, "kcld", tauF, "klad", 0.0, Fvv, "ac") || E.axpby(Foo, Loo, no * no, 1.0, 0.0) ||
+             K(2.0 * nl, ovoo, "lcki", t1, "lc", 1.0, Loo, "ki") || K(-nl, ovoo, "kcli", t1, "lc", 1.0, Loo, "ki") ||
+             E.axpby(Fvv, Lvv, nv * nv, 1.0, 0.0) || K(2.0 * nl, ovvv, "kdac", t1, "kd", 1.0, Lvv, "ac") ||
+             K(-nl, ovvv, "kcad", t1, "kd", 1.0, Lvv, "ac");
         // singles residual
-        rc = rc || E.contract(1.0, Fvv, "ac", t1, "ic", 0.0, r1, "ia") ||
-             E.contract(-1.0, Foo, "ki", t1, "ka", 1.0, r1, "ia") ||
-             E.contract(2.0, Fov, "kc", t2, "kica", 1.0, r1, "ia") ||
-             E.contract(-1.0, Fov, "kc", t2, "ikca", 1.0, r1, "ia") ||
-             E.contract(1.0, Fov, "kc", t1, "ic", 0.0, Xoo, "ki") ||
-             E.contract(1.0, Xoo, "ki", t1, "ka", 1.0, r1, "ia") ||
-             E.contract(2.0, ovvo, "kcai", t1, "kc", 1.0, r1, "ia") ||
-             E.contract(-1.0, oovv, "kiac", t1, "kc", 1.0, r1, "ia") ||
-             E.contract(2.0, ovvv, "kdac", tk, "ikcd", 1.0, r1, "ia") ||
-             E.contract(-1.0, ovvv, "kcad", tk, "ikcd", 1.0, r1, "ia") ||
-             E.contract(-2.0, ovoo, "lcki", tl, "klac", 1.0, r1, "ia") ||
-             E.contract(1.0, ovoo, "kcli", tl, "klac", 1.0, r1, "ia");
+        if (S)
+            rc = rc || K(fF, L, "kcld", t1, "ld", 0.0, Fov, "kc") || K(fF, Fvv, "ac", t1, "ic", 0.0, r1, "ia") ||
+                 K(-fF, Foo, "ki", t1, "ka", 1.0, r1, "ia") || K(2.0 * q2, Fov, "kc", t2, "kica", 1.0, r1, "ia") ||
+                 K(-q2, Fov, "kc", t2, "ikca", 1.0, r1, "ia") || K(nl, Fov, "kc", t1, "ic", 0.0, Xoo, "ki") ||
+                 K(nl, Xoo, "ki", t1, "ka", 1.0, r1, "ia") ||
+                 E.contract(2.0, ovvo, "kcai", t1, "kc", 1.0, r1, "ia") ||
+                 E.contract(-1.0, oovv, "kiac", t1, "kc", 1.0, r1, "ia") ||
+                 E.contract(2.0, ovvv, "kdac", tk, "ikcd", 1.0, r1, "ia") ||
+                 E.contract(-1.0, ovvv, "kcad", tk, "ikcd", 1.0, r1, "ia") ||
+                 E.contract(-2.0, ovoo, "lcki", tl, "klac", 1.0, r1, "ia") ||
+                 E.contract(1.0, ovoo, "kcli", tl, "klac", 1.0, r1, "ia");
+        else
+            rc = rc || zero(r1, n1);
         // two-body intermediates
         rc = rc || E.axpby(O2, Woooo, no * no * no * no, 1.0, 0.0) ||
-             E.contract(1.0, ovoo, "lcki", t1, "jc", 1.0, Woooo, "klij") ||
-             E.contract(1.0, ovoo, "kclj", t1, "ic", 1.0, Woooo, "klij") ||
-             E.contract(1.0, ovov, "kcld", tau, "ijcd", 1.0, Woooo, "klij") ||
+             K(nl, ovoo, "lcki", t1, "jc", 1.0, Woooo, "klij") || K(nl, ovoo, "kclj", t1, "ic", 1.0, Woooo, "klij") ||
+             K(fF, ovov, "kcld", tauF, "ijcd", 1.0, Woooo, "klij") ||
              E.permute(1.0, ovvo, "kcai", 0.0, Wvoov, "akic") ||
-             E.contract(1.0, ovvv, "kcad", t1, "id", 1.0, Wvoov, "akic") ||
-             E.contract(-1.0, ovoo, "kcli", t1, "la", 1.0, Wvoov, "akic") ||
-             // y[ilda] = -t2/2 - t1_id t1_la, z = y + t2[ilad]: the amplitude
+             K(nl, ovvv, "kcad", t1, "id", 1.0, Wvoov, "akic") || K(-nl, ovoo, "kcli", t1, "la", 1.0, Wvoov, "akic") ||
+             // y[ilda] = -q2 t2/2 - nl t1_id t1_la, z = y + q2 t2[ilad]: the amplitude
              // combinations the ovov terms of W_voov and W_vovo share
-             E.contract(-1.0, t1, "id", t1, "la", 0.0, tl, "ilda") || E.axpby(t2, tl, o2v2, -0.5, 1.0) ||
-             E.axpby(tl, tk, o2v2, 1.0, 0.0) || E.permute(1.0, t2, "ilad", 1.0, tk, "ilda") ||
-             E.contract(1.0, ovov, "ldkc", tk, "ilda", 1.0, Wvoov, "akic") ||
-             E.contract(-0.5, ovov, "lckd", t2, "ilad", 1.0, Wvoov, "akic") ||
+             K(-nl, t1, "id", t1, "la", 0.0, tl, "ilda") || E.axpby(t2, tl, o2v2, -0.5 * q2, 1.0) ||
+             E.axpby(tl, tk, o2v2, 1.0, 0.0) || (q2 != 0.0 && E.permute(q2, t2, "ilad", 1.0, tk, "ilda")) ||
+             K(fF, ovov, "ldkc", tk, "ilda", 1.0, Wvoov, "akic") ||
+             K(-0.5 * q2, ovov, "lckd", t2, "ilad", 1.0, Wvoov, "akic") ||
              E.permute(1.0, oovv, "kiac", 0.0, Wvovo, "akci") ||
-             E.contract(1.0, ovvv, "kdac", t1, "id", 1.0, Wvovo, "akci") ||
-             E.contract(-1.0, ovoo, "lcki", t1, "la", 1.0, Wvovo, "akci") ||
-             E.contract(1.0, ovov, "lckd", tl, "ilda", 1.0, Wvovo, "akci");
+             K(nl, ovvv, "kdac", t1, "id", 1.0, Wvovo, "akci") || K(-nl, ovoo, "lcki", t1, "la", 1.0, Wvovo, "akci") ||
+             K(fF, ovov, "lckd", tl, "ilda", 1.0, Wvovo, "akci");
         // doubles residual
-        rc = rc || E.permute(1.0, ovov, "iajb", 0.0, r2, "ijab") ||
-             E.contract(1.0, ovvv, "iacb", t1, "jc", 0.0, x, "ijab") ||
-             E.contract(1.0, oovv, "kibc", t1, "jc", 0.0, u, "kibj") ||
-             E.contract(-1.0, u, "kibj", t1, "ka", 1.0, x, "ijab") || E.add_pair_sym(x, r2) ||
-             E.contract(-1.0, ovoo, "iajk", t1, "kb", 0.0, x, "ijab") ||
-             E.contract(1.0, ovvo, "kcai", t1, "jc", 0.0, u, "kaij") ||
-             E.contract(-1.0, u, "kaij", t1, "kb", 1.0, x, "ijab") || E.add_pair_sym(x, r2) ||
-             E.contract(1.0, Woooo, "klij", tau, "klab", 1.0, r2, "ijab") ||
+        rc = rc || E.permute(1.0, ovov, "iajb", 0.0, r2, "ijab");
+        if (S)
+            rc = rc || E.contract(1.0, ovvv, "iacb", t1, "jc", 0.0, x, "ijab") ||
+                 K(nl, oovv, "kibc", t1, "jc", 0.0, u, "kibj") || K(-nl, u, "kibj", t1, "ka", 1.0, x, "ijab") ||
+                 E.add_pair_sym(x, r2) || E.contract(-1.0, ovoo, "iajk", t1, "kb", 0.0, x, "ijab") ||
+                 K(nl, ovvo, "kcai", t1, "jc", 0.0, u, "kaij") || K(-nl, u, "kaij", t1, "kb", 1.0, x, "ijab") ||
+                 E.add_pair_sym(x, r2);
+        rc = rc || E.contract(1.0, Woooo, "klij", tau, "klab", 1.0, r2, "ijab") ||
              // W_vvvv = (ac|bd) - sum_k (kd|ac) t_kb - sum_k (kc|bd) t_ka against tau:
              // the t1 parts as Y[kaij] = sum_cd (kd|ac) tau_ijcd, then -P(t_kb Y)
              E.contract(1.0, tau, "ijcd", V2, "abcd", 1.0, r2, "ijab") ||
-             E.contract(1.0, ovvv, "kdac", tau, "ijcd", 0.0, u, "kaij") ||
-             E.contract(-1.0, u, "kaij", t1, "kb", 0.0, x, "ijab") || E.add_pair_sym(x, r2) ||
-             E.contract(1.0, Lvv, "ac", t2, "ijcb", 0.0, x, "ijab") ||
-             E.contract(-1.0, Loo, "ki", t2, "kjab", 1.0, x, "ijab") ||
+             (NL && (E.contract(1.0, ovvv, "kdac", tau, "ijcd", 0.0, u, "kaij") ||
+                     E.contract(-1.0, u, "kaij", t1, "kb", 0.0, x, "ijab") || E.add_pair_sym(x, r2))) ||
+             K(fF, Lvv, "ac", t2, "ijcb", 0.0, x, "ijab") || K(-fF, Loo, "ki", t2, "kjab", 1.0, x, "ijab") ||
              // 2 W_voov[akic] - W_vovo[akci] against the same amplitudes
              E.axpby(Wvoov, tk, o2v2, 2.0, 0.0) || E.permute(-1.0, Wvovo, "akci", 1.0, tk, "akic") ||
              E.contract(1.0, tk, "akic", t2, "kjcb", 1.0, x, "ijab") ||
@@ -562,5 +579,7 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, doub
     if (E.contract(1.0, L, "iajb", t1, "jb", 0.0, Xov, "ia") || E.reduce(Xov, t1, n1, 0, &out->e_singles) ||
         E.reduce(Lp, t2, o2v2, 0, &out->e_doubles))
         return -3;
+    // E(CORRELATION): singles + doubles for CCSD, the doubles part otherwise (src/pcc.f90:202-207)
+    out->e_corr = method == CC_CCSD ? out->e_singles + out->e_doubles : out->e_doubles;
     return 0;
 }

@@ -2986,13 +2986,18 @@ int polyeri_mo_integrals(polyeri_ctx* ctx, const double* co, int lo, int hi, dou
     return 0;
 }
 
-int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall, int ivirtcore,
-                 double conv, int maxiter, int ndiis, polyeri_cc_result* res) {
-    if (!ctx || !co || !eps || !res) return fail(POLYERI_E_ARG, "polyeri_ccsd: null argument");
+int polyeri_cc(polyeri_ctx* ctx, const char* method, const double* co, const double* eps, int icore, int iocc,
+               int nall, int ivirtcore, double conv, int maxiter, int ndiis, polyeri_cc_result* res) {
+    if (!ctx || !method || !co || !eps || !res) return fail(POLYERI_E_ARG, "polyeri_cc: null argument");
+    static const char* const names[] = {"CCSD", "CCD", "LCCD", "LCCSD", "QCISD"};
+    int m = -1;
+    for (int q = 0; q < 5; ++q)
+        if (std::string(method) == names[q]) m = q;
+    if (m < 0) return fail(POLYERI_E_ARG, "UNKNOWN OR UNIMPLEMENTED COUPLED-CLUSTER METHOD");
     const int hi = nall - ivirtcore, no = iocc - icore;
     if (icore < 0 || no < 1 || ivirtcore < 0 || hi <= iocc || nall > ctx->rb.ncgs)
-        return fail(POLYERI_E_ARG, "polyeri_ccsd: empty occupied or virtual space");
-    if (!(conv > 0.0) || maxiter < 1) return fail(POLYERI_E_ARG, "polyeri_ccsd: bad convergence request");
+        return fail(POLYERI_E_ARG, "polyeri_cc: empty occupied or virtual space");
+    if (!(conv > 0.0) || maxiter < 1) return fail(POLYERI_E_ARG, "polyeri_cc: bad convergence request");
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(ctx->device));
     DevBuf buf;
@@ -3002,13 +3007,13 @@ int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icor
     if (rc) return rc;
     CcSolveOut o{};
     std::string err;
-    rc = polyeri_cc_solve(dG, hi - icore, no, eps + icore, conv, maxiter, ndiis, &o, ctx->stream, &err);
-    if (rc) return fail(rc == -2 ? POLYERI_E_NOMEM : rc == -1 ? POLYERI_E_ARG : POLYERI_E_HIP, "polyeri_ccsd: " + err);
+    rc = polyeri_cc_solve(dG, hi - icore, no, eps + icore, m, conv, maxiter, ndiis, &o, ctx->stream, &err);
+    if (rc) return fail(rc == -2 ? POLYERI_E_NOMEM : rc == -1 ? POLYERI_E_ARG : POLYERI_E_HIP, "polyeri_cc: " + err);
     res->e2 = o.e2;
     res->e3 = o.e3;
     res->e_singles = o.e_singles;
     res->e_doubles = o.e_doubles;
-    res->e_corr = o.e_singles + o.e_doubles;
+    res->e_corr = o.e_corr;
     res->residual = o.residual;
     res->iterations = o.iterations;
     res->converged = o.converged;
@@ -3017,8 +3022,13 @@ int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icor
     res->iter_ms = o.iter_ms;
     res->iter_gflop = o.iter_gflop;
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (!std::isfinite(res->e_corr)) return fail(POLYERI_E_NONFINITE, "polyeri_ccsd: non-finite energy");
+    if (!std::isfinite(res->e_corr)) return fail(POLYERI_E_NONFINITE, "polyeri_cc: non-finite energy");
     return 0;
+}
+
+int polyeri_ccsd(polyeri_ctx* ctx, const double* co, const double* eps, int icore, int iocc, int nall, int ivirtcore,
+                 double conv, int maxiter, int ndiis, polyeri_cc_result* res) {
+    return polyeri_cc(ctx, "CCSD", co, eps, icore, iocc, nall, ivirtcore, conv, maxiter, ndiis, res);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,3 +85,37 @@ def test_ccsd_rejects_bad_ranges(h2o):
     ctx, r, _ = h2o
     with pytest.raises(pa.PolyeriError, match="empty occupied"):
         ctx.ccsd(r["co"], r["eps"], 5, 5, r["n_indep"])
+
+
+@pytest.mark.parametrize("method", ["CCD", "LCCD", "LCCSD", "QCISD"])
+def test_cc_methods_match_oracle(h2o, method):
+    """COPTN(70) = CCD, LCCD, LCCSD, QCISD (src/pcc.f90:39-66) on the device
+    vs the oracle's degree components of the CCSD residual, H2O/cc-pVDZ
+    frozen core, E(CORRELATION) (the doubles energy) within 1e-10 Eh."""
+    ctx, r, feri = h2o
+    n, iocc, icore = r["n_indep"], r["iocc"], 1
+    print(f"case: {method} H2O/cc-pVDZ icore {icore}", flush=True)
+    res = ctx.ccsd(r["co"], r["eps"], icore, iocc, n, conv=1e-11, maxiter=100, method=method)
+    g = cc.mo_integrals(feri, r["co"], icore, n)
+    ref = cc.ccsd(g, r["eps"][icore:n], iocc - icore, conv=1e-12, method=method)
+    print(f"{method}: corr {res['e_corr']:.12f} oracle {ref['e_corr']:.12f} iterations {res['iterations']}",
+          flush=True)
+    assert res["converged"] == 1
+    assert abs(res["e_corr"] - ref["e_corr"]) < 1e-10
+    assert abs(res["e_doubles"] - ref["e_doubles"]) < 1e-10
+    if method in ("CCD", "LCCD"):
+        assert res["e_singles"] == 0.0
+
+
+def test_qcisd_two_electron_exact():
+    """QCISD on the device is exact for two electrons (H2/6-31G**, 2.2 bohr)."""
+    atoms = [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.3, 2.2)]
+    print("case: QCISD = FCI, H2/6-31G**", flush=True)
+    ctx = pa.Context(pa.read_basis("6-31Gss", atoms), schwarz_tol=0.0)
+    r = ctx.self_consistent_field(scfconv=1e-12, lindep=1e-12, diis=2)
+    n = r["n_indep"]
+    res = ctx.ccsd(r["co"], r["eps"], 0, 1, n, conv=1e-11, method="QCISD")
+    efci = cc.fci_two_electron(ctx.mo_integrals(r["co"], 0, n), r["eps"][:n])
+    assert abs(res["e_corr"] - efci) < 1e-9
+    with pytest.raises(pa.PolyeriError, match="UNKNOWN OR UNIMPLEMENTED"):
+        ctx.ccsd(r["co"], r["eps"], 0, 1, n, method="D45")

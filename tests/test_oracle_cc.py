@@ -67,3 +67,83 @@ def test_ccsd_two_electron_exact(basis):
         efci = cc.fci_two_electron(g, r["eps"][:n])
         assert efci < -1e-3
         assert abs(res["e_corr"] - efci) < 1e-10, (res["e_corr"], efci)
+
+
+def test_residual_components_reassemble():
+    """The degree components of the CCSD residual sum back to the residual
+    at (t1, t2) and at (2 t1, -t2) (exact polynomial of degree 4 / 2)."""
+    rng = np.random.default_rng(3)
+    no, nv = 2, 4
+    n = no + nv
+    g = rng.standard_normal((n, n, n, n))
+    g = g + g.transpose(1, 0, 2, 3)
+    g = g + g.transpose(0, 1, 3, 2)
+    g = g + g.transpose(2, 3, 0, 1)
+    I = cc._Ints(g, no)
+    t1 = 0.3 * rng.standard_normal((no, nv))
+    t2 = rng.standard_normal((no, no, nv, nv))
+    t2 = 0.3 * (t2 + t2.transpose(1, 0, 3, 2))
+    C1, C2 = cc.residual_components(I, t1, t2)
+    for lam, mu in ((1.0, 1.0), (2.0, -1.0), (0.5, 0.7)):
+        r1, r2 = cc.ccsd_residual(I, lam * t1, mu * t2)
+        s1 = sum(lam ** p * mu ** q * C1[p][q] for p in range(5) for q in range(3))
+        s2 = sum(lam ** p * mu ** q * C2[p][q] for p in range(5) for q in range(3))
+        assert np.max(np.abs(s1 - r1)) < 1e-10 * max(1.0, np.max(np.abs(r1)))
+        assert np.max(np.abs(s2 - r2)) < 1e-10 * max(1.0, np.max(np.abs(r2)))
+    # no singles residual of degree 0 in t1 beyond the t2-linear one, none above t1^3
+    assert np.max(np.abs(C1[4][0])) < 1e-10 and np.max(np.abs(C1[0][2])) < 1e-10
+
+
+def test_lccd_is_the_linear_solve():
+    """LCCD is a linear system: the iterated amplitudes equal the direct
+    solution of D t - L_lin t = V (the linear doubles operator, itself
+    pinned by the golden E(3)), and E(LCCD) is near E(2) + E(3) (H2O/6-31G)."""
+    b = oracle.Basis(basis_path("6-31G"), M.H2O_GOLDEN)
+    r = b.scf(inttol=0.0, scfconv=1e-11, lindep=1e-12, diis=2, direct=False)
+    g = cc.mo_integrals(b.eri_tensor(), r["co"], 0, r["n_indep"])
+    no = r["iocc"]
+    e = r["eps"][:r["n_indep"]]
+    res = cc.ccsd(g, e, no, conv=1e-12, method="LCCD")
+    I = cc._Ints(g, no)
+    _, d2 = cc._denoms(e, no)
+    nv = d2.shape[2]
+    m = no * no * nv * nv
+    A = np.zeros((m, m))
+    for k in range(m):
+        u = np.zeros(m)
+        u[k] = 1.0
+        A[:, k] = cc._linear_doubles(I, u.reshape(d2.shape)).ravel()
+    t = np.linalg.solve(np.diag(d2.ravel()) - A, I.ovov.transpose(0, 2, 1, 3).ravel())
+    ed = float(np.einsum("iajb,ijab->", I.L, t.reshape(d2.shape)))
+    assert abs(res["e_corr"] - ed) < 1e-10
+    e2, e3 = cc.mp2(g, e, no), cc.mp3(g, e, no)
+    assert res["e_corr"] < e2 + e3 + 1e-3 and abs(res["e_corr"] - (e2 + e3)) < 5e-3
+
+
+@pytest.mark.parametrize("method", ["QCISD", "CCSD"])
+def test_two_electron_methods_exact(method):
+    """QCISD and CCSD are exact for two electrons (T1 T2 and T2^2 vanish),
+    H2/6-31G** stretched to 2.2 bohr; LCCSD and CCD are not."""
+    atoms = [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.3, 2.2)]
+    b = oracle.Basis(basis_path("6-31Gss"), atoms)
+    r = b.scf(inttol=0.0, scfconv=1e-12, lindep=1e-12, diis=2, direct=False)
+    n = r["n_indep"]
+    g = cc.mo_integrals(b.eri_tensor(), r["co"], 0, n)
+    efci = cc.fci_two_electron(g, r["eps"][:n])
+    res = cc.ccsd(g, r["eps"][:n], 1, conv=1e-12, method=method)
+    assert abs(res["e_corr"] - efci) < 1e-10
+    if method == "QCISD":
+        for other in ("LCCSD", "CCD"):
+            assert abs(cc.ccsd(g, r["eps"][:n], 1, conv=1e-12, method=other)["e_corr"] - efci) > 1e-6
+
+
+def test_ccd_two_electron_no_singles_exact():
+    """H2/STO-3G has one occupied and one virtual orbital of different
+    symmetry: no singles, so CCD (and LCCD is not) is exact."""
+    atoms = [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 1.4)]
+    b = oracle.Basis(basis_path("STO-3G"), atoms)
+    r = b.scf(inttol=0.0, scfconv=1e-12, lindep=1e-12, diis=2, direct=False)
+    g = cc.mo_integrals(b.eri_tensor(), r["co"], 0, 2)
+    efci = cc.fci_two_electron(g, r["eps"][:2])
+    assert abs(cc.ccsd(g, r["eps"][:2], 1, conv=1e-12, method="CCD")["e_corr"] - efci) < 1e-12
+    assert abs(cc.ccsd(g, r["eps"][:2], 1, conv=1e-12, method="LCCD")["e_corr"] - efci) > 1e-6
