@@ -962,17 +962,23 @@ k_eri_gc(EriArgs g) {
                 }
             }
             const double sq = bp == kp ? 0.5 : 1.0;
+            // ket member outer: its J_cd block is summed over the bra members
+            // in registers and added once
 #pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                if (i >= mb.m) break;
-                const int ia = (mb.code >> (2 * i)) & 1, ib = (mb.code >> (2 * i + 1)) & 1;
-                const int fa = ia ? mb.fa1 : mb.fa0, fb = ib ? mb.fb1 : mb.fb0;
+            for (int j = 0; j < MK; ++j) {
+                if (j >= mk.m) break;
+                const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
+                const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
+                double jcd[NC];
 #pragma unroll
-                for (int j = 0; j < MK; ++j) {
-                    if (j >= mk.m) break;
+                for (int c = 0; c < NC; ++c) jcd[c] = 0.0;
+                bool any = false;
+#pragma unroll
+                for (int i = 0; i < MB; ++i) {
+                    if (i >= mb.m) break;
                     if (mb.q[i] * mk.q[j] < g.qtol) continue;
-                    const int kc = (mk.code >> (2 * j)) & 1, kd = (mk.code >> (2 * j + 1)) & 1;
-                    const int fc = kc ? mk.fa1 : mk.fa0, fd = kd ? mk.fb1 : mk.fb0;
+                    const int ia = (mb.code >> (2 * i)) & 1, ib = (mb.code >> (2 * i + 1)) & 1;
+                    const int fa = ia ? mb.fa1 : mb.fa0, fb = ib ? mb.fb1 : mb.fb0;
                     if (screen) {
                         const size_t ns = g.nsh;
                         const int a = ia ? mb.sa1 : mb.sa0, bb = ib ? mb.sb1 : mb.sb0;
@@ -1006,7 +1012,12 @@ k_eri_gc(EriArgs g) {
                     for (int c = 0; c < NC; ++c) atomicAdd(&rowb[fc + c], s * cb.Kbc[c]);
                     atomicAdd(&rowb[fd], s * cb.Kbd[0]);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) jk_add(&J[(size_t)(fc + c) * n + fd], s * cb.Jcd[c], g.jk_copies);
+                    for (int c = 0; c < NC; ++c) jcd[c] += s * cb.Jcd[c];
+                    any = true;
+                }
+                if (any) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) jk_add(&J[(size_t)(fc + c) * n + fd], jcd[c], g.jk_copies);
                 }
             }
         }
