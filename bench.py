@@ -188,7 +188,7 @@ def main():
     ap.add_argument("--mp2-transforms", default="f64,rocblas", help="comma list of f64, rocblas, bf16x3")
     ap.add_argument("--mp2-only", action="store_true", help="print the MP2 line alone (for profiling runs)")
     ap.add_argument("--no-ccsd", action="store_true", help="skip the CCSD line")
-    ap.add_argument("--ccsd-nwater", type=int, default=4)
+    ap.add_argument("--ccsd-nwater", type=int, default=6)
     ap.add_argument("--ccsd-only", action="store_true", help="print the CCSD line alone (for profiling runs)")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="run the library's copy kernel over this many MiB first (counter scaling)")

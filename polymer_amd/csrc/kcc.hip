@@ -355,10 +355,11 @@ int polyeri_cc_solve(const double* dG, int nmo, int no_, const double* eps, int 
         if (err) *err = "cc: empty occupied or virtual space";
         return -1;
     }
-    // every tensor must fit the GEMM's 2 GiB buffer records
-    const long long big = std::max({nv * nv * nv * nv, no * nv * nv * nv, no * no * nv * nv, n * n * n * n});
-    if (big * 8 >= (1LL << 31)) {
-        if (err) *err = "cc: active space too large for one pass (nmo^4 or nv^4 beyond 2 GiB)";
+    // the transposes index with 32-bit unsigned integers: every tensor,
+    // the MO integrals included, below 2^31 elements
+    const long long big = std::max({nv * nv * nv * nv, no * nv * nv * nv, no * no * nv * nv, no * no * no * no});
+    if (std::max(big, n * n * n * n) >= (1LL << 31)) {
+        if (err) *err = "cc: active space too large for one pass (nmo^4 or nv^4 elements beyond 2^31)";
         return -1;
     }
     ndiis = std::max(1, std::min(ndiis, 16));

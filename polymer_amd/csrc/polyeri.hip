@@ -2937,8 +2937,8 @@ static int mo_tensor_device(polyeri_ctx* ctx, const double* co, int lo, int hi, 
     const int n = ctx->ib.nbf, nc = ctx->rb.ncgs, m = hi - lo;
     if (lo < 0 || m < 1 || hi > nc) return fail(POLYERI_E_ARG, "MO integrals: bad orbital range");
     const long N = n, M = m, N3 = (long)n * n * n, M2 = (long)m * m;
-    if ((double)m * n * n * n * 8 >= 2147483648.0 || (double)M2 * M2 * 8 >= 2147483648.0)
-        return fail(POLYERI_E_ARG, "MO integrals: transform intermediates beyond 2 GiB");
+    if ((double)M2 * M2 >= 2147483648.0)
+        return fail(POLYERI_E_ARG, "MO integrals: more than 2^31 MO integrals");
     hipStream_t s = ctx->stream;
     double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT;
     int rc;
