@@ -354,7 +354,9 @@ int polyeri_mo_integrals(polyeri_ctx *ctx, const double *co, int lo, int hi, dou
  * closed-shell HF orbitals (co, eps as polyeri_mp2_energy), every amplitude
  * contraction on the device's FP64 MFMA GEMM, DIIS over ndiis vectors
  * (IOPTN(54)), converged when max |t_new - t_old| < conv.  E(SINGLES) and
- * E(DOUBLES) as the reference prints them (src/pcc.f90:1041-1166). */
+ * E(DOUBLES) as the reference prints them (src/pcc.f90:1041-1166).
+ * Returns POLYERI_E_NOTCONV ("CC ITERATION DID NOT CONVERGE") after maxiter
+ * iterations without convergence, with res holding the last iterate. */
 typedef struct {
     double e2, e3;                /* E(2), E(3) */
     double e_singles, e_doubles;  /* sum L t1 t1, sum L t2 (L_iajb = 2 (ia|jb) - (ib|ja)) */

@@ -3023,6 +3023,8 @@ int polyeri_cc(polyeri_ctx* ctx, const char* method, const double* co, const dou
     res->iter_gflop = o.iter_gflop;
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!std::isfinite(res->e_corr)) return fail(POLYERI_E_NONFINITE, "polyeri_cc: non-finite energy");
+    // the reference aborts here (src/pcc.f90:218); res holds the last iterate
+    if (!res->converged) return fail(POLYERI_E_NOTCONV, "CC ITERATION DID NOT CONVERGE");
     return 0;
 }
 

@@ -85,6 +85,8 @@ def test_ccsd_rejects_bad_ranges(h2o):
     ctx, r, _ = h2o
     with pytest.raises(pa.PolyeriError, match="empty occupied"):
         ctx.ccsd(r["co"], r["eps"], 5, 5, r["n_indep"])
+    with pytest.raises(pa.PolyeriError, match="CC ITERATION DID NOT CONVERGE"):
+        ctx.ccsd(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], conv=1e-14, maxiter=3)
 
 
 @pytest.mark.parametrize("method", ["CCD", "LCCD", "LCCSD", "QCISD"])

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 EH = 1e-8
 
 
-def test_stored_h2o_jk_scf_mp2_golden(golden, tmp_path):
+def test_stored_h2o_jk_scf_mp2_ccsd_golden(golden, tmp_path):
     g = golden["H2O_ccpVDZ_spherical"]
     print("case: DIRECT .F. H2O/cc-pVDZ spherical golden", flush=True)
     b = pa.read_basis("cc-pVDZ", M.H2O_GOLDEN)
@@ -46,6 +46,12 @@ def test_stored_h2o_jk_scf_mp2_golden(golden, tmp_path):
     print(f"E {r['e_total']:.12f} (shipped {g['e_total']:.12f}), E(2) {e2s + e2t:.15f} "
           f"(shipped {g['e2_cc']:.15f})", flush=True)
     assert abs(e2s + e2t - g["e2_cc"]) < EH
+    # the rest of the shipped run: E(3) and CCSD from the same stored integrals
+    c = ctx.ccsd(r["co"], r["eps"], 0, r["iocc"], r["n_indep"], conv=1e-11)
+    print(f"E(3) {c['e3']:.15f} (shipped {g['e3_cc']:.15f}), E(CORRELATION) {c['e_corr']:.15f} "
+          f"(shipped {g['e_cc_corr']:.15f})", flush=True)
+    assert abs(c["e3"] - g["e3_cc"]) < EH and abs(c["e_corr"] - g["e_cc_corr"]) < EH
+    assert abs(r["e_total"] + c["e_corr"] - g["e_ccsd"]) < EH
     ctx.set_direct(True)                               # back to the direct build
     j2, _ = ctx.fourindex_eri(p)
     assert np.max(np.abs(j2 - jd)) < 1e-12
