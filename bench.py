@@ -368,7 +368,7 @@ def main():
         scf["note"] = ("fp64_reference: FP64, purification + commutator DIIS to RMS density change 1e-10 (energy of "
                        "the idempotent input density with its own J/K); the reference algorithm's energy at RMS 1e-8 "
                        "(fp64_plain) carries an error of first order in the last density change")
-        scf["north_star"] = {"target_speedup": 5.0, "speedup": scf["combined"]["wall_speedup_vs_fp64_plain"],
+        scf["speedup_target"] = {"target_speedup": 5.0, "speedup": scf["combined"]["wall_speedup_vs_fp64_plain"],
                              "dE_vs_fp64_reference": scf["combined"]["dE_vs_fp64_reference"],
                              "target_dE": 1e-6}
 
