@@ -95,29 +95,37 @@ def optimize(evaluate, x0, optconv=4e-4, gdiisterm=4, maxcycles=MAXOPTCYCLES):
 
 
 def optimize_molecule(basis_name, atoms, optconv=4e-4, gdiisterm=4, maxcycles=MAXOPTCYCLES, schwarz_tol=1e-12,
-                      scf=None, device=0):
+                      scf=None, device=0, restart=True):
     """The optimiser on the device path: each cycle builds a context for the
     current geometry, runs its SCF (options scf, as for
-    Context.self_consistent_field) and takes Context.scf_gradient.  atoms:
-    (Z, x, y, z) in bohr.  Returns optimize()'s dict with atoms at the
-    final geometry."""
+    Context.self_consistent_field) and takes Context.scf_gradient.  With
+    restart, every cycle after the first starts its SCF from the previous
+    cycle's converged density, as the reference's single point restores the
+    stored density before its SCF (RESTORE_DENSITYMATRIX,
+    src/psinglepoint.f90:192-194).  atoms: (Z, x, y, z) in bohr.  Returns
+    optimize()'s dict with atoms at the final geometry and the SCF
+    iterations of each cycle."""
     import polymer_amd as pa
     z = [int(a[0]) for a in atoms]
     opts = dict(scfconv=1e-10, diis=2, scfcycles=100)
     if scf:
         opts.update(scf)
+    state = {"P": None, "iterations": []}
 
     def evaluate(x):
         geom = [(zi, float(p[0]), float(p[1]), float(p[2])) for zi, p in zip(z, x)]
         ctx = pa.Context(pa.read_basis(basis_name, geom), schwarz_tol=schwarz_tol, device=device)
         try:
-            r = ctx.self_consistent_field(**opts)
+            r = ctx.self_consistent_field(guess=state["P"] if restart else None, **opts)
             if not r["converged"]:
                 raise OptimizationError("SCF did not converge during the geometry optimisation")
+            state["P"] = r["P"]
+            state["iterations"].append(r["iterations"])
             return r["e_total"], ctx.scf_gradient(r)
         finally:
             ctx.close()
 
     out = optimize(evaluate, np.array([[a[1], a[2], a[3]] for a in atoms]), optconv, gdiisterm, maxcycles)
     out["atoms"] = [(zi, float(p[0]), float(p[1]), float(p[2])) for zi, p in zip(z, out["x"])]
+    out["scf_iterations"] = state["iterations"]
     return out

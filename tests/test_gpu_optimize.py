@@ -18,6 +18,11 @@ def test_device_trajectory_equals_cpu_trajectory():
     within 1e-9 Eh of the CPU run."""
     print("case: optimisation H2O/STO-3G, device vs CPU", flush=True)
     dev = O.optimize_molecule("STO-3G", H2O_START, schwarz_tol=0.0, scf={"scfconv": 1e-11, "scfcycles": 200})
+    cold = O.optimize_molecule("STO-3G", H2O_START, schwarz_tol=0.0, scf={"scfconv": 1e-11, "scfcycles": 200},
+                               restart=False)
+    print(f"SCF iterations with restart {dev['scf_iterations']} without {cold['scf_iterations']}", flush=True)
+    assert sum(dev["scf_iterations"][1:]) < sum(cold["scf_iterations"][1:])
+    assert np.max(np.abs(dev["x"] - cold["x"])) < 1e-7
     cpu = O.optimize(oracle_evaluate("STO-3G"), np.array([a[1:] for a in H2O_START]))
     print(f"cycles device {dev['cycles']} cpu {cpu['cycles']}; E {dev['energy']:.10f} {cpu['energy']:.10f}",
           flush=True)
