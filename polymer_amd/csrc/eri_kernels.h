@@ -1306,8 +1306,9 @@ struct GradArgs {
     const double* Qp;
     double tol, xfac, ptol;
 };
-// grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: npair doubles
-int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream);
+// grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: npair
+// doubles, the pair bounds when have_qp, else computed here
+int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream, bool have_qp = false);
 // the same sum on the host (host pointers), grad overwritten
 int polyeri_grad_host(const GradArgs& g, double* grad);
 // One record of the reference's .ao files (kstored.hip): packed cells,

@@ -23,11 +23,13 @@ const RowHost kRowHost[6] = {polyeri_grad_row_host_0, polyeri_grad_row_host_1, p
 
 }  // namespace
 
-int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream) {
+int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream, bool have_qp) {
     const int npair = g.npair;
     if (g.nsh <= 0 || npair <= 0 || g.natom <= 0 || 3 * g.natom * 8 > 48 * 1024) return -1;
-    hipLaunchKernelGGL(k_grad_schwarz, dim3((npair + 255) / 256), dim3(256), 0, stream, g, d_Qp);
-    if (hipGetLastError() != hipSuccess) return -2;
+    if (!have_qp) {
+        hipLaunchKernelGGL(k_grad_schwarz, dim3((npair + 255) / 256), dim3(256), 0, stream, g, d_Qp);
+        if (hipGetLastError() != hipSuccess) return -2;
+    }
     GradArgs gq = g;
     gq.Qp = d_Qp;
     for (int X = 0; X < 6; ++X)

@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <unordered_map>
 
 #include "../../include/polyeri.h"
 #include "polyeri_internal.h"
@@ -4101,10 +4102,26 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
     HIPCHK(hipMemcpyAsync(dD, h.D.data(), h.D.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dDsh, h.dsh.data(), h.dsh.size() * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblk(3 * (size_t)natom)), dim3(256), 0, s, dG, 3 * (size_t)natom, 0.0);
+    // pair bounds: the context's Schwarz bounds Q_ab = sqrt(max |(ab|ab)|) of
+    // the same shell pairs (pairs the context dropped by primitive screening: 0)
+    std::vector<double> qp(npair, 0.0);
+    {
+        std::unordered_map<long long, double> qmap;
+        qmap.reserve(ctx->pairs.size() * 2);
+        for (size_t i = 0; i < ctx->pairs.size(); ++i) {
+            const int a = ctx->pairs[i].a, b = ctx->pairs[i].b;
+            qmap[(long long)std::min(a, b) * nsh + std::max(a, b)] = ctx->Q[i];
+        }
+        for (size_t t = 0; t < npair; ++t) {
+            auto it = qmap.find((long long)std::min(h.pa[t], h.pb[t]) * nsh + std::max(h.pa[t], h.pb[t]));
+            if (it != qmap.end()) qp[t] = it->second;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dQp, qp.data(), npair * sizeof(double), hipMemcpyHostToDevice, s));
     GradArgs g{nsh, natom, nbf, (int)npair, {0}, dL, dAt, dF0, dP0, dNp, dPa, dPb, dXyz, dAl, dCo, dD, dDsh,
                nullptr, ctx->opts.schwarz_tol, hfexchange, ctx->opts.prim_tol};
     std::copy(h.seg, h.seg + NPAIRCLASS + 1, g.seg);
-    if (polyeri_grad_launch(g, dQp, dG, s)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
+    if (polyeri_grad_launch(g, dQp, dG, s, true)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
     HIPCHK(hipMemcpyAsync(grad, dG, 3 * (size_t)natom * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int i = 0; i < 3 * natom; ++i)
