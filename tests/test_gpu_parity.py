@@ -157,7 +157,8 @@ def test_water16_size_independent_properties():
 
 def test_fortran_driver_golden(golden):
     """The Fortran driver (include/polyeri_mod.f90 ISO_C_BINDING module,
-    amdflang) reproduces the H2O/cc-pVDZ spherical golden on the GPU."""
+    amdflang) reproduces the H2O/cc-pVDZ spherical golden on the GPU: the
+    SCF, E(2), E(3) and the CCSD energies of the shipped run."""
     import os
     import re
     import subprocess
@@ -166,9 +167,15 @@ def test_fortran_driver_golden(golden):
     out = subprocess.run([exe, basis_path("cc-pVDZ")], capture_output=True, text=True, timeout=120)
     print(out.stdout[-800:], out.stderr[-400:], flush=True)
     assert out.returncode == 0
+    g = golden["H2O_ccpVDZ_spherical"]
     e = float(re.search(r"TOTAL SCF ENERGY\s+=\s+(\S+)", out.stdout).group(1))
-    assert abs(e - golden["H2O_ccpVDZ_spherical"]["e_total"]) < EH
+    assert abs(e - g["e_total"]) < EH
     assert re.search(r"ITERATIONS =\s+30", out.stdout)
+    # the rest of the shipped job through the same module: E(2), E(3), CCSD
+    for key, pat in (("e2_cc", r"E\(2\)\s+=\s+(\S+)"), ("e3_cc", r"E\(3\) =\s+(\S+)"),
+                     ("e_cc_corr", r"E\(CORRELATION\)\s+=\s+(\S+)"), ("e_ccsd", r"E\(CCSD\)\s+=\s+(\S+)")):
+        v = float(re.search(pat, out.stdout).group(1))
+        assert abs(v - g[key]) < EH, (key, v, g[key])
 
 
 def test_store_fourindex_eri_golden_count_and_mp2(golden, tmp_path):
