@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <vector>
 
 #include "eri_engine.h"
 
@@ -1305,10 +1306,41 @@ struct GradArgs {
     const double* dsh;
     const double* Qp;
     double tol, xfac, ptol;
+    // the Schwarz survivors of one class pair as chunks: chunk c holds the
+    // quartets coff[c] .. coff[c + 1] - 1, bra pair x0 + cij[2c] and ket
+    // pairs y0 + cij[2c + 1] + (t - coff[c]); null: the full product
+    const unsigned long long* coff = nullptr;
+    const int* cij = nullptr;
+    int nchunk = 0;
+};
+
+// Schwarz-surviving quartets of every class pair (X, Y <= X) of pairs sorted
+// by polyeri_grad_sort_pairs: inside a class, pairs of one primitive count
+// (a group) are in descending bound order, so the kets of a group that
+// survive with a given bra are a prefix of the group.  Class pair k = 6X + Y
+// has n[k] chunks, offsets off[off0[k]] .. off[off0[k] + n[k]] and
+// (row, first column) pairs ij[ij0[k] ..].
+struct GradChunks {
+    std::vector<unsigned long long> off;
+    std::vector<int> ij;
+    size_t off0[36], ij0[36];
+    int n[36];
+    unsigned long long nq[36];
 };
 // grad (3 natom, zeroed by the caller) += dE2/dR on the device; d_Qp: npair
-// doubles, the pair bounds when have_qp, else computed here
-int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream, bool have_qp = false);
+// doubles, the pair bounds when have_qp, else computed here.  ch (optional,
+// with d_off / d_ij its off / ij arrays on the device): launch only the
+// Schwarz survivors it lists.
+int polyeri_grad_launch(const GradArgs& g, double* d_Qp, double* d_grad, hipStream_t stream, bool have_qp = false,
+                        const GradChunks* ch = nullptr, const unsigned long long* d_off = nullptr,
+                        const int* d_ij = nullptr);
+// Orders the pairs of every class segment by primitive count (descending,
+// as they come) and, inside one count, by bound (descending); pa, pb and qp
+// permuted together.  False (nothing changed) when a bound is not finite.
+bool polyeri_grad_sort_pairs(int* pa, int* pb, double* qp, const int* np, const int* seg);
+// The chunks of sorted pairs whose bound product is >= tol.
+void polyeri_grad_chunks(const int* pa, const int* pb, const int* np, const double* qp, const int* seg, double tol,
+                         GradChunks& ch);
 // the same sum on the host (host pointers), grad overwritten
 int polyeri_grad_host(const GradArgs& g, double* grad);
 // One record of the reference's .ao files (kstored.hip): packed cells,

@@ -392,9 +392,23 @@ POLY_HD void quartet_cls(const GradArgs& g, const RysTabs& rt, int pij, int pkl,
     }
 }
 
-// quartet t of the class pair: (x0 + i, y0 + j), j <= i when the classes
-// are equal (lower triangle), else i = t / ny, j = t % ny
-POLY_HD void class_pair_index(unsigned long long t, int same, int ny, int& i, int& j) {
+// quartet t of the class pair: (x0 + i, y0 + j).  With chunks: the chunk
+// whose range holds t (binary search; chunks are never empty, so their
+// offsets increase strictly), i its row and j its first column plus the
+// distance from its start.  Without: j <= i when the classes are equal (lower
+// triangle), else i = t / ny, j = t % ny.
+POLY_HD void class_pair_index(const GradArgs& g, unsigned long long t, int same, int ny, int& i, int& j) {
+    if (g.coff) {
+        int lo = 0, hi = g.nchunk - 1;         // largest c with coff[c] <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (g.coff[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        i = g.cij[2 * lo];
+        j = g.cij[2 * lo + 1] + (int)(t - g.coff[lo]);
+        return;
+    }
     if (same) {
         tri_index(t, i, j);
     } else {
@@ -431,7 +445,7 @@ __global__ void __launch_bounds__(256) k_eri_grad(GradArgs g, int x0, int y0, in
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
          t += (unsigned long long)gridDim.x * blockDim.x) {
         int i, j;
-        class_pair_index(t, same, ny, i, j);
+        class_pair_index(g, t, same, ny, i, j);
         quartet_cls<LA, LB, LC, LD>(g, rt, x0 + i, y0 + j, [&](int at, int e, double v) {
             if (v != 0.0) atomicAdd(&sg[3 * at + e], v);
         });
@@ -466,7 +480,7 @@ struct GradHostRun {
     void operator()() const {
         for (unsigned long long t = 0; t < nq; ++t) {
             int i, j;
-            class_pair_index(t, same, ny, i, j);
+            class_pair_index(gq, t, same, ny, i, j);
             quartet_cls<A, B, C, D>(gq, rt, x0 + i, y0 + j, [&](int at, int e, double v) { grad[3 * at + e] += v; });
         }
     }

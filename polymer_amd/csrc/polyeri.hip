@@ -4088,8 +4088,6 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
         (rc = buf.get(&dD, h.D.size())) || (rc = buf.get(&dDsh, h.dsh.size())) || (rc = buf.get(&dQp, npair)) ||
         (rc = buf.get(&dG, 3 * (size_t)natom)) || (rc = buf.get(&dPa, npair)) || (rc = buf.get(&dPb, npair)))
         return rc;
-    HIPCHK(hipMemcpyAsync(dPa, h.pa.data(), npair * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dPb, h.pb.data(), npair * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     hipStream_t s = ctx->stream;
     HIPCHK(hipMemcpyAsync(dL, h.L.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dAt, h.atom.data(), nsh * sizeof(int), hipMemcpyHostToDevice, s));
@@ -4117,11 +4115,27 @@ int polyeri_eri_gradient(polyeri_ctx* ctx, const double* p_c, double hfexchange,
             if (it != qmap.end()) qp[t] = it->second;
         }
     }
+    // only the quartets with Q_ab Q_cd >= tol are launched (pairs reordered
+    // by bound inside each class and primitive count)
+    const bool sorted = polyeri_grad_sort_pairs(h.pa.data(), h.pb.data(), qp.data(), h.np.data(), h.seg);
+    GradChunks ch;
+    unsigned long long* dOff = nullptr;
+    int* dIj = nullptr;
+    if (sorted) {
+        polyeri_grad_chunks(h.pa.data(), h.pb.data(), h.np.data(), qp.data(), h.seg, ctx->opts.schwarz_tol, ch);
+        if ((rc = buf.get(&dOff, ch.off.size())) || (rc = buf.get(&dIj, std::max<size_t>(ch.ij.size(), 1)))) return rc;
+        HIPCHK(hipMemcpyAsync(dOff, ch.off.data(), ch.off.size() * 8, hipMemcpyHostToDevice, s));
+        if (!ch.ij.empty())
+            HIPCHK(hipMemcpyAsync(dIj, ch.ij.data(), ch.ij.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipMemcpyAsync(dPa, h.pa.data(), npair * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dPb, h.pb.data(), npair * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dQp, qp.data(), npair * sizeof(double), hipMemcpyHostToDevice, s));
     GradArgs g{nsh, natom, nbf, (int)npair, {0}, dL, dAt, dF0, dP0, dNp, dPa, dPb, dXyz, dAl, dCo, dD, dDsh,
                nullptr, ctx->opts.schwarz_tol, hfexchange, ctx->opts.prim_tol};
     std::copy(h.seg, h.seg + NPAIRCLASS + 1, g.seg);
-    if (polyeri_grad_launch(g, dQp, dG, s, true)) return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
+    if (polyeri_grad_launch(g, dQp, dG, s, true, sorted ? &ch : nullptr, dOff, dIj))
+        return fail(POLYERI_E_HIP, "polyeri_eri_gradient: launch failed");
     HIPCHK(hipMemcpyAsync(grad, dG, 3 * (size_t)natom * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int i = 0; i < 3 * natom; ++i)

@@ -120,3 +120,26 @@ def test_scf_gradient_matches_energy_differences():
           f"sum over atoms {np.max(np.abs(g.sum(axis=0))):.1e}")
     assert err < 2e-6
     assert np.max(np.abs(g.sum(axis=0))) < 1e-9
+
+
+def test_eri_gradient_screened_enumeration():
+    """With a Schwarz threshold only the quartets with Q_ab Q_cd >= tol are
+    enumerated (pairs ordered by bound inside each primitive count, the
+    survivors of each bra taken as prefixes).  Three waters 8 bohr apart drop
+    most cross quartets; a dropped prefix element, a lost group or a wrong
+    diagonal (pair ab == pair cd) changes the gradient by far more than the
+    screening error against the unscreened sum, and the survivors' sum stays
+    translation invariant."""
+    atoms = [(z, x + 8.0 * k, y, w + 3.0 * (k % 2)) for k in range(3) for (z, x, y, w) in M.H2O_GOLDEN]
+    b = pa.read_basis("6-31Gss", atoms)
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal((b.ncgs, b.ncgs))
+    P = 0.1 * (P + P.T)
+    g0 = pa.eri_gradient_host(b, P, 1.0, schwarz_tol=0.0)
+    for tol in (1e-12, 1e-9, 1e-6):
+        g = pa.eri_gradient_host(b, P, 1.0, schwarz_tol=tol)
+        err = np.max(np.abs(g - g0))
+        print(f"tol {tol:.0e}: max |g - g(tol 0)| = {err:.2e}")
+        assert err < 100 * tol * max(1.0, np.max(np.abs(g0)))
+        assert np.max(np.abs(g.sum(axis=0))) < 1e-10 * max(1.0, np.max(np.abs(g0)))
+    assert np.max(np.abs(g - g0)) > 0.0          # tol 1e-6 does drop quartets
