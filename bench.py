@@ -118,8 +118,9 @@ def mp2_line(nwater, seed, transforms):
             mp2["dE2_bf16x3_vs_mfma_f64"] = mp2["mfma_bf16x3"]["e2"] - f["e2"]
         mp2["roofline"] = {"bound": "mfma_fp64", "achieved": f["transform_tflops"], "peak": FP64_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": f["transform_tflops"] / FP64_PEAK_TFLOPS,
-                           "flops_model": "2 no n^4 + 2 no^2 n^3 + 2 no^2 nv n^2 + 2 no^2 nv^2 n (four quarter "
-                                          "transforms, occupied indices first)"}
+                           "flops_model": "2 npk n^2 no + 2 npk no n nv + 2 no^2 nv n^2 + 2 no^2 nv^2 n, npk = "
+                                          "n (n + 1) / 2 (four quarter transforms of the tensor with its bra "
+                                          "pair packed: the ket indices first)"}
     return mp2
 
 

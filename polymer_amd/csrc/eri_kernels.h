@@ -51,6 +51,7 @@ struct EriArgs {
     int nbf;
     double* out;               // tensor (MODE_TENSOR, MODE_PBC) or Schwarz (MODE_SCHWARZ)
     int jk_copies;             // 1: one J/K (device-scope atomics); 8: one per XCD
+    int tpack;                 // MODE_TENSOR: the pair-packed layout of TensorCb
     const int2* items;         // explicit (bra pair, ket position) items of a density-screened list, or null
     const unsigned* count;     // number of items (device), with items
     const double* ksoa;        // ket primitive pairs, bucket-interleaved fields (8 x kfs doubles)
@@ -219,13 +220,27 @@ struct SchwarzCb {
     }
 };
 
+// Writes one integral (ij|kl) to every position of the ERI tensor it fills:
+// the full n^4 tensor at ((i n + j) n + k) n + l (8 positions), or, pk, the
+// tensor with the first pair packed, A(P, k, l) at (P n + k) n + l with
+// P = max(i, j) (max(i, j) + 1) / 2 + min(i, j) (4 positions, half the size).
 struct TensorCb {
     double* out;
     size_t n;
     int fa, fb, fc, fd;
+    int pk;
     __device__ __forceinline__ void operator()(int a, int b, int c, int d, double x) {
         const size_t n1 = n, n2 = n1 * n, n3 = n2 * n;
         const size_t i = fa + a, j = fb + b, k = fc + c, l = fd + d;
+        if (pk) {
+            const size_t pij = i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+            const size_t pkl = k >= l ? k * (k + 1) / 2 + l : l * (l + 1) / 2 + k;
+            out[pij * n2 + k * n1 + l] = x;
+            out[pij * n2 + l * n1 + k] = x;
+            out[pkl * n2 + i * n1 + j] = x;
+            out[pkl * n2 + j * n1 + i] = x;
+            return;
+        }
         out[i * n3 + j * n2 + k * n1 + l] = x;
         out[j * n3 + i * n2 + k * n1 + l] = x;
         out[i * n3 + j * n2 + l * n1 + k] = x;
@@ -649,7 +664,7 @@ __device__ __forceinline__ void eri_item(const EriArgs& g, long long t) {
     const SoaKet kv{g.ksoa + g.bucket_soa[b] + (pos - g.bucket_start[b]), g.bucket_start[b + 1] - g.bucket_start[b],
                     g.kfs};
     if (MODE == MODE_TENSOR) {
-        TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
+        TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd, g.tpack};
         quartet<double, LA, LB, LC, LD>(bra, mb.nprim, kv, mk.nprim, ABx, ABy, ABz, CDx, CDy, CDz, g.boysA, cb);
     } else {
         const double s = (fa == fb ? 0.5 : 1.0) * (fc == fd ? 0.5 : 1.0) * (bp == kp ? 0.5 : 1.0);

@@ -369,7 +369,7 @@ k_eri_coop(EriArgs g, int prog, int na, int nb, int nc, int nd, int pbs) {
                 jk_add_fx(dst, (double)out[o], g.jk_copies);
             }
         } else if (MODE == MODE_TENSOR) {
-            TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd};
+            TensorCb cb{g.out, (size_t)n, fa, fb, fc, fd, g.tpack};
             for (int k = lane; k < d.nfin; k += 64) {
                 const uint32_t f = g_prog_aux[d.fin_off + k];
                 cb((f >> 16) & 15, (f >> 20) & 15, (f >> 24) & 15, f >> 28, (double)sl[f & 0xffff]);

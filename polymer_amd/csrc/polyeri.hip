@@ -434,6 +434,7 @@ struct ScfArena {
 };
 
 struct polyeri_ctx {
+    int tpack = 0;                 // MODE_TENSOR launches write the pair-packed tensor
     RefBasis rb;
     IntBasis ib;
     std::vector<PairInfo> pairs;
@@ -626,6 +627,7 @@ static int launch_class(polyeri_ctx* c, int mode, int la, int lb, int lc, int ld
     a.boys = c->d_boys; a.boysA = c->d_boys + BOYS_AOFF;
     a.D = c->d_D; a.J = c->d_J; a.K = c->d_K; a.nbf = c->ib.nbf; a.out = out;
     a.jk_copies = c->jk_copies;
+    a.tpack = mode == MODE_TENSOR ? c->tpack : 0;
     EriKernel k = mode == MODE_JK ? kernel_for<MODE_JK>(la, lb, lc, ld)
                 : mode == MODE_JK32 ? kernel_for<MODE_JK32>(la, lb, lc, ld)
                 : mode == MODE_SCHWARZ ? kernel_for<MODE_SCHWARZ>(la, lb, lc, ld)
@@ -1988,11 +1990,14 @@ int polyeri_one_electron(polyeri_ctx* ctx, double* s_c, double* t_c, double* n_c
     return 0;
 }
 
-// Full internal ERI tensor on the device (nbf^4), screened like the J/K build.
-static int tensor_device(polyeri_ctx* c, double* d_t) {
+// Internal ERI tensor on the device, screened like the J/K build: the full
+// nbf^4 tensor, or (packed, computed integrals only) the tensor with its
+// first pair packed, nbf (nbf + 1) / 2 x nbf^2 (TensorCb).
+static int tensor_device(polyeri_ctx* c, double* d_t, bool packed = false) {
     if (c->pbc) return fail(POLYERI_E_ARG, "periodic context: no molecular ERI tensor");
+    if (packed && c->use_stored) return fail(POLYERI_E_ARG, "packed tensor: computed integrals only");
     size_t n = c->ib.nbf;
-    size_t n4 = n * n * n * n;
+    size_t n4 = packed ? n * (n + 1) / 2 * n * n : n * n * n * n;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n4)), dim3(256), 0, c->stream, d_t, n4, 0.0);
     HIPCHK(hipGetLastError());
     if (c->use_stored) {   // DIRECT .F.: the tensor from the stored records
@@ -2000,12 +2005,14 @@ static int tensor_device(polyeri_ctx* c, double* d_t) {
             return fail(POLYERI_E_HIP, "stored-integral tensor launch failed");
         return 0;
     }
+    c->tpack = packed;
     for (const ClassPairWork& cw : c->work) {
         int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
                               cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr,
                               LaunchCfg(), cw.d_bsoa);
-        if (rc) return rc;
+        if (rc) { c->tpack = 0; return rc; }
     }
+    c->tpack = 0;
     return 0;
 }
 
@@ -2777,6 +2784,45 @@ int polyeri_scf(polyeri_ctx* ctx, const polyeri_scf_opts* o, double* p_c, double
 // ---------------------------------------------------------------------------
 // MP2 (one k-point): AO integrals on the device, four quarter transforms
 // ---------------------------------------------------------------------------
+// Y(s, l, a, i) = X2(P(s, l), i, a): the half-transformed integrals (sl|ia)
+// of the packed bra pair P(s, l) = max (max + 1) / 2 + min, X2 at
+// (P no + i) nv + a, spread to Y at ((s n + l) nv + a) no + i.  One block per
+// P: the contiguous no x nv block is read once, in 32 x 32 tiles through LDS,
+// and written transposed (nv x no, contiguous) to (s, l) and (l, s).
+__global__ void __launch_bounds__(256) k_mp2_unpack(const double* __restrict__ X2, int n, int no, int nv,
+                                                    double* __restrict__ Y) {
+    __shared__ double tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const size_t nb = (size_t)no * nv, npk = (size_t)n * (n + 1) / 2;
+    const int ti = (no + 31) / 32, ta = (nv + 31) / 32;
+    for (size_t P = blockIdx.x; P < npk; P += gridDim.x) {
+        int s = (int)((std::sqrt(8.0 * (double)P + 1.0) - 1.0) * 0.5);
+        while ((size_t)s * (s + 1) / 2 > P) --s;
+        while ((size_t)(s + 1) * (s + 2) / 2 <= P) ++s;
+        const int l = (int)(P - (size_t)s * (s + 1) / 2);
+        const double* src = X2 + P * nb;
+        double* d0 = Y + ((size_t)s * n + l) * nb;
+        double* d1 = Y + ((size_t)l * n + s) * nb;
+        for (int t = 0; t < ti * ta; ++t) {
+            const int i0 = (t / ta) * 32, a0 = (t % ta) * 32;
+            for (int r = ty; r < 32; r += 8) {
+                const int i = i0 + r, a = a0 + tx;
+                if (i < no && a < nv) tile[r][tx] = src[(size_t)i * nv + a];
+            }
+            __syncthreads();
+            for (int r = ty; r < 32; r += 8) {
+                const int a = a0 + r, i = i0 + tx;
+                if (i < no && a < nv) {
+                    const double v = tile[tx][r];
+                    d0[(size_t)a * no + i] = v;
+                    if (s != l) d1[(size_t)a * no + i] = v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // nqp > 0: also the quasi-particle bands (IOPTN(44)): the occupied-side
 // transforms run over icore .. min(iocc + nqp, nall - ivirtcore) - 1 and the
 // virtual side of the third quarter from max(iocc - nqp, icore), as the
@@ -2798,18 +2844,31 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     rocblas_handle h = ctx->blas;
     hipStream_t s = ctx->stream;
     int n = ctx->ib.nbf, nc = ctx->rb.ncgs;
-    size_t n4 = (size_t)n * n * n * n;
+    // computed integrals: the tensor with its bra pair packed (half the
+    // memory and half the scattered writes); stored records: the full tensor
+    const bool packed = !ctx->use_stored;
+    const size_t npk = (size_t)n * (n + 1) / 2;
+    const size_t n4 = packed ? npk * n * n : (size_t)n * n * n * n;
+    if (packed && (npk * nop > (size_t)INT_MAX || (size_t)n * nvp * nop > (size_t)INT_MAX))
+        return fail(POLYERI_E_ARG, "polyeri_mp2_energy: basis too large");
     DevBuf buf;
-    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc, *dQ;
-    // occupied-side indices first: the largest intermediate is nop n^3
-    size_t x1 = (size_t)nop * n * n * n, x2 = (size_t)nop * nop * n * n, x3 = (size_t)nop * nop * nvp * n;
-    size_t x4 = (size_t)nop * nop * nvp * nv;
+    double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc, *dQ, *dY = nullptr;
+    // occupied-side indices first: the largest intermediate is nop n^3 (full
+    // tensor) or nop n npk (packed)
+    const size_t x1 = packed ? npk * nop * n : (size_t)nop * n * n * n;
+    const size_t x2 = packed ? npk * nop * nvp : (size_t)nop * nop * n * n;
+    const size_t x3 = (size_t)nop * nop * nvp * n, x4 = (size_t)nop * nop * nvp * nv;
+    const size_t ny = packed ? (size_t)n * n * nvp * nop : 0;
     if ((rc = buf.get(&dTen, n4)) || (rc = buf.get(&dCO, (size_t)nc * nc)) || (rc = buf.get(&dC, (size_t)n * nc)) ||
         (rc = buf.get(&dX1, x1)) || (rc = buf.get(&dX2, x2)) || (rc = buf.get(&dX3, x3)) || (rc = buf.get(&dT4, x4)) ||
         (rc = buf.get(&dEps, nc)) || (rc = buf.get(&dAcc, 2)) || (rc = buf.get(&dQ, (size_t)nc)))
         return rc;
+    if (packed) {   // Y reuses the tensor's memory (read by the first transform only) when it fits
+        if (ny <= n4) dY = dTen;
+        else if ((rc = buf.get(&dY, ny))) return rc;
+    }
     HIPCHK(hipEventRecord(ctx->ev0, s));
-    if ((rc = tensor_device(ctx, dTen))) return rc;
+    if ((rc = tensor_device(ctx, dTen, packed))) return rc;
     hipEvent_t et = nullptr;
     HIPCHK(hipEventCreate(&et));
     struct EvGuard { hipEvent_t e; ~EvGuard() { if (e) (void)hipEventDestroy(e); } } eg{et};
@@ -2830,7 +2889,40 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     //   X3(j, i, a, m) = sum_l X2(j, i, m, l) Ca(l, a)      [batched over m]
     //   T(j, i, a, b)  = sum_m X3(j, i, a, m) Cv(m, b)  = (ia|jb)
     const long N = n, NO = nop, NA = nvp, NO2 = (long)nop * nop, N3 = (long)n * n * n;
-    if (gemm == POLYERI_GEMM_ROCBLAS) {
+    // Packed tensor A(P(s, l), u, m) = (sl|um): the ket indices first, the
+    // packed pair spread out after the second quarter:
+    //   X1(P, i, m) = sum_u A(P, u, m) Co(u, i)      X1[(P nop + i) n + m]   [batched over P]
+    //   X2(P, i, a) = sum_m X1(P, i, m) Ca(m, a)     X2[(P nop + i) nvp + a]
+    //   Y(s, l, a, i) = X2(P(s, l), i, a)            Y[((s n + l) nvp + a) nop + i]
+    //   X3(j, i, a, l) = sum_s Co(s, j) Y(s, l, a, i)  X3[j + nop (i + nop (a + nvp l))]
+    //   T(j, i, a, b) = sum_l X3(j, i, a, l) Cv(l, b)  = (jb|ia) = (ia|jb)
+    const long NPK = (long)npk, NYC = (long)n * nvp * nop;
+    auto unpack = [&]() -> int {
+        const unsigned blocks = (unsigned)std::min<size_t>(npk, 65536);
+        hipLaunchKernelGGL(k_mp2_unpack, dim3(blocks), dim3(256), 0, s, dX2, n, nop, nvp, dY);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    };
+    if (packed && gemm == POLYERI_GEMM_ROCBLAS) {
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, nop, n, &one, dTen, n,
+                                            (rocblas_stride)N * N, Co, n, 0, &zero, dX1, n, (rocblas_stride)NO * N,
+                                            (int)npk));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nvp, (int)(npk * nop), n, &one, Ca,
+                            n, dX1, n, &zero, dX2, nvp));
+        if (unpack()) return fail(POLYERI_E_HIP, "MP2 unpack launch failed");
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, nop, (int)NYC, n, &one, Co, n,
+                            dY, (int)NYC, &zero, dX3, nop));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nop * nop * nvp, nv, n, &one, dX3,
+                            nop * nop * nvp, Cv, n, &zero, dT4, nop * nop * nvp));
+    } else if (packed) {
+        const int mode = gemm == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
+        int r = polyeri_gemm_launch(mode, n, nop, n, dTen, 1, N, N * N, Co, 1, N, 0, dX1, 1, N, NO * N, (int)npk, s);
+        if (!r) r = polyeri_gemm_launch(mode, (int)(npk * nop), nvp, n, dX1, N, 1, 0, Ca, 1, N, 0, dX2, NA, 1, 0, 1, s);
+        if (!r) r = unpack();
+        if (!r) r = polyeri_gemm_launch(mode, nop, (int)NYC, n, Co, N, 1, 0, dY, NYC, 1, 0, dX3, 1, NO, 0, 1, s);
+        if (!r) r = polyeri_gemm_launch(mode, nop * nop * nvp, nv, n, dX3, 1, NO2 * NA, 0, Cv, 1, N, 0, dT4, 1,
+                                        NO2 * NA, 0, 1, s);
+        if (r) return fail(POLYERI_E_HIP, "MP2 transform launch failed");
+    } else if (gemm == POLYERI_GEMM_ROCBLAS) {
         // column-major views: X1 is nop x n^3, X2 (as its transpose) nop x (nop n) per l, X3 nop^2 x nvp per m
         RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, nop, n * n * n, n, &one, Co, n,
                             dTen, n * n * n, &zero, dX1, nop));
@@ -2858,8 +2950,11 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     HIPCHK(hipEventSynchronize(ctx->ev1));
     ctx->last_mp2_tensor_ms = event_ms(ctx->ev0, et);
     ctx->last_mp2_transform_ms = event_ms(et, ctx->ev1);
-    ctx->last_mp2_flops = 2.0 * nop * (double)n * n * n * n + 2.0 * nop * (double)nop * n * n * n +
-                          2.0 * nop * (double)nop * nvp * n * n + 2.0 * nop * (double)nop * nvp * nv * n;
+    ctx->last_mp2_flops =
+        packed ? 2.0 * (double)npk * n * n * nop + 2.0 * (double)npk * nop * n * nvp +
+                     2.0 * nop * (double)n * n * nvp * nop + 2.0 * nop * (double)nop * nvp * nv * n
+               : 2.0 * nop * (double)n * n * n * n + 2.0 * nop * (double)nop * n * n * n +
+                     2.0 * nop * (double)nop * nvp * n * n + 2.0 * nop * (double)nop * nvp * nv * n;
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, dAcc, (size_t)2, 0.0);
     {
         const size_t tile = (size_t)no * (no + 1) * 8;
