@@ -2849,7 +2849,7 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     const bool packed = !ctx->use_stored;
     const size_t npk = (size_t)n * (n + 1) / 2;
     const size_t n4 = packed ? npk * n * n : (size_t)n * n * n * n;
-    if (packed && (npk * nop > (size_t)INT_MAX || (size_t)n * nvp * nop > (size_t)INT_MAX))
+    if (packed && (npk * n > (size_t)INT_MAX || (size_t)n * nvp * nop > (size_t)INT_MAX))
         return fail(POLYERI_E_ARG, "polyeri_mp2_energy: basis too large");
     DevBuf buf;
     double *dTen, *dCO, *dC, *dX1, *dX2, *dX3, *dT4, *dEps, *dAcc, *dQ, *dY = nullptr;
@@ -2889,10 +2889,10 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
     //   X3(j, i, a, m) = sum_l X2(j, i, m, l) Ca(l, a)      [batched over m]
     //   T(j, i, a, b)  = sum_m X3(j, i, a, m) Cv(m, b)  = (ia|jb)
     const long N = n, NO = nop, NA = nvp, NO2 = (long)nop * nop, N3 = (long)n * n * n;
-    // Packed tensor A(P(s, l), u, m) = (sl|um): the ket indices first, the
-    // packed pair spread out after the second quarter:
-    //   X1(P, i, m) = sum_u A(P, u, m) Co(u, i)      X1[(P nop + i) n + m]   [batched over P]
-    //   X2(P, i, a) = sum_m X1(P, i, m) Ca(m, a)     X2[(P nop + i) nvp + a]
+    // Packed tensor A(P(s, l), u, m) = (sl|um) = A(P, m, u): the ket indices
+    // first, the packed pair spread out after the second quarter:
+    //   X1(P, m, i) = sum_u A(P, m, u) Co(u, i)      X1[(P n + m) nop + i]   [rows (P, m): one product]
+    //   X2(P, i, a) = sum_m X1(P, m, i) Ca(m, a)     X2[(P nop + i) nvp + a] [batched over P]
     //   Y(s, l, a, i) = X2(P(s, l), i, a)            Y[((s n + l) nvp + a) nop + i]
     //   X3(j, i, a, l) = sum_s Co(s, j) Y(s, l, a, i)  X3[j + nop (i + nop (a + nvp l))]
     //   T(j, i, a, b) = sum_l X3(j, i, a, l) Cv(l, b)  = (jb|ia) = (ia|jb)
@@ -2903,11 +2903,11 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
         return hipGetLastError() == hipSuccess ? 0 : -2;
     };
     if (packed && gemm == POLYERI_GEMM_ROCBLAS) {
-        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, nop, n, &one, dTen, n,
-                                            (rocblas_stride)N * N, Co, n, 0, &zero, dX1, n, (rocblas_stride)NO * N,
-                                            (int)npk));
-        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nvp, (int)(npk * nop), n, &one, Ca,
-                            n, dX1, n, &zero, dX2, nvp));
+        RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nop, (int)(npk * n), n, &one, Co, n,
+                            dTen, n, &zero, dX1, nop));
+        RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_transpose, nvp, nop, n,
+                                            &one, Ca, n, 0, dX1, nop, (rocblas_stride)N * NO, &zero, dX2, nvp,
+                                            (rocblas_stride)NO * NA, (int)npk));
         if (unpack()) return fail(POLYERI_E_HIP, "MP2 unpack launch failed");
         RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, nop, (int)NYC, n, &one, Co, n,
                             dY, (int)NYC, &zero, dX3, nop));
@@ -2915,8 +2915,9 @@ static int mp2_energy(polyeri_ctx* ctx, const double* co, const double* eps, int
                             nop * nop * nvp, Cv, n, &zero, dT4, nop * nop * nvp));
     } else if (packed) {
         const int mode = gemm == POLYERI_GEMM_MFMA_F64 ? GEMM_F64 : GEMM_BF16X3;
-        int r = polyeri_gemm_launch(mode, n, nop, n, dTen, 1, N, N * N, Co, 1, N, 0, dX1, 1, N, NO * N, (int)npk, s);
-        if (!r) r = polyeri_gemm_launch(mode, (int)(npk * nop), nvp, n, dX1, N, 1, 0, Ca, 1, N, 0, dX2, NA, 1, 0, 1, s);
+        int r = polyeri_gemm_launch(mode, (int)(npk * n), nop, n, dTen, N, 1, 0, Co, 1, N, 0, dX1, NO, 1, 0, 1, s);
+        if (!r) r = polyeri_gemm_launch(mode, nop, nvp, n, dX1, 1, NO, N * NO, Ca, 1, N, 0, dX2, NA, 1, NO * NA,
+                                        (int)npk, s);
         if (!r) r = unpack();
         if (!r) r = polyeri_gemm_launch(mode, nop, (int)NYC, n, Co, N, 1, 0, dY, NYC, 1, 0, dX3, 1, NO, 0, 1, s);
         if (!r) r = polyeri_gemm_launch(mode, nop * nop * nvp, nv, n, dX3, 1, NO2 * NA, 0, Cv, 1, N, 0, dT4, 1,
