@@ -855,7 +855,9 @@ __device__ __forceinline__ double gc_dmax(const EriArgs& g, int x0, int x1, int 
 // kept.  BT: the block copies the coarse Boys rows of its L into LDS (in
 // front of the rows) and reads them there instead of the aligned global
 // table, whose row gathers miss the L1 on most primitive iterations.
-template <int LA, int LC, bool BT>
+// TS: each member quartet's integrals are written to the ERI tensor g.out
+// (TensorCb, layout g.tpack) instead of digested; no LDS rows, no J/K.
+template <int LA, int LC, bool BT, bool TS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LA == 1 && LC == 0 ? 3 : 1)))
 k_eri_gc(EriArgs g) {
     constexpr int NA = ncart(LA), NC = ncart(LC), NI = NA * NC;
@@ -888,7 +890,8 @@ k_eri_gc(EriArgs g) {
         if (g.dtol > 0.0 && g.segq &&
             fmax(fmax(mb.q[0], mb.q[1]), fmax(mb.q[2], mb.q[3])) * g.segq[sg] * g.dmax[0] < g.dtol)
             continue;   // block-uniform: no item of the segment survives
-        for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
+        if constexpr (!TS)
+            for (int i = threadIdx.x; i < total; i += blockDim.x) seg_lds[i] = 0.0;
         __syncthreads();
         const GcPrim* bra = g.gprims + mb.prim_off;
         double jab[MB * NA];
@@ -1008,6 +1011,14 @@ k_eri_gc(EriArgs g) {
                     double v[NI];
                     StoreCb<1, NC, 1, double> st{v};
                     ClassCode<LA, 0, LC, 0, double>::hrr(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, acc + (i * MK + j) * NI, st);
+                    if constexpr (TS) {
+                        TensorCb tc{g.out, (size_t)n, fa, fb, fc, fd, g.tpack};
+#pragma unroll
+                        for (int a = 0; a < NA; ++a)
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) tc(a, 0, c, 0, v[a * NC + c]);
+                        continue;
+                    }
                     RegDigest<NA, 1, NC, 1, double> cb;
                     cb.load(g.D, n, fa, fb, fc, fd);
 #pragma unroll
@@ -1037,10 +1048,12 @@ k_eri_gc(EriArgs g) {
                 }
             }
         }
+        if constexpr (!TS) {
 #pragma unroll
-        for (int i = 0; i < MB * NA; ++i) {
-            const double w = wave_sum(jab[i]);
-            if (lane == 0 && i < mb.m * NA) atomicAdd(&Jl[i], w);
+            for (int i = 0; i < MB * NA; ++i) {
+                const double w = wave_sum(jab[i]);
+                if (lane == 0 && i < mb.m * NA) atomicAdd(&Jl[i], w);
+            }
         }
         if (screen) {
 #pragma unroll
@@ -1048,19 +1061,22 @@ k_eri_gc(EriArgs g) {
             if (lane == 0 && nkept) atomicAdd(g.kept, nkept);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < (2 * NA + 2) * n; i += blockDim.x) {
-            const double x = Kr[i];
-            if (x != 0.0) {
-                const int r = i / n, col = i - r * n;
-                const int f = r < 2 * NA ? (r < NA ? mb.fa0 + r : mb.fa1 + (r - NA)) : (r == 2 * NA ? mb.fb0 : mb.fb1);
-                jk_add(&K[(size_t)f * n + col], x, g.jk_copies);
+        if constexpr (!TS) {
+            for (int i = threadIdx.x; i < (2 * NA + 2) * n; i += blockDim.x) {
+                const double x = Kr[i];
+                if (x != 0.0) {
+                    const int r = i / n, col = i - r * n;
+                    const int f =
+                        r < 2 * NA ? (r < NA ? mb.fa0 + r : mb.fa1 + (r - NA)) : (r == 2 * NA ? mb.fb0 : mb.fb1);
+                    jk_add(&K[(size_t)f * n + col], x, g.jk_copies);
+                }
             }
-        }
-        for (int i = threadIdx.x; i < mb.m * NA; i += blockDim.x) {
-            const int j = i / NA, a = i - j * NA;
-            const int fa = ((mb.code >> (2 * j)) & 1) ? mb.fa1 : mb.fa0;
-            const int fb = ((mb.code >> (2 * j + 1)) & 1) ? mb.fb1 : mb.fb0;
-            jk_add(&J[(size_t)(fa + a) * n + fb], Jl[i], g.jk_copies);
+            for (int i = threadIdx.x; i < mb.m * NA; i += blockDim.x) {
+                const int j = i / NA, a = i - j * NA;
+                const int fa = ((mb.code >> (2 * j)) & 1) ? mb.fa1 : mb.fa0;
+                const int fb = ((mb.code >> (2 * j + 1)) & 1) ? mb.fb1 : mb.fb0;
+                jk_add(&J[(size_t)(fa + a) * n + fb], Jl[i], g.jk_copies);
+            }
         }
         __syncthreads();
     }
@@ -1281,7 +1297,7 @@ inline int polyeri_coop_default_pbs(int nint) { return nint < 200 ? 3 : 0; }
 // class (la s|lc s) has one, and the launch over a.segs (grid blocks).
 size_t polyeri_gc_lds_bytes(int la, int nbf);
 bool polyeri_gc_class(int la, int lc);
-int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream);
+int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream, bool tensor = false);
 // The same for k_eri_gck (shell-pair bra (la lb|, generally contracted kets |lc s)).
 size_t polyeri_gck_lds_bytes(int la, int lb, int nbf);
 bool polyeri_gck_class(int la, int lb, int lc);

@@ -14,15 +14,22 @@ bool polyeri_gc_class(int la, int lc) {
     return (la == 0 && lc == 0) || (la == 1 && lc <= 1) || (la == 2 && lc == 0);
 }
 
-int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream) {
+// tensor: the form that writes the integrals to a.out (no LDS rows)
+int polyeri_gc_launch(const EriArgs& a, int la, int lc, int block, int grid, hipStream_t stream, bool tensor) {
     if (!polyeri_gc_class(la, lc) || a.nseg <= 0 || grid <= 0) return -1;
-    size_t lds = polyeri_gc_lds_bytes(la, a.nbf);
+    size_t lds = tensor ? 0 : polyeri_gc_lds_bytes(la, a.nbf);
     if (lds > 65536) return -1;
     // the coarse Boys rows go into LDS where they fit beside the rows
     const bool bt = lds + BOYS_CROWS * sizeof(double) <= 65536;
     if (bt) lds += BOYS_CROWS * sizeof(double);
     void (*k)(EriArgs) = nullptr;
-    if (la == 0) k = bt ? k_eri_gc<0, 0, true> : k_eri_gc<0, 0, false>;
+    if (tensor) {
+        if (!bt) return -1;
+        if (la == 0) k = k_eri_gc<0, 0, true, true>;
+        else if (la == 1 && lc == 0) k = k_eri_gc<1, 0, true, true>;
+        else if (la == 1) k = k_eri_gc<1, 1, true, true>;
+        else k = k_eri_gc<2, 0, true, true>;
+    } else if (la == 0) k = bt ? k_eri_gc<0, 0, true> : k_eri_gc<0, 0, false>;
     else if (la == 1 && lc == 0) k = bt ? k_eri_gc<1, 0, true> : k_eri_gc<1, 0, false>;
     else if (la == 1) k = bt ? k_eri_gc<1, 1, true> : k_eri_gc<1, 1, false>;
     else k = bt ? k_eri_gc<2, 0, true> : k_eri_gc<2, 0, false>;

@@ -710,11 +710,15 @@ static bool screens_in_kernel(const polyeri_ctx* c, const ClassPairWork& cw) {
 }
 
 // J/K launch of a list's generally contracted form (one block per segment),
-// density-screened inside the kernel when scr is given.
-static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const SegScreen* scr = nullptr) {
+// density-screened inside the kernel when scr is given; with tensor (lists of
+// shell-pair-group bras only), the integrals written to it instead.
+static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const SegScreen* scr = nullptr,
+                     double* tensor = nullptr) {
     const auto& g = cw.gc;
     if (g.nwork <= 0) return 0;
     EriArgs a{};
+    a.out = tensor;
+    a.tpack = tensor ? c->tpack : 0;
     a.bra_ids = g.d_bra; a.ket_ids = g.d_ket; a.row_off = g.d_row; a.nrows = g.nrows; a.nwork = g.nwork;
     a.rb_off = g.d_rb; a.bucket_start = g.d_bstart; a.nbucket = g.nbucket; a.rb_first = nullptr;
     a.ksoa = c->d_gksoa; a.kfs = c->gkfs; a.bucket_soa = g.d_bsoa;
@@ -729,12 +733,13 @@ static int launch_gc(polyeri_ctx* c, const ClassPairWork& cw, int block, const S
     }
     if (block != 64 && block != 128) block = 256;
     if (g.kind == 1) {   // shell-pair bra
+        if (tensor) return fail(POLYERI_E_ARG, "generally contracted ket lists have no tensor form");
         a.prims = c->d_prims; a.meta = c->d_meta; a.ab = c->d_ab; a.Qs = c->d_Q; a.psh = c->d_psh;
         if (polyeri_gck_launch(a, cw.la, cw.lb, cw.lc, block, a.nseg, c->stream))
             return fail(POLYERI_E_HIP, "generally contracted ket kernel launch failed");
         return 0;
     }
-    if (polyeri_gc_launch(a, cw.la, cw.lc, block, a.nseg, c->stream))
+    if (polyeri_gc_launch(a, cw.la, cw.lc, block, a.nseg, c->stream, tensor != nullptr))
         return fail(POLYERI_E_HIP, "generally contracted kernel launch failed");
     return 0;
 }
@@ -2007,9 +2012,13 @@ static int tensor_device(polyeri_ctx* c, double* d_t, bool packed = false) {
     }
     c->tpack = packed;
     for (const ClassPairWork& cw : c->work) {
-        int rc = launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
-                              cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr, nullptr,
-                              LaunchCfg(), cw.d_bsoa);
+        // lists with a generally contracted form of shell-pair groups: one
+        // primitive loop per group quartet for all its member quartets
+        const bool gc = cw.gc.nwork > 0 && cw.gc.kind == 0;
+        int rc = gc ? launch_gc(c, cw, cw.gc_block, nullptr, d_t)
+                    : launch_class(c, MODE_TENSOR, cw.la, cw.lb, cw.lc, cw.ld, cw.d_bra, cw.d_ket, cw.d_row,
+                                   cw.nrows, cw.nwork, d_t, cw.d_rb, cw.d_bstart, cw.nbucket, cw.d_rf, nullptr,
+                                   nullptr, LaunchCfg(), cw.d_bsoa);
         if (rc) { c->tpack = 0; return rc; }
     }
     c->tpack = 0;
